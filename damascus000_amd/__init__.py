@@ -1,0 +1,486 @@
+"""damascus000_amd — Damascus Fold's commit-and-fold path on MI355X (gfx950).
+
+A Python mirror of the reference's public surface (src/lib.rs:6-10: DamascusProver,
+DamascusVerifier, ModuleSisCommitter, SystemParams, RuntimeConfig, Statement / RoundRecord
+bytes) over the C ABI of include/damascus_hip.h. All compute runs in libdamascus_hip.so on the
+GPU; there is no CPU path. Importing fails loudly when the library has not been built.
+
+Z_q values are Python ints or numpy uint64 arrays whose last axis is (lo, hi), the 16-byte
+little-endian encoding of Fp (src/algebra/field.rs:81-83).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "Q", "K", "PRIMES", "DamascusError", "SystemParams", "RuntimeConfig", "Context", "backend_ready",
+    "device_summary", "ModuleSisCommitter", "DamascusProver", "DamascusVerifier", "ntt_batch", "fold_batch",
+    "ring_mul_batch", "prove", "verify", "prove_verify_device", "derive_subseed", "lib", "LIB_PATH", "DeviceBuffer",
+]
+
+Q = (1 << 112) - 75
+K = 8
+PRIMES = (3892314113, 2281701377, 2013265921, 2885681153, 2483027969, 1811939329, 469762049, 4194304001)
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdamascus_hip.so")
+
+DMX_OK = 0
+ERRORS = {
+    -1: "ERR_ARG", -2: "ERR_NO_DEVICE", -3: "ERR_HIP", -4: "ERR_MEMORY_LIMIT", -5: "ERR_ROUND",
+    -6: "ERR_INVARIANT", -7: "ERR_SHAPE", -8: "ERR_OPENING", -9: "ERR_DECODE", -10: "ERR_NOT_FINAL",
+    -11: "ERR_ZERO_TERMS", -12: "ERR_OOM", -13: "ERR_BUFFER", -14: "ERR_POISONED",
+}
+ERR_CODE = {v: k for k, v in ERRORS.items()}
+
+
+class DamascusError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+        self.code = code
+        self.name = ERRORS.get(code, str(code))
+
+
+_lib = None
+_c = ctypes
+_VP = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+_SZP = ctypes.POINTER(ctypes.c_size_t)
+
+
+class SystemParams(ctypes.Structure):
+    """SystemParams (src/utils/config.rs:24-44): seeds only, the dimensions follow the file."""
+    _fields_ = [("seed_generators", ctypes.c_uint8 * 32), ("epoch_seed", ctypes.c_uint8 * 32)]
+
+    def __init__(self, seed_generators=bytes([7] * 32), epoch_seed=bytes([11] * 32)):
+        super().__init__()
+        self.seed_generators[:] = list(bytes(seed_generators))
+        self.epoch_seed[:] = list(bytes(epoch_seed))
+
+
+class RuntimeConfig(ctypes.Structure):
+    """RuntimeConfig (src/utils/config.rs:69-87)"""
+    _fields_ = [("max_preprocess_bytes", ctypes.c_uint64), ("ntt_enabled", ctypes.c_int32),
+                ("parallel_enabled", ctypes.c_int32), ("gpu_enabled", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("gpu_min_elements", ctypes.c_uint64)]
+
+    def __init__(self, max_preprocess_bytes=512 << 20, ntt_enabled=True, parallel_enabled=True, gpu_enabled=True,
+                 gpu_min_elements=4096):
+        super().__init__()
+        self.max_preprocess_bytes = max_preprocess_bytes
+        self.ntt_enabled = int(ntt_enabled)
+        self.parallel_enabled = int(parallel_enabled)
+        self.gpu_enabled = int(gpu_enabled)
+        self.gpu_min_elements = gpu_min_elements
+
+
+class RoundOutput(ctypes.Structure):
+    _fields_ = [("vector_fold_ns", ctypes.c_uint64), ("poly_fold_ns", ctypes.c_uint64),
+                ("total_round_ns", ctypes.c_uint64)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "dmx_backend_ready": (ctypes.c_int, []),
+    "dmx_device_summary": (ctypes.c_int, [ctypes.c_char_p, _SZ]),
+    "dmx_last_error": (ctypes.c_char_p, []),
+    "dmx_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_VP)]),
+    "dmx_ctx_destroy": (None, [_VP]),
+    "dmx_ntt_batch": (ctypes.c_int, [_VP] * 3 + [_SZ] + [_VP] * 4),
+    "dmx_fold_batch": (ctypes.c_int, [_VP] * 6 + [_SZ, ctypes.c_uint64, ctypes.c_uint64]),
+    "dmx_ring_mul_batch": (ctypes.c_int, [_VP] * 5 + [_SZ]),
+    "dmx_derive_subseed": (ctypes.c_int, [_VP, _VP, _SZ, _VP]),
+    "dmx_derive_generators": (ctypes.c_int, [_VP, _VP, _SZ, _SZ, _VP]),
+    "dmx_commit": (ctypes.c_int, [_VP, _VP, _VP, _SZ, _SZ, _VP, _SZP]),
+    "dmx_commit_with_generators": (ctypes.c_int, [_VP, _VP, _VP, _SZ, _SZ, _VP, _SZP]),
+    "dmx_prover_new": (ctypes.c_int, [_VP, _VP, _SZ, _VP, _VP, ctypes.POINTER(_VP)]),
+    "dmx_prover_free": (None, [_VP]),
+    "dmx_prover_rounds_total": (_SZ, [_VP]),
+    "dmx_prover_current_round": (_SZ, [_VP]),
+    "dmx_prover_current_dimensions": (None, [_VP, _SZP, _SZP]),
+    "dmx_prover_statement": (ctypes.c_int, [_VP, _VP, _SZ, _SZP]),
+    "dmx_prover_current_commitment": (ctypes.c_int, [_VP, _VP, _SZ, _SZP]),
+    "dmx_prover_fold_round": (ctypes.c_int, [_VP, _SZ, _VP, _SZ, _SZP, _VP]),
+    "dmx_prover_final_opening": (ctypes.c_int, [_VP, _VP]),
+    "dmx_verifier_new": (ctypes.c_int, [_VP, _VP, _VP, _SZ, ctypes.POINTER(_VP)]),
+    "dmx_verifier_clone": (_VP, [_VP]),
+    "dmx_verifier_free": (None, [_VP]),
+    "dmx_verifier_round": (_SZ, [_VP]),
+    "dmx_verifier_current_dimensions": (None, [_VP, _SZP, _SZP]),
+    "dmx_verifier_current_commitment": (ctypes.c_int, [_VP, _VP, _SZ, _SZP]),
+    "dmx_verifier_update_commitment": (ctypes.c_int, [_VP, _VP, _SZ]),
+    "dmx_verifier_verify_final_opening": (ctypes.c_int, [_VP, _VP]),
+    "dmx_prove": (ctypes.c_int, [_VP, _VP, _SZ, _VP, _VP, _VP, _SZ, _SZP, _VP, _SZ, _SZP, _VP]),
+    "dmx_verify": (ctypes.c_int, [_VP, _VP, _VP, _SZ, _VP, _SZ, _VP]),
+    "dmx_prove_verify_device": (ctypes.c_int, [_VP, _VP, _SZ, _VP, _VP, _SZ, _SZP, _VP, _SZ, _SZP, _VP]),
+    "dmx_device_alloc": (ctypes.c_int, [_VP, _SZ, ctypes.POINTER(_VP)]),
+    "dmx_device_free": (ctypes.c_int, [_VP, _VP]),
+    "dmx_memcpy_htod": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
+    "dmx_memcpy_dtoh": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
+    "dmx_device_copy": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
+    "dmx_kernel_timing_enable": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dmx_kernel_timing": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
+                                         ctypes.POINTER(ctypes.c_double)]),
+    "dmx_statement_size": (_SZ, [_SZ]),
+    "dmx_record_size": (_SZ, [_SZ, _SZ]),
+    "dmx_proof_records_size": (_SZ, [ctypes.c_uint64]),
+}
+
+
+def lib():
+    """The loaded libdamascus_hip.so. Raises ImportError when it was never built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `make -C damascus000_amd/csrc` "
+                              "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_VP)
+
+
+def _check(rc):
+    if rc != DMX_OK:
+        raise DamascusError(rc, lib().dmx_last_error().decode(errors="replace"))
+
+
+def _bytes_ptr(b):
+    buf = ctypes.create_string_buffer(bytes(b), max(1, len(b)))
+    return buf
+
+
+def to_words(vals):
+    vals = [int(v) for v in vals]
+    out = np.zeros((len(vals), 2), dtype=np.uint64)
+    for i, v in enumerate(vals):
+        out[i, 0] = v & 0xFFFFFFFFFFFFFFFF
+        out[i, 1] = v >> 64
+    return out
+
+
+def from_words(w):
+    w = np.asarray(w, dtype=np.uint64).reshape(-1, 2)
+    return [int(lo) | (int(hi) << 64) for lo, hi in w]
+
+
+def backend_ready():
+    """cuda_backend_ready (src/utils/gpu.rs:23-25): library built, device present, DAMASCUS_GPU != 0"""
+    return bool(lib().dmx_backend_ready())
+
+
+def device_summary():
+    buf = ctypes.create_string_buffer(256)
+    lib().dmx_device_summary(buf, 256)
+    return buf.value.decode()
+
+
+class Context:
+    """Device context: NTT tables and CRT constants resident in HBM, shared by every object."""
+    _default = None
+
+    def __init__(self, device=0):
+        self.h = _VP()
+        _check(lib().dmx_ctx_create(device, ctypes.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dmx_ctx_destroy(self.h)
+            self.h = None
+
+    @classmethod
+    def default(cls):
+        if cls._default is None:
+            cls._default = Context()
+        return cls._default
+
+    def upload(self, data):
+        """copy host bytes into a new device buffer; returns a DeviceBuffer"""
+        buf = DeviceBuffer(self, len(data))
+        _check(lib().dmx_memcpy_htod(self.h, _VP(buf.ptr), _bytes_ptr(data), len(data)))
+        return buf
+
+    def kernel_timing(self, on=None):
+        """enable (on=True/False) or read the forward-transform timing record:
+        returns (total_ms, launches, lane_cycles) and clears it"""
+        if on is not None:
+            _check(lib().dmx_kernel_timing_enable(self.h, int(bool(on))))
+            return None
+        ms, n, w = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double()
+        _check(lib().dmx_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(w)))
+        return ms.value, n.value, w.value
+
+
+class DeviceBuffer:
+    """device memory owned through the context (no torch: one HIP runtime per process)"""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        p = _VP()
+        _check(lib().dmx_device_alloc(ctx.h, nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def copy_from(self, other):
+        _check(lib().dmx_device_copy(self.ctx.h, _VP(self.ptr), _VP(other.ptr), min(self.nbytes, other.nbytes)))
+
+    def read(self, nbytes=None):
+        n = self.nbytes if nbytes is None else nbytes
+        out = ctypes.create_string_buffer(max(n, 1))
+        _check(lib().dmx_memcpy_dtoh(self.ctx.h, out, _VP(self.ptr), n))
+        return out.raw[:n]
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.dmx_device_free(self.ctx.h, _VP(self.ptr))
+            self.ptr = None
+
+
+def derive_subseed(seed, label):
+    """derive_subseed (src/commitment/sis.rs:208-210): keyed BLAKE3(seed, label)"""
+    out = ctypes.create_string_buffer(32)
+    _check(lib().dmx_derive_subseed(_bytes_ptr(seed), _bytes_ptr(label), len(label), out))
+    return out.raw
+
+
+class ModuleSisCommitter:
+    """ModuleSisCommitter (src/commitment/sis.rs:67-196) on the GPU"""
+
+    def __init__(self, seed=bytes([7] * 32), ctx=None):
+        self.ctx = ctx or Context.default()
+        self.seed = bytes(seed)
+        self.g_seed = derive_subseed(self.seed, b"g0")
+
+    def generator_seed(self):
+        return self.g_seed
+
+    def generators_for(self, vector_len, ring_len):
+        out = np.zeros((vector_len, K, ring_len, 2), dtype=np.uint64)
+        _check(lib().dmx_derive_generators(self.ctx.h, _bytes_ptr(self.g_seed), vector_len, ring_len, _ptr(out)))
+        return out
+
+    def commit(self, witness):
+        """witness: (rows, ring_len, 2) words -> (8, ring_len_out, 2)"""
+        w = np.ascontiguousarray(witness, dtype=np.uint64)
+        rows = w.shape[0]
+        n = w.shape[1] if rows else 1
+        out = np.zeros((K, max(n, 1), 2), dtype=np.uint64)
+        ol = _SZ(0)
+        _check(lib().dmx_commit(self.ctx.h, _bytes_ptr(self.seed), _ptr(w), rows, n, _ptr(out), ctypes.byref(ol)))
+        return out[:, : ol.value]
+
+    def commit_with_generators(self, witness, gens):
+        w = np.ascontiguousarray(witness, dtype=np.uint64)
+        g = np.ascontiguousarray(gens, dtype=np.uint64)
+        rows = w.shape[0]
+        n = w.shape[1] if rows else 1
+        out = np.zeros((K, max(n, 1), 2), dtype=np.uint64)
+        ol = _SZ(0)
+        _check(lib().dmx_commit_with_generators(self.ctx.h, _ptr(w), _ptr(g), rows, n, _ptr(out), ctypes.byref(ol)))
+        return out[:, : ol.value]
+
+
+class DamascusProver:
+    """DamascusProver (src/protocol/prover.rs:41-313)"""
+
+    def __init__(self, handle, ctx):
+        self.h = handle
+        self.ctx = ctx
+
+    @classmethod
+    def initialize(cls, data, params=None, config=None, ctx=None):
+        """initialize / initialize_with_config (prover.rs:57-111) from the file's bytes"""
+        ctx = ctx or Context.default()
+        params = params or SystemParams()
+        config = config or RuntimeConfig()
+        h = _VP()
+        buf = _bytes_ptr(data)
+        _check(lib().dmx_prover_new(ctx.h, buf, len(data), ctypes.byref(params), ctypes.byref(config), ctypes.byref(h)))
+        return cls(h, ctx)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dmx_prover_free(self.h)
+            self.h = None
+
+    def rounds_total(self):
+        return lib().dmx_prover_rounds_total(self.h)
+
+    def current_round(self):
+        return lib().dmx_prover_current_round(self.h)
+
+    def current_dimensions(self):
+        r, n = _SZ(), _SZ()
+        lib().dmx_prover_current_dimensions(self.h, ctypes.byref(r), ctypes.byref(n))
+        return r.value, n.value
+
+    def statement(self):
+        """bincode bytes of the Statement (src/commitment/sis.rs:23-30)"""
+        n = _SZ()
+        _check(lib().dmx_prover_statement(self.h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().dmx_prover_statement(self.h, buf, n.value, ctypes.byref(n)))
+        return buf.raw
+
+    def current_commitment(self):
+        _, n = self.current_dimensions()
+        out = np.zeros((K, n, 2), dtype=np.uint64)
+        ln = _SZ()
+        _check(lib().dmx_prover_current_commitment(self.h, _ptr(out), out.size, ctypes.byref(ln)))
+        return out
+
+    def fold_round(self, round_idx, with_timing=False):
+        """fold_round (prover.rs:140-313): returns the bincode RoundRecord"""
+        _, n = self.current_dimensions()
+        cap = lib().dmx_record_size(max(n, 1), max(n // 2, 1))
+        buf = ctypes.create_string_buffer(cap)
+        got = _SZ()
+        t = RoundOutput()
+        _check(lib().dmx_prover_fold_round(self.h, round_idx, buf, cap, ctypes.byref(got), ctypes.byref(t)))
+        rec = buf.raw[: got.value]
+        if with_timing:
+            return rec, {"vector_fold_ns": t.vector_fold_ns, "poly_fold_ns": t.poly_fold_ns,
+                         "total_round_ns": t.total_round_ns}
+        return rec
+
+    def final_opening(self):
+        """final_opening (prover.rs:133-138): None until every round has run"""
+        out = np.zeros(2, dtype=np.uint64)
+        rc = lib().dmx_prover_final_opening(self.h, _ptr(out))
+        return None if rc != DMX_OK else from_words(out)[0]
+
+
+class DamascusVerifier:
+    """DamascusVerifier (src/protocol/verifier.rs:11-139)"""
+
+    def __init__(self, params, statement, ctx=None, _handle=None):
+        self.ctx = ctx or Context.default()
+        if _handle is not None:
+            self.h = _VP(_handle)
+            return
+        self.h = _VP()
+        params = params or SystemParams()
+        st = bytes(statement)
+        _check(lib().dmx_verifier_new(self.ctx.h, ctypes.byref(params), _bytes_ptr(st), len(st), ctypes.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.dmx_verifier_free(self.h)
+            self.h = None
+
+    def clone(self):
+        h = lib().dmx_verifier_clone(self.h)
+        if not h:
+            raise DamascusError(-3, "verifier clone failed")
+        return DamascusVerifier(None, None, ctx=self.ctx, _handle=h)
+
+    def round(self):
+        return lib().dmx_verifier_round(self.h)
+
+    def current_dimensions(self):
+        r, n = _SZ(), _SZ()
+        lib().dmx_verifier_current_dimensions(self.h, ctypes.byref(r), ctypes.byref(n))
+        return r.value, n.value
+
+    def current_commitment(self):
+        _, n = self.current_dimensions()
+        out = np.zeros((K, n, 2), dtype=np.uint64)
+        ln = _SZ()
+        _check(lib().dmx_verifier_current_commitment(self.h, _ptr(out), out.size, ctypes.byref(ln)))
+        return out
+
+    def update_commitment(self, record):
+        rec = bytes(record)
+        _check(lib().dmx_verifier_update_commitment(self.h, _bytes_ptr(rec), len(rec)))
+
+    def verify_final_opening(self, opening):
+        _check(lib().dmx_verifier_verify_final_opening(self.h, _ptr(to_words([opening]))))
+
+
+def prove(data, params=None, config=None, ctx=None):
+    """initialize + every fold_round; returns (statement, records_blob, opening)"""
+    ctx = ctx or Context.default()
+    params = params or SystemParams()
+    config = config or RuntimeConfig(max_preprocess_bytes=(1 << 64) - 1)
+    st_cap = lib().dmx_statement_size(2048 * 4)
+    rec_cap = lib().dmx_proof_records_size(len(data))
+    st = ctypes.create_string_buffer(st_cap)
+    recs = ctypes.create_string_buffer(max(rec_cap, 1))
+    sl, rl = _SZ(), _SZ()
+    op = np.zeros(2, dtype=np.uint64)
+    _check(lib().dmx_prove(ctx.h, _bytes_ptr(data), len(data), ctypes.byref(params), ctypes.byref(config), st, st_cap,
+                           ctypes.byref(sl), recs, rec_cap, ctypes.byref(rl), _ptr(op)))
+    return st.raw[: sl.value], recs.raw[: rl.value], from_words(op)[0]
+
+
+def verify(statement, records, opening, params=None, ctx=None):
+    ctx = ctx or Context.default()
+    params = params or SystemParams()
+    st, rb = bytes(statement), bytes(records)
+    _check(lib().dmx_verify(ctx.h, ctypes.byref(params), _bytes_ptr(st), len(st), _bytes_ptr(rb), len(rb),
+                            _ptr(to_words([opening]))))
+
+
+def prove_verify_device(d_file_ptr, length, params=None, ctx=None, want_proof=True):
+    """prove + replay with the file already in device memory (a device pointer as int).
+    Returns (statement, records_blob, opening) when want_proof, else None."""
+    ctx = ctx or Context.default()
+    params = params or SystemParams()
+    if want_proof:
+        st_cap = lib().dmx_statement_size(4096)
+        rec_cap = lib().dmx_proof_records_size(length)
+        st = ctypes.create_string_buffer(st_cap)
+        recs = ctypes.create_string_buffer(max(rec_cap, 1))
+        sl, rl = _SZ(), _SZ()
+        op = np.zeros(2, dtype=np.uint64)
+        _check(lib().dmx_prove_verify_device(ctx.h, _VP(d_file_ptr), length, ctypes.byref(params), st, st_cap,
+                                             ctypes.byref(sl), recs, rec_cap, ctypes.byref(rl), _ptr(op)))
+        return st.raw[: sl.value], recs.raw[: rl.value], from_words(op)[0]
+    _check(lib().dmx_prove_verify_device(ctx.h, _VP(d_file_ptr), length, ctypes.byref(params), None, 0, None, None, 0,
+                                         None, None))
+    return None
+
+
+def ntt_batch(a, b, n, primes, stage_roots, inv_roots, inv_sizes):
+    """damascus_cuda_ntt_batch replacement (src/utils/gpu.rs:140-207): returns (rc, out)"""
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    out = np.zeros_like(a)
+    arrs = [np.ascontiguousarray(x, dtype=np.uint64) for x in (primes, stage_roots, inv_roots, inv_sizes)]
+    rc = lib().dmx_ntt_batch(_ptr(a), _ptr(b), _ptr(out), n, *[_ptr(x) for x in arrs])
+    return rc, out
+
+
+def fold_batch(left, right, challenge):
+    """damascus_cuda_fold_batch replacement (src/utils/gpu.rs:95-138) on lists of ints"""
+    n = len(left)
+    lw, rw = to_words(left), to_words(right)
+    l_lo, l_hi = np.ascontiguousarray(lw[:, 0]), np.ascontiguousarray(lw[:, 1])
+    r_lo, r_hi = np.ascontiguousarray(rw[:, 0]), np.ascontiguousarray(rw[:, 1])
+    o_lo, o_hi = np.zeros(n, np.uint64), np.zeros(n, np.uint64)
+    rc = lib().dmx_fold_batch(_ptr(l_lo), _ptr(l_hi), _ptr(r_lo), _ptr(r_hi), _ptr(o_lo), _ptr(o_hi), n,
+                              challenge & 0xFFFFFFFFFFFFFFFF, challenge >> 64)
+    if rc != 0:
+        return rc, None
+    return rc, [int(a) | (int(b) << 64) for a, b in zip(o_lo, o_hi)]
+
+
+def ring_mul_batch(a_list, b_list, ctx=None):
+    """negacyclic products a_i * b_i of mixed power-of-two degrees; lists of (n_i, 2) words"""
+    ctx = ctx or Context.default()
+    logs = np.array([int(x.shape[0]).bit_length() - 1 for x in a_list], dtype=np.uint32)
+    A = np.ascontiguousarray(np.concatenate(a_list), dtype=np.uint64)
+    B = np.ascontiguousarray(np.concatenate(b_list), dtype=np.uint64)
+    out = np.zeros_like(A)
+    _check(lib().dmx_ring_mul_batch(ctx.h, _ptr(A), _ptr(B), _ptr(out), _ptr(logs), len(a_list)))
+    res, pos = [], 0
+    for x in a_list:
+        res.append(out[pos: pos + x.shape[0]])
+        pos += x.shape[0]
+    return res

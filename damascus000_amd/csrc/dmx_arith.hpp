@@ -1,0 +1,293 @@
+// Exact integer arithmetic shared by host and device code.
+//
+// Z_q, q = 2^112 - 75 (src/utils/config.rs:4; src/algebra/field.rs), held as four 32-bit
+// little-endian limbs in a 16-byte word: the same bytes as Fp::to_le_bytes (field.rs:81-83).
+// Every routine returns the canonical residue in [0, q), so any correct reduction order gives
+// the reference's bits.
+//
+// Mod-p arithmetic for the eight CRT primes (config.rs:9-18), all below 2^32, uses Montgomery
+// reduction with R = 2^32 built on 32-bit mul_lo / mul_hi:
+//   redc(T)  = T * R^-1 mod p          for T < p * R, via t = T_hi - mulhi(T_lo * p^-1, p)
+//   mulc(b,w) = b * w mod p            for any 32-bit b, with w stored as (w*R mod p) and
+//                                       (w*R*p^-1 mod 2^32) so one product is never formed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DMX_HD __host__ __device__ __forceinline__
+
+namespace dmx {
+
+constexpr int K = 8;            // MODULE_RANK (config.rs:7)
+constexpr int NPRIMES = 8;      // CRT_PRIMES.len() (config.rs:9-18)
+constexpr int BYTES_PER_COEFF = 13;
+constexpr int MIN_FOLD_DEPTH = 6;
+constexpr int MAX_LOG_N = 13;   // ring degrees up to 2^13 (witness depth d <= 13)
+constexpr int TW_SIZE = 1 << MAX_LOG_N;
+
+constexpr uint32_t PRIMES[NPRIMES] = {3892314113u, 2281701377u, 2013265921u, 2885681153u,
+                                      2483027969u, 1811939329u, 469762049u,  4194304001u};
+
+// ------------------------------------------------------------------------------------------
+// Z_q
+// ------------------------------------------------------------------------------------------
+struct alignas(16) zq {
+    uint32_t w[4];
+};
+
+constexpr uint32_t Q0 = 0xFFFFFFB5u;  // 2^112 - 75, limb 0
+constexpr uint32_t Q1 = 0xFFFFFFFFu;
+constexpr uint32_t Q2 = 0xFFFFFFFFu;
+constexpr uint32_t Q3 = 0x0000FFFFu;
+
+DMX_HD zq zq_zero() { return zq{{0, 0, 0, 0}}; }
+DMX_HD zq zq_from_u64(uint64_t v) { return zq{{(uint32_t)v, (uint32_t)(v >> 32), 0, 0}}; }
+DMX_HD bool zq_is_zero(const zq& a) { return (a.w[0] | a.w[1] | a.w[2] | a.w[3]) == 0; }
+DMX_HD bool zq_eq(const zq& a, const zq& b) {
+    return ((a.w[0] ^ b.w[0]) | (a.w[1] ^ b.w[1]) | (a.w[2] ^ b.w[2]) | (a.w[3] ^ b.w[3])) == 0;
+}
+
+// a >= q for a < 2^113 held in 4 limbs (limb 3 may carry bit 16)
+DMX_HD bool zq_ge_q(const zq& a) {
+    if (a.w[3] != Q3) return a.w[3] > Q3;
+    if (a.w[2] != Q2) return a.w[2] > Q2;
+    if (a.w[1] != Q1) return a.w[1] > Q1;
+    return a.w[0] >= Q0;
+}
+
+DMX_HD zq zq_sub_q(const zq& a) {
+    zq r;
+    uint64_t t = (uint64_t)a.w[0] - Q0;
+    r.w[0] = (uint32_t)t;
+    uint64_t br = (t >> 63);
+    t = (uint64_t)a.w[1] - Q1 - br;
+    r.w[1] = (uint32_t)t;
+    br = (t >> 63);
+    t = (uint64_t)a.w[2] - Q2 - br;
+    r.w[2] = (uint32_t)t;
+    br = (t >> 63);
+    r.w[3] = a.w[3] - Q3 - (uint32_t)br;
+    return r;
+}
+
+// canonical add (field.rs:134-145)
+DMX_HD zq zq_add(const zq& a, const zq& b) {
+    zq r;
+    uint64_t t = (uint64_t)a.w[0] + b.w[0];
+    r.w[0] = (uint32_t)t;
+    t = (uint64_t)a.w[1] + b.w[1] + (t >> 32);
+    r.w[1] = (uint32_t)t;
+    t = (uint64_t)a.w[2] + b.w[2] + (t >> 32);
+    r.w[2] = (uint32_t)t;
+    r.w[3] = a.w[3] + b.w[3] + (uint32_t)(t >> 32);
+    return zq_ge_q(r) ? zq_sub_q(r) : r;
+}
+
+// canonical sub (field.rs:153-161): a - b, plus q when it borrows
+DMX_HD zq zq_sub(const zq& a, const zq& b) {
+    zq r;
+    uint64_t t = (uint64_t)a.w[0] - b.w[0];
+    r.w[0] = (uint32_t)t;
+    uint64_t br = t >> 63;
+    t = (uint64_t)a.w[1] - b.w[1] - br;
+    r.w[1] = (uint32_t)t;
+    br = t >> 63;
+    t = (uint64_t)a.w[2] - b.w[2] - br;
+    r.w[2] = (uint32_t)t;
+    br = t >> 63;
+    uint64_t t3 = (uint64_t)a.w[3] - b.w[3] - br;
+    r.w[3] = (uint32_t)t3;
+    if (t3 >> 63) {  // negative: add q back
+        uint64_t s = (uint64_t)r.w[0] + Q0;
+        r.w[0] = (uint32_t)s;
+        s = (uint64_t)r.w[1] + Q1 + (s >> 32);
+        r.w[1] = (uint32_t)s;
+        s = (uint64_t)r.w[2] + Q2 + (s >> 32);
+        r.w[2] = (uint32_t)s;
+        r.w[3] = r.w[3] + Q3 + (uint32_t)(s >> 32);
+    }
+    return r;
+}
+
+// canonical negation (field.rs:183-193)
+DMX_HD zq zq_neg(const zq& a) { return zq_is_zero(a) ? a : zq_sub(zq{{Q0, Q1, Q2, Q3}}, a); }
+
+// Reduce a value given as 8 x 32-bit limbs (< 2^256) mod q, using 2^112 = 75 (mod q):
+// x = lo112 + mid112 * 2^112 + top32 * 2^224  ==  lo + 75*mid + 5625*top.
+DMX_HD zq zq_reduce256(const uint32_t x[8]) {
+    // split into 112-bit pieces
+    uint32_t lo[4] = {x[0], x[1], x[2], x[3] & 0xFFFFu};
+    uint32_t mid[4] = {(x[3] >> 16) | (x[4] << 16), (x[4] >> 16) | (x[5] << 16),
+                       (x[5] >> 16) | (x[6] << 16), (x[6] >> 16)};
+    uint32_t top = x[7];
+    // s = lo + 75*mid + 5625*top  (< 2^119)
+    uint64_t c;
+    uint32_t s[4];
+    c = (uint64_t)lo[0] + (uint64_t)mid[0] * 75u + (uint64_t)top * 5625u;
+    s[0] = (uint32_t)c;
+    c = (c >> 32) + lo[1] + (uint64_t)mid[1] * 75u;
+    s[1] = (uint32_t)c;
+    c = (c >> 32) + lo[2] + (uint64_t)mid[2] * 75u;
+    s[2] = (uint32_t)c;
+    c = (c >> 32) + lo[3] + (uint64_t)mid[3] * 75u;
+    s[3] = (uint32_t)c;  // < 2^23 since total < 2^119
+    // fold bits >= 112 once more: s = (s mod 2^112) + 75 * (s >> 112)
+    uint32_t hi = s[3] >> 16;
+    zq r;
+    c = (uint64_t)s[0] + (uint64_t)hi * 75u;
+    r.w[0] = (uint32_t)c;
+    c = (c >> 32) + s[1];
+    r.w[1] = (uint32_t)c;
+    c = (c >> 32) + s[2];
+    r.w[2] = (uint32_t)c;
+    r.w[3] = (s[3] & 0xFFFFu) + (uint32_t)(c >> 32);
+    if (zq_ge_q(r)) r = zq_sub_q(r);
+    if (zq_ge_q(r)) r = zq_sub_q(r);
+    return r;
+}
+
+// 112 x 112 -> 224-bit schoolbook product into 8 limbs
+DMX_HD void zq_mul_wide(const zq& a, const zq& b, uint32_t out[8]) {
+    uint64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        uint64_t carry = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint64_t t = (uint64_t)a.w[i] * b.w[j] + (uint32_t)acc[i + j] + carry;
+            acc[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        acc[i + 4] = carry;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) out[i] = (uint32_t)acc[i];
+}
+
+// canonical product (field.rs:229-258 computes the same residue)
+DMX_HD zq zq_mul(const zq& a, const zq& b) {
+    uint32_t t[8];
+    zq_mul_wide(a, b, t);
+    return zq_reduce256(t);
+}
+
+// a + c*b
+DMX_HD zq zq_fma(const zq& a, const zq& b, const zq& c) { return zq_add(a, zq_mul(b, c)); }
+
+// Fermat inverse x^(q-2) (field.rs:106-109)
+DMX_HD zq zq_pow(zq base, const zq& e) {
+    zq r = zq_from_u64(1);
+    for (int limb = 0; limb < 4; ++limb) {
+        uint32_t bits = e.w[limb];
+        int nb = (limb == 3) ? 16 : 32;
+        for (int i = 0; i < nb; ++i) {
+            if (bits & 1u) r = zq_mul(r, base);
+            base = zq_mul(base, base);
+            bits >>= 1;
+        }
+    }
+    return r;
+}
+DMX_HD zq zq_inv(const zq& a) { return zq_pow(a, zq{{Q0 - 2, Q1, Q2, Q3}}); }
+
+// LE256 bytes (as 8 words) mod q: Fp::from_le_bytes_mod_order on a 32-byte digest (field.rs:45-63)
+DMX_HD zq zq_from_digest(const uint32_t h[8]) { return zq_reduce256(h); }
+
+// 13-byte little-endian chunk (< 2^104 < q): Fp::from_le_chunk (field.rs:65-71)
+
+// ------------------------------------------------------------------------------------------
+// mod-p (Montgomery, R = 2^32)
+// ------------------------------------------------------------------------------------------
+struct PrimeConsts {
+    uint32_t p;
+    uint32_t pinv;       // p^-1 mod 2^32
+    uint32_t r4_m, r4_mp;  // R^3 as a mulc constant: (R^4 mod p, (R^4 mod p) * pinv)
+    uint32_t two64;      // 2^64 mod p
+    uint32_t q_mod_p;    // q mod p
+    uint32_t pad[2];
+};
+
+#if defined(__HIP_DEVICE_COMPILE__)
+DMX_HD uint32_t mulhi32(uint32_t a, uint32_t b) { return __umulhi(a, b); }
+#else
+DMX_HD uint32_t mulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
+#endif
+
+// T * R^-1 mod p for T < p*R (result in [0, p))
+DMX_HD uint32_t redc(uint64_t T, uint32_t p, uint32_t pinv) {
+    uint32_t lo = (uint32_t)T, hi = (uint32_t)(T >> 32);
+    uint32_t m = lo * pinv;
+    uint32_t mh = mulhi32(m, p);
+    uint32_t t = hi - mh;
+    return hi < mh ? t + p : t;
+}
+
+// b * w mod p for any 32-bit b; (wm, wmp) = (w*R mod p, wm*pinv mod 2^32)
+DMX_HD uint32_t mulc(uint32_t b, uint32_t wm, uint32_t wmp, uint32_t p) {
+    uint32_t hi = mulhi32(b, wm);
+    uint32_t m = b * wmp;
+    uint32_t mh = mulhi32(m, p);
+    uint32_t t = hi - mh;
+    return hi < mh ? t + p : t;
+}
+
+DMX_HD uint32_t addp(uint32_t a, uint32_t b, uint32_t p) {
+    uint32_t d = p - b;  // in (0, p]
+    uint32_t s = a - d;
+    return a < d ? s + p : s;
+}
+DMX_HD uint32_t subp(uint32_t a, uint32_t b, uint32_t p) {
+    uint32_t s = a - b;
+    return a < b ? s + p : s;
+}
+
+// x mod p for a canonical Z_q coefficient: y = x*R^-3 by three REDC steps, then * R^3.
+DMX_HD uint32_t zq_mod_p(const zq& x, const PrimeConsts& c) {
+    uint32_t y = redc((uint64_t)x.w[0], c.p, c.pinv);
+    y = redc((uint64_t)y + x.w[1], c.p, c.pinv);
+    y = redc((uint64_t)y + x.w[2], c.p, c.pinv);
+    y = y + x.w[3];  // < p + 2^16 < 2^32
+    return mulc(y, c.r4_m, c.r4_mp, c.p);
+}
+
+// Residue of the centred representative of x (x - q when x > (q-1)/2). Products of centred
+// representatives are bounded by m*(q/2)^2, which keeps every commitment sum of up to 2^24
+// coefficient products inside the CRT range |V| < P/2 (P ~ 2^248).
+DMX_HD bool zq_gt_half(const zq& x) {  // x > (q-1)/2 = 2^111 - 38
+    if (x.w[3] != 0x7FFFu) return x.w[3] > 0x7FFFu;
+    if (x.w[2] != 0xFFFFFFFFu || x.w[1] != 0xFFFFFFFFu) return false;
+    return x.w[0] > 0xFFFFFFDAu;
+}
+DMX_HD uint32_t zq_mod_p_centred(const zq& x, const PrimeConsts& c) {
+    const uint32_t r = zq_mod_p(x, c);
+    return zq_gt_half(x) ? subp(r, c.q_mod_p, c.p) : r;
+}
+
+// (hi * 2^64 + lo) mod p for the u96 accumulators of the MAC kernels
+DMX_HD uint32_t reduce_u96(uint64_t lo, uint32_t hi, const PrimeConsts& c) {
+    uint64_t a = lo % c.p;
+    uint64_t b = ((uint64_t)(hi % c.p) * c.two64) % c.p;
+    uint64_t s = a + b;
+    return (uint32_t)(s >= c.p ? s - c.p : s);
+}
+
+// ------------------------------------------------------------------------------------------
+// host-side helpers for constants
+// ------------------------------------------------------------------------------------------
+inline uint64_t host_powmod(uint64_t b, uint64_t e, uint64_t p) {
+    unsigned __int128 r = 1, x = b % p;
+    while (e) {
+        if (e & 1) r = (r * x) % p;
+        x = (x * x) % p;
+        e >>= 1;
+    }
+    return (uint64_t)r;
+}
+inline uint32_t host_inv32(uint32_t p) {  // p^-1 mod 2^32 (Newton)
+    uint32_t x = p;
+    for (int i = 0; i < 5; ++i) x *= 2u - p * x;
+    return x;
+}
+
+}  // namespace dmx

@@ -1,0 +1,138 @@
+// Kernel-side data structures and the launch wrappers the host runtime calls.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "dmx_arith.hpp"
+
+namespace dmx {
+
+// Device-resident NTT tables (one context owns them).
+// zeta[p][k] = psi^brv13(k) with psi a primitive 2^14-th root mod p; the first m entries are
+// exactly the size-m negacyclic table (the tables nest), so one table serves every degree.
+struct NttTables {
+    const uint32_t* zeta_m;    // [NPRIMES][TW_SIZE]  Montgomery-form twiddles
+    const uint32_t* zeta_mp;   // [NPRIMES][TW_SIZE]  twiddle * p^-1 mod 2^32
+    const uint32_t* izeta_m;   // inverse twiddles psi^-brv13(k)
+    const uint32_t* izeta_mp;
+    const uint32_t* invn_m;    // [NPRIMES][MAX_LOG_N+1]  m^-1 (Montgomery form)
+    const uint32_t* invn_mp;
+};
+
+// Garner / centred-lift constants for the CRT primes (src/algebra/ntt.rs:379-427)
+struct CrtConsts {
+    uint32_t mod_prefix[NPRIMES][NPRIMES];  // (p_0..p_{j-1}) mod p_i, j < i
+    uint32_t inv[NPRIMES];                  // ((p_0..p_{i-1}) mod p_i)^-1 mod p_i
+    uint32_t half_digits[NPRIMES];          // mixed-radix digits of floor(P/2)
+    zq prefix_q[NPRIMES];                   // (p_0..p_{j-1}) mod q
+    zq P_q;                                 // P mod q
+};
+
+// Where a forward-NTT row comes from.
+enum RowMode : int {
+    ROW_PLAIN = 0,   // x = a[row][c]
+    ROW_VFOLD = 1,   // x = a[row][c] + s * a[row + b_off][c]            (vector fold)
+    ROW_PFOLD_W = 2, // x = a[row][2c] + s * a[row][2c+1]                (poly fold, witness)
+    ROW_PFOLD_G = 3, // x = a[row][2c] + s * (X * odd)[c]                 (poly fold, generators)
+    ROW_GEN = 4,     // x = keyed_blake3(g_seed, "g"||i||k||c) mod q, row = i*K + k
+};
+
+struct RowSrc {
+    const zq* a;       // source rows (ROW_GEN: unused)
+    zq* dst;           // when non-null the Z_q row is stored here (row-major, m per row)
+    size_t b_off;      // ROW_VFOLD partner offset in rows
+    const zq* scal;    // ROW_*FOLD scalar (device)
+    int mode;
+    uint32_t gseed[8]; // ROW_GEN key words
+};
+
+void set_device_constants(const PrimeConsts* pc, const CrtConsts* crt, hipStream_t s);
+
+// forward negacyclic NTT of `rows` rows of degree 2^log_m, all 8 primes; out[(row*8+p)*m + c]
+void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out,
+                         const NttTables& tw, hipStream_t s);
+// inverse NTT in place on `slices` slices of [8][m]: data[(slice*8 + p)*m + c], times m^-1
+void launch_ntt_inv(uint32_t* data, size_t slices, int log_m, const NttTables& tw, hipStream_t s);
+// inv = sum over all rows W[i]*G[i,k]; L = sum_{i<h} G[i+h,k]*W[i]; R = sum_{i<h} G[i,k]*W[i+h]
+// (h = rows/2; L/R skipped when out_L == nullptr). Outputs [k][8][m] residues.
+void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
+                      uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, hipStream_t s);
+// vector-stage invariant at degree m plus the poly-stage cross terms at degree m/2, from the
+// NTT-domain even/odd split: out_inv [k][8][m], out_L/out_R [k][8][m/2]
+void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
+                      uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, const NttTables& tw,
+                      hipStream_t s);
+// Z_q rows only (no transform): dst[row][c] = row_value(src, row, c)
+void launch_fold_rows_zq(const RowSrc& src, size_t rows, int log_m, hipStream_t s);
+void launch_final_check(const zq* g, const zq* c, const zq* opening, size_t n, uint32_t* flags,
+                        uint32_t bit, hipStream_t s);
+// residues [slice][8][m] -> Z_q [slice][m]
+void launch_crt_lift(const uint32_t* res, zq* out, size_t slices, int log_m, hipStream_t s);
+
+// commitment updates (tiny, K x m)
+// out = c + sinv*l + s*r          (VEC; scalars at scal[0]=s, scal[1]=s^-1)
+// out = even(c) + sinv*l + s*r    (POLY; c has 2m coefficients per coordinate)
+void launch_com_update(const zq* c, const zq* l, const zq* r, const zq* scal, int poly, int log_m,
+                       zq* out, hipStream_t s);
+// plain 16-byte copy (HBM calibration)
+void launch_copy16(void* dst, const void* src, size_t n16, hipStream_t s);
+// out = a + b over n Z_q coefficients
+void launch_zq_add(const zq* a, const zq* b, zq* out, size_t n, hipStream_t s);
+// flags[bit] set when a != b over n coefficients
+void launch_com_check(const zq* a, const zq* b, size_t n, uint32_t* flags, uint32_t bit,
+                      hipStream_t s);
+
+// witness expansion (src/utils/io.rs:51-54): 13-byte chunks into a dim x dim Z_q matrix
+void launch_expand(const uint8_t* file, size_t len, zq* out, size_t capacity, hipStream_t s);
+
+// ---- BLAKE3 tree hashing on device ----
+struct TranscriptMsg {
+    const uint8_t* epoch_seed;  // 32 B device
+    const uint8_t* file_id;     // 32 B device
+    uint32_t label_len;
+    uint8_t label[4];
+    uint64_t round;
+    const zq* mods[3];
+    uint32_t lens[3];
+};
+size_t transcript_len(uint32_t label_len, const uint32_t lens[3]);
+void launch_transcript_pack(const TranscriptMsg& t, uint8_t* msg, size_t len, hipStream_t s);
+void launch_b3_chunks(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags,
+                      uint32_t* cvs, hipStream_t s);
+// merge groups of 1024 chunk CVs into one CV each (non-root)
+void launch_b3_group_merge(const uint32_t* cvs, size_t n, const uint32_t key[8], uint32_t flags,
+                           uint32_t* out, hipStream_t s);
+// final merge to the root. digest_out (8 words) when non-null. For a challenge
+// (chal_out != nullptr): LE256(digest) mod q, re-hashing with counter+1 while zero, then
+// chal_out[0] = x, chal_out[1] = x^-1. msg/len are needed for n_chunks == 1 and retries.
+void launch_b3_root(const uint32_t* cvs, size_t n, const uint8_t* msg, size_t len,
+                    const uint32_t key[8], uint32_t flags, uint32_t* digest_out, zq* chal_out,
+                    uint32_t* err_flags, int chunk_level, hipStream_t s);
+// whole-buffer hash (chunks, group merges, root); cv_scratch holds device_hash_scratch_words
+void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags,
+                 uint32_t* cv_scratch, size_t cv_scratch_words, uint32_t* digest_out, zq* chal_out,
+                 uint32_t* err_flags, hipStream_t s);
+size_t device_hash_scratch_words(size_t len);
+
+// generator expansion into Z_q rows: g[(i*K + k)*m + c] (src/commitment/sis.rs:212-245)
+void launch_gen_generators(const uint32_t gseed[8], size_t rows, int log_m, zq* out,
+                           hipStream_t s);
+
+// the reference-compatible batch entry points
+void launch_fold_pairs(const uint64_t* l_lo, const uint64_t* l_hi, const uint64_t* r_lo,
+                       const uint64_t* r_hi, uint64_t* o_lo, uint64_t* o_hi, size_t len,
+                       uint64_t c_lo, uint64_t c_hi, hipStream_t s);
+void launch_cyclic_ntt_batch(uint64_t* a, uint64_t* b, size_t n, const uint64_t* primes,
+                             const uint64_t* stage_roots, const uint64_t* inv_roots,
+                             const uint64_t* inv_sizes, hipStream_t s);
+// elementwise residue products: a[i] = a[i] * b[i] (mod p of slice)
+void launch_pointwise(uint32_t* a, const uint32_t* b, size_t slices, int log_m, hipStream_t s);
+// gather rows by index into a dense block (for ragged batching)
+void launch_gather_rows(const zq* src, const uint64_t* offs, const uint32_t* idx, size_t count,
+                        int log_m, zq* dst, hipStream_t s);
+void launch_scatter_rows(const zq* src, const uint64_t* offs, const uint32_t* idx, size_t count,
+                         int log_m, zq* dst, hipStream_t s);
+
+}  // namespace dmx

@@ -1,0 +1,666 @@
+// Negacyclic CRT-NTT kernels for gfx950: forward transforms with fused row producers
+// (fold / generator hash), NTT-domain multiply-accumulate reductions for the commitment
+// mat-vec, inverse transforms and the exact centred CRT lift to Z_q.
+//
+// Reference behaviour: src/algebra/ntt.rs:112-224 (ring product via CRT over eight primes),
+// ntt.rs:400-427 (centred lift), src/commitment/sis.rs:146-167 (commitment sum).
+// The reference transforms each product separately with a 2n-point cyclic NTT; here each ring
+// element is transformed once with an n-point negacyclic NTT and the commitment sum is formed
+// in the NTT domain, so a commitment costs (rows*(1+K) + K) transforms instead of 3*rows*K.
+#include "dmx_blake3.hpp"
+#include "dmx_kernels.hpp"
+
+namespace dmx {
+
+__constant__ PrimeConsts c_pc[NPRIMES];
+__constant__ CrtConsts c_crt;
+
+void set_device_constants(const PrimeConsts* pc, const CrtConsts* crt, hipStream_t s) {
+    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_pc), pc, sizeof(PrimeConsts) * NPRIMES, 0,
+                                 hipMemcpyHostToDevice, s);
+    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_crt), crt, sizeof(CrtConsts), 0, hipMemcpyHostToDevice, s);
+}
+
+constexpr int NTT_THREADS = 256;
+constexpr int LDS_WORDS = 16384;  // 64 KiB: all 8 primes up to degree 2048
+
+__device__ __forceinline__ zq load_zq(const zq* p) {
+    uint4 v = *reinterpret_cast<const uint4*>(p);
+    return zq{{v.x, v.y, v.z, v.w}};
+}
+__device__ __forceinline__ void store_zq(zq* p, const zq& x) {
+    *reinterpret_cast<uint4*>(p) = make_uint4(x.w[0], x.w[1], x.w[2], x.w[3]);
+}
+
+// keyed BLAKE3 of "g" || u64le(i) || u64le(k) || u64le(c) (25 bytes, one block), reduced mod q
+// (src/commitment/sis.rs:231-245)
+__device__ __forceinline__ zq gen_coeff(const uint32_t key[8], uint64_t i, uint32_t k, uint64_t c) {
+    uint32_t m[16];
+    m[0] = 0x67u | ((uint32_t)i << 8);
+    m[1] = (uint32_t)(i >> 24);
+    m[2] = (uint32_t)(i >> 56) | (k << 8);
+    m[3] = 0;  // k < 2^24
+    m[4] = ((uint32_t)c) << 8;
+    m[5] = (uint32_t)(c >> 24);
+    m[6] = (uint32_t)(c >> 56);
+#pragma unroll
+    for (int t = 7; t < 16; ++t) m[t] = 0;
+    // byte 17 starts u64le(c): bytes 17..24 -> word 4 bits 8.., word 5, word 6 bits 0..7
+    uint32_t h[8];
+    b3::compress_cv(key, m, 0, 25,
+                    b3::CHUNK_START | b3::CHUNK_END | b3::ROOT | b3::KEYED_HASH, h);
+    return zq_from_digest(h);
+}
+
+// produce coefficient c of row `row` per the row source
+__device__ __forceinline__ zq row_value(const RowSrc& src, size_t row, uint32_t c, int log_m) {
+    const size_t m = (size_t)1 << log_m;
+    switch (src.mode) {
+        case ROW_PLAIN:
+            return load_zq(src.a + row * m + c);
+        case ROW_VFOLD: {
+            zq a = load_zq(src.a + row * m + c);
+            zq b = load_zq(src.a + (row + src.b_off) * m + c);
+            return zq_add(a, zq_mul(b, *src.scal));
+        }
+        case ROW_PFOLD_W: {
+            const zq* base = src.a + row * (2 * m);
+            zq e = load_zq(base + 2 * c), o = load_zq(base + 2 * c + 1);
+            return zq_add(e, zq_mul(o, *src.scal));
+        }
+        case ROW_PFOLD_G: {
+            const zq* base = src.a + row * (2 * m);
+            zq e = load_zq(base + 2 * c);
+            // (X * odd)[c] in Z_q[X]/(X^m + 1): odd[c-1] for c >= 1, -odd[m-1] at c = 0
+            zq xo = c ? load_zq(base + 2 * c - 1) : zq_neg(load_zq(base + 2 * m - 1));
+            return zq_add(e, zq_mul(xo, *src.scal));
+        }
+        default: {  // ROW_GEN
+            uint64_t i = row / K;
+            uint32_t k = (uint32_t)(row % K);
+            return gen_coeff(src.gseed, i, k, c);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// forward negacyclic NTT (Cooley-Tukey, natural order in, bit-reversed order out)
+// butterfly: (a, b) -> (a + z*b, a - z*b), z = zeta[m/(2*len) + block]
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(RowSrc src, int log_m,
+                                                               uint32_t* __restrict__ out,
+                                                               NttTables tw) {
+    extern __shared__ uint32_t sh[];
+    const int m = 1 << log_m;
+    const size_t row = blockIdx.x;
+    const int pg = min(NPRIMES, LDS_WORDS >> log_m);  // primes per LDS pass
+    for (int p0 = 0; p0 < NPRIMES; p0 += pg) {
+        // 1) produce the row and reduce it into the primes of this group
+        for (int c = threadIdx.x; c < m; c += NTT_THREADS) {
+            zq x = row_value(src, row, c, log_m);
+            if (src.dst && p0 == 0) store_zq(src.dst + row * m + c, x);
+#pragma unroll 1
+            for (int q = 0; q < pg; ++q) sh[q * m + c] = zq_mod_p_centred(x, c_pc[p0 + q]);
+        }
+        __syncthreads();
+        // 2) log_m butterfly layers in LDS
+        const int half = m >> 1;
+        for (int len = half; len >= 1; len >>= 1) {
+            const int lg = __builtin_ctz(len);
+            for (int t = threadIdx.x; t < pg * half; t += NTT_THREADS) {
+                const int q = t >> (log_m - 1);
+                const int b = t & (half - 1);
+                const int blk = b >> lg;
+                const int j = (blk << (lg + 1)) | (b & (len - 1));
+                const int kz = (half >> lg) + blk;  // m/(2 len) + blk
+                const int pi = p0 + q;
+                const uint32_t p = c_pc[pi].p;
+                const uint32_t zm = tw.zeta_m[pi * TW_SIZE + kz];
+                const uint32_t zmp = tw.zeta_mp[pi * TW_SIZE + kz];
+                uint32_t* s = sh + q * m;
+                const uint32_t u = s[j];
+                const uint32_t v = mulc(s[j + len], zm, zmp, p);
+                s[j] = addp(u, v, p);
+                s[j + len] = subp(u, v, p);
+            }
+            __syncthreads();
+        }
+        // 3) write out
+        for (int t = threadIdx.x; t < pg * m; t += NTT_THREADS) {
+            const int q = t >> log_m, c = t & (m - 1);
+            out[(row * NPRIMES + p0 + q) * m + c] = sh[t];
+        }
+        __syncthreads();
+    }
+}
+
+void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out,
+                         const NttTables& tw, hipStream_t s) {
+    if (rows == 0) return;
+    const int m = 1 << log_m;
+    const int pg = std::min(NPRIMES, LDS_WORDS >> log_m);
+    const size_t lds = (size_t)pg * m * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_ntt_fwd_rows, dim3((unsigned)rows), dim3(NTT_THREADS), lds, s, src, log_m,
+                       out, tw);
+}
+
+// ---------------------------------------------------------------------------------------
+// inverse (Gentleman-Sande): (x, y) -> (x + y, (x - y) * zinv), then * m^-1
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_inv(uint32_t* __restrict__ data, int log_m,
+                                                         NttTables tw) {
+    extern __shared__ uint32_t sh[];
+    const int m = 1 << log_m;
+    const size_t slice = blockIdx.x;  // (slice, prime) pairs: blockIdx.x = slice*8 + p
+    const int pi = (int)(slice % NPRIMES);
+    const uint32_t p = c_pc[pi].p;
+    uint32_t* d = data + slice * m;
+    for (int c = threadIdx.x; c < m; c += NTT_THREADS) sh[c] = d[c];
+    __syncthreads();
+    const int half = m >> 1;
+    for (int len = 1; len < m; len <<= 1) {
+        const int lg = __builtin_ctz(len);
+        for (int b = threadIdx.x; b < half; b += NTT_THREADS) {
+            const int blk = b >> lg;
+            const int j = (blk << (lg + 1)) | (b & (len - 1));
+            const int kz = (half >> lg) + blk;
+            const uint32_t u = sh[j], v = sh[j + len];
+            sh[j] = addp(u, v, p);
+            sh[j + len] = mulc(subp(u, v, p), tw.izeta_m[pi * TW_SIZE + kz],
+                               tw.izeta_mp[pi * TW_SIZE + kz], p);
+        }
+        __syncthreads();
+    }
+    const uint32_t im = tw.invn_m[pi * (MAX_LOG_N + 1) + log_m];
+    const uint32_t imp = tw.invn_mp[pi * (MAX_LOG_N + 1) + log_m];
+    for (int c = threadIdx.x; c < m; c += NTT_THREADS) d[c] = mulc(sh[c], im, imp, p);
+}
+
+void launch_ntt_inv(uint32_t* data, size_t slices, int log_m, const NttTables& tw, hipStream_t s) {
+    if (slices == 0) return;
+    const size_t lds = ((size_t)1 << log_m) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_ntt_inv, dim3((unsigned)(slices * NPRIMES)), dim3(NTT_THREADS), lds, s, data,
+                       log_m, tw);
+}
+
+// ---------------------------------------------------------------------------------------
+// NTT-domain multiply-accumulate: sums of residue products kept exact in 96 bits
+// ---------------------------------------------------------------------------------------
+struct Acc96 {
+    uint64_t lo;
+    uint32_t hi;
+    __device__ __forceinline__ void zero() {
+        lo = 0;
+        hi = 0;
+    }
+    __device__ __forceinline__ void mac(uint32_t a, uint32_t b) {
+        const uint64_t pr = (uint64_t)a * b;
+        lo += pr;
+        hi += (lo < pr) ? 1u : 0u;
+    }
+};
+
+constexpr int MAC_THREADS = 256;
+
+// thread = (k, p, c) with c fastest
+__global__ void __launch_bounds__(MAC_THREADS) k_mac_pairs(const uint32_t* __restrict__ What,
+                                                           const uint32_t* __restrict__ Ghat,
+                                                           uint32_t rows, int log_m,
+                                                           uint32_t* __restrict__ out_inv,
+                                                           uint32_t* __restrict__ out_L,
+                                                           uint32_t* __restrict__ out_R) {
+    const size_t m = (size_t)1 << log_m;
+    const size_t gid = (size_t)blockIdx.x * MAC_THREADS + threadIdx.x;
+    if (gid >= (size_t)K * NPRIMES * m) return;
+    const uint32_t c = (uint32_t)(gid & (m - 1));
+    const uint32_t p = (uint32_t)((gid >> log_m) % NPRIMES);
+    const uint32_t k = (uint32_t)((gid >> log_m) / NPRIMES);
+    Acc96 inv, L, R;
+    inv.zero();
+    L.zero();
+    R.zero();
+    const size_t wstride = NPRIMES * m;             // per witness row
+    const size_t gstride = (size_t)K * NPRIMES * m;  // per generator row i
+    const uint32_t* wp = What + p * m + c;
+    const uint32_t* gp = Ghat + ((size_t)k * NPRIMES + p) * m + c;
+    const bool cross = out_L != nullptr && rows > 1;  // rows is even when cross terms are asked
+    if (!cross) {
+        for (uint32_t i = 0; i < rows; ++i) inv.mac(wp[i * wstride], gp[i * gstride]);
+    } else {
+        const uint32_t h = rows >> 1;
+        for (uint32_t i = 0; i < h; ++i) {
+            const uint32_t w0 = wp[i * wstride], w1 = wp[(i + h) * wstride];
+            const uint32_t g0 = gp[i * gstride], g1 = gp[(i + h) * gstride];
+            inv.mac(w0, g0);
+            inv.mac(w1, g1);
+            L.mac(g1, w0);
+            R.mac(g0, w1);
+        }
+    }
+    const PrimeConsts pc = c_pc[p];
+    const size_t o = ((size_t)k * NPRIMES + p) * m + c;
+    out_inv[o] = reduce_u96(inv.lo, inv.hi, pc);
+    if (cross) {
+        out_L[o] = reduce_u96(L.lo, L.hi, pc);
+        out_R[o] = reduce_u96(R.lo, R.hi, pc);
+    }
+}
+
+void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
+                      uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, hipStream_t s) {
+    const size_t n = (size_t)K * NPRIMES << log_m;
+    hipLaunchKernelGGL(k_mac_pairs, dim3((unsigned)((n + MAC_THREADS - 1) / MAC_THREADS)),
+                       dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, out_inv, out_L,
+                       out_R);
+}
+
+// thread = (k, p, t), t < m/2, over the position pair (2t, 2t+1).
+// With F = NTT_m(f): F[2t] = f(r_t), F[2t+1] = f(-r_t), r_t = zeta[m/2 + t]; the degree-m/2
+// transforms of the even/odd parts are (F[2t]+F[2t+1])/2 and (F[2t]-F[2t+1])/(2 r_t), and
+// multiplying by Y = X^2 is multiplying by r_t^2. Hence
+//   L_poly[t] = r_t/4     * sum_i (W0+W1)(G0-G1)     (even(m~) against X*odd(g~))
+//   R_poly[t] = 1/(4 r_t) * sum_i (W0-W1)(G0+G1)     (odd(m~)  against even(g~))
+// which four exact sums S00, S11, S01, S10 provide together with the invariant (S00, S11).
+__global__ void __launch_bounds__(MAC_THREADS) k_mac_split(const uint32_t* __restrict__ What,
+                                                           const uint32_t* __restrict__ Ghat,
+                                                           uint32_t rows, int log_m,
+                                                           uint32_t* __restrict__ out_inv,
+                                                           uint32_t* __restrict__ out_L,
+                                                           uint32_t* __restrict__ out_R,
+                                                           NttTables tw) {
+    const size_t m = (size_t)1 << log_m;
+    const size_t h = m >> 1;
+    const size_t gid = (size_t)blockIdx.x * MAC_THREADS + threadIdx.x;
+    if (gid >= (size_t)K * NPRIMES * h) return;
+    const uint32_t t = (uint32_t)(gid & (h - 1));
+    const uint32_t p = (uint32_t)((gid >> (log_m - 1)) % NPRIMES);
+    const uint32_t k = (uint32_t)((gid >> (log_m - 1)) / NPRIMES);
+    Acc96 s00, s11, s01, s10;
+    s00.zero();
+    s11.zero();
+    s01.zero();
+    s10.zero();
+    const size_t wstride = NPRIMES * m;
+    const size_t gstride = (size_t)K * NPRIMES * m;
+    const uint2* wp = reinterpret_cast<const uint2*>(What + p * m) + t;
+    const uint2* gp = reinterpret_cast<const uint2*>(Ghat + ((size_t)k * NPRIMES + p) * m) + t;
+    for (uint32_t i = 0; i < rows; ++i) {
+        const uint2 w = wp[i * (wstride / 2)];
+        const uint2 g = gp[i * (gstride / 2)];
+        s00.mac(w.x, g.x);
+        s11.mac(w.y, g.y);
+        s01.mac(w.x, g.y);
+        s10.mac(w.y, g.x);
+    }
+    const PrimeConsts pc = c_pc[p];
+    const uint32_t P = pc.p;
+    const uint32_t a00 = reduce_u96(s00.lo, s00.hi, pc), a11 = reduce_u96(s11.lo, s11.hi, pc);
+    const uint32_t a01 = reduce_u96(s01.lo, s01.hi, pc), a10 = reduce_u96(s10.lo, s10.hi, pc);
+    const size_t o = ((size_t)k * NPRIMES + p) * m + 2 * t;
+    out_inv[o] = a00;
+    out_inv[o + 1] = a11;
+    const uint32_t d = subp(a00, a11, P);
+    const uint32_t Lp = addp(subp(d, a01, P), a10, P);
+    const uint32_t Rp = subp(addp(d, a01, P), a10, P);
+    // * r_t and * r_t^-1, then * 1/4 (inv4 = invn for log 2)
+    const uint32_t kz = (uint32_t)h + t;
+    const uint32_t i4m = tw.invn_m[p * (MAX_LOG_N + 1) + 2], i4mp = tw.invn_mp[p * (MAX_LOG_N + 1) + 2];
+    const uint32_t lv = mulc(mulc(Lp, tw.zeta_m[p * TW_SIZE + kz], tw.zeta_mp[p * TW_SIZE + kz], P), i4m, i4mp, P);
+    const uint32_t rv = mulc(mulc(Rp, tw.izeta_m[p * TW_SIZE + kz], tw.izeta_mp[p * TW_SIZE + kz], P), i4m, i4mp, P);
+    const size_t oh = ((size_t)k * NPRIMES + p) * h + t;
+    out_L[oh] = lv;
+    out_R[oh] = rv;
+}
+
+void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
+                      uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, const NttTables& tw,
+                      hipStream_t s) {
+    const size_t n = (size_t)K * NPRIMES << (log_m - 1);
+    hipLaunchKernelGGL(k_mac_split, dim3((unsigned)((n + MAC_THREADS - 1) / MAC_THREADS)),
+                       dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, out_inv, out_L,
+                       out_R, tw);
+}
+
+// ---------------------------------------------------------------------------------------
+// centred CRT lift (ntt.rs:400-427): Garner mixed-radix digits v_i in CRT_PRIMES order,
+// x = sum v_i * (p_0..p_{i-1}); if x > floor(P/2) the result is (x - P) mod q.
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_crt_lift(const uint32_t* __restrict__ res,
+                                                  zq* __restrict__ out, size_t total, int log_m) {
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= total) return;
+    const size_t m = (size_t)1 << log_m;
+    const size_t slice = gid >> log_m, c = gid & (m - 1);
+    uint32_t r[NPRIMES], v[NPRIMES];
+#pragma unroll
+    for (int i = 0; i < NPRIMES; ++i) r[i] = res[(slice * NPRIMES + i) * m + c];
+    v[0] = r[0];
+#pragma unroll
+    for (int i = 1; i < NPRIMES; ++i) {
+        const uint64_t pi = c_pc[i].p;
+        uint64_t acc = v[0] % pi;
+#pragma unroll
+        for (int j = 1; j < i; ++j) acc = (acc + (uint64_t)v[j] * c_crt.mod_prefix[i][j]) % pi;
+        const uint64_t delta = (r[i] + pi - acc) % pi;
+        v[i] = (uint32_t)((delta * c_crt.inv[i]) % pi);
+    }
+    // centred comparison on digits, most significant first
+    int gt = 0;
+#pragma unroll
+    for (int i = NPRIMES - 1; i >= 0; --i) {
+        if (gt == 0 && v[i] != c_crt.half_digits[i]) gt = v[i] > c_crt.half_digits[i] ? 1 : -1;
+    }
+    // x mod q = sum v_i * prefix_q[i]  (< 2^148), reduced as a 256-bit value
+    uint64_t acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < NPRIMES; ++i) {
+        uint64_t carry = 0;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const uint64_t t = (uint64_t)v[i] * c_crt.prefix_q[i].w[l] + (uint32_t)acc[l] + carry;
+            acc[l] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        uint64_t t = (uint32_t)acc[4] + carry;
+        acc[4] = (uint32_t)t;
+        acc[5] += t >> 32;
+    }
+    uint32_t x8[8] = {(uint32_t)acc[0], (uint32_t)acc[1], (uint32_t)acc[2], (uint32_t)acc[3],
+                      (uint32_t)acc[4], (uint32_t)acc[5], 0, 0};
+    zq xq = zq_reduce256(x8);
+    if (gt > 0) xq = zq_sub(xq, c_crt.P_q);
+    store_zq(out + gid, xq);
+}
+
+void launch_crt_lift(const uint32_t* res, zq* out, size_t slices, int log_m, hipStream_t s) {
+    const size_t total = slices << log_m;
+    if (!total) return;
+    hipLaunchKernelGGL(k_crt_lift, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, res, out,
+                       total, log_m);
+}
+
+// Z_q-only row production (the verifier's generator folds, verifier.rs:141-179)
+__global__ void k_fold_rows_zq(RowSrc src, size_t total, int log_m) {
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= total) return;
+    const size_t row = gid >> log_m;
+    const uint32_t c = (uint32_t)(gid & (((size_t)1 << log_m) - 1));
+    store_zq(src.dst + gid, row_value(src, row, c, log_m));
+}
+
+void launch_fold_rows_zq(const RowSrc& src, size_t rows, int log_m, hipStream_t s) {
+    const size_t total = rows << log_m;
+    if (!total) return;
+    hipLaunchKernelGGL(k_fold_rows_zq, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, total,
+                       log_m);
+}
+
+// final opening (verifier.rs:122-134): flags |= bit when g[i] * opening != c[i]
+__global__ void k_final_check(const zq* g, const zq* c, const zq* opening, size_t n, uint32_t* flags,
+                              uint32_t bit) {
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= n) return;
+    if (!zq_eq(zq_mul(load_zq(g + gid), *opening), load_zq(c + gid))) atomicOr(flags, bit);
+}
+
+void launch_final_check(const zq* g, const zq* c, const zq* opening, size_t n, uint32_t* flags,
+                        uint32_t bit, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_final_check, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, c, opening,
+                       n, flags, bit);
+}
+
+// ---------------------------------------------------------------------------------------
+// commitment bookkeeping
+// ---------------------------------------------------------------------------------------
+__global__ void k_com_update(const zq* c, const zq* l, const zq* r, const zq* scal, int poly,
+                             int log_m, zq* out) {
+    const size_t m = (size_t)1 << log_m;
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= K * m) return;
+    const size_t k = gid >> log_m, i = gid & (m - 1);
+    const zq s = scal[0], sinv = scal[1];
+    const zq base = poly ? load_zq(c + k * (2 * m) + 2 * i) : load_zq(c + gid);
+    zq v = zq_add(base, zq_mul(load_zq(l + gid), sinv));
+    v = zq_add(v, zq_mul(load_zq(r + gid), s));
+    store_zq(out + gid, v);
+}
+
+void launch_com_update(const zq* c, const zq* l, const zq* r, const zq* scal, int poly, int log_m,
+                       zq* out, hipStream_t s) {
+    const size_t n = (size_t)K << log_m;
+    hipLaunchKernelGGL(k_com_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c, l, r,
+                       scal, poly, log_m, out);
+}
+
+__global__ void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+void launch_copy16(void* dst, const void* src, size_t n16, hipStream_t s) {
+    if (!n16) return;
+    const size_t blocks = std::min<size_t>((n16 + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, s, static_cast<uint4*>(dst),
+                       static_cast<const uint4*>(src), n16);
+}
+
+__global__ void k_zq_add(const zq* a, const zq* b, zq* out, size_t n) {
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= n) return;
+    store_zq(out + gid, zq_add(load_zq(a + gid), load_zq(b + gid)));
+}
+
+void launch_zq_add(const zq* a, const zq* b, zq* out, size_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_zq_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, out, n);
+}
+
+__global__ void k_com_check(const zq* a, const zq* b, size_t n, uint32_t* flags, uint32_t bit) {
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= n) return;
+    if (!zq_eq(load_zq(a + gid), load_zq(b + gid))) atomicOr(flags, bit);
+}
+
+void launch_com_check(const zq* a, const zq* b, size_t n, uint32_t* flags, uint32_t bit,
+                      hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_com_check, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n,
+                       flags, bit);
+}
+
+// ---------------------------------------------------------------------------------------
+// witness expansion (io.rs:51-54) and generator expansion (sis.rs:212-245)
+// ---------------------------------------------------------------------------------------
+__global__ void k_expand(const uint8_t* __restrict__ file, size_t len, zq* __restrict__ out,
+                         size_t capacity) {
+    const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= capacity) return;
+    uint32_t w[4] = {0, 0, 0, 0};
+    const size_t off = j * BYTES_PER_COEFF;
+    if (off < len) {
+        const size_t n = len - off < BYTES_PER_COEFF ? len - off : BYTES_PER_COEFF;
+        for (size_t s = 0; s < n; ++s) w[s >> 2] |= (uint32_t)file[off + s] << (8 * (s & 3));
+    }
+    store_zq(out + j, zq{{w[0], w[1], w[2], w[3]}});  // < 2^104 < q: already canonical
+}
+
+void launch_expand(const uint8_t* file, size_t len, zq* out, size_t capacity, hipStream_t s) {
+    hipLaunchKernelGGL(k_expand, dim3((unsigned)((capacity + 255) / 256)), dim3(256), 0, s, file, len,
+                       out, capacity);
+}
+
+struct Key8 {
+    uint32_t w[8];
+};
+
+__global__ void k_gen_generators(Key8 key, size_t total, int log_m, zq* __restrict__ out) {
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= total) return;
+    const size_t m = (size_t)1 << log_m;
+    const size_t row = gid >> log_m, c = gid & (m - 1);
+    store_zq(out + gid, gen_coeff(key.w, row / K, (uint32_t)(row % K), c));
+}
+
+void launch_gen_generators(const uint32_t gseed[8], size_t rows, int log_m, zq* out, hipStream_t s) {
+    Key8 k;
+    for (int i = 0; i < 8; ++i) k.w[i] = gseed[i];
+    const size_t total = rows << log_m;
+    if (!total) return;
+    hipLaunchKernelGGL(k_gen_generators, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, k,
+                       total, log_m, out);
+}
+
+// ---------------------------------------------------------------------------------------
+// reference-compatible batch entry points (src/utils/gpu.rs:69-93)
+// ---------------------------------------------------------------------------------------
+// out = l + c*r mod q over SoA lo/hi words (cuda/fold_kernel.cu:189-206 behaviour)
+__global__ void k_fold_pairs(const uint64_t* l_lo, const uint64_t* l_hi, const uint64_t* r_lo,
+                             const uint64_t* r_hi, uint64_t* o_lo, uint64_t* o_hi, size_t len,
+                             uint64_t c_lo, uint64_t c_hi) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= len) return;
+    auto canon = [](uint64_t lo, uint64_t hi) {
+        uint32_t x8[8] = {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32), 0, 0, 0, 0};
+        return zq_reduce256(x8);
+    };
+    const zq l = canon(l_lo[i], l_hi[i]), r = canon(r_lo[i], r_hi[i]), c = canon(c_lo, c_hi);
+    const zq o = zq_add(l, zq_mul(r, c));
+    o_lo[i] = (uint64_t)o.w[0] | ((uint64_t)o.w[1] << 32);
+    o_hi[i] = (uint64_t)o.w[2] | ((uint64_t)o.w[3] << 32);
+}
+
+void launch_fold_pairs(const uint64_t* l_lo, const uint64_t* l_hi, const uint64_t* r_lo,
+                       const uint64_t* r_hi, uint64_t* o_lo, uint64_t* o_hi, size_t len,
+                       uint64_t c_lo, uint64_t c_hi, hipStream_t s) {
+    hipLaunchKernelGGL(k_fold_pairs, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, l_lo, l_hi,
+                       r_lo, r_hi, o_lo, o_hi, len, c_lo, c_hi);
+}
+
+// Cyclic product with caller-supplied stage roots, the algorithm of ntt.rs:58-96 on each
+// prime: bit-reverse, radix-2 DIT stages (twiddle wlen^j), pointwise, inverse, * inv_size.
+__device__ __forceinline__ uint64_t mulmod64(uint64_t a, uint64_t b, uint64_t p) {
+    return (a * b) % p;  // a, b < p < 2^32
+}
+__device__ uint64_t powmod64(uint64_t b, uint64_t e, uint64_t p) {
+    uint64_t r = 1 % p;
+    while (e) {
+        if (e & 1) r = mulmod64(r, b, p);
+        b = mulmod64(b, b, p);
+        e >>= 1;
+    }
+    return r;
+}
+
+__global__ void k_cyc_prep(uint64_t* a, size_t n, int log_n, const uint64_t* primes) {
+    // reduce inputs and bit-reverse every prime slice: grid (n/256, NPRIMES)
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int pi = blockIdx.y;
+    if (i >= n) return;
+    uint64_t* s = a + (size_t)pi * n;
+    const uint64_t p = primes[pi];
+    const size_t r = log_n ? (size_t)(__brevll((unsigned long long)i) >> (64 - log_n)) : 0;
+    if (i < r) {
+        const uint64_t x = s[i] % p, y = s[r] % p;
+        s[i] = y;
+        s[r] = x;
+    } else if (i == r) {
+        s[i] = s[i] % p;
+    }
+}
+
+__global__ void k_cyc_stage(uint64_t* a, size_t n, int stage, const uint64_t* primes,
+                            const uint64_t* roots, int log_n) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int pi = blockIdx.y;
+    if (t >= n / 2) return;
+    const uint64_t p = primes[pi];
+    const size_t len = (size_t)2 << stage, half = len >> 1;
+    const size_t grp = t / half, off = t % half;
+    const size_t lft = grp * len + off, rgt = lft + half;
+    uint64_t* s = a + (size_t)pi * n;
+    const uint64_t w = powmod64(roots[(size_t)pi * log_n + stage] % p, off, p);
+    const uint64_t tv = mulmod64(s[rgt], w, p), u = s[lft];
+    s[lft] = (u + tv) % p;
+    s[rgt] = (u + p - tv) % p;
+}
+
+__global__ void k_cyc_pointwise(uint64_t* a, const uint64_t* b, size_t n, const uint64_t* primes) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int pi = blockIdx.y;
+    if (i >= n) return;
+    a[(size_t)pi * n + i] = mulmod64(a[(size_t)pi * n + i], b[(size_t)pi * n + i], primes[pi]);
+}
+
+__global__ void k_cyc_scale(uint64_t* a, size_t n, const uint64_t* primes, const uint64_t* inv_sizes) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int pi = blockIdx.y;
+    if (i >= n) return;
+    a[(size_t)pi * n + i] = mulmod64(a[(size_t)pi * n + i], inv_sizes[pi] % primes[pi], primes[pi]);
+}
+
+void launch_cyclic_ntt_batch(uint64_t* a, uint64_t* b, size_t n, const uint64_t* primes,
+                             const uint64_t* stage_roots, const uint64_t* inv_roots,
+                             const uint64_t* inv_sizes, hipStream_t s) {
+    int log_n = 0;
+    while (((size_t)1 << log_n) < n) ++log_n;
+    const dim3 gv((unsigned)((n + 255) / 256), NPRIMES), gb((unsigned)((n / 2 + 255) / 256), NPRIMES);
+    hipLaunchKernelGGL(k_cyc_prep, gv, dim3(256), 0, s, a, n, log_n, primes);
+    hipLaunchKernelGGL(k_cyc_prep, gv, dim3(256), 0, s, b, n, log_n, primes);
+    for (int st = 0; st < log_n; ++st) {
+        hipLaunchKernelGGL(k_cyc_stage, gb, dim3(256), 0, s, a, n, st, primes, stage_roots, log_n);
+        hipLaunchKernelGGL(k_cyc_stage, gb, dim3(256), 0, s, b, n, st, primes, stage_roots, log_n);
+    }
+    hipLaunchKernelGGL(k_cyc_pointwise, gv, dim3(256), 0, s, a, b, n, primes);
+    hipLaunchKernelGGL(k_cyc_prep, gv, dim3(256), 0, s, a, n, log_n, primes);
+    for (int st = 0; st < log_n; ++st)
+        hipLaunchKernelGGL(k_cyc_stage, gb, dim3(256), 0, s, a, n, st, primes, inv_roots, log_n);
+    hipLaunchKernelGGL(k_cyc_scale, gv, dim3(256), 0, s, a, n, primes, inv_sizes);
+}
+
+// ---------------------------------------------------------------------------------------
+// ring products (ragged batches grouped by degree)
+// ---------------------------------------------------------------------------------------
+__global__ void k_pointwise(uint32_t* a, const uint32_t* b, size_t total, int log_m) {
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= total) return;
+    const uint32_t p = (uint32_t)((gid >> log_m) % NPRIMES);
+    const uint64_t pr = (uint64_t)a[gid] * b[gid];
+    a[gid] = (uint32_t)(pr % c_pc[p].p);
+}
+
+void launch_pointwise(uint32_t* a, const uint32_t* b, size_t slices, int log_m, hipStream_t s) {
+    const size_t total = slices * NPRIMES << log_m;
+    if (!total) return;
+    hipLaunchKernelGGL(k_pointwise, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, b, total,
+                       log_m);
+}
+
+__global__ void k_gather_rows(const zq* src, const uint64_t* offs, const uint32_t* idx, size_t total,
+                              int log_m, zq* dst, int scatter) {
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= total) return;
+    const size_t m = (size_t)1 << log_m;
+    const size_t r = gid >> log_m, c = gid & (m - 1);
+    const size_t o = offs[idx[r]] + c;
+    if (scatter)
+        store_zq(dst + o, load_zq(src + gid));
+    else
+        store_zq(dst + gid, load_zq(src + o));
+}
+
+void launch_gather_rows(const zq* src, const uint64_t* offs, const uint32_t* idx, size_t count,
+                        int log_m, zq* dst, hipStream_t s) {
+    const size_t total = count << log_m;
+    if (!total) return;
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, offs,
+                       idx, total, log_m, dst, 0);
+}
+void launch_scatter_rows(const zq* src, const uint64_t* offs, const uint32_t* idx, size_t count,
+                         int log_m, zq* dst, hipStream_t s) {
+    const size_t total = count << log_m;
+    if (!total) return;
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, offs,
+                       idx, total, log_m, dst, 1);
+}
+
+}  // namespace dmx

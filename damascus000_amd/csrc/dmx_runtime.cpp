@@ -1,0 +1,1499 @@
+// Host runtime for the gfx950 commit-and-fold path: context (device tables), prover and
+// verifier state machines, bincode wire format, and the C ABI of include/damascus_hip.h.
+//
+// The host never computes on protocol data: every commitment, fold, transcript hash and
+// challenge runs on the device stream. The host validates shapes, orders launches and moves
+// the bytes the caller asked for.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/damascus_hip.h"
+#include "dmx_blake3.hpp"
+#include "dmx_kernels.hpp"
+
+using namespace dmx;
+
+// ------------------------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(DMX_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char* dmx_last_error(void) { return g_err.c_str(); }
+
+// ------------------------------------------------------------------------------------------
+// backend probe (src/utils/gpu.rs:19-67 semantics: DAMASCUS_GPU=0 disables)
+// ------------------------------------------------------------------------------------------
+static bool gpu_disabled_by_env() {
+    const char* v = getenv("DAMASCUS_GPU");
+    return v && strcmp(v, "0") == 0;
+}
+
+extern "C" int dmx_backend_ready(void) {
+    if (gpu_disabled_by_env()) return 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n > 0 ? 1 : 0;
+}
+
+extern "C" int dmx_device_summary(char* buf, size_t cap) {
+    std::string s;
+    if (gpu_disabled_by_env()) {
+        s = "disabled by DAMASCUS_GPU=0";
+    } else {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            s = "no HIP device";
+        } else {
+            hipDeviceProp_t p;
+            (void)hipGetDeviceProperties(&p, 0);
+            char t[256];
+            snprintf(t, sizeof t, "%s (%s), %d CUs, %zu MiB", p.name, p.gcnArchName,
+                     p.multiProcessorCount, (size_t)(p.totalGlobalMem >> 20));
+            s = t;
+        }
+    }
+    if (buf && cap) {
+        strncpy(buf, s.c_str(), cap - 1);
+        buf[cap - 1] = 0;
+    }
+    return (int)s.size();
+}
+
+// ------------------------------------------------------------------------------------------
+// layout (src/utils/io.rs:72-120)
+// ------------------------------------------------------------------------------------------
+static size_t coeff_count_for(uint64_t len) {
+    uint64_t c = (len + BYTES_PER_COEFF - 1) / BYTES_PER_COEFF;
+    return c < 1 ? 1 : (size_t)c;
+}
+static int depth_for(uint64_t len) {
+    size_t cc = coeff_count_for(len);
+    int d = MIN_FOLD_DEPTH;
+    while (((size_t)1 << (2 * d)) < cc) ++d;
+    return d;
+}
+
+// ------------------------------------------------------------------------------------------
+// bincode 1.3 (fixint, little endian)
+// ------------------------------------------------------------------------------------------
+static size_t module_bytes(size_t n) { return 8 + (size_t)K * (8 + 16 * n); }
+extern "C" size_t dmx_statement_size(size_t ring_len) { return 32 + 8 + 8 + module_bytes(ring_len) + 32; }
+extern "C" size_t dmx_record_size(size_t lv, size_t lp) { return 8 + 2 * module_bytes(lv) + 2 * module_bytes(lp); }
+extern "C" size_t dmx_proof_records_size(uint64_t file_len) {
+    int d = depth_for(file_len);
+    size_t t = 0;
+    for (int j = 0; j < d; ++j) {
+        size_t m = (size_t)1 << (d - j);
+        t += dmx_record_size(m, m / 2);
+    }
+    return t;
+}
+
+static uint8_t* put_u64(uint8_t* p, uint64_t v) {
+    for (int i = 0; i < 8; ++i) *p++ = (uint8_t)(v >> (8 * i));
+    return p;
+}
+static uint8_t* put_module(uint8_t* p, const zq* v, size_t n) {
+    p = put_u64(p, K);
+    for (int k = 0; k < K; ++k) {
+        p = put_u64(p, n);
+        memcpy(p, v + (size_t)k * n, 16 * n);
+        p += 16 * n;
+    }
+    return p;
+}
+
+struct Reader {
+    const uint8_t* p;
+    size_t left;
+    bool u64(uint64_t& v) {
+        if (left < 8) return false;
+        v = 0;
+        for (int i = 0; i < 8; ++i) v |= (uint64_t)p[i] << (8 * i);
+        p += 8;
+        left -= 8;
+        return true;
+    }
+};
+
+// ModuleElement<8> (module.rs:104-115): a sequence of exactly 8 Polys of one length; each Fp
+// is read as u128 and reduced mod q (field.rs:121-129).
+static int read_module(Reader& r, std::vector<zq>& out, size_t& len) {
+    uint64_t n;
+    if (!r.u64(n)) return fail(DMX_ERR_DECODE, "decode: truncated module length");
+    bool shape_ok = true;
+    len = 0;
+    for (uint64_t k = 0; k < n; ++k) {
+        uint64_t l;
+        if (!r.u64(l) || l > r.left / 16) return fail(DMX_ERR_DECODE, "decode: truncated polynomial");
+        if (k == 0) {
+            len = (size_t)l;
+            out.assign((size_t)K * len, zq_zero());
+        } else if (l != len) {
+            shape_ok = false;
+        }
+        for (uint64_t c = 0; c < l; ++c) {
+            uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int b = 0; b < 16; ++b) w[b >> 2] |= (uint32_t)r.p[b] << (8 * (b & 3));
+            r.p += 16;
+            r.left -= 16;
+            if (k < K && shape_ok) out[k * len + c] = zq_reduce256(w);
+        }
+    }
+    if (n != K) return fail(DMX_ERR_DECODE, "decode: module rank mismatch (%llu)", (unsigned long long)n);
+    if (!shape_ok) return fail(DMX_ERR_DECODE, "decode: module coordinate length mismatch");
+    return DMX_OK;
+}
+
+struct Record {
+    uint64_t round;
+    std::vector<zq> m[4];  // l_vec, r_vec, l_poly, r_poly
+    size_t len[4];
+};
+
+static int decode_record(const uint8_t* buf, size_t n, Record& rec) {
+    Reader r{buf, n};
+    if (!r.u64(rec.round)) return fail(DMX_ERR_DECODE, "decode: truncated round");
+    for (int i = 0; i < 4; ++i) {
+        int rc = read_module(r, rec.m[i], rec.len[i]);
+        if (rc) return rc;
+    }
+    return DMX_OK;  // trailing bytes are allowed, as bincode::deserialize does
+}
+
+struct Statement {
+    uint8_t file_id[32];
+    uint64_t original_len;
+    uint64_t d;
+    std::vector<zq> com0;
+    size_t com_len;
+    uint8_t g_seed[32];
+};
+
+static int decode_statement(const uint8_t* buf, size_t n, Statement& st) {
+    Reader r{buf, n};
+    if (r.left < 32) return fail(DMX_ERR_DECODE, "decode: truncated statement");
+    memcpy(st.file_id, r.p, 32);
+    r.p += 32;
+    r.left -= 32;
+    if (!r.u64(st.original_len) || !r.u64(st.d)) return fail(DMX_ERR_DECODE, "decode: truncated statement");
+    int rc = read_module(r, st.com0, st.com_len);
+    if (rc) return rc;
+    if (r.left < 32) return fail(DMX_ERR_DECODE, "decode: truncated g_0_seed");
+    memcpy(st.g_seed, r.p, 32);
+    return DMX_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// context: constants and NTT tables
+// ------------------------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t n) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = n;
+        if (n == 0) return DMX_OK;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(DMX_ERR_OOM, "device allocation of %zu bytes failed", n);
+        }
+        return DMX_OK;
+    }
+    template <class T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+struct TimedLaunch {
+    hipEvent_t a, b;
+    size_t rows;
+    int log_m;
+    int mode;
+};
+
+struct dmx_ctx {
+    int device = 0;
+    DevBuf tables;
+    NttTables tw{};
+    PrimeConsts pc[NPRIMES];
+    CrtConsts crt;
+    std::mutex mu;
+    bool timing = false;
+    std::vector<TimedLaunch> launches;
+    std::vector<hipEvent_t> event_pool;
+    ~dmx_ctx() {
+        for (auto& t : launches) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+        for (auto e : event_pool) (void)hipEventDestroy(e);
+    }
+    hipEvent_t take_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+};
+
+// algorithmic VALU work of one forward-transform row of degree 2^log_m, in lane-cycles with the
+// fixed weights: butterfly 19, mod-p reduction 42, Z_q multiply-add 204, BLAKE3 compression 860
+static double fwd_row_work(int log_m, int mode) {
+    const double m = (double)((size_t)1 << log_m);
+    double w = NPRIMES * m * 42.0 + NPRIMES * (m / 2) * log_m * 19.0;
+    if (mode == ROW_GEN) w += m * 860.0;
+    if (mode == ROW_VFOLD || mode == ROW_PFOLD_W || mode == ROW_PFOLD_G) w += m * 204.0;
+    return w;
+}
+
+// forward transforms go through here so that kernel timing can bracket them
+static void fwd_rows(dmx_ctx* ctx, const RowSrc& src, size_t rows, int log_m, uint32_t* out, hipStream_t s) {
+    if (!ctx->timing || rows == 0) {
+        launch_ntt_fwd_rows(src, rows, log_m, out, ctx->tw, s);
+        return;
+    }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    TimedLaunch t{ctx->take_event(), ctx->take_event(), rows, log_m, src.mode};
+    (void)hipEventRecord(t.a, s);
+    launch_ntt_fwd_rows(src, rows, log_m, out, ctx->tw, s);
+    (void)hipEventRecord(t.b, s);
+    ctx->launches.push_back(t);
+}
+
+extern "C" int dmx_kernel_timing_enable(dmx_ctx* ctx, int on) {
+    if (!ctx) return fail(DMX_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->timing = on != 0;
+    return DMX_OK;
+}
+
+extern "C" int dmx_kernel_timing(dmx_ctx* ctx, double* total_ms, uint64_t* launches, double* lane_cycles) {
+    if (!ctx) return fail(DMX_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    double ms = 0, work = 0;
+    for (auto& t : ctx->launches) {
+        HIP_TRY(hipEventSynchronize(t.b));
+        float e = 0;
+        HIP_TRY(hipEventElapsedTime(&e, t.a, t.b));
+        ms += e;
+        work += (double)t.rows * fwd_row_work(t.log_m, t.mode);
+        ctx->event_pool.push_back(t.a);
+        ctx->event_pool.push_back(t.b);
+    }
+    if (total_ms) *total_ms = ms;
+    if (launches) *launches = ctx->launches.size();
+    if (lane_cycles) *lane_cycles = work;
+    ctx->launches.clear();
+    return DMX_OK;
+}
+
+extern "C" int dmx_device_alloc(dmx_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(DMX_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(DMX_ERR_OOM, "device allocation of %zu bytes failed", bytes);
+    }
+    return DMX_OK;
+}
+extern "C" int dmx_device_free(dmx_ctx* ctx, void* ptr) {
+    if (!ctx) return fail(DMX_ERR_ARG, "null context");
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return DMX_OK;
+}
+extern "C" int dmx_memcpy_htod(dmx_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(DMX_ERR_ARG, "null argument");
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return DMX_OK;
+}
+extern "C" int dmx_memcpy_dtoh(dmx_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(DMX_ERR_ARG, "null argument");
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return DMX_OK;
+}
+extern "C" int dmx_device_copy(dmx_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src)) || (bytes & 15)) return fail(DMX_ERR_ARG, "bad argument");
+    launch_copy16(dst, src, bytes / 16, 0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    return DMX_OK;
+}
+
+static uint64_t smallest_generator(uint64_t p) {
+    uint64_t m = p - 1, f[64];
+    int nf = 0;
+    for (uint64_t d = 2; d * d <= m; ++d) {
+        if (m % d == 0) {
+            f[nf++] = d;
+            while (m % d == 0) m /= d;
+        }
+    }
+    if (m > 1) f[nf++] = m;
+    for (uint64_t g = 2;; ++g) {
+        bool ok = true;
+        for (int i = 0; i < nf; ++i)
+            if (host_powmod(g, (p - 1) / f[i], p) == 1) {
+                ok = false;
+                break;
+            }
+        if (ok) return g;
+    }
+}
+
+static uint32_t brv(uint32_t x, int bits) {
+    uint32_t r = 0;
+    for (int i = 0; i < bits; ++i)
+        if (x & (1u << i)) r |= 1u << (bits - 1 - i);
+    return r;
+}
+
+// u32-limb big integers for the CRT constants
+typedef std::vector<uint32_t> big;
+static void big_mul(big& a, uint32_t m) {
+    uint64_t c = 0;
+    for (auto& w : a) {
+        uint64_t t = (uint64_t)w * m + c;
+        w = (uint32_t)t;
+        c = t >> 32;
+    }
+    if (c) a.push_back((uint32_t)c);
+}
+static uint32_t big_divmod(big& a, uint32_t d) {
+    uint64_t r = 0;
+    for (size_t i = a.size(); i-- > 0;) {
+        uint64_t cur = (r << 32) | a[i];
+        a[i] = (uint32_t)(cur / d);
+        r = cur % d;
+    }
+    return (uint32_t)r;
+}
+
+static void build_constants(dmx_ctx* c, std::vector<uint32_t>& host_tables) {
+    const size_t L = MAX_LOG_N + 1;
+    host_tables.assign((size_t)NPRIMES * TW_SIZE * 4 + (size_t)NPRIMES * L * 2, 0);
+    uint32_t* zm = host_tables.data();
+    uint32_t* zmp = zm + NPRIMES * TW_SIZE;
+    uint32_t* izm = zmp + NPRIMES * TW_SIZE;
+    uint32_t* izmp = izm + NPRIMES * TW_SIZE;
+    uint32_t* inm = izmp + NPRIMES * TW_SIZE;
+    uint32_t* inmp = inm + NPRIMES * L;
+    const zq q = zq{{Q0, Q1, Q2, Q3}};
+    for (int i = 0; i < NPRIMES; ++i) {
+        const uint64_t p = PRIMES[i];
+        PrimeConsts& k = c->pc[i];
+        memset(&k, 0, sizeof k);
+        k.p = (uint32_t)p;
+        k.pinv = host_inv32((uint32_t)p);
+        const uint64_t R = (1ull << 32) % p;
+        const uint64_t r4 = host_powmod(R, 4, p);
+        k.r4_m = (uint32_t)r4;
+        k.r4_mp = (uint32_t)r4 * k.pinv;
+        k.two64 = (uint32_t)host_powmod(R, 2, p);
+        // q mod p
+        uint64_t qm = 0;
+        for (int l = 3; l >= 0; --l) qm = ((qm << 32) | q.w[l]) % p;
+        k.q_mod_p = (uint32_t)qm;
+        // twiddles: psi a primitive 2^(MAX_LOG_N+1)-th root of unity
+        const uint64_t g = smallest_generator(p);
+        const uint64_t psi = host_powmod(g, (p - 1) >> (MAX_LOG_N + 1), p);
+        const uint64_t order = 1ull << (MAX_LOG_N + 1);
+        for (uint32_t t = 0; t < (uint32_t)TW_SIZE; ++t) {
+            const uint64_t e = brv(t, MAX_LOG_N);
+            const uint64_t z = host_powmod(psi, e, p), iz = host_powmod(psi, (order - e) % order, p);
+            const uint32_t zmv = (uint32_t)((z << 32) % p), izmv = (uint32_t)((iz << 32) % p);
+            zm[i * TW_SIZE + t] = zmv;
+            zmp[i * TW_SIZE + t] = zmv * k.pinv;
+            izm[i * TW_SIZE + t] = izmv;
+            izmp[i * TW_SIZE + t] = izmv * k.pinv;
+        }
+        for (size_t lg = 0; lg < L; ++lg) {
+            const uint64_t inv = host_powmod((1ull << lg) % p, p - 2, p);
+            const uint32_t v = (uint32_t)((inv << 32) % p);
+            inm[i * L + lg] = v;
+            inmp[i * L + lg] = v * k.pinv;
+        }
+    }
+    // CRT constants (ntt.rs:379-398): prefix products in CRT_PRIMES order
+    CrtConsts& cr = c->crt;
+    memset(&cr, 0, sizeof cr);
+    for (int i = 0; i < NPRIMES; ++i) {
+        const uint64_t pi = PRIMES[i];
+        uint64_t pref = 1;
+        for (int j = 0; j < i; ++j) {
+            cr.mod_prefix[i][j] = (uint32_t)pref;
+            pref = (pref * PRIMES[j]) % pi;
+        }
+        cr.mod_prefix[i][i] = (uint32_t)pref;
+        cr.inv[i] = i ? (uint32_t)host_powmod(pref, pi - 2, pi) : 1;
+    }
+    big P{1};
+    zq pq = zq_from_u64(1);
+    for (int i = 0; i < NPRIMES; ++i) {
+        cr.prefix_q[i] = pq;
+        pq = zq_mul(pq, zq_from_u64(PRIMES[i]));
+        big_mul(P, PRIMES[i]);
+    }
+    cr.P_q = pq;
+    // floor(P/2) in mixed radix
+    big h = P;
+    uint32_t carry = 0;
+    for (size_t i = h.size(); i-- > 0;) {
+        uint32_t nc = h[i] & 1u;
+        h[i] = (h[i] >> 1) | (carry << 31);
+        carry = nc;
+    }
+    for (int i = 0; i < NPRIMES; ++i) cr.half_digits[i] = big_divmod(h, PRIMES[i]);
+}
+
+extern "C" int dmx_ctx_create(int device_index, dmx_ctx** out) {
+    if (!out) return fail(DMX_ERR_ARG, "null output");
+    if (!dmx_backend_ready()) return fail(DMX_ERR_NO_DEVICE, "no usable HIP device (or DAMASCUS_GPU=0)");
+    HIP_TRY(hipSetDevice(device_index));
+    std::unique_ptr<dmx_ctx> c(new dmx_ctx);
+    c->device = device_index;
+    std::vector<uint32_t> host;
+    build_constants(c.get(), host);
+    int rc = c->tables.alloc(host.size() * 4);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(c->tables.p, host.data(), host.size() * 4, hipMemcpyHostToDevice));
+    uint32_t* base = c->tables.as<uint32_t>();
+    c->tw.zeta_m = base;
+    c->tw.zeta_mp = base + NPRIMES * TW_SIZE;
+    c->tw.izeta_m = base + 2 * NPRIMES * TW_SIZE;
+    c->tw.izeta_mp = base + 3 * NPRIMES * TW_SIZE;
+    c->tw.invn_m = base + 4 * NPRIMES * TW_SIZE;
+    c->tw.invn_mp = base + 4 * NPRIMES * TW_SIZE + NPRIMES * (MAX_LOG_N + 1);
+    set_device_constants(c->pc, &c->crt, 0);
+    HIP_TRY(hipDeviceSynchronize());
+    *out = c.release();
+    return DMX_OK;
+}
+
+extern "C" void dmx_ctx_destroy(dmx_ctx* ctx) { delete ctx; }
+
+// ------------------------------------------------------------------------------------------
+// shared per-object device workspace for transcripts
+// ------------------------------------------------------------------------------------------
+static const uint32_t IVW[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                                0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
+
+struct Transcriber {
+    DevBuf msg, cvs, seeds;  // seeds: epoch_seed[32] | file_id[32]
+    size_t msg_cap = 0, cv_words = 0;
+    int init(size_t max_ring, size_t extra_hash_len) {
+        const uint32_t lens[3] = {(uint32_t)max_ring, (uint32_t)max_ring, (uint32_t)max_ring};
+        msg_cap = transcript_len(4, lens);
+        cv_words = device_hash_scratch_words(std::max(msg_cap, extra_hash_len));
+        int rc = msg.alloc(msg_cap);
+        if (!rc) rc = cvs.alloc(cv_words * 4);
+        if (!rc) rc = seeds.alloc(64);
+        return rc;
+    }
+    const uint8_t* epoch() const { return seeds.as<uint8_t>(); }
+    uint8_t* file_id() const { return seeds.as<uint8_t>() + 32; }
+    // challenge (transcript.rs:42-63) written as (x, x^-1) to out
+    void challenge(const char* label, uint64_t round, const zq* c, size_t lc, const zq* l, size_t ll,
+                   const zq* r, size_t lr, zq* out, uint32_t* flags, hipStream_t s) {
+        TranscriptMsg t;
+        t.epoch_seed = epoch();
+        t.file_id = file_id();
+        t.label_len = (uint32_t)strlen(label);
+        memset(t.label, 0, 4);
+        memcpy(t.label, label, t.label_len);
+        t.round = round;
+        t.mods[0] = c;
+        t.mods[1] = l;
+        t.mods[2] = r;
+        t.lens[0] = (uint32_t)lc;
+        t.lens[1] = (uint32_t)ll;
+        t.lens[2] = (uint32_t)lr;
+        const size_t len = transcript_len(t.label_len, t.lens);
+        launch_transcript_pack(t, msg.as<uint8_t>(), len, s);
+        device_hash(msg.as<uint8_t>(), len, IVW, 0, cvs.as<uint32_t>(), cv_words, nullptr, out, flags, s);
+    }
+};
+
+static void gseed_words(const uint8_t g_seed[32], uint32_t w[8]) { b3::words_le(g_seed, w, 8); }
+
+// ------------------------------------------------------------------------------------------
+// prover (src/protocol/prover.rs:56-313)
+// ------------------------------------------------------------------------------------------
+enum : uint32_t { FLAG_ZERO_CHALLENGE = 0xC0000000u };
+static uint32_t vec_bit(size_t j) { return 1u << (2 * j); }
+static uint32_t poly_bit(size_t j) { return 1u << (2 * j + 1); }
+
+struct dmx_prover {
+    dmx_ctx* ctx = nullptr;
+    hipStream_t s = nullptr;
+    dmx_system_params params{};
+    uint8_t g_seed[32];
+    uint32_t gw[8];
+    uint64_t orig_len = 0;
+    int d = 0;
+    size_t N = 0;
+    size_t round = 0;
+    bool poisoned = false;
+    DevBuf file, W_a, W_b, G_a, G_b, What_a, What_b, Ghat_a, Ghat_b, res, lifted, C, Ct, com0, recs,
+        scal, flags;
+    std::vector<size_t> rec_off;  // offset (coefficients) of round j's record block
+    Transcriber tr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    const uint8_t* d_file = nullptr;
+
+    ~dmx_prover() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    size_t m_at(size_t j) const { return N >> j; }
+    zq* rec_lvec(size_t j) const { return recs.as<zq>() + rec_off[j]; }
+    zq* rec_rvec(size_t j) const { return rec_lvec(j) + (size_t)K * m_at(j); }
+    zq* rec_lpoly(size_t j) const { return rec_lvec(j) + (size_t)2 * K * m_at(j); }
+    zq* rec_rpoly(size_t j) const { return rec_lpoly(j) + (size_t)K * (m_at(j) / 2); }
+    uint32_t* res_at(size_t words) const { return res.as<uint32_t>() + words; }
+    zq* sc(size_t i) const { return scal.as<zq>() + i; }
+};
+
+static int prover_alloc(dmx_prover* p) {
+    const size_t N = p->N;
+    const size_t NN = N * N;
+    int rc = 0;
+    rc = rc ? rc : p->W_a.alloc(NN * 16);
+    rc = rc ? rc : p->W_b.alloc(NN / 2 * 16);
+    rc = rc ? rc : p->G_a.alloc(NN * K * 16);
+    rc = rc ? rc : p->G_b.alloc(NN / 2 * K * 16);
+    rc = rc ? rc : p->What_a.alloc(NN * NPRIMES * 4);
+    rc = rc ? rc : p->What_b.alloc(NN / 2 * NPRIMES * 4);
+    rc = rc ? rc : p->Ghat_a.alloc(NN * K * NPRIMES * 4);
+    rc = rc ? rc : p->Ghat_b.alloc(NN / 2 * K * NPRIMES * 4);
+    rc = rc ? rc : p->res.alloc((size_t)3 * K * NPRIMES * N * 4);
+    rc = rc ? rc : p->lifted.alloc((size_t)K * N * 16);
+    rc = rc ? rc : p->C.alloc((size_t)K * N * 16);
+    rc = rc ? rc : p->Ct.alloc((size_t)K * N * 16);
+    rc = rc ? rc : p->com0.alloc((size_t)K * N * 16);
+    size_t total = 0;
+    p->rec_off.resize(p->d + 1);
+    for (int j = 0; j <= p->d; ++j) {
+        p->rec_off[j] = total;
+        size_t m = N >> j;
+        total += (size_t)K * (2 * m + m);
+    }
+    rc = rc ? rc : p->recs.alloc(total * 16);
+    rc = rc ? rc : p->scal.alloc((size_t)4 * (p->d + 1) * 16);
+    rc = rc ? rc : p->flags.alloc(256);
+    rc = rc ? rc : p->tr.init(N, p->orig_len);
+    return rc;
+}
+
+// initialize_with_config (prover.rs:61-111): file_id, witness, generators, com_0 and the
+// round-0 vector cross terms, all from one pass over the generator transforms.
+static int prover_initialize(dmx_prover* p) {
+    hipStream_t s = p->s;
+    const size_t N = p->N;
+    const int d = p->d;
+    const NttTables& tw = p->ctx->tw;
+    HIP_TRY(hipMemsetAsync(p->flags.p, 0, 256, s));
+    HIP_TRY(hipMemcpyAsync(p->tr.seeds.p, p->params.epoch_seed, 32, hipMemcpyHostToDevice, s));
+    device_hash(p->d_file, p->orig_len, IVW, 0, p->tr.cvs.as<uint32_t>(), p->tr.cv_words,
+                reinterpret_cast<uint32_t*>(p->tr.file_id()), nullptr, p->flags.as<uint32_t>(), s);
+    launch_expand(p->d_file, p->orig_len, p->W_a.as<zq>(), N * N, s);
+    RowSrc gen{};
+    gen.mode = ROW_GEN;
+    gen.dst = p->G_a.as<zq>();
+    memcpy(gen.gseed, p->gw, 32);
+    fwd_rows(p->ctx, gen, (size_t)K * N, d, p->Ghat_a.as<uint32_t>(), s);
+    RowSrc pl{};
+    pl.mode = ROW_PLAIN;
+    pl.a = p->W_a.as<zq>();
+    fwd_rows(p->ctx, pl, N, d, p->What_a.as<uint32_t>(), s);
+    const size_t slot = (size_t)K * NPRIMES * N;
+    launch_mac_pairs(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), N, d, p->res_at(0), p->res_at(slot),
+                     p->res_at(2 * slot), s);
+    launch_ntt_inv(p->res_at(0), 3 * K, d, tw, s);
+    launch_crt_lift(p->res_at(0), p->C.as<zq>(), K, d, s);
+    launch_crt_lift(p->res_at(slot), p->rec_lvec(0), 2 * K, d, s);  // L_vec, R_vec adjacent
+    HIP_TRY(hipMemcpyAsync(p->com0.p, p->C.p, (size_t)K * N * 16, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipGetLastError());
+    return DMX_OK;
+}
+
+// fold_round (prover.rs:140-313) for round j, m = N >> j: both stages and both invariants.
+static int prover_round(dmx_prover* p, size_t j) {
+    hipStream_t s = p->s;
+    const NttTables& tw = p->ctx->tw;
+    const size_t m = p->m_at(j), h = m / 2;
+    const int lm = p->d - (int)j;
+    uint32_t* flags = p->flags.as<uint32_t>();
+    HIP_TRY(hipEventRecord(p->ev[0], s));
+    // ---- vector stage ----
+    p->tr.challenge("vec", j, p->C.as<zq>(), m, p->rec_lvec(j), m, p->rec_rvec(j), m, p->sc(4 * j), flags, s);
+    launch_com_update(p->C.as<zq>(), p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j), 0, lm, p->Ct.as<zq>(), s);
+    RowSrc vw{};
+    vw.mode = ROW_VFOLD;
+    vw.a = p->W_a.as<zq>();
+    vw.b_off = h;
+    vw.scal = p->sc(4 * j);  // m_L + x * m_R
+    vw.dst = p->W_b.as<zq>();
+    fwd_rows(p->ctx, vw, h, lm, p->What_b.as<uint32_t>(), s);
+    RowSrc vg{};
+    vg.mode = ROW_VFOLD;
+    vg.a = p->G_a.as<zq>();
+    vg.b_off = h * K;
+    vg.scal = p->sc(4 * j + 1);  // g_L + x^-1 * g_R
+    vg.dst = p->G_b.as<zq>();
+    fwd_rows(p->ctx, vg, h * K, lm, p->Ghat_b.as<uint32_t>(), s);
+    // invariant at degree m and the poly cross terms at degree m/2 from one pass
+    const size_t s0 = (size_t)K * NPRIMES * m;
+    launch_mac_split(p->What_b.as<uint32_t>(), p->Ghat_b.as<uint32_t>(), h, lm, p->res_at(0), p->res_at(s0),
+                     p->res_at(s0 + (size_t)K * NPRIMES * h), tw, s);
+    launch_ntt_inv(p->res_at(0), K, lm, tw, s);
+    launch_ntt_inv(p->res_at(s0), 2 * K, lm - 1, tw, s);
+    launch_crt_lift(p->res_at(0), p->lifted.as<zq>(), K, lm, s);
+    launch_crt_lift(p->res_at(s0), p->rec_lpoly(j), 2 * K, lm - 1, s);
+    launch_com_check(p->lifted.as<zq>(), p->Ct.as<zq>(), (size_t)K * m, flags, vec_bit(j), s);
+    HIP_TRY(hipEventRecord(p->ev[1], s));
+    // ---- poly stage ----
+    p->tr.challenge("poly", j, p->Ct.as<zq>(), m, p->rec_lpoly(j), h, p->rec_rpoly(j), h, p->sc(4 * j + 2), flags, s);
+    launch_com_update(p->Ct.as<zq>(), p->rec_lpoly(j), p->rec_rpoly(j), p->sc(4 * j + 2), 1, lm - 1,
+                      p->C.as<zq>(), s);
+    RowSrc pw{};
+    pw.mode = ROW_PFOLD_W;
+    pw.a = p->W_b.as<zq>();
+    pw.scal = p->sc(4 * j + 2);  // e + y * o
+    pw.dst = p->W_a.as<zq>();
+    fwd_rows(p->ctx, pw, h, lm - 1, p->What_a.as<uint32_t>(), s);
+    RowSrc pg{};
+    pg.mode = ROW_PFOLD_G;
+    pg.a = p->G_b.as<zq>();
+    pg.scal = p->sc(4 * j + 3);  // g_e + y^-1 * X g_o
+    pg.dst = p->G_a.as<zq>();
+    fwd_rows(p->ctx, pg, h * K, lm - 1, p->Ghat_a.as<uint32_t>(), s);
+    // post-poly invariant, plus the next round's vector cross terms from the same pass
+    const size_t s1 = (size_t)K * NPRIMES * h;
+    const bool next = h > 1;
+    launch_mac_pairs(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), h, lm - 1, p->res_at(0),
+                     next ? p->res_at(s1) : nullptr, next ? p->res_at(2 * s1) : nullptr, s);
+    launch_ntt_inv(p->res_at(0), next ? 3 * K : K, lm - 1, tw, s);
+    launch_crt_lift(p->res_at(0), p->lifted.as<zq>(), K, lm - 1, s);
+    if (next) launch_crt_lift(p->res_at(s1), p->rec_lvec(j + 1), 2 * K, lm - 1, s);
+    launch_com_check(p->lifted.as<zq>(), p->C.as<zq>(), (size_t)K * h, flags, poly_bit(j), s);
+    HIP_TRY(hipEventRecord(p->ev[2], s));
+    HIP_TRY(hipGetLastError());
+    return DMX_OK;
+}
+
+static int check_round_flags(dmx_prover* p, size_t j, uint32_t f) {
+    if (f & FLAG_ZERO_CHALLENGE) return fail(DMX_ERR_HIP, "challenge resampling exhausted");
+    if (f & vec_bit(j)) return fail(DMX_ERR_INVARIANT, "vector folding invariant failed (round %zu)", j);
+    if (f & poly_bit(j)) return fail(DMX_ERR_INVARIANT, "round folding invariant failed (round %zu)", j);
+    return DMX_OK;
+}
+
+static int encode_record(dmx_prover* p, size_t j, uint8_t* out, size_t cap, size_t* len) {
+    const size_t m = p->m_at(j), h = m / 2;
+    const size_t need = dmx_record_size(m, h);
+    if (len) *len = need;
+    if (!out) return DMX_OK;
+    if (cap < need) return fail(DMX_ERR_BUFFER, "record buffer too small (%zu < %zu)", cap, need);
+    std::vector<zq> host((size_t)K * (2 * m + m));
+    HIP_TRY(hipMemcpyAsync(host.data(), p->rec_lvec(j), host.size() * 16, hipMemcpyDeviceToHost, p->s));
+    HIP_TRY(hipStreamSynchronize(p->s));
+    uint8_t* q = put_u64(out, j);
+    q = put_module(q, host.data(), m);
+    q = put_module(q, host.data() + (size_t)K * m, m);
+    q = put_module(q, host.data() + (size_t)2 * K * m, h);
+    put_module(q, host.data() + (size_t)2 * K * m + (size_t)K * h, h);
+    return DMX_OK;
+}
+
+static int prover_create(dmx_ctx* ctx, const uint8_t* file, const void* d_file, size_t len,
+                         const dmx_system_params* params, const dmx_runtime_config* cfg, dmx_prover** out,
+                         bool sync = true) {
+    if (!ctx || !params || !out || (!file && !d_file && len)) return fail(DMX_ERR_ARG, "null argument");
+    if (cfg && !cfg->gpu_enabled) return fail(DMX_ERR_NO_DEVICE, "gpu_enabled = false");
+    if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
+    const int d = depth_for(len);
+    const size_t N = (size_t)1 << d;
+    const uint64_t limit = cfg ? cfg->max_preprocess_bytes : (512ull << 20);
+    if (N * N > UINT64_MAX / 32 || (uint64_t)(N * N * 32) > limit)
+        return fail(DMX_ERR_MEMORY_LIMIT,
+                    "full witness expansion requires %llu bytes, above max_preprocess_bytes=%llu; split the file into smaller statements",
+                    (unsigned long long)(N * N * 32), (unsigned long long)limit);
+    if (d > 12) return fail(DMX_ERR_ARG, "depth %d exceeds the supported 12", d);
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<dmx_prover> p(new dmx_prover);
+    p->ctx = ctx;
+    p->params = *params;
+    p->orig_len = len;
+    p->d = d;
+    p->N = N;
+    HIP_TRY(hipStreamCreate(&p->s));
+    for (auto& e : p->ev) HIP_TRY(hipEventCreate(&e));
+    b3::host_hash_short(params->seed_generators, (const uint8_t*)"g0", 2, p->g_seed);  // sis.rs:70
+    gseed_words(p->g_seed, p->gw);
+    int rc = prover_alloc(p.get());
+    if (rc) return rc;
+    if (d_file) {
+        p->d_file = static_cast<const uint8_t*>(d_file);
+    } else {
+        rc = p->file.alloc(std::max<size_t>(len, 16));
+        if (rc) return rc;
+        if (len) HIP_TRY(hipMemcpyAsync(p->file.p, file, len, hipMemcpyHostToDevice, p->s));
+        p->d_file = p->file.as<uint8_t>();
+    }
+    rc = prover_initialize(p.get());
+    if (rc) return rc;
+    if (sync) {
+        HIP_TRY(hipStreamSynchronize(p->s));
+        uint32_t f = 0;
+        HIP_TRY(hipMemcpy(&f, p->flags.p, 4, hipMemcpyDeviceToHost));
+        if (f & FLAG_ZERO_CHALLENGE) return fail(DMX_ERR_HIP, "hashing failed");
+    }
+    *out = p.release();
+    return DMX_OK;
+}
+
+extern "C" int dmx_prover_new(dmx_ctx* ctx, const uint8_t* file, size_t len, const dmx_system_params* params,
+                              const dmx_runtime_config* cfg, dmx_prover** out) {
+    return prover_create(ctx, file, nullptr, len, params, cfg, out);
+}
+
+extern "C" void dmx_prover_free(dmx_prover* p) { delete p; }
+extern "C" size_t dmx_prover_rounds_total(const dmx_prover* p) { return p ? (size_t)p->d : 0; }
+extern "C" size_t dmx_prover_current_round(const dmx_prover* p) { return p ? p->round : 0; }
+extern "C" void dmx_prover_current_dimensions(const dmx_prover* p, size_t* rows, size_t* ring_len) {
+    const size_t m = p->N >> p->round;
+    if (rows) *rows = m;
+    if (ring_len) *ring_len = m;
+}
+
+extern "C" int dmx_prover_statement(const dmx_prover* p, uint8_t* out, size_t cap, size_t* len) {
+    if (!p) return fail(DMX_ERR_ARG, "null prover");
+    const size_t need = dmx_statement_size(p->N);
+    if (len) *len = need;
+    if (!out) return DMX_OK;
+    if (cap < need) return fail(DMX_ERR_BUFFER, "statement buffer too small");
+    std::vector<zq> com((size_t)K * p->N);
+    uint8_t fid[32];
+    HIP_TRY(hipMemcpyAsync(com.data(), p->com0.p, com.size() * 16, hipMemcpyDeviceToHost, p->s));
+    HIP_TRY(hipMemcpyAsync(fid, p->tr.file_id(), 32, hipMemcpyDeviceToHost, p->s));
+    HIP_TRY(hipStreamSynchronize(p->s));
+    uint8_t* q = out;
+    memcpy(q, fid, 32);
+    q = put_u64(q + 32, p->orig_len);
+    q = put_u64(q, (uint64_t)p->d);
+    q = put_module(q, com.data(), p->N);
+    memcpy(q, p->g_seed, 32);
+    return DMX_OK;
+}
+
+extern "C" int dmx_prover_current_commitment(const dmx_prover* p, uint64_t* out, size_t cap_words, size_t* ring_len) {
+    const size_t m = p->N >> p->round;
+    if (ring_len) *ring_len = m;
+    if (!out) return DMX_OK;
+    if (cap_words < (size_t)2 * K * m) return fail(DMX_ERR_BUFFER, "commitment buffer too small");
+    HIP_TRY(hipMemcpyAsync(out, p->C.p, (size_t)K * m * 16, hipMemcpyDeviceToHost, p->s));
+    HIP_TRY(hipStreamSynchronize(p->s));
+    return DMX_OK;
+}
+
+extern "C" int dmx_prover_fold_round(dmx_prover* p, size_t j, uint8_t* record_out, size_t cap, size_t* len,
+                                     dmx_round_output* timing) {
+    if (!p) return fail(DMX_ERR_ARG, "null prover");
+    if (p->poisoned) return fail(DMX_ERR_POISONED, "prover unusable after an earlier failure");
+    if (j != p->round) return fail(DMX_ERR_ROUND, "round index mismatch");
+    if (p->round >= (size_t)p->d) return fail(DMX_ERR_ROUND, "all rounds have finished");
+    const size_t m = p->m_at(j);
+    if (record_out && cap < dmx_record_size(m, m / 2)) {
+        if (len) *len = dmx_record_size(m, m / 2);
+        return fail(DMX_ERR_BUFFER, "record buffer too small");
+    }
+    int rc = prover_round(p, j);
+    if (!rc) rc = hipStreamSynchronize(p->s) == hipSuccess ? DMX_OK : fail(DMX_ERR_HIP, "stream sync failed");
+    uint32_t f = 0;
+    if (!rc && hipMemcpy(&f, p->flags.p, 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(DMX_ERR_HIP, "flags");
+    if (!rc) rc = check_round_flags(p, j, f);
+    if (rc) {
+        p->poisoned = true;
+        return rc;
+    }
+    if (timing) {
+        float a = 0, b = 0;
+        (void)hipEventElapsedTime(&a, p->ev[0], p->ev[1]);
+        (void)hipEventElapsedTime(&b, p->ev[1], p->ev[2]);
+        timing->vector_fold_ns = (uint64_t)(a * 1e6);
+        timing->poly_fold_ns = (uint64_t)(b * 1e6);
+        timing->total_round_ns = (uint64_t)((a + b) * 1e6);
+    }
+    rc = encode_record(p, j, record_out, cap, len);
+    if (rc) return rc;
+    p->round += 1;
+    return DMX_OK;
+}
+
+extern "C" int dmx_prover_final_opening(const dmx_prover* p, uint64_t out[2]) {
+    if (!p || p->round != (size_t)p->d) return fail(DMX_ERR_NOT_FINAL, "rounds remain before the final opening");
+    HIP_TRY(hipMemcpyAsync(out, p->W_a.p, 16, hipMemcpyDeviceToHost, p->s));
+    HIP_TRY(hipStreamSynchronize(p->s));
+    return DMX_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// verifier (src/protocol/verifier.rs:21-134)
+// ------------------------------------------------------------------------------------------
+struct dmx_verifier {
+    dmx_ctx* ctx = nullptr;
+    hipStream_t s = nullptr;
+    dmx_system_params params{};
+    uint8_t file_id[32], g_seed[32];
+    size_t rounds = 0, round = 0;
+    size_t rows = 0, glen = 0, clen = 0;
+    size_t g_cap = 0, c_cap = 0;
+    DevBuf G0, G1, C, Ct, rec, scal, flags, opening;
+    int gcur = 0;  // which of G0/G1 holds the state
+    Transcriber tr;
+    ~dmx_verifier() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+    zq* G() const { return (gcur ? G1 : G0).as<zq>(); }
+    zq* Gother() const { return (gcur ? G0 : G1).as<zq>(); }
+};
+
+static int verifier_alloc(dmx_verifier* v) {
+    int rc = 0;
+    rc = rc ? rc : v->G0.alloc(v->g_cap * 16);
+    rc = rc ? rc : v->G1.alloc(std::max<size_t>(v->g_cap / 2, 1) * 16);
+    rc = rc ? rc : v->C.alloc((size_t)K * v->c_cap * 16);
+    rc = rc ? rc : v->Ct.alloc((size_t)K * v->c_cap * 16);
+    rc = rc ? rc : v->rec.alloc((size_t)4 * K * v->c_cap * 16);
+    rc = rc ? rc : v->scal.alloc(4 * 16);
+    rc = rc ? rc : v->flags.alloc(256);
+    rc = rc ? rc : v->opening.alloc(16);
+    rc = rc ? rc : v->tr.init(v->c_cap, 0);
+    return rc;
+}
+
+// DamascusVerifier::new (verifier.rs:22-43): generators re-derived from the statement's
+// g_0_seed on the device. com0 / file_id come from host memory, or device memory when
+// `on_device` (the fused prove+verify path).
+static int verifier_make(dmx_ctx* ctx, const dmx_system_params* params, const uint8_t* file_id, uint64_t orig_len,
+                         uint64_t d, const zq* com0, size_t com_len, const uint8_t g_seed[32], bool on_device,
+                         dmx_verifier** out, hipEvent_t wait_before_copy = nullptr) {
+    if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
+    if (com_len == 0 || (com_len & (com_len - 1)) || com_len > ((size_t)1 << MAX_LOG_N))
+        return fail(DMX_ERR_SHAPE, "statement ring length %zu is not a supported power of two", com_len);
+    const int ld = depth_for(orig_len);
+    if (ld > 12) return fail(DMX_ERR_ARG, "depth %d exceeds the supported 12", ld);
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<dmx_verifier> v(new dmx_verifier);
+    v->ctx = ctx;
+    v->params = *params;
+    memcpy(v->g_seed, g_seed, 32);
+    v->rounds = (size_t)d;
+    v->rows = (size_t)1 << ld;
+    v->glen = v->clen = com_len;
+    v->g_cap = v->rows * K * v->glen;
+    v->c_cap = v->clen;
+    HIP_TRY(hipStreamCreate(&v->s));
+    int rc = verifier_alloc(v.get());
+    if (rc) return rc;
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIP_TRY(hipMemcpyAsync(v->tr.seeds.p, params->epoch_seed, 32, hipMemcpyHostToDevice, v->s));
+    HIP_TRY(hipMemsetAsync(v->flags.p, 0, 256, v->s));
+    uint32_t gw[8];
+    gseed_words(g_seed, gw);
+    int lg = 0;
+    while (((size_t)1 << lg) < v->glen) ++lg;
+    launch_gen_generators(gw, v->rows * K, lg, v->G0.as<zq>(), v->s);  // verifier.rs:31-33
+    if (wait_before_copy) HIP_TRY(hipStreamWaitEvent(v->s, wait_before_copy, 0));
+    HIP_TRY(hipMemcpyAsync(v->tr.file_id(), file_id, 32, kind, v->s));
+    HIP_TRY(hipMemcpyAsync(v->C.p, com0, (size_t)K * v->clen * 16, kind, v->s));
+    HIP_TRY(hipGetLastError());
+    *out = v.release();
+    return DMX_OK;
+}
+
+extern "C" int dmx_verifier_new(dmx_ctx* ctx, const dmx_system_params* params, const uint8_t* statement, size_t len,
+                                dmx_verifier** out) {
+    if (!ctx || !params || !statement || !out) return fail(DMX_ERR_ARG, "null argument");
+    Statement st;
+    int rc = decode_statement(statement, len, st);
+    if (rc) return rc;
+    dmx_verifier* v = nullptr;
+    rc = verifier_make(ctx, params, st.file_id, st.original_len, st.d, st.com0.data(), st.com_len, st.g_seed, false, &v);
+    if (rc) return rc;
+    memcpy(v->file_id, st.file_id, 32);
+    if (hipStreamSynchronize(v->s) != hipSuccess) {
+        delete v;
+        return fail(DMX_ERR_HIP, "verifier setup failed");
+    }
+    *out = v;
+    return DMX_OK;
+}
+
+extern "C" dmx_verifier* dmx_verifier_clone(const dmx_verifier* v) {
+    if (!v) return nullptr;
+    std::unique_ptr<dmx_verifier> c(new dmx_verifier);
+    c->ctx = v->ctx;
+    c->params = v->params;
+    memcpy(c->file_id, v->file_id, 32);
+    memcpy(c->g_seed, v->g_seed, 32);
+    c->rounds = v->rounds;
+    c->round = v->round;
+    c->rows = v->rows;
+    c->glen = v->glen;
+    c->clen = v->clen;
+    c->g_cap = v->rows * K * v->glen;
+    c->c_cap = v->clen;
+    if (hipStreamCreate(&c->s) != hipSuccess) return nullptr;
+    if (verifier_alloc(c.get())) return nullptr;
+    (void)hipStreamSynchronize(v->s);
+    if (hipMemcpyAsync(c->G0.p, v->G(), c->g_cap * 16, hipMemcpyDeviceToDevice, c->s) != hipSuccess) return nullptr;
+    if (hipMemcpyAsync(c->C.p, v->C.p, (size_t)K * v->clen * 16, hipMemcpyDeviceToDevice, c->s) != hipSuccess)
+        return nullptr;
+    if (hipMemcpyAsync(c->tr.seeds.p, v->tr.seeds.p, 64, hipMemcpyDeviceToDevice, c->s) != hipSuccess) return nullptr;
+    (void)hipMemsetAsync(c->flags.p, 0, 256, c->s);
+    if (hipStreamSynchronize(c->s) != hipSuccess) return nullptr;
+    return c.release();
+}
+
+extern "C" void dmx_verifier_free(dmx_verifier* v) { delete v; }
+extern "C" size_t dmx_verifier_round(const dmx_verifier* v) { return v ? v->round : 0; }
+extern "C" void dmx_verifier_current_dimensions(const dmx_verifier* v, size_t* rows, size_t* ring_len) {
+    if (rows) *rows = v->rows;
+    if (ring_len) *ring_len = v->clen;
+}
+extern "C" int dmx_verifier_current_commitment(const dmx_verifier* v, uint64_t* out, size_t cap_words, size_t* ring_len) {
+    if (ring_len) *ring_len = v->clen;
+    if (!out) return DMX_OK;
+    if (cap_words < (size_t)2 * K * v->clen) return fail(DMX_ERR_BUFFER, "commitment buffer too small");
+    HIP_TRY(hipMemcpyAsync(out, v->C.p, (size_t)K * v->clen * 16, hipMemcpyDeviceToHost, v->s));
+    HIP_TRY(hipStreamSynchronize(v->s));
+    return DMX_OK;
+}
+
+static bool all_zero(const std::vector<zq>& m) {
+    for (const zq& x : m)
+        if (!zq_is_zero(x)) return false;
+    return true;
+}
+
+static int log2_exact(size_t n) {
+    int l = 0;
+    while (((size_t)1 << l) < n) ++l;
+    return l;
+}
+
+// One replay round on the device (verifier.rs:57-120) with the round's four cross-term
+// modules already in device memory and their shapes already checked. No host sync.
+static int verifier_round_launch(dmx_verifier* v, const zq* d_l, const zq* d_r, const zq* d_lp, const zq* d_rp) {
+    hipStream_t s = v->s;
+    const size_t n = v->clen;
+    const bool vec = v->rows > 1, poly = v->glen > 1;
+    uint32_t* flags = v->flags.as<uint32_t>();
+    const int lg = log2_exact(v->glen);
+    size_t rows = v->rows;
+    const zq* Ct = v->C.as<zq>();
+    if (vec) {
+        v->tr.challenge("vec", v->round, v->C.as<zq>(), n, d_l, n, d_r, n, v->scal.as<zq>(), flags, s);
+        launch_com_update(v->C.as<zq>(), d_l, d_r, v->scal.as<zq>(), 0, log2_exact(n), v->Ct.as<zq>(), s);
+        Ct = v->Ct.as<zq>();
+        RowSrc f{};
+        f.mode = ROW_VFOLD;
+        f.a = v->G();
+        f.b_off = (rows / 2) * K;
+        f.scal = v->scal.as<zq>() + 1;  // g_L + x^-1 g_R (verifier.rs:141-151)
+        f.dst = v->Gother();
+        launch_fold_rows_zq(f, (rows / 2) * K, lg, s);
+        v->gcur ^= 1;
+        rows /= 2;
+    }
+    size_t nclen = n, nglen = v->glen;
+    if (poly) {
+        v->tr.challenge("poly", v->round, Ct, n, d_lp, n / 2, d_rp, n / 2, v->scal.as<zq>() + 2, flags, s);
+        launch_com_update(Ct, d_lp, d_rp, v->scal.as<zq>() + 2, 1, log2_exact(n / 2), v->C.as<zq>(), s);
+        RowSrc f{};
+        f.mode = ROW_PFOLD_G;
+        f.a = v->G();
+        f.scal = v->scal.as<zq>() + 3;  // g_e + y^-1 X g_o (verifier.rs:153-179)
+        f.dst = v->Gother();
+        launch_fold_rows_zq(f, rows * K, lg - 1, s);
+        v->gcur ^= 1;
+        nclen = n / 2;
+        nglen = v->glen / 2;
+    } else if (vec) {
+        HIP_TRY(hipMemcpyAsync(v->C.p, v->Ct.p, (size_t)K * n * 16, hipMemcpyDeviceToDevice, s));
+    }
+    HIP_TRY(hipGetLastError());
+    v->rows = rows;
+    v->clen = nclen;
+    v->glen = nglen;
+    v->round += 1;
+    return DMX_OK;
+}
+
+// update_commitment (verifier.rs:57-120); every check runs before any device work, so a
+// rejected record leaves the state unchanged, as in the reference.
+extern "C" int dmx_verifier_update_commitment(dmx_verifier* v, const uint8_t* record, size_t len) {
+    if (!v || !record) return fail(DMX_ERR_ARG, "null argument");
+    Record rec;
+    int rc = decode_record(record, len, rec);
+    if (rc) return rc;
+    if (rec.round != v->round) return fail(DMX_ERR_ROUND, "round mismatch in round record");
+    const size_t n = v->clen;
+    const bool vec = v->rows > 1, poly = v->glen > 1;
+    if (vec) {
+        if (rec.len[0] != n || rec.len[1] != n) return fail(DMX_ERR_SHAPE, "module ring length mismatch");
+    } else {
+        if (rec.len[0] != n || rec.len[1] != n || !all_zero(rec.m[0]) || !all_zero(rec.m[1]))
+            return fail(DMX_ERR_ZERO_TERMS, "scalar vector stage must emit zero vec cross-terms");
+    }
+    if (poly) {
+        if (n != v->glen || n < 2) return fail(DMX_ERR_SHAPE, "module ring length mismatch");
+        if (rec.len[2] != n / 2 || rec.len[3] != n / 2) return fail(DMX_ERR_SHAPE, "module ring length mismatch");
+    } else {
+        if (rec.len[2] != 1 || rec.len[3] != 1 || !all_zero(rec.m[2]) || !all_zero(rec.m[3]))
+            return fail(DMX_ERR_ZERO_TERMS, "scalar stage must emit zero poly cross-terms");
+    }
+    hipStream_t s = v->s;
+    zq* d_l = v->rec.as<zq>();
+    zq* d_r = d_l + (size_t)K * n;
+    zq* d_lp = d_l + (size_t)2 * K * n;
+    zq* d_rp = d_lp + (size_t)K * (n / 2 ? n / 2 : 1);
+    if (vec) {
+        HIP_TRY(hipMemcpyAsync(d_l, rec.m[0].data(), (size_t)K * n * 16, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_r, rec.m[1].data(), (size_t)K * n * 16, hipMemcpyHostToDevice, s));
+    }
+    if (poly) {
+        HIP_TRY(hipMemcpyAsync(d_lp, rec.m[2].data(), (size_t)K * (n / 2) * 16, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_rp, rec.m[3].data(), (size_t)K * (n / 2) * 16, hipMemcpyHostToDevice, s));
+    }
+    // state (rows/lens/round) only advances after the device work is known good
+    const size_t rows0 = v->rows, clen0 = v->clen, glen0 = v->glen, round0 = v->round;
+    const int gcur0 = v->gcur;
+    rc = verifier_round_launch(v, d_l, d_r, d_lp, d_rp);
+    uint32_t f = 0;
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(DMX_ERR_HIP, "stream sync failed");
+    if (!rc && hipMemcpy(&f, v->flags.p, 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(DMX_ERR_HIP, "flags");
+    if (!rc && f) rc = fail(DMX_ERR_HIP, "challenge resampling exhausted");
+    if (rc) {
+        v->rows = rows0;
+        v->clen = clen0;
+        v->glen = glen0;
+        v->round = round0;
+        v->gcur = gcur0;
+    }
+    return rc;
+}
+
+static int verifier_final_launch(dmx_verifier* v, const zq* d_opening) {
+    if (v->round != v->rounds) return fail(DMX_ERR_NOT_FINAL, "cannot verify final opening before all rounds replay");
+    if (v->rows != 1) return fail(DMX_ERR_NOT_FINAL, "terminal generator family must have length one");
+    if (v->glen != v->clen) return fail(DMX_ERR_OPENING, "final opening mismatch");
+    HIP_TRY(hipMemsetAsync(v->flags.p, 0, 4, v->s));
+    launch_final_check(v->G(), v->C.as<zq>(), d_opening, (size_t)K * v->glen, v->flags.as<uint32_t>(), 1, v->s);
+    HIP_TRY(hipGetLastError());
+    return DMX_OK;
+}
+
+static int verifier_final_result(dmx_verifier* v) {
+    uint32_t f = 0;
+    HIP_TRY(hipMemcpyAsync(&f, v->flags.p, 4, hipMemcpyDeviceToHost, v->s));
+    HIP_TRY(hipStreamSynchronize(v->s));
+    if (f) return fail(DMX_ERR_OPENING, "final opening mismatch");
+    return DMX_OK;
+}
+
+// verify_final_opening (verifier.rs:122-134): g_d[0] * opening == C_d
+extern "C" int dmx_verifier_verify_final_opening(const dmx_verifier* v_, const uint64_t opening[2]) {
+    dmx_verifier* v = const_cast<dmx_verifier*>(v_);
+    if (!v || !opening) return fail(DMX_ERR_ARG, "null argument");
+    uint32_t w[8] = {(uint32_t)opening[0], (uint32_t)(opening[0] >> 32), (uint32_t)opening[1],
+                     (uint32_t)(opening[1] >> 32), 0, 0, 0, 0};
+    zq o = zq_reduce256(w);  // an Fp is always canonical
+    HIP_TRY(hipMemcpyAsync(v->opening.p, &o, 16, hipMemcpyHostToDevice, v->s));
+    int rc = verifier_final_launch(v, v->opening.as<zq>());
+    if (rc) return rc;
+    return verifier_final_result(v);
+}
+
+// ------------------------------------------------------------------------------------------
+// whole jobs
+// ------------------------------------------------------------------------------------------
+extern "C" int dmx_prove(dmx_ctx* ctx, const uint8_t* file, size_t len, const dmx_system_params* params,
+                         const dmx_runtime_config* cfg, uint8_t* st_out, size_t st_cap, size_t* st_len,
+                         uint8_t* rec_out, size_t rec_cap, size_t* rec_len, uint64_t opening_out[2]) {
+    dmx_prover* p = nullptr;
+    int rc = dmx_prover_new(ctx, file, len, params, cfg, &p);
+    if (rc) return rc;
+    std::unique_ptr<dmx_prover> guard(p);
+    rc = dmx_prover_statement(p, st_out, st_cap, st_len);
+    size_t pos = 0;
+    for (size_t j = 0; !rc && j < (size_t)p->d; ++j) {
+        size_t got = 0;
+        const size_t need = dmx_record_size(p->N >> j, p->N >> (j + 1));
+        uint8_t* dst = (rec_out && pos + need <= rec_cap) ? rec_out + pos : nullptr;
+        rc = dmx_prover_fold_round(p, j, dst, need, &got, nullptr);
+        pos += need;
+    }
+    if (rec_len) *rec_len = pos;
+    if (!rc && rec_out && pos > rec_cap) rc = fail(DMX_ERR_BUFFER, "records buffer too small");
+    if (!rc && opening_out) rc = dmx_prover_final_opening(p, opening_out);
+    return rc;
+}
+
+extern "C" int dmx_verify(dmx_ctx* ctx, const dmx_system_params* params, const uint8_t* st, size_t st_len,
+                          const uint8_t* recs, size_t rec_len, const uint64_t opening[2]) {
+    dmx_verifier* v = nullptr;
+    int rc = dmx_verifier_new(ctx, params, st, st_len, &v);
+    if (rc) return rc;
+    std::unique_ptr<dmx_verifier> guard(v);
+    size_t pos = 0;
+    while (!rc && pos < rec_len) {
+        Record r;
+        rc = decode_record(recs + pos, rec_len - pos, r);
+        if (rc) break;
+        size_t used = 8;
+        for (int i = 0; i < 4; ++i) used += module_bytes(r.len[i]);
+        rc = dmx_verifier_update_commitment(v, recs + pos, used);
+        pos += used;
+    }
+    if (!rc) rc = dmx_verifier_verify_final_opening(v, opening);
+    return rc;
+}
+
+// prove + replay from a device-resident file. The verifier runs on its own stream and
+// consumes the prover's device-resident records; its generator derivation and folds overlap
+// the prover's next round. Both parties recompute every challenge themselves.
+extern "C" int dmx_prove_verify_device(dmx_ctx* ctx, const void* d_file, size_t len, const dmx_system_params* params,
+                                       uint8_t* st_out, size_t st_cap, size_t* st_len, uint8_t* rec_out,
+                                       size_t rec_cap, size_t* rec_len, uint64_t opening_out[2]) {
+    if (!ctx || !params || (!d_file && len)) return fail(DMX_ERR_ARG, "null argument");
+    dmx_runtime_config cfg{};
+    cfg.max_preprocess_bytes = UINT64_MAX;  // examples/full_flow.rs:29
+    cfg.ntt_enabled = cfg.parallel_enabled = cfg.gpu_enabled = 1;
+    dmx_prover* p = nullptr;
+    int rc = prover_create(ctx, nullptr, d_file, len, params, &cfg, &p, /*sync=*/false);
+    if (rc) return rc;
+    std::unique_ptr<dmx_prover> pg(p);
+    hipEvent_t ready;
+    HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    struct EvGuard {
+        hipEvent_t e;
+        ~EvGuard() { (void)hipEventDestroy(e); }
+    } eg{ready};
+    HIP_TRY(hipEventRecord(ready, p->s));
+    dmx_verifier* v = nullptr;
+    rc = verifier_make(ctx, params, p->tr.file_id(), len, (uint64_t)p->d, p->com0.as<zq>(), p->N, p->g_seed, true, &v,
+                       ready);
+    if (rc) return rc;
+    std::unique_ptr<dmx_verifier> vg(v);
+    for (size_t j = 0; j < (size_t)p->d; ++j) {
+        rc = prover_round(p, j);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(ready, p->s));
+        HIP_TRY(hipStreamWaitEvent(v->s, ready, 0));
+        rc = verifier_round_launch(v, p->rec_lvec(j), p->rec_rvec(j), p->rec_lpoly(j), p->rec_rpoly(j));
+        if (rc) return rc;
+    }
+    p->round = (size_t)p->d;
+    rc = verifier_final_launch(v, p->W_a.as<zq>());  // m* = message[0].coeffs[0] (prover.rs:133-138)
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(p->s));
+    uint32_t f = 0;
+    HIP_TRY(hipMemcpy(&f, p->flags.p, 4, hipMemcpyDeviceToHost));
+    for (size_t j = 0; j < (size_t)p->d; ++j) {
+        rc = check_round_flags(p, j, f);
+        if (rc) return rc;
+    }
+    rc = verifier_final_result(v);
+    if (rc) return rc;
+    if (st_len || st_out) {
+        rc = dmx_prover_statement(p, st_out, st_cap, st_len);
+        if (rc) return rc;
+    }
+    if (rec_len || rec_out) {
+        const size_t need = dmx_proof_records_size(len);
+        if (rec_len) *rec_len = need;
+        if (rec_out) {
+            if (rec_cap < need) return fail(DMX_ERR_BUFFER, "records buffer too small");
+            size_t pos = 0;
+            for (size_t j = 0; j < (size_t)p->d; ++j) {
+                size_t got = 0;
+                rc = encode_record(p, j, rec_out + pos, rec_cap - pos, &got);
+                if (rc) return rc;
+                pos += got;
+            }
+        }
+    }
+    if (opening_out) {
+        HIP_TRY(hipMemcpy(opening_out, p->W_a.p, 16, hipMemcpyDeviceToHost));
+    }
+    return DMX_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// reference-compatible batch entry points (return codes follow cuda/*.cu: 0 ok, 1 args, 2 device)
+// ------------------------------------------------------------------------------------------
+extern "C" int dmx_fold_batch(const uint64_t* l_lo, const uint64_t* l_hi, const uint64_t* r_lo, const uint64_t* r_hi,
+                              uint64_t* o_lo, uint64_t* o_hi, size_t len, uint64_t c_lo, uint64_t c_hi) {
+    if (!l_lo || !l_hi || !r_lo || !r_hi || !o_lo || !o_hi || len == 0) return 1;
+    if (!dmx_backend_ready()) return 2;
+    DevBuf b;
+    if (b.alloc(len * 8 * 6)) return 2;
+    uint64_t* d = b.as<uint64_t>();
+    const size_t by = len * 8;
+    if (hipMemcpy(d, l_lo, by, hipMemcpyHostToDevice) || hipMemcpy(d + len, l_hi, by, hipMemcpyHostToDevice) ||
+        hipMemcpy(d + 2 * len, r_lo, by, hipMemcpyHostToDevice) ||
+        hipMemcpy(d + 3 * len, r_hi, by, hipMemcpyHostToDevice))
+        return 2;
+    launch_fold_pairs(d, d + len, d + 2 * len, d + 3 * len, d + 4 * len, d + 5 * len, len, c_lo, c_hi, 0);
+    if (hipGetLastError() != hipSuccess) return 2;
+    if (hipMemcpy(o_lo, d + 4 * len, by, hipMemcpyDeviceToHost) || hipMemcpy(o_hi, d + 5 * len, by, hipMemcpyDeviceToHost))
+        return 2;
+    return 0;
+}
+
+extern "C" int dmx_ntt_batch(const uint64_t* a, const uint64_t* b, uint64_t* out, size_t n, const uint64_t* primes,
+                             const uint64_t* stage_roots, const uint64_t* inv_roots, const uint64_t* inv_sizes) {
+    if (!a || !b || !out || !primes || !stage_roots || !inv_roots || !inv_sizes || n == 0 || (n & (n - 1))) return 1;
+    for (int i = 0; i < NPRIMES; ++i)
+        if (primes[i] < 2 || primes[i] >= (1ull << 32)) return 1;
+    if (!dmx_backend_ready()) return 2;
+    int log_n = 0;
+    while (((size_t)1 << log_n) < n) ++log_n;
+    const size_t words = (size_t)NPRIMES * n, roots = (size_t)NPRIMES * std::max(log_n, 1);
+    DevBuf buf;
+    if (buf.alloc((2 * words + 2 * roots + 2 * NPRIMES) * 8)) return 2;
+    uint64_t* da = buf.as<uint64_t>();
+    uint64_t* db = da + words;
+    uint64_t* dr = db + words;
+    uint64_t* dir = dr + roots;
+    uint64_t* dp = dir + roots;
+    uint64_t* dinv = dp + NPRIMES;
+    if (hipMemcpy(da, a, words * 8, hipMemcpyHostToDevice) || hipMemcpy(db, b, words * 8, hipMemcpyHostToDevice) ||
+        (log_n && hipMemcpy(dr, stage_roots, (size_t)NPRIMES * log_n * 8, hipMemcpyHostToDevice)) ||
+        (log_n && hipMemcpy(dir, inv_roots, (size_t)NPRIMES * log_n * 8, hipMemcpyHostToDevice)) ||
+        hipMemcpy(dp, primes, NPRIMES * 8, hipMemcpyHostToDevice) ||
+        hipMemcpy(dinv, inv_sizes, NPRIMES * 8, hipMemcpyHostToDevice))
+        return 2;
+    launch_cyclic_ntt_batch(da, db, n, dp, dr, dir, dinv, 0);
+    if (hipGetLastError() != hipSuccess) return 2;
+    if (hipMemcpy(out, da, words * 8, hipMemcpyDeviceToHost)) return 2;
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// commitments and products through the context
+// ------------------------------------------------------------------------------------------
+extern "C" int dmx_derive_subseed(const uint8_t seed[32], const uint8_t* label, size_t label_len, uint8_t out[32]) {
+    if (!seed || !out || (label_len && !label) || label_len > 64) return fail(DMX_ERR_ARG, "bad argument");
+    b3::host_hash_short(seed, label, label_len, out);
+    return DMX_OK;
+}
+
+extern "C" int dmx_derive_generators(dmx_ctx* ctx, const uint8_t g_seed[32], size_t rows, size_t ring_len,
+                                     uint64_t* out) {
+    if (!ctx || !g_seed || !out) return fail(DMX_ERR_ARG, "null argument");
+    if (!ring_len || (ring_len & (ring_len - 1)) || ring_len > ((size_t)1 << MAX_LOG_N))
+        return fail(DMX_ERR_SHAPE, "ring length must be a power of two <= 8192");
+    if (!rows) return DMX_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevBuf g;
+    int rc = g.alloc(rows * K * ring_len * 16);
+    if (rc) return rc;
+    uint32_t gw[8];
+    gseed_words(g_seed, gw);
+    launch_gen_generators(gw, rows * K, log2_exact(ring_len), g.as<zq>(), 0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, g.p, rows * K * ring_len * 16, hipMemcpyDeviceToHost));
+    return DMX_OK;
+}
+
+// commit_with_generators_ntt (sis.rs:146-167) on device; gens == nullptr derives them from
+// g_seed. The sum over rows is split so that each partial stays inside the CRT range.
+static int commit_impl(dmx_ctx* ctx, const uint64_t* witness, const uint64_t* gens, const uint8_t* g_seed,
+                       size_t rows, size_t ring_len, uint64_t* com_out, size_t* com_len) {
+    if (!ctx || !com_out || (rows && !witness)) return fail(DMX_ERR_ARG, "null argument");
+    if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
+    if (rows == 0) {
+        memset(com_out, 0, (size_t)K * 16);
+        if (com_len) *com_len = 1;
+        return DMX_OK;
+    }
+    if (!ring_len || (ring_len & (ring_len - 1)) || ring_len > ((size_t)1 << MAX_LOG_N))
+        return fail(DMX_ERR_SHAPE, "ring length must be a power of two <= 8192");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int lg = log2_exact(ring_len);
+    const size_t m = ring_len;
+    // rows per partial sum: rows * m <= 2^24 keeps |V| <= rows*m*(q/2)^2 < P/2
+    const size_t chunk = std::max<size_t>(1, ((size_t)1 << 24) / m);
+    const size_t crow = std::min(rows, chunk);
+    DevBuf W, G, What, Ghat, res, part, acc;
+    int rc = 0;
+    rc = rc ? rc : W.alloc(crow * m * 16);
+    rc = rc ? rc : G.alloc(crow * K * m * 16);
+    rc = rc ? rc : What.alloc(crow * NPRIMES * m * 4);
+    rc = rc ? rc : Ghat.alloc(crow * K * NPRIMES * m * 4);
+    rc = rc ? rc : res.alloc((size_t)K * NPRIMES * m * 4);
+    rc = rc ? rc : part.alloc((size_t)K * m * 16);
+    rc = rc ? rc : acc.alloc((size_t)K * m * 16);
+    if (rc) return rc;
+    hipStream_t s = 0;
+    HIP_TRY(hipMemsetAsync(acc.p, 0, (size_t)K * m * 16, s));
+    uint32_t gw[8] = {0};
+    if (g_seed) gseed_words(g_seed, gw);
+    for (size_t r0 = 0; r0 < rows; r0 += crow) {
+        const size_t nr = std::min(crow, rows - r0);
+        HIP_TRY(hipMemcpyAsync(W.p, witness + 2 * r0 * m, nr * m * 16, hipMemcpyHostToDevice, s));
+        RowSrc pw{};
+        pw.mode = ROW_PLAIN;
+        pw.a = W.as<zq>();
+        launch_ntt_fwd_rows(pw, nr, lg, What.as<uint32_t>(), ctx->tw, s);
+        RowSrc pg{};
+        if (gens) {
+            HIP_TRY(hipMemcpyAsync(G.p, gens + 2 * r0 * K * m, nr * K * m * 16, hipMemcpyHostToDevice, s));
+            pg.mode = ROW_PLAIN;
+            pg.a = G.as<zq>();
+            launch_ntt_fwd_rows(pg, nr * K, lg, Ghat.as<uint32_t>(), ctx->tw, s);
+        } else {
+            // generator rows r0.. of the family: generate the full family prefix, keep the slice
+            DevBuf full;
+            rc = full.alloc((r0 + nr) * K * m * 16);
+            if (rc) return rc;
+            launch_gen_generators(gw, (r0 + nr) * K, lg, full.as<zq>(), s);
+            pg.mode = ROW_PLAIN;
+            pg.a = full.as<zq>() + r0 * K * m;
+            launch_ntt_fwd_rows(pg, nr * K, lg, Ghat.as<uint32_t>(), ctx->tw, s);
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        launch_mac_pairs(What.as<uint32_t>(), Ghat.as<uint32_t>(), nr, lg, res.as<uint32_t>(), nullptr, nullptr, s);
+        launch_ntt_inv(res.as<uint32_t>(), K, lg, ctx->tw, s);
+        launch_crt_lift(res.as<uint32_t>(), part.as<zq>(), K, lg, s);
+        launch_zq_add(acc.as<zq>(), part.as<zq>(), acc.as<zq>(), (size_t)K * m, s);
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(com_out, acc.p, (size_t)K * m * 16, hipMemcpyDeviceToHost));
+    if (com_len) *com_len = m;
+    return DMX_OK;
+}
+
+extern "C" int dmx_commit(dmx_ctx* ctx, const uint8_t seed[32], const uint64_t* witness, size_t rows, size_t ring_len,
+                          uint64_t* com_out, size_t* com_len) {
+    if (!seed) return fail(DMX_ERR_ARG, "null seed");
+    uint8_t gs[32];
+    b3::host_hash_short(seed, (const uint8_t*)"g0", 2, gs);
+    return commit_impl(ctx, witness, nullptr, gs, rows, ring_len, com_out, com_len);
+}
+
+extern "C" int dmx_commit_with_generators(dmx_ctx* ctx, const uint64_t* witness, const uint64_t* gens, size_t rows,
+                                          size_t ring_len, uint64_t* com_out, size_t* com_len) {
+    if (rows && !gens) return fail(DMX_ERR_ARG, "null generators");
+    return commit_impl(ctx, witness, gens, nullptr, rows, ring_len, com_out, com_len);
+}
+
+// ragged negacyclic products: group by degree, transform both operands, multiply, invert, lift
+extern "C" int dmx_ring_mul_batch(dmx_ctx* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out,
+                                  const uint32_t* log_n, size_t count) {
+    if (!ctx || (count && (!a || !b || !out || !log_n))) return fail(DMX_ERR_ARG, "null argument");
+    if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
+    if (!count) return DMX_OK;
+    std::vector<uint64_t> offs(count);
+    size_t total = 0;
+    for (size_t i = 0; i < count; ++i) {
+        if (log_n[i] > (uint32_t)MAX_LOG_N) return fail(DMX_ERR_SHAPE, "degree 2^%u exceeds 2^13", log_n[i]);
+        offs[i] = total;
+        total += (size_t)1 << log_n[i];
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = 0;
+    DevBuf dA, dB, dO, dOff, dIdx;
+    int rc = 0;
+    rc = rc ? rc : dA.alloc(total * 16);
+    rc = rc ? rc : dB.alloc(total * 16);
+    rc = rc ? rc : dO.alloc(total * 16);
+    rc = rc ? rc : dOff.alloc(count * 8);
+    rc = rc ? rc : dIdx.alloc(count * 4);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(dA.p, a, total * 16, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dB.p, b, total * 16, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dOff.p, offs.data(), count * 8, hipMemcpyHostToDevice, s));
+    std::vector<uint32_t> order(count);
+    for (size_t i = 0; i < count; ++i) order[i] = (uint32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return log_n[x] < log_n[y]; });
+    HIP_TRY(hipMemcpyAsync(dIdx.p, order.data(), count * 4, hipMemcpyHostToDevice, s));
+    // largest group's scratch
+    size_t maxw = 0;
+    for (size_t i = 0; i < count;) {
+        size_t j = i;
+        while (j < count && log_n[order[j]] == log_n[order[i]]) ++j;
+        maxw = std::max(maxw, (j - i) << log_n[order[i]]);
+        i = j;
+    }
+    DevBuf ga, gb, ra, rb;
+    rc = rc ? rc : ga.alloc(maxw * 16);
+    rc = rc ? rc : gb.alloc(maxw * 16);
+    rc = rc ? rc : ra.alloc(maxw * NPRIMES * 4);
+    rc = rc ? rc : rb.alloc(maxw * NPRIMES * 4);
+    if (rc) return rc;
+    for (size_t i = 0; i < count;) {
+        size_t j = i;
+        while (j < count && log_n[order[j]] == log_n[order[i]]) ++j;
+        const int lg = (int)log_n[order[i]];
+        const size_t cnt = j - i;
+        const uint32_t* idx = dIdx.as<uint32_t>() + i;
+        launch_gather_rows(dA.as<zq>(), dOff.as<uint64_t>(), idx, cnt, lg, ga.as<zq>(), s);
+        launch_gather_rows(dB.as<zq>(), dOff.as<uint64_t>(), idx, cnt, lg, gb.as<zq>(), s);
+        RowSrc sa{}, sb{};
+        sa.mode = sb.mode = ROW_PLAIN;
+        sa.a = ga.as<zq>();
+        sb.a = gb.as<zq>();
+        launch_ntt_fwd_rows(sa, cnt, lg, ra.as<uint32_t>(), ctx->tw, s);
+        launch_ntt_fwd_rows(sb, cnt, lg, rb.as<uint32_t>(), ctx->tw, s);
+        launch_pointwise(ra.as<uint32_t>(), rb.as<uint32_t>(), cnt, lg, s);
+        launch_ntt_inv(ra.as<uint32_t>(), cnt, lg, ctx->tw, s);
+        launch_crt_lift(ra.as<uint32_t>(), ga.as<zq>(), cnt, lg, s);
+        launch_scatter_rows(ga.as<zq>(), dOff.as<uint64_t>(), idx, cnt, lg, dO.as<zq>(), s);
+        i = j;
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dO.p, total * 16, hipMemcpyDeviceToHost));
+    return DMX_OK;
+}
