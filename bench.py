@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run a device copy of N MiB and exit")
+    ap.add_argument("--calibrate-after", type=int, default=0,
+                    help="after the job, run a device copy of N MiB (HBM-counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (jobs shard across GPUs "
@@ -88,16 +90,22 @@ def main():
     proof = None
     for _ in range(args.warmup):
         proof = dm.prove_verify_device(buf.ptr, FILE_BYTES, params)
-    ctx.kernel_timing(True)
-    ctx.kernel_timing()  # clear
     t0 = time.perf_counter()
     for _ in range(args.steps):
         out = dm.prove_verify_device(buf.ptr, FILE_BYTES, params)
     t1 = time.perf_counter()
-    k_ms, k_launches, k_work = ctx.kernel_timing()
-    ctx.kernel_timing(False)
     if proof is not None and out != proof:
         raise SystemExit("proof changed between runs")
+    # kernel roofline: the same job run eagerly with HIP events around every forward-transform
+    # launch on its own stream (graph replay carries no per-kernel events)
+    ctx.kernel_timing(True)
+    ctx.kernel_timing()  # clear
+    timing_steps = max(1, min(args.steps, 3))
+    for _ in range(timing_steps):
+        if dm.prove_verify_device(buf.ptr, FILE_BYTES, params) != out:
+            raise SystemExit("eager and graph runs disagree")
+    k_ms, k_launches, k_work = ctx.kernel_timing()
+    ctx.kernel_timing(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = FILE_BYTES / (1 << 20) * args.steps / (t1 - t0)
     per_launch_ms = k_ms / max(k_launches, 1)
@@ -121,12 +129,19 @@ def main():
         "roofline": {"bound": "valu", "kernel": "k_ntt_fwd_rows", "achieved": round(achieved, 3),
                      "peak": round(PEAK_LANE_CYCLES / 1e12, 3), "unit": "T lane-cycles/s",
                      "frac": round(achieved / (PEAK_LANE_CYCLES / 1e12), 4), "traffic": load_traffic(),
-                     "launches_per_step": k_launches // max(args.steps, 1),
-                     "kernel_ms_per_step": round(k_ms / args.steps, 3), "work_weights": WORK_WEIGHTS},
+                     "launches_per_step": k_launches // timing_steps,
+                     "kernel_ms_per_step": round(k_ms / timing_steps, 3), "work_weights": WORK_WEIGHTS,
+                     "timing": f"HIP events around each launch over {timing_steps} eager steps after the timed region"},
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(data, out, threads)
+    if args.calibrate_after:
+        n = args.calibrate_after << 20
+        a, b = dm.DeviceBuffer(ctx, n), dm.DeviceBuffer(ctx, n)
+        for _ in range(3):
+            b.copy_from(a)
+        line["calibration_copy_bytes"] = n
     print(json.dumps(line), flush=True)
 
 

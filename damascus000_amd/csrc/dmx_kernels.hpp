@@ -25,6 +25,9 @@ struct NttTables {
 struct CrtConsts {
     uint32_t mod_prefix[NPRIMES][NPRIMES];  // (p_0..p_{j-1}) mod p_i, j < i
     uint32_t inv[NPRIMES];                  // ((p_0..p_{i-1}) mod p_i)^-1 mod p_i
+    uint32_t pre_m[NPRIMES][NPRIMES];       // mod_prefix as mulc constants (w*R mod p_i)
+    uint32_t pre_mp[NPRIMES][NPRIMES];      //   and (w*R*p_i^-1 mod 2^32)
+    uint32_t inv_m[NPRIMES], inv_mp[NPRIMES];
     uint32_t half_digits[NPRIMES];          // mixed-radix digits of floor(P/2)
     zq prefix_q[NPRIMES];                   // (p_0..p_{j-1}) mod q
     zq P_q;                                 // P mod q
@@ -76,7 +79,9 @@ void launch_crt_lift(const uint32_t* res, zq* out, size_t slices, int log_m, hip
 // out = even(c) + sinv*l + s*r    (POLY; c has 2m coefficients per coordinate)
 void launch_com_update(const zq* c, const zq* l, const zq* r, const zq* scal, int poly, int log_m,
                        zq* out, hipStream_t s);
-// plain 16-byte copy (HBM calibration)
+// fill n 32-bit words (graph-safe replacement for memset)
+void launch_fill32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+// plain 16-byte copy (HBM calibration, graph-safe device copies)
 void launch_copy16(void* dst, const void* src, size_t n16, hipStream_t s);
 // out = a + b over n Z_q coefficients
 void launch_zq_add(const zq* a, const zq* b, zq* out, size_t n, hipStream_t s);

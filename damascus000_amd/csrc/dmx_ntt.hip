@@ -337,12 +337,12 @@ __global__ void __launch_bounds__(256) k_crt_lift(const uint32_t* __restrict__ r
     v[0] = r[0];
 #pragma unroll
     for (int i = 1; i < NPRIMES; ++i) {
-        const uint64_t pi = c_pc[i].p;
-        uint64_t acc = v[0] % pi;
+        const uint32_t pi = c_pc[i].p;
+        // acc = (v_0 + v_1 p_0 + ... + v_{i-1} p_0..p_{i-2}) mod p_i; mulc reduces any 32-bit v_j
+        uint32_t acc = 0;
 #pragma unroll
-        for (int j = 1; j < i; ++j) acc = (acc + (uint64_t)v[j] * c_crt.mod_prefix[i][j]) % pi;
-        const uint64_t delta = (r[i] + pi - acc) % pi;
-        v[i] = (uint32_t)((delta * c_crt.inv[i]) % pi);
+        for (int j = 0; j < i; ++j) acc = addp(acc, mulc(v[j], c_crt.pre_m[i][j], c_crt.pre_mp[i][j], pi), pi);
+        v[i] = mulc(subp(r[i], acc, pi), c_crt.inv_m[i], c_crt.inv_mp[i], pi);
     }
     // centred comparison on digits, most significant first
     int gt = 0;
@@ -431,6 +431,16 @@ void launch_com_update(const zq* c, const zq* l, const zq* r, const zq* scal, in
     const size_t n = (size_t)K << log_m;
     hipLaunchKernelGGL(k_com_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c, l, r,
                        scal, poly, log_m, out);
+}
+
+__global__ void k_fill32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
 }
 
 __global__ void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {

@@ -461,6 +461,15 @@ static void build_constants(dmx_ctx* c, std::vector<uint32_t>& host_tables) {
         }
         cr.mod_prefix[i][i] = (uint32_t)pref;
         cr.inv[i] = i ? (uint32_t)host_powmod(pref, pi - 2, pi) : 1;
+        const uint32_t pinv = host_inv32((uint32_t)pi);
+        for (int j = 0; j < i; ++j) {
+            const uint32_t wm = (uint32_t)(((uint64_t)cr.mod_prefix[i][j] << 32) % pi);
+            cr.pre_m[i][j] = wm;
+            cr.pre_mp[i][j] = wm * pinv;
+        }
+        const uint32_t im = (uint32_t)(((uint64_t)cr.inv[i] << 32) % pi);
+        cr.inv_m[i] = im;
+        cr.inv_mp[i] = im * pinv;
     }
     big P{1};
     zq pq = zq_from_u64(1);
@@ -569,6 +578,7 @@ struct dmx_prover {
     size_t N = 0;
     size_t round = 0;
     bool poisoned = false;
+    bool capturing = false;  // inside hipGraph capture: no timing events
     DevBuf file, W_a, W_b, G_a, G_b, What_a, What_b, Ghat_a, Ghat_b, res, lifted, C, Ct, com0, recs,
         scal, flags;
     std::vector<size_t> rec_off;  // offset (coefficients) of round j's record block
@@ -628,8 +638,7 @@ static int prover_initialize(dmx_prover* p) {
     const size_t N = p->N;
     const int d = p->d;
     const NttTables& tw = p->ctx->tw;
-    HIP_TRY(hipMemsetAsync(p->flags.p, 0, 256, s));
-    HIP_TRY(hipMemcpyAsync(p->tr.seeds.p, p->params.epoch_seed, 32, hipMemcpyHostToDevice, s));
+    launch_fill32(p->flags.as<uint32_t>(), 64, 0, s);
     device_hash(p->d_file, p->orig_len, IVW, 0, p->tr.cvs.as<uint32_t>(), p->tr.cv_words,
                 reinterpret_cast<uint32_t*>(p->tr.file_id()), nullptr, p->flags.as<uint32_t>(), s);
     launch_expand(p->d_file, p->orig_len, p->W_a.as<zq>(), N * N, s);
@@ -648,7 +657,7 @@ static int prover_initialize(dmx_prover* p) {
     launch_ntt_inv(p->res_at(0), 3 * K, d, tw, s);
     launch_crt_lift(p->res_at(0), p->C.as<zq>(), K, d, s);
     launch_crt_lift(p->res_at(slot), p->rec_lvec(0), 2 * K, d, s);  // L_vec, R_vec adjacent
-    HIP_TRY(hipMemcpyAsync(p->com0.p, p->C.p, (size_t)K * N * 16, hipMemcpyDeviceToDevice, s));
+    launch_copy16(p->com0.p, p->C.p, (size_t)K * N, s);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
@@ -660,7 +669,7 @@ static int prover_round(dmx_prover* p, size_t j) {
     const size_t m = p->m_at(j), h = m / 2;
     const int lm = p->d - (int)j;
     uint32_t* flags = p->flags.as<uint32_t>();
-    HIP_TRY(hipEventRecord(p->ev[0], s));
+    if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[0], s));
     // ---- vector stage ----
     p->tr.challenge("vec", j, p->C.as<zq>(), m, p->rec_lvec(j), m, p->rec_rvec(j), m, p->sc(4 * j), flags, s);
     launch_com_update(p->C.as<zq>(), p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j), 0, lm, p->Ct.as<zq>(), s);
@@ -687,7 +696,7 @@ static int prover_round(dmx_prover* p, size_t j) {
     launch_crt_lift(p->res_at(0), p->lifted.as<zq>(), K, lm, s);
     launch_crt_lift(p->res_at(s0), p->rec_lpoly(j), 2 * K, lm - 1, s);
     launch_com_check(p->lifted.as<zq>(), p->Ct.as<zq>(), (size_t)K * m, flags, vec_bit(j), s);
-    HIP_TRY(hipEventRecord(p->ev[1], s));
+    if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[1], s));
     // ---- poly stage ----
     p->tr.challenge("poly", j, p->Ct.as<zq>(), m, p->rec_lpoly(j), h, p->rec_rpoly(j), h, p->sc(4 * j + 2), flags, s);
     launch_com_update(p->Ct.as<zq>(), p->rec_lpoly(j), p->rec_rpoly(j), p->sc(4 * j + 2), 1, lm - 1,
@@ -713,7 +722,7 @@ static int prover_round(dmx_prover* p, size_t j) {
     launch_crt_lift(p->res_at(0), p->lifted.as<zq>(), K, lm - 1, s);
     if (next) launch_crt_lift(p->res_at(s1), p->rec_lvec(j + 1), 2 * K, lm - 1, s);
     launch_com_check(p->lifted.as<zq>(), p->C.as<zq>(), (size_t)K * h, flags, poly_bit(j), s);
-    HIP_TRY(hipEventRecord(p->ev[2], s));
+    if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[2], s));
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
@@ -742,12 +751,32 @@ static int encode_record(dmx_prover* p, size_t j, uint8_t* out, size_t cap, size
     return DMX_OK;
 }
 
+// every round's record with a single device-to-host copy (the blocks are contiguous)
+static int encode_records(dmx_prover* p, uint8_t* out) {
+    const size_t total = p->rec_off[p->d];
+    std::vector<zq> host(total);
+    HIP_TRY(hipMemcpyAsync(host.data(), p->recs.p, total * 16, hipMemcpyDeviceToHost, p->s));
+    HIP_TRY(hipStreamSynchronize(p->s));
+    uint8_t* q = out;
+    for (size_t j = 0; j < (size_t)p->d; ++j) {
+        const size_t m = p->m_at(j), h = m / 2;
+        const zq* b = host.data() + p->rec_off[j];
+        q = put_u64(q, j);
+        q = put_module(q, b, m);
+        q = put_module(q, b + (size_t)K * m, m);
+        q = put_module(q, b + (size_t)2 * K * m, h);
+        q = put_module(q, b + (size_t)2 * K * m + (size_t)K * h, h);
+    }
+    return DMX_OK;
+}
+
 static int prover_create(dmx_ctx* ctx, const uint8_t* file, const void* d_file, size_t len,
                          const dmx_system_params* params, const dmx_runtime_config* cfg, dmx_prover** out,
-                         bool sync = true) {
-    if (!ctx || !params || !out || (!file && !d_file && len)) return fail(DMX_ERR_ARG, "null argument");
+                         bool sync = true, size_t own_len = 0) {
+    if (!ctx || !params || !out || (!file && !d_file && len && !own_len)) return fail(DMX_ERR_ARG, "null argument");
     if (cfg && !cfg->gpu_enabled) return fail(DMX_ERR_NO_DEVICE, "gpu_enabled = false");
     if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
+    if (own_len) len = own_len;
     const int d = depth_for(len);
     const size_t N = (size_t)1 << d;
     const uint64_t limit = cfg ? cfg->max_preprocess_bytes : (512ull << 20);
@@ -769,6 +798,14 @@ static int prover_create(dmx_ctx* ctx, const uint8_t* file, const void* d_file, 
     gseed_words(p->g_seed, p->gw);
     int rc = prover_alloc(p.get());
     if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(p->tr.seeds.p, params->epoch_seed, 32, hipMemcpyHostToDevice, p->s));
+    if (own_len) {  // a fused job: the file arrives later, the length is fixed now
+        rc = p->file.alloc(std::max<size_t>(len, 16));
+        if (rc) return rc;
+        p->d_file = p->file.as<uint8_t>();
+        *out = p.release();
+        return DMX_OK;
+    }
     if (d_file) {
         p->d_file = static_cast<const uint8_t*>(d_file);
     } else {
@@ -877,6 +914,8 @@ extern "C" int dmx_prover_final_opening(const dmx_prover* p, uint64_t out[2]) {
 // ------------------------------------------------------------------------------------------
 // verifier (src/protocol/verifier.rs:21-134)
 // ------------------------------------------------------------------------------------------
+static int log2_exact(size_t n);
+
 struct dmx_verifier {
     dmx_ctx* ctx = nullptr;
     hipStream_t s = nullptr;
@@ -909,6 +948,21 @@ static int verifier_alloc(dmx_verifier* v) {
     return rc;
 }
 
+// Device part of DamascusVerifier::new: flags, generator derivation from g_0_seed
+// (verifier.rs:31-33) and, for the fused job, the device-resident statement values copied in
+// once `wait` (the prover's initialisation) has completed. Kernels only: graph-capture safe.
+static int verifier_enqueue_setup(dmx_verifier* v, const uint8_t* d_file_id, const zq* d_com0, hipEvent_t wait) {
+    launch_fill32(v->flags.as<uint32_t>(), 64, 0, v->s);
+    uint32_t gw[8];
+    gseed_words(v->g_seed, gw);
+    launch_gen_generators(gw, v->rows * K, log2_exact(v->glen), v->G0.as<zq>(), v->s);
+    if (wait) HIP_TRY(hipStreamWaitEvent(v->s, wait, 0));
+    if (d_file_id) launch_copy16(v->tr.file_id(), d_file_id, 2, v->s);
+    if (d_com0) launch_copy16(v->C.p, d_com0, (size_t)K * v->clen, v->s);
+    HIP_TRY(hipGetLastError());
+    return DMX_OK;
+}
+
 // DamascusVerifier::new (verifier.rs:22-43): generators re-derived from the statement's
 // g_0_seed on the device. com0 / file_id come from host memory, or device memory when
 // `on_device` (the fused prove+verify path).
@@ -933,18 +987,13 @@ static int verifier_make(dmx_ctx* ctx, const dmx_system_params* params, const ui
     HIP_TRY(hipStreamCreate(&v->s));
     int rc = verifier_alloc(v.get());
     if (rc) return rc;
-    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     HIP_TRY(hipMemcpyAsync(v->tr.seeds.p, params->epoch_seed, 32, hipMemcpyHostToDevice, v->s));
-    HIP_TRY(hipMemsetAsync(v->flags.p, 0, 256, v->s));
-    uint32_t gw[8];
-    gseed_words(g_seed, gw);
-    int lg = 0;
-    while (((size_t)1 << lg) < v->glen) ++lg;
-    launch_gen_generators(gw, v->rows * K, lg, v->G0.as<zq>(), v->s);  // verifier.rs:31-33
-    if (wait_before_copy) HIP_TRY(hipStreamWaitEvent(v->s, wait_before_copy, 0));
-    HIP_TRY(hipMemcpyAsync(v->tr.file_id(), file_id, 32, kind, v->s));
-    HIP_TRY(hipMemcpyAsync(v->C.p, com0, (size_t)K * v->clen * 16, kind, v->s));
-    HIP_TRY(hipGetLastError());
+    if (!on_device) {
+        HIP_TRY(hipMemcpyAsync(v->tr.file_id(), file_id, 32, hipMemcpyHostToDevice, v->s));
+        HIP_TRY(hipMemcpyAsync(v->C.p, com0, (size_t)K * v->clen * 16, hipMemcpyHostToDevice, v->s));
+        rc = verifier_enqueue_setup(v.get(), nullptr, nullptr, nullptr);
+        if (rc) return rc;
+    }
     *out = v.release();
     return DMX_OK;
 }
@@ -988,7 +1037,7 @@ extern "C" dmx_verifier* dmx_verifier_clone(const dmx_verifier* v) {
     if (hipMemcpyAsync(c->C.p, v->C.p, (size_t)K * v->clen * 16, hipMemcpyDeviceToDevice, c->s) != hipSuccess)
         return nullptr;
     if (hipMemcpyAsync(c->tr.seeds.p, v->tr.seeds.p, 64, hipMemcpyDeviceToDevice, c->s) != hipSuccess) return nullptr;
-    (void)hipMemsetAsync(c->flags.p, 0, 256, c->s);
+    launch_fill32(c->flags.as<uint32_t>(), 64, 0, c->s);
     if (hipStreamSynchronize(c->s) != hipSuccess) return nullptr;
     return c.release();
 }
@@ -1058,7 +1107,7 @@ static int verifier_round_launch(dmx_verifier* v, const zq* d_l, const zq* d_r, 
         nclen = n / 2;
         nglen = v->glen / 2;
     } else if (vec) {
-        HIP_TRY(hipMemcpyAsync(v->C.p, v->Ct.p, (size_t)K * n * 16, hipMemcpyDeviceToDevice, s));
+        launch_copy16(v->C.p, v->Ct.p, (size_t)K * n, s);
     }
     HIP_TRY(hipGetLastError());
     v->rows = rows;
@@ -1126,7 +1175,7 @@ static int verifier_final_launch(dmx_verifier* v, const zq* d_opening) {
     if (v->round != v->rounds) return fail(DMX_ERR_NOT_FINAL, "cannot verify final opening before all rounds replay");
     if (v->rows != 1) return fail(DMX_ERR_NOT_FINAL, "terminal generator family must have length one");
     if (v->glen != v->clen) return fail(DMX_ERR_OPENING, "final opening mismatch");
-    HIP_TRY(hipMemsetAsync(v->flags.p, 0, 4, v->s));
+    launch_fill32(v->flags.as<uint32_t>(), 1, 0, v->s);
     launch_final_check(v->G(), v->C.as<zq>(), d_opening, (size_t)K * v->glen, v->flags.as<uint32_t>(), 1, v->s);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
@@ -1198,52 +1247,165 @@ extern "C" int dmx_verify(dmx_ctx* ctx, const dmx_system_params* params, const u
     return rc;
 }
 
-// prove + replay from a device-resident file. The verifier runs on its own stream and
-// consumes the prover's device-resident records; its generator derivation and folds overlap
-// the prover's next round. Both parties recompute every challenge themselves.
-extern "C" int dmx_prove_verify_device(dmx_ctx* ctx, const void* d_file, size_t len, const dmx_system_params* params,
-                                       uint8_t* st_out, size_t st_cap, size_t* st_len, uint8_t* rec_out,
-                                       size_t rec_cap, size_t* rec_len, uint64_t opening_out[2]) {
-    if (!ctx || !params || (!d_file && len)) return fail(DMX_ERR_ARG, "null argument");
-    dmx_runtime_config cfg{};
-    cfg.max_preprocess_bytes = UINT64_MAX;  // examples/full_flow.rs:29
-    cfg.ntt_enabled = cfg.parallel_enabled = cfg.gpu_enabled = 1;
-    dmx_prover* p = nullptr;
-    int rc = prover_create(ctx, nullptr, d_file, len, params, &cfg, &p, /*sync=*/false);
+// ------------------------------------------------------------------------------------------
+// fused device job: prove + replay of a device-resident file.
+// The prover and the verifier each run on their own stream; the verifier consumes the prover's
+// device-resident records, so its generator derivation and folds overlap the prover's next
+// round, and both parties recompute every challenge themselves. Workspaces are cached per
+// (depth, length, seeds) in the context and the whole two-stream job is captured once into a
+// hipGraph that later jobs replay: one launch instead of ~400.
+// ------------------------------------------------------------------------------------------
+struct FusedJob {
+    int d = 0;
+    size_t len = 0;
+    dmx_system_params params{};
+    std::unique_ptr<dmx_prover> p;
+    std::unique_ptr<dmx_verifier> v;
+    std::vector<hipEvent_t> evs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~FusedJob() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto e : evs) (void)hipEventDestroy(e);
+    }
+};
+
+static std::mutex g_jobs_mu;
+static std::vector<std::pair<dmx_ctx*, std::unique_ptr<FusedJob>>> g_jobs;
+
+// all device work of one job, enqueued on p->s (forking to v->s); no host syncs
+static int fused_enqueue(FusedJob* J) {
+    dmx_prover* p = J->p.get();
+    dmx_verifier* v = J->v.get();
+    // reset host-side bookkeeping of both parties to the start of a job
+    p->round = 0;
+    v->round = 0;
+    v->rows = p->N;
+    v->glen = v->clen = p->N;
+    v->gcur = 0;
+    int rc = prover_initialize(p);
     if (rc) return rc;
-    std::unique_ptr<dmx_prover> pg(p);
-    hipEvent_t ready;
-    HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-    struct EvGuard {
-        hipEvent_t e;
-        ~EvGuard() { (void)hipEventDestroy(e); }
-    } eg{ready};
-    HIP_TRY(hipEventRecord(ready, p->s));
-    dmx_verifier* v = nullptr;
-    rc = verifier_make(ctx, params, p->tr.file_id(), len, (uint64_t)p->d, p->com0.as<zq>(), p->N, p->g_seed, true, &v,
-                       ready);
+    HIP_TRY(hipEventRecord(J->evs[0], p->s));
+    HIP_TRY(hipStreamWaitEvent(v->s, J->evs[0], 0));  // fork the verifier stream
+    rc = verifier_enqueue_setup(v, p->tr.file_id(), p->com0.as<zq>(), nullptr);
     if (rc) return rc;
-    std::unique_ptr<dmx_verifier> vg(v);
     for (size_t j = 0; j < (size_t)p->d; ++j) {
         rc = prover_round(p, j);
         if (rc) return rc;
-        HIP_TRY(hipEventRecord(ready, p->s));
-        HIP_TRY(hipStreamWaitEvent(v->s, ready, 0));
+        HIP_TRY(hipEventRecord(J->evs[1 + j], p->s));
+        HIP_TRY(hipStreamWaitEvent(v->s, J->evs[1 + j], 0));
         rc = verifier_round_launch(v, p->rec_lvec(j), p->rec_rvec(j), p->rec_lpoly(j), p->rec_rpoly(j));
         if (rc) return rc;
     }
     p->round = (size_t)p->d;
     rc = verifier_final_launch(v, p->W_a.as<zq>());  // m* = message[0].coeffs[0] (prover.rs:133-138)
     if (rc) return rc;
+    HIP_TRY(hipEventRecord(J->evs[J->evs.size() - 1], v->s));
+    HIP_TRY(hipStreamWaitEvent(p->s, J->evs[J->evs.size() - 1], 0));  // join
+    return DMX_OK;
+}
+
+static int fused_job_get(dmx_ctx* ctx, size_t len, const dmx_system_params* params, FusedJob** out) {
+    const int d = depth_for(len);
+    std::lock_guard<std::mutex> g(g_jobs_mu);
+    for (auto& e : g_jobs) {
+        FusedJob* J = e.second.get();
+        if (e.first == ctx && J->d == d && J->len == len && !memcmp(&J->params, params, sizeof *params)) {
+            *out = J;
+            return DMX_OK;
+        }
+    }
+    // one cached job per (context, depth): replace an older one of the same depth
+    for (size_t i = 0; i < g_jobs.size(); ++i)
+        if (g_jobs[i].first == ctx && g_jobs[i].second->d == d) {
+            g_jobs.erase(g_jobs.begin() + i);
+            break;
+        }
+    std::unique_ptr<FusedJob> J(new FusedJob);
+    J->d = d;
+    J->len = len;
+    J->params = *params;
+    dmx_runtime_config cfg{};
+    cfg.max_preprocess_bytes = UINT64_MAX;  // examples/full_flow.rs:29
+    cfg.ntt_enabled = cfg.parallel_enabled = cfg.gpu_enabled = 1;
+    dmx_prover* p = nullptr;
+    // the job owns its file buffer; callers' files are copied in before each replay
+    int rc = prover_create(ctx, nullptr, nullptr, 0, params, &cfg, &p, /*sync=*/false, len);
+    if (rc) return rc;
+    J->p.reset(p);
+    dmx_verifier* v = nullptr;
+    rc = verifier_make(ctx, params, nullptr, len, (uint64_t)d, nullptr, p->N, p->g_seed, true, &v);
+    if (rc) return rc;
+    J->v.reset(v);
+    J->evs.resize((size_t)d + 2);
+    for (auto& e : J->evs) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_TRY(hipStreamSynchronize(p->s));
-    uint32_t f = 0;
-    HIP_TRY(hipMemcpy(&f, p->flags.p, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipStreamSynchronize(v->s));
+    g_jobs.emplace_back(ctx, std::move(J));
+    *out = g_jobs.back().second.get();
+    return DMX_OK;
+}
+
+static bool graphs_enabled() {
+    const char* e = getenv("DAMASCUS_GRAPH");
+    return !(e && strcmp(e, "0") == 0);
+}
+
+static int fused_run(FusedJob* J, bool timing) {
+    dmx_prover* p = J->p.get();
+    if (!graphs_enabled() || timing) {  // eager: kernel timing needs per-launch events
+        return fused_enqueue(J);
+    }
+    if (!J->exec) {
+        p->capturing = true;
+        HIP_TRY(hipStreamBeginCapture(p->s, hipStreamCaptureModeThreadLocal));
+        int rc = fused_enqueue(J);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(p->s, &graph);
+        p->capturing = false;
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(DMX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        J->graph = graph;
+        HIP_TRY(hipGraphInstantiate(&J->exec, J->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(J->exec, p->s));
+    return DMX_OK;
+}
+
+extern "C" int dmx_prove_verify_device(dmx_ctx* ctx, const void* d_file, size_t len, const dmx_system_params* params,
+                                       uint8_t* st_out, size_t st_cap, size_t* st_len, uint8_t* rec_out,
+                                       size_t rec_cap, size_t* rec_len, uint64_t opening_out[2]) {
+    if (!ctx || !params || (!d_file && len)) return fail(DMX_ERR_ARG, "null argument");
+    if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
+    if (depth_for(len) > 12) return fail(DMX_ERR_ARG, "depth %d exceeds the supported 12", depth_for(len));
+    HIP_TRY(hipSetDevice(ctx->device));
+    FusedJob* J = nullptr;
+    int rc = fused_job_get(ctx, len, params, &J);
+    if (rc) return rc;
+    dmx_prover* p = J->p.get();
+    dmx_verifier* v = J->v.get();
+    if (len) HIP_TRY(hipMemcpyAsync(p->file.p, d_file, len, hipMemcpyDeviceToDevice, p->s));
+    bool timing;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        timing = ctx->timing;
+    }
+    rc = fused_run(J, timing);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(p->s));
+    uint32_t f[2] = {0, 0};
+    HIP_TRY(hipMemcpy(&f[0], p->flags.p, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&f[1], v->flags.p, 4, hipMemcpyDeviceToHost));
     for (size_t j = 0; j < (size_t)p->d; ++j) {
-        rc = check_round_flags(p, j, f);
+        rc = check_round_flags(p, j, f[0]);
         if (rc) return rc;
     }
-    rc = verifier_final_result(v);
-    if (rc) return rc;
+    if (f[1] & 1u) return fail(DMX_ERR_OPENING, "final opening mismatch");
+    if (f[1]) return fail(DMX_ERR_HIP, "challenge resampling exhausted");
     if (st_len || st_out) {
         rc = dmx_prover_statement(p, st_out, st_cap, st_len);
         if (rc) return rc;
@@ -1253,18 +1415,11 @@ extern "C" int dmx_prove_verify_device(dmx_ctx* ctx, const void* d_file, size_t 
         if (rec_len) *rec_len = need;
         if (rec_out) {
             if (rec_cap < need) return fail(DMX_ERR_BUFFER, "records buffer too small");
-            size_t pos = 0;
-            for (size_t j = 0; j < (size_t)p->d; ++j) {
-                size_t got = 0;
-                rc = encode_record(p, j, rec_out + pos, rec_cap - pos, &got);
-                if (rc) return rc;
-                pos += got;
-            }
+            rc = encode_records(p, rec_out);
+            if (rc) return rc;
         }
     }
-    if (opening_out) {
-        HIP_TRY(hipMemcpy(opening_out, p->W_a.p, 16, hipMemcpyDeviceToHost));
-    }
+    if (opening_out) HIP_TRY(hipMemcpy(opening_out, p->W_a.p, 16, hipMemcpyDeviceToHost));
     return DMX_OK;
 }
 

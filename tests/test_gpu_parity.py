@@ -289,6 +289,30 @@ def test_prove_verify_device_matches_host_api():
     assert dm.prove_verify_device(buf.ptr, len(data), want_proof=False) is None
 
 
+def test_graph_replays_match_eager_runs():
+    """the fused job is captured into a hipGraph on first use and replayed afterwards: every
+    replay, with eager work and synchronisations in between, equals the eager result"""
+    import os
+
+    data = np.random.default_rng(123).integers(0, 256, size=13 * 5000, dtype=np.uint8).tobytes()  # d = 7
+    ctx = dm.Context.default()
+    buf = ctx.upload(data)
+    eager = dm.prove(data)
+    os.environ["DAMASCUS_GRAPH"] = "0"
+    try:
+        assert dm.prove_verify_device(buf.ptr, len(data)) == eager
+    finally:
+        os.environ.pop("DAMASCUS_GRAPH")
+    for rep in range(4):
+        say("graph replay", rep)
+        assert dm.prove_verify_device(buf.ptr, len(data)) == eager, rep
+        other = W.pattern_bytes(8192, rep)  # eager work between replays
+        st, recs, op = dm.prove(other)
+        dm.verify(st, recs, op)
+    buf2 = ctx.upload(bytes(reversed(data)))  # same shape, different file: replays with new bytes
+    assert dm.prove_verify_device(buf2.ptr, len(data)) == dm.prove(bytes(reversed(data)))
+
+
 def test_kernel_timing_counts_forward_transforms():
     ctx = dm.Context.default()
     data = W.pattern_bytes(8192, 3)
