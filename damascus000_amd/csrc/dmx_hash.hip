@@ -16,6 +16,72 @@ struct KeyW {
 };
 
 // ---------------------------------------------------------------------------------------
+// Quad-cooperative compression: four lanes (q = lane & 3) hold the four columns of the 4x4
+// BLAKE3 state, lane q keeping (s[q], s[4+q], s[8+q], s[12+q]). The column step is one G per
+// lane; the diagonal step rotates b, c, d within the quad by 1, 2, 3 lanes (DPP quad_perm),
+// applies G and rotates back. Every lane holds the 16 message words and picks its pair per
+// round with two lane-mask selects. A compression costs ~300 VALU instructions per lane
+// instead of ~680 on one lane, which is what bounds a 16-block chunk's latency.
+// All four lanes of a quad must be active together.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t qrot1(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x39, 0xF, 0xF, false); }
+__device__ __forceinline__ uint32_t qrot2(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false); }
+__device__ __forceinline__ uint32_t qrot3(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x93, 0xF, 0xF, false); }
+
+struct QuadSel {
+    bool b0, b1;  // bits of the lane's column index
+    __device__ __forceinline__ uint32_t operator()(uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3) const {
+        const uint32_t t0 = b0 ? v1 : v0, t1 = b0 ? v3 : v2;
+        return b1 ? t1 : t0;
+    }
+};
+
+#define DMX_QG(mx, my)                 \
+    a = a + b + (mx);                  \
+    d = b3::rotr(d ^ a, 16);           \
+    c = c + d;                         \
+    b = b3::rotr(b ^ c, 12);           \
+    a = a + b + (my);                  \
+    d = b3::rotr(d ^ a, 8);            \
+    c = c + d;                         \
+    b = b3::rotr(b ^ c, 7);
+
+#define DMX_QROUND(s0, s1, s2, s3, s4, s5, s6, s7, s8, s9, s10, s11, s12, s13, s14, s15) \
+    {                                                                                     \
+        const uint32_t mx = sel(m[s0], m[s2], m[s4], m[s6]);                              \
+        const uint32_t my = sel(m[s1], m[s3], m[s5], m[s7]);                              \
+        DMX_QG(mx, my)                                                                    \
+    }                                                                                     \
+    b = qrot1(b);                                                                         \
+    c = qrot2(c);                                                                         \
+    d = qrot3(d);                                                                         \
+    {                                                                                     \
+        const uint32_t mx = sel(m[s8], m[s10], m[s12], m[s14]);                           \
+        const uint32_t my = sel(m[s9], m[s11], m[s13], m[s15]);                           \
+        DMX_QG(mx, my)                                                                    \
+    }                                                                                     \
+    b = qrot3(b);                                                                         \
+    c = qrot2(c);                                                                         \
+    d = qrot1(d);
+
+// (ca, cb) = lane q's chaining-value words (cv[q], cv[4+q]); returns them updated
+__device__ __forceinline__ void compress_quad(uint32_t& ca, uint32_t& cb, const uint32_t m[16], uint64_t counter,
+                                              uint32_t block_len, uint32_t flags, const QuadSel& sel) {
+    uint32_t a = ca, b = cb;
+    uint32_t c = sel(b3::iv(0), b3::iv(1), b3::iv(2), b3::iv(3));
+    uint32_t d = sel((uint32_t)counter, (uint32_t)(counter >> 32), block_len, flags);
+    DMX_QROUND(0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15)
+    DMX_QROUND(2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8)
+    DMX_QROUND(3, 4, 10, 12, 13, 2, 7, 14, 6, 5, 9, 0, 11, 15, 8, 1)
+    DMX_QROUND(10, 7, 12, 9, 14, 3, 13, 15, 4, 0, 11, 2, 5, 8, 1, 6)
+    DMX_QROUND(12, 13, 9, 11, 15, 10, 14, 8, 7, 2, 5, 3, 0, 1, 6, 4)
+    DMX_QROUND(9, 14, 11, 5, 8, 12, 15, 1, 13, 3, 0, 10, 2, 6, 4, 7)
+    DMX_QROUND(11, 15, 5, 0, 1, 9, 8, 6, 14, 10, 2, 12, 3, 4, 7, 13)
+    ca = a ^ c;
+    cb = b ^ d;
+}
+
+// ---------------------------------------------------------------------------------------
 // transcript message layout (transcript.rs:50-61, 77-90):
 //   epoch_seed[32] | u64 len(label) | label | u64 32 | file_id[32] | u64 0 | u64 round |
 //   3 x ( u64 8 | 8 x ( u64 len | len x 16-byte LE coefficients ) ) | u32 counter
@@ -104,12 +170,12 @@ __device__ void load_block(const uint8_t* msg, size_t len, size_t off, uint32_t 
     }
 }
 
-// CV of chunk `ci`; with root != 0 the last block carries ROOT and the full digest is returned
-__device__ void chunk_cv(const uint8_t* msg, size_t len, size_t ci, const uint32_t key[8],
-                         uint32_t flags, bool root, uint32_t out[8]) {
-    uint32_t cv[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) cv[i] = key[i];
+// CV of chunk `ci` computed by one quad; with `root` the last block carries ROOT, giving the
+// digest. Lane q returns (cv[q], cv[4+q]).
+__device__ void chunk_cv_quad(const uint8_t* msg, size_t len, size_t ci, const uint32_t key[8], uint32_t flags,
+                              bool root, const QuadSel& sel, uint32_t& ca, uint32_t& cb) {
+    ca = sel(key[0], key[1], key[2], key[3]);
+    cb = sel(key[4], key[5], key[6], key[7]);
     const size_t start = ci * b3::CHUNK_LEN;
     const size_t end = (start + b3::CHUNK_LEN < len) ? start + b3::CHUNK_LEN : len;
     const size_t nb = end > start ? (end - start + 63) / 64 : 1;
@@ -124,21 +190,21 @@ __device__ void chunk_cv(const uint8_t* msg, size_t len, size_t ci, const uint32
         }
         uint32_t fl = flags | (b == 0 ? b3::CHUNK_START : 0) | (b + 1 == nb ? b3::CHUNK_END : 0);
         if (root && b + 1 == nb) fl |= b3::ROOT;
-        b3::compress_cv(cv, m, ci, blen, fl, cv);
+        compress_quad(ca, cb, m, ci, blen, fl, sel);
     }
-#pragma unroll
-    for (int i = 0; i < 8; ++i) out[i] = cv[i];
 }
 
+// one quad per chunk, 16 chunks per 64-thread block; the guard is uniform within a quad
 __global__ void __launch_bounds__(64) k_b3_chunks(const uint8_t* __restrict__ msg, size_t len, KeyW key,
                                                   uint32_t flags, uint32_t* __restrict__ cvs, size_t n) {
-    const size_t ci = (size_t)blockIdx.x * 64 + threadIdx.x;
+    const uint32_t q = threadIdx.x & 3;
+    const size_t ci = (size_t)blockIdx.x * 16 + (threadIdx.x >> 2);
     if (ci >= n) return;
-    uint32_t cv[8];
-    chunk_cv(msg, len, ci, key.w, flags, false, cv);
-    uint4* o = reinterpret_cast<uint4*>(cvs + ci * 8);
-    o[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
-    o[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+    const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
+    uint32_t ca, cb;
+    chunk_cv_quad(msg, len, ci, key.w, flags, false, sel, ca, cb);
+    cvs[ci * 8 + q] = ca;
+    cvs[ci * 8 + 4 + q] = cb;
 }
 
 static inline size_t nchunks_of(size_t len) { return len == 0 ? 1 : (len + 1023) / 1024; }
@@ -148,52 +214,52 @@ void launch_b3_chunks(const uint8_t* msg, size_t len, const uint32_t key[8], uin
     KeyW k;
     for (int i = 0; i < 8; ++i) k.w[i] = key[i];
     const size_t n = nchunks_of(len);
-    hipLaunchKernelGGL(k_b3_chunks, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, msg, len, k, flags,
+    hipLaunchKernelGGL(k_b3_chunks, dim3((unsigned)((n + 15) / 16)), dim3(64), 0, s, msg, len, k, flags,
                        cvs, n);
 }
 
-// pairwise bottom-up merge of `n` <= 2 * blockDim.x CVs held in LDS; returns with the single
-// CV in sh[0..8); with `final_root` the last compression carries ROOT (n >= 2).
-// Every thread reaches every barrier: a thread owns at most one pair per level.
-__device__ void merge_lds(uint32_t* sh, uint32_t n, const uint32_t key[8], uint32_t flags, bool final_root) {
+// Pairwise bottom-up merge of the n CVs in `a` (LDS), one quad per parent node, ping-ponging
+// between `a` and `b`; returns the buffer holding the single result. With `final_root` the last
+// compression carries ROOT (n >= 2). Loop bounds are uniform, so every thread reaches every
+// barrier; the caller has synchronised after filling `a`.
+__device__ uint32_t* merge_lds(uint32_t* a, uint32_t* b, uint32_t n, const uint32_t key[8], uint32_t flags,
+                               bool final_root) {
+    const uint32_t q = threadIdx.x & 3, quad = threadIdx.x >> 2, nquads = blockDim.x >> 2;
+    const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
     while (n > 1) {
         const uint32_t pairs = n >> 1;
-        const bool last = (n == 2) && final_root;
-        const uint32_t i = threadIdx.x;
-        uint32_t o[8];
-        if (i < pairs) {
-            uint32_t l[8], r[8];
+        const uint32_t fl = flags | b3::PARENT | ((n == 2 && final_root) ? b3::ROOT : 0);
+        for (uint32_t base = 0; base < pairs; base += nquads) {
+            const uint32_t pr = base + quad;
+            if (pr < pairs) {
+                uint32_t m[16];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                l[j] = sh[(2 * i) * 8 + j];
-                r[j] = sh[(2 * i + 1) * 8 + j];
+                for (int i = 0; i < 16; ++i) m[i] = a[2 * pr * 8 + i];
+                uint32_t ca = sel(key[0], key[1], key[2], key[3]), cb = sel(key[4], key[5], key[6], key[7]);
+                compress_quad(ca, cb, m, 0, b3::BLOCK_LEN, fl, sel);
+                b[pr * 8 + q] = ca;
+                b[pr * 8 + 4 + q] = cb;
             }
-            b3::parent_cv(l, r, key, flags | (last ? b3::ROOT : 0), o);
         }
-        __syncthreads();  // all reads of this level happen before any write
-        if (i < pairs) {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) sh[i * 8 + j] = o[j];
-        }
+        if ((n & 1) && threadIdx.x < 8) b[pairs * 8 + threadIdx.x] = a[(n - 1) * 8 + threadIdx.x];
         __syncthreads();
-        if ((n & 1) && threadIdx.x == 0) {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) sh[pairs * 8 + j] = sh[(n - 1) * 8 + j];
-        }
-        __syncthreads();
+        uint32_t* t = a;
+        a = b;
+        b = t;
         n = pairs + (n & 1);
     }
+    return a;
 }
 
 __global__ void __launch_bounds__(1024) k_b3_group_merge(const uint32_t* __restrict__ cvs, size_t n, KeyW key,
                                                          uint32_t flags, uint32_t* __restrict__ out) {
-    __shared__ uint32_t sh[1024 * 8];
+    __shared__ uint32_t sa[1024 * 8], sb[512 * 8 + 8];
     const size_t base = (size_t)blockIdx.x * 1024;
     const uint32_t cnt = (uint32_t)((n - base) < 1024 ? (n - base) : 1024);
-    for (uint32_t i = threadIdx.x; i < cnt * 8; i += blockDim.x) sh[i] = cvs[base * 8 + i];
+    for (uint32_t i = threadIdx.x; i < cnt * 8; i += blockDim.x) sa[i] = cvs[base * 8 + i];
     __syncthreads();
-    merge_lds(sh, cnt, key.w, flags, false);
-    if (threadIdx.x < 8) out[blockIdx.x * 8 + threadIdx.x] = sh[threadIdx.x];
+    const uint32_t* r = merge_lds(sa, sb, cnt, key.w, flags, false);
+    if (threadIdx.x < 8) out[blockIdx.x * 8 + threadIdx.x] = r[threadIdx.x];
 }
 
 void launch_b3_group_merge(const uint32_t* cvs, size_t n, const uint32_t key[8], uint32_t flags,
@@ -266,34 +332,151 @@ __device__ zq zq_inv_binary(zq a) {
     return is_one(u) ? x1 : x2;
 }
 
+// Fermat inverse x^(q-2) in 28-bit limbs. q - 2 = 2^112 - 77 has bits 8..111 set and low
+// byte 0xB3, so x^(2^k - 1) for k = 1, 2, 4, ..., 64, 96, 104 by doubling, then the low byte:
+// 111 squarings and 16 multiplications. With 28-bit limbs every product column is a sum of at
+// most four 56-bit terms, so it accumulates in 64 bits with no carry handling.
+struct z28 {
+    uint32_t l[4];  // limbs < 2^28 (value < 2^112, not necessarily reduced)
+};
+constexpr uint32_t M28 = (1u << 28) - 1;
+
+__device__ __forceinline__ z28 z28_from(const zq& x) {
+    z28 r;
+    r.l[0] = x.w[0] & M28;
+    r.l[1] = ((x.w[0] >> 28) | (x.w[1] << 4)) & M28;
+    r.l[2] = ((x.w[1] >> 24) | (x.w[2] << 8)) & M28;
+    r.l[3] = ((x.w[2] >> 20) | (x.w[3] << 12)) & M28;
+    return r;
+}
+
+__device__ __forceinline__ zq z28_to(const z28& a) {  // canonical
+    zq r;
+    r.w[0] = a.l[0] | (a.l[1] << 28);
+    r.w[1] = (a.l[1] >> 4) | (a.l[2] << 24);
+    r.w[2] = (a.l[2] >> 8) | (a.l[3] << 20);
+    r.w[3] = a.l[3] >> 12;
+    return zq_ge_q(r) ? zq_sub_q(r) : r;
+}
+
+__device__ __forceinline__ uint64_t mul32(uint32_t a, uint32_t b) { return (uint64_t)a * b; }
+
+// column sums c0..c6 (each < 2^58) -> 28-bit limbs of an equivalent value < 2^112
+__device__ __forceinline__ z28 z28_reduce(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3, uint64_t c4,
+                                          uint64_t c5, uint64_t c6) {
+    uint32_t r[8];
+    uint64_t t = c0;
+    r[0] = (uint32_t)t & M28;
+    t = c1 + (t >> 28);
+    r[1] = (uint32_t)t & M28;
+    t = c2 + (t >> 28);
+    r[2] = (uint32_t)t & M28;
+    t = c3 + (t >> 28);
+    r[3] = (uint32_t)t & M28;
+    t = c4 + (t >> 28);
+    r[4] = (uint32_t)t & M28;
+    t = c5 + (t >> 28);
+    r[5] = (uint32_t)t & M28;
+    t = c6 + (t >> 28);
+    r[6] = (uint32_t)t & M28;
+    r[7] = (uint32_t)(t >> 28);
+    // 2^112 = 75 (mod q): fold limbs 4..7 onto 0..3, then the overflow above 2^112 twice more
+    t = r[0] + mul32(75, r[4]);
+    uint32_t v0 = (uint32_t)t & M28;
+    t = r[1] + mul32(75, r[5]) + (t >> 28);
+    uint32_t v1 = (uint32_t)t & M28;
+    t = r[2] + mul32(75, r[6]) + (t >> 28);
+    uint32_t v2 = (uint32_t)t & M28;
+    t = r[3] + mul32(75, r[7]) + (t >> 28);
+    uint32_t v3 = (uint32_t)t & M28;
+    uint32_t o = (uint32_t)(t >> 28);
+    for (int k = 0; k < 2; ++k) {
+        uint32_t u = v0 + 75 * o;
+        v0 = u & M28;
+        u = v1 + (u >> 28);
+        v1 = u & M28;
+        u = v2 + (u >> 28);
+        v2 = u & M28;
+        u = v3 + (u >> 28);
+        v3 = u & M28;
+        o = u >> 28;
+    }
+    return z28{{v0, v1, v2, v3}};
+}
+
+__device__ __forceinline__ z28 z28_mul(const z28& a, const z28& b) {
+    const uint32_t *x = a.l, *y = b.l;
+    return z28_reduce(mul32(x[0], y[0]), mul32(x[0], y[1]) + mul32(x[1], y[0]),
+                      mul32(x[0], y[2]) + mul32(x[1], y[1]) + mul32(x[2], y[0]),
+                      mul32(x[0], y[3]) + mul32(x[1], y[2]) + mul32(x[2], y[1]) + mul32(x[3], y[0]),
+                      mul32(x[1], y[3]) + mul32(x[2], y[2]) + mul32(x[3], y[1]),
+                      mul32(x[2], y[3]) + mul32(x[3], y[2]), mul32(x[3], y[3]));
+}
+
+__device__ __forceinline__ z28 z28_sqr(const z28& a) {
+    const uint32_t* x = a.l;
+    const uint32_t d0 = 2 * x[0], d1 = 2 * x[1], d2 = 2 * x[2];
+    return z28_reduce(mul32(x[0], x[0]), mul32(d0, x[1]), mul32(d0, x[2]) + mul32(x[1], x[1]),
+                      mul32(d0, x[3]) + mul32(d1, x[2]), mul32(d1, x[3]) + mul32(x[2], x[2]), mul32(d2, x[3]),
+                      mul32(x[3], x[3]));
+}
+
+__device__ __forceinline__ z28 z28_sqn(z28 a, int n) {
+    for (int i = 0; i < n; ++i) a = z28_sqr(a);
+    return a;
+}
+
+__device__ zq zq_inv_fermat(const zq& xin) {
+    const z28 x = z28_from(xin);
+    const z28 t2 = z28_mul(z28_sqr(x), x);
+    const z28 t4 = z28_mul(z28_sqn(t2, 2), t2);
+    const z28 t8 = z28_mul(z28_sqn(t4, 4), t4);
+    const z28 t16 = z28_mul(z28_sqn(t8, 8), t8);
+    const z28 t32 = z28_mul(z28_sqn(t16, 16), t16);
+    const z28 t64 = z28_mul(z28_sqn(t32, 32), t32);
+    const z28 t96 = z28_mul(z28_sqn(t64, 32), t32);
+    const z28 t104 = z28_mul(z28_sqn(t96, 8), t8);
+    // low byte 0xB3 = 1011 0011, most significant bit first
+    z28 r = z28_mul(z28_sqr(t104), x);  // 1
+    r = z28_sqr(r);                     // 0
+    r = z28_mul(z28_sqr(r), x);         // 1
+    r = z28_mul(z28_sqr(r), x);         // 1
+    r = z28_sqn(r, 2);                  // 0 0
+    r = z28_mul(z28_sqr(r), x);         // 1
+    r = z28_mul(z28_sqr(r), x);         // 1
+    return z28_to(r);
+}
+
 __device__ void write_challenge_pair(zq* chal_out, const zq& x) {
     chal_out[0] = x;
-    chal_out[1] = zq_inv_binary(x);
+    chal_out[1] = zq_inv_fermat(x);
 }
 
 // Root merge; one workgroup of 1024 threads over n <= 1024 CVs. `chunk_level` says the CVs
 // are chunk CVs (needed to redo the counter chunks of a zero challenge).
-__global__ void __launch_bounds__(1024) k_b3_root(const uint32_t* __restrict__ cvs, size_t n,
-                                                  uint8_t* msg, size_t len, KeyW key, uint32_t flags,
-                                                  uint32_t* digest_out, zq* chal_out, uint32_t* err,
-                                                  int chunk_level) {
-    __shared__ uint32_t sh[1024 * 8];
+__global__ void __launch_bounds__(1024) k_b3_root(uint32_t* __restrict__ cvs, size_t n, uint8_t* msg, size_t len,
+                                                  KeyW key, uint32_t flags, uint32_t* digest_out, zq* chal_out,
+                                                  uint32_t* err, int chunk_level) {
+    __shared__ uint32_t sa[1024 * 8], sb[512 * 8 + 8];
     __shared__ uint32_t dig[8];
     __shared__ int again;
-    uint32_t counter = 0;
+    const uint32_t q = threadIdx.x & 3;
+    const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
+    uint32_t counter = 0;  // used by thread 0
     for (;;) {
         if (n == 1) {
-            if (threadIdx.x == 0) {
-                uint32_t d[8];
-                chunk_cv(msg, len, 0, key.w, flags, true, d);
-                for (int j = 0; j < 8; ++j) dig[j] = d[j];
+            if (threadIdx.x < 4) {
+                uint32_t ca, cb;
+                chunk_cv_quad(msg, len, 0, key.w, flags, true, sel, ca, cb);
+                dig[q] = ca;
+                dig[4 + q] = cb;
             }
             __syncthreads();
         } else {
-            for (uint32_t i = threadIdx.x; i < n * 8; i += blockDim.x) sh[i] = cvs[i];
+            for (uint32_t i = threadIdx.x; i < n * 8; i += blockDim.x) sa[i] = cvs[i];
             __syncthreads();
-            merge_lds(sh, (uint32_t)n, key.w, flags, true);
-            if (threadIdx.x < 8) dig[threadIdx.x] = sh[threadIdx.x];
+            const uint32_t* r = merge_lds(sa, sb, (uint32_t)n, key.w, flags, true);
+            if (threadIdx.x < 8) dig[threadIdx.x] = r[threadIdx.x];
             __syncthreads();
         }
         if (threadIdx.x == 0) {
@@ -306,6 +489,10 @@ __global__ void __launch_bounds__(1024) k_b3_root(const uint32_t* __restrict__ c
                 const zq x = zq_from_digest(h);
                 if (!zq_is_zero(x)) {
                     write_challenge_pair(chal_out, x);
+                } else if (!chunk_level) {
+                    atomicOr(err, 0x40000000u);  // group CVs cannot be patched here
+                } else if (counter >= 64) {
+                    atomicOr(err, 0x80000000u);  // unreachable in practice (2^-112 per try)
                 } else {
                     // sample_nonzero_challenge (transcript.rs:66-75): counter + 1, hash again
                     counter += 1;
@@ -313,29 +500,21 @@ __global__ void __launch_bounds__(1024) k_b3_root(const uint32_t* __restrict__ c
                     again = 1;
                 }
             }
+            __threadfence();
         }
         __syncthreads();
-        if (!again) break;
-        if (!chunk_level) {  // group CVs cannot be patched here; reported to the host
-            if (threadIdx.x == 0) atomicOr(err, 0x40000000u);
-            break;
-        }
-        // recompute the chunk(s) holding the counter bytes
-        if (threadIdx.x == 0 && n > 1) {
+        if (!again) break;  // uniform: every thread reads the same shared flag
+        if (threadIdx.x < 4 && n > 1) {  // quad 0 recomputes the chunk(s) holding the counter
             const size_t c0 = (len - 4) / 1024, c1 = (len - 1) / 1024;
-            uint32_t* cw = const_cast<uint32_t*>(cvs);
             for (size_t c = c0; c <= c1; ++c) {
-                uint32_t d[8];
-                chunk_cv(msg, len, c, key.w, flags, false, d);
-                for (int j = 0; j < 8; ++j) cw[c * 8 + j] = d[j];
+                uint32_t ca, cb;
+                chunk_cv_quad(msg, len, c, key.w, flags, false, sel, ca, cb);
+                cvs[c * 8 + q] = ca;
+                cvs[c * 8 + 4 + q] = cb;
             }
-            __threadfence_block();
+            __threadfence();
         }
         __syncthreads();
-        if (counter > 64 && threadIdx.x == 0) {  // unreachable in practice (2^-112 per try)
-            atomicOr(err, 0x80000000u);
-            break;
-        }
     }
 }
 
@@ -344,8 +523,8 @@ void launch_b3_root(const uint32_t* cvs, size_t n, const uint8_t* msg, size_t le
                     uint32_t* err_flags, int chunk_level, hipStream_t s) {
     KeyW k;
     for (int i = 0; i < 8; ++i) k.w[i] = key[i];
-    hipLaunchKernelGGL(k_b3_root, dim3(1), dim3(1024), 0, s, cvs, n, const_cast<uint8_t*>(msg), len, k,
-                       flags, digest_out, chal_out, err_flags, chunk_level);
+    hipLaunchKernelGGL(k_b3_root, dim3(1), dim3(1024), 0, s, const_cast<uint32_t*>(cvs), n,
+                       const_cast<uint8_t*>(msg), len, k, flags, digest_out, chal_out, err_flags, chunk_level);
 }
 
 // Hash a device buffer: chunk CVs, group merges while more than 1024 values remain, root.
