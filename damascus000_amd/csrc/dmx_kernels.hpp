@@ -60,13 +60,17 @@ void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* ou
 void launch_ntt_inv(uint32_t* data, size_t slices, int log_m, const NttTables& tw, hipStream_t s);
 // inv = sum over all rows W[i]*G[i,k]; L = sum_{i<h} G[i+h,k]*W[i]; R = sum_{i<h} G[i,k]*W[i+h]
 // (h = rows/2; L/R skipped when out_L == nullptr). Outputs [k][8][m] residues.
+// `part` is scratch of mac_part_words(log_m) words: rows are split into segments whose partial
+// sums a second kernel adds.
 void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
-                      uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, hipStream_t s);
+                      uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, uint32_t* part, hipStream_t s);
 // vector-stage invariant at degree m plus the poly-stage cross terms at degree m/2, from the
 // NTT-domain even/odd split: out_inv [k][8][m], out_L/out_R [k][8][m/2]
 void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
-                      uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, const NttTables& tw,
-                      hipStream_t s);
+                      uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, uint32_t* part,
+                      const NttTables& tw, hipStream_t s);
+// scratch words the MAC passes need at degrees up to 2^max_log_m
+size_t mac_part_words(int max_log_m);
 // Z_q rows only (no transform): dst[row][c] = row_value(src, row, c)
 void launch_fold_rows_zq(const RowSrc& src, size_t rows, int log_m, hipStream_t s);
 void launch_final_check(const zq* g, const zq* c, const zq* opening, size_t n, uint32_t* flags,

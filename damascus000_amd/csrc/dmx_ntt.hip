@@ -246,13 +246,6 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_pairs(const uint32_t* __res
     }
 }
 
-void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
-                      uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, hipStream_t s) {
-    const size_t n = (size_t)K * NPRIMES << log_m;
-    hipLaunchKernelGGL(k_mac_pairs, dim3((unsigned)((n + MAC_THREADS - 1) / MAC_THREADS)),
-                       dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, out_inv, out_L,
-                       out_R);
-}
 
 // thread = (k, p, t), t < m/2, over the position pair (2t, 2t+1).
 // With F = NTT_m(f): F[2t] = f(r_t), F[2t+1] = f(-r_t), r_t = zeta[m/2 + t]; the degree-m/2
@@ -312,13 +305,226 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_split(const uint32_t* __res
     out_R[oh] = rv;
 }
 
-void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
-                      uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, const NttTables& tw,
-                      hipStream_t s) {
-    const size_t n = (size_t)K * NPRIMES << (log_m - 1);
-    hipLaunchKernelGGL(k_mac_split, dim3((unsigned)((n + MAC_THREADS - 1) / MAC_THREADS)),
-                       dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, out_inv, out_L,
-                       out_R, tw);
+
+// Row segments per MAC pass: enough waves to keep HBM busy, each segment at least 16 rows.
+static inline uint32_t mac_segments(size_t rows, int log_m) {
+    const size_t quads = ((size_t)K * NPRIMES << log_m) / 4;  // threads per segment
+    size_t seg = 1;
+    while (seg < 32 && rows / (2 * seg) >= 16 && quads * seg < (size_t)1 << 18) seg *= 2;
+    return (uint32_t)seg;
+}
+
+__device__ __forceinline__ uint4 ld4(const uint32_t* p) { return *reinterpret_cast<const uint4*>(p); }
+
+// Pass over generator rows for the post-poly invariant and the next round's vector cross
+// terms: thread = (segment, k, p, 4 consecutive positions). Partial sums, reduced mod p, go
+// to part[seg][slot][k][p][c] (slot 0 = inv, 1 = L, 2 = R).
+__global__ void __launch_bounds__(MAC_THREADS) k_mac_pairs4(const uint32_t* __restrict__ What,
+                                                            const uint32_t* __restrict__ Ghat, uint32_t rows,
+                                                            int log_m, uint32_t nseg, uint32_t seg_rows, int cross,
+                                                            uint32_t* __restrict__ part) {
+    const size_t m = (size_t)1 << log_m;
+    const size_t per_seg = (size_t)K * NPRIMES * m / 4;
+    const size_t gid = (size_t)blockIdx.x * MAC_THREADS + threadIdx.x;
+    if (gid >= (size_t)nseg * per_seg) return;
+    const uint32_t seg = (uint32_t)(gid / per_seg);
+    const size_t r = gid - (size_t)seg * per_seg;
+    const uint32_t c = (uint32_t)((r * 4) & (m - 1));
+    const uint32_t kp = (uint32_t)((r * 4) >> log_m);  // k * 8 + p
+    const uint32_t p = kp % NPRIMES;
+    const size_t wstride = NPRIMES * m, gstride = (size_t)K * NPRIMES * m;
+    const uint32_t* wp = What + p * m + c;
+    const uint32_t* gp = Ghat + (size_t)kp * m + c;
+    Acc96 inv[4], L[4], R[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        inv[e].zero();
+        L[e].zero();
+        R[e].zero();
+    }
+    if (!cross) {
+        const uint32_t i0 = seg * seg_rows, i1 = min(rows, i0 + seg_rows);
+        for (uint32_t i = i0; i < i1; ++i) {
+            const uint4 w = ld4(wp + i * wstride), g = ld4(gp + i * gstride);
+            inv[0].mac(w.x, g.x);
+            inv[1].mac(w.y, g.y);
+            inv[2].mac(w.z, g.z);
+            inv[3].mac(w.w, g.w);
+        }
+    } else {
+        const uint32_t h = rows >> 1;
+        const uint32_t i0 = seg * seg_rows, i1 = min(h, i0 + seg_rows);
+        for (uint32_t i = i0; i < i1; ++i) {
+            const uint4 w0 = ld4(wp + i * wstride), w1 = ld4(wp + (i + h) * wstride);
+            const uint4 g0 = ld4(gp + i * gstride), g1 = ld4(gp + (i + h) * gstride);
+            const uint32_t W0[4] = {w0.x, w0.y, w0.z, w0.w}, W1[4] = {w1.x, w1.y, w1.z, w1.w};
+            const uint32_t G0[4] = {g0.x, g0.y, g0.z, g0.w}, G1[4] = {g1.x, g1.y, g1.z, g1.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                inv[e].mac(W0[e], G0[e]);
+                inv[e].mac(W1[e], G1[e]);
+                L[e].mac(G1[e], W0[e]);
+                R[e].mac(G0[e], W1[e]);
+            }
+        }
+    }
+    const PrimeConsts pc = c_pc[p];
+    const size_t slot = (size_t)K * NPRIMES * m;
+    uint32_t* o = part + (size_t)seg * 3 * slot + (size_t)kp * m + c;
+    uint4 v = make_uint4(reduce_u96(inv[0].lo, inv[0].hi, pc), reduce_u96(inv[1].lo, inv[1].hi, pc),
+                         reduce_u96(inv[2].lo, inv[2].hi, pc), reduce_u96(inv[3].lo, inv[3].hi, pc));
+    *reinterpret_cast<uint4*>(o) = v;
+    if (cross) {
+        v = make_uint4(reduce_u96(L[0].lo, L[0].hi, pc), reduce_u96(L[1].lo, L[1].hi, pc),
+                       reduce_u96(L[2].lo, L[2].hi, pc), reduce_u96(L[3].lo, L[3].hi, pc));
+        *reinterpret_cast<uint4*>(o + slot) = v;
+        v = make_uint4(reduce_u96(R[0].lo, R[0].hi, pc), reduce_u96(R[1].lo, R[1].hi, pc),
+                       reduce_u96(R[2].lo, R[2].hi, pc), reduce_u96(R[3].lo, R[3].hi, pc));
+        *reinterpret_cast<uint4*>(o + 2 * slot) = v;
+    }
+}
+
+// sum the segment partials: out[slot][k][p][c] for nslots slots
+__global__ void k_mac_pairs_reduce(const uint32_t* __restrict__ part, uint32_t nseg, int log_m, int nslots,
+                                   uint32_t* __restrict__ out_inv, uint32_t* __restrict__ out_L,
+                                   uint32_t* __restrict__ out_R) {
+    const size_t m = (size_t)1 << log_m;
+    const size_t slot = (size_t)K * NPRIMES * m;
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= slot * nslots) return;
+    const size_t sl = gid / slot, e = gid - sl * slot;
+    const uint32_t P = c_pc[(e >> log_m) % NPRIMES].p;
+    uint64_t acc = 0;
+    for (uint32_t sg = 0; sg < nseg; ++sg) acc += part[(size_t)sg * 3 * slot + sl * slot + e];
+    uint32_t* o = sl == 0 ? out_inv : (sl == 1 ? out_L : out_R);
+    o[e] = (uint32_t)(acc % P);
+}
+
+void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
+                      uint32_t* out_L, uint32_t* out_R, uint32_t* part, hipStream_t s) {
+    const bool cross = out_L != nullptr && rows > 1;
+    const size_t half = cross ? rows / 2 : rows;
+    const uint32_t nseg = log_m >= 2 ? mac_segments(half, log_m) : 1;
+    const uint32_t seg_rows = (uint32_t)((half + nseg - 1) / nseg);
+    const size_t m = (size_t)1 << log_m;
+    if (log_m < 2) {  // degrees 1 and 2: the scalar kernel
+        const size_t n = (size_t)K * NPRIMES << log_m;
+        hipLaunchKernelGGL(k_mac_pairs, dim3((unsigned)((n + MAC_THREADS - 1) / MAC_THREADS)), dim3(MAC_THREADS), 0,
+                           s, What, Ghat, (uint32_t)rows, log_m, out_inv, out_L, out_R);
+        return;
+    }
+    const size_t threads = (size_t)nseg * K * NPRIMES * m / 4;
+    hipLaunchKernelGGL(k_mac_pairs4, dim3((unsigned)((threads + MAC_THREADS - 1) / MAC_THREADS)), dim3(MAC_THREADS),
+                       0, s, What, Ghat, (uint32_t)rows, log_m, nseg, seg_rows, (int)cross, part);
+    const int nslots = cross ? 3 : 1;
+    const size_t n = (size_t)K * NPRIMES * m * nslots;
+    hipLaunchKernelGGL(k_mac_pairs_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nseg, log_m,
+                       nslots, out_inv, out_L, out_R);
+}
+
+// Vector-stage invariant (degree m) and poly-stage cross terms (degree m/2) in one pass;
+// thread = (segment, k, p, two position pairs). Partial sums S00, S11, S01, S10 per pair go to
+// part[seg][4][k][p][t].
+__global__ void __launch_bounds__(MAC_THREADS) k_mac_split4(const uint32_t* __restrict__ What,
+                                                            const uint32_t* __restrict__ Ghat, uint32_t rows,
+                                                            int log_m, uint32_t nseg, uint32_t seg_rows,
+                                                            uint32_t* __restrict__ part) {
+    const size_t m = (size_t)1 << log_m, h = m >> 1;
+    const size_t per_seg = (size_t)K * NPRIMES * m / 4;
+    const size_t gid = (size_t)blockIdx.x * MAC_THREADS + threadIdx.x;
+    if (gid >= (size_t)nseg * per_seg) return;
+    const uint32_t seg = (uint32_t)(gid / per_seg);
+    const size_t r = gid - (size_t)seg * per_seg;
+    const uint32_t c = (uint32_t)((r * 4) & (m - 1));  // positions c..c+3 = pairs t, t+1
+    const uint32_t kp = (uint32_t)((r * 4) >> log_m);
+    const uint32_t p = kp % NPRIMES;
+    const size_t wstride = NPRIMES * m, gstride = (size_t)K * NPRIMES * m;
+    const uint32_t* wp = What + p * m + c;
+    const uint32_t* gp = Ghat + (size_t)kp * m + c;
+    Acc96 s00[2], s11[2], s01[2], s10[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        s00[e].zero();
+        s11[e].zero();
+        s01[e].zero();
+        s10[e].zero();
+    }
+    const uint32_t i0 = seg * seg_rows, i1 = min(rows, i0 + seg_rows);
+    for (uint32_t i = i0; i < i1; ++i) {
+        const uint4 w = ld4(wp + i * wstride), g = ld4(gp + i * gstride);
+        s00[0].mac(w.x, g.x);
+        s11[0].mac(w.y, g.y);
+        s01[0].mac(w.x, g.y);
+        s10[0].mac(w.y, g.x);
+        s00[1].mac(w.z, g.z);
+        s11[1].mac(w.w, g.w);
+        s01[1].mac(w.z, g.w);
+        s10[1].mac(w.w, g.z);
+    }
+    const PrimeConsts pc = c_pc[p];
+    const size_t slot = (size_t)K * NPRIMES * h;
+    uint32_t* o = part + (size_t)seg * 4 * slot + (size_t)kp * h + c / 2;
+    *reinterpret_cast<uint2*>(o) = make_uint2(reduce_u96(s00[0].lo, s00[0].hi, pc), reduce_u96(s00[1].lo, s00[1].hi, pc));
+    *reinterpret_cast<uint2*>(o + slot) = make_uint2(reduce_u96(s11[0].lo, s11[0].hi, pc), reduce_u96(s11[1].lo, s11[1].hi, pc));
+    *reinterpret_cast<uint2*>(o + 2 * slot) = make_uint2(reduce_u96(s01[0].lo, s01[0].hi, pc), reduce_u96(s01[1].lo, s01[1].hi, pc));
+    *reinterpret_cast<uint2*>(o + 3 * slot) = make_uint2(reduce_u96(s10[0].lo, s10[0].hi, pc), reduce_u96(s10[1].lo, s10[1].hi, pc));
+}
+
+__global__ void k_mac_split_reduce(const uint32_t* __restrict__ part, uint32_t nseg, int log_m,
+                                   uint32_t* __restrict__ out_inv, uint32_t* __restrict__ out_L,
+                                   uint32_t* __restrict__ out_R, NttTables tw) {
+    const size_t m = (size_t)1 << log_m, h = m >> 1;
+    const size_t slot = (size_t)K * NPRIMES * h;
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= slot) return;
+    const uint32_t t = (uint32_t)(gid & (h - 1));
+    const uint32_t kp = (uint32_t)(gid >> (log_m - 1));
+    const uint32_t p = kp % NPRIMES;
+    const uint32_t P = c_pc[p].p;
+    uint64_t a[4] = {0, 0, 0, 0};
+    for (uint32_t sg = 0; sg < nseg; ++sg)
+        for (int j = 0; j < 4; ++j) a[j] += part[(size_t)sg * 4 * slot + j * slot + gid];
+    const uint32_t a00 = (uint32_t)(a[0] % P), a11 = (uint32_t)(a[1] % P), a01 = (uint32_t)(a[2] % P),
+                   a10 = (uint32_t)(a[3] % P);
+    const size_t o = (size_t)kp * m + 2 * t;
+    out_inv[o] = a00;
+    out_inv[o + 1] = a11;
+    const uint32_t d = subp(a00, a11, P);
+    const uint32_t Lp = addp(subp(d, a01, P), a10, P);
+    const uint32_t Rp = subp(addp(d, a01, P), a10, P);
+    const uint32_t kz = (uint32_t)h + t;
+    const uint32_t i4m = tw.invn_m[p * (MAX_LOG_N + 1) + 2], i4mp = tw.invn_mp[p * (MAX_LOG_N + 1) + 2];
+    out_L[(size_t)kp * h + t] =
+        mulc(mulc(Lp, tw.zeta_m[p * TW_SIZE + kz], tw.zeta_mp[p * TW_SIZE + kz], P), i4m, i4mp, P);
+    out_R[(size_t)kp * h + t] =
+        mulc(mulc(Rp, tw.izeta_m[p * TW_SIZE + kz], tw.izeta_mp[p * TW_SIZE + kz], P), i4m, i4mp, P);
+}
+
+void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
+                      uint32_t* out_L, uint32_t* out_R, uint32_t* part, const NttTables& tw, hipStream_t s) {
+    if (log_m < 2) {  // degree 2: the scalar kernel
+        const size_t n = (size_t)K * NPRIMES << (log_m - 1);
+        hipLaunchKernelGGL(k_mac_split, dim3((unsigned)((n + MAC_THREADS - 1) / MAC_THREADS)), dim3(MAC_THREADS), 0,
+                           s, What, Ghat, (uint32_t)rows, log_m, out_inv, out_L, out_R, tw);
+        return;
+    }
+    const uint32_t nseg = mac_segments(rows * 2, log_m);
+    const uint32_t seg_rows = (uint32_t)((rows + nseg - 1) / nseg);
+    const size_t m = (size_t)1 << log_m;
+    const size_t threads = (size_t)nseg * K * NPRIMES * m / 4;
+    hipLaunchKernelGGL(k_mac_split4, dim3((unsigned)((threads + MAC_THREADS - 1) / MAC_THREADS)), dim3(MAC_THREADS),
+                       0, s, What, Ghat, (uint32_t)rows, log_m, nseg, seg_rows, part);
+    const size_t n = (size_t)K * NPRIMES * (m / 2);
+    hipLaunchKernelGGL(k_mac_split_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nseg, log_m,
+                       out_inv, out_L, out_R, tw);
+}
+
+size_t mac_part_words(int max_log_m) {
+    // a launch holds at most max(2^18, quads) threads of 4 positions, in at most 3 slots
+    size_t w = 0;
+    for (int lg = 2; lg <= max_log_m; ++lg)
+        w = std::max(w, (size_t)mac_segments(SIZE_MAX / 2, lg) * 3 * K * NPRIMES << lg);
+    return std::max(w, (size_t)1);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -579,7 +579,7 @@ struct dmx_prover {
     size_t round = 0;
     bool poisoned = false;
     bool capturing = false;  // inside hipGraph capture: no timing events
-    DevBuf file, W_a, W_b, G_a, G_b, What_a, What_b, Ghat_a, Ghat_b, res, lifted, C, Ct, com0, recs,
+    DevBuf file, W_a, W_b, G_a, G_b, What_a, What_b, Ghat_a, Ghat_b, res, part, lifted, C, Ct, com0, recs,
         scal, flags;
     std::vector<size_t> rec_off;  // offset (coefficients) of round j's record block
     Transcriber tr;
@@ -613,6 +613,7 @@ static int prover_alloc(dmx_prover* p) {
     rc = rc ? rc : p->Ghat_a.alloc(NN * K * NPRIMES * 4);
     rc = rc ? rc : p->Ghat_b.alloc(NN / 2 * K * NPRIMES * 4);
     rc = rc ? rc : p->res.alloc((size_t)3 * K * NPRIMES * N * 4);
+    rc = rc ? rc : p->part.alloc(mac_part_words(p->d) * 4);
     rc = rc ? rc : p->lifted.alloc((size_t)K * N * 16);
     rc = rc ? rc : p->C.alloc((size_t)K * N * 16);
     rc = rc ? rc : p->Ct.alloc((size_t)K * N * 16);
@@ -653,7 +654,7 @@ static int prover_initialize(dmx_prover* p) {
     fwd_rows(p->ctx, pl, N, d, p->What_a.as<uint32_t>(), s);
     const size_t slot = (size_t)K * NPRIMES * N;
     launch_mac_pairs(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), N, d, p->res_at(0), p->res_at(slot),
-                     p->res_at(2 * slot), s);
+                     p->res_at(2 * slot), p->part.as<uint32_t>(), s);
     launch_ntt_inv(p->res_at(0), 3 * K, d, tw, s);
     launch_crt_lift(p->res_at(0), p->C.as<zq>(), K, d, s);
     launch_crt_lift(p->res_at(slot), p->rec_lvec(0), 2 * K, d, s);  // L_vec, R_vec adjacent
@@ -690,7 +691,7 @@ static int prover_round(dmx_prover* p, size_t j) {
     // invariant at degree m and the poly cross terms at degree m/2 from one pass
     const size_t s0 = (size_t)K * NPRIMES * m;
     launch_mac_split(p->What_b.as<uint32_t>(), p->Ghat_b.as<uint32_t>(), h, lm, p->res_at(0), p->res_at(s0),
-                     p->res_at(s0 + (size_t)K * NPRIMES * h), tw, s);
+                     p->res_at(s0 + (size_t)K * NPRIMES * h), p->part.as<uint32_t>(), tw, s);
     launch_ntt_inv(p->res_at(0), K, lm, tw, s);
     launch_ntt_inv(p->res_at(s0), 2 * K, lm - 1, tw, s);
     launch_crt_lift(p->res_at(0), p->lifted.as<zq>(), K, lm, s);
@@ -717,7 +718,7 @@ static int prover_round(dmx_prover* p, size_t j) {
     const size_t s1 = (size_t)K * NPRIMES * h;
     const bool next = h > 1;
     launch_mac_pairs(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), h, lm - 1, p->res_at(0),
-                     next ? p->res_at(s1) : nullptr, next ? p->res_at(2 * s1) : nullptr, s);
+                     next ? p->res_at(s1) : nullptr, next ? p->res_at(2 * s1) : nullptr, p->part.as<uint32_t>(), s);
     launch_ntt_inv(p->res_at(0), next ? 3 * K : K, lm - 1, tw, s);
     launch_crt_lift(p->res_at(0), p->lifted.as<zq>(), K, lm - 1, s);
     if (next) launch_crt_lift(p->res_at(s1), p->rec_lvec(j + 1), 2 * K, lm - 1, s);
@@ -1520,7 +1521,7 @@ static int commit_impl(dmx_ctx* ctx, const uint64_t* witness, const uint64_t* ge
     // rows per partial sum: rows * m <= 2^24 keeps |V| <= rows*m*(q/2)^2 < P/2
     const size_t chunk = std::max<size_t>(1, ((size_t)1 << 24) / m);
     const size_t crow = std::min(rows, chunk);
-    DevBuf W, G, What, Ghat, res, part, acc;
+    DevBuf W, G, What, Ghat, res, part, acc, mpart;
     int rc = 0;
     rc = rc ? rc : W.alloc(crow * m * 16);
     rc = rc ? rc : G.alloc(crow * K * m * 16);
@@ -1529,6 +1530,7 @@ static int commit_impl(dmx_ctx* ctx, const uint64_t* witness, const uint64_t* ge
     rc = rc ? rc : res.alloc((size_t)K * NPRIMES * m * 4);
     rc = rc ? rc : part.alloc((size_t)K * m * 16);
     rc = rc ? rc : acc.alloc((size_t)K * m * 16);
+    rc = rc ? rc : mpart.alloc(mac_part_words(lg) * 4);
     if (rc) return rc;
     hipStream_t s = 0;
     HIP_TRY(hipMemsetAsync(acc.p, 0, (size_t)K * m * 16, s));
@@ -1558,7 +1560,8 @@ static int commit_impl(dmx_ctx* ctx, const uint64_t* witness, const uint64_t* ge
             launch_ntt_fwd_rows(pg, nr * K, lg, Ghat.as<uint32_t>(), ctx->tw, s);
             HIP_TRY(hipStreamSynchronize(s));
         }
-        launch_mac_pairs(What.as<uint32_t>(), Ghat.as<uint32_t>(), nr, lg, res.as<uint32_t>(), nullptr, nullptr, s);
+        launch_mac_pairs(What.as<uint32_t>(), Ghat.as<uint32_t>(), nr, lg, res.as<uint32_t>(), nullptr, nullptr,
+                         mpart.as<uint32_t>(), s);
         launch_ntt_inv(res.as<uint32_t>(), K, lg, ctx->tw, s);
         launch_crt_lift(res.as<uint32_t>(), part.as<zq>(), K, lg, s);
         launch_zq_add(acc.as<zq>(), part.as<zq>(), acc.as<zq>(), (size_t)K * m, s);
