@@ -16,7 +16,8 @@ import numpy as np
 __all__ = [
     "Q", "K", "PRIMES", "DamascusError", "SystemParams", "RuntimeConfig", "Context", "backend_ready",
     "device_summary", "ModuleSisCommitter", "DamascusProver", "DamascusVerifier", "ntt_batch", "fold_batch",
-    "ring_mul_batch", "prove", "verify", "prove_verify_device", "derive_subseed", "lib", "LIB_PATH", "DeviceBuffer",
+    "ring_mul_batch", "prove", "verify", "prove_verify_device", "prove_verify_many", "derive_subseed", "lib", "LIB_PATH",
+    "DeviceBuffer",
 ]
 
 Q = (1 << 112) - 75
@@ -78,6 +79,14 @@ class RoundOutput(ctypes.Structure):
                 ("total_round_ns", ctypes.c_uint64)]
 
 
+class Job(ctypes.Structure):
+    """dmx_job: one file of a ragged batch and where its proof goes"""
+    _fields_ = [("file", _VP), ("len", ctypes.c_uint64), ("params", SystemParams), ("statement", _VP),
+                ("statement_cap", _SZ), ("statement_len", _SZ), ("records", _VP), ("records_cap", _SZ),
+                ("records_len", _SZ), ("opening", ctypes.c_uint64 * 2), ("depth", ctypes.c_int32),
+                ("status", ctypes.c_int32)]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "dmx_backend_ready": (ctypes.c_int, []),
@@ -112,6 +121,7 @@ _SIGS = {
     "dmx_prove": (ctypes.c_int, [_VP, _VP, _SZ, _VP, _VP, _VP, _SZ, _SZP, _VP, _SZ, _SZP, _VP]),
     "dmx_verify": (ctypes.c_int, [_VP, _VP, _VP, _SZ, _VP, _SZ, _VP]),
     "dmx_prove_verify_device": (ctypes.c_int, [_VP, _VP, _SZ, _VP, _VP, _SZ, _SZP, _VP, _SZ, _SZP, _VP]),
+    "dmx_prove_verify_many": (ctypes.c_int, [_VP, _VP, _SZ, ctypes.c_int, _SZ]),
     "dmx_device_alloc": (ctypes.c_int, [_VP, _SZ, ctypes.POINTER(_VP)]),
     "dmx_device_free": (ctypes.c_int, [_VP, _VP]),
     "dmx_memcpy_htod": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
@@ -445,6 +455,54 @@ def prove_verify_device(d_file_ptr, length, params=None, ctx=None, want_proof=Tr
     _check(lib().dmx_prove_verify_device(ctx.h, _VP(d_file_ptr), length, ctypes.byref(params), None, 0, None, None, 0,
                                          None, None))
     return None
+
+
+def prove_verify_many(files, params=None, ctx=None, max_batch=0, want_proof=True, raise_on_error=True):
+    """prove + replay of many files in one ragged batch (dmx_prove_verify_many).
+
+    files: bytes objects (copied from host memory) or (device_pointer, length) pairs.
+    params: one SystemParams for all, or a list with one per file.
+    Returns a list of (statement, records_blob, opening) per file, or of status codes when
+    raise_on_error is False and a job failed (then the entry is the negative code)."""
+    ctx = ctx or Context.default()
+    n = len(files)
+    if n == 0:
+        return []
+    on_host = not isinstance(files[0], tuple)
+    plist = params if isinstance(params, (list, tuple)) else [params or SystemParams()] * n
+    jobs = (Job * n)()
+    keep = []
+    for i, f in enumerate(files):
+        if on_host:
+            buf = ctypes.create_string_buffer(bytes(f), max(len(f), 1))
+            keep.append(buf)
+            jobs[i].file = ctypes.cast(buf, _VP)
+            jobs[i].len = len(f)
+        else:
+            jobs[i].file = _VP(f[0])
+            jobs[i].len = f[1]
+        jobs[i].params = plist[i]
+        if want_proof:
+            st = ctypes.create_string_buffer(lib().dmx_statement_size(4096))
+            rc = ctypes.create_string_buffer(max(lib().dmx_proof_records_size(jobs[i].len), 1))
+            keep += [st, rc]
+            jobs[i].statement, jobs[i].statement_cap = ctypes.cast(st, _VP), len(st)
+            jobs[i].records, jobs[i].records_cap = ctypes.cast(rc, _VP), len(rc)
+    rc = lib().dmx_prove_verify_many(ctx.h, jobs, n, int(on_host), max_batch)
+    if rc != DMX_OK and raise_on_error:
+        _check(rc)
+    out = []
+    for i in range(n):
+        j = jobs[i]
+        if j.status != DMX_OK:
+            out.append(int(j.status))
+        elif want_proof:
+            st = ctypes.string_at(j.statement, j.statement_len)
+            recs = ctypes.string_at(j.records, j.records_len)
+            out.append((st, recs, int(j.opening[0]) | (int(j.opening[1]) << 64)))
+        else:
+            out.append(None)
+    return out
 
 
 def ntt_batch(a, b, n, primes, stage_roots, inv_roots, inv_sizes):

@@ -265,11 +265,12 @@ DMX_HD uint32_t zq_mod_p_centred(const zq& x, const PrimeConsts& c) {
 }
 
 // (hi * 2^64 + lo) mod p for the u96 accumulators of the MAC kernels
+// (three REDC steps give V * R^-3, then * R^3; each REDC input stays below p*R)
 DMX_HD uint32_t reduce_u96(uint64_t lo, uint32_t hi, const PrimeConsts& c) {
-    uint64_t a = lo % c.p;
-    uint64_t b = ((uint64_t)(hi % c.p) * c.two64) % c.p;
-    uint64_t s = a + b;
-    return (uint32_t)(s >= c.p ? s - c.p : s);
+    uint32_t y = redc((uint32_t)lo, c.p, c.pinv);
+    y = redc((uint64_t)y + (uint32_t)(lo >> 32), c.p, c.pinv);
+    y = redc((uint64_t)y + hi, c.p, c.pinv);
+    return mulc(y, c.r4_m, c.r4_mp, c.p);
 }
 
 // ------------------------------------------------------------------------------------------

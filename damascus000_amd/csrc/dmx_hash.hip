@@ -128,10 +128,15 @@ __device__ uint8_t transcript_byte(const TranscriptMsg& t, size_t o, uint32_t co
     return le_byte(counter, (uint32_t)o);
 }
 
-__global__ void k_transcript_pack(TranscriptMsg t, uint8_t* __restrict__ msg, size_t len) {
+__global__ void k_transcript_pack(TranscriptMsg t, uint8_t* __restrict__ msg, size_t len, size_t js) {
     const size_t w = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t o = w * 4;
     if (o >= len) return;
+    const size_t jo = job_off(js);
+    t.epoch_seed = jp(t.epoch_seed, jo);
+    t.file_id = jp(t.file_id, jo);
+    for (int i = 0; i < 3; ++i) t.mods[i] = jp(t.mods[i], jo);
+    msg = jp(msg, jo);
     uint32_t v = 0;
     for (uint32_t b = 0; b < 4 && o + b < len; ++b) v |= (uint32_t)transcript_byte(t, o + b, 0) << (8 * b);
     if (o + 4 <= len) {
@@ -141,10 +146,11 @@ __global__ void k_transcript_pack(TranscriptMsg t, uint8_t* __restrict__ msg, si
     }
 }
 
-void launch_transcript_pack(const TranscriptMsg& t, uint8_t* msg, size_t len, hipStream_t s) {
+void launch_transcript_pack(const TranscriptMsg& t, uint8_t* msg, size_t len, hipStream_t s, const Jobs& jb) {
     const size_t words = (len + 3) / 4;
-    hipLaunchKernelGGL(k_transcript_pack, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, t, msg,
-                       len);
+    if (jb.n == 0) return;
+    hipLaunchKernelGGL(k_transcript_pack, dim3((unsigned)((words + 255) / 256), jb.n), dim3(256), 0, s, t, msg,
+                       len, jb.stride);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -194,12 +200,27 @@ __device__ void chunk_cv_quad(const uint8_t* msg, size_t len, size_t ci, const u
     }
 }
 
+__host__ __device__ inline size_t nchunks_of(size_t len) { return len == 0 ? 1 : (len + 1023) / 1024; }
+
+// a job's own length when lengths live on the device (bounded by the launch capacity)
+__device__ __forceinline__ size_t job_len(size_t cap, const uint64_t* dlen, size_t jo) {
+    return dlen ? (size_t)min((uint64_t)cap, *jp(dlen, jo)) : cap;
+}
+
 // one quad per chunk, 16 chunks per 64-thread block; the guard is uniform within a quad
 __global__ void __launch_bounds__(64) k_b3_chunks(const uint8_t* __restrict__ msg, size_t len, KeyW key,
-                                                  uint32_t flags, uint32_t* __restrict__ cvs, size_t n) {
+                                                  uint32_t flags, uint32_t* __restrict__ cvs, size_t n, size_t js,
+                                                  const uint64_t* dlen) {
     const uint32_t q = threadIdx.x & 3;
     const size_t ci = (size_t)blockIdx.x * 16 + (threadIdx.x >> 2);
+    const size_t jo = job_off(js);
+    if (dlen) {
+        len = job_len(len, dlen, jo);
+        n = nchunks_of(len);
+    }
     if (ci >= n) return;
+    msg = jp(msg, jo);
+    cvs = jp(cvs, jo);
     const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
     uint32_t ca, cb;
     chunk_cv_quad(msg, len, ci, key.w, flags, false, sel, ca, cb);
@@ -207,15 +228,14 @@ __global__ void __launch_bounds__(64) k_b3_chunks(const uint8_t* __restrict__ ms
     cvs[ci * 8 + 4 + q] = cb;
 }
 
-static inline size_t nchunks_of(size_t len) { return len == 0 ? 1 : (len + 1023) / 1024; }
 
-void launch_b3_chunks(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags,
-                      uint32_t* cvs, hipStream_t s) {
+static void launch_b3_chunks(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags,
+                             uint32_t* cvs, hipStream_t s, const Jobs& jb, const uint64_t* dlen) {
     KeyW k;
     for (int i = 0; i < 8; ++i) k.w[i] = key[i];
     const size_t n = nchunks_of(len);
-    hipLaunchKernelGGL(k_b3_chunks, dim3((unsigned)((n + 15) / 16)), dim3(64), 0, s, msg, len, k, flags,
-                       cvs, n);
+    hipLaunchKernelGGL(k_b3_chunks, dim3((unsigned)((n + 15) / 16), jb.n), dim3(64), 0, s, msg, len, k, flags,
+                       cvs, n, jb.stride, dlen);
 }
 
 // Pairwise bottom-up merge of the n CVs in `a` (LDS), one quad per parent node, ping-ponging
@@ -251,10 +271,20 @@ __device__ uint32_t* merge_lds(uint32_t* a, uint32_t* b, uint32_t n, const uint3
     return a;
 }
 
+// With `dlen`, `n` is the capacity's chunk count and each job uses its own; a job whose chunk
+// values fit one group has no group level and skips.
 __global__ void __launch_bounds__(1024) k_b3_group_merge(const uint32_t* __restrict__ cvs, size_t n, KeyW key,
-                                                         uint32_t flags, uint32_t* __restrict__ out) {
+                                                         uint32_t flags, uint32_t* __restrict__ out, size_t js,
+                                                         const uint64_t* dlen, size_t cap_len) {
     __shared__ uint32_t sa[1024 * 8], sb[512 * 8 + 8];
     const size_t base = (size_t)blockIdx.x * 1024;
+    const size_t jo = job_off(js);
+    if (dlen) {
+        n = nchunks_of(job_len(cap_len, dlen, jo));
+        if (n <= 1024 || base >= n) return;  // uniform over the workgroup
+    }
+    cvs = jp(cvs, jo);
+    out = jp(out, jo);
     const uint32_t cnt = (uint32_t)((n - base) < 1024 ? (n - base) : 1024);
     for (uint32_t i = threadIdx.x; i < cnt * 8; i += blockDim.x) sa[i] = cvs[base * 8 + i];
     __syncthreads();
@@ -262,12 +292,12 @@ __global__ void __launch_bounds__(1024) k_b3_group_merge(const uint32_t* __restr
     if (threadIdx.x < 8) out[blockIdx.x * 8 + threadIdx.x] = r[threadIdx.x];
 }
 
-void launch_b3_group_merge(const uint32_t* cvs, size_t n, const uint32_t key[8], uint32_t flags,
-                           uint32_t* out, hipStream_t s) {
+static void launch_b3_group_merge(const uint32_t* cvs, size_t n, const uint32_t key[8], uint32_t flags,
+                                  uint32_t* out, hipStream_t s, const Jobs& jb, const uint64_t* dlen, size_t cap_len) {
     KeyW k;
     for (int i = 0; i < 8; ++i) k.w[i] = key[i];
-    hipLaunchKernelGGL(k_b3_group_merge, dim3((unsigned)((n + 1023) / 1024)), dim3(1024), 0, s, cvs, n, k,
-                       flags, out);
+    hipLaunchKernelGGL(k_b3_group_merge, dim3((unsigned)((n + 1023) / 1024), jb.n), dim3(1024), 0, s, cvs, n, k,
+                       flags, out, jb.stride, dlen, cap_len);
 }
 
 // modular inverse in Z_q by the binary extended Euclid algorithm (same value as x^(q-2))
@@ -452,14 +482,35 @@ __device__ void write_challenge_pair(zq* chal_out, const zq& x) {
     chal_out[1] = zq_inv_fermat(x);
 }
 
-// Root merge; one workgroup of 1024 threads over n <= 1024 CVs. `chunk_level` says the CVs
-// are chunk CVs (needed to redo the counter chunks of a zero challenge).
+// Root merge; one workgroup of 1024 threads per job over n <= 1024 CVs. `chunk_level` says the
+// CVs are chunk CVs (needed to redo the counter chunks of a zero challenge). With `dlen` the job's
+// own length picks the level: chunk values at `cvs` (<= 1024 chunks) or group values at
+// cvs + l1_off.
 __global__ void __launch_bounds__(1024) k_b3_root(uint32_t* __restrict__ cvs, size_t n, uint8_t* msg, size_t len,
                                                   KeyW key, uint32_t flags, uint32_t* digest_out, zq* chal_out,
-                                                  uint32_t* err, int chunk_level) {
+                                                  uint32_t* err, int chunk_level, size_t js, const uint64_t* dlen,
+                                                  size_t l1_off) {
     __shared__ uint32_t sa[1024 * 8], sb[512 * 8 + 8];
     __shared__ uint32_t dig[8];
     __shared__ int again;
+    const size_t jo = job_off(js);
+    cvs = jp(cvs, jo);
+    msg = jp(msg, jo);
+    digest_out = jp(digest_out, jo);
+    chal_out = jp(chal_out, jo);
+    err = jp(err, jo);
+    if (dlen) {
+        len = job_len(len, dlen, jo);
+        const size_t nj = nchunks_of(len);
+        if (nj > 1024) {
+            cvs += l1_off;
+            n = (nj + 1023) / 1024;
+            chunk_level = 0;
+        } else {
+            n = nj;
+            chunk_level = 1;
+        }
+    }
     const uint32_t q = threadIdx.x & 3;
     const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
     uint32_t counter = 0;  // used by thread 0
@@ -518,32 +569,40 @@ __global__ void __launch_bounds__(1024) k_b3_root(uint32_t* __restrict__ cvs, si
     }
 }
 
-void launch_b3_root(const uint32_t* cvs, size_t n, const uint8_t* msg, size_t len,
-                    const uint32_t key[8], uint32_t flags, uint32_t* digest_out, zq* chal_out,
-                    uint32_t* err_flags, int chunk_level, hipStream_t s) {
+static void launch_b3_root(const uint32_t* cvs, size_t n, const uint8_t* msg, size_t len, const uint32_t key[8],
+                           uint32_t flags, uint32_t* digest_out, zq* chal_out, uint32_t* err_flags, int chunk_level,
+                           hipStream_t s, const Jobs& jb, const uint64_t* dlen, size_t l1_off) {
     KeyW k;
     for (int i = 0; i < 8; ++i) k.w[i] = key[i];
-    hipLaunchKernelGGL(k_b3_root, dim3(1), dim3(1024), 0, s, const_cast<uint32_t*>(cvs), n,
-                       const_cast<uint8_t*>(msg), len, k, flags, digest_out, chal_out, err_flags, chunk_level);
+    hipLaunchKernelGGL(k_b3_root, dim3(1, jb.n), dim3(1024), 0, s, const_cast<uint32_t*>(cvs), n,
+                       const_cast<uint8_t*>(msg), len, k, flags, digest_out, chal_out, err_flags, chunk_level,
+                       jb.stride, dlen, l1_off);
 }
 
 // Hash a device buffer: chunk CVs, group merges while more than 1024 values remain, root.
 void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags,
                  uint32_t* cv_scratch, size_t cv_scratch_words, uint32_t* digest_out, zq* chal_out,
-                 uint32_t* err_flags, hipStream_t s) {
+                 uint32_t* err_flags, hipStream_t s, const Jobs& jb, const uint64_t* dlen) {
+    if (jb.n == 0) return;
     const size_t n = nchunks_of(len);
-    launch_b3_chunks(msg, len, key, flags, cv_scratch, s);
+    (void)cv_scratch_words;
+    launch_b3_chunks(msg, len, key, flags, cv_scratch, s, jb, dlen);
+    if (dlen) {  // per-job lengths: at most one group level
+        if (n > 1024) launch_b3_group_merge(cv_scratch, n, key, flags, cv_scratch + n * 8, s, jb, dlen, len);
+        launch_b3_root(cv_scratch, n, msg, len, key, flags, digest_out, chal_out, err_flags, 1, s, jb, dlen, n * 8);
+        return;
+    }
     uint32_t* cur = cv_scratch;
     size_t cnt = n;
     uint32_t* next = cv_scratch + n * 8;
     while (cnt > 1024) {
-        launch_b3_group_merge(cur, cnt, key, flags, next, s);
+        launch_b3_group_merge(cur, cnt, key, flags, next, s, jb, nullptr, 0);
         cnt = (cnt + 1023) / 1024;
         cur = next;
         next = next + cnt * 8;
     }
-    (void)cv_scratch_words;
-    launch_b3_root(cur, cnt, msg, len, key, flags, digest_out, chal_out, err_flags, cur == cv_scratch, s);
+    launch_b3_root(cur, cnt, msg, len, key, flags, digest_out, chal_out, err_flags, cur == cv_scratch, s, jb, nullptr,
+                   0);
 }
 
 size_t device_hash_scratch_words(size_t len) {

@@ -43,58 +43,80 @@ enum RowMode : int {
 };
 
 struct RowSrc {
-    const zq* a;       // source rows (ROW_GEN: unused)
-    zq* dst;           // when non-null the Z_q row is stored here (row-major, m per row)
-    size_t b_off;      // ROW_VFOLD partner offset in rows
-    const zq* scal;    // ROW_*FOLD scalar (device)
+    const zq* a;            // source rows (ROW_GEN: unused)
+    zq* dst;                // when non-null the Z_q row is stored here (row-major, m per row)
+    size_t b_off;           // ROW_VFOLD partner offset in rows
+    const zq* scal;         // ROW_*FOLD scalar (device)
     int mode;
-    uint32_t gseed[8]; // ROW_GEN key words
+    const uint32_t* gseed;  // ROW_GEN key words (device, 8 words)
 };
+
+// A batch of jobs of one shape. Job b's device buffers sit at byte offset b * stride from job
+// 0's, so a kernel handles job blockIdx.y by offsetting every per-job pointer it was given
+// (null pointers stay null). Tables and constants are shared and never offset.
+struct Jobs {
+    uint32_t n = 1;
+    size_t stride = 0;
+};
+
+template <class T>
+__device__ __forceinline__ T* jp(T* p, size_t jo) {
+    return p ? reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + jo) : p;
+}
+__device__ __forceinline__ size_t job_off(size_t js) { return (size_t)blockIdx.y * js; }
 
 void set_device_constants(const PrimeConsts* pc, const CrtConsts* crt, hipStream_t s);
 
 // forward negacyclic NTT of `rows` rows of degree 2^log_m, all 8 primes; out[(row*8+p)*m + c]
 void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out,
-                         const NttTables& tw, hipStream_t s);
+                         const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
 // inverse NTT in place on `slices` slices of [8][m]: data[(slice*8 + p)*m + c], times m^-1
-void launch_ntt_inv(uint32_t* data, size_t slices, int log_m, const NttTables& tw, hipStream_t s);
+void launch_ntt_inv(uint32_t* data, size_t slices, int log_m, const NttTables& tw, hipStream_t s,
+                    const Jobs& jb = Jobs{});
 // inv = sum over all rows W[i]*G[i,k]; L = sum_{i<h} G[i+h,k]*W[i]; R = sum_{i<h} G[i,k]*W[i+h]
 // (h = rows/2; L/R skipped when out_L == nullptr). Outputs [k][8][m] residues.
 // `part` is scratch of mac_part_words(log_m) words: rows are split into segments whose partial
 // sums a second kernel adds.
 void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
-                      uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, uint32_t* part, hipStream_t s);
+                      uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, uint32_t* part, hipStream_t s,
+                      const Jobs& jb = Jobs{});
 // vector-stage invariant at degree m plus the poly-stage cross terms at degree m/2, from the
 // NTT-domain even/odd split: out_inv [k][8][m], out_L/out_R [k][8][m/2]
 void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
                       uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, uint32_t* part,
-                      const NttTables& tw, hipStream_t s);
+                      const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
 // scratch words the MAC passes need at degrees up to 2^max_log_m
 size_t mac_part_words(int max_log_m);
 // Z_q rows only (no transform): dst[row][c] = row_value(src, row, c)
-void launch_fold_rows_zq(const RowSrc& src, size_t rows, int log_m, hipStream_t s);
+void launch_fold_rows_zq(const RowSrc& src, size_t rows, int log_m, hipStream_t s, const Jobs& jb = Jobs{});
 void launch_final_check(const zq* g, const zq* c, const zq* opening, size_t n, uint32_t* flags,
-                        uint32_t bit, hipStream_t s);
+                        uint32_t bit, hipStream_t s, const Jobs& jb = Jobs{});
 // residues [slice][8][m] -> Z_q [slice][m]
-void launch_crt_lift(const uint32_t* res, zq* out, size_t slices, int log_m, hipStream_t s);
+void launch_crt_lift(const uint32_t* res, zq* out, size_t slices, int log_m, hipStream_t s,
+                     const Jobs& jb = Jobs{});
 
 // commitment updates (tiny, K x m)
 // out = c + sinv*l + s*r          (VEC; scalars at scal[0]=s, scal[1]=s^-1)
 // out = even(c) + sinv*l + s*r    (POLY; c has 2m coefficients per coordinate)
 void launch_com_update(const zq* c, const zq* l, const zq* r, const zq* scal, int poly, int log_m,
-                       zq* out, hipStream_t s);
+                       zq* out, hipStream_t s, const Jobs& jb = Jobs{});
 // fill n 32-bit words (graph-safe replacement for memset)
-void launch_fill32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+void launch_fill32(uint32_t* p, size_t n, uint32_t v, hipStream_t s, const Jobs& jb = Jobs{});
 // plain 16-byte copy (HBM calibration, graph-safe device copies)
-void launch_copy16(void* dst, const void* src, size_t n16, hipStream_t s);
+void launch_copy16(void* dst, const void* src, size_t n16, hipStream_t s, const Jobs& jb = Jobs{});
 // out = a + b over n Z_q coefficients
 void launch_zq_add(const zq* a, const zq* b, zq* out, size_t n, hipStream_t s);
 // flags[bit] set when a != b over n coefficients
 void launch_com_check(const zq* a, const zq* b, size_t n, uint32_t* flags, uint32_t bit,
-                      hipStream_t s);
+                      hipStream_t s, const Jobs& jb = Jobs{});
+
+// copy each job's file from the device address in its header (u64 len, u64 src) into dst
+void launch_copy_in(const uint64_t* meta, uint8_t* dst, size_t cap, hipStream_t s, const Jobs& jb);
 
 // witness expansion (src/utils/io.rs:51-54): 13-byte chunks into a dim x dim Z_q matrix
-void launch_expand(const uint8_t* file, size_t len, zq* out, size_t capacity, hipStream_t s);
+// With `dlen` (device, per job) the length is read there and `len` only bounds it.
+void launch_expand(const uint8_t* file, size_t len, zq* out, size_t capacity, hipStream_t s,
+                   const Jobs& jb = Jobs{}, const uint64_t* dlen = nullptr);
 
 // ---- BLAKE3 tree hashing on device ----
 struct TranscriptMsg {
@@ -107,27 +129,22 @@ struct TranscriptMsg {
     uint32_t lens[3];
 };
 size_t transcript_len(uint32_t label_len, const uint32_t lens[3]);
-void launch_transcript_pack(const TranscriptMsg& t, uint8_t* msg, size_t len, hipStream_t s);
-void launch_b3_chunks(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags,
-                      uint32_t* cvs, hipStream_t s);
-// merge groups of 1024 chunk CVs into one CV each (non-root)
-void launch_b3_group_merge(const uint32_t* cvs, size_t n, const uint32_t key[8], uint32_t flags,
-                           uint32_t* out, hipStream_t s);
-// final merge to the root. digest_out (8 words) when non-null. For a challenge
-// (chal_out != nullptr): LE256(digest) mod q, re-hashing with counter+1 while zero, then
-// chal_out[0] = x, chal_out[1] = x^-1. msg/len are needed for n_chunks == 1 and retries.
-void launch_b3_root(const uint32_t* cvs, size_t n, const uint8_t* msg, size_t len,
-                    const uint32_t key[8], uint32_t flags, uint32_t* digest_out, zq* chal_out,
-                    uint32_t* err_flags, int chunk_level, hipStream_t s);
-// whole-buffer hash (chunks, group merges, root); cv_scratch holds device_hash_scratch_words
+void launch_transcript_pack(const TranscriptMsg& t, uint8_t* msg, size_t len, hipStream_t s,
+                            const Jobs& jb = Jobs{});
+// Whole-buffer BLAKE3 (chunks, group merges, root). cv_scratch holds device_hash_scratch_words.
+// digest_out (8 words) when non-null. For a challenge (chal_out != nullptr): LE256(digest) mod q,
+// re-hashed with counter+1 while zero, then chal_out[0] = x, chal_out[1] = x^-1.
+// With `dlen` (device, per job) each job's length is read there and `len` is the capacity that
+// sizes the launch; at most two tree levels (lengths up to 1 GiB) are supported then.
 void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags,
                  uint32_t* cv_scratch, size_t cv_scratch_words, uint32_t* digest_out, zq* chal_out,
-                 uint32_t* err_flags, hipStream_t s);
+                 uint32_t* err_flags, hipStream_t s, const Jobs& jb = Jobs{}, const uint64_t* dlen = nullptr);
 size_t device_hash_scratch_words(size_t len);
 
 // generator expansion into Z_q rows: g[(i*K + k)*m + c] (src/commitment/sis.rs:212-245)
-void launch_gen_generators(const uint32_t gseed[8], size_t rows, int log_m, zq* out,
-                           hipStream_t s);
+// gseed: 8 key words in device memory
+void launch_gen_generators(const uint32_t* gseed, size_t rows, int log_m, zq* out, hipStream_t s,
+                           const Jobs& jb = Jobs{});
 
 // the reference-compatible batch entry points
 void launch_fold_pairs(const uint64_t* l_lo, const uint64_t* l_hi, const uint64_t* r_lo,

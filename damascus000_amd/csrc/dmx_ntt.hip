@@ -34,7 +34,7 @@ __device__ __forceinline__ void store_zq(zq* p, const zq& x) {
 
 // keyed BLAKE3 of "g" || u64le(i) || u64le(k) || u64le(c) (25 bytes, one block), reduced mod q
 // (src/commitment/sis.rs:231-245)
-__device__ __forceinline__ zq gen_coeff(const uint32_t key[8], uint64_t i, uint32_t k, uint64_t c) {
+__device__ __forceinline__ zq gen_coeff(const uint32_t* key, uint64_t i, uint32_t k, uint64_t c) {
     uint32_t m[16];
     m[0] = 0x67u | ((uint32_t)i << 8);
     m[1] = (uint32_t)(i >> 24);
@@ -53,7 +53,8 @@ __device__ __forceinline__ zq gen_coeff(const uint32_t key[8], uint64_t i, uint3
 }
 
 // produce coefficient c of row `row` per the row source
-__device__ __forceinline__ zq row_value(const RowSrc& src, size_t row, uint32_t c, int log_m) {
+// `key` holds the ROW_GEN key words (registers or device memory)
+__device__ __forceinline__ zq row_value(const RowSrc& src, const uint32_t* key, size_t row, uint32_t c, int log_m) {
     const size_t m = (size_t)1 << log_m;
     switch (src.mode) {
         case ROW_PLAIN:
@@ -78,7 +79,7 @@ __device__ __forceinline__ zq row_value(const RowSrc& src, size_t row, uint32_t 
         default: {  // ROW_GEN
             uint64_t i = row / K;
             uint32_t k = (uint32_t)(row % K);
-            return gen_coeff(src.gseed, i, k, c);
+            return gen_coeff(key, i, k, c);
         }
     }
 }
@@ -89,15 +90,26 @@ __device__ __forceinline__ zq row_value(const RowSrc& src, size_t row, uint32_t 
 // ---------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(RowSrc src, int log_m,
                                                                uint32_t* __restrict__ out,
-                                                               NttTables tw) {
+                                                               NttTables tw, size_t js) {
     extern __shared__ uint32_t sh[];
+    const size_t jo = job_off(js);
+    src.a = jp(src.a, jo);
+    src.dst = jp(src.dst, jo);
+    src.scal = jp(src.scal, jo);
+    src.gseed = jp(src.gseed, jo);
+    out = jp(out, jo);
+    uint32_t key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (src.mode == ROW_GEN) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) key[i] = src.gseed[i];
+    }
     const int m = 1 << log_m;
     const size_t row = blockIdx.x;
     const int pg = min(NPRIMES, LDS_WORDS >> log_m);  // primes per LDS pass
     for (int p0 = 0; p0 < NPRIMES; p0 += pg) {
         // 1) produce the row and reduce it into the primes of this group
         for (int c = threadIdx.x; c < m; c += NTT_THREADS) {
-            zq x = row_value(src, row, c, log_m);
+            zq x = row_value(src, key, row, c, log_m);
             if (src.dst && p0 == 0) store_zq(src.dst + row * m + c, x);
 #pragma unroll 1
             for (int q = 0; q < pg; ++q) sh[q * m + c] = zq_mod_p_centred(x, c_pc[p0 + q]);
@@ -135,21 +147,22 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(RowSrc src, int lo
 }
 
 void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out,
-                         const NttTables& tw, hipStream_t s) {
-    if (rows == 0) return;
+                         const NttTables& tw, hipStream_t s, const Jobs& jb) {
+    if (rows == 0 || jb.n == 0) return;
     const int m = 1 << log_m;
     const int pg = std::min(NPRIMES, LDS_WORDS >> log_m);
     const size_t lds = (size_t)pg * m * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_ntt_fwd_rows, dim3((unsigned)rows), dim3(NTT_THREADS), lds, s, src, log_m,
-                       out, tw);
+    hipLaunchKernelGGL(k_ntt_fwd_rows, dim3((unsigned)rows, jb.n), dim3(NTT_THREADS), lds, s, src, log_m,
+                       out, tw, jb.stride);
 }
 
 // ---------------------------------------------------------------------------------------
 // inverse (Gentleman-Sande): (x, y) -> (x + y, (x - y) * zinv), then * m^-1
 // ---------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(NTT_THREADS) k_ntt_inv(uint32_t* __restrict__ data, int log_m,
-                                                         NttTables tw) {
+                                                         NttTables tw, size_t js) {
     extern __shared__ uint32_t sh[];
+    data = jp(data, job_off(js));
     const int m = 1 << log_m;
     const size_t slice = blockIdx.x;  // (slice, prime) pairs: blockIdx.x = slice*8 + p
     const int pi = (int)(slice % NPRIMES);
@@ -176,11 +189,12 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_inv(uint32_t* __restrict__ 
     for (int c = threadIdx.x; c < m; c += NTT_THREADS) d[c] = mulc(sh[c], im, imp, p);
 }
 
-void launch_ntt_inv(uint32_t* data, size_t slices, int log_m, const NttTables& tw, hipStream_t s) {
-    if (slices == 0) return;
+void launch_ntt_inv(uint32_t* data, size_t slices, int log_m, const NttTables& tw, hipStream_t s,
+                    const Jobs& jb) {
+    if (slices == 0 || jb.n == 0) return;
     const size_t lds = ((size_t)1 << log_m) * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_ntt_inv, dim3((unsigned)(slices * NPRIMES)), dim3(NTT_THREADS), lds, s, data,
-                       log_m, tw);
+    hipLaunchKernelGGL(k_ntt_inv, dim3((unsigned)(slices * NPRIMES), jb.n), dim3(NTT_THREADS), lds, s, data,
+                       log_m, tw, jb.stride);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -208,10 +222,16 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_pairs(const uint32_t* __res
                                                            uint32_t rows, int log_m,
                                                            uint32_t* __restrict__ out_inv,
                                                            uint32_t* __restrict__ out_L,
-                                                           uint32_t* __restrict__ out_R) {
+                                                           uint32_t* __restrict__ out_R, size_t js) {
     const size_t m = (size_t)1 << log_m;
     const size_t gid = (size_t)blockIdx.x * MAC_THREADS + threadIdx.x;
     if (gid >= (size_t)K * NPRIMES * m) return;
+    const size_t jo = job_off(js);
+    What = jp(What, jo);
+    Ghat = jp(Ghat, jo);
+    out_inv = jp(out_inv, jo);
+    out_L = jp(out_L, jo);
+    out_R = jp(out_R, jo);
     const uint32_t c = (uint32_t)(gid & (m - 1));
     const uint32_t p = (uint32_t)((gid >> log_m) % NPRIMES);
     const uint32_t k = (uint32_t)((gid >> log_m) / NPRIMES);
@@ -260,11 +280,17 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_split(const uint32_t* __res
                                                            uint32_t* __restrict__ out_inv,
                                                            uint32_t* __restrict__ out_L,
                                                            uint32_t* __restrict__ out_R,
-                                                           NttTables tw) {
+                                                           NttTables tw, size_t js) {
     const size_t m = (size_t)1 << log_m;
     const size_t h = m >> 1;
     const size_t gid = (size_t)blockIdx.x * MAC_THREADS + threadIdx.x;
     if (gid >= (size_t)K * NPRIMES * h) return;
+    const size_t jo = job_off(js);
+    What = jp(What, jo);
+    Ghat = jp(Ghat, jo);
+    out_inv = jp(out_inv, jo);
+    out_L = jp(out_L, jo);
+    out_R = jp(out_R, jo);
     const uint32_t t = (uint32_t)(gid & (h - 1));
     const uint32_t p = (uint32_t)((gid >> (log_m - 1)) % NPRIMES);
     const uint32_t k = (uint32_t)((gid >> (log_m - 1)) / NPRIMES);
@@ -322,11 +348,15 @@ __device__ __forceinline__ uint4 ld4(const uint32_t* p) { return *reinterpret_ca
 __global__ void __launch_bounds__(MAC_THREADS) k_mac_pairs4(const uint32_t* __restrict__ What,
                                                             const uint32_t* __restrict__ Ghat, uint32_t rows,
                                                             int log_m, uint32_t nseg, uint32_t seg_rows, int cross,
-                                                            uint32_t* __restrict__ part) {
+                                                            uint32_t* __restrict__ part, size_t js) {
     const size_t m = (size_t)1 << log_m;
     const size_t per_seg = (size_t)K * NPRIMES * m / 4;
     const size_t gid = (size_t)blockIdx.x * MAC_THREADS + threadIdx.x;
     if (gid >= (size_t)nseg * per_seg) return;
+    const size_t jo = job_off(js);
+    What = jp(What, jo);
+    Ghat = jp(Ghat, jo);
+    part = jp(part, jo);
     const uint32_t seg = (uint32_t)(gid / per_seg);
     const size_t r = gid - (size_t)seg * per_seg;
     const uint32_t c = (uint32_t)((r * 4) & (m - 1));
@@ -387,21 +417,27 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_pairs4(const uint32_t* __re
 // sum the segment partials: out[slot][k][p][c] for nslots slots
 __global__ void k_mac_pairs_reduce(const uint32_t* __restrict__ part, uint32_t nseg, int log_m, int nslots,
                                    uint32_t* __restrict__ out_inv, uint32_t* __restrict__ out_L,
-                                   uint32_t* __restrict__ out_R) {
+                                   uint32_t* __restrict__ out_R, size_t js) {
     const size_t m = (size_t)1 << log_m;
     const size_t slot = (size_t)K * NPRIMES * m;
     const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= slot * nslots) return;
+    const size_t jo = job_off(js);
+    part = jp(part, jo);
+    out_inv = jp(out_inv, jo);
+    out_L = jp(out_L, jo);
+    out_R = jp(out_R, jo);
     const size_t sl = gid / slot, e = gid - sl * slot;
-    const uint32_t P = c_pc[(e >> log_m) % NPRIMES].p;
+    const PrimeConsts pc = c_pc[(e >> log_m) % NPRIMES];
     uint64_t acc = 0;
     for (uint32_t sg = 0; sg < nseg; ++sg) acc += part[(size_t)sg * 3 * slot + sl * slot + e];
     uint32_t* o = sl == 0 ? out_inv : (sl == 1 ? out_L : out_R);
-    o[e] = (uint32_t)(acc % P);
+    o[e] = reduce_u96(acc, 0, pc);
 }
 
 void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
-                      uint32_t* out_L, uint32_t* out_R, uint32_t* part, hipStream_t s) {
+                      uint32_t* out_L, uint32_t* out_R, uint32_t* part, hipStream_t s, const Jobs& jb) {
+    if (jb.n == 0) return;
     const bool cross = out_L != nullptr && rows > 1;
     const size_t half = cross ? rows / 2 : rows;
     const uint32_t nseg = log_m >= 2 ? mac_segments(half, log_m) : 1;
@@ -409,17 +445,19 @@ void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, i
     const size_t m = (size_t)1 << log_m;
     if (log_m < 2) {  // degrees 1 and 2: the scalar kernel
         const size_t n = (size_t)K * NPRIMES << log_m;
-        hipLaunchKernelGGL(k_mac_pairs, dim3((unsigned)((n + MAC_THREADS - 1) / MAC_THREADS)), dim3(MAC_THREADS), 0,
-                           s, What, Ghat, (uint32_t)rows, log_m, out_inv, out_L, out_R);
+        hipLaunchKernelGGL(k_mac_pairs, dim3((unsigned)((n + MAC_THREADS - 1) / MAC_THREADS), jb.n),
+                           dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, out_inv, out_L, out_R,
+                           jb.stride);
         return;
     }
     const size_t threads = (size_t)nseg * K * NPRIMES * m / 4;
-    hipLaunchKernelGGL(k_mac_pairs4, dim3((unsigned)((threads + MAC_THREADS - 1) / MAC_THREADS)), dim3(MAC_THREADS),
-                       0, s, What, Ghat, (uint32_t)rows, log_m, nseg, seg_rows, (int)cross, part);
+    hipLaunchKernelGGL(k_mac_pairs4, dim3((unsigned)((threads + MAC_THREADS - 1) / MAC_THREADS), jb.n),
+                       dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, nseg, seg_rows, (int)cross, part,
+                       jb.stride);
     const int nslots = cross ? 3 : 1;
     const size_t n = (size_t)K * NPRIMES * m * nslots;
-    hipLaunchKernelGGL(k_mac_pairs_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nseg, log_m,
-                       nslots, out_inv, out_L, out_R);
+    hipLaunchKernelGGL(k_mac_pairs_reduce, dim3((unsigned)((n + 255) / 256), jb.n), dim3(256), 0, s, part, nseg,
+                       log_m, nslots, out_inv, out_L, out_R, jb.stride);
 }
 
 // Vector-stage invariant (degree m) and poly-stage cross terms (degree m/2) in one pass;
@@ -428,11 +466,15 @@ void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, i
 __global__ void __launch_bounds__(MAC_THREADS) k_mac_split4(const uint32_t* __restrict__ What,
                                                             const uint32_t* __restrict__ Ghat, uint32_t rows,
                                                             int log_m, uint32_t nseg, uint32_t seg_rows,
-                                                            uint32_t* __restrict__ part) {
+                                                            uint32_t* __restrict__ part, size_t js) {
     const size_t m = (size_t)1 << log_m, h = m >> 1;
     const size_t per_seg = (size_t)K * NPRIMES * m / 4;
     const size_t gid = (size_t)blockIdx.x * MAC_THREADS + threadIdx.x;
     if (gid >= (size_t)nseg * per_seg) return;
+    const size_t jo = job_off(js);
+    What = jp(What, jo);
+    Ghat = jp(Ghat, jo);
+    part = jp(part, jo);
     const uint32_t seg = (uint32_t)(gid / per_seg);
     const size_t r = gid - (size_t)seg * per_seg;
     const uint32_t c = (uint32_t)((r * 4) & (m - 1));  // positions c..c+3 = pairs t, t+1
@@ -472,20 +514,26 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_split4(const uint32_t* __re
 
 __global__ void k_mac_split_reduce(const uint32_t* __restrict__ part, uint32_t nseg, int log_m,
                                    uint32_t* __restrict__ out_inv, uint32_t* __restrict__ out_L,
-                                   uint32_t* __restrict__ out_R, NttTables tw) {
+                                   uint32_t* __restrict__ out_R, NttTables tw, size_t js) {
     const size_t m = (size_t)1 << log_m, h = m >> 1;
     const size_t slot = (size_t)K * NPRIMES * h;
     const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= slot) return;
+    const size_t jo = job_off(js);
+    part = jp(part, jo);
+    out_inv = jp(out_inv, jo);
+    out_L = jp(out_L, jo);
+    out_R = jp(out_R, jo);
     const uint32_t t = (uint32_t)(gid & (h - 1));
     const uint32_t kp = (uint32_t)(gid >> (log_m - 1));
     const uint32_t p = kp % NPRIMES;
-    const uint32_t P = c_pc[p].p;
+    const PrimeConsts pc = c_pc[p];
+    const uint32_t P = pc.p;
     uint64_t a[4] = {0, 0, 0, 0};
     for (uint32_t sg = 0; sg < nseg; ++sg)
         for (int j = 0; j < 4; ++j) a[j] += part[(size_t)sg * 4 * slot + j * slot + gid];
-    const uint32_t a00 = (uint32_t)(a[0] % P), a11 = (uint32_t)(a[1] % P), a01 = (uint32_t)(a[2] % P),
-                   a10 = (uint32_t)(a[3] % P);
+    const uint32_t a00 = reduce_u96(a[0], 0, pc), a11 = reduce_u96(a[1], 0, pc), a01 = reduce_u96(a[2], 0, pc),
+                   a10 = reduce_u96(a[3], 0, pc);
     const size_t o = (size_t)kp * m + 2 * t;
     out_inv[o] = a00;
     out_inv[o + 1] = a11;
@@ -501,22 +549,25 @@ __global__ void k_mac_split_reduce(const uint32_t* __restrict__ part, uint32_t n
 }
 
 void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
-                      uint32_t* out_L, uint32_t* out_R, uint32_t* part, const NttTables& tw, hipStream_t s) {
+                      uint32_t* out_L, uint32_t* out_R, uint32_t* part, const NttTables& tw, hipStream_t s,
+                      const Jobs& jb) {
+    if (jb.n == 0) return;
     if (log_m < 2) {  // degree 2: the scalar kernel
         const size_t n = (size_t)K * NPRIMES << (log_m - 1);
-        hipLaunchKernelGGL(k_mac_split, dim3((unsigned)((n + MAC_THREADS - 1) / MAC_THREADS)), dim3(MAC_THREADS), 0,
-                           s, What, Ghat, (uint32_t)rows, log_m, out_inv, out_L, out_R, tw);
+        hipLaunchKernelGGL(k_mac_split, dim3((unsigned)((n + MAC_THREADS - 1) / MAC_THREADS), jb.n),
+                           dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, out_inv, out_L, out_R, tw,
+                           jb.stride);
         return;
     }
     const uint32_t nseg = mac_segments(rows * 2, log_m);
     const uint32_t seg_rows = (uint32_t)((rows + nseg - 1) / nseg);
     const size_t m = (size_t)1 << log_m;
     const size_t threads = (size_t)nseg * K * NPRIMES * m / 4;
-    hipLaunchKernelGGL(k_mac_split4, dim3((unsigned)((threads + MAC_THREADS - 1) / MAC_THREADS)), dim3(MAC_THREADS),
-                       0, s, What, Ghat, (uint32_t)rows, log_m, nseg, seg_rows, part);
+    hipLaunchKernelGGL(k_mac_split4, dim3((unsigned)((threads + MAC_THREADS - 1) / MAC_THREADS), jb.n),
+                       dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, nseg, seg_rows, part, jb.stride);
     const size_t n = (size_t)K * NPRIMES * (m / 2);
-    hipLaunchKernelGGL(k_mac_split_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nseg, log_m,
-                       out_inv, out_L, out_R, tw);
+    hipLaunchKernelGGL(k_mac_split_reduce, dim3((unsigned)((n + 255) / 256), jb.n), dim3(256), 0, s, part, nseg,
+                       log_m, out_inv, out_L, out_R, tw, jb.stride);
 }
 
 size_t mac_part_words(int max_log_m) {
@@ -532,9 +583,11 @@ size_t mac_part_words(int max_log_m) {
 // x = sum v_i * (p_0..p_{i-1}); if x > floor(P/2) the result is (x - P) mod q.
 // ---------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_crt_lift(const uint32_t* __restrict__ res,
-                                                  zq* __restrict__ out, size_t total, int log_m) {
+                                                  zq* __restrict__ out, size_t total, int log_m, size_t js) {
     const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= total) return;
+    res = jp(res, job_off(js));
+    out = jp(out, job_off(js));
     const size_t m = (size_t)1 << log_m;
     const size_t slice = gid >> log_m, c = gid & (m - 1);
     uint32_t r[NPRIMES], v[NPRIMES];
@@ -578,52 +631,68 @@ __global__ void __launch_bounds__(256) k_crt_lift(const uint32_t* __restrict__ r
     store_zq(out + gid, xq);
 }
 
-void launch_crt_lift(const uint32_t* res, zq* out, size_t slices, int log_m, hipStream_t s) {
+void launch_crt_lift(const uint32_t* res, zq* out, size_t slices, int log_m, hipStream_t s, const Jobs& jb) {
     const size_t total = slices << log_m;
-    if (!total) return;
-    hipLaunchKernelGGL(k_crt_lift, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, res, out,
-                       total, log_m);
+    if (!total || jb.n == 0) return;
+    hipLaunchKernelGGL(k_crt_lift, dim3((unsigned)((total + 255) / 256), jb.n), dim3(256), 0, s, res, out,
+                       total, log_m, jb.stride);
 }
 
 // Z_q-only row production (the verifier's generator folds, verifier.rs:141-179)
-__global__ void k_fold_rows_zq(RowSrc src, size_t total, int log_m) {
+__global__ void k_fold_rows_zq(RowSrc src, size_t total, int log_m, size_t js) {
     const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= total) return;
+    const size_t jo = job_off(js);
+    src.a = jp(src.a, jo);
+    src.dst = jp(src.dst, jo);
+    src.scal = jp(src.scal, jo);
+    src.gseed = jp(src.gseed, jo);
     const size_t row = gid >> log_m;
     const uint32_t c = (uint32_t)(gid & (((size_t)1 << log_m) - 1));
-    store_zq(src.dst + gid, row_value(src, row, c, log_m));
+    store_zq(src.dst + gid, row_value(src, src.gseed, row, c, log_m));
 }
 
-void launch_fold_rows_zq(const RowSrc& src, size_t rows, int log_m, hipStream_t s) {
+void launch_fold_rows_zq(const RowSrc& src, size_t rows, int log_m, hipStream_t s, const Jobs& jb) {
     const size_t total = rows << log_m;
-    if (!total) return;
-    hipLaunchKernelGGL(k_fold_rows_zq, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, total,
-                       log_m);
+    if (!total || jb.n == 0) return;
+    hipLaunchKernelGGL(k_fold_rows_zq, dim3((unsigned)((total + 255) / 256), jb.n), dim3(256), 0, s, src, total,
+                       log_m, jb.stride);
 }
 
 // final opening (verifier.rs:122-134): flags |= bit when g[i] * opening != c[i]
 __global__ void k_final_check(const zq* g, const zq* c, const zq* opening, size_t n, uint32_t* flags,
-                              uint32_t bit) {
+                              uint32_t bit, size_t js) {
     const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= n) return;
+    const size_t jo = job_off(js);
+    g = jp(g, jo);
+    c = jp(c, jo);
+    opening = jp(opening, jo);
+    flags = jp(flags, jo);
     if (!zq_eq(zq_mul(load_zq(g + gid), *opening), load_zq(c + gid))) atomicOr(flags, bit);
 }
 
 void launch_final_check(const zq* g, const zq* c, const zq* opening, size_t n, uint32_t* flags,
-                        uint32_t bit, hipStream_t s) {
-    if (!n) return;
-    hipLaunchKernelGGL(k_final_check, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, c, opening,
-                       n, flags, bit);
+                        uint32_t bit, hipStream_t s, const Jobs& jb) {
+    if (!n || jb.n == 0) return;
+    hipLaunchKernelGGL(k_final_check, dim3((unsigned)((n + 255) / 256), jb.n), dim3(256), 0, s, g, c, opening,
+                       n, flags, bit, jb.stride);
 }
 
 // ---------------------------------------------------------------------------------------
 // commitment bookkeeping
 // ---------------------------------------------------------------------------------------
 __global__ void k_com_update(const zq* c, const zq* l, const zq* r, const zq* scal, int poly,
-                             int log_m, zq* out) {
+                             int log_m, zq* out, size_t js) {
     const size_t m = (size_t)1 << log_m;
     const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= K * m) return;
+    const size_t jo = job_off(js);
+    c = jp(c, jo);
+    l = jp(l, jo);
+    r = jp(r, jo);
+    scal = jp(scal, jo);
+    out = jp(out, jo);
     const size_t k = gid >> log_m, i = gid & (m - 1);
     const zq s = scal[0], sinv = scal[1];
     const zq base = poly ? load_zq(c + k * (2 * m) + 2 * i) : load_zq(c + gid);
@@ -633,31 +702,59 @@ __global__ void k_com_update(const zq* c, const zq* l, const zq* r, const zq* sc
 }
 
 void launch_com_update(const zq* c, const zq* l, const zq* r, const zq* scal, int poly, int log_m,
-                       zq* out, hipStream_t s) {
+                       zq* out, hipStream_t s, const Jobs& jb) {
     const size_t n = (size_t)K << log_m;
-    hipLaunchKernelGGL(k_com_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c, l, r,
-                       scal, poly, log_m, out);
+    if (jb.n == 0) return;
+    hipLaunchKernelGGL(k_com_update, dim3((unsigned)((n + 255) / 256), jb.n), dim3(256), 0, s, c, l, r,
+                       scal, poly, log_m, out, jb.stride);
 }
 
-__global__ void k_fill32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+__global__ void k_fill32(uint32_t* __restrict__ p, size_t n, uint32_t v, size_t js) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = v;
+    if (i < n) jp(p, job_off(js))[i] = v;
 }
 
-void launch_fill32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
-    if (!n) return;
-    hipLaunchKernelGGL(k_fill32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+void launch_fill32(uint32_t* p, size_t n, uint32_t v, hipStream_t s, const Jobs& jb) {
+    if (!n || jb.n == 0) return;
+    hipLaunchKernelGGL(k_fill32, dim3((unsigned)((n + 255) / 256), jb.n), dim3(256), 0, s, p, n, v, jb.stride);
 }
 
-__global__ void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
+__global__ void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n, size_t js) {
+    dst = jp(dst, job_off(js));
+    src = jp(src, job_off(js));
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
 }
 
-void launch_copy16(void* dst, const void* src, size_t n16, hipStream_t s) {
-    if (!n16) return;
+void launch_copy16(void* dst, const void* src, size_t n16, hipStream_t s, const Jobs& jb) {
+    if (!n16 || jb.n == 0) return;
     const size_t blocks = std::min<size_t>((n16 + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, s, static_cast<uint4*>(dst),
-                       static_cast<const uint4*>(src), n16);
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks, jb.n), dim3(256), 0, s, static_cast<uint4*>(dst),
+                       static_cast<const uint4*>(src), n16, jb.stride);
+}
+
+// Batched file copy-in: job b's header holds (length, source address); the bytes go to the
+// job's file buffer. A zero source means the file is already in place.
+__global__ void k_copy_in(const uint64_t* __restrict__ meta, uint8_t* __restrict__ dst, size_t cap, size_t js) {
+    const size_t jo = job_off(js);
+    meta = jp(meta, jo);
+    dst = jp(dst, jo);
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(meta[1]);
+    const size_t len = (size_t)min((uint64_t)cap, meta[0]);
+    const size_t i = ((size_t)blockIdx.x * 256 + threadIdx.x) * 16;
+    if (!src || i >= len) return;
+    if (i + 16 <= len && ((reinterpret_cast<uintptr_t>(src) & 15) == 0)) {
+        *reinterpret_cast<uint4*>(dst + i) = *reinterpret_cast<const uint4*>(src + i);
+    } else {
+        const size_t e = min(len, i + 16);
+        for (size_t b = i; b < e; ++b) dst[b] = src[b];
+    }
+}
+
+void launch_copy_in(const uint64_t* meta, uint8_t* dst, size_t cap, hipStream_t s, const Jobs& jb) {
+    const size_t threads = (cap + 15) / 16;
+    if (!threads || jb.n == 0) return;
+    hipLaunchKernelGGL(k_copy_in, dim3((unsigned)((threads + 255) / 256), jb.n), dim3(256), 0, s, meta, dst, cap,
+                       jb.stride);
 }
 
 __global__ void k_zq_add(const zq* a, const zq* b, zq* out, size_t n) {
@@ -671,26 +768,34 @@ void launch_zq_add(const zq* a, const zq* b, zq* out, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_zq_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, out, n);
 }
 
-__global__ void k_com_check(const zq* a, const zq* b, size_t n, uint32_t* flags, uint32_t bit) {
+__global__ void k_com_check(const zq* a, const zq* b, size_t n, uint32_t* flags, uint32_t bit, size_t js) {
     const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= n) return;
+    const size_t jo = job_off(js);
+    a = jp(a, jo);
+    b = jp(b, jo);
+    flags = jp(flags, jo);
     if (!zq_eq(load_zq(a + gid), load_zq(b + gid))) atomicOr(flags, bit);
 }
 
 void launch_com_check(const zq* a, const zq* b, size_t n, uint32_t* flags, uint32_t bit,
-                      hipStream_t s) {
-    if (!n) return;
-    hipLaunchKernelGGL(k_com_check, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n,
-                       flags, bit);
+                      hipStream_t s, const Jobs& jb) {
+    if (!n || jb.n == 0) return;
+    hipLaunchKernelGGL(k_com_check, dim3((unsigned)((n + 255) / 256), jb.n), dim3(256), 0, s, a, b, n,
+                       flags, bit, jb.stride);
 }
 
 // ---------------------------------------------------------------------------------------
 // witness expansion (io.rs:51-54) and generator expansion (sis.rs:212-245)
 // ---------------------------------------------------------------------------------------
 __global__ void k_expand(const uint8_t* __restrict__ file, size_t len, zq* __restrict__ out,
-                         size_t capacity) {
+                         size_t capacity, size_t js, const uint64_t* dlen) {
     const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= capacity) return;
+    const size_t jo = job_off(js);
+    file = jp(file, jo);
+    out = jp(out, jo);
+    if (dlen) len = (size_t)min((uint64_t)len, *jp(dlen, jo));
     uint32_t w[4] = {0, 0, 0, 0};
     const size_t off = j * BYTES_PER_COEFF;
     if (off < len) {
@@ -700,30 +805,32 @@ __global__ void k_expand(const uint8_t* __restrict__ file, size_t len, zq* __res
     store_zq(out + j, zq{{w[0], w[1], w[2], w[3]}});  // < 2^104 < q: already canonical
 }
 
-void launch_expand(const uint8_t* file, size_t len, zq* out, size_t capacity, hipStream_t s) {
-    hipLaunchKernelGGL(k_expand, dim3((unsigned)((capacity + 255) / 256)), dim3(256), 0, s, file, len,
-                       out, capacity);
+void launch_expand(const uint8_t* file, size_t len, zq* out, size_t capacity, hipStream_t s, const Jobs& jb,
+                   const uint64_t* dlen) {
+    if (jb.n == 0) return;
+    hipLaunchKernelGGL(k_expand, dim3((unsigned)((capacity + 255) / 256), jb.n), dim3(256), 0, s, file, len,
+                       out, capacity, jb.stride, dlen);
 }
 
-struct Key8 {
-    uint32_t w[8];
-};
-
-__global__ void k_gen_generators(Key8 key, size_t total, int log_m, zq* __restrict__ out) {
+__global__ void k_gen_generators(const uint32_t* gseed, size_t total, int log_m, zq* __restrict__ out,
+                                 size_t js) {
     const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= total) return;
+    const size_t jo = job_off(js);
+    uint32_t key[8];
+    const uint32_t* g = jp(gseed, jo);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) key[i] = g[i];
     const size_t m = (size_t)1 << log_m;
     const size_t row = gid >> log_m, c = gid & (m - 1);
-    store_zq(out + gid, gen_coeff(key.w, row / K, (uint32_t)(row % K), c));
+    store_zq(jp(out, jo) + gid, gen_coeff(key, row / K, (uint32_t)(row % K), c));
 }
 
-void launch_gen_generators(const uint32_t gseed[8], size_t rows, int log_m, zq* out, hipStream_t s) {
-    Key8 k;
-    for (int i = 0; i < 8; ++i) k.w[i] = gseed[i];
+void launch_gen_generators(const uint32_t* gseed, size_t rows, int log_m, zq* out, hipStream_t s, const Jobs& jb) {
     const size_t total = rows << log_m;
-    if (!total) return;
-    hipLaunchKernelGGL(k_gen_generators, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, k,
-                       total, log_m, out);
+    if (!total || jb.n == 0) return;
+    hipLaunchKernelGGL(k_gen_generators, dim3((unsigned)((total + 255) / 256), jb.n), dim3(256), 0, s, gseed,
+                       total, log_m, out, jb.stride);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -283,15 +284,16 @@ static double fwd_row_work(int log_m, int mode) {
 }
 
 // forward transforms go through here so that kernel timing can bracket them
-static void fwd_rows(dmx_ctx* ctx, const RowSrc& src, size_t rows, int log_m, uint32_t* out, hipStream_t s) {
+static void fwd_rows(dmx_ctx* ctx, const RowSrc& src, size_t rows, int log_m, uint32_t* out, hipStream_t s,
+                     const Jobs& jb) {
     if (!ctx->timing || rows == 0) {
-        launch_ntt_fwd_rows(src, rows, log_m, out, ctx->tw, s);
+        launch_ntt_fwd_rows(src, rows, log_m, out, ctx->tw, s, jb);
         return;
     }
     std::lock_guard<std::mutex> g(ctx->mu);
-    TimedLaunch t{ctx->take_event(), ctx->take_event(), rows, log_m, src.mode};
+    TimedLaunch t{ctx->take_event(), ctx->take_event(), rows * jb.n, log_m, src.mode};
     (void)hipEventRecord(t.a, s);
-    launch_ntt_fwd_rows(src, rows, log_m, out, ctx->tw, s);
+    launch_ntt_fwd_rows(src, rows, log_m, out, ctx->tw, s, jb);
     (void)hipEventRecord(t.b, s);
     ctx->launches.push_back(t);
 }
@@ -514,31 +516,106 @@ extern "C" int dmx_ctx_create(int device_index, dmx_ctx** out) {
     return DMX_OK;
 }
 
-extern "C" void dmx_ctx_destroy(dmx_ctx* ctx) { delete ctx; }
+static void batch_forget_context(dmx_ctx* ctx);
+extern "C" void dmx_ctx_destroy(dmx_ctx* ctx) {
+    if (!ctx) return;
+    batch_forget_context(ctx);  // cached batch workspaces hold the context's tables
+    delete ctx;
+}
 
 // ------------------------------------------------------------------------------------------
-// shared per-object device workspace for transcripts
+// job workspaces. Every device buffer of one job lives in one block; a batch of jobs of one
+// depth is n such blocks at a fixed stride, so one launch covers every job (Jobs{n, stride}).
 // ------------------------------------------------------------------------------------------
+struct Slice {
+    void* p = nullptr;
+    size_t bytes = 0;
+    template <class T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+struct Layout {
+    size_t size = 0;
+    std::vector<std::pair<Slice*, size_t>> items;
+    void add(Slice& s, size_t bytes) {
+        size = (size + 255) & ~(size_t)255;
+        items.emplace_back(&s, size);
+        s.bytes = bytes;
+        size += std::max<size_t>(bytes, 16);
+    }
+    size_t stride() const { return (size + 255) & ~(size_t)255; }
+    int commit(DevBuf& store, uint32_t jobs) {
+        int rc = store.alloc(stride() * jobs);
+        if (rc) return rc;
+        for (auto& it : items) it.first->p = store.as<uint8_t>() + it.second;
+        return DMX_OK;
+    }
+};
+
+// Per-job header in device memory, written by the host before a job runs.
+struct JobMeta {
+    uint64_t len;        // file length in bytes
+    uint64_t src;        // device address the file is copied in from (0: already in place)
+    uint64_t pad[2];
+    uint32_t gseed[8];   // generator key words: g_0_seed (sis.rs:70)
+    uint8_t epoch[32];   // epoch_seed
+    uint8_t file_id[32]; // written by the device
+};
+static_assert(sizeof(JobMeta) == 128, "job header layout");
+constexpr size_t META_GSEED = 32, META_SEEDS = 64;
+
+template <class T>
+static T* job_ptr(T* p, const Jobs& jb, uint32_t b) {
+    return reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + (size_t)b * jb.stride);
+}
+
+// n rows of `width` bytes between pitched buffers; one 2-D copy while the pitches are modest
+static hipError_t copy_rows(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t n,
+                            hipMemcpyKind kind, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (dpitch < ((size_t)1 << 30) && spitch < ((size_t)1 << 30))
+        return hipMemcpy2DAsync(dst, dpitch, src, spitch, width, n, kind, s);
+    for (size_t i = 0; i < n; ++i) {
+        hipError_t e = hipMemcpyAsync(static_cast<uint8_t*>(dst) + i * dpitch,
+                                      static_cast<const uint8_t*>(src) + i * spitch, width, kind, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 static const uint32_t IVW[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
                                 0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
 
+static void gseed_words(const uint8_t g_seed[32], uint32_t w[8]) { b3::words_le(g_seed, w, 8); }
+
+static JobMeta make_meta(uint64_t len, const void* src, const uint8_t g_seed[32], const uint8_t epoch[32]) {
+    JobMeta m;
+    memset(&m, 0, sizeof m);
+    m.len = len;
+    m.src = reinterpret_cast<uint64_t>(src);
+    gseed_words(g_seed, m.gseed);
+    memcpy(m.epoch, epoch, 32);
+    return m;
+}
+
 struct Transcriber {
-    DevBuf msg, cvs, seeds;  // seeds: epoch_seed[32] | file_id[32]
+    Slice msg, cvs;
+    uint8_t* seeds = nullptr;  // epoch_seed[32] | file_id[32], inside the job header
     size_t msg_cap = 0, cv_words = 0;
-    int init(size_t max_ring, size_t extra_hash_len) {
+    void plan(Layout& L, size_t max_ring, size_t extra_hash_len) {
         const uint32_t lens[3] = {(uint32_t)max_ring, (uint32_t)max_ring, (uint32_t)max_ring};
         msg_cap = transcript_len(4, lens);
         cv_words = device_hash_scratch_words(std::max(msg_cap, extra_hash_len));
-        int rc = msg.alloc(msg_cap);
-        if (!rc) rc = cvs.alloc(cv_words * 4);
-        if (!rc) rc = seeds.alloc(64);
-        return rc;
+        L.add(msg, msg_cap);
+        L.add(cvs, cv_words * 4);
     }
-    const uint8_t* epoch() const { return seeds.as<uint8_t>(); }
-    uint8_t* file_id() const { return seeds.as<uint8_t>() + 32; }
-    // challenge (transcript.rs:42-63) written as (x, x^-1) to out
+    const uint8_t* epoch() const { return seeds; }
+    uint8_t* file_id() const { return seeds + 32; }
+    // challenge (transcript.rs:42-63) written as (x, x^-1) to out, for every job
     void challenge(const char* label, uint64_t round, const zq* c, size_t lc, const zq* l, size_t ll,
-                   const zq* r, size_t lr, zq* out, uint32_t* flags, hipStream_t s) {
+                   const zq* r, size_t lr, zq* out, uint32_t* flags, hipStream_t s, const Jobs& jb) {
         TranscriptMsg t;
         t.epoch_seed = epoch();
         t.file_id = file_id();
@@ -553,12 +630,10 @@ struct Transcriber {
         t.lens[1] = (uint32_t)ll;
         t.lens[2] = (uint32_t)lr;
         const size_t len = transcript_len(t.label_len, t.lens);
-        launch_transcript_pack(t, msg.as<uint8_t>(), len, s);
-        device_hash(msg.as<uint8_t>(), len, IVW, 0, cvs.as<uint32_t>(), cv_words, nullptr, out, flags, s);
+        launch_transcript_pack(t, msg.as<uint8_t>(), len, s, jb);
+        device_hash(msg.as<uint8_t>(), len, IVW, 0, cvs.as<uint32_t>(), cv_words, nullptr, out, flags, s, jb);
     }
 };
-
-static void gseed_words(const uint8_t g_seed[32], uint32_t w[8]) { b3::words_le(g_seed, w, 8); }
 
 // ------------------------------------------------------------------------------------------
 // prover (src/protocol/prover.rs:56-313)
@@ -570,21 +645,24 @@ static uint32_t poly_bit(size_t j) { return 1u << (2 * j + 1); }
 struct dmx_prover {
     dmx_ctx* ctx = nullptr;
     hipStream_t s = nullptr;
-    dmx_system_params params{};
-    uint8_t g_seed[32];
-    uint32_t gw[8];
-    uint64_t orig_len = 0;
+    Jobs jb;                                      // jobs of this depth proved side by side
+    std::vector<dmx_system_params> params;        // per job
+    std::vector<std::array<uint8_t, 32>> g_seed;  // per job: g_0_seed (sis.rs:70)
+    std::vector<uint64_t> orig_len;               // per job
     int d = 0;
     size_t N = 0;
     size_t round = 0;
     bool poisoned = false;
     bool capturing = false;  // inside hipGraph capture: no timing events
-    DevBuf file, W_a, W_b, G_a, G_b, What_a, What_b, Ghat_a, Ghat_b, res, part, lifted, C, Ct, com0, recs,
+    size_t hash_cap = 0;     // bound on every job's file length (sizes the file hash launch)
+    DevBuf store;            // the workspace, unless a batch owns it
+    Slice meta, file, W_a, W_b, G_a, G_b, What_a, What_b, Ghat_a, Ghat_b, res, part, lifted, C, Ct, com0, recs,
         scal, flags;
     std::vector<size_t> rec_off;  // offset (coefficients) of round j's record block
     Transcriber tr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     const uint8_t* d_file = nullptr;
+    std::vector<JobMeta> stage;  // host copy of the job headers
 
     ~dmx_prover() {
         for (auto& e : ev)
@@ -598,26 +676,30 @@ struct dmx_prover {
     zq* rec_rpoly(size_t j) const { return rec_lpoly(j) + (size_t)K * (m_at(j) / 2); }
     uint32_t* res_at(size_t words) const { return res.as<uint32_t>() + words; }
     zq* sc(size_t i) const { return scal.as<zq>() + i; }
+    const uint64_t* dlen() const { return meta.as<uint64_t>(); }
+    const uint32_t* gseed() const { return reinterpret_cast<const uint32_t*>(meta.as<uint8_t>() + META_GSEED); }
 };
 
-static int prover_alloc(dmx_prover* p) {
+// file_cap = 0: the file stays where the caller keeps it (one job)
+static void prover_plan(dmx_prover* p, Layout& L, size_t file_cap) {
     const size_t N = p->N;
     const size_t NN = N * N;
-    int rc = 0;
-    rc = rc ? rc : p->W_a.alloc(NN * 16);
-    rc = rc ? rc : p->W_b.alloc(NN / 2 * 16);
-    rc = rc ? rc : p->G_a.alloc(NN * K * 16);
-    rc = rc ? rc : p->G_b.alloc(NN / 2 * K * 16);
-    rc = rc ? rc : p->What_a.alloc(NN * NPRIMES * 4);
-    rc = rc ? rc : p->What_b.alloc(NN / 2 * NPRIMES * 4);
-    rc = rc ? rc : p->Ghat_a.alloc(NN * K * NPRIMES * 4);
-    rc = rc ? rc : p->Ghat_b.alloc(NN / 2 * K * NPRIMES * 4);
-    rc = rc ? rc : p->res.alloc((size_t)3 * K * NPRIMES * N * 4);
-    rc = rc ? rc : p->part.alloc(mac_part_words(p->d) * 4);
-    rc = rc ? rc : p->lifted.alloc((size_t)K * N * 16);
-    rc = rc ? rc : p->C.alloc((size_t)K * N * 16);
-    rc = rc ? rc : p->Ct.alloc((size_t)K * N * 16);
-    rc = rc ? rc : p->com0.alloc((size_t)K * N * 16);
+    L.add(p->meta, sizeof(JobMeta));
+    if (file_cap) L.add(p->file, file_cap);
+    L.add(p->W_a, NN * 16);
+    L.add(p->W_b, NN / 2 * 16);
+    L.add(p->G_a, NN * K * 16);
+    L.add(p->G_b, NN / 2 * K * 16);
+    L.add(p->What_a, NN * NPRIMES * 4);
+    L.add(p->What_b, NN / 2 * NPRIMES * 4);
+    L.add(p->Ghat_a, NN * K * NPRIMES * 4);
+    L.add(p->Ghat_b, NN / 2 * K * NPRIMES * 4);
+    L.add(p->res, (size_t)3 * K * NPRIMES * N * 4);
+    L.add(p->part, mac_part_words(p->d) * 4);
+    L.add(p->lifted, (size_t)K * N * 16);
+    L.add(p->C, (size_t)K * N * 16);
+    L.add(p->Ct, (size_t)K * N * 16);
+    L.add(p->com0, (size_t)K * N * 16);
     size_t total = 0;
     p->rec_off.resize(p->d + 1);
     for (int j = 0; j <= p->d; ++j) {
@@ -625,40 +707,45 @@ static int prover_alloc(dmx_prover* p) {
         size_t m = N >> j;
         total += (size_t)K * (2 * m + m);
     }
-    rc = rc ? rc : p->recs.alloc(total * 16);
-    rc = rc ? rc : p->scal.alloc((size_t)4 * (p->d + 1) * 16);
-    rc = rc ? rc : p->flags.alloc(256);
-    rc = rc ? rc : p->tr.init(N, p->orig_len);
-    return rc;
+    L.add(p->recs, total * 16);
+    L.add(p->scal, (size_t)4 * (p->d + 1) * 16);
+    L.add(p->flags, 256);
+    p->tr.plan(L, N, p->hash_cap);
+}
+
+static void prover_bind(dmx_prover* p) {
+    p->tr.seeds = p->meta.as<uint8_t>() + META_SEEDS;
+    if (p->file.p) p->d_file = p->file.as<uint8_t>();
 }
 
 // initialize_with_config (prover.rs:61-111): file_id, witness, generators, com_0 and the
 // round-0 vector cross terms, all from one pass over the generator transforms.
 static int prover_initialize(dmx_prover* p) {
     hipStream_t s = p->s;
+    const Jobs& jb = p->jb;
     const size_t N = p->N;
     const int d = p->d;
     const NttTables& tw = p->ctx->tw;
-    launch_fill32(p->flags.as<uint32_t>(), 64, 0, s);
-    device_hash(p->d_file, p->orig_len, IVW, 0, p->tr.cvs.as<uint32_t>(), p->tr.cv_words,
-                reinterpret_cast<uint32_t*>(p->tr.file_id()), nullptr, p->flags.as<uint32_t>(), s);
-    launch_expand(p->d_file, p->orig_len, p->W_a.as<zq>(), N * N, s);
+    launch_fill32(p->flags.as<uint32_t>(), 64, 0, s, jb);
+    device_hash(p->d_file, p->hash_cap, IVW, 0, p->tr.cvs.as<uint32_t>(), p->tr.cv_words,
+                reinterpret_cast<uint32_t*>(p->tr.file_id()), nullptr, p->flags.as<uint32_t>(), s, jb, p->dlen());
+    launch_expand(p->d_file, p->hash_cap, p->W_a.as<zq>(), N * N, s, jb, p->dlen());
     RowSrc gen{};
     gen.mode = ROW_GEN;
     gen.dst = p->G_a.as<zq>();
-    memcpy(gen.gseed, p->gw, 32);
-    fwd_rows(p->ctx, gen, (size_t)K * N, d, p->Ghat_a.as<uint32_t>(), s);
+    gen.gseed = p->gseed();
+    fwd_rows(p->ctx, gen, (size_t)K * N, d, p->Ghat_a.as<uint32_t>(), s, jb);
     RowSrc pl{};
     pl.mode = ROW_PLAIN;
     pl.a = p->W_a.as<zq>();
-    fwd_rows(p->ctx, pl, N, d, p->What_a.as<uint32_t>(), s);
+    fwd_rows(p->ctx, pl, N, d, p->What_a.as<uint32_t>(), s, jb);
     const size_t slot = (size_t)K * NPRIMES * N;
     launch_mac_pairs(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), N, d, p->res_at(0), p->res_at(slot),
-                     p->res_at(2 * slot), p->part.as<uint32_t>(), s);
-    launch_ntt_inv(p->res_at(0), 3 * K, d, tw, s);
-    launch_crt_lift(p->res_at(0), p->C.as<zq>(), K, d, s);
-    launch_crt_lift(p->res_at(slot), p->rec_lvec(0), 2 * K, d, s);  // L_vec, R_vec adjacent
-    launch_copy16(p->com0.p, p->C.p, (size_t)K * N, s);
+                     p->res_at(2 * slot), p->part.as<uint32_t>(), s, jb);
+    launch_ntt_inv(p->res_at(0), 3 * K, d, tw, s, jb);
+    launch_crt_lift(p->res_at(0), p->C.as<zq>(), K, d, s, jb);
+    launch_crt_lift(p->res_at(slot), p->rec_lvec(0), 2 * K, d, s, jb);  // L_vec, R_vec adjacent
+    launch_copy16(p->com0.p, p->C.p, (size_t)K * N, s, jb);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
@@ -666,69 +753,72 @@ static int prover_initialize(dmx_prover* p) {
 // fold_round (prover.rs:140-313) for round j, m = N >> j: both stages and both invariants.
 static int prover_round(dmx_prover* p, size_t j) {
     hipStream_t s = p->s;
+    const Jobs& jb = p->jb;
     const NttTables& tw = p->ctx->tw;
     const size_t m = p->m_at(j), h = m / 2;
     const int lm = p->d - (int)j;
     uint32_t* flags = p->flags.as<uint32_t>();
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[0], s));
     // ---- vector stage ----
-    p->tr.challenge("vec", j, p->C.as<zq>(), m, p->rec_lvec(j), m, p->rec_rvec(j), m, p->sc(4 * j), flags, s);
-    launch_com_update(p->C.as<zq>(), p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j), 0, lm, p->Ct.as<zq>(), s);
+    p->tr.challenge("vec", j, p->C.as<zq>(), m, p->rec_lvec(j), m, p->rec_rvec(j), m, p->sc(4 * j), flags, s, jb);
+    launch_com_update(p->C.as<zq>(), p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j), 0, lm, p->Ct.as<zq>(), s, jb);
     RowSrc vw{};
     vw.mode = ROW_VFOLD;
     vw.a = p->W_a.as<zq>();
     vw.b_off = h;
     vw.scal = p->sc(4 * j);  // m_L + x * m_R
     vw.dst = p->W_b.as<zq>();
-    fwd_rows(p->ctx, vw, h, lm, p->What_b.as<uint32_t>(), s);
+    fwd_rows(p->ctx, vw, h, lm, p->What_b.as<uint32_t>(), s, jb);
     RowSrc vg{};
     vg.mode = ROW_VFOLD;
     vg.a = p->G_a.as<zq>();
     vg.b_off = h * K;
     vg.scal = p->sc(4 * j + 1);  // g_L + x^-1 * g_R
     vg.dst = p->G_b.as<zq>();
-    fwd_rows(p->ctx, vg, h * K, lm, p->Ghat_b.as<uint32_t>(), s);
+    fwd_rows(p->ctx, vg, h * K, lm, p->Ghat_b.as<uint32_t>(), s, jb);
     // invariant at degree m and the poly cross terms at degree m/2 from one pass
     const size_t s0 = (size_t)K * NPRIMES * m;
     launch_mac_split(p->What_b.as<uint32_t>(), p->Ghat_b.as<uint32_t>(), h, lm, p->res_at(0), p->res_at(s0),
-                     p->res_at(s0 + (size_t)K * NPRIMES * h), p->part.as<uint32_t>(), tw, s);
-    launch_ntt_inv(p->res_at(0), K, lm, tw, s);
-    launch_ntt_inv(p->res_at(s0), 2 * K, lm - 1, tw, s);
-    launch_crt_lift(p->res_at(0), p->lifted.as<zq>(), K, lm, s);
-    launch_crt_lift(p->res_at(s0), p->rec_lpoly(j), 2 * K, lm - 1, s);
-    launch_com_check(p->lifted.as<zq>(), p->Ct.as<zq>(), (size_t)K * m, flags, vec_bit(j), s);
+                     p->res_at(s0 + (size_t)K * NPRIMES * h), p->part.as<uint32_t>(), tw, s, jb);
+    launch_ntt_inv(p->res_at(0), K, lm, tw, s, jb);
+    launch_ntt_inv(p->res_at(s0), 2 * K, lm - 1, tw, s, jb);
+    launch_crt_lift(p->res_at(0), p->lifted.as<zq>(), K, lm, s, jb);
+    launch_crt_lift(p->res_at(s0), p->rec_lpoly(j), 2 * K, lm - 1, s, jb);
+    launch_com_check(p->lifted.as<zq>(), p->Ct.as<zq>(), (size_t)K * m, flags, vec_bit(j), s, jb);
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[1], s));
     // ---- poly stage ----
-    p->tr.challenge("poly", j, p->Ct.as<zq>(), m, p->rec_lpoly(j), h, p->rec_rpoly(j), h, p->sc(4 * j + 2), flags, s);
-    launch_com_update(p->Ct.as<zq>(), p->rec_lpoly(j), p->rec_rpoly(j), p->sc(4 * j + 2), 1, lm - 1,
-                      p->C.as<zq>(), s);
+    p->tr.challenge("poly", j, p->Ct.as<zq>(), m, p->rec_lpoly(j), h, p->rec_rpoly(j), h, p->sc(4 * j + 2), flags, s,
+                    jb);
+    launch_com_update(p->Ct.as<zq>(), p->rec_lpoly(j), p->rec_rpoly(j), p->sc(4 * j + 2), 1, lm - 1, p->C.as<zq>(), s,
+                      jb);
     RowSrc pw{};
     pw.mode = ROW_PFOLD_W;
     pw.a = p->W_b.as<zq>();
     pw.scal = p->sc(4 * j + 2);  // e + y * o
     pw.dst = p->W_a.as<zq>();
-    fwd_rows(p->ctx, pw, h, lm - 1, p->What_a.as<uint32_t>(), s);
+    fwd_rows(p->ctx, pw, h, lm - 1, p->What_a.as<uint32_t>(), s, jb);
     RowSrc pg{};
     pg.mode = ROW_PFOLD_G;
     pg.a = p->G_b.as<zq>();
     pg.scal = p->sc(4 * j + 3);  // g_e + y^-1 * X g_o
     pg.dst = p->G_a.as<zq>();
-    fwd_rows(p->ctx, pg, h * K, lm - 1, p->Ghat_a.as<uint32_t>(), s);
+    fwd_rows(p->ctx, pg, h * K, lm - 1, p->Ghat_a.as<uint32_t>(), s, jb);
     // post-poly invariant, plus the next round's vector cross terms from the same pass
     const size_t s1 = (size_t)K * NPRIMES * h;
     const bool next = h > 1;
     launch_mac_pairs(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), h, lm - 1, p->res_at(0),
-                     next ? p->res_at(s1) : nullptr, next ? p->res_at(2 * s1) : nullptr, p->part.as<uint32_t>(), s);
-    launch_ntt_inv(p->res_at(0), next ? 3 * K : K, lm - 1, tw, s);
-    launch_crt_lift(p->res_at(0), p->lifted.as<zq>(), K, lm - 1, s);
-    if (next) launch_crt_lift(p->res_at(s1), p->rec_lvec(j + 1), 2 * K, lm - 1, s);
-    launch_com_check(p->lifted.as<zq>(), p->C.as<zq>(), (size_t)K * h, flags, poly_bit(j), s);
+                     next ? p->res_at(s1) : nullptr, next ? p->res_at(2 * s1) : nullptr, p->part.as<uint32_t>(), s,
+                     jb);
+    launch_ntt_inv(p->res_at(0), next ? 3 * K : K, lm - 1, tw, s, jb);
+    launch_crt_lift(p->res_at(0), p->lifted.as<zq>(), K, lm - 1, s, jb);
+    if (next) launch_crt_lift(p->res_at(s1), p->rec_lvec(j + 1), 2 * K, lm - 1, s, jb);
+    launch_com_check(p->lifted.as<zq>(), p->C.as<zq>(), (size_t)K * h, flags, poly_bit(j), s, jb);
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[2], s));
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
 
-static int check_round_flags(dmx_prover* p, size_t j, uint32_t f) {
+static int check_round_flags(size_t j, uint32_t f) {
     if (f & FLAG_ZERO_CHALLENGE) return fail(DMX_ERR_HIP, "challenge resampling exhausted");
     if (f & vec_bit(j)) return fail(DMX_ERR_INVARIANT, "vector folding invariant failed (round %zu)", j);
     if (f & poly_bit(j)) return fail(DMX_ERR_INVARIANT, "round folding invariant failed (round %zu)", j);
@@ -752,32 +842,59 @@ static int encode_record(dmx_prover* p, size_t j, uint8_t* out, size_t cap, size
     return DMX_OK;
 }
 
-// every round's record with a single device-to-host copy (the blocks are contiguous)
-static int encode_records(dmx_prover* p, uint8_t* out) {
-    const size_t total = p->rec_off[p->d];
-    std::vector<zq> host(total);
-    HIP_TRY(hipMemcpyAsync(host.data(), p->recs.p, total * 16, hipMemcpyDeviceToHost, p->s));
-    HIP_TRY(hipStreamSynchronize(p->s));
+// every round's record from the host copy of one job's record blocks
+static void encode_records_from(const dmx_prover* p, const zq* host, uint8_t* out) {
     uint8_t* q = out;
     for (size_t j = 0; j < (size_t)p->d; ++j) {
         const size_t m = p->m_at(j), h = m / 2;
-        const zq* b = host.data() + p->rec_off[j];
+        const zq* b = host + p->rec_off[j];
         q = put_u64(q, j);
         q = put_module(q, b, m);
         q = put_module(q, b + (size_t)K * m, m);
         q = put_module(q, b + (size_t)2 * K * m, h);
         q = put_module(q, b + (size_t)2 * K * m + (size_t)K * h, h);
     }
+}
+
+static void encode_statement_from(const dmx_prover* p, uint32_t b, const uint8_t fid[32], const zq* com,
+                                  uint8_t* out) {
+    uint8_t* q = out;
+    memcpy(q, fid, 32);
+    q = put_u64(q + 32, p->orig_len[b]);
+    q = put_u64(q, (uint64_t)p->d);
+    q = put_module(q, com, p->N);
+    memcpy(q, p->g_seed[b].data(), 32);
+}
+
+static int prover_new_object(dmx_ctx* ctx, int d, uint32_t n, dmx_prover** out) {
+    std::unique_ptr<dmx_prover> p(new dmx_prover);
+    p->ctx = ctx;
+    p->d = d;
+    p->N = (size_t)1 << d;
+    p->jb = Jobs{n, 0};
+    p->params.resize(n);
+    p->g_seed.resize(n);
+    p->orig_len.assign(n, 0);
+    p->stage.resize(n);
+    HIP_TRY(hipStreamCreate(&p->s));
+    for (auto& e : p->ev) HIP_TRY(hipEventCreate(&e));
+    *out = p.release();
     return DMX_OK;
 }
 
+static void prover_set_job(dmx_prover* p, uint32_t b, const dmx_system_params* params, uint64_t len,
+                           const void* src) {
+    p->params[b] = *params;
+    p->orig_len[b] = len;
+    b3::host_hash_short(params->seed_generators, (const uint8_t*)"g0", 2, p->g_seed[b].data());  // sis.rs:70
+    p->stage[b] = make_meta(len, src, p->g_seed[b].data(), params->epoch_seed);
+}
+
 static int prover_create(dmx_ctx* ctx, const uint8_t* file, const void* d_file, size_t len,
-                         const dmx_system_params* params, const dmx_runtime_config* cfg, dmx_prover** out,
-                         bool sync = true, size_t own_len = 0) {
-    if (!ctx || !params || !out || (!file && !d_file && len && !own_len)) return fail(DMX_ERR_ARG, "null argument");
+                         const dmx_system_params* params, const dmx_runtime_config* cfg, dmx_prover** out) {
+    if (!ctx || !params || !out || (!file && !d_file && len)) return fail(DMX_ERR_ARG, "null argument");
     if (cfg && !cfg->gpu_enabled) return fail(DMX_ERR_NO_DEVICE, "gpu_enabled = false");
     if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
-    if (own_len) len = own_len;
     const int d = depth_for(len);
     const size_t N = (size_t)1 << d;
     const uint64_t limit = cfg ? cfg->max_preprocess_bytes : (512ull << 20);
@@ -787,42 +904,26 @@ static int prover_create(dmx_ctx* ctx, const uint8_t* file, const void* d_file, 
                     (unsigned long long)(N * N * 32), (unsigned long long)limit);
     if (d > 12) return fail(DMX_ERR_ARG, "depth %d exceeds the supported 12", d);
     HIP_TRY(hipSetDevice(ctx->device));
-    std::unique_ptr<dmx_prover> p(new dmx_prover);
-    p->ctx = ctx;
-    p->params = *params;
-    p->orig_len = len;
-    p->d = d;
-    p->N = N;
-    HIP_TRY(hipStreamCreate(&p->s));
-    for (auto& e : p->ev) HIP_TRY(hipEventCreate(&e));
-    b3::host_hash_short(params->seed_generators, (const uint8_t*)"g0", 2, p->g_seed);  // sis.rs:70
-    gseed_words(p->g_seed, p->gw);
-    int rc = prover_alloc(p.get());
+    dmx_prover* raw = nullptr;
+    int rc = prover_new_object(ctx, d, 1, &raw);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(p->tr.seeds.p, params->epoch_seed, 32, hipMemcpyHostToDevice, p->s));
-    if (own_len) {  // a fused job: the file arrives later, the length is fixed now
-        rc = p->file.alloc(std::max<size_t>(len, 16));
-        if (rc) return rc;
-        p->d_file = p->file.as<uint8_t>();
-        *out = p.release();
-        return DMX_OK;
-    }
-    if (d_file) {
-        p->d_file = static_cast<const uint8_t*>(d_file);
-    } else {
-        rc = p->file.alloc(std::max<size_t>(len, 16));
-        if (rc) return rc;
-        if (len) HIP_TRY(hipMemcpyAsync(p->file.p, file, len, hipMemcpyHostToDevice, p->s));
-        p->d_file = p->file.as<uint8_t>();
-    }
+    std::unique_ptr<dmx_prover> p(raw);
+    p->hash_cap = len;
+    Layout L;
+    prover_plan(p.get(), L, d_file ? 0 : std::max<size_t>(len, 16));
+    rc = L.commit(p->store, 1);
+    if (rc) return rc;
+    prover_bind(p.get());
+    if (d_file) p->d_file = static_cast<const uint8_t*>(d_file);
+    prover_set_job(p.get(), 0, params, len, nullptr);
+    HIP_TRY(hipMemcpyAsync(p->meta.p, p->stage.data(), sizeof(JobMeta), hipMemcpyHostToDevice, p->s));
+    if (!d_file && len) HIP_TRY(hipMemcpyAsync(p->file.p, file, len, hipMemcpyHostToDevice, p->s));
     rc = prover_initialize(p.get());
     if (rc) return rc;
-    if (sync) {
-        HIP_TRY(hipStreamSynchronize(p->s));
-        uint32_t f = 0;
-        HIP_TRY(hipMemcpy(&f, p->flags.p, 4, hipMemcpyDeviceToHost));
-        if (f & FLAG_ZERO_CHALLENGE) return fail(DMX_ERR_HIP, "hashing failed");
-    }
+    HIP_TRY(hipStreamSynchronize(p->s));
+    uint32_t f = 0;
+    HIP_TRY(hipMemcpy(&f, p->flags.p, 4, hipMemcpyDeviceToHost));
+    if (f & FLAG_ZERO_CHALLENGE) return fail(DMX_ERR_HIP, "hashing failed");
     *out = p.release();
     return DMX_OK;
 }
@@ -852,12 +953,7 @@ extern "C" int dmx_prover_statement(const dmx_prover* p, uint8_t* out, size_t ca
     HIP_TRY(hipMemcpyAsync(com.data(), p->com0.p, com.size() * 16, hipMemcpyDeviceToHost, p->s));
     HIP_TRY(hipMemcpyAsync(fid, p->tr.file_id(), 32, hipMemcpyDeviceToHost, p->s));
     HIP_TRY(hipStreamSynchronize(p->s));
-    uint8_t* q = out;
-    memcpy(q, fid, 32);
-    q = put_u64(q + 32, p->orig_len);
-    q = put_u64(q, (uint64_t)p->d);
-    q = put_module(q, com.data(), p->N);
-    memcpy(q, p->g_seed, 32);
+    encode_statement_from(p, 0, fid, com.data(), out);
     return DMX_OK;
 }
 
@@ -886,7 +982,7 @@ extern "C" int dmx_prover_fold_round(dmx_prover* p, size_t j, uint8_t* record_ou
     if (!rc) rc = hipStreamSynchronize(p->s) == hipSuccess ? DMX_OK : fail(DMX_ERR_HIP, "stream sync failed");
     uint32_t f = 0;
     if (!rc && hipMemcpy(&f, p->flags.p, 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(DMX_ERR_HIP, "flags");
-    if (!rc) rc = check_round_flags(p, j, f);
+    if (!rc) rc = check_round_flags(j, f);
     if (rc) {
         p->poisoned = true;
         return rc;
@@ -915,86 +1011,105 @@ extern "C" int dmx_prover_final_opening(const dmx_prover* p, uint64_t out[2]) {
 // ------------------------------------------------------------------------------------------
 // verifier (src/protocol/verifier.rs:21-134)
 // ------------------------------------------------------------------------------------------
-static int log2_exact(size_t n);
+static int log2_exact(size_t n) {
+    int l = 0;
+    while (((size_t)1 << l) < n) ++l;
+    return l;
+}
 
 struct dmx_verifier {
     dmx_ctx* ctx = nullptr;
     hipStream_t s = nullptr;
-    dmx_system_params params{};
-    uint8_t file_id[32], g_seed[32];
+    Jobs jb;
     size_t rounds = 0, round = 0;
     size_t rows = 0, glen = 0, clen = 0;
     size_t g_cap = 0, c_cap = 0;
-    DevBuf G0, G1, C, Ct, rec, scal, flags, opening;
+    DevBuf store;  // the workspace, unless a batch owns it
+    Slice meta, G0, G1, C, Ct, rec, scal, flags, opening;
     int gcur = 0;  // which of G0/G1 holds the state
     Transcriber tr;
+    std::vector<JobMeta> stage;
     ~dmx_verifier() {
         if (s) (void)hipStreamDestroy(s);
     }
     zq* G() const { return (gcur ? G1 : G0).as<zq>(); }
     zq* Gother() const { return (gcur ? G0 : G1).as<zq>(); }
+    const uint32_t* gseed() const { return reinterpret_cast<const uint32_t*>(meta.as<uint8_t>() + META_GSEED); }
 };
 
-static int verifier_alloc(dmx_verifier* v) {
-    int rc = 0;
-    rc = rc ? rc : v->G0.alloc(v->g_cap * 16);
-    rc = rc ? rc : v->G1.alloc(std::max<size_t>(v->g_cap / 2, 1) * 16);
-    rc = rc ? rc : v->C.alloc((size_t)K * v->c_cap * 16);
-    rc = rc ? rc : v->Ct.alloc((size_t)K * v->c_cap * 16);
-    rc = rc ? rc : v->rec.alloc((size_t)4 * K * v->c_cap * 16);
-    rc = rc ? rc : v->scal.alloc(4 * 16);
-    rc = rc ? rc : v->flags.alloc(256);
-    rc = rc ? rc : v->opening.alloc(16);
-    rc = rc ? rc : v->tr.init(v->c_cap, 0);
-    return rc;
+static void verifier_plan(dmx_verifier* v, Layout& L) {
+    L.add(v->meta, sizeof(JobMeta));
+    L.add(v->G0, v->g_cap * 16);
+    L.add(v->G1, std::max<size_t>(v->g_cap / 2, 1) * 16);
+    L.add(v->C, (size_t)K * v->c_cap * 16);
+    L.add(v->Ct, (size_t)K * v->c_cap * 16);
+    L.add(v->rec, (size_t)4 * K * v->c_cap * 16);
+    L.add(v->scal, 4 * 16);
+    L.add(v->flags, 256);
+    L.add(v->opening, 16);
+    v->tr.plan(L, v->c_cap, 0);
 }
 
-// Device part of DamascusVerifier::new: flags, generator derivation from g_0_seed
-// (verifier.rs:31-33) and, for the fused job, the device-resident statement values copied in
-// once `wait` (the prover's initialisation) has completed. Kernels only: graph-capture safe.
-static int verifier_enqueue_setup(dmx_verifier* v, const uint8_t* d_file_id, const zq* d_com0, hipEvent_t wait) {
-    launch_fill32(v->flags.as<uint32_t>(), 64, 0, v->s);
-    uint32_t gw[8];
-    gseed_words(v->g_seed, gw);
-    launch_gen_generators(gw, v->rows * K, log2_exact(v->glen), v->G0.as<zq>(), v->s);
-    if (wait) HIP_TRY(hipStreamWaitEvent(v->s, wait, 0));
-    if (d_file_id) launch_copy16(v->tr.file_id(), d_file_id, 2, v->s);
-    if (d_com0) launch_copy16(v->C.p, d_com0, (size_t)K * v->clen, v->s);
+static void verifier_bind(dmx_verifier* v) { v->tr.seeds = v->meta.as<uint8_t>() + META_SEEDS; }
+
+// Generator derivation from g_0_seed (verifier.rs:31-33); depends on the seed only, so a fused
+// job starts it before the prover has produced the statement. Kernels only.
+static int verifier_enqueue_gen(dmx_verifier* v) {
+    launch_fill32(v->flags.as<uint32_t>(), 64, 0, v->s, v->jb);
+    launch_gen_generators(v->gseed(), v->rows * K, log2_exact(v->glen), v->G0.as<zq>(), v->s, v->jb);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
 
-// DamascusVerifier::new (verifier.rs:22-43): generators re-derived from the statement's
-// g_0_seed on the device. com0 / file_id come from host memory, or device memory when
-// `on_device` (the fused prove+verify path).
+// the statement's file_id and com_0 from device memory (the fused path)
+static int verifier_enqueue_statement(dmx_verifier* v, const uint8_t* d_file_id, const zq* d_com0) {
+    launch_copy16(v->tr.file_id(), d_file_id, 2, v->s, v->jb);
+    launch_copy16(v->C.p, d_com0, (size_t)K * v->clen, v->s, v->jb);
+    HIP_TRY(hipGetLastError());
+    return DMX_OK;
+}
+
+static int verifier_new_object(dmx_ctx* ctx, size_t d, size_t rows, size_t ring_len, uint32_t n,
+                               dmx_verifier** out) {
+    std::unique_ptr<dmx_verifier> v(new dmx_verifier);
+    v->ctx = ctx;
+    v->jb = Jobs{n, 0};
+    v->rounds = d;
+    v->rows = rows;
+    v->glen = v->clen = ring_len;
+    v->g_cap = v->rows * K * v->glen;
+    v->c_cap = v->clen;
+    v->stage.resize(n);
+    HIP_TRY(hipStreamCreate(&v->s));
+    *out = v.release();
+    return DMX_OK;
+}
+
+// DamascusVerifier::new (verifier.rs:22-43) from a host statement: generators re-derived from
+// the statement's g_0_seed on the device.
 static int verifier_make(dmx_ctx* ctx, const dmx_system_params* params, const uint8_t* file_id, uint64_t orig_len,
-                         uint64_t d, const zq* com0, size_t com_len, const uint8_t g_seed[32], bool on_device,
-                         dmx_verifier** out, hipEvent_t wait_before_copy = nullptr) {
+                         uint64_t d, const zq* com0, size_t com_len, const uint8_t g_seed[32], dmx_verifier** out) {
     if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
     if (com_len == 0 || (com_len & (com_len - 1)) || com_len > ((size_t)1 << MAX_LOG_N))
         return fail(DMX_ERR_SHAPE, "statement ring length %zu is not a supported power of two", com_len);
     const int ld = depth_for(orig_len);
     if (ld > 12) return fail(DMX_ERR_ARG, "depth %d exceeds the supported 12", ld);
     HIP_TRY(hipSetDevice(ctx->device));
-    std::unique_ptr<dmx_verifier> v(new dmx_verifier);
-    v->ctx = ctx;
-    v->params = *params;
-    memcpy(v->g_seed, g_seed, 32);
-    v->rounds = (size_t)d;
-    v->rows = (size_t)1 << ld;
-    v->glen = v->clen = com_len;
-    v->g_cap = v->rows * K * v->glen;
-    v->c_cap = v->clen;
-    HIP_TRY(hipStreamCreate(&v->s));
-    int rc = verifier_alloc(v.get());
+    dmx_verifier* raw = nullptr;
+    int rc = verifier_new_object(ctx, (size_t)d, (size_t)1 << ld, com_len, 1, &raw);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(v->tr.seeds.p, params->epoch_seed, 32, hipMemcpyHostToDevice, v->s));
-    if (!on_device) {
-        HIP_TRY(hipMemcpyAsync(v->tr.file_id(), file_id, 32, hipMemcpyHostToDevice, v->s));
-        HIP_TRY(hipMemcpyAsync(v->C.p, com0, (size_t)K * v->clen * 16, hipMemcpyHostToDevice, v->s));
-        rc = verifier_enqueue_setup(v.get(), nullptr, nullptr, nullptr);
-        if (rc) return rc;
-    }
+    std::unique_ptr<dmx_verifier> v(raw);
+    Layout L;
+    verifier_plan(v.get(), L);
+    rc = L.commit(v->store, 1);
+    if (rc) return rc;
+    verifier_bind(v.get());
+    v->stage[0] = make_meta(orig_len, nullptr, g_seed, params->epoch_seed);
+    memcpy(v->stage[0].file_id, file_id, 32);
+    HIP_TRY(hipMemcpyAsync(v->meta.p, v->stage.data(), sizeof(JobMeta), hipMemcpyHostToDevice, v->s));
+    HIP_TRY(hipMemcpyAsync(v->C.p, com0, (size_t)K * v->clen * 16, hipMemcpyHostToDevice, v->s));
+    rc = verifier_enqueue_gen(v.get());
+    if (rc) return rc;
     *out = v.release();
     return DMX_OK;
 }
@@ -1006,9 +1121,8 @@ extern "C" int dmx_verifier_new(dmx_ctx* ctx, const dmx_system_params* params, c
     int rc = decode_statement(statement, len, st);
     if (rc) return rc;
     dmx_verifier* v = nullptr;
-    rc = verifier_make(ctx, params, st.file_id, st.original_len, st.d, st.com0.data(), st.com_len, st.g_seed, false, &v);
+    rc = verifier_make(ctx, params, st.file_id, st.original_len, st.d, st.com0.data(), st.com_len, st.g_seed, &v);
     if (rc) return rc;
-    memcpy(v->file_id, st.file_id, 32);
     if (hipStreamSynchronize(v->s) != hipSuccess) {
         delete v;
         return fail(DMX_ERR_HIP, "verifier setup failed");
@@ -1019,25 +1133,22 @@ extern "C" int dmx_verifier_new(dmx_ctx* ctx, const dmx_system_params* params, c
 
 extern "C" dmx_verifier* dmx_verifier_clone(const dmx_verifier* v) {
     if (!v) return nullptr;
-    std::unique_ptr<dmx_verifier> c(new dmx_verifier);
-    c->ctx = v->ctx;
-    c->params = v->params;
-    memcpy(c->file_id, v->file_id, 32);
-    memcpy(c->g_seed, v->g_seed, 32);
-    c->rounds = v->rounds;
+    dmx_verifier* raw = nullptr;
+    if (verifier_new_object(v->ctx, v->rounds, v->rows, v->clen, 1, &raw)) return nullptr;
+    std::unique_ptr<dmx_verifier> c(raw);
     c->round = v->round;
-    c->rows = v->rows;
     c->glen = v->glen;
-    c->clen = v->clen;
     c->g_cap = v->rows * K * v->glen;
-    c->c_cap = v->clen;
-    if (hipStreamCreate(&c->s) != hipSuccess) return nullptr;
-    if (verifier_alloc(c.get())) return nullptr;
+    Layout L;
+    verifier_plan(c.get(), L);
+    if (L.commit(c->store, 1)) return nullptr;
+    verifier_bind(c.get());
     (void)hipStreamSynchronize(v->s);
     if (hipMemcpyAsync(c->G0.p, v->G(), c->g_cap * 16, hipMemcpyDeviceToDevice, c->s) != hipSuccess) return nullptr;
     if (hipMemcpyAsync(c->C.p, v->C.p, (size_t)K * v->clen * 16, hipMemcpyDeviceToDevice, c->s) != hipSuccess)
         return nullptr;
-    if (hipMemcpyAsync(c->tr.seeds.p, v->tr.seeds.p, 64, hipMemcpyDeviceToDevice, c->s) != hipSuccess) return nullptr;
+    if (hipMemcpyAsync(c->meta.p, v->meta.p, sizeof(JobMeta), hipMemcpyDeviceToDevice, c->s) != hipSuccess)
+        return nullptr;
     launch_fill32(c->flags.as<uint32_t>(), 64, 0, c->s);
     if (hipStreamSynchronize(c->s) != hipSuccess) return nullptr;
     return c.release();
@@ -1064,16 +1175,11 @@ static bool all_zero(const std::vector<zq>& m) {
     return true;
 }
 
-static int log2_exact(size_t n) {
-    int l = 0;
-    while (((size_t)1 << l) < n) ++l;
-    return l;
-}
-
 // One replay round on the device (verifier.rs:57-120) with the round's four cross-term
 // modules already in device memory and their shapes already checked. No host sync.
 static int verifier_round_launch(dmx_verifier* v, const zq* d_l, const zq* d_r, const zq* d_lp, const zq* d_rp) {
     hipStream_t s = v->s;
+    const Jobs& jb = v->jb;
     const size_t n = v->clen;
     const bool vec = v->rows > 1, poly = v->glen > 1;
     uint32_t* flags = v->flags.as<uint32_t>();
@@ -1081,8 +1187,8 @@ static int verifier_round_launch(dmx_verifier* v, const zq* d_l, const zq* d_r, 
     size_t rows = v->rows;
     const zq* Ct = v->C.as<zq>();
     if (vec) {
-        v->tr.challenge("vec", v->round, v->C.as<zq>(), n, d_l, n, d_r, n, v->scal.as<zq>(), flags, s);
-        launch_com_update(v->C.as<zq>(), d_l, d_r, v->scal.as<zq>(), 0, log2_exact(n), v->Ct.as<zq>(), s);
+        v->tr.challenge("vec", v->round, v->C.as<zq>(), n, d_l, n, d_r, n, v->scal.as<zq>(), flags, s, jb);
+        launch_com_update(v->C.as<zq>(), d_l, d_r, v->scal.as<zq>(), 0, log2_exact(n), v->Ct.as<zq>(), s, jb);
         Ct = v->Ct.as<zq>();
         RowSrc f{};
         f.mode = ROW_VFOLD;
@@ -1090,25 +1196,25 @@ static int verifier_round_launch(dmx_verifier* v, const zq* d_l, const zq* d_r, 
         f.b_off = (rows / 2) * K;
         f.scal = v->scal.as<zq>() + 1;  // g_L + x^-1 g_R (verifier.rs:141-151)
         f.dst = v->Gother();
-        launch_fold_rows_zq(f, (rows / 2) * K, lg, s);
+        launch_fold_rows_zq(f, (rows / 2) * K, lg, s, jb);
         v->gcur ^= 1;
         rows /= 2;
     }
     size_t nclen = n, nglen = v->glen;
     if (poly) {
-        v->tr.challenge("poly", v->round, Ct, n, d_lp, n / 2, d_rp, n / 2, v->scal.as<zq>() + 2, flags, s);
-        launch_com_update(Ct, d_lp, d_rp, v->scal.as<zq>() + 2, 1, log2_exact(n / 2), v->C.as<zq>(), s);
+        v->tr.challenge("poly", v->round, Ct, n, d_lp, n / 2, d_rp, n / 2, v->scal.as<zq>() + 2, flags, s, jb);
+        launch_com_update(Ct, d_lp, d_rp, v->scal.as<zq>() + 2, 1, log2_exact(n / 2), v->C.as<zq>(), s, jb);
         RowSrc f{};
         f.mode = ROW_PFOLD_G;
         f.a = v->G();
         f.scal = v->scal.as<zq>() + 3;  // g_e + y^-1 X g_o (verifier.rs:153-179)
         f.dst = v->Gother();
-        launch_fold_rows_zq(f, rows * K, lg - 1, s);
+        launch_fold_rows_zq(f, rows * K, lg - 1, s, jb);
         v->gcur ^= 1;
         nclen = n / 2;
         nglen = v->glen / 2;
     } else if (vec) {
-        launch_copy16(v->C.p, v->Ct.p, (size_t)K * n, s);
+        launch_copy16(v->C.p, v->Ct.p, (size_t)K * n, s, jb);
     }
     HIP_TRY(hipGetLastError());
     v->rows = rows;
@@ -1176,8 +1282,8 @@ static int verifier_final_launch(dmx_verifier* v, const zq* d_opening) {
     if (v->round != v->rounds) return fail(DMX_ERR_NOT_FINAL, "cannot verify final opening before all rounds replay");
     if (v->rows != 1) return fail(DMX_ERR_NOT_FINAL, "terminal generator family must have length one");
     if (v->glen != v->clen) return fail(DMX_ERR_OPENING, "final opening mismatch");
-    launch_fill32(v->flags.as<uint32_t>(), 1, 0, v->s);
-    launch_final_check(v->G(), v->C.as<zq>(), d_opening, (size_t)K * v->glen, v->flags.as<uint32_t>(), 1, v->s);
+    launch_fill32(v->flags.as<uint32_t>(), 1, 0, v->s, v->jb);
+    launch_final_check(v->G(), v->C.as<zq>(), d_opening, (size_t)K * v->glen, v->flags.as<uint32_t>(), 1, v->s, v->jb);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
@@ -1249,53 +1355,92 @@ extern "C" int dmx_verify(dmx_ctx* ctx, const dmx_system_params* params, const u
 }
 
 // ------------------------------------------------------------------------------------------
-// fused device job: prove + replay of a device-resident file.
-// The prover and the verifier each run on their own stream; the verifier consumes the prover's
-// device-resident records, so its generator derivation and folds overlap the prover's next
-// round, and both parties recompute every challenge themselves. Workspaces are cached per
-// (depth, length, seeds) in the context and the whole two-stream job is captured once into a
-// hipGraph that later jobs replay: one launch instead of ~400.
+// Batched device jobs: prove + replay of n files of one depth side by side.
+// The prover and the verifier each run on their own stream over one shared workspace (n job
+// blocks); every launch covers all n jobs. The verifier's generator derivation starts with the
+// job (it needs only the seed), and from then on it consumes the prover's device-resident
+// records round by round, recomputing every challenge itself. File lengths, seeds and source
+// addresses live in the per-job headers, so one captured hipGraph per (depth, n) replays for
+// any files of that depth: one launch instead of ~400 per batch.
 // ------------------------------------------------------------------------------------------
-struct FusedJob {
+struct BatchJob {
+    dmx_ctx* ctx = nullptr;
     int d = 0;
-    size_t len = 0;
-    dmx_system_params params{};
+    uint32_t n = 0;
+    DevBuf store;
     std::unique_ptr<dmx_prover> p;
     std::unique_ptr<dmx_verifier> v;
     std::vector<hipEvent_t> evs;
+    hipEvent_t done = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    ~FusedJob() {
+    bool busy = false;
+    uint64_t last_use = 0;
+    ~BatchJob() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (auto e : evs) (void)hipEventDestroy(e);
+        if (done) (void)hipEventDestroy(done);
     }
+    size_t bytes() const { return store.bytes; }
 };
 
-static std::mutex g_jobs_mu;
-static std::vector<std::pair<dmx_ctx*, std::unique_ptr<FusedJob>>> g_jobs;
+static size_t max_file_len(int d) { return (size_t)BYTES_PER_COEFF << (2 * d); }
 
-// all device work of one job, enqueued on p->s (forking to v->s); no host syncs
-static int fused_enqueue(FusedJob* J) {
+static int batch_create(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
+    std::unique_ptr<BatchJob> J(new BatchJob);
+    J->ctx = ctx;
+    J->d = d;
+    J->n = n;
+    dmx_prover* p = nullptr;
+    int rc = prover_new_object(ctx, d, n, &p);
+    if (rc) return rc;
+    J->p.reset(p);
+    p->hash_cap = max_file_len(d);
+    dmx_verifier* v = nullptr;
+    rc = verifier_new_object(ctx, (size_t)d, p->N, p->N, n, &v);
+    if (rc) return rc;
+    J->v.reset(v);
+    Layout L;
+    prover_plan(p, L, p->hash_cap);
+    verifier_plan(v, L);
+    rc = L.commit(J->store, n);
+    if (rc) return rc;
+    p->jb = v->jb = Jobs{n, L.stride()};
+    prover_bind(p);
+    verifier_bind(v);
+    J->evs.resize((size_t)d + 3);
+    for (auto& e : J->evs) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&J->done, hipEventDisableTiming));
+    *out = J.release();
+    return DMX_OK;
+}
+
+// all device work of one batch, enqueued on p->s (forking to v->s); no host syncs
+static int batch_enqueue(BatchJob* J) {
     dmx_prover* p = J->p.get();
     dmx_verifier* v = J->v.get();
-    // reset host-side bookkeeping of both parties to the start of a job
     p->round = 0;
     v->round = 0;
     v->rows = p->N;
     v->glen = v->clen = p->N;
     v->gcur = 0;
-    int rc = prover_initialize(p);
-    if (rc) return rc;
     HIP_TRY(hipEventRecord(J->evs[0], p->s));
     HIP_TRY(hipStreamWaitEvent(v->s, J->evs[0], 0));  // fork the verifier stream
-    rc = verifier_enqueue_setup(v, p->tr.file_id(), p->com0.as<zq>(), nullptr);
+    int rc = verifier_enqueue_gen(v);
+    if (rc) return rc;
+    launch_copy_in(p->meta.as<uint64_t>(), p->file.as<uint8_t>(), p->hash_cap, p->s, p->jb);
+    rc = prover_initialize(p);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(J->evs[1], p->s));
+    HIP_TRY(hipStreamWaitEvent(v->s, J->evs[1], 0));
+    rc = verifier_enqueue_statement(v, p->tr.file_id(), p->com0.as<zq>());
     if (rc) return rc;
     for (size_t j = 0; j < (size_t)p->d; ++j) {
         rc = prover_round(p, j);
         if (rc) return rc;
-        HIP_TRY(hipEventRecord(J->evs[1 + j], p->s));
-        HIP_TRY(hipStreamWaitEvent(v->s, J->evs[1 + j], 0));
+        HIP_TRY(hipEventRecord(J->evs[2 + j], p->s));
+        HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + j], 0));
         rc = verifier_round_launch(v, p->rec_lvec(j), p->rec_rvec(j), p->rec_lpoly(j), p->rec_rpoly(j));
         if (rc) return rc;
     }
@@ -1307,74 +1452,205 @@ static int fused_enqueue(FusedJob* J) {
     return DMX_OK;
 }
 
-static int fused_job_get(dmx_ctx* ctx, size_t len, const dmx_system_params* params, FusedJob** out) {
-    const int d = depth_for(len);
-    std::lock_guard<std::mutex> g(g_jobs_mu);
-    for (auto& e : g_jobs) {
-        FusedJob* J = e.second.get();
-        if (e.first == ctx && J->d == d && J->len == len && !memcmp(&J->params, params, sizeof *params)) {
-            *out = J;
-            return DMX_OK;
-        }
-    }
-    // one cached job per (context, depth): replace an older one of the same depth
-    for (size_t i = 0; i < g_jobs.size(); ++i)
-        if (g_jobs[i].first == ctx && g_jobs[i].second->d == d) {
-            g_jobs.erase(g_jobs.begin() + i);
-            break;
-        }
-    std::unique_ptr<FusedJob> J(new FusedJob);
-    J->d = d;
-    J->len = len;
-    J->params = *params;
-    dmx_runtime_config cfg{};
-    cfg.max_preprocess_bytes = UINT64_MAX;  // examples/full_flow.rs:29
-    cfg.ntt_enabled = cfg.parallel_enabled = cfg.gpu_enabled = 1;
-    dmx_prover* p = nullptr;
-    // the job owns its file buffer; callers' files are copied in before each replay
-    int rc = prover_create(ctx, nullptr, nullptr, 0, params, &cfg, &p, /*sync=*/false, len);
-    if (rc) return rc;
-    J->p.reset(p);
-    dmx_verifier* v = nullptr;
-    rc = verifier_make(ctx, params, nullptr, len, (uint64_t)d, nullptr, p->N, p->g_seed, true, &v);
-    if (rc) return rc;
-    J->v.reset(v);
-    J->evs.resize((size_t)d + 2);
-    for (auto& e : J->evs) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    HIP_TRY(hipStreamSynchronize(p->s));
-    HIP_TRY(hipStreamSynchronize(v->s));
-    g_jobs.emplace_back(ctx, std::move(J));
-    *out = g_jobs.back().second.get();
-    return DMX_OK;
-}
-
 static bool graphs_enabled() {
     const char* e = getenv("DAMASCUS_GRAPH");
     return !(e && strcmp(e, "0") == 0);
 }
 
-static int fused_run(FusedJob* J, bool timing) {
+// Writes the job headers (lengths, copy-in sources, seeds) and launches the batch. Files are
+// device addresses (copied in by the first kernel) or host addresses (copied here).
+struct JobIn {
+    const void* file;
+    uint64_t len;
+    const dmx_system_params* params;
+};
+
+static int batch_launch(BatchJob* J, const JobIn* jobs, bool host_files) {
     dmx_prover* p = J->p.get();
+    dmx_verifier* v = J->v.get();
+    for (uint32_t b = 0; b < J->n; ++b) {
+        prover_set_job(p, b, jobs[b].params, jobs[b].len, host_files ? nullptr : jobs[b].file);
+        v->stage[b] = p->stage[b];
+    }
+    const size_t st = p->jb.stride;
+    HIP_TRY(copy_rows(p->meta.p, st, p->stage.data(), sizeof(JobMeta), sizeof(JobMeta), J->n,
+                             hipMemcpyHostToDevice, p->s));
+    HIP_TRY(copy_rows(v->meta.p, st, v->stage.data(), sizeof(JobMeta), sizeof(JobMeta), J->n,
+                             hipMemcpyHostToDevice, p->s));
+    if (host_files)
+        for (uint32_t b = 0; b < J->n; ++b)
+            if (jobs[b].len)
+                HIP_TRY(hipMemcpyAsync(job_ptr(p->file.p, p->jb, b), jobs[b].file, jobs[b].len, hipMemcpyHostToDevice,
+                                       p->s));
+    bool timing;
+    {
+        std::lock_guard<std::mutex> g(J->ctx->mu);
+        timing = J->ctx->timing;
+    }
     if (!graphs_enabled() || timing) {  // eager: kernel timing needs per-launch events
-        return fused_enqueue(J);
-    }
-    if (!J->exec) {
-        p->capturing = true;
-        HIP_TRY(hipStreamBeginCapture(p->s, hipStreamCaptureModeThreadLocal));
-        int rc = fused_enqueue(J);
-        hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamEndCapture(p->s, &graph);
-        p->capturing = false;
-        if (rc) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return rc;
+        int rc = batch_enqueue(J);
+        if (rc) return rc;
+    } else {
+        if (!J->exec) {
+            p->capturing = true;
+            HIP_TRY(hipStreamBeginCapture(p->s, hipStreamCaptureModeThreadLocal));
+            int rc = batch_enqueue(J);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(p->s, &graph);
+            p->capturing = false;
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(DMX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            J->graph = graph;
+            HIP_TRY(hipGraphInstantiate(&J->exec, J->graph, nullptr, nullptr, 0));
         }
-        if (e != hipSuccess) return fail(DMX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-        J->graph = graph;
-        HIP_TRY(hipGraphInstantiate(&J->exec, J->graph, nullptr, nullptr, 0));
+        HIP_TRY(hipGraphLaunch(J->exec, p->s));
     }
-    HIP_TRY(hipGraphLaunch(J->exec, p->s));
+    HIP_TRY(hipEventRecord(J->done, p->s));
     return DMX_OK;
+}
+
+// Output of one job of a finished batch.
+struct JobOut {
+    uint8_t* statement;
+    size_t statement_cap;
+    size_t* statement_len;
+    uint8_t* records;
+    size_t records_cap;
+    size_t* records_len;
+    uint64_t* opening;
+    int* status;
+};
+
+// Waits for the batch, checks every job's flags and encodes the proofs.
+static int batch_collect(BatchJob* J, const JobOut* outs) {
+    dmx_prover* p = J->p.get();
+    dmx_verifier* v = J->v.get();
+    const uint32_t n = J->n;
+    const size_t st = p->jb.stride;
+    HIP_TRY(hipEventSynchronize(J->done));
+    std::vector<uint32_t> pf(n), vf(n);
+    std::vector<zq> com((size_t)n * K * p->N), recs((size_t)n * p->rec_off[p->d]), mstar(n);
+    std::vector<uint8_t> fid((size_t)n * 32);
+    hipStream_t s = p->s;
+    HIP_TRY(copy_rows(pf.data(), 4, p->flags.p, st, 4, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(copy_rows(vf.data(), 4, v->flags.p, st, 4, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(copy_rows(fid.data(), 32, p->tr.file_id(), st, 32, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(copy_rows(com.data(), (size_t)K * p->N * 16, p->com0.p, st, (size_t)K * p->N * 16, n,
+                             hipMemcpyDeviceToHost, s));
+    const size_t rb = p->rec_off[p->d] * 16;
+    HIP_TRY(copy_rows(recs.data(), rb, p->recs.p, st, rb, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(copy_rows(mstar.data(), 16, p->W_a.p, st, 16, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    int first = DMX_OK;
+    for (uint32_t b = 0; b < n; ++b) {
+        const JobOut& o = outs[b];
+        int rc = DMX_OK;
+        for (size_t j = 0; !rc && j < (size_t)p->d; ++j) rc = check_round_flags(j, pf[b]);
+        if (!rc && (vf[b] & 1u)) rc = fail(DMX_ERR_OPENING, "final opening mismatch");
+        if (!rc && vf[b]) rc = fail(DMX_ERR_HIP, "challenge resampling exhausted");
+        const size_t st_need = dmx_statement_size(p->N);
+        if (o.statement_len) *o.statement_len = st_need;
+        if (!rc && o.statement) {
+            if (o.statement_cap < st_need)
+                rc = fail(DMX_ERR_BUFFER, "statement buffer too small");
+            else
+                encode_statement_from(p, b, fid.data() + 32 * b, com.data() + (size_t)b * K * p->N, o.statement);
+        }
+        const size_t rec_need = dmx_proof_records_size(p->orig_len[b]);
+        if (o.records_len) *o.records_len = rec_need;
+        if (!rc && o.records) {
+            if (o.records_cap < rec_need)
+                rc = fail(DMX_ERR_BUFFER, "records buffer too small");
+            else
+                encode_records_from(p, recs.data() + (size_t)b * p->rec_off[p->d], o.records);
+        }
+        if (!rc && o.opening) {
+            o.opening[0] = (uint64_t)mstar[b].w[0] | ((uint64_t)mstar[b].w[1] << 32);
+            o.opening[1] = (uint64_t)mstar[b].w[2] | ((uint64_t)mstar[b].w[3] << 32);
+        }
+        if (o.status) *o.status = rc;
+        if (rc && !first) first = rc;
+    }
+    return first;
+}
+
+// Cache of batch workspaces and their graphs, per (context, depth, n). Idle workspaces beyond
+// the cache budget (DAMASCUS_CACHE_GB, default 32) are freed least recently used first.
+static std::mutex g_batches_mu;
+static std::vector<std::unique_ptr<BatchJob>> g_batches;
+static uint64_t g_batch_clock = 0;
+
+static size_t cache_budget() {
+    const char* e = getenv("DAMASCUS_CACHE_GB");
+    const double gb = e ? atof(e) : 32.0;
+    return (size_t)(gb * (double)(1ull << 30));
+}
+
+static void batch_trim_locked() {
+    size_t idle = 0;
+    for (auto& b : g_batches)
+        if (!b->busy) idle += b->bytes();
+    const size_t budget = cache_budget();
+    while (idle > budget) {
+        size_t victim = SIZE_MAX;
+        for (size_t i = 0; i < g_batches.size(); ++i)
+            if (!g_batches[i]->busy && (victim == SIZE_MAX || g_batches[i]->last_use < g_batches[victim]->last_use))
+                victim = i;
+        if (victim == SIZE_MAX) break;
+        idle -= g_batches[victim]->bytes();
+        g_batches.erase(g_batches.begin() + victim);
+    }
+}
+
+static int batch_acquire(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
+    {
+        std::lock_guard<std::mutex> g(g_batches_mu);
+        for (auto& b : g_batches)
+            if (b->ctx == ctx && b->d == d && b->n == n && !b->busy) {
+                b->busy = true;
+                b->last_use = ++g_batch_clock;
+                *out = b.get();
+                return DMX_OK;
+            }
+    }
+    BatchJob* J = nullptr;
+    int rc = batch_create(ctx, d, n, &J);
+    if (rc == DMX_ERR_OOM) {  // free idle workspaces and try once more
+        {
+            std::lock_guard<std::mutex> g(g_batches_mu);
+            for (size_t i = 0; i < g_batches.size();)
+                if (!g_batches[i]->busy)
+                    g_batches.erase(g_batches.begin() + i);
+                else
+                    ++i;
+        }
+        rc = batch_create(ctx, d, n, &J);
+    }
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(g_batches_mu);
+    J->busy = true;
+    J->last_use = ++g_batch_clock;
+    g_batches.emplace_back(J);
+    *out = J;
+    return DMX_OK;
+}
+
+static void batch_release(BatchJob* J) {
+    std::lock_guard<std::mutex> g(g_batches_mu);
+    J->busy = false;
+    batch_trim_locked();
+}
+
+static void batch_forget_context(dmx_ctx* ctx) {
+    std::lock_guard<std::mutex> g(g_batches_mu);
+    for (size_t i = 0; i < g_batches.size();)
+        if (g_batches[i]->ctx == ctx)
+            g_batches.erase(g_batches.begin() + i);
+        else
+            ++i;
 }
 
 extern "C" int dmx_prove_verify_device(dmx_ctx* ctx, const void* d_file, size_t len, const dmx_system_params* params,
@@ -1382,45 +1658,96 @@ extern "C" int dmx_prove_verify_device(dmx_ctx* ctx, const void* d_file, size_t 
                                        size_t rec_cap, size_t* rec_len, uint64_t opening_out[2]) {
     if (!ctx || !params || (!d_file && len)) return fail(DMX_ERR_ARG, "null argument");
     if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
-    if (depth_for(len) > 12) return fail(DMX_ERR_ARG, "depth %d exceeds the supported 12", depth_for(len));
+    const int d = depth_for(len);
+    if (d > 12) return fail(DMX_ERR_ARG, "depth %d exceeds the supported 12", d);
     HIP_TRY(hipSetDevice(ctx->device));
-    FusedJob* J = nullptr;
-    int rc = fused_job_get(ctx, len, params, &J);
+    BatchJob* J = nullptr;
+    int rc = batch_acquire(ctx, d, 1, &J);
     if (rc) return rc;
-    dmx_prover* p = J->p.get();
-    dmx_verifier* v = J->v.get();
-    if (len) HIP_TRY(hipMemcpyAsync(p->file.p, d_file, len, hipMemcpyDeviceToDevice, p->s));
-    bool timing;
-    {
-        std::lock_guard<std::mutex> g(ctx->mu);
-        timing = ctx->timing;
+    const JobIn in{d_file, len, params};
+    int status = DMX_OK;
+    const JobOut out{st_out, st_cap, st_len, rec_out, rec_cap, rec_len, opening_out, &status};
+    rc = batch_launch(J, &in, false);
+    if (!rc) rc = batch_collect(J, &out);
+    batch_release(J);
+    return rc;
+}
+
+// Ragged batch: jobs of any depths. Jobs are grouped by depth into batches of at most
+// `max_batch` (0: sized by workspace, about 16 GiB per batch); every batch is launched before
+// any is awaited, so batches of different depths run concurrently on their own streams.
+extern "C" int dmx_prove_verify_many(dmx_ctx* ctx, dmx_job* jobs, size_t count, int files_on_host,
+                                     size_t max_batch) {
+    if (!ctx || (count && !jobs)) return fail(DMX_ERR_ARG, "null argument");
+    if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<std::vector<size_t>> by_d(13);
+    for (size_t i = 0; i < count; ++i) {
+        if (!jobs[i].file && jobs[i].len) return fail(DMX_ERR_ARG, "job %zu: null file", i);
+        const int d = depth_for(jobs[i].len);
+        if (d > 12) return fail(DMX_ERR_ARG, "job %zu: depth %d exceeds the supported 12", i, d);
+        jobs[i].depth = d;
+        jobs[i].status = DMX_ERR_HIP;
+        by_d[d].push_back(i);
     }
-    rc = fused_run(J, timing);
-    if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(p->s));
-    uint32_t f[2] = {0, 0};
-    HIP_TRY(hipMemcpy(&f[0], p->flags.p, 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(&f[1], v->flags.p, 4, hipMemcpyDeviceToHost));
-    for (size_t j = 0; j < (size_t)p->d; ++j) {
-        rc = check_round_flags(p, j, f[0]);
-        if (rc) return rc;
-    }
-    if (f[1] & 1u) return fail(DMX_ERR_OPENING, "final opening mismatch");
-    if (f[1]) return fail(DMX_ERR_HIP, "challenge resampling exhausted");
-    if (st_len || st_out) {
-        rc = dmx_prover_statement(p, st_out, st_cap, st_len);
-        if (rc) return rc;
-    }
-    if (rec_len || rec_out) {
-        const size_t need = dmx_proof_records_size(len);
-        if (rec_len) *rec_len = need;
-        if (rec_out) {
-            if (rec_cap < need) return fail(DMX_ERR_BUFFER, "records buffer too small");
-            rc = encode_records(p, rec_out);
-            if (rc) return rc;
+    struct Launched {
+        BatchJob* J;
+        std::vector<size_t> idx;
+    };
+    std::vector<Launched> live;
+    int rc = DMX_OK;
+    for (int d = 12; d >= 0 && !rc; --d) {  // deepest first: their chains are longest
+        const std::vector<size_t>& ids = by_d[d];
+        if (ids.empty()) continue;
+        // workspace per job, from one layout of depth d
+        size_t per_job = 0;
+        {
+            dmx_prover tp;
+            tp.d = d;
+            tp.N = (size_t)1 << d;
+            tp.hash_cap = max_file_len(d);
+            dmx_verifier tv;
+            tv.rows = tv.glen = tv.clen = tv.c_cap = tp.N;
+            tv.g_cap = tp.N * K * tp.N;
+            Layout L;
+            prover_plan(&tp, L, tp.hash_cap);
+            verifier_plan(&tv, L);
+            per_job = L.stride();
+        }
+        size_t cap = max_batch ? max_batch : std::max<size_t>(1, ((size_t)16 << 30) / per_job);
+        cap = std::min<size_t>(cap, 65535);
+        for (size_t at = 0; at < ids.size() && !rc; at += cap) {
+            const size_t nb = std::min(cap, ids.size() - at);
+            Launched L{nullptr, std::vector<size_t>(ids.begin() + at, ids.begin() + at + nb)};
+            rc = batch_acquire(ctx, d, (uint32_t)nb, &L.J);
+            if (rc) break;
+            std::vector<JobIn> in(nb);
+            for (size_t k = 0; k < nb; ++k) {
+                const dmx_job& jj = jobs[L.idx[k]];
+                in[k] = JobIn{jj.file, jj.len, &jj.params};
+            }
+            rc = batch_launch(L.J, in.data(), files_on_host != 0);
+            live.push_back(std::move(L));
         }
     }
-    if (opening_out) HIP_TRY(hipMemcpy(opening_out, p->W_a.p, 16, hipMemcpyDeviceToHost));
+    for (auto& L : live) {
+        std::vector<JobOut> outs(L.idx.size());
+        for (size_t k = 0; k < L.idx.size(); ++k) {
+            dmx_job& jj = jobs[L.idx[k]];
+            outs[k] = JobOut{jj.statement, jj.statement_cap, &jj.statement_len, jj.records, jj.records_cap,
+                             &jj.records_len, jj.opening, &jj.status};
+        }
+        if (!rc) {
+            const int r2 = batch_collect(L.J, outs.data());
+            (void)r2;  // per-job status carries the outcome
+        } else {
+            (void)hipEventSynchronize(L.J->done);
+        }
+        batch_release(L.J);
+    }
+    if (rc) return rc;
+    for (size_t i = 0; i < count; ++i)
+        if (jobs[i].status != DMX_OK) return fail(jobs[i].status, "job %zu failed", i);
     return DMX_OK;
 }
 
@@ -1491,12 +1818,14 @@ extern "C" int dmx_derive_generators(dmx_ctx* ctx, const uint8_t g_seed[32], siz
         return fail(DMX_ERR_SHAPE, "ring length must be a power of two <= 8192");
     if (!rows) return DMX_OK;
     HIP_TRY(hipSetDevice(ctx->device));
-    DevBuf g;
+    DevBuf g, key;
     int rc = g.alloc(rows * K * ring_len * 16);
+    if (!rc) rc = key.alloc(32);
     if (rc) return rc;
     uint32_t gw[8];
     gseed_words(g_seed, gw);
-    launch_gen_generators(gw, rows * K, log2_exact(ring_len), g.as<zq>(), 0);
+    HIP_TRY(hipMemcpy(key.p, gw, 32, hipMemcpyHostToDevice));
+    launch_gen_generators(key.as<uint32_t>(), rows * K, log2_exact(ring_len), g.as<zq>(), 0);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpy(out, g.p, rows * K * ring_len * 16, hipMemcpyDeviceToHost));
     return DMX_OK;
@@ -1536,6 +1865,10 @@ static int commit_impl(dmx_ctx* ctx, const uint64_t* witness, const uint64_t* ge
     HIP_TRY(hipMemsetAsync(acc.p, 0, (size_t)K * m * 16, s));
     uint32_t gw[8] = {0};
     if (g_seed) gseed_words(g_seed, gw);
+    DevBuf key;
+    rc = key.alloc(32);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(key.p, gw, 32, hipMemcpyHostToDevice));
     for (size_t r0 = 0; r0 < rows; r0 += crow) {
         const size_t nr = std::min(crow, rows - r0);
         HIP_TRY(hipMemcpyAsync(W.p, witness + 2 * r0 * m, nr * m * 16, hipMemcpyHostToDevice, s));
@@ -1554,7 +1887,7 @@ static int commit_impl(dmx_ctx* ctx, const uint64_t* witness, const uint64_t* ge
             DevBuf full;
             rc = full.alloc((r0 + nr) * K * m * 16);
             if (rc) return rc;
-            launch_gen_generators(gw, (r0 + nr) * K, lg, full.as<zq>(), s);
+            launch_gen_generators(key.as<uint32_t>(), (r0 + nr) * K, lg, full.as<zq>(), s);
             pg.mode = ROW_PLAIN;
             pg.a = full.as<zq>() + r0 * K * m;
             launch_ntt_fwd_rows(pg, nr * K, lg, Ghat.as<uint32_t>(), ctx->tw, s);

@@ -158,6 +158,35 @@ int dmx_prove_verify_device(dmx_ctx *ctx, const void *d_file, size_t len,
                             size_t statement_cap, size_t *statement_len, uint8_t *records_out,
                             size_t records_cap, size_t *records_len, uint64_t opening_out[2]);
 
+/* One job of a batch: a file to prove and replay, and where its proof goes.
+ * Inputs: file (device address, or host address when files_on_host), len, params.
+ * Outputs: statement / records (bincode, as dmx_prove writes them; NULL skips), their
+ * lengths, the opening m*, depth, and status (DMX_OK when the proof was produced and the
+ * replay accepted it). */
+typedef struct dmx_job {
+    const void *file;
+    uint64_t len;
+    dmx_system_params params;
+    uint8_t *statement;
+    size_t statement_cap;
+    size_t statement_len;
+    uint8_t *records;
+    size_t records_cap;
+    size_t records_len;
+    uint64_t opening[2];
+    int32_t depth;
+    int32_t status;
+} dmx_job;
+
+/* prove + replay of `count` independent files of any sizes (a ragged batch). Jobs of one depth
+ * run side by side in shared launches (at most max_batch per launch; 0 sizes batches by
+ * workspace), batches of different depths run concurrently, and every result is bit-identical
+ * to proving that file alone. No reference counterpart: the reference proves one file per
+ * call (examples/full_flow.rs:10-112); this is the batched form of that loop. Returns DMX_OK
+ * when every job succeeded; per-job outcomes are in jobs[i].status. */
+int dmx_prove_verify_many(dmx_ctx *ctx, dmx_job *jobs, size_t count, int files_on_host,
+                          size_t max_batch);
+
 /* ---- device buffers for callers that keep inputs resident in HBM. No counterpart in the
  * reference (its boundary takes host buffers only); for benchmarks and serving loops that
  * hand over files already on the device. */

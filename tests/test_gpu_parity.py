@@ -313,6 +313,43 @@ def test_graph_replays_match_eager_runs():
     assert dm.prove_verify_device(buf2.ptr, len(data)) == dm.prove(bytes(reversed(data)))
 
 
+def _rand_file(rng, n):
+    return rng.integers(0, 256, size=n, dtype=np.uint8).tobytes()
+
+
+def test_batch_of_one_depth_equals_solo_runs():
+    """five d=6 files of different lengths and seeds proved side by side in one launch sequence:
+    each proof equals proving that file alone and the oracle's, so no job reads another's data"""
+    rng = np.random.default_rng(2024)
+    sizes = [0, 1, 4096, 13 * 4096, 50000]  # all d = 6 (capacity 53,248 bytes)
+    files = [_rand_file(rng, n) for n in sizes]
+    params = [dm.SystemParams(bytes([i + 1] * 32), bytes([2 * i + 3] * 32)) for i in range(len(files))]
+    say("batch d6", sizes)
+    out = dm.prove_verify_many(files, params)
+    for f, p, o in zip(files, params, out):
+        assert o == dm.prove(f, p), len(f)
+    rc, ost, orecs, oop = O.full_flow(files[3], bytes([4] * 32), bytes([9] * 32))
+    assert rc == 0 and out[3] == (ost, orecs, oop)
+
+
+def test_ragged_batch_mixed_depths_device_files():
+    """a ragged batch over d = 6..8 with device-resident files, split into launches of at most
+    two jobs, run twice with new contents of the same lengths (the cached graphs replay):
+    every proof equals its solo run"""
+    rng = np.random.default_rng(77)
+    sizes = [100, 60000, 200000, 53248, 300000, 13 * 4 ** 7, 700000, 5]  # d = 6,7,7,6,8,7,8,6
+    ctx = dm.Context.default()
+    for rep in range(2):
+        files = [_rand_file(rng, n) for n in sizes]
+        bufs = [ctx.upload(f) for f in files]
+        say("ragged batch rep", rep, sizes)
+        out = dm.prove_verify_many([(b.ptr, len(f)) for b, f in zip(bufs, files)], max_batch=2)
+        for f, o in zip(files, out):
+            assert o == dm.prove(f), len(f)
+    rc, ost, orecs, oop = O.full_flow(files[2])
+    assert rc == 0 and out[2] == (ost, orecs, oop)
+
+
 def test_kernel_timing_counts_forward_transforms():
     ctx = dm.Context.default()
     data = W.pattern_bytes(8192, 3)
