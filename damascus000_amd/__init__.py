@@ -16,7 +16,7 @@ import numpy as np
 __all__ = [
     "Q", "K", "PRIMES", "DamascusError", "SystemParams", "RuntimeConfig", "Context", "backend_ready",
     "device_summary", "ModuleSisCommitter", "DamascusProver", "DamascusVerifier", "ntt_batch", "fold_batch",
-    "ring_mul_batch", "prove", "verify", "prove_verify_device", "prove_verify_many", "derive_subseed", "lib", "LIB_PATH",
+    "ring_mul_batch", "prove", "verify", "prove_verify_device", "prove_verify_many", "Server", "derive_subseed", "lib", "LIB_PATH",
     "DeviceBuffer",
 ]
 
@@ -29,7 +29,7 @@ DMX_OK = 0
 ERRORS = {
     -1: "ERR_ARG", -2: "ERR_NO_DEVICE", -3: "ERR_HIP", -4: "ERR_MEMORY_LIMIT", -5: "ERR_ROUND",
     -6: "ERR_INVARIANT", -7: "ERR_SHAPE", -8: "ERR_OPENING", -9: "ERR_DECODE", -10: "ERR_NOT_FINAL",
-    -11: "ERR_ZERO_TERMS", -12: "ERR_OOM", -13: "ERR_BUFFER", -14: "ERR_POISONED",
+    -11: "ERR_ZERO_TERMS", -12: "ERR_OOM", -13: "ERR_BUFFER", -14: "ERR_POISONED", -15: "ERR_PENDING",
 }
 ERR_CODE = {v: k for k, v in ERRORS.items()}
 
@@ -87,6 +87,11 @@ class Job(ctypes.Structure):
                 ("status", ctypes.c_int32)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_uint32),
+                ("max_inflight", ctypes.c_uint32), ("files_on_host", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "dmx_backend_ready": (ctypes.c_int, []),
@@ -122,6 +127,10 @@ _SIGS = {
     "dmx_verify": (ctypes.c_int, [_VP, _VP, _VP, _SZ, _VP, _SZ, _VP]),
     "dmx_prove_verify_device": (ctypes.c_int, [_VP, _VP, _SZ, _VP, _VP, _SZ, _SZP, _VP, _SZ, _SZP, _VP]),
     "dmx_prove_verify_many": (ctypes.c_int, [_VP, _VP, _SZ, ctypes.c_int, _SZ]),
+    "dmx_server_create": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(_VP)]),
+    "dmx_server_submit": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_uint64)]),
+    "dmx_server_wait": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]),
+    "dmx_server_destroy": (None, [_VP]),
     "dmx_device_alloc": (ctypes.c_int, [_VP, _SZ, ctypes.POINTER(_VP)]),
     "dmx_device_free": (ctypes.c_int, [_VP, _VP]),
     "dmx_memcpy_htod": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
@@ -503,6 +512,76 @@ def prove_verify_many(files, params=None, ctx=None, max_batch=0, want_proof=True
         else:
             out.append(None)
     return out
+
+
+def _job_for(f, params, want_proof, keep):
+    j = Job()
+    if isinstance(f, tuple):
+        j.file, j.len = _VP(f[0]), f[1]
+    else:
+        buf = ctypes.create_string_buffer(bytes(f), max(len(f), 1))
+        keep.append(buf)
+        j.file, j.len = ctypes.cast(buf, _VP), len(f)
+    j.params = params
+    if want_proof:
+        st = ctypes.create_string_buffer(lib().dmx_statement_size(4096))
+        rc = ctypes.create_string_buffer(max(lib().dmx_proof_records_size(j.len), 1))
+        keep += [st, rc]
+        j.statement, j.statement_cap = ctypes.cast(st, _VP), len(st)
+        j.records, j.records_cap = ctypes.cast(rc, _VP), len(rc)
+    return j
+
+
+def _job_result(j, want_proof):
+    if j.status != DMX_OK:
+        return int(j.status)
+    if not want_proof:
+        return None
+    return (ctypes.string_at(j.statement, j.statement_len), ctypes.string_at(j.records, j.records_len),
+            int(j.opening[0]) | (int(j.opening[1]) << 64))
+
+
+class Server:
+    """Continuous-batching prove+verify service (dmx_server_*). Files are (device_pointer,
+    length) pairs, or bytes when files_on_host. submit() returns a ticket; wait(ticket) returns
+    (result, latency_seconds) where result is (statement, records, opening) or an error code."""
+
+    def __init__(self, ctx=None, hbm_budget=0, max_batch=0, max_inflight=0, files_on_host=False):
+        self.ctx = ctx or Context.default()
+        self.files_on_host = bool(files_on_host)
+        cfg = ServerConfig(hbm_budget, max_batch, max_inflight, int(files_on_host), 0)
+        h = _VP()
+        _check(lib().dmx_server_create(self.ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self._live = {}
+
+    def submit(self, f, params=None, want_proof=True):
+        keep = []
+        j = _job_for(f, params or SystemParams(), want_proof, keep)
+        t = ctypes.c_uint64()
+        _check(lib().dmx_server_submit(self.h, ctypes.byref(j), ctypes.byref(t)))
+        self._live[t.value] = (j, keep, want_proof)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        lat = ctypes.c_double()
+        rc = lib().dmx_server_wait(self.h, ticket, timeout_ms, ctypes.byref(lat))
+        if rc == ERR_CODE["ERR_PENDING"]:
+            return None
+        _check(rc)
+        j, keep, want = self._live.pop(ticket)
+        return _job_result(j, want), lat.value
+
+    def close(self):
+        if self.h:
+            lib().dmx_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def ntt_batch(a, b, n, primes, stage_roots, inv_roots, inv_sizes):

@@ -185,15 +185,16 @@ __device__ void chunk_cv_quad(const uint8_t* msg, size_t len, size_t ci, const u
     const size_t start = ci * b3::CHUNK_LEN;
     const size_t end = (start + b3::CHUNK_LEN < len) ? start + b3::CHUNK_LEN : len;
     const size_t nb = end > start ? (end - start + 63) / 64 : 1;
-    for (size_t b = 0; b < nb; ++b) {
-        uint32_t m[16], blen;
-        if (end > start)
-            load_block(msg, len, start + 64 * b, m, blen);
-        else {
+    // block b + 1 is loaded while block b is compressed, so only the first load's latency shows
+    uint32_t m[16], nxt[16], blen = 0, nlen = 0;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) m[i] = 0;
-            blen = 0;
-        }
+    for (int i = 0; i < 16; ++i) nxt[i] = 0;
+    if (end > start) load_block(msg, len, start, nxt, nlen);
+    for (size_t b = 0; b < nb; ++b) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) m[i] = nxt[i];
+        blen = nlen;
+        if (b + 1 < nb) load_block(msg, len, start + 64 * (b + 1), nxt, nlen);
         uint32_t fl = flags | (b == 0 ? b3::CHUNK_START : 0) | (b + 1 == nb ? b3::CHUNK_END : 0);
         if (root && b + 1 == nb) fl |= b3::ROOT;
         compress_quad(ca, cb, m, ci, blen, fl, sel);
@@ -530,6 +531,22 @@ __global__ void __launch_bounds__(1024) k_b3_root(uint32_t* __restrict__ cvs, si
             if (threadIdx.x < 8) dig[threadIdx.x] = r[threadIdx.x];
             __syncthreads();
         }
+        // The challenge and its inverse are computed by all of wave 0 in vector registers
+        // (identical values in every lane); lane 0 stores them. A single-lane block is
+        // compiled to scalar code, with about twice the instructions for the 112-bit products.
+        if (threadIdx.x < 64 && chal_out) {
+            uint32_t h[8];
+            // identity quad permutation: same value, but a per-lane register to the compiler
+            for (int j = 0; j < 8; ++j) h[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)dig[j], 0xE4, 0xF, 0xF, false);
+            const zq x = zq_from_digest(h);
+            if (!zq_is_zero(x)) {
+                const zq xi = zq_inv_fermat(x);
+                if (threadIdx.x == 0) {
+                    chal_out[0] = x;
+                    chal_out[1] = xi;
+                }
+            }
+        }
         if (threadIdx.x == 0) {
             again = 0;
             if (digest_out)
@@ -539,7 +556,7 @@ __global__ void __launch_bounds__(1024) k_b3_root(uint32_t* __restrict__ cvs, si
                 for (int j = 0; j < 8; ++j) h[j] = dig[j];
                 const zq x = zq_from_digest(h);
                 if (!zq_is_zero(x)) {
-                    write_challenge_pair(chal_out, x);
+                    // written above
                 } else if (!chunk_level) {
                     atomicOr(err, 0x40000000u);  // group CVs cannot be patched here
                 } else if (counter >= 64) {

@@ -7,6 +7,9 @@
 // The reference transforms each product separately with a 2n-point cyclic NTT; here each ring
 // element is transformed once with an n-point negacyclic NTT and the commitment sum is formed
 // in the NTT domain, so a commitment costs (rows*(1+K) + K) transforms instead of 3*rows*K.
+#include <cstdlib>
+#include <cstring>
+
 #include "dmx_blake3.hpp"
 #include "dmx_kernels.hpp"
 
@@ -146,9 +149,142 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(RowSrc src, int lo
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Register-blocked forward transform for degrees 2^6..2^11. A row's 8 prime slices are each
+// handled by TP threads holding E = m/TP elements (TP <= E). Pass 1: thread l of a slice holds
+// elements l + TP*i and runs every layer with len >= TP in registers (its twiddles depend only
+// on i, so they are shared by the slice's threads). One LDS transpose later, pass 2: thread l
+// holds the contiguous elements E*l .. E*l+E-1 and runs the layers len < TP. Two LDS round
+// trips per row instead of one per layer. LDS slots are XOR-swizzled so that both access
+// patterns, and the slices sharing a 32-lane LDS pass, hit distinct banks.
+// ---------------------------------------------------------------------------------------
+template <int E, int TP>
+struct RegNtt {
+    static constexpr int M = E * TP;
+    static constexpr int LOG_E = __builtin_ctz(E);
+    static constexpr int LOG_TP = __builtin_ctz(TP);
+    static constexpr int LOG_M = LOG_E + LOG_TP;
+    static constexpr int TPR = NPRIMES * TP;  // threads per row
+    static constexpr int RPB = NTT_THREADS / TPR;
+    static constexpr int SWM = (E < 32 ? E : 32) - 1;
+    static constexpr size_t LDS_BYTES = (size_t)RPB * NPRIMES * M * 4;
+    __device__ static __forceinline__ int slot(int q, int x) {
+        return q * M + (x ^ ((x >> LOG_E) & SWM) ^ ((q * TP) & 31));
+    }
+};
+
+__device__ __forceinline__ void ct_butterfly(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp, uint32_t p) {
+    const uint32_t u = x;
+    const uint32_t v = mulc(y, w, wp, p);
+    x = addp(u, v, p);
+    y = subp(u, v, p);
+}
+
+template <int E, int TP>
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(RowSrc src, uint32_t* __restrict__ out, NttTables tw,
+                                                             size_t js, uint32_t rows) {
+    using G = RegNtt<E, TP>;
+    extern __shared__ uint32_t sh[];
+    const size_t jo = job_off(js);
+    src.a = jp(src.a, jo);
+    src.dst = jp(src.dst, jo);
+    src.scal = jp(src.scal, jo);
+    src.gseed = jp(src.gseed, jo);
+    out = jp(out, jo);
+    uint32_t key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (src.mode == ROW_GEN) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) key[i] = src.gseed[i];
+    }
+    const int tr = threadIdx.x % G::TPR;
+    const int rsub = threadIdx.x / G::TPR;
+    const size_t row = (size_t)blockIdx.x * G::RPB + rsub;
+    const bool valid = row < rows;
+    uint32_t* S = sh + rsub * NPRIMES * G::M;
+    // produce the row once, reduce into the 8 primes
+    if (valid) {
+        for (int c = tr; c < G::M; c += G::TPR) {
+            const zq x = row_value(src, key, row, c, G::LOG_M);
+            if (src.dst) store_zq(src.dst + row * G::M + c, x);
+#pragma unroll
+            for (int q = 0; q < NPRIMES; ++q) S[G::slot(q, c)] = zq_mod_p_centred(x, c_pc[q]);
+        }
+    }
+    __syncthreads();
+    const int q = tr >> G::LOG_TP, l = tr & (TP - 1);
+    const uint32_t p = c_pc[q].p, pinv = c_pc[q].pinv;
+    const uint32_t* zt = tw.zeta_m + q * TW_SIZE;
+    uint32_t a[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) a[i] = S[G::slot(q, l + TP * i)];
+    // pass 1: len = s*TP for s = E/2 .. 1; block b of 2s register slots uses zeta[E/(2s) + b]
+#pragma unroll
+    for (int s = E / 2; s >= 1; s >>= 1) {
+#pragma unroll
+        for (int b = 0; b < E / (2 * s); ++b) {
+            const uint32_t w = zt[E / (2 * s) + b];
+            const uint32_t wp = w * pinv;
+#pragma unroll
+            for (int k = 0; k < s; ++k) ct_butterfly(a[2 * s * b + k], a[2 * s * b + k + s], w, wp, p);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < E; ++i) S[G::slot(q, l + TP * i)] = a[i];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) a[e] = S[G::slot(q, E * l + e)];
+    // pass 2: len = TP/2 .. 1 inside the thread's E contiguous elements;
+    // block b uses zeta[M/(2 len) + E*l/(2 len) + b]
+#pragma unroll
+    for (int len = TP / 2; len >= 1; len >>= 1) {
+        const int base = G::M / (2 * len) + (E * l) / (2 * len);
+#pragma unroll
+        for (int b = 0; b < E / (2 * len); ++b) {
+            const uint32_t w = zt[base + b];
+            const uint32_t wp = w * pinv;
+#pragma unroll
+            for (int k = 0; k < len; ++k) ct_butterfly(a[2 * len * b + k], a[2 * len * b + k + len], w, wp, p);
+        }
+    }
+    if (valid) {
+        uint4* o = reinterpret_cast<uint4*>(out + (row * NPRIMES + q) * G::M + E * l);
+#pragma unroll
+        for (int e = 0; e < E; e += 4) o[e / 4] = make_uint4(a[e], a[e + 1], a[e + 2], a[e + 3]);
+    }
+}
+
+template <int E, int TP>
+static void launch_reg(const RowSrc& src, size_t rows, uint32_t* out, const NttTables& tw, hipStream_t s,
+                       const Jobs& jb) {
+    using G = RegNtt<E, TP>;
+    const size_t blocks = (rows + G::RPB - 1) / G::RPB;
+    hipLaunchKernelGGL((k_ntt_fwd_reg<E, TP>), dim3((unsigned)blocks, jb.n), dim3(NTT_THREADS), G::LDS_BYTES, s, src,
+                       out, tw, jb.stride, (uint32_t)rows);
+}
+
+// DAMASCUS_NTT_KERNEL=layer keeps every degree on the per-layer kernel (for comparisons)
+static bool reg_ntt_enabled() {
+    static const bool on = [] {
+        const char* e = getenv("DAMASCUS_NTT_KERNEL");
+        return !(e && strcmp(e, "layer") == 0);
+    }();
+    return on;
+}
+
 void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out,
                          const NttTables& tw, hipStream_t s, const Jobs& jb) {
     if (rows == 0 || jb.n == 0) return;
+    if (reg_ntt_enabled()) {
+        switch (log_m) {
+            case 6: return launch_reg<8, 8>(src, rows, out, tw, s, jb);
+            case 7: return launch_reg<16, 8>(src, rows, out, tw, s, jb);
+            case 8: return launch_reg<16, 16>(src, rows, out, tw, s, jb);
+            case 9: return launch_reg<32, 16>(src, rows, out, tw, s, jb);
+            case 10: return launch_reg<32, 32>(src, rows, out, tw, s, jb);
+            case 11: return launch_reg<64, 32>(src, rows, out, tw, s, jb);
+            default: break;
+        }
+    }
     const int m = 1 << log_m;
     const int pg = std::min(NPRIMES, LDS_WORDS >> log_m);
     const size_t lds = (size_t)pg * m * sizeof(uint32_t);

@@ -8,6 +8,11 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+#include <unordered_map>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -718,18 +723,26 @@ static void prover_bind(dmx_prover* p) {
     if (p->file.p) p->d_file = p->file.as<uint8_t>();
 }
 
-// initialize_with_config (prover.rs:61-111): file_id, witness, generators, com_0 and the
-// round-0 vector cross terms, all from one pass over the generator transforms.
-static int prover_initialize(dmx_prover* p) {
+// initialize_with_config (prover.rs:61-111), first part: file_id and the witness.
+static int prover_init_file(dmx_prover* p) {
+    hipStream_t s = p->s;
+    const Jobs& jb = p->jb;
+    launch_fill32(p->flags.as<uint32_t>(), 64, 0, s, jb);
+    device_hash(p->d_file, p->hash_cap, IVW, 0, p->tr.cvs.as<uint32_t>(), p->tr.cv_words,
+                reinterpret_cast<uint32_t*>(p->tr.file_id()), nullptr, p->flags.as<uint32_t>(), s, jb, p->dlen());
+    launch_expand(p->d_file, p->hash_cap, p->W_a.as<zq>(), p->N * p->N, s, jb, p->dlen());
+    HIP_TRY(hipGetLastError());
+    return DMX_OK;
+}
+
+// second part: generators, com_0 and the round-0 vector cross terms, all from one pass over
+// the generator transforms.
+static int prover_init_commit(dmx_prover* p) {
     hipStream_t s = p->s;
     const Jobs& jb = p->jb;
     const size_t N = p->N;
     const int d = p->d;
     const NttTables& tw = p->ctx->tw;
-    launch_fill32(p->flags.as<uint32_t>(), 64, 0, s, jb);
-    device_hash(p->d_file, p->hash_cap, IVW, 0, p->tr.cvs.as<uint32_t>(), p->tr.cv_words,
-                reinterpret_cast<uint32_t*>(p->tr.file_id()), nullptr, p->flags.as<uint32_t>(), s, jb, p->dlen());
-    launch_expand(p->d_file, p->hash_cap, p->W_a.as<zq>(), N * N, s, jb, p->dlen());
     RowSrc gen{};
     gen.mode = ROW_GEN;
     gen.dst = p->G_a.as<zq>();
@@ -748,6 +761,11 @@ static int prover_initialize(dmx_prover* p) {
     launch_copy16(p->com0.p, p->C.p, (size_t)K * N, s, jb);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
+}
+
+static int prover_initialize(dmx_prover* p) {
+    int rc = prover_init_file(p);
+    return rc ? rc : prover_init_commit(p);
 }
 
 // fold_round (prover.rs:140-313) for round j, m = N >> j: both stages and both invariants.
@@ -1425,12 +1443,16 @@ static int batch_enqueue(BatchJob* J) {
     v->rows = p->N;
     v->glen = v->clen = p->N;
     v->gcur = 0;
-    HIP_TRY(hipEventRecord(J->evs[0], p->s));
-    HIP_TRY(hipStreamWaitEvent(v->s, J->evs[0], 0));  // fork the verifier stream
-    int rc = verifier_enqueue_gen(v);
-    if (rc) return rc;
     launch_copy_in(p->meta.as<uint64_t>(), p->file.as<uint8_t>(), p->hash_cap, p->s, p->jb);
-    rc = prover_initialize(p);
+    int rc = prover_init_file(p);
+    if (rc) return rc;
+    // fork the verifier stream: its generator expansion (it needs only the seed) overlaps the
+    // prover's generator transforms rather than the latency-bound file hash before them
+    HIP_TRY(hipEventRecord(J->evs[0], p->s));
+    HIP_TRY(hipStreamWaitEvent(v->s, J->evs[0], 0));
+    rc = verifier_enqueue_gen(v);
+    if (rc) return rc;
+    rc = prover_init_commit(p);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(J->evs[1], p->s));
     HIP_TRY(hipStreamWaitEvent(v->s, J->evs[1], 0));
@@ -1673,6 +1695,21 @@ extern "C" int dmx_prove_verify_device(dmx_ctx* ctx, const void* d_file, size_t 
     return rc;
 }
 
+// device memory of one job's block in a batch of depth d
+static size_t job_workspace_bytes(int d) {
+    dmx_prover tp;
+    tp.d = d;
+    tp.N = (size_t)1 << d;
+    tp.hash_cap = max_file_len(d);
+    dmx_verifier tv;
+    tv.rows = tv.glen = tv.clen = tv.c_cap = tp.N;
+    tv.g_cap = tp.N * K * tp.N;
+    Layout L;
+    prover_plan(&tp, L, tp.hash_cap);
+    verifier_plan(&tv, L);
+    return L.stride();
+}
+
 // Ragged batch: jobs of any depths. Jobs are grouped by depth into batches of at most
 // `max_batch` (0: sized by workspace, about 16 GiB per batch); every batch is launched before
 // any is awaited, so batches of different depths run concurrently on their own streams.
@@ -1699,21 +1736,7 @@ extern "C" int dmx_prove_verify_many(dmx_ctx* ctx, dmx_job* jobs, size_t count, 
     for (int d = 12; d >= 0 && !rc; --d) {  // deepest first: their chains are longest
         const std::vector<size_t>& ids = by_d[d];
         if (ids.empty()) continue;
-        // workspace per job, from one layout of depth d
-        size_t per_job = 0;
-        {
-            dmx_prover tp;
-            tp.d = d;
-            tp.N = (size_t)1 << d;
-            tp.hash_cap = max_file_len(d);
-            dmx_verifier tv;
-            tv.rows = tv.glen = tv.clen = tv.c_cap = tp.N;
-            tv.g_cap = tp.N * K * tp.N;
-            Layout L;
-            prover_plan(&tp, L, tp.hash_cap);
-            verifier_plan(&tv, L);
-            per_job = L.stride();
-        }
+        const size_t per_job = job_workspace_bytes(d);
         size_t cap = max_batch ? max_batch : std::max<size_t>(1, ((size_t)16 << 30) / per_job);
         cap = std::min<size_t>(cap, 65535);
         for (size_t at = 0; at < ids.size() && !rc; at += cap) {
@@ -1987,4 +2010,203 @@ extern "C" int dmx_ring_mul_batch(dmx_ctx* ctx, const uint64_t* a, const uint64_
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpy(out, dO.p, total * 16, hipMemcpyDeviceToHost));
     return DMX_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Serving loop: continuous batching. A scheduler thread owns the device side: it launches
+// batches from the arrival queue whenever a batch slot and device memory are free, and polls
+// the in-flight batches' completion events, so jobs that arrive while others run are admitted
+// at once and overlap them on the device.
+// ------------------------------------------------------------------------------------------
+struct ServedJob {
+    dmx_job* job = nullptr;
+    int d = 0;
+    double t_submit = 0, t_done = 0;
+    bool done = false;
+};
+
+struct dmx_server {
+    dmx_ctx* ctx = nullptr;
+    dmx_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<uint64_t> queue;  // tickets in arrival order
+    std::unordered_map<uint64_t, ServedJob> jobs;
+    struct Flight {
+        BatchJob* J;
+        std::vector<uint64_t> tickets;
+        size_t bytes;
+    };
+    std::vector<Flight> flights;
+    size_t hbm_used = 0;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    std::thread th;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+static void server_finish(dmx_server* S, dmx_server::Flight& f, int launch_rc) {
+    std::vector<JobOut> outs(f.tickets.size());
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        for (size_t k = 0; k < f.tickets.size(); ++k) {
+            dmx_job& jj = *S->jobs[f.tickets[k]].job;
+            outs[k] = JobOut{jj.statement, jj.statement_cap, &jj.statement_len, jj.records, jj.records_cap,
+                             &jj.records_len, jj.opening, &jj.status};
+        }
+    }
+    if (!launch_rc) {
+        (void)batch_collect(f.J, outs.data());
+    } else {
+        (void)hipEventSynchronize(f.J->done);
+        for (auto& o : outs) *o.status = launch_rc;
+    }
+    batch_release(f.J);
+    std::lock_guard<std::mutex> g(S->mu);
+    const double t = S->now();
+    for (uint64_t tk : f.tickets) {
+        ServedJob& sj = S->jobs[tk];
+        sj.done = true;
+        sj.t_done = t;
+    }
+    S->hbm_used -= f.bytes;
+    S->cv_done.notify_all();
+}
+
+static void server_loop(dmx_server* S) {
+    (void)hipSetDevice(S->ctx->device);
+    const uint32_t max_batch = S->cfg.max_batch ? S->cfg.max_batch : 64;
+    const uint32_t max_inflight = S->cfg.max_inflight ? S->cfg.max_inflight : 4;
+    for (;;) {
+        // completions, in any order
+        for (size_t i = 0; i < S->flights.size();) {
+            if (hipEventQuery(S->flights[i].J->done) == hipSuccess) {
+                server_finish(S, S->flights[i], DMX_OK);
+                S->flights.erase(S->flights.begin() + i);
+            } else {
+                (void)hipGetLastError();  // "not ready" must not surface at the next launch check
+                ++i;
+            }
+        }
+        // admission: the oldest waiting job's depth goes first; a batch takes the waiting jobs
+        // of that depth in arrival order, a power of two of them, as many as memory allows
+        for (;;) {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (S->queue.empty() || S->flights.size() >= max_inflight) break;
+            const int d = S->jobs[S->queue.front()].d;
+            std::vector<size_t> pos;
+            for (size_t i = 0; i < S->queue.size() && pos.size() < max_batch; ++i)
+                if (S->jobs[S->queue[i]].d == d) pos.push_back(i);
+            size_t nb = 1;
+            while (nb * 2 <= pos.size()) nb *= 2;
+            const size_t per = job_workspace_bytes(d);
+            while (nb > 1 && S->hbm_used + nb * per > S->cfg.hbm_budget_bytes) nb /= 2;
+            if (S->hbm_used + nb * per > S->cfg.hbm_budget_bytes && !S->flights.empty()) break;  // wait for memory
+            dmx_server::Flight f{nullptr, {}, nb * per};
+            for (size_t k = 0; k < nb; ++k) f.tickets.push_back(S->queue[pos[k]]);
+            for (size_t k = nb; k-- > 0;) S->queue.erase(S->queue.begin() + pos[k]);
+            std::vector<JobIn> in(nb);
+            for (size_t k = 0; k < nb; ++k) {
+                const dmx_job& jj = *S->jobs[f.tickets[k]].job;
+                in[k] = JobIn{jj.file, jj.len, &jj.params};
+            }
+            S->hbm_used += f.bytes;
+            lk.unlock();
+            int rc = batch_acquire(S->ctx, d, (uint32_t)nb, &f.J);
+            if (!rc) rc = batch_launch(f.J, in.data(), S->cfg.files_on_host != 0);
+            if (rc) {
+                if (f.J) {
+                    server_finish(S, f, rc);
+                } else {  // no workspace: fail these jobs
+                    std::lock_guard<std::mutex> g(S->mu);
+                    const double t = S->now();
+                    for (uint64_t tk : f.tickets) {
+                        ServedJob& sj = S->jobs[tk];
+                        sj.job->status = rc;
+                        sj.done = true;
+                        sj.t_done = t;
+                    }
+                    S->hbm_used -= f.bytes;
+                    S->cv_done.notify_all();
+                }
+                continue;
+            }
+            S->flights.push_back(std::move(f));
+        }
+        std::unique_lock<std::mutex> lk(S->mu);
+        if (S->stop && S->queue.empty() && S->flights.empty()) break;
+        if (S->flights.empty() && S->queue.empty())
+            S->cv_work.wait(lk, [S] { return S->stop || !S->queue.empty(); });
+        else  // poll the in-flight batches
+            S->cv_work.wait_for(lk, std::chrono::microseconds(20));
+    }
+}
+
+extern "C" int dmx_server_create(dmx_ctx* ctx, const dmx_server_config* cfg, dmx_server** out) {
+    if (!ctx || !out) return fail(DMX_ERR_ARG, "null argument");
+    if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<dmx_server> S(new dmx_server);
+    S->ctx = ctx;
+    if (cfg) S->cfg = *cfg;
+    if (!S->cfg.hbm_budget_bytes) {
+        size_t free_b = 0, total_b = 0;
+        HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        S->cfg.hbm_budget_bytes = free_b / 4 * 3;
+    }
+    S->th = std::thread(server_loop, S.get());
+    *out = S.release();
+    return DMX_OK;
+}
+
+extern "C" int dmx_server_submit(dmx_server* S, dmx_job* job, uint64_t* ticket) {
+    if (!S || !job || !ticket) return fail(DMX_ERR_ARG, "null argument");
+    if (!job->file && job->len) return fail(DMX_ERR_ARG, "null file");
+    const int d = depth_for(job->len);
+    if (d > 12) return fail(DMX_ERR_ARG, "depth %d exceeds the supported 12", d);
+    if (job_workspace_bytes(d) > S->cfg.hbm_budget_bytes)
+        return fail(DMX_ERR_MEMORY_LIMIT, "job workspace exceeds the server's device memory budget");
+    job->depth = d;
+    job->status = DMX_ERR_PENDING;
+    std::lock_guard<std::mutex> g(S->mu);
+    const uint64_t t = S->next_ticket++;
+    ServedJob sj;
+    sj.job = job;
+    sj.d = d;
+    sj.t_submit = S->now();
+    S->jobs[t] = sj;
+    S->queue.push_back(t);
+    *ticket = t;
+    S->cv_work.notify_one();
+    return DMX_OK;
+}
+
+extern "C" int dmx_server_wait(dmx_server* S, uint64_t ticket, int64_t timeout_ms, double* latency_s) {
+    if (!S) return fail(DMX_ERR_ARG, "null server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(ticket);
+    if (it == S->jobs.end()) return fail(DMX_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+    auto ready = [&] { return S->jobs[ticket].done; };
+    if (timeout_ms < 0)
+        S->cv_done.wait(lk, ready);
+    else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+        return DMX_ERR_PENDING;
+    const ServedJob& sj = S->jobs[ticket];
+    if (latency_s) *latency_s = sj.t_done - sj.t_submit;
+    S->jobs.erase(ticket);
+    return DMX_OK;
+}
+
+extern "C" void dmx_server_destroy(dmx_server* S) {
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->stop = true;
+        S->cv_work.notify_all();
+    }
+    if (S->th.joinable()) S->th.join();
+    delete S;
 }

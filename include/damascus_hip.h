@@ -45,6 +45,7 @@ extern "C" {
 #define DMX_ERR_OOM -12         /* device allocation failed */
 #define DMX_ERR_BUFFER -13      /* caller's output buffer too small; *len says how much */
 #define DMX_ERR_POISONED -14    /* an earlier error left the object unusable */
+#define DMX_ERR_PENDING -15     /* a served job has not finished within the wait */
 
 typedef struct dmx_ctx dmx_ctx;
 typedef struct dmx_prover dmx_prover;
@@ -186,6 +187,30 @@ typedef struct dmx_job {
  * when every job succeeded; per-job outcomes are in jobs[i].status. */
 int dmx_prove_verify_many(dmx_ctx *ctx, dmx_job *jobs, size_t count, int files_on_host,
                           size_t max_batch);
+
+/* ---- serving: continuous batching of commit+fold jobs. No counterpart in the reference
+ * (it proves one file per call); the loop that a service built on examples/full_flow.rs would
+ * run. Jobs are submitted as they arrive; a scheduler thread admits queued jobs whenever a
+ * batch slot and device memory are free (jobs of one depth share launches, batches run
+ * concurrently), so requests arriving while others run overlap on the device. Results are
+ * bit-identical to proving each file alone. The dmx_job and its file and output buffers must
+ * stay valid until the job has been waited for. */
+typedef struct dmx_server_config {
+    uint64_t hbm_budget_bytes; /* device memory for in-flight workspaces (0: 3/4 of free memory) */
+    uint32_t max_batch;        /* jobs per launch (0: 64); batches are powers of two */
+    uint32_t max_inflight;     /* batches on the device at once (0: 4) */
+    int32_t files_on_host;     /* job files are host addresses (copied in by the server) */
+    int32_t reserved;
+} dmx_server_config;
+typedef struct dmx_server dmx_server;
+int dmx_server_create(dmx_ctx *ctx, const dmx_server_config *cfg, dmx_server **out);
+int dmx_server_submit(dmx_server *srv, dmx_job *job, uint64_t *ticket);
+/* Waits up to timeout_ms (negative: no limit) for a job. DMX_OK when it has finished (its
+ * outcome is in job->status; latency_s = submit to verified result) and the ticket is
+ * released; DMX_ERR_PENDING when it is still running. */
+int dmx_server_wait(dmx_server *srv, uint64_t ticket, int64_t timeout_ms, double *latency_s);
+/* finishes every submitted job, then stops the scheduler */
+void dmx_server_destroy(dmx_server *srv);
 
 /* ---- device buffers for callers that keep inputs resident in HBM. No counterpart in the
  * reference (its boundary takes host buffers only); for benchmarks and serving loops that

@@ -179,7 +179,7 @@ def test_prover_bit_exact_with_oracle(size, tweak):
 
 
 def test_c1_random_file_full_flow_matches_oracle():
-    """53,248 seeded uniform bytes, d=6, default seeds: statement, records, m*"""
+    """the CPU-runnable case: 53,248 seeded uniform bytes, d=6, default seeds: statement, records, m*"""
     data = np.random.default_rng(20261203).integers(0, 256, size=53248, dtype=np.uint8).tobytes()
     say("C1 full flow")
     st, recs, op = dm.prove(data)
@@ -389,3 +389,48 @@ def test_every_byte_flip_of_first_two_records_is_rejected_on_gpu():
             except dm.DamascusError:
                 rejected = True
             assert rejected, (ri, bi)
+
+
+def test_server_results_equal_solo_runs_in_the_mix():
+    """jobs submitted to the continuous-batching server while earlier ones run (depths 6..8,
+    staggered arrivals, batches of up to 4, three batches in flight) give exactly the proofs of
+    solo runs, and the oracle's"""
+    import time
+
+    rng = np.random.default_rng(515)
+    sizes = [70000, 100, 300000, 53248, 250000, 9000, 800000, 60000, 13, 200000]
+    files = [_rand_file(rng, n) for n in sizes]
+    params = [dm.SystemParams(bytes([i + 3] * 32), bytes([i + 5] * 32)) for i in range(len(files))]
+    ctx = dm.Context.default()
+    bufs = [ctx.upload(f) for f in files]
+    srv = dm.Server(ctx, max_batch=4, max_inflight=3)
+    try:
+        tickets = []
+        for i, (b, f) in enumerate(zip(bufs, files)):
+            say("server submit", i, len(f))
+            tickets.append(srv.submit((b.ptr, len(f)), params[i]))
+            if i % 3 == 2:
+                time.sleep(0.002)  # later arrivals land while earlier batches run
+        res = [srv.wait(t) for t in tickets]
+    finally:
+        srv.close()
+    for f, p, (r, lat) in zip(files, params, res):
+        assert r == dm.prove(f, p), len(f)
+        assert lat > 0
+    rc, ost, orecs, oop = O.full_flow(files[1], bytes([4] * 32), bytes([6] * 32))
+    assert rc == 0 and res[1][0] == (ost, orecs, oop)
+
+
+def test_server_host_files_and_pending_wait():
+    rng = np.random.default_rng(516)
+    files = [_rand_file(rng, n) for n in (5000, 120000, 40000)]
+    srv = dm.Server(files_on_host=True, max_batch=2)
+    try:
+        tickets = [srv.submit(f) for f in files]
+        res = [srv.wait(t) for t in tickets]
+        with pytest.raises(dm.DamascusError):
+            srv.wait(tickets[0])  # a ticket is released once waited for
+    finally:
+        srv.close()
+    for f, (r, lat) in zip(files, res):
+        assert r == dm.prove(f)
