@@ -1,19 +1,30 @@
 #!/usr/bin/env python3
-"""Bit-exact commit+fold throughput on one MI355X.
+"""Bit-exact commit+fold throughput on one MI355X, on a mixed d=7..11 served load.
 
-Workload: one 8 MiB file of seeded uniform random bytes (645,278 coefficients, d = 10, a 1024 x
-1024 witness), K = 8, default seeds. One step = the whole job on the GPU with the file already
-resident in HBM: file_id, witness expansion, generator expansion, the initial commitment, ten
-fold rounds with both invariant commitments, every Fiat-Shamir challenge, the verifier's replay
-(its own generator expansion, folds and challenges) and the final opening check, then the proof
-(statement + records + m*) copied to the host. A step counts only when the verifier accepts.
+Headline (`value`): a served trace of open-loop Poisson arrivals of commit+fold requests, file
+sizes log-uniform 64 KiB..16 MiB (d = 7..11, up to 4,194,304 coefficients), seeded uniform
+bytes, K = 8, files resident in HBM before the clock starts. Each request is the whole job:
+file_id, witness and generator expansion, the initial commitment, every fold round with both
+invariant commitments and every Fiat-Shamir challenge, the verifier's own replay (its own
+generator expansion, folds and challenges) and the final opening check, then the proof bytes
+copied to the host. The served path is the continuous-batching server (dmx_server_*); the plain
+path runs the same trace one request at a time through dmx_prove_verify_device. `value` is MiB
+of request files per second from the first arrival to the last verified result; p50/p99 are
+arrival-to-verified-result latencies. Every served proof is compared with the plain path's.
 
-Prints one JSON line. `value` is MiB of file per second over the timed steps.
+Also reported on the same JSON line:
+  single_file   one 8 MiB file (645,278 coefficients, d = 10), the plain path, graph replays
+  batched_files 256 files log-uniform 4 KiB..4 MiB (d = 6..10), each with its own generator seed,
+                one ragged batch (dmx_prove_verify_many) against a loop of the plain path
+  roofline      the forward-transform kernels (the dominant compute), from HIP events around
+                each launch of a prover run on the 8 MiB file
+  cpu_baseline  the oracle (C restatement of the reference CPU algorithm) on the 8 MiB file
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -23,12 +34,135 @@ FILE_BYTES = 8 << 20
 SEED = 2
 PEAK_LANE_CYCLES = 256 * 64 * 2.4e9  # CUs x lanes x clock: 39.32 T lane-cycles/s
 WORK_WEIGHTS = "butterfly 19, mod-p reduction 42, Z_q mul-add 204, BLAKE3 compression 860 lane-cycles"
+DEFAULT_REQUESTS = 400
+DEFAULT_RATE = 2000.0  # requests per second (about 5.6 GiB/s offered)
 
 
-def workload_bytes():
+def rand_bytes(rng, n):
+    import numpy as np
+
+    return rng.integers(0, 256, size=int(n), dtype=np.uint8).tobytes()
+
+
+def single_file():
     import numpy as np
 
     return np.random.default_rng(SEED).integers(0, 256, size=FILE_BYTES, dtype=np.uint8).tobytes()
+
+
+def served_trace(requests, rate, seed=5):
+    """(arrival_seconds, file_bytes) with Poisson arrivals and log-uniform 64 KiB..16 MiB sizes"""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=requests)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    sizes = np.floor(65536 * 256.0 ** rng.random(requests)).astype(np.int64)
+    return [(float(a), rand_bytes(rng, n)) for a, n in zip(arrivals, sizes)]
+
+
+def batch_files(count=256, seed=4):
+    """log-uniform 4 KiB..4 MiB, per-file generator seeds derived from the index"""
+    import numpy as np
+
+    import damascus000_amd as dm
+
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(4096 * 1024.0 ** rng.random(count)).astype(np.int64)
+    files = [rand_bytes(rng, n) for n in sizes]
+    params = [dm.SystemParams(dm.derive_subseed(bytes([7] * 32), b"file-%d" % i), bytes([11] * 32))
+              for i in range(count)]
+    return files, params
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(q * len(xs)))]
+
+
+def run_served(dm, ctx, dev, arrivals):
+    srv = dm.Server(ctx, max_batch=64, max_inflight=4)
+    n = len(dev)
+    tickets = [None] * n
+    t_sub = [0.0] * n
+    t0 = time.perf_counter()
+
+    def submitter():
+        for i, (d, a) in enumerate(zip(dev, arrivals)):
+            now = time.perf_counter() - t0
+            if now < a:
+                time.sleep(a - now)
+            t_sub[i] = time.perf_counter() - t0
+            tickets[i] = srv.submit(d)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    results, lat = [None] * n, [0.0] * n
+    for i in range(n):
+        while tickets[i] is None:
+            time.sleep(0.0002)
+        r, server_lat = srv.wait(tickets[i])
+        results[i] = r
+        lat[i] = (t_sub[i] - arrivals[i]) + server_lat
+    th.join()
+    srv.close()
+    makespan = max(a + l for a, l in zip(arrivals, lat)) - arrivals[0]
+    return results, lat, makespan
+
+
+def run_plain(dm, dev, arrivals):
+    n = len(dev)
+    results, lat = [None] * n, [0.0] * n
+    t0 = time.perf_counter()
+    for i, ((p, ln), a) in enumerate(zip(dev, arrivals)):
+        now = time.perf_counter() - t0
+        if now < a:
+            time.sleep(a - now)
+        results[i] = dm.prove_verify_device(p, ln)
+        lat[i] = time.perf_counter() - t0 - a
+    makespan = max(a + l for a, l in zip(arrivals, lat)) - arrivals[0]
+    return results, lat, makespan
+
+
+def mixed_served(dm, ctx, requests, rate):
+    trace = served_trace(requests, rate)
+    arrivals = [a for a, _ in trace]
+    bufs = [ctx.upload(f) for _, f in trace]
+    dev = [(b.ptr, len(f)) for b, (_, f) in zip(bufs, trace)]
+    total = sum(len(f) for _, f in trace) / (1 << 20)
+    run_served(dm, ctx, dev, arrivals)  # warm: workspaces and captured graphs per (depth, batch)
+    served, s_lat, s_span = run_served(dm, ctx, dev, arrivals)
+    run_plain(dm, dev[: min(len(dev), 32)], arrivals[: min(len(dev), 32)])  # warm
+    plain, p_lat, p_span = run_plain(dm, dev, arrivals)
+    if any(not isinstance(r, tuple) for r in served):
+        raise SystemExit("a served request failed")
+    if served != plain:
+        raise SystemExit("served and plain proofs differ")
+    return {
+        "mib": total, "served_mib_s": total / s_span, "plain_mib_s": total / p_span,
+        "served_p50_ms": pct(s_lat, 0.5) * 1e3, "served_p99_ms": pct(s_lat, 0.99) * 1e3,
+        "plain_p50_ms": pct(p_lat, 0.5) * 1e3, "plain_p99_ms": pct(p_lat, 0.99) * 1e3,
+        "offered_mib_s": total / max(arrivals[-1] - arrivals[0], 1e-9),
+    }
+
+
+def batched_files(dm, ctx):
+    files, params = batch_files()
+    bufs = [ctx.upload(f) for f in files]
+    dev = [(b.ptr, len(f)) for b, f in zip(bufs, files)]
+    total = sum(len(f) for f in files) / (1 << 20)
+    dm.prove_verify_many(dev, params)  # warm
+    t0 = time.perf_counter()
+    batched = dm.prove_verify_many(dev, params)
+    tb = time.perf_counter() - t0
+    plain = [dm.prove_verify_device(p, n, pr) for (p, n), pr in zip(dev[:16], params[:16])]  # warm
+    t0 = time.perf_counter()
+    plain = [dm.prove_verify_device(p, n, pr) for (p, n), pr in zip(dev, params)]
+    tp = time.perf_counter() - t0
+    if batched != plain:
+        raise SystemExit("batched and plain proofs differ")
+    return {"files": len(files), "mib": round(total, 2), "batched_mib_s": round(total / tb, 1),
+            "plain_mib_s": round(total / tp, 1), "speedup": round(tp / tb, 2), "bit_exact_vs_plain": True}
 
 
 def cpu_baseline(data, gpu_proof, threads):
@@ -42,12 +176,12 @@ def cpu_baseline(data, gpu_proof, threads):
     dt = time.perf_counter() - t0
     same = rc == 0 and (st, recs, op) == gpu_proof
     return {"value": round(FILE_BYTES / (1 << 20) / dt, 4), "unit": "MiB/s", "cores": threads, "kind": "port",
-            "sample": f"the full job (prove + verify) on the same 8 MiB file, {dt:.1f} s",
+            "sample": f"the full job (prove + verify) on the 8 MiB single file, {dt:.1f} s",
             "bit_exact_with_gpu": bool(same)}
 
 
 def load_traffic():
-    path = os.path.join(ROOT, "profiles", "ntt_fwd_rows_c2_traffic.json")
+    path = os.path.join(ROOT, "profiles", "ntt_fwd_c2_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -57,8 +191,11 @@ def load_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=10, help="timed graph replays of the 8 MiB single file")
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--requests", type=int, default=DEFAULT_REQUESTS, help="served trace length")
+    ap.add_argument("--rate", type=float, default=DEFAULT_RATE, help="served trace arrival rate (requests/s)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the served trace and the 256-file batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run a device copy of N MiB and exit")
@@ -84,55 +221,71 @@ def main():
         print(json.dumps({"copy_bytes": n, "launches": 3}))
         return
 
-    data = workload_bytes()
+    # ---- single 8 MiB file, plain path (graph replays) ----
+    data = single_file()
     buf = ctx.upload(data)
-    params = dm.SystemParams()  # seed_generators [7;32], epoch_seed [11;32] (config.rs:19-20)
     proof = None
     for _ in range(args.warmup):
-        proof = dm.prove_verify_device(buf.ptr, FILE_BYTES, params)
+        proof = dm.prove_verify_device(buf.ptr, FILE_BYTES)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        out = dm.prove_verify_device(buf.ptr, FILE_BYTES, params)
+        out = dm.prove_verify_device(buf.ptr, FILE_BYTES)
     t1 = time.perf_counter()
     if proof is not None and out != proof:
         raise SystemExit("proof changed between runs")
-    # kernel roofline: the same job run eagerly with HIP events around every forward-transform
-    # launch on its own stream (graph replay carries no per-kernel events)
+    single_ms = (t1 - t0) * 1e3 / args.steps
+    single = {"mib_s": round(FILE_BYTES / (1 << 20) / (single_ms * 1e-3), 1), "ms_per_job": round(single_ms, 3),
+              "file_bytes": FILE_BYTES, "depth": 10, "steps": args.steps}
+
+    # ---- roofline of the forward transforms: a prover run alone, HIP events per launch ----
     ctx.kernel_timing(True)
     ctx.kernel_timing()  # clear
-    timing_steps = max(1, min(args.steps, 3))
-    for _ in range(timing_steps):
-        if dm.prove_verify_device(buf.ptr, FILE_BYTES, params) != out:
-            raise SystemExit("eager and graph runs disagree")
+    timing_runs = 3
+    for _ in range(timing_runs):
+        if dm.prove(data) != out:
+            raise SystemExit("prover API and fused job disagree")
     k_ms, k_launches, k_work = ctx.kernel_timing()
     ctx.kernel_timing(False)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = FILE_BYTES / (1 << 20) * args.steps / (t1 - t0)
-    per_launch_ms = k_ms / max(k_launches, 1)
-    achieved = (k_work / max(k_launches, 1)) / (per_launch_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
+    achieved = (k_work / max(k_launches, 1)) / ((k_ms / max(k_launches, 1)) * 1e-3) / 1e12 if k_ms > 0 else 0.0
+
     line = {
         "metric": "bit-exact commit+fold MiB/s on mixed d=6–11 load; p50/p99 latency vs plain GPU",
-        "value": round(value, 3),
+        "value": None,
         "unit": "MiB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32 (mod-p residues), u128 (Z_q)",
         "data": "synthetic: seeded uniform random bytes",
-        "config": {"workload": "single 8 MiB file, plain GPU path (prove + verifier replay), d=10, 1024x1024 "
-                               "witness, K=8", "file_bytes": FILE_BYTES, "depth": 10, "module_rank": 8,
-                   "parallelism": "one job per GPU"},
-        "roofline": {"bound": "valu", "kernel": "k_ntt_fwd_rows", "achieved": round(achieved, 3),
-                     "peak": round(PEAK_LANE_CYCLES / 1e12, 3), "unit": "T lane-cycles/s",
-                     "frac": round(achieved / (PEAK_LANE_CYCLES / 1e12), 4), "traffic": load_traffic(),
-                     "launches_per_step": k_launches // timing_steps,
-                     "kernel_ms_per_step": round(k_ms / timing_steps, 3), "work_weights": WORK_WEIGHTS,
-                     "timing": f"HIP events around each launch over {timing_steps} eager steps after the timed region"},
+        "config": {"workload": f"served trace: {args.requests} commit+fold requests (prove + verifier replay), "
+                               f"Poisson arrivals at {args.rate:g}/s, sizes log-uniform 64 KiB-16 MiB (d=7..11), "
+                               "K=8, files resident in HBM", "requests": args.requests, "rate_per_s": args.rate,
+                   "module_rank": 8, "parallelism": "one GPU; jobs of one depth share launches"},
+        "single_file": single,
+        "roofline": {"bound": "valu", "kernel": "k_ntt_fwd_reg / k_ntt_fwd_rows (forward transforms)",
+                     "achieved": round(achieved, 3), "peak": round(PEAK_LANE_CYCLES / 1e12, 3),
+                     "unit": "T lane-cycles/s", "frac": round(achieved / (PEAK_LANE_CYCLES / 1e12), 4),
+                     "traffic": load_traffic(), "launches_per_job": k_launches // timing_runs,
+                     "kernel_ms_per_job": round(k_ms / timing_runs, 3), "work_weights": WORK_WEIGHTS,
+                     "timing": f"HIP events around each forward-transform launch of {timing_runs} prover runs on "
+                               "the 8 MiB file, after the timed region"},
     }
+    if not args.no_mixed:
+        mx = mixed_served(dm, ctx, args.requests, args.rate)
+        line["value"] = round(mx["served_mib_s"], 3)
+        line["ms_per_step"] = round(mx["mib"] / mx["served_mib_s"] * 1e3 / args.requests, 4)
+        line["served"] = {k: round(v, 2) for k, v in mx.items()}
+        line["served"]["speedup_vs_plain"] = round(mx["served_mib_s"] / mx["plain_mib_s"], 2)
+        line["served"]["bit_exact_vs_plain"] = True
+        line["batched_files"] = batched_files(dm, ctx)
+    else:
+        line["value"] = single["mib_s"]
+        line["ms_per_step"] = single["ms_per_job"]
+        line["config"]["workload"] = "single 8 MiB file, plain path (mixed load skipped)"
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(data, out, threads)

@@ -49,6 +49,7 @@ struct RowSrc {
     const zq* scal;         // ROW_*FOLD scalar (device)
     int mode;
     const uint32_t* gseed;  // ROW_GEN key words (device, 8 words)
+    int a_shared;           // `a` is one buffer read by every job of a batch (not offset per job)
 };
 
 // A batch of jobs of one shape. Job b's device buffers sit at byte offset b * stride from job
@@ -79,7 +80,7 @@ void launch_ntt_inv(uint32_t* data, size_t slices, int log_m, const NttTables& t
 // sums a second kernel adds.
 void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
                       uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, uint32_t* part, hipStream_t s,
-                      const Jobs& jb = Jobs{});
+                      const Jobs& jb = Jobs{}, bool g_shared = false);
 // vector-stage invariant at degree m plus the poly-stage cross terms at degree m/2, from the
 // NTT-domain even/odd split: out_inv [k][8][m], out_L/out_R [k][8][m/2]
 void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,

@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(RowSrc src, int lo
                                                                NttTables tw, size_t js) {
     extern __shared__ uint32_t sh[];
     const size_t jo = job_off(js);
-    src.a = jp(src.a, jo);
+    if (!src.a_shared) src.a = jp(src.a, jo);
     src.dst = jp(src.dst, jo);
     src.scal = jp(src.scal, jo);
     src.gseed = jp(src.gseed, jo);
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(RowSrc src, uint32_
     using G = RegNtt<E, TP>;
     extern __shared__ uint32_t sh[];
     const size_t jo = job_off(js);
-    src.a = jp(src.a, jo);
+    if (!src.a_shared) src.a = jp(src.a, jo);
     src.dst = jp(src.dst, jo);
     src.scal = jp(src.scal, jo);
     src.gseed = jp(src.gseed, jo);
@@ -358,13 +358,13 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_pairs(const uint32_t* __res
                                                            uint32_t rows, int log_m,
                                                            uint32_t* __restrict__ out_inv,
                                                            uint32_t* __restrict__ out_L,
-                                                           uint32_t* __restrict__ out_R, size_t js) {
+                                                           uint32_t* __restrict__ out_R, size_t js, size_t gjs) {
     const size_t m = (size_t)1 << log_m;
     const size_t gid = (size_t)blockIdx.x * MAC_THREADS + threadIdx.x;
     if (gid >= (size_t)K * NPRIMES * m) return;
     const size_t jo = job_off(js);
     What = jp(What, jo);
-    Ghat = jp(Ghat, jo);
+    Ghat = jp(Ghat, (size_t)blockIdx.y * gjs);
     out_inv = jp(out_inv, jo);
     out_L = jp(out_L, jo);
     out_R = jp(out_R, jo);
@@ -416,14 +416,14 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_split(const uint32_t* __res
                                                            uint32_t* __restrict__ out_inv,
                                                            uint32_t* __restrict__ out_L,
                                                            uint32_t* __restrict__ out_R,
-                                                           NttTables tw, size_t js) {
+                                                           NttTables tw, size_t js, size_t gjs) {
     const size_t m = (size_t)1 << log_m;
     const size_t h = m >> 1;
     const size_t gid = (size_t)blockIdx.x * MAC_THREADS + threadIdx.x;
     if (gid >= (size_t)K * NPRIMES * h) return;
     const size_t jo = job_off(js);
     What = jp(What, jo);
-    Ghat = jp(Ghat, jo);
+    Ghat = jp(Ghat, (size_t)blockIdx.y * gjs);
     out_inv = jp(out_inv, jo);
     out_L = jp(out_L, jo);
     out_R = jp(out_R, jo);
@@ -484,14 +484,14 @@ __device__ __forceinline__ uint4 ld4(const uint32_t* p) { return *reinterpret_ca
 __global__ void __launch_bounds__(MAC_THREADS) k_mac_pairs4(const uint32_t* __restrict__ What,
                                                             const uint32_t* __restrict__ Ghat, uint32_t rows,
                                                             int log_m, uint32_t nseg, uint32_t seg_rows, int cross,
-                                                            uint32_t* __restrict__ part, size_t js) {
+                                                            uint32_t* __restrict__ part, size_t js, size_t gjs) {
     const size_t m = (size_t)1 << log_m;
     const size_t per_seg = (size_t)K * NPRIMES * m / 4;
     const size_t gid = (size_t)blockIdx.x * MAC_THREADS + threadIdx.x;
     if (gid >= (size_t)nseg * per_seg) return;
     const size_t jo = job_off(js);
     What = jp(What, jo);
-    Ghat = jp(Ghat, jo);
+    Ghat = jp(Ghat, (size_t)blockIdx.y * gjs);
     part = jp(part, jo);
     const uint32_t seg = (uint32_t)(gid / per_seg);
     const size_t r = gid - (size_t)seg * per_seg;
@@ -572,8 +572,9 @@ __global__ void k_mac_pairs_reduce(const uint32_t* __restrict__ part, uint32_t n
 }
 
 void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
-                      uint32_t* out_L, uint32_t* out_R, uint32_t* part, hipStream_t s, const Jobs& jb) {
+                      uint32_t* out_L, uint32_t* out_R, uint32_t* part, hipStream_t s, const Jobs& jb, bool g_shared) {
     if (jb.n == 0) return;
+    const size_t gjs = g_shared ? 0 : jb.stride;
     const bool cross = out_L != nullptr && rows > 1;
     const size_t half = cross ? rows / 2 : rows;
     const uint32_t nseg = log_m >= 2 ? mac_segments(half, log_m) : 1;
@@ -583,13 +584,13 @@ void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, i
         const size_t n = (size_t)K * NPRIMES << log_m;
         hipLaunchKernelGGL(k_mac_pairs, dim3((unsigned)((n + MAC_THREADS - 1) / MAC_THREADS), jb.n),
                            dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, out_inv, out_L, out_R,
-                           jb.stride);
+                           jb.stride, gjs);
         return;
     }
     const size_t threads = (size_t)nseg * K * NPRIMES * m / 4;
     hipLaunchKernelGGL(k_mac_pairs4, dim3((unsigned)((threads + MAC_THREADS - 1) / MAC_THREADS), jb.n),
                        dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, nseg, seg_rows, (int)cross, part,
-                       jb.stride);
+                       jb.stride, gjs);
     const int nslots = cross ? 3 : 1;
     const size_t n = (size_t)K * NPRIMES * m * nslots;
     hipLaunchKernelGGL(k_mac_pairs_reduce, dim3((unsigned)((n + 255) / 256), jb.n), dim3(256), 0, s, part, nseg,
@@ -602,14 +603,14 @@ void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, i
 __global__ void __launch_bounds__(MAC_THREADS) k_mac_split4(const uint32_t* __restrict__ What,
                                                             const uint32_t* __restrict__ Ghat, uint32_t rows,
                                                             int log_m, uint32_t nseg, uint32_t seg_rows,
-                                                            uint32_t* __restrict__ part, size_t js) {
+                                                            uint32_t* __restrict__ part, size_t js, size_t gjs) {
     const size_t m = (size_t)1 << log_m, h = m >> 1;
     const size_t per_seg = (size_t)K * NPRIMES * m / 4;
     const size_t gid = (size_t)blockIdx.x * MAC_THREADS + threadIdx.x;
     if (gid >= (size_t)nseg * per_seg) return;
     const size_t jo = job_off(js);
     What = jp(What, jo);
-    Ghat = jp(Ghat, jo);
+    Ghat = jp(Ghat, (size_t)blockIdx.y * gjs);
     part = jp(part, jo);
     const uint32_t seg = (uint32_t)(gid / per_seg);
     const size_t r = gid - (size_t)seg * per_seg;
@@ -688,11 +689,12 @@ void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, i
                       uint32_t* out_L, uint32_t* out_R, uint32_t* part, const NttTables& tw, hipStream_t s,
                       const Jobs& jb) {
     if (jb.n == 0) return;
+    const size_t gjs = jb.stride;
     if (log_m < 2) {  // degree 2: the scalar kernel
         const size_t n = (size_t)K * NPRIMES << (log_m - 1);
         hipLaunchKernelGGL(k_mac_split, dim3((unsigned)((n + MAC_THREADS - 1) / MAC_THREADS), jb.n),
                            dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, out_inv, out_L, out_R, tw,
-                           jb.stride);
+                           jb.stride, gjs);
         return;
     }
     const uint32_t nseg = mac_segments(rows * 2, log_m);
@@ -700,7 +702,8 @@ void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, i
     const size_t m = (size_t)1 << log_m;
     const size_t threads = (size_t)nseg * K * NPRIMES * m / 4;
     hipLaunchKernelGGL(k_mac_split4, dim3((unsigned)((threads + MAC_THREADS - 1) / MAC_THREADS), jb.n),
-                       dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, nseg, seg_rows, part, jb.stride);
+                       dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, nseg, seg_rows, part, jb.stride,
+                       gjs);
     const size_t n = (size_t)K * NPRIMES * (m / 2);
     hipLaunchKernelGGL(k_mac_split_reduce, dim3((unsigned)((n + 255) / 256), jb.n), dim3(256), 0, s, part, nseg,
                        log_m, out_inv, out_L, out_R, tw, jb.stride);
@@ -779,7 +782,7 @@ __global__ void k_fold_rows_zq(RowSrc src, size_t total, int log_m, size_t js) {
     const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= total) return;
     const size_t jo = job_off(js);
-    src.a = jp(src.a, jo);
+    if (!src.a_shared) src.a = jp(src.a, jo);
     src.dst = jp(src.dst, jo);
     src.scal = jp(src.scal, jo);
     src.gseed = jp(src.gseed, jo);

@@ -668,6 +668,10 @@ struct dmx_prover {
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     const uint8_t* d_file = nullptr;
     std::vector<JobMeta> stage;  // host copy of the job headers
+    // Resident generator matrix shared by every job (one seed): Z_q rows and their transforms.
+    // When set, no job expands or transforms its own generators.
+    const zq* G0_src = nullptr;
+    const uint32_t* Ghat0_src = nullptr;
 
     ~dmx_prover() {
         for (auto& e : ev)
@@ -743,18 +747,21 @@ static int prover_init_commit(dmx_prover* p) {
     const size_t N = p->N;
     const int d = p->d;
     const NttTables& tw = p->ctx->tw;
-    RowSrc gen{};
-    gen.mode = ROW_GEN;
-    gen.dst = p->G_a.as<zq>();
-    gen.gseed = p->gseed();
-    fwd_rows(p->ctx, gen, (size_t)K * N, d, p->Ghat_a.as<uint32_t>(), s, jb);
+    if (!p->Ghat0_src) {
+        RowSrc gen{};
+        gen.mode = ROW_GEN;
+        gen.dst = p->G_a.as<zq>();
+        gen.gseed = p->gseed();
+        fwd_rows(p->ctx, gen, (size_t)K * N, d, p->Ghat_a.as<uint32_t>(), s, jb);
+    }
     RowSrc pl{};
     pl.mode = ROW_PLAIN;
     pl.a = p->W_a.as<zq>();
     fwd_rows(p->ctx, pl, N, d, p->What_a.as<uint32_t>(), s, jb);
     const size_t slot = (size_t)K * NPRIMES * N;
-    launch_mac_pairs(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), N, d, p->res_at(0), p->res_at(slot),
-                     p->res_at(2 * slot), p->part.as<uint32_t>(), s, jb);
+    const bool shared = p->Ghat0_src != nullptr;
+    launch_mac_pairs(p->What_a.as<uint32_t>(), shared ? p->Ghat0_src : p->Ghat_a.as<uint32_t>(), N, d, p->res_at(0),
+                     p->res_at(slot), p->res_at(2 * slot), p->part.as<uint32_t>(), s, jb, shared);
     launch_ntt_inv(p->res_at(0), 3 * K, d, tw, s, jb);
     launch_crt_lift(p->res_at(0), p->C.as<zq>(), K, d, s, jb);
     launch_crt_lift(p->res_at(slot), p->rec_lvec(0), 2 * K, d, s, jb);  // L_vec, R_vec adjacent
@@ -789,7 +796,8 @@ static int prover_round(dmx_prover* p, size_t j) {
     fwd_rows(p->ctx, vw, h, lm, p->What_b.as<uint32_t>(), s, jb);
     RowSrc vg{};
     vg.mode = ROW_VFOLD;
-    vg.a = p->G_a.as<zq>();
+    vg.a = (j == 0 && p->G0_src) ? p->G0_src : p->G_a.as<zq>();
+    vg.a_shared = j == 0 && p->G0_src;
     vg.b_off = h * K;
     vg.scal = p->sc(4 * j + 1);  // g_L + x^-1 * g_R
     vg.dst = p->G_b.as<zq>();
@@ -1047,6 +1055,7 @@ struct dmx_verifier {
     int gcur = 0;  // which of G0/G1 holds the state
     Transcriber tr;
     std::vector<JobMeta> stage;
+    const zq* G0_src = nullptr;  // resident generator matrix shared by every job (see dmx_prover)
     ~dmx_verifier() {
         if (s) (void)hipStreamDestroy(s);
     }
@@ -1074,7 +1083,8 @@ static void verifier_bind(dmx_verifier* v) { v->tr.seeds = v->meta.as<uint8_t>()
 // job starts it before the prover has produced the statement. Kernels only.
 static int verifier_enqueue_gen(dmx_verifier* v) {
     launch_fill32(v->flags.as<uint32_t>(), 64, 0, v->s, v->jb);
-    launch_gen_generators(v->gseed(), v->rows * K, log2_exact(v->glen), v->G0.as<zq>(), v->s, v->jb);
+    if (!v->G0_src)
+        launch_gen_generators(v->gseed(), v->rows * K, log2_exact(v->glen), v->G0.as<zq>(), v->s, v->jb);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
@@ -1210,7 +1220,8 @@ static int verifier_round_launch(dmx_verifier* v, const zq* d_l, const zq* d_r, 
         Ct = v->Ct.as<zq>();
         RowSrc f{};
         f.mode = ROW_VFOLD;
-        f.a = v->G();
+        f.a = (v->round == 0 && v->G0_src) ? v->G0_src : v->G();
+        f.a_shared = v->round == 0 && v->G0_src;
         f.b_off = (rows / 2) * K;
         f.scal = v->scal.as<zq>() + 1;  // g_L + x^-1 g_R (verifier.rs:141-151)
         f.dst = v->Gother();
@@ -1381,6 +1392,87 @@ extern "C" int dmx_verify(dmx_ctx* ctx, const dmx_system_params* params, const u
 // addresses live in the per-job headers, so one captured hipGraph per (depth, n) replays for
 // any files of that depth: one launch instead of ~400 per batch.
 // ------------------------------------------------------------------------------------------
+// Resident generator matrices per (context, g_0_seed, depth): the Z_q rows g[i][k] (sis.rs:212-229)
+// and their forward transforms, derived once and read by every later job under that seed. A
+// served stream of requests under one SystemParams then spends no work on generator expansion.
+// DAMASCUS_GEN_CACHE sets the number of matrices kept (default 8; 0 disables).
+struct GenEntry {
+    dmx_ctx* ctx = nullptr;
+    int d = 0;
+    uint8_t seed[32];
+    uint32_t kw[8];
+    DevBuf G, Ghat, key;
+    hipEvent_t ready = nullptr;
+    int refs = 0;  // batch graphs captured against this matrix
+    uint64_t last_use = 0;
+    ~GenEntry() {
+        if (ready) (void)hipEventDestroy(ready);
+    }
+};
+static std::mutex g_gen_mu;
+static std::vector<std::unique_ptr<GenEntry>> g_gen;
+static uint64_t g_gen_clock = 0;
+
+static size_t gen_cache_slots() {
+    const char* e = getenv("DAMASCUS_GEN_CACHE");
+    return e ? (size_t)atoi(e) : 8;
+}
+
+// The matrix for (seed, d), derived on stream s when new; nullptr when the cache is off or full
+// of matrices still in use. The caller waits on ->ready before reading it.
+static int gen_acquire(dmx_ctx* ctx, const uint8_t seed[32], int d, hipStream_t s, GenEntry** out) {
+    *out = nullptr;
+    const size_t slots = gen_cache_slots();
+    if (!slots) return DMX_OK;
+    std::lock_guard<std::mutex> g(g_gen_mu);
+    for (auto& e : g_gen)
+        if (e->ctx == ctx && e->d == d && !memcmp(e->seed, seed, 32)) {
+            e->refs++;
+            e->last_use = ++g_gen_clock;
+            *out = e.get();
+            return DMX_OK;
+        }
+    while (g_gen.size() >= slots) {
+        size_t victim = SIZE_MAX;
+        for (size_t i = 0; i < g_gen.size(); ++i)
+            if (!g_gen[i]->refs && (victim == SIZE_MAX || g_gen[i]->last_use < g_gen[victim]->last_use)) victim = i;
+        if (victim == SIZE_MAX) return DMX_OK;  // every matrix is in use: derive per job instead
+        g_gen.erase(g_gen.begin() + victim);
+    }
+    std::unique_ptr<GenEntry> e(new GenEntry);
+    e->ctx = ctx;
+    e->d = d;
+    memcpy(e->seed, seed, 32);
+    gseed_words(seed, e->kw);
+    const size_t N = (size_t)1 << d;
+    int rc = e->G.alloc(K * N * N * 16);
+    if (!rc) rc = e->Ghat.alloc(K * N * N * NPRIMES * 4);
+    if (!rc) rc = e->key.alloc(32);
+    if (rc) {
+        (void)hipGetLastError();
+        return DMX_OK;  // no room: derive per job
+    }
+    HIP_TRY(hipEventCreateWithFlags(&e->ready, hipEventDisableTiming));
+    HIP_TRY(hipMemcpyAsync(e->key.p, e->kw, 32, hipMemcpyHostToDevice, s));
+    RowSrc gen{};
+    gen.mode = ROW_GEN;
+    gen.dst = e->G.as<zq>();
+    gen.gseed = e->key.as<uint32_t>();
+    launch_ntt_fwd_rows(gen, K * N, d, e->Ghat.as<uint32_t>(), ctx->tw, s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e->ready, s));
+    e->refs = 1;
+    e->last_use = ++g_gen_clock;
+    *out = e.get();
+    g_gen.push_back(std::move(e));
+    return DMX_OK;
+}
+
+static void gen_release(GenEntry* e) {
+    std::lock_guard<std::mutex> g(g_gen_mu);
+    e->refs--;
+}
+
 struct BatchJob {
     dmx_ctx* ctx = nullptr;
     int d = 0;
@@ -1390,11 +1482,23 @@ struct BatchJob {
     std::unique_ptr<dmx_verifier> v;
     std::vector<hipEvent_t> evs;
     hipEvent_t done = nullptr;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;  // jobs expand their own generators
     hipGraphExec_t exec = nullptr;
+    hipGraph_t graph_r = nullptr;  // jobs read the resident matrix `gen`
+    hipGraphExec_t exec_r = nullptr;
+    GenEntry* gen = nullptr;
     bool busy = false;
     uint64_t last_use = 0;
+    void drop_resident() {
+        if (exec_r) (void)hipGraphExecDestroy(exec_r);
+        if (graph_r) (void)hipGraphDestroy(graph_r);
+        exec_r = nullptr;
+        graph_r = nullptr;
+        if (gen) gen_release(gen);
+        gen = nullptr;
+    }
     ~BatchJob() {
+        drop_resident();
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (auto e : evs) (void)hipEventDestroy(e);
@@ -1487,13 +1591,34 @@ struct JobIn {
     const dmx_system_params* params;
 };
 
-static int batch_launch(BatchJob* J, const JobIn* jobs, bool host_files) {
+// With `resident`, a batch whose jobs share one generator seed reads the resident matrix for
+// that seed instead of expanding generators per job.
+static int batch_launch(BatchJob* J, const JobIn* jobs, bool host_files, bool resident) {
     dmx_prover* p = J->p.get();
     dmx_verifier* v = J->v.get();
+    bool same_seed = true;
     for (uint32_t b = 0; b < J->n; ++b) {
         prover_set_job(p, b, jobs[b].params, jobs[b].len, host_files ? nullptr : jobs[b].file);
         v->stage[b] = p->stage[b];
+        same_seed = same_seed && p->g_seed[b] == p->g_seed[0];
     }
+    GenEntry* E = nullptr;
+    if (resident && same_seed) {
+        if (J->gen && !memcmp(J->gen->seed, p->g_seed[0].data(), 32)) {
+            E = J->gen;
+        } else {
+            int rc = gen_acquire(J->ctx, p->g_seed[0].data(), J->d, p->s, &E);
+            if (rc) return rc;
+            if (E) {
+                J->drop_resident();
+                J->gen = E;
+            }
+        }
+    }
+    if (E) HIP_TRY(hipStreamWaitEvent(p->s, E->ready, 0));
+    p->G0_src = E ? E->G.as<zq>() : nullptr;
+    p->Ghat0_src = E ? E->Ghat.as<uint32_t>() : nullptr;
+    v->G0_src = p->G0_src;
     const size_t st = p->jb.stride;
     HIP_TRY(copy_rows(p->meta.p, st, p->stage.data(), sizeof(JobMeta), sizeof(JobMeta), J->n,
                              hipMemcpyHostToDevice, p->s));
@@ -1513,7 +1638,9 @@ static int batch_launch(BatchJob* J, const JobIn* jobs, bool host_files) {
         int rc = batch_enqueue(J);
         if (rc) return rc;
     } else {
-        if (!J->exec) {
+        hipGraph_t& graph_slot = E ? J->graph_r : J->graph;
+        hipGraphExec_t& exec_slot = E ? J->exec_r : J->exec;
+        if (!exec_slot) {
             p->capturing = true;
             HIP_TRY(hipStreamBeginCapture(p->s, hipStreamCaptureModeThreadLocal));
             int rc = batch_enqueue(J);
@@ -1525,10 +1652,10 @@ static int batch_launch(BatchJob* J, const JobIn* jobs, bool host_files) {
                 return rc;
             }
             if (e != hipSuccess) return fail(DMX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-            J->graph = graph;
-            HIP_TRY(hipGraphInstantiate(&J->exec, J->graph, nullptr, nullptr, 0));
+            graph_slot = graph;
+            HIP_TRY(hipGraphInstantiate(&exec_slot, graph_slot, nullptr, nullptr, 0));
         }
-        HIP_TRY(hipGraphLaunch(J->exec, p->s));
+        HIP_TRY(hipGraphLaunch(exec_slot, p->s));
     }
     HIP_TRY(hipEventRecord(J->done, p->s));
     return DMX_OK;
@@ -1600,15 +1727,24 @@ static int batch_collect(BatchJob* J, const JobOut* outs) {
 }
 
 // Cache of batch workspaces and their graphs, per (context, depth, n). Idle workspaces beyond
-// the cache budget (DAMASCUS_CACHE_GB, default 32) are freed least recently used first.
+// the cache budget (DAMASCUS_CACHE_GB, default half the device memory) are freed least
+// recently used first.
 static std::mutex g_batches_mu;
 static std::vector<std::unique_ptr<BatchJob>> g_batches;
 static uint64_t g_batch_clock = 0;
 
 static size_t cache_budget() {
     const char* e = getenv("DAMASCUS_CACHE_GB");
-    const double gb = e ? atof(e) : 32.0;
-    return (size_t)(gb * (double)(1ull << 30));
+    if (e) return (size_t)(atof(e) * (double)(1ull << 30));
+    static size_t half = [] {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            (void)hipGetLastError();
+            return (size_t)32 << 30;
+        }
+        return total_b / 2;
+    }();
+    return half;
 }
 
 static void batch_trim_locked() {
@@ -1667,10 +1803,18 @@ static void batch_release(BatchJob* J) {
 }
 
 static void batch_forget_context(dmx_ctx* ctx) {
-    std::lock_guard<std::mutex> g(g_batches_mu);
-    for (size_t i = 0; i < g_batches.size();)
-        if (g_batches[i]->ctx == ctx)
-            g_batches.erase(g_batches.begin() + i);
+    {
+        std::lock_guard<std::mutex> g(g_batches_mu);
+        for (size_t i = 0; i < g_batches.size();)
+            if (g_batches[i]->ctx == ctx)
+                g_batches.erase(g_batches.begin() + i);
+            else
+                ++i;
+    }
+    std::lock_guard<std::mutex> g(g_gen_mu);
+    for (size_t i = 0; i < g_gen.size();)
+        if (g_gen[i]->ctx == ctx)
+            g_gen.erase(g_gen.begin() + i);
         else
             ++i;
 }
@@ -1689,7 +1833,7 @@ extern "C" int dmx_prove_verify_device(dmx_ctx* ctx, const void* d_file, size_t 
     const JobIn in{d_file, len, params};
     int status = DMX_OK;
     const JobOut out{st_out, st_cap, st_len, rec_out, rec_cap, rec_len, opening_out, &status};
-    rc = batch_launch(J, &in, false);
+    rc = batch_launch(J, &in, false, false);  // the plain path derives generators per call
     if (!rc) rc = batch_collect(J, &out);
     batch_release(J);
     return rc;
@@ -1749,7 +1893,7 @@ extern "C" int dmx_prove_verify_many(dmx_ctx* ctx, dmx_job* jobs, size_t count, 
                 const dmx_job& jj = jobs[L.idx[k]];
                 in[k] = JobIn{jj.file, jj.len, &jj.params};
             }
-            rc = batch_launch(L.J, in.data(), files_on_host != 0);
+            rc = batch_launch(L.J, in.data(), files_on_host != 0, true);
             live.push_back(std::move(L));
         }
     }
@@ -2116,7 +2260,7 @@ static void server_loop(dmx_server* S) {
             S->hbm_used += f.bytes;
             lk.unlock();
             int rc = batch_acquire(S->ctx, d, (uint32_t)nb, &f.J);
-            if (!rc) rc = batch_launch(f.J, in.data(), S->cfg.files_on_host != 0);
+            if (!rc) rc = batch_launch(f.J, in.data(), S->cfg.files_on_host != 0, true);
             if (rc) {
                 if (f.J) {
                     server_finish(S, f, rc);
