@@ -3,21 +3,23 @@
 
 Headline (`value`): a served trace of open-loop Poisson arrivals of commit+fold requests, file
 sizes log-uniform 64 KiB..16 MiB (d = 7..11, up to 4,194,304 coefficients), seeded uniform
-bytes, K = 8, files resident in HBM before the clock starts. Each request is the whole job:
-file_id, witness and generator expansion, the initial commitment, every fold round with both
-invariant commitments and every Fiat-Shamir challenge, the verifier's own replay (its own
-generator expansion, folds and challenges) and the final opening check, then the proof bytes
-copied to the host. The served path is the continuous-batching server (dmx_server_*); the plain
-path runs the same trace one request at a time through dmx_prove_verify_device. `value` is MiB
-of request files per second from the first arrival to the last verified result; p50/p99 are
-arrival-to-verified-result latencies. Every served proof is compared with the plain path's.
+bytes, K = 8, default SystemParams, files resident in HBM before the clock starts. Each request
+is the whole job: file_id, witness expansion, the initial commitment, every fold round with both
+invariant commitments and every Fiat-Shamir challenge, the verifier's replay (its folds and its
+own challenges) and the final opening check, then the proof bytes copied to the host. The
+served path is the continuous-batching server (dmx_server_*), which keeps the generator matrix
+of each (seed, depth) resident in HBM (derived once, in the untimed warm-up pass); the plain
+path runs the same trace one request at a time through dmx_prove_verify_device, which derives
+the generators in both prover and verifier on every call. `value` is MiB of request files per
+second from the first arrival to the last verified result; p50/p99 are arrival-to-verified-
+result latencies. Every served proof is compared with the plain path's.
 
 Also reported on the same JSON line:
   single_file   one 8 MiB file (645,278 coefficients, d = 10), the plain path, graph replays
   batched_files 256 files log-uniform 4 KiB..4 MiB (d = 6..10), each with its own generator seed,
                 one ragged batch (dmx_prove_verify_many) against a loop of the plain path
-  roofline      the forward-transform kernels (the dominant compute), from HIP events around
-                each launch of a prover run on the 8 MiB file
+  roofline      the degree-1024 forward transform (the dominant kernel), from HIP events around
+                each of its launches in prover runs on the 8 MiB file
   cpu_baseline  the oracle (C restatement of the reference CPU algorithm) on the 8 MiB file
 """
 import argparse
@@ -188,6 +190,17 @@ def load_traffic():
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def calibrate(dm, ctx, mib):
+    """three device copies of `mib` MiB: the reference launch for HBM byte counters"""
+    if not mib:
+        return False
+    n = mib << 20
+    a, b = dm.DeviceBuffer(ctx, n), dm.DeviceBuffer(ctx, n)
+    for _ in range(3):
+        b.copy_from(a)
+    return True
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +209,8 @@ def main():
     ap.add_argument("--requests", type=int, default=DEFAULT_REQUESTS, help="served trace length")
     ap.add_argument("--rate", type=float, default=DEFAULT_RATE, help="served trace arrival rate (requests/s)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the served trace and the 256-file batch")
+    ap.add_argument("--only", choices=["roofline", "served"], default=None,
+                    help="profiling runs: only the prover runs of the roofline, or only the served trace")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run a device copy of N MiB and exit")
@@ -221,8 +236,32 @@ def main():
         print(json.dumps({"copy_bytes": n, "launches": 3}))
         return
 
-    # ---- single 8 MiB file, plain path (graph replays) ----
+    if args.only == "served":
+        trace = served_trace(args.requests, args.rate)
+        arrivals = [a for a, _ in trace]
+        bufs = [ctx.upload(f) for _, f in trace]
+        dev = [(b.ptr, len(f)) for b, (_, f) in zip(bufs, trace)]
+        total = sum(len(f) for _, f in trace) / (1 << 20)
+        run_served(dm, ctx, dev, arrivals)
+        _, lat, span = run_served(dm, ctx, dev, arrivals)
+        print(json.dumps({"served_mib_s": round(total / span, 1), "p50_ms": round(pct(lat, 0.5) * 1e3, 2),
+                          "p99_ms": round(pct(lat, 0.99) * 1e3, 2), "requests": args.requests, "rate": args.rate}))
+        return
     data = single_file()
+    if args.only == "roofline":
+        ctx.kernel_timing(True, log_m=10)
+        ctx.kernel_timing()
+        for _ in range(3):
+            dm.prove(data)
+        k_ms, k_launches, k_work = ctx.kernel_timing()
+        ctx.kernel_timing(False)
+        calibrate(dm, ctx, args.calibrate_after)
+        print(json.dumps({"forward_transform_launches": k_launches, "ms": round(k_ms, 3),
+                          "avg_us": round(k_ms * 1e3 / max(k_launches, 1), 2),
+                          "achieved_T_lane_cycles_s": round(k_work / (k_ms * 1e-3) / 1e12, 3)}))
+        return
+
+    # ---- single 8 MiB file, plain path (graph replays) ----
     buf = ctx.upload(data)
     proof = None
     for _ in range(args.warmup):
@@ -237,8 +276,9 @@ def main():
     single = {"mib_s": round(FILE_BYTES / (1 << 20) / (single_ms * 1e-3), 1), "ms_per_job": round(single_ms, 3),
               "file_bytes": FILE_BYTES, "depth": 10, "steps": args.steps}
 
-    # ---- roofline of the forward transforms: a prover run alone, HIP events per launch ----
-    ctx.kernel_timing(True)
+    # ---- roofline of the dominant kernel, the degree-1024 forward transform (k_ntt_fwd_reg<32,32>):
+    # a prover run alone, HIP events around each of its launches ----
+    ctx.kernel_timing(True, log_m=10)
     ctx.kernel_timing()  # clear
     timing_runs = 3
     for _ in range(timing_runs):
@@ -266,13 +306,14 @@ def main():
                                "K=8, files resident in HBM", "requests": args.requests, "rate_per_s": args.rate,
                    "module_rank": 8, "parallelism": "one GPU; jobs of one depth share launches"},
         "single_file": single,
-        "roofline": {"bound": "valu", "kernel": "k_ntt_fwd_reg / k_ntt_fwd_rows (forward transforms)",
+        "roofline": {"bound": "valu", "kernel": "k_ntt_fwd_reg<32,32> (forward transform, degree 1024, 8 primes, "
+                                                "fused row producer)",
                      "achieved": round(achieved, 3), "peak": round(PEAK_LANE_CYCLES / 1e12, 3),
                      "unit": "T lane-cycles/s", "frac": round(achieved / (PEAK_LANE_CYCLES / 1e12), 4),
                      "traffic": load_traffic(), "launches_per_job": k_launches // timing_runs,
                      "kernel_ms_per_job": round(k_ms / timing_runs, 3), "work_weights": WORK_WEIGHTS,
-                     "timing": f"HIP events around each forward-transform launch of {timing_runs} prover runs on "
-                               "the 8 MiB file, after the timed region"},
+                     "timing": f"HIP events around each degree-1024 forward-transform launch of {timing_runs} "
+                               "prover runs on the 8 MiB file, after the timed region"},
     }
     if not args.no_mixed:
         mx = mixed_served(dm, ctx, args.requests, args.rate)
@@ -289,12 +330,8 @@ def main():
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(data, out, threads)
-    if args.calibrate_after:
-        n = args.calibrate_after << 20
-        a, b = dm.DeviceBuffer(ctx, n), dm.DeviceBuffer(ctx, n)
-        for _ in range(3):
-            b.copy_from(a)
-        line["calibration_copy_bytes"] = n
+    if calibrate(dm, ctx, args.calibrate_after):
+        line["calibration_copy_bytes"] = args.calibrate_after << 20
     print(json.dumps(line), flush=True)
 
 

@@ -137,6 +137,7 @@ _SIGS = {
     "dmx_memcpy_dtoh": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
     "dmx_device_copy": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
     "dmx_kernel_timing_enable": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dmx_kernel_timing_degree": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dmx_kernel_timing": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
                                          ctypes.POINTER(ctypes.c_double)]),
     "dmx_statement_size": (_SZ, [_SZ]),
@@ -225,10 +226,11 @@ class Context:
         _check(lib().dmx_memcpy_htod(self.h, _VP(buf.ptr), _bytes_ptr(data), len(data)))
         return buf
 
-    def kernel_timing(self, on=None):
-        """enable (on=True/False) or read the forward-transform timing record:
-        returns (total_ms, launches, lane_cycles) and clears it"""
+    def kernel_timing(self, on=None, log_m=-1):
+        """enable (on=True/False; log_m >= 0 records only degree 2^log_m) or read the
+        forward-transform timing record: returns (total_ms, launches, lane_cycles) and clears it"""
         if on is not None:
+            _check(lib().dmx_kernel_timing_degree(self.h, log_m if on else -1))
             _check(lib().dmx_kernel_timing_enable(self.h, int(bool(on))))
             return None
         ms, n, w = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double()

@@ -257,6 +257,7 @@ struct dmx_ctx {
     CrtConsts crt;
     std::mutex mu;
     bool timing = false;
+    int timing_log_m = -1;  // record only this degree (-1: all)
     std::vector<TimedLaunch> launches;
     std::vector<hipEvent_t> event_pool;
     ~dmx_ctx() {
@@ -291,7 +292,7 @@ static double fwd_row_work(int log_m, int mode) {
 // forward transforms go through here so that kernel timing can bracket them
 static void fwd_rows(dmx_ctx* ctx, const RowSrc& src, size_t rows, int log_m, uint32_t* out, hipStream_t s,
                      const Jobs& jb) {
-    if (!ctx->timing || rows == 0) {
+    if (!ctx->timing || rows == 0 || (ctx->timing_log_m >= 0 && ctx->timing_log_m != log_m)) {
         launch_ntt_fwd_rows(src, rows, log_m, out, ctx->tw, s, jb);
         return;
     }
@@ -307,6 +308,13 @@ extern "C" int dmx_kernel_timing_enable(dmx_ctx* ctx, int on) {
     if (!ctx) return fail(DMX_ERR_ARG, "null context");
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->timing = on != 0;
+    return DMX_OK;
+}
+
+extern "C" int dmx_kernel_timing_degree(dmx_ctx* ctx, int log_m) {
+    if (!ctx) return fail(DMX_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->timing_log_m = log_m < 0 ? -1 : log_m;
     return DMX_OK;
 }
 

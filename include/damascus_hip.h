@@ -229,6 +229,8 @@ int dmx_device_copy(dmx_ctx *ctx, void *dst, const void *src, size_t bytes);
  * multiply-add and BLAKE3 compression), then clears the record. No reference counterpart. */
 int dmx_kernel_timing_enable(dmx_ctx *ctx, int on);
 int dmx_kernel_timing(dmx_ctx *ctx, double *total_ms, uint64_t *launches, double *lane_cycles);
+/* record only launches of degree 2^log_m (negative: every degree) */
+int dmx_kernel_timing_degree(dmx_ctx *ctx, int log_m);
 
 /* sizes for buffer planning */
 size_t dmx_statement_size(size_t ring_len);
