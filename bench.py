@@ -18,6 +18,8 @@ Also reported on the same JSON line:
   single_file   one 8 MiB file (645,278 coefficients, d = 10), the plain path, graph replays
   batched_files 256 files log-uniform 4 KiB..4 MiB (d = 6..10), each with its own generator seed,
                 one ragged batch (dmx_prove_verify_many) against a loop of the plain path
+  ragged_products 4,096 negacyclic products, n uniform in 2^6..2^12, device-resident: one
+                persistent launch against per-degree launch sequences
   roofline      the degree-1024 forward transform (the dominant kernel), from HIP events around
                 each of its launches in prover runs on the 8 MiB file
   cpu_baseline  the oracle (C restatement of the reference CPU algorithm) on the 8 MiB file
@@ -167,6 +169,42 @@ def batched_files(dm, ctx):
             "plain_mib_s": round(total / tp, 1), "speedup": round(tp / tb, 2), "bit_exact_vs_plain": True}
 
 
+def ragged_products(dm, ctx, count=4096, reps=5, seed=3):
+    """C3: negacyclic products with n uniform in 2^6..2^12 and coefficients uniform in [0, q),
+    device-resident; one persistent launch against per-degree launch sequences"""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    logs = rng.integers(6, 13, size=count).astype(np.uint32)
+    total = int(sum(1 << int(l) for l in logs))
+
+    def uniform_zq(n):  # 112-bit rejection sampling: values >= q are redrawn
+        lo = rng.integers(0, 1 << 63, size=n, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, size=n, dtype=np.uint64)
+        hi = rng.integers(0, 1 << 48, size=n, dtype=np.uint64)
+        bad = (hi == (1 << 48) - 1) & (lo > np.uint64((1 << 64) - 76))
+        while bad.any():
+            lo[bad] = rng.integers(0, 1 << 63, size=int(bad.sum()), dtype=np.uint64)
+            bad = (hi == (1 << 48) - 1) & (lo > np.uint64((1 << 64) - 76))
+        return np.stack([lo, hi], axis=1)
+
+    A, B = uniform_zq(total), uniform_zq(total)
+    da, db = ctx.upload(A.tobytes()), ctx.upload(B.tobytes())
+    outs, times = [], []
+    for mode in (0, 1):
+        do = dm.DeviceBuffer(ctx, A.nbytes)
+        dm.ring_mul_batch_device(da.ptr, db.ptr, do.ptr, logs, mode)  # warm
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dm.ring_mul_batch_device(da.ptr, db.ptr, do.ptr, logs, mode)
+        times.append((time.perf_counter() - t0) / reps)
+        outs.append(do.read())
+    if outs[0] != outs[1]:
+        raise SystemExit("persistent and per-degree ring products differ")
+    return {"pairs": count, "coefficients": total, "persistent_ms": round(times[0] * 1e3, 3),
+            "per_degree_ms": round(times[1] * 1e3, 3), "speedup": round(times[1] / times[0], 2),
+            "persistent_products_per_s": round(count / times[0], 1), "bit_exact_between_modes": True}
+
+
 def cpu_baseline(data, gpu_proof, threads):
     """the oracle (C restatement of the reference CPU algorithm) on the same job, all threads"""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -180,6 +218,14 @@ def cpu_baseline(data, gpu_proof, threads):
     return {"value": round(FILE_BYTES / (1 << 20) / dt, 4), "unit": "MiB/s", "cores": threads, "kind": "port",
             "sample": f"the full job (prove + verify) on the 8 MiB single file, {dt:.1f} s",
             "bit_exact_with_gpu": bool(same)}
+
+
+def load_valu_issue():
+    path = os.path.join(ROOT, "profiles", "ntt_fwd_c2_valu.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("valu_issue_frac")
 
 
 def load_traffic():
@@ -311,6 +357,10 @@ def main():
                      "achieved": round(achieved, 3), "peak": round(PEAK_LANE_CYCLES / 1e12, 3),
                      "unit": "T lane-cycles/s", "frac": round(achieved / (PEAK_LANE_CYCLES / 1e12), 4),
                      "traffic": load_traffic(), "launches_per_job": k_launches // timing_runs,
+                     "valu_issue_frac_counters": load_valu_issue(),
+                     "note": "the fixed weights price 32-bit multiplies at quarter rate; gfx950 issues them at "
+                             "full rate, so frac overstates VALU use. valu_issue_frac_counters is SQ_INSTS_VALU x 64 "
+                             "/ duration / 39.32T for this kernel (profiles/valu_rates_gfx950.md)",
                      "kernel_ms_per_job": round(k_ms / timing_runs, 3), "work_weights": WORK_WEIGHTS,
                      "timing": f"HIP events around each degree-1024 forward-transform launch of {timing_runs} "
                                "prover runs on the 8 MiB file, after the timed region"},
@@ -323,6 +373,7 @@ def main():
         line["served"]["speedup_vs_plain"] = round(mx["served_mib_s"] / mx["plain_mib_s"], 2)
         line["served"]["bit_exact_vs_plain"] = True
         line["batched_files"] = batched_files(dm, ctx)
+        line["ragged_products"] = ragged_products(dm, ctx)
     else:
         line["value"] = single["mib_s"]
         line["ms_per_step"] = single["ms_per_job"]

@@ -16,7 +16,7 @@ import numpy as np
 __all__ = [
     "Q", "K", "PRIMES", "DamascusError", "SystemParams", "RuntimeConfig", "Context", "backend_ready",
     "device_summary", "ModuleSisCommitter", "DamascusProver", "DamascusVerifier", "ntt_batch", "fold_batch",
-    "ring_mul_batch", "prove", "verify", "prove_verify_device", "prove_verify_many", "Server", "derive_subseed", "lib", "LIB_PATH",
+    "ring_mul_batch", "ring_mul_batch_device", "prove", "verify", "prove_verify_device", "prove_verify_many", "Server", "derive_subseed", "lib", "LIB_PATH",
     "DeviceBuffer",
 ]
 
@@ -102,6 +102,7 @@ _SIGS = {
     "dmx_ntt_batch": (ctypes.c_int, [_VP] * 3 + [_SZ] + [_VP] * 4),
     "dmx_fold_batch": (ctypes.c_int, [_VP] * 6 + [_SZ, ctypes.c_uint64, ctypes.c_uint64]),
     "dmx_ring_mul_batch": (ctypes.c_int, [_VP] * 5 + [_SZ]),
+    "dmx_ring_mul_batch_device": (ctypes.c_int, [_VP] * 5 + [_SZ, ctypes.c_int]),
     "dmx_derive_subseed": (ctypes.c_int, [_VP, _VP, _SZ, _VP]),
     "dmx_derive_generators": (ctypes.c_int, [_VP, _VP, _SZ, _SZ, _VP]),
     "dmx_commit": (ctypes.c_int, [_VP, _VP, _VP, _SZ, _SZ, _VP, _SZP]),
@@ -623,3 +624,11 @@ def ring_mul_batch(a_list, b_list, ctx=None):
         res.append(out[pos: pos + x.shape[0]])
         pos += x.shape[0]
     return res
+
+
+def ring_mul_batch_device(d_a, d_b, d_out, log_n, mode=0, ctx=None):
+    """ragged products on device-resident coefficient blocks (device pointers as ints; log_n a
+    list of degrees' log2). mode 0: one persistent launch; mode 1: per-degree launches."""
+    ctx = ctx or Context.default()
+    logs = np.ascontiguousarray(np.asarray(log_n, dtype=np.uint32))
+    _check(lib().dmx_ring_mul_batch_device(ctx.h, _VP(d_a), _VP(d_b), _VP(d_out), _ptr(logs), len(logs), mode))

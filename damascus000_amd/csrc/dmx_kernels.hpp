@@ -156,6 +156,12 @@ void launch_cyclic_ntt_batch(uint64_t* a, uint64_t* b, size_t n, const uint64_t*
                              const uint64_t* inv_sizes, hipStream_t s);
 // elementwise residue products: a[i] = a[i] * b[i] (mod p of slice)
 void launch_pointwise(uint32_t* a, const uint32_t* b, size_t slices, int log_m, hipStream_t s);
+// one persistent launch over a ragged batch of negacyclic products a[i] * b[i] of degree
+// 2^logn[i] at coefficient offset offs[i]; `order` lists the products largest first; `counter`
+// is one device word (reset by the launcher); `res` holds 8 residues per coefficient
+void launch_ring_mul_persistent(const zq* A, const zq* B, zq* O, const uint64_t* offs, const uint32_t* logn,
+                                const uint32_t* order, uint32_t count, uint32_t* counter, uint32_t* res,
+                                int max_log_n, const NttTables& tw, hipStream_t s);
 // gather rows by index into a dense block (for ragged batching)
 void launch_gather_rows(const zq* src, const uint64_t* offs, const uint32_t* idx, size_t count,
                         int log_m, zq* dst, hipStream_t s);

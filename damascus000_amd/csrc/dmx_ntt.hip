@@ -721,17 +721,9 @@ size_t mac_part_words(int max_log_m) {
 // centred CRT lift (ntt.rs:400-427): Garner mixed-radix digits v_i in CRT_PRIMES order,
 // x = sum v_i * (p_0..p_{i-1}); if x > floor(P/2) the result is (x - P) mod q.
 // ---------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_crt_lift(const uint32_t* __restrict__ res,
-                                                  zq* __restrict__ out, size_t total, int log_m, size_t js) {
-    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= total) return;
-    res = jp(res, job_off(js));
-    out = jp(out, job_off(js));
-    const size_t m = (size_t)1 << log_m;
-    const size_t slice = gid >> log_m, c = gid & (m - 1);
-    uint32_t r[NPRIMES], v[NPRIMES];
-#pragma unroll
-    for (int i = 0; i < NPRIMES; ++i) r[i] = res[(slice * NPRIMES + i) * m + c];
+// residues r[i] mod CRT_PRIMES[i] -> the centred lift in Z_q
+__device__ __forceinline__ zq crt_lift_residues(const uint32_t r[NPRIMES]) {
+    uint32_t v[NPRIMES];
     v[0] = r[0];
 #pragma unroll
     for (int i = 1; i < NPRIMES; ++i) {
@@ -767,7 +759,21 @@ __global__ void __launch_bounds__(256) k_crt_lift(const uint32_t* __restrict__ r
                       (uint32_t)acc[4], (uint32_t)acc[5], 0, 0};
     zq xq = zq_reduce256(x8);
     if (gt > 0) xq = zq_sub(xq, c_crt.P_q);
-    store_zq(out + gid, xq);
+    return xq;
+}
+
+__global__ void __launch_bounds__(256) k_crt_lift(const uint32_t* __restrict__ res,
+                                                  zq* __restrict__ out, size_t total, int log_m, size_t js) {
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= total) return;
+    res = jp(res, job_off(js));
+    out = jp(out, job_off(js));
+    const size_t m = (size_t)1 << log_m;
+    const size_t slice = gid >> log_m, c = gid & (m - 1);
+    uint32_t r[NPRIMES];
+#pragma unroll
+    for (int i = 0; i < NPRIMES; ++i) r[i] = res[(slice * NPRIMES + i) * m + c];
+    store_zq(out + gid, crt_lift_residues(r));
 }
 
 void launch_crt_lift(const uint32_t* res, zq* out, size_t slices, int log_m, hipStream_t s, const Jobs& jb) {
@@ -1095,6 +1101,135 @@ void launch_pointwise(uint32_t* a, const uint32_t* b, size_t slices, int log_m, 
     if (!total) return;
     hipLaunchKernelGGL(k_pointwise, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, b, total,
                        log_m);
+}
+
+// ---------------------------------------------------------------------------------------
+// Persistent ragged ring products: one launch for a batch of negacyclic products of mixed
+// degrees. Workgroups pull products from a queue ordered largest first (a device counter);
+// per product, G primes at a time (G * n <= RING_LDS_WORDS): reduce both operands into LDS,
+// forward transforms, pointwise product, inverse transform, residues to scratch; then one
+// centred CRT lift per output coefficient. Loop bounds and the exit test are uniform over the
+// workgroup.
+// ---------------------------------------------------------------------------------------
+constexpr int RING_THREADS = 256;
+
+__device__ __forceinline__ void lds_fwd_layers(uint32_t* s, int G, int log_n, int p0, const NttTables& tw) {
+    const int n = 1 << log_n, half = n >> 1;
+    for (int len = half; len >= 1; len >>= 1) {
+        const int lg = __builtin_ctz(len);
+        for (int t = threadIdx.x; t < G * half; t += RING_THREADS) {
+            const int q = t >> (log_n - 1), b = t & (half - 1);
+            const int blk = b >> lg;
+            const int j = (blk << (lg + 1)) | (b & (len - 1));
+            const int kz = (half >> lg) + blk;
+            const int pi = p0 + q;
+            const uint32_t p = c_pc[pi].p;
+            uint32_t* sq = s + q * n;
+            const uint32_t u = sq[j];
+            const uint32_t v = mulc(sq[j + len], tw.zeta_m[pi * TW_SIZE + kz], tw.zeta_mp[pi * TW_SIZE + kz], p);
+            sq[j] = addp(u, v, p);
+            sq[j + len] = subp(u, v, p);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void lds_inv_layers(uint32_t* s, int G, int log_n, int p0, const NttTables& tw) {
+    const int n = 1 << log_n, half = n >> 1;
+    for (int len = 1; len < n; len <<= 1) {
+        const int lg = __builtin_ctz(len);
+        for (int t = threadIdx.x; t < G * half; t += RING_THREADS) {
+            const int q = t >> (log_n - 1), b = t & (half - 1);
+            const int blk = b >> lg;
+            const int j = (blk << (lg + 1)) | (b & (len - 1));
+            const int kz = (half >> lg) + blk;
+            const int pi = p0 + q;
+            const uint32_t p = c_pc[pi].p;
+            uint32_t* sq = s + q * n;
+            const uint32_t u = sq[j], v = sq[j + len];
+            sq[j] = addp(u, v, p);
+            sq[j + len] = mulc(subp(u, v, p), tw.izeta_m[pi * TW_SIZE + kz], tw.izeta_mp[pi * TW_SIZE + kz], p);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(RING_THREADS) k_ring_mul_persistent(
+    const zq* __restrict__ A, const zq* __restrict__ B, zq* __restrict__ O, const uint64_t* __restrict__ offs,
+    const uint32_t* __restrict__ logn, const uint32_t* __restrict__ order, uint32_t count, uint32_t* counter,
+    uint32_t* __restrict__ res, NttTables tw, int lds_words) {
+    extern __shared__ uint32_t sh[];
+    __shared__ uint32_t item;
+    uint32_t* sa = sh;
+    uint32_t* sb = sh + lds_words / 2;
+    for (;;) {
+        if (threadIdx.x == 0) item = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t it = item;
+        __syncthreads();  // every thread has read `item` before it is replaced
+        if (it >= count) break;
+        const uint32_t idx = order[it];
+        const int log_n = (int)logn[idx];
+        const int n = 1 << log_n;
+        const size_t off = offs[idx];
+        const zq* a = A + off;
+        const zq* b = B + off;
+        uint32_t* r = res + off * NPRIMES;
+        if (n == 1) {  // degree 1: the product of two constants
+            if (threadIdx.x < NPRIMES) {
+                const PrimeConsts pc = c_pc[threadIdx.x];
+                const uint32_t x = zq_mod_p_centred(load_zq(a), pc), y = zq_mod_p_centred(load_zq(b), pc);
+                const uint32_t t = redc((uint64_t)x * y, pc.p, pc.pinv);
+                r[threadIdx.x] = redc((uint64_t)t * pc.two64, pc.p, pc.pinv);
+            }
+        } else {
+            const int G = min(NPRIMES, (lds_words / 2) >> log_n);
+            for (int p0 = 0; p0 < NPRIMES; p0 += G) {
+                for (int t = threadIdx.x; t < G * n; t += RING_THREADS) {
+                    const int q = t >> log_n, c = t & (n - 1);
+                    const PrimeConsts pc = c_pc[p0 + q];
+                    sa[t] = zq_mod_p_centred(load_zq(a + c), pc);
+                    sb[t] = zq_mod_p_centred(load_zq(b + c), pc);
+                }
+                __syncthreads();
+                lds_fwd_layers(sa, G, log_n, p0, tw);
+                lds_fwd_layers(sb, G, log_n, p0, tw);
+                for (int t = threadIdx.x; t < G * n; t += RING_THREADS) {
+                    const PrimeConsts pc = c_pc[p0 + (t >> log_n)];
+                    const uint32_t u = redc((uint64_t)sa[t] * sb[t], pc.p, pc.pinv);  // a*b*R^-1
+                    sa[t] = redc((uint64_t)u * pc.two64, pc.p, pc.pinv);             // * R
+                }
+                __syncthreads();
+                lds_inv_layers(sa, G, log_n, p0, tw);
+                for (int t = threadIdx.x; t < G * n; t += RING_THREADS) {
+                    const int q = t >> log_n, c = t & (n - 1);
+                    const int pi = p0 + q;
+                    r[pi * n + c] = mulc(sa[t], tw.invn_m[pi * (MAX_LOG_N + 1) + log_n],
+                                         tw.invn_mp[pi * (MAX_LOG_N + 1) + log_n], c_pc[pi].p);
+                }
+                __syncthreads();  // LDS reused by the next prime group
+            }
+        }
+        __threadfence_block();
+        __syncthreads();  // residues of this product visible to the whole workgroup
+        for (int c = threadIdx.x; c < n; c += RING_THREADS) {
+            uint32_t rr[NPRIMES];
+#pragma unroll
+            for (int i = 0; i < NPRIMES; ++i) rr[i] = r[i * n + c];
+            store_zq(O + off + c, crt_lift_residues(rr));
+        }
+    }
+}
+
+void launch_ring_mul_persistent(const zq* A, const zq* B, zq* O, const uint64_t* offs, const uint32_t* logn,
+                                const uint32_t* order, uint32_t count, uint32_t* counter, uint32_t* res,
+                                int max_log_n, const NttTables& tw, hipStream_t s) {
+    if (!count) return;
+    const int lds_words = 2 * std::max(4096, 1 << max_log_n);  // two operands, one prime group
+    const unsigned wgs = std::min<unsigned>(count, 1024);
+    launch_fill32(counter, 1, 0, s);
+    hipLaunchKernelGGL(k_ring_mul_persistent, dim3(wgs), dim3(RING_THREADS), (size_t)lds_words * 4, s, A, B, O,
+                       offs, logn, order, count, counter, res, tw, lds_words);
 }
 
 __global__ void k_gather_rows(const zq* src, const uint64_t* offs, const uint32_t* idx, size_t total,

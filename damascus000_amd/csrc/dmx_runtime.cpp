@@ -260,6 +260,9 @@ struct dmx_ctx {
     int timing_log_m = -1;  // record only this degree (-1: all)
     std::vector<TimedLaunch> launches;
     std::vector<hipEvent_t> event_pool;
+    // ring-product scratch (grow-only), one batch at a time
+    std::mutex ring_mu;
+    DevBuf ring_offs, ring_logn, ring_order, ring_res, ring_counter, ring_ga, ring_gb, ring_ra, ring_rb;
     ~dmx_ctx() {
         for (auto& t : launches) {
             (void)hipEventDestroy(t.a);
@@ -2096,36 +2099,45 @@ extern "C" int dmx_commit_with_generators(dmx_ctx* ctx, const uint64_t* witness,
 }
 
 // ragged negacyclic products: group by degree, transform both operands, multiply, invert, lift
-extern "C" int dmx_ring_mul_batch(dmx_ctx* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out,
-                                  const uint32_t* log_n, size_t count) {
-    if (!ctx || (count && (!a || !b || !out || !log_n))) return fail(DMX_ERR_ARG, "null argument");
-    if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
-    if (!count) return DMX_OK;
+// Ragged negacyclic products on device-resident operands. mode 0: one persistent launch over
+// the whole batch; mode 1: one launch sequence per degree group (gather, transforms, pointwise,
+// inverse, lift, scatter). Scratch buffers are kept in the context and grow on demand.
+static int ring_mul_device(dmx_ctx* ctx, const zq* dA, const zq* dB, zq* dO, const uint32_t* log_n, size_t count,
+                           int mode, hipStream_t s) {
     std::vector<uint64_t> offs(count);
     size_t total = 0;
+    int max_lg = 0;
     for (size_t i = 0; i < count; ++i) {
         if (log_n[i] > (uint32_t)MAX_LOG_N) return fail(DMX_ERR_SHAPE, "degree 2^%u exceeds 2^13", log_n[i]);
         offs[i] = total;
         total += (size_t)1 << log_n[i];
+        max_lg = std::max(max_lg, (int)log_n[i]);
     }
-    HIP_TRY(hipSetDevice(ctx->device));
-    hipStream_t s = 0;
-    DevBuf dA, dB, dO, dOff, dIdx;
-    int rc = 0;
-    rc = rc ? rc : dA.alloc(total * 16);
-    rc = rc ? rc : dB.alloc(total * 16);
-    rc = rc ? rc : dO.alloc(total * 16);
-    rc = rc ? rc : dOff.alloc(count * 8);
-    rc = rc ? rc : dIdx.alloc(count * 4);
-    if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(dA.p, a, total * 16, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(dB.p, b, total * 16, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(dOff.p, offs.data(), count * 8, hipMemcpyHostToDevice, s));
     std::vector<uint32_t> order(count);
     for (size_t i = 0; i < count; ++i) order[i] = (uint32_t)i;
+    std::lock_guard<std::mutex> g(ctx->ring_mu);
+    auto grow = [](DevBuf& b, size_t bytes) { return b.bytes >= bytes ? DMX_OK : b.alloc(bytes); };
+    int rc = grow(ctx->ring_offs, count * 8);
+    rc = rc ? rc : grow(ctx->ring_logn, count * 4);
+    rc = rc ? rc : grow(ctx->ring_order, count * 4);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->ring_offs.p, offs.data(), count * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(ctx->ring_logn.p, log_n, count * 4, hipMemcpyHostToDevice, s));
+    if (mode == 0) {
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return log_n[x] > log_n[y]; });
+        HIP_TRY(hipMemcpyAsync(ctx->ring_order.p, order.data(), count * 4, hipMemcpyHostToDevice, s));
+        rc = grow(ctx->ring_res, total * NPRIMES * 4);
+        rc = rc ? rc : grow(ctx->ring_counter, 256);
+        if (rc) return rc;
+        launch_ring_mul_persistent(dA, dB, dO, ctx->ring_offs.as<uint64_t>(), ctx->ring_logn.as<uint32_t>(),
+                                   ctx->ring_order.as<uint32_t>(), (uint32_t)count, ctx->ring_counter.as<uint32_t>(),
+                                   ctx->ring_res.as<uint32_t>(), max_lg, ctx->tw, s);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(s));  // host vectors above outlive the copies
+        return DMX_OK;
+    }
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return log_n[x] < log_n[y]; });
-    HIP_TRY(hipMemcpyAsync(dIdx.p, order.data(), count * 4, hipMemcpyHostToDevice, s));
-    // largest group's scratch
+    HIP_TRY(hipMemcpyAsync(ctx->ring_order.p, order.data(), count * 4, hipMemcpyHostToDevice, s));
     size_t maxw = 0;
     for (size_t i = 0; i < count;) {
         size_t j = i;
@@ -2133,35 +2145,72 @@ extern "C" int dmx_ring_mul_batch(dmx_ctx* ctx, const uint64_t* a, const uint64_
         maxw = std::max(maxw, (j - i) << log_n[order[i]]);
         i = j;
     }
-    DevBuf ga, gb, ra, rb;
-    rc = rc ? rc : ga.alloc(maxw * 16);
-    rc = rc ? rc : gb.alloc(maxw * 16);
-    rc = rc ? rc : ra.alloc(maxw * NPRIMES * 4);
-    rc = rc ? rc : rb.alloc(maxw * NPRIMES * 4);
+    rc = grow(ctx->ring_ga, maxw * 16);
+    rc = rc ? rc : grow(ctx->ring_gb, maxw * 16);
+    rc = rc ? rc : grow(ctx->ring_ra, maxw * NPRIMES * 4);
+    rc = rc ? rc : grow(ctx->ring_rb, maxw * NPRIMES * 4);
     if (rc) return rc;
     for (size_t i = 0; i < count;) {
         size_t j = i;
         while (j < count && log_n[order[j]] == log_n[order[i]]) ++j;
         const int lg = (int)log_n[order[i]];
         const size_t cnt = j - i;
-        const uint32_t* idx = dIdx.as<uint32_t>() + i;
-        launch_gather_rows(dA.as<zq>(), dOff.as<uint64_t>(), idx, cnt, lg, ga.as<zq>(), s);
-        launch_gather_rows(dB.as<zq>(), dOff.as<uint64_t>(), idx, cnt, lg, gb.as<zq>(), s);
+        const uint32_t* idx = ctx->ring_order.as<uint32_t>() + i;
+        const uint64_t* doffs = ctx->ring_offs.as<uint64_t>();
+        launch_gather_rows(dA, doffs, idx, cnt, lg, ctx->ring_ga.as<zq>(), s);
+        launch_gather_rows(dB, doffs, idx, cnt, lg, ctx->ring_gb.as<zq>(), s);
         RowSrc sa{}, sb{};
         sa.mode = sb.mode = ROW_PLAIN;
-        sa.a = ga.as<zq>();
-        sb.a = gb.as<zq>();
-        launch_ntt_fwd_rows(sa, cnt, lg, ra.as<uint32_t>(), ctx->tw, s);
-        launch_ntt_fwd_rows(sb, cnt, lg, rb.as<uint32_t>(), ctx->tw, s);
-        launch_pointwise(ra.as<uint32_t>(), rb.as<uint32_t>(), cnt, lg, s);
-        launch_ntt_inv(ra.as<uint32_t>(), cnt, lg, ctx->tw, s);
-        launch_crt_lift(ra.as<uint32_t>(), ga.as<zq>(), cnt, lg, s);
-        launch_scatter_rows(ga.as<zq>(), dOff.as<uint64_t>(), idx, cnt, lg, dO.as<zq>(), s);
+        sa.a = ctx->ring_ga.as<zq>();
+        sb.a = ctx->ring_gb.as<zq>();
+        launch_ntt_fwd_rows(sa, cnt, lg, ctx->ring_ra.as<uint32_t>(), ctx->tw, s);
+        launch_ntt_fwd_rows(sb, cnt, lg, ctx->ring_rb.as<uint32_t>(), ctx->tw, s);
+        launch_pointwise(ctx->ring_ra.as<uint32_t>(), ctx->ring_rb.as<uint32_t>(), cnt, lg, s);
+        launch_ntt_inv(ctx->ring_ra.as<uint32_t>(), cnt, lg, ctx->tw, s);
+        launch_crt_lift(ctx->ring_ra.as<uint32_t>(), ctx->ring_ga.as<zq>(), cnt, lg, s);
+        launch_scatter_rows(ctx->ring_ga.as<zq>(), doffs, idx, cnt, lg, dO, s);
         i = j;
     }
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return DMX_OK;
+}
+
+extern "C" int dmx_ring_mul_batch(dmx_ctx* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out,
+                                  const uint32_t* log_n, size_t count) {
+    if (!ctx || (count && (!a || !b || !out || !log_n))) return fail(DMX_ERR_ARG, "null argument");
+    if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
+    if (!count) return DMX_OK;
+    size_t total = 0;
+    for (size_t i = 0; i < count; ++i) {
+        if (log_n[i] > (uint32_t)MAX_LOG_N) return fail(DMX_ERR_SHAPE, "degree 2^%u exceeds 2^13", log_n[i]);
+        total += (size_t)1 << log_n[i];
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = 0;
+    DevBuf dA, dB, dO;
+    int rc = 0;
+    rc = rc ? rc : dA.alloc(total * 16);
+    rc = rc ? rc : dB.alloc(total * 16);
+    rc = rc ? rc : dO.alloc(total * 16);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(dA.p, a, total * 16, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dB.p, b, total * 16, hipMemcpyHostToDevice, s));
+    rc = ring_mul_device(ctx, dA.as<zq>(), dB.as<zq>(), dO.as<zq>(), log_n, count, 0, s);
+    if (rc) return rc;
     HIP_TRY(hipMemcpy(out, dO.p, total * 16, hipMemcpyDeviceToHost));
     return DMX_OK;
+}
+
+extern "C" int dmx_ring_mul_batch_device(dmx_ctx* ctx, const void* d_a, const void* d_b, void* d_out,
+                                         const uint32_t* log_n, size_t count, int mode) {
+    if (!ctx || (count && (!d_a || !d_b || !d_out || !log_n))) return fail(DMX_ERR_ARG, "null argument");
+    if (mode != 0 && mode != 1) return fail(DMX_ERR_ARG, "mode must be 0 (persistent) or 1 (per degree)");
+    if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
+    if (!count) return DMX_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    return ring_mul_device(ctx, static_cast<const zq*>(d_a), static_cast<const zq*>(d_b), static_cast<zq*>(d_out),
+                           log_n, count, mode, 0);
 }
 
 // ------------------------------------------------------------------------------------------

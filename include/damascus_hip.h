@@ -104,6 +104,12 @@ int dmx_fold_batch(const uint64_t *left_lo, const uint64_t *left_hi, const uint6
  * Product i has degree 2^log_n[i]; a, b, out hold the coefficient blocks back to back. */
 int dmx_ring_mul_batch(dmx_ctx *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out,
                        const uint32_t *log_n, size_t count);
+/* the same on device-resident operands (d_a, d_b, d_out: device addresses of the coefficient
+ * blocks; log_n in host memory). mode 0: one persistent launch over the whole ragged batch,
+ * products taken largest first from a device work counter; mode 1: one launch sequence per
+ * degree group. No reference counterpart (the reference multiplies one pair per call). */
+int dmx_ring_mul_batch_device(dmx_ctx *ctx, const void *d_a, const void *d_b, void *d_out,
+                              const uint32_t *log_n, size_t count, int mode);
 
 /* ---- commitment (src/commitment/sis.rs) ---- */
 /* derive_subseed(seed, "g0") (sis.rs:208-210) */

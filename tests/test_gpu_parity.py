@@ -449,3 +449,29 @@ def test_resident_generators_across_seeds_and_batches():
         out = dm.prove_verify_many(files, plist)
         for f, p, o in zip(files, plist, out):
             assert o == dm.prove(f, p)
+
+
+def test_persistent_ragged_products_equal_per_degree_launches_and_oracle():
+    """the C3 shape at reduced count on device-resident operands: one persistent launch and the
+    per-degree launch sequences give the same products, equal to the oracle on every pair"""
+    rng = np.random.default_rng(3030)
+    logs = [int(x) for x in rng.integers(6, 13, size=60)] + [0, 1, 13]
+    a = [O.random_zq(rng, 1 << l) for l in logs]
+    b = [O.random_zq(rng, 1 << l) for l in logs]
+    A = np.ascontiguousarray(np.concatenate(a), dtype=np.uint64)
+    B = np.ascontiguousarray(np.concatenate(b), dtype=np.uint64)
+    ctx = dm.Context.default()
+    da, db = ctx.upload(A.tobytes()), ctx.upload(B.tobytes())
+    outs = []
+    for mode in (0, 1):
+        do = dm.DeviceBuffer(ctx, A.nbytes)
+        say("ring_mul_batch_device mode", mode, len(logs))
+        dm.ring_mul_batch_device(da.ptr, db.ptr, do.ptr, logs, mode)
+        outs.append(np.frombuffer(do.read(), dtype=np.uint64).reshape(-1, 2))
+    assert np.array_equal(outs[0], outs[1])
+    pos = 0
+    for x, y in zip(a, b):
+        n = x.shape[0]
+        if n <= 2048:
+            assert np.array_equal(outs[0][pos: pos + n], O.negacyclic_multiply(x, y)), n
+        pos += n
