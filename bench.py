@@ -321,6 +321,23 @@ def main():
     single_ms = (t1 - t0) * 1e3 / args.steps
     single = {"mib_s": round(FILE_BYTES / (1 << 20) / (single_ms * 1e-3), 1), "ms_per_job": round(single_ms, 3),
               "file_bytes": FILE_BYTES, "depth": 10, "steps": args.steps}
+    # the largest served depth: a 16 MiB file is d = 11 (2048 x 2048 witness)
+    import numpy as np
+
+    big = np.random.default_rng(SEED + 11).integers(0, 256, size=16 << 20, dtype=np.uint8).tobytes()
+    bbuf = ctx.upload(big)
+    dm.prove_verify_device(bbuf.ptr, len(big))
+    t0 = time.perf_counter()
+    for _ in range(3):
+        dm.prove_verify_device(bbuf.ptr, len(big), want_proof=False)
+    d11_ms = (time.perf_counter() - t0) * 1e3 / 3
+    W_PROVER_D11 = 225.25e9  # lane-cycles of the d = 11 prover at the fixed weights
+    single["d11"] = {"file_bytes": len(big), "ms_per_job": round(d11_ms, 3),
+                     "mib_s": round(len(big) / (1 << 20) / (d11_ms * 1e-3), 1),
+                     "prover_work_frac_of_peak": round(W_PROVER_D11 / (d11_ms * 1e-3) / PEAK_LANE_CYCLES, 3),
+                     "note": "prove + verifier replay timed together; the prover's fixed-weight work over the "
+                             "whole job's time is a lower bound on the prover's own fraction"}
+    del bbuf
 
     # ---- roofline of the dominant kernel, the degree-1024 forward transform (k_ntt_fwd_reg<32,32>):
     # a prover run alone, HIP events around each of its launches ----

@@ -103,6 +103,7 @@ _SIGS = {
     "dmx_fold_batch": (ctypes.c_int, [_VP] * 6 + [_SZ, ctypes.c_uint64, ctypes.c_uint64]),
     "dmx_ring_mul_batch": (ctypes.c_int, [_VP] * 5 + [_SZ]),
     "dmx_ring_mul_batch_device": (ctypes.c_int, [_VP] * 5 + [_SZ, ctypes.c_int]),
+    "dmx_ring_mul_batch_cfg": (ctypes.c_int, [_VP] * 5 + [_SZ, _VP]),
     "dmx_derive_subseed": (ctypes.c_int, [_VP, _VP, _SZ, _VP]),
     "dmx_derive_generators": (ctypes.c_int, [_VP, _VP, _SZ, _SZ, _VP]),
     "dmx_commit": (ctypes.c_int, [_VP, _VP, _VP, _SZ, _SZ, _VP, _SZP]),
@@ -611,14 +612,19 @@ def fold_batch(left, right, challenge):
     return rc, [int(a) | (int(b) << 64) for a, b in zip(o_lo, o_hi)]
 
 
-def ring_mul_batch(a_list, b_list, ctx=None):
-    """negacyclic products a_i * b_i of mixed power-of-two degrees; lists of (n_i, 2) words"""
+def ring_mul_batch(a_list, b_list, ctx=None, config=None):
+    """negacyclic products a_i * b_i of mixed power-of-two degrees; lists of (n_i, 2) words.
+    config.ntt_enabled = False takes the schoolbook product (RingElement::mul, poly.rs:134-140)."""
     ctx = ctx or Context.default()
     logs = np.array([int(x.shape[0]).bit_length() - 1 for x in a_list], dtype=np.uint32)
     A = np.ascontiguousarray(np.concatenate(a_list), dtype=np.uint64)
     B = np.ascontiguousarray(np.concatenate(b_list), dtype=np.uint64)
     out = np.zeros_like(A)
-    _check(lib().dmx_ring_mul_batch(ctx.h, _ptr(A), _ptr(B), _ptr(out), _ptr(logs), len(a_list)))
+    if config is None:
+        _check(lib().dmx_ring_mul_batch(ctx.h, _ptr(A), _ptr(B), _ptr(out), _ptr(logs), len(a_list)))
+    else:
+        _check(lib().dmx_ring_mul_batch_cfg(ctx.h, _ptr(A), _ptr(B), _ptr(out), _ptr(logs), len(a_list),
+                                            ctypes.byref(config)))
     res, pos = [], 0
     for x in a_list:
         res.append(out[pos: pos + x.shape[0]])

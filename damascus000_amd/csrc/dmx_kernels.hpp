@@ -162,6 +162,9 @@ void launch_pointwise(uint32_t* a, const uint32_t* b, size_t slices, int log_m, 
 void launch_ring_mul_persistent(const zq* A, const zq* B, zq* O, const uint64_t* offs, const uint32_t* logn,
                                 const uint32_t* order, uint32_t count, uint32_t* counter, uint32_t* res,
                                 int max_log_n, const NttTables& tw, hipStream_t s);
+// schoolbook products (the reference's ntt_enabled = false path), one workgroup per product
+void launch_ring_mul_naive(const zq* A, const zq* B, zq* O, const uint64_t* offs, const uint32_t* logn,
+                           uint32_t count, hipStream_t s);
 // gather rows by index into a dense block (for ragged batching)
 void launch_gather_rows(const zq* src, const uint64_t* offs, const uint32_t* idx, size_t count,
                         int log_m, zq* dst, hipStream_t s);

@@ -1232,6 +1232,45 @@ void launch_ring_mul_persistent(const zq* A, const zq* B, zq* O, const uint64_t*
                        offs, logn, order, count, counter, res, tw, lds_words);
 }
 
+// Schoolbook negacyclic products, the reference's product with ntt_enabled = false
+// (naive_negacyclic, src/algebra/ntt.rs:226-240, chosen in RingElement::mul, src/algebra/poly.rs:
+// 134-140). One workgroup per product; thread k sums a_i * b_(k-i) with the wrapped terms
+// negated, as exact 256-bit sums (at most 8192 products below 2^224), reduced once.
+__device__ __forceinline__ void acc256_add(uint32_t acc[8], const uint32_t t[8]) {
+    uint64_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        c += (uint64_t)acc[i] + t[i];
+        acc[i] = (uint32_t)c;
+        c >>= 32;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ring_mul_naive(const zq* __restrict__ A, const zq* __restrict__ B,
+                                                        zq* __restrict__ O, const uint64_t* __restrict__ offs,
+                                                        const uint32_t* __restrict__ logn) {
+    const uint32_t idx = blockIdx.x;
+    const int n = 1 << logn[idx];
+    const size_t off = offs[idx];
+    const zq* a = A + off;
+    const zq* b = B + off;
+    for (int k = threadIdx.x; k < n; k += 256) {
+        uint32_t pos[8] = {0, 0, 0, 0, 0, 0, 0, 0}, neg[8] = {0, 0, 0, 0, 0, 0, 0, 0}, t[8];
+        for (int i = 0; i < n; ++i) {
+            const int j = k - i;
+            zq_mul_wide(load_zq(a + i), load_zq(b + (j >= 0 ? j : j + n)), t);
+            acc256_add(j >= 0 ? pos : neg, t);
+        }
+        store_zq(O + off + k, zq_sub(zq_reduce256(pos), zq_reduce256(neg)));
+    }
+}
+
+void launch_ring_mul_naive(const zq* A, const zq* B, zq* O, const uint64_t* offs, const uint32_t* logn,
+                           uint32_t count, hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_ring_mul_naive, dim3(count), dim3(256), 0, s, A, B, O, offs, logn);
+}
+
 __global__ void k_gather_rows(const zq* src, const uint64_t* offs, const uint32_t* idx, size_t total,
                               int log_m, zq* dst, int scatter) {
     const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;

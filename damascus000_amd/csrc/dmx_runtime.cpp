@@ -2123,6 +2123,13 @@ static int ring_mul_device(dmx_ctx* ctx, const zq* dA, const zq* dB, zq* dO, con
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(ctx->ring_offs.p, offs.data(), count * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(ctx->ring_logn.p, log_n, count * 4, hipMemcpyHostToDevice, s));
+    if (mode == 2) {  // schoolbook (ntt_enabled = false)
+        launch_ring_mul_naive(dA, dB, dO, ctx->ring_offs.as<uint64_t>(), ctx->ring_logn.as<uint32_t>(),
+                              (uint32_t)count, s);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(s));
+        return DMX_OK;
+    }
     if (mode == 0) {
         std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return log_n[x] > log_n[y]; });
         HIP_TRY(hipMemcpyAsync(ctx->ring_order.p, order.data(), count * 4, hipMemcpyHostToDevice, s));
@@ -2202,10 +2209,39 @@ extern "C" int dmx_ring_mul_batch(dmx_ctx* ctx, const uint64_t* a, const uint64_
     return DMX_OK;
 }
 
+// RingElement::mul(rhs, ntt_enabled, gpu_enabled) (src/algebra/poly.rs:134-140) over a ragged
+// batch: NTT products, or the schoolbook product when cfg->ntt_enabled == 0; gpu_enabled == 0
+// returns DMX_ERR_NO_DEVICE (the caller's CPU path). Bit-identical either way.
+extern "C" int dmx_ring_mul_batch_cfg(dmx_ctx* ctx, const uint64_t* a, const uint64_t* b, uint64_t* out,
+                                      const uint32_t* log_n, size_t count, const dmx_runtime_config* cfg) {
+    if (cfg && !cfg->gpu_enabled) return fail(DMX_ERR_NO_DEVICE, "gpu_enabled = false");
+    if (!cfg || cfg->ntt_enabled) return dmx_ring_mul_batch(ctx, a, b, out, log_n, count);
+    if (!ctx || (count && (!a || !b || !out || !log_n))) return fail(DMX_ERR_ARG, "null argument");
+    if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
+    if (!count) return DMX_OK;
+    size_t total = 0;
+    for (size_t i = 0; i < count; ++i) {
+        if (log_n[i] > (uint32_t)MAX_LOG_N) return fail(DMX_ERR_SHAPE, "degree 2^%u exceeds 2^13", log_n[i]);
+        total += (size_t)1 << log_n[i];
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevBuf dA, dB, dO;
+    int rc = dA.alloc(total * 16);
+    rc = rc ? rc : dB.alloc(total * 16);
+    rc = rc ? rc : dO.alloc(total * 16);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(dA.p, a, total * 16, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dB.p, b, total * 16, hipMemcpyHostToDevice));
+    rc = ring_mul_device(ctx, dA.as<zq>(), dB.as<zq>(), dO.as<zq>(), log_n, count, 2, 0);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(out, dO.p, total * 16, hipMemcpyDeviceToHost));
+    return DMX_OK;
+}
+
 extern "C" int dmx_ring_mul_batch_device(dmx_ctx* ctx, const void* d_a, const void* d_b, void* d_out,
                                          const uint32_t* log_n, size_t count, int mode) {
     if (!ctx || (count && (!d_a || !d_b || !d_out || !log_n))) return fail(DMX_ERR_ARG, "null argument");
-    if (mode != 0 && mode != 1) return fail(DMX_ERR_ARG, "mode must be 0 (persistent) or 1 (per degree)");
+    if (mode < 0 || mode > 2) return fail(DMX_ERR_ARG, "mode must be 0 (persistent), 1 (per degree) or 2 (schoolbook)");
     if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
     if (!count) return DMX_OK;
     HIP_TRY(hipSetDevice(ctx->device));
@@ -2304,7 +2340,10 @@ static void server_loop(dmx_server* S) {
             size_t nb = 1;
             while (nb * 2 <= pos.size()) nb *= 2;
             const size_t per = job_workspace_bytes(d);
-            while (nb > 1 && S->hbm_used + nb * per > S->cfg.hbm_budget_bytes) nb /= 2;
+            // one batch's workspace stays within 1/8 of the budget (at least one job), so a few
+            // large batches cannot pin all memory and force workspace churn
+            while (nb > 1 && (S->hbm_used + nb * per > S->cfg.hbm_budget_bytes || nb * per > S->cfg.hbm_budget_bytes / 8))
+                nb /= 2;
             if (S->hbm_used + nb * per > S->cfg.hbm_budget_bytes && !S->flights.empty()) break;  // wait for memory
             dmx_server::Flight f{nullptr, {}, nb * per};
             for (size_t k = 0; k < nb; ++k) f.tickets.push_back(S->queue[pos[k]]);

@@ -104,10 +104,17 @@ int dmx_fold_batch(const uint64_t *left_lo, const uint64_t *left_hi, const uint6
  * Product i has degree 2^log_n[i]; a, b, out hold the coefficient blocks back to back. */
 int dmx_ring_mul_batch(dmx_ctx *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out,
                        const uint32_t *log_n, size_t count);
+/* RingElement::mul(rhs, ntt_enabled, gpu_enabled) (src/algebra/poly.rs:134-140) over the
+ * same ragged batch: config->ntt_enabled = 0 selects the schoolbook product the reference then
+ * uses (naive_negacyclic, src/algebra/ntt.rs:226-240); gpu_enabled = 0 returns
+ * DMX_ERR_NO_DEVICE. Results are identical either way. */
+int dmx_ring_mul_batch_cfg(dmx_ctx *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out,
+                           const uint32_t *log_n, size_t count, const dmx_runtime_config *config);
 /* the same on device-resident operands (d_a, d_b, d_out: device addresses of the coefficient
  * blocks; log_n in host memory). mode 0: one persistent launch over the whole ragged batch,
  * products taken largest first from a device work counter; mode 1: one launch sequence per
- * degree group. No reference counterpart (the reference multiplies one pair per call). */
+ * degree group; mode 2: the schoolbook product. No reference counterpart (the reference
+ * multiplies one pair per call). */
 int dmx_ring_mul_batch_device(dmx_ctx *ctx, const void *d_a, const void *d_b, void *d_out,
                               const uint32_t *log_n, size_t count, int mode);
 

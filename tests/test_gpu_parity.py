@@ -475,3 +475,23 @@ def test_persistent_ragged_products_equal_per_degree_launches_and_oracle():
         if n <= 2048:
             assert np.array_equal(outs[0][pos: pos + n], O.negacyclic_multiply(x, y)), n
         pos += n
+
+
+def test_ntt_off_schoolbook_products_match_ntt_products_and_oracle():
+    """RuntimeConfig.ntt_enabled = false takes the reference's schoolbook product
+    (naive_negacyclic); results are identical to the NTT path and to the oracle"""
+    rng = np.random.default_rng(4040)
+    logs = [0, 1, 3, 6, 9, 10, 12]
+    a = [O.random_zq(rng, 1 << l) for l in logs]
+    b = [O.random_zq(rng, 1 << l) for l in logs]
+    a.append(O.to_words([O.Q - 1] * 256))
+    b.append(O.to_words([O.Q - 1] * 256))
+    say("schoolbook products", logs)
+    naive = dm.ring_mul_batch(a, b, config=dm.RuntimeConfig(ntt_enabled=False))
+    fast = dm.ring_mul_batch(a, b)
+    for x, y, p, q in zip(a, b, naive, fast):
+        assert np.array_equal(p, q), x.shape[0]
+        if x.shape[0] <= 1024:
+            assert np.array_equal(p, O.negacyclic_multiply(x, y))
+    with pytest.raises(dm.DamascusError):
+        dm.ring_mul_batch(a[:1], b[:1], config=dm.RuntimeConfig(gpu_enabled=False))
