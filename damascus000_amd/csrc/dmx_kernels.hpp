@@ -52,6 +52,13 @@ struct RowSrc {
     int a_shared;           // `a` is one buffer read by every job of a batch (not offset per job)
 };
 
+// one segment of a forward-transform launch: rows from `src`, residues to `out`
+struct FwdSeg {
+    RowSrc src;
+    uint32_t* out;
+    uint32_t rows;
+};
+
 // A batch of jobs of one shape. Job b's device buffers sit at byte offset b * stride from job
 // 0's, so a kernel handles job blockIdx.y by offsetting every per-job pointer it was given
 // (null pointers stay null). Tables and constants are shared and never offset.
@@ -71,6 +78,9 @@ void set_device_constants(const PrimeConsts* pc, const CrtConsts* crt, hipStream
 // forward negacyclic NTT of `rows` rows of degree 2^log_m, all 8 primes; out[(row*8+p)*m + c]
 void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out,
                          const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
+// two row sets of the same degree in one launch (segment 2 may be empty)
+void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, const RowSrc& src2, size_t rows2,
+                          uint32_t* out2, int log_m, const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
 // inverse NTT in place on `slices` slices of [8][m]: data[(slice*8 + p)*m + c], times m^-1
 void launch_ntt_inv(uint32_t* data, size_t slices, int log_m, const NttTables& tw, hipStream_t s,
                     const Jobs& jb = Jobs{});
@@ -86,6 +96,14 @@ void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, i
 void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
                       uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, uint32_t* part,
                       const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
+// the same passes with the partial-sum reduction and the inverse transforms fused (outputs are
+// the inverse-transformed residues, scaled by 1/m); split: out_lr holds L then R
+void launch_mac_pairs_inv(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
+                          uint32_t* out_L, uint32_t* out_R, uint32_t* part, const NttTables& tw, hipStream_t s,
+                          const Jobs& jb = Jobs{}, bool g_shared = false);
+void launch_mac_split_inv(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
+                          uint32_t* out_lr, uint32_t* part, const NttTables& tw, hipStream_t s,
+                          const Jobs& jb = Jobs{});
 // scratch words the MAC passes need at degrees up to 2^max_log_m
 size_t mac_part_words(int max_log_m);
 // Z_q rows only (no transform): dst[row][c] = row_value(src, row, c)
@@ -95,6 +113,20 @@ void launch_final_check(const zq* g, const zq* c, const zq* opening, size_t n, u
 // residues [slice][8][m] -> Z_q [slice][m]
 void launch_crt_lift(const uint32_t* res, zq* out, size_t slices, int log_m, hipStream_t s,
                      const Jobs& jb = Jobs{});
+
+// commitment update operands: c' = c + scal[1]*l + scal[0]*r (poly: even(c) + ...)
+struct ComUpd {
+    const zq* c;
+    const zq* l;
+    const zq* r;
+    const zq* scal;
+    int poly;
+};
+// lift K commitment slices (checking them against the update of `cu`, written to com_out) and
+// lr_slices cross-term slices (to out_lr) in one launch; see k_lift_check
+void launch_lift_check(const uint32_t* res_a, int log_ma, const ComUpd& cu, zq* com_out, zq* com_out2,
+                       uint32_t* flags, uint32_t bit, const uint32_t* res_b, zq* out_lr, size_t lr_slices, int log_mb,
+                       hipStream_t s, const Jobs& jb = Jobs{});
 
 // commitment updates (tiny, K x m)
 // out = c + sinv*l + s*r          (VEC; scalars at scal[0]=s, scal[1]=s^-1)

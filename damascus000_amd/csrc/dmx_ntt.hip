@@ -91,10 +91,12 @@ __device__ __forceinline__ zq row_value(const RowSrc& src, const uint32_t* key, 
 // forward negacyclic NTT (Cooley-Tukey, natural order in, bit-reversed order out)
 // butterfly: (a, b) -> (a + z*b, a - z*b), z = zeta[m/(2*len) + block]
 // ---------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(RowSrc src, int log_m,
-                                                               uint32_t* __restrict__ out,
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(FwdSeg s1, FwdSeg s2, uint32_t blocks1, int log_m,
                                                                NttTables tw, size_t js) {
     extern __shared__ uint32_t sh[];
+    const bool second = blockIdx.x >= blocks1;  // uniform over the workgroup
+    RowSrc src = second ? s2.src : s1.src;
+    uint32_t* out = second ? s2.out : s1.out;
     const size_t jo = job_off(js);
     if (!src.a_shared) src.a = jp(src.a, jo);
     src.dst = jp(src.dst, jo);
@@ -107,7 +109,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(RowSrc src, int lo
         for (int i = 0; i < 8; ++i) key[i] = src.gseed[i];
     }
     const int m = 1 << log_m;
-    const size_t row = blockIdx.x;
+    const size_t row = second ? blockIdx.x - blocks1 : blockIdx.x;
     const int pg = min(NPRIMES, LDS_WORDS >> log_m);  // primes per LDS pass
     for (int p0 = 0; p0 < NPRIMES; p0 += pg) {
         // 1) produce the row and reduce it into the primes of this group
@@ -181,10 +183,14 @@ __device__ __forceinline__ void ct_butterfly(uint32_t& x, uint32_t& y, uint32_t 
 }
 
 template <int E, int TP>
-__global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(RowSrc src, uint32_t* __restrict__ out, NttTables tw,
-                                                             size_t js, uint32_t rows) {
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(FwdSeg s1, FwdSeg s2, uint32_t blocks1, NttTables tw,
+                                                             size_t js) {
     using G = RegNtt<E, TP>;
     extern __shared__ uint32_t sh[];
+    const bool second = blockIdx.x >= blocks1;  // uniform over the workgroup
+    RowSrc src = second ? s2.src : s1.src;
+    uint32_t* out = second ? s2.out : s1.out;
+    const uint32_t rows = second ? s2.rows : s1.rows;
     const size_t jo = job_off(js);
     if (!src.a_shared) src.a = jp(src.a, jo);
     src.dst = jp(src.dst, jo);
@@ -198,7 +204,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(RowSrc src, uint32_
     }
     const int tr = threadIdx.x % G::TPR;
     const int rsub = threadIdx.x / G::TPR;
-    const size_t row = (size_t)blockIdx.x * G::RPB + rsub;
+    const size_t row = (size_t)(second ? blockIdx.x - blocks1 : blockIdx.x) * G::RPB + rsub;
     const bool valid = row < rows;
     uint32_t* S = sh + rsub * NPRIMES * G::M;
     // produce the row once, reduce into the 8 primes
@@ -254,12 +260,11 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(RowSrc src, uint32_
 }
 
 template <int E, int TP>
-static void launch_reg(const RowSrc& src, size_t rows, uint32_t* out, const NttTables& tw, hipStream_t s,
-                       const Jobs& jb) {
+static void launch_reg(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, hipStream_t s, const Jobs& jb) {
     using G = RegNtt<E, TP>;
-    const size_t blocks = (rows + G::RPB - 1) / G::RPB;
-    hipLaunchKernelGGL((k_ntt_fwd_reg<E, TP>), dim3((unsigned)blocks, jb.n), dim3(NTT_THREADS), G::LDS_BYTES, s, src,
-                       out, tw, jb.stride, (uint32_t)rows);
+    const size_t b1 = (s1.rows + G::RPB - 1) / G::RPB, b2 = (s2.rows + G::RPB - 1) / G::RPB;
+    hipLaunchKernelGGL((k_ntt_fwd_reg<E, TP>), dim3((unsigned)(b1 + b2), jb.n), dim3(NTT_THREADS), G::LDS_BYTES, s,
+                       s1, s2, (uint32_t)b1, tw, jb.stride);
 }
 
 // DAMASCUS_NTT_KERNEL=layer keeps every degree on the per-layer kernel (for comparisons)
@@ -271,25 +276,31 @@ static bool reg_ntt_enabled() {
     return on;
 }
 
-void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out,
-                         const NttTables& tw, hipStream_t s, const Jobs& jb) {
-    if (rows == 0 || jb.n == 0) return;
+void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, const RowSrc& src2, size_t rows2,
+                          uint32_t* out2, int log_m, const NttTables& tw, hipStream_t s, const Jobs& jb) {
+    if (rows1 + rows2 == 0 || jb.n == 0) return;
+    const FwdSeg s1{src1, out1, (uint32_t)rows1}, s2{src2, out2, (uint32_t)rows2};
     if (reg_ntt_enabled()) {
         switch (log_m) {
-            case 6: return launch_reg<8, 8>(src, rows, out, tw, s, jb);
-            case 7: return launch_reg<16, 8>(src, rows, out, tw, s, jb);
-            case 8: return launch_reg<16, 16>(src, rows, out, tw, s, jb);
-            case 9: return launch_reg<32, 16>(src, rows, out, tw, s, jb);
-            case 10: return launch_reg<32, 32>(src, rows, out, tw, s, jb);
-            case 11: return launch_reg<64, 32>(src, rows, out, tw, s, jb);
+            case 6: return launch_reg<8, 8>(s1, s2, tw, s, jb);
+            case 7: return launch_reg<16, 8>(s1, s2, tw, s, jb);
+            case 8: return launch_reg<16, 16>(s1, s2, tw, s, jb);
+            case 9: return launch_reg<32, 16>(s1, s2, tw, s, jb);
+            case 10: return launch_reg<32, 32>(s1, s2, tw, s, jb);
+            case 11: return launch_reg<64, 32>(s1, s2, tw, s, jb);
             default: break;
         }
     }
     const int m = 1 << log_m;
     const int pg = std::min(NPRIMES, LDS_WORDS >> log_m);
     const size_t lds = (size_t)pg * m * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_ntt_fwd_rows, dim3((unsigned)rows, jb.n), dim3(NTT_THREADS), lds, s, src, log_m,
-                       out, tw, jb.stride);
+    hipLaunchKernelGGL(k_ntt_fwd_rows, dim3((unsigned)(rows1 + rows2), jb.n), dim3(NTT_THREADS), lds, s, s1, s2,
+                       (uint32_t)rows1, log_m, tw, jb.stride);
+}
+
+void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out, const NttTables& tw,
+                         hipStream_t s, const Jobs& jb) {
+    launch_ntt_fwd_rows2(src, rows, out, RowSrc{}, 0, nullptr, log_m, tw, s, jb);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -709,6 +720,160 @@ void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, i
                        log_m, out_inv, out_L, out_R, tw, jb.stride);
 }
 
+// ---------------------------------------------------------------------------------------
+// MAC passes with their reductions and inverse transforms fused: the segment partials of one
+// (k, p) slice are summed into LDS, transformed back there and scaled by 1/m. One workgroup
+// per (k, p); degrees 1 and 2 keep the scalar kernels and separate inverse launches.
+// ---------------------------------------------------------------------------------------
+constexpr int FIN_THREADS = 256;
+
+// Gentleman-Sande inverse layers on `cnt` adjacent slices of 2^log_n words in LDS (no scaling)
+__device__ __forceinline__ void lds_intt_slices(uint32_t* s, int cnt, int log_n, int pi, const NttTables& tw) {
+    const int n = 1 << log_n, half = n >> 1;
+    const uint32_t p = c_pc[pi].p;
+    for (int len = 1; len < n; len <<= 1) {
+        const int lg = __builtin_ctz(len);
+        for (int t = threadIdx.x; t < cnt * half; t += FIN_THREADS) {
+            const int sl = t >> (log_n - 1), b = t & (half - 1);
+            const int blk = b >> lg;
+            const int j = (blk << (lg + 1)) | (b & (len - 1));
+            const int kz = (half >> lg) + blk;
+            uint32_t* sq = s + sl * n;
+            const uint32_t u = sq[j], v = sq[j + len];
+            sq[j] = addp(u, v, p);
+            sq[j + len] = mulc(subp(u, v, p), tw.izeta_m[pi * TW_SIZE + kz], tw.izeta_mp[pi * TW_SIZE + kz], p);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(FIN_THREADS) k_mac_pairs_finish(const uint32_t* __restrict__ part, uint32_t nseg,
+                                                                  int log_m, int nslots, uint32_t* __restrict__ out_inv,
+                                                                  uint32_t* __restrict__ out_L,
+                                                                  uint32_t* __restrict__ out_R, NttTables tw,
+                                                                  size_t js) {
+    extern __shared__ uint32_t sh[];
+    const size_t jo = job_off(js);
+    part = jp(part, jo);
+    out_inv = jp(out_inv, jo);
+    out_L = jp(out_L, jo);
+    out_R = jp(out_R, jo);
+    const int m = 1 << log_m;
+    const uint32_t kp = blockIdx.x, p = kp % NPRIMES;
+    const PrimeConsts pc = c_pc[p];
+    const size_t slot = (size_t)K * NPRIMES * m;
+    for (int e = threadIdx.x; e < nslots * m; e += FIN_THREADS) {
+        const int sl = e >> log_m, c = e & (m - 1);
+        uint64_t acc = 0;
+        for (uint32_t sg = 0; sg < nseg; ++sg) acc += part[(size_t)sg * 3 * slot + sl * slot + (size_t)kp * m + c];
+        sh[e] = reduce_u96(acc, 0, pc);
+    }
+    __syncthreads();
+    lds_intt_slices(sh, nslots, log_m, (int)p, tw);
+    const uint32_t im = tw.invn_m[p * (MAX_LOG_N + 1) + log_m], imp = tw.invn_mp[p * (MAX_LOG_N + 1) + log_m];
+    for (int e = threadIdx.x; e < nslots * m; e += FIN_THREADS) {
+        const int sl = e >> log_m, c = e & (m - 1);
+        uint32_t* o = sl == 0 ? out_inv : (sl == 1 ? out_L : out_R);
+        o[(size_t)kp * m + c] = mulc(sh[e], im, imp, pc.p);
+    }
+}
+
+// split pass: inv (degree m) from (S00, S11); L, R (degree m/2) from the four sums, as in
+// k_mac_split_reduce; then the three inverse transforms. out_lr holds L then R ([k][p][m/2] each).
+__global__ void __launch_bounds__(FIN_THREADS) k_mac_split_finish(const uint32_t* __restrict__ part, uint32_t nseg,
+                                                                  int log_m, uint32_t* __restrict__ out_inv,
+                                                                  uint32_t* __restrict__ out_lr, NttTables tw,
+                                                                  size_t js) {
+    extern __shared__ uint32_t sh[];
+    const size_t jo = job_off(js);
+    part = jp(part, jo);
+    out_inv = jp(out_inv, jo);
+    out_lr = jp(out_lr, jo);
+    const int m = 1 << log_m, h = m >> 1;
+    const uint32_t kp = blockIdx.x, p = kp % NPRIMES;
+    const PrimeConsts pc = c_pc[p];
+    const uint32_t P = pc.p;
+    const size_t slot = (size_t)K * NPRIMES * h;
+    uint32_t* si = sh;
+    uint32_t* sl = sh + m;  // L then R, adjacent
+    uint32_t* sr = sl + h;
+    const uint32_t i4m = tw.invn_m[p * (MAX_LOG_N + 1) + 2], i4mp = tw.invn_mp[p * (MAX_LOG_N + 1) + 2];
+    for (int t = threadIdx.x; t < h; t += FIN_THREADS) {
+        uint64_t a[4] = {0, 0, 0, 0};
+        for (uint32_t sg = 0; sg < nseg; ++sg)
+            for (int j = 0; j < 4; ++j) a[j] += part[(size_t)sg * 4 * slot + j * slot + (size_t)kp * h + t];
+        const uint32_t a00 = reduce_u96(a[0], 0, pc), a11 = reduce_u96(a[1], 0, pc),
+                       a01 = reduce_u96(a[2], 0, pc), a10 = reduce_u96(a[3], 0, pc);
+        si[2 * t] = a00;
+        si[2 * t + 1] = a11;
+        const uint32_t d = subp(a00, a11, P);
+        const uint32_t Lp = addp(subp(d, a01, P), a10, P);
+        const uint32_t Rp = subp(addp(d, a01, P), a10, P);
+        const uint32_t kz = (uint32_t)h + t;
+        sl[t] = mulc(mulc(Lp, tw.zeta_m[p * TW_SIZE + kz], tw.zeta_mp[p * TW_SIZE + kz], P), i4m, i4mp, P);
+        sr[t] = mulc(mulc(Rp, tw.izeta_m[p * TW_SIZE + kz], tw.izeta_mp[p * TW_SIZE + kz], P), i4m, i4mp, P);
+    }
+    __syncthreads();
+    lds_intt_slices(si, 1, log_m, (int)p, tw);
+    lds_intt_slices(sl, 2, log_m - 1, (int)p, tw);
+    const uint32_t im = tw.invn_m[p * (MAX_LOG_N + 1) + log_m], imp = tw.invn_mp[p * (MAX_LOG_N + 1) + log_m];
+    const uint32_t ih = tw.invn_m[p * (MAX_LOG_N + 1) + log_m - 1], ihp = tw.invn_mp[p * (MAX_LOG_N + 1) + log_m - 1];
+    for (int c = threadIdx.x; c < m; c += FIN_THREADS) out_inv[(size_t)kp * m + c] = mulc(si[c], im, imp, P);
+    for (int t = threadIdx.x; t < h; t += FIN_THREADS) {
+        out_lr[(size_t)kp * h + t] = mulc(sl[t], ih, ihp, P);
+        out_lr[slot + (size_t)kp * h + t] = mulc(sr[t], ih, ihp, P);
+    }
+}
+
+void launch_mac_pairs_inv(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
+                          uint32_t* out_L, uint32_t* out_R, uint32_t* part, const NttTables& tw, hipStream_t s,
+                          const Jobs& jb, bool g_shared) {
+    if (jb.n == 0) return;
+    const bool cross = out_L != nullptr && rows > 1;
+    if (log_m < 2) {
+        launch_mac_pairs(What, Ghat, rows, log_m, out_inv, out_L, out_R, part, s, jb, g_shared);
+        launch_ntt_inv(out_inv, K, log_m, tw, s, jb);
+        if (cross) {
+            launch_ntt_inv(out_L, K, log_m, tw, s, jb);
+            launch_ntt_inv(out_R, K, log_m, tw, s, jb);
+        }
+        return;
+    }
+    const size_t gjs = g_shared ? 0 : jb.stride;
+    const size_t half = cross ? rows / 2 : rows;
+    const uint32_t nseg = mac_segments(half, log_m);
+    const uint32_t seg_rows = (uint32_t)((half + nseg - 1) / nseg);
+    const size_t m = (size_t)1 << log_m;
+    const size_t threads = (size_t)nseg * K * NPRIMES * m / 4;
+    hipLaunchKernelGGL(k_mac_pairs4, dim3((unsigned)((threads + MAC_THREADS - 1) / MAC_THREADS), jb.n),
+                       dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, nseg, seg_rows, (int)cross, part,
+                       jb.stride, gjs);
+    const int nslots = cross ? 3 : 1;
+    hipLaunchKernelGGL(k_mac_pairs_finish, dim3(K * NPRIMES, jb.n), dim3(FIN_THREADS), (size_t)nslots * m * 4, s,
+                       part, nseg, log_m, nslots, out_inv, out_L, out_R, tw, jb.stride);
+}
+
+void launch_mac_split_inv(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
+                          uint32_t* out_lr, uint32_t* part, const NttTables& tw, hipStream_t s, const Jobs& jb) {
+    if (jb.n == 0) return;
+    const size_t m = (size_t)1 << log_m;
+    if (log_m < 2) {
+        launch_mac_split(What, Ghat, rows, log_m, out_inv, out_lr, out_lr + (size_t)K * NPRIMES * (m / 2), part, tw, s,
+                         jb);
+        launch_ntt_inv(out_inv, K, log_m, tw, s, jb);
+        launch_ntt_inv(out_lr, 2 * K, log_m - 1, tw, s, jb);
+        return;
+    }
+    const uint32_t nseg = mac_segments(rows * 2, log_m);
+    const uint32_t seg_rows = (uint32_t)((rows + nseg - 1) / nseg);
+    const size_t threads = (size_t)nseg * K * NPRIMES * m / 4;
+    hipLaunchKernelGGL(k_mac_split4, dim3((unsigned)((threads + MAC_THREADS - 1) / MAC_THREADS), jb.n),
+                       dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, nseg, seg_rows, part, jb.stride,
+                       jb.stride);
+    hipLaunchKernelGGL(k_mac_split_finish, dim3(K * NPRIMES, jb.n), dim3(FIN_THREADS), (size_t)2 * m * 4, s, part,
+                       nseg, log_m, out_inv, out_lr, tw, jb.stride);
+}
+
 size_t mac_part_words(int max_log_m) {
     // a launch holds at most max(2^18, quads) threads of 4 positions, in at most 3 slots
     size_t w = 0;
@@ -781,6 +946,60 @@ void launch_crt_lift(const uint32_t* res, zq* out, size_t slices, int log_m, hip
     if (!total || jb.n == 0) return;
     hipLaunchKernelGGL(k_crt_lift, dim3((unsigned)((total + 255) / 256), jb.n), dim3(256), 0, s, res, out,
                        total, log_m, jb.stride);
+}
+
+// Lifts a stage's commitment residues and its cross terms in one launch. Part A (K slices of
+// degree 2^log_ma): the lifted commitment is compared with the commitment update
+// c' = c + s^-1 l + s r (poly: even(c) + ...), which is written to com_out, and a mismatch sets
+// `bit` in flags (the folding invariant); with cu.c == nullptr the lift itself is written to
+// com_out (and com_out2). Part B: lr_slices slices of degree 2^log_mb lifted to out_lr.
+__global__ void __launch_bounds__(256) k_lift_check(const uint32_t* __restrict__ res_a, int log_ma, ComUpd cu,
+                                                    zq* com_out, zq* com_out2, uint32_t* flags, uint32_t bit,
+                                                    const uint32_t* __restrict__ res_b, zq* __restrict__ out_lr,
+                                                    uint32_t lr_slices, int log_mb, size_t js) {
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t ta = (size_t)K << log_ma, tb = (size_t)lr_slices << log_mb;
+    if (gid >= ta + tb) return;
+    const size_t jo = job_off(js);
+    uint32_t r[NPRIMES];
+    if (gid < ta) {
+        const size_t m = (size_t)1 << log_ma;
+        const size_t k = gid >> log_ma, i = gid & (m - 1);
+        res_a = jp(res_a, jo);
+#pragma unroll
+        for (int p = 0; p < NPRIMES; ++p) r[p] = res_a[(k * NPRIMES + p) * m + i];
+        const zq v = crt_lift_residues(r);
+        if (!cu.c) {
+            store_zq(jp(com_out, jo) + gid, v);
+            if (com_out2) store_zq(jp(com_out2, jo) + gid, v);
+            return;
+        }
+        const zq* c = jp(cu.c, jo);
+        const zq* scal = jp(cu.scal, jo);
+        const zq s = scal[0], sinv = scal[1];
+        const zq base = cu.poly ? load_zq(c + k * (2 * m) + 2 * i) : load_zq(c + gid);
+        zq e = zq_add(base, zq_mul(load_zq(jp(cu.l, jo) + gid), sinv));
+        e = zq_add(e, zq_mul(load_zq(jp(cu.r, jo) + gid), s));
+        store_zq(jp(com_out, jo) + gid, e);
+        if (!zq_eq(v, e)) atomicOr(jp(flags, jo), bit);
+    } else {
+        const size_t g = gid - ta;
+        const size_t m = (size_t)1 << log_mb;
+        const size_t sl = g >> log_mb, c = g & (m - 1);
+        res_b = jp(res_b, jo);
+#pragma unroll
+        for (int p = 0; p < NPRIMES; ++p) r[p] = res_b[(sl * NPRIMES + p) * m + c];
+        store_zq(jp(out_lr, jo) + g, crt_lift_residues(r));
+    }
+}
+
+void launch_lift_check(const uint32_t* res_a, int log_ma, const ComUpd& cu, zq* com_out, zq* com_out2,
+                       uint32_t* flags, uint32_t bit, const uint32_t* res_b, zq* out_lr, size_t lr_slices, int log_mb,
+                       hipStream_t s, const Jobs& jb) {
+    const size_t total = ((size_t)K << log_ma) + (lr_slices << log_mb);
+    if (!total || jb.n == 0) return;
+    hipLaunchKernelGGL(k_lift_check, dim3((unsigned)((total + 255) / 256), jb.n), dim3(256), 0, s, res_a, log_ma, cu,
+                       com_out, com_out2, flags, bit, res_b, out_lr, (uint32_t)lr_slices, log_mb, jb.stride);
 }
 
 // Z_q-only row production (the verifier's generator folds, verifier.rs:141-179)

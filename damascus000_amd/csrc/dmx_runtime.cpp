@@ -244,9 +244,7 @@ struct DevBuf {
 
 struct TimedLaunch {
     hipEvent_t a, b;
-    size_t rows;
-    int log_m;
-    int mode;
+    double work;  // algorithmic lane-cycles of the launch
 };
 
 struct dmx_ctx {
@@ -292,17 +290,20 @@ static double fwd_row_work(int log_m, int mode) {
     return w;
 }
 
-// forward transforms go through here so that kernel timing can bracket them
-static void fwd_rows(dmx_ctx* ctx, const RowSrc& src, size_t rows, int log_m, uint32_t* out, hipStream_t s,
-                     const Jobs& jb) {
-    if (!ctx->timing || rows == 0 || (ctx->timing_log_m >= 0 && ctx->timing_log_m != log_m)) {
-        launch_ntt_fwd_rows(src, rows, log_m, out, ctx->tw, s, jb);
+// forward transforms go through here so that kernel timing can bracket them; two row sets of
+// one degree share a launch
+static void fwd_rows2(dmx_ctx* ctx, const RowSrc& src1, size_t rows1, uint32_t* out1, const RowSrc& src2,
+                      size_t rows2, uint32_t* out2, int log_m, hipStream_t s, const Jobs& jb) {
+    if (!ctx->timing || rows1 + rows2 == 0 || (ctx->timing_log_m >= 0 && ctx->timing_log_m != log_m)) {
+        launch_ntt_fwd_rows2(src1, rows1, out1, src2, rows2, out2, log_m, ctx->tw, s, jb);
         return;
     }
     std::lock_guard<std::mutex> g(ctx->mu);
-    TimedLaunch t{ctx->take_event(), ctx->take_event(), rows * jb.n, log_m, src.mode};
+    const double work = (double)jb.n * ((double)rows1 * fwd_row_work(log_m, src1.mode) +
+                                        (double)rows2 * fwd_row_work(log_m, src2.mode));
+    TimedLaunch t{ctx->take_event(), ctx->take_event(), work};
     (void)hipEventRecord(t.a, s);
-    launch_ntt_fwd_rows(src, rows, log_m, out, ctx->tw, s, jb);
+    launch_ntt_fwd_rows2(src1, rows1, out1, src2, rows2, out2, log_m, ctx->tw, s, jb);
     (void)hipEventRecord(t.b, s);
     ctx->launches.push_back(t);
 }
@@ -330,7 +331,7 @@ extern "C" int dmx_kernel_timing(dmx_ctx* ctx, double* total_ms, uint64_t* launc
         float e = 0;
         HIP_TRY(hipEventElapsedTime(&e, t.a, t.b));
         ms += e;
-        work += (double)t.rows * fwd_row_work(t.log_m, t.mode);
+        work += t.work;
         ctx->event_pool.push_back(t.a);
         ctx->event_pool.push_back(t.b);
     }
@@ -758,25 +759,23 @@ static int prover_init_commit(dmx_prover* p) {
     const size_t N = p->N;
     const int d = p->d;
     const NttTables& tw = p->ctx->tw;
-    if (!p->Ghat0_src) {
-        RowSrc gen{};
-        gen.mode = ROW_GEN;
-        gen.dst = p->G_a.as<zq>();
-        gen.gseed = p->gseed();
-        fwd_rows(p->ctx, gen, (size_t)K * N, d, p->Ghat_a.as<uint32_t>(), s, jb);
-    }
     RowSrc pl{};
     pl.mode = ROW_PLAIN;
     pl.a = p->W_a.as<zq>();
-    fwd_rows(p->ctx, pl, N, d, p->What_a.as<uint32_t>(), s, jb);
+    RowSrc gen{};
+    gen.mode = ROW_GEN;
+    gen.dst = p->G_a.as<zq>();
+    gen.gseed = p->gseed();
+    // witness rows, and the generator rows unless a resident matrix serves every job
+    fwd_rows2(p->ctx, pl, N, p->What_a.as<uint32_t>(), gen, p->Ghat0_src ? 0 : (size_t)K * N,
+              p->Ghat_a.as<uint32_t>(), d, s, jb);
     const size_t slot = (size_t)K * NPRIMES * N;
     const bool shared = p->Ghat0_src != nullptr;
-    launch_mac_pairs(p->What_a.as<uint32_t>(), shared ? p->Ghat0_src : p->Ghat_a.as<uint32_t>(), N, d, p->res_at(0),
-                     p->res_at(slot), p->res_at(2 * slot), p->part.as<uint32_t>(), s, jb, shared);
-    launch_ntt_inv(p->res_at(0), 3 * K, d, tw, s, jb);
-    launch_crt_lift(p->res_at(0), p->C.as<zq>(), K, d, s, jb);
-    launch_crt_lift(p->res_at(slot), p->rec_lvec(0), 2 * K, d, s, jb);  // L_vec, R_vec adjacent
-    launch_copy16(p->com0.p, p->C.p, (size_t)K * N, s, jb);
+    launch_mac_pairs_inv(p->What_a.as<uint32_t>(), shared ? p->Ghat0_src : p->Ghat_a.as<uint32_t>(), N, d,
+                         p->res_at(0), p->res_at(slot), p->res_at(2 * slot), p->part.as<uint32_t>(), tw, s, jb, shared);
+    // com_0 (to C and com0) and L_vec, R_vec (adjacent) in one lift
+    launch_lift_check(p->res_at(0), d, ComUpd{}, p->C.as<zq>(), p->com0.as<zq>(), nullptr, 0, p->res_at(slot),
+                      p->rec_lvec(0), 2 * K, d, s, jb);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
@@ -797,14 +796,12 @@ static int prover_round(dmx_prover* p, size_t j) {
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[0], s));
     // ---- vector stage ----
     p->tr.challenge("vec", j, p->C.as<zq>(), m, p->rec_lvec(j), m, p->rec_rvec(j), m, p->sc(4 * j), flags, s, jb);
-    launch_com_update(p->C.as<zq>(), p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j), 0, lm, p->Ct.as<zq>(), s, jb);
     RowSrc vw{};
     vw.mode = ROW_VFOLD;
     vw.a = p->W_a.as<zq>();
     vw.b_off = h;
     vw.scal = p->sc(4 * j);  // m_L + x * m_R
     vw.dst = p->W_b.as<zq>();
-    fwd_rows(p->ctx, vw, h, lm, p->What_b.as<uint32_t>(), s, jb);
     RowSrc vg{};
     vg.mode = ROW_VFOLD;
     vg.a = (j == 0 && p->G0_src) ? p->G0_src : p->G_a.as<zq>();
@@ -812,44 +809,40 @@ static int prover_round(dmx_prover* p, size_t j) {
     vg.b_off = h * K;
     vg.scal = p->sc(4 * j + 1);  // g_L + x^-1 * g_R
     vg.dst = p->G_b.as<zq>();
-    fwd_rows(p->ctx, vg, h * K, lm, p->Ghat_b.as<uint32_t>(), s, jb);
-    // invariant at degree m and the poly cross terms at degree m/2 from one pass
+    fwd_rows2(p->ctx, vw, h, p->What_b.as<uint32_t>(), vg, h * K, p->Ghat_b.as<uint32_t>(), lm, s, jb);
+    // invariant at degree m and the poly cross terms at degree m/2 from one pass; the lift
+    // checks the invariant against C_t = C + x^-1 L + x R (written to Ct) and emits L/R_poly
     const size_t s0 = (size_t)K * NPRIMES * m;
-    launch_mac_split(p->What_b.as<uint32_t>(), p->Ghat_b.as<uint32_t>(), h, lm, p->res_at(0), p->res_at(s0),
-                     p->res_at(s0 + (size_t)K * NPRIMES * h), p->part.as<uint32_t>(), tw, s, jb);
-    launch_ntt_inv(p->res_at(0), K, lm, tw, s, jb);
-    launch_ntt_inv(p->res_at(s0), 2 * K, lm - 1, tw, s, jb);
-    launch_crt_lift(p->res_at(0), p->lifted.as<zq>(), K, lm, s, jb);
-    launch_crt_lift(p->res_at(s0), p->rec_lpoly(j), 2 * K, lm - 1, s, jb);
-    launch_com_check(p->lifted.as<zq>(), p->Ct.as<zq>(), (size_t)K * m, flags, vec_bit(j), s, jb);
+    launch_mac_split_inv(p->What_b.as<uint32_t>(), p->Ghat_b.as<uint32_t>(), h, lm, p->res_at(0), p->res_at(s0),
+                         p->part.as<uint32_t>(), tw, s, jb);
+    launch_lift_check(p->res_at(0), lm, ComUpd{p->C.as<zq>(), p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j), 0},
+                      p->Ct.as<zq>(), nullptr, flags, vec_bit(j), p->res_at(s0), p->rec_lpoly(j), 2 * K, lm - 1, s, jb);
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[1], s));
     // ---- poly stage ----
     p->tr.challenge("poly", j, p->Ct.as<zq>(), m, p->rec_lpoly(j), h, p->rec_rpoly(j), h, p->sc(4 * j + 2), flags, s,
                     jb);
-    launch_com_update(p->Ct.as<zq>(), p->rec_lpoly(j), p->rec_rpoly(j), p->sc(4 * j + 2), 1, lm - 1, p->C.as<zq>(), s,
-                      jb);
     RowSrc pw{};
     pw.mode = ROW_PFOLD_W;
     pw.a = p->W_b.as<zq>();
     pw.scal = p->sc(4 * j + 2);  // e + y * o
     pw.dst = p->W_a.as<zq>();
-    fwd_rows(p->ctx, pw, h, lm - 1, p->What_a.as<uint32_t>(), s, jb);
     RowSrc pg{};
     pg.mode = ROW_PFOLD_G;
     pg.a = p->G_b.as<zq>();
     pg.scal = p->sc(4 * j + 3);  // g_e + y^-1 * X g_o
     pg.dst = p->G_a.as<zq>();
-    fwd_rows(p->ctx, pg, h * K, lm - 1, p->Ghat_a.as<uint32_t>(), s, jb);
+    fwd_rows2(p->ctx, pw, h, p->What_a.as<uint32_t>(), pg, h * K, p->Ghat_a.as<uint32_t>(), lm - 1, s, jb);
     // post-poly invariant, plus the next round's vector cross terms from the same pass
     const size_t s1 = (size_t)K * NPRIMES * h;
     const bool next = h > 1;
-    launch_mac_pairs(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), h, lm - 1, p->res_at(0),
-                     next ? p->res_at(s1) : nullptr, next ? p->res_at(2 * s1) : nullptr, p->part.as<uint32_t>(), s,
-                     jb);
-    launch_ntt_inv(p->res_at(0), next ? 3 * K : K, lm - 1, tw, s, jb);
-    launch_crt_lift(p->res_at(0), p->lifted.as<zq>(), K, lm - 1, s, jb);
-    if (next) launch_crt_lift(p->res_at(s1), p->rec_lvec(j + 1), 2 * K, lm - 1, s, jb);
-    launch_com_check(p->lifted.as<zq>(), p->C.as<zq>(), (size_t)K * h, flags, poly_bit(j), s, jb);
+    launch_mac_pairs_inv(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), h, lm - 1, p->res_at(0),
+                         next ? p->res_at(s1) : nullptr, next ? p->res_at(2 * s1) : nullptr, p->part.as<uint32_t>(), tw,
+                         s, jb);
+    // the lift checks the invariant against C' = even(C_t) + y^-1 L + y R (written to C)
+    launch_lift_check(p->res_at(0), lm - 1,
+                      ComUpd{p->Ct.as<zq>(), p->rec_lpoly(j), p->rec_rpoly(j), p->sc(4 * j + 2), 1}, p->C.as<zq>(),
+                      nullptr, flags, poly_bit(j), next ? p->res_at(s1) : nullptr, p->rec_lvec(j + 1),
+                      next ? 2 * K : 0, lm - 1, s, jb);
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[2], s));
     HIP_TRY(hipGetLastError());
     return DMX_OK;
@@ -2071,9 +2064,8 @@ static int commit_impl(dmx_ctx* ctx, const uint64_t* witness, const uint64_t* ge
             launch_ntt_fwd_rows(pg, nr * K, lg, Ghat.as<uint32_t>(), ctx->tw, s);
             HIP_TRY(hipStreamSynchronize(s));
         }
-        launch_mac_pairs(What.as<uint32_t>(), Ghat.as<uint32_t>(), nr, lg, res.as<uint32_t>(), nullptr, nullptr,
-                         mpart.as<uint32_t>(), s);
-        launch_ntt_inv(res.as<uint32_t>(), K, lg, ctx->tw, s);
+        launch_mac_pairs_inv(What.as<uint32_t>(), Ghat.as<uint32_t>(), nr, lg, res.as<uint32_t>(), nullptr, nullptr,
+                             mpart.as<uint32_t>(), ctx->tw, s);
         launch_crt_lift(res.as<uint32_t>(), part.as<zq>(), K, lg, s);
         launch_zq_add(acc.as<zq>(), part.as<zq>(), acc.as<zq>(), (size_t)K * m, s);
         HIP_TRY(hipStreamSynchronize(s));

@@ -358,7 +358,7 @@ def test_kernel_timing_counts_forward_transforms():
     dm.prove_verify_device(buf.ptr, len(data), want_proof=False)
     ms, n, work = ctx.kernel_timing()
     ctx.kernel_timing(False)
-    assert n == 2 + 4 * 6 and ms > 0 and work > 0  # init (gen + witness) + 4 per round
+    assert n == 1 + 2 * 6 and ms > 0 and work > 0  # init (witness + generators) + 2 per round
 
 
 def test_every_byte_flip_of_first_two_records_is_rejected_on_gpu():
