@@ -128,28 +128,90 @@ __device__ uint8_t transcript_byte(const TranscriptMsg& t, size_t o, uint32_t co
     return le_byte(counter, (uint32_t)o);
 }
 
+// Where message byte o of the header (o < 96 + label_len) comes from: a device byte, or an
+// immediate value.
+__device__ __forceinline__ const uint8_t* header_src(const TranscriptMsg& t, uint32_t o, uint32_t& imm) {
+    imm = 0;
+    if (o < 32) return t.epoch_seed + o;
+    o -= 32;
+    if (o < 8) {
+        imm = le_byte(t.label_len, o);
+        return nullptr;
+    }
+    o -= 8;
+    if (o < t.label_len) {
+        const uint32_t lw = (uint32_t)t.label[0] | ((uint32_t)t.label[1] << 8) | ((uint32_t)t.label[2] << 16) |
+                            ((uint32_t)t.label[3] << 24);
+        imm = (lw >> (8 * o)) & 0xFFu;
+        return nullptr;
+    }
+    o -= t.label_len;
+    if (o < 8) {
+        imm = le_byte(32, o);
+        return nullptr;
+    }
+    o -= 8;
+    if (o < 32) return t.file_id + o;
+    o -= 32;
+    if (o < 8) return nullptr;  // epoch index 0
+    o -= 8;
+    imm = le_byte(t.round, o);
+    return nullptr;
+}
+
+// One thread per coefficient of the three modules: one 16-byte load, 16 byte stores at its
+// message offset (the modules start at an odd offset for a 3-byte label). The thread of
+// coefficient 0 of a coordinate also writes that coordinate's u64 length (and, for coordinate
+// 0, the module's u64 K). Eight more threads write the header (16 bytes each) and the zero
+// counter.
 __global__ void k_transcript_pack(TranscriptMsg t, uint8_t* __restrict__ msg, size_t len, size_t js) {
-    const size_t w = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t o = w * 4;
-    if (o >= len) return;
+    const uint32_t gid = (uint32_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t c0 = K * t.lens[0], c1 = c0 + K * t.lens[1], total = c1 + K * t.lens[2];
+    if (gid >= total + 8) return;
     const size_t jo = job_off(js);
-    t.epoch_seed = jp(t.epoch_seed, jo);
-    t.file_id = jp(t.file_id, jo);
-    for (int i = 0; i < 3; ++i) t.mods[i] = jp(t.mods[i], jo);
     msg = jp(msg, jo);
-    uint32_t v = 0;
-    for (uint32_t b = 0; b < 4 && o + b < len; ++b) v |= (uint32_t)transcript_byte(t, o + b, 0) << (8 * b);
-    if (o + 4 <= len) {
-        *reinterpret_cast<uint32_t*>(msg + o) = v;
-    } else {
-        for (uint32_t b = 0; o + b < len; ++b) msg[o + b] = (uint8_t)(v >> (8 * b));
+    const uint32_t H = 96 + t.label_len;
+    if (gid >= total) {  // header slice or counter
+        t.epoch_seed = jp(t.epoch_seed, jo);
+        t.file_id = jp(t.file_id, jo);
+        const uint32_t h0 = (gid - total) * 16;
+        if (h0 < H) {
+            const uint32_t h1 = min(H, h0 + 16);
+            for (uint32_t o = h0; o < h1; ++o) {
+                uint32_t imm;
+                const uint8_t* src = header_src(t, o, imm);
+                msg[o] = src ? *src : (uint8_t)imm;
+            }
+        }
+        if (gid == total + 7)
+            for (uint32_t b = 0; b < 4; ++b) msg[len - 4 + b] = 0;  // counter, zero until a resample
+        return;
+    }
+    const int mi = gid < c0 ? 0 : (gid < c1 ? 1 : 2);
+    const uint32_t rem = gid - (mi == 0 ? 0 : (mi == 1 ? c0 : c1));
+    const uint32_t n = t.lens[mi];
+    const uint32_t k = rem >> __builtin_ctz(n), c = rem & (n - 1);
+    uint32_t mstart = H;
+    for (int i = 0; i < mi; ++i) mstart += 8 + K * (8 + 16 * t.lens[i]);
+    const uint32_t coord = mstart + 8 + k * (8 + 16 * n);  // the coordinate's u64 length
+    const uint4 v = *reinterpret_cast<const uint4*>(jp(t.mods[mi], jo) + rem);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    uint8_t* d = msg + coord + 8 + 16 * c;
+#pragma unroll
+    for (int b = 0; b < 16; ++b) d[b] = (uint8_t)(w[b >> 2] >> (8 * (b & 3)));
+    if (c == 0) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) msg[coord + b] = le_byte(n, b);
+        if (k == 0)
+#pragma unroll
+            for (int b = 0; b < 8; ++b) msg[mstart + b] = le_byte(K, b);
     }
 }
 
 void launch_transcript_pack(const TranscriptMsg& t, uint8_t* msg, size_t len, hipStream_t s, const Jobs& jb) {
-    const size_t words = (len + 3) / 4;
+    const size_t threads = (size_t)K * (t.lens[0] + t.lens[1] + t.lens[2]) + 8;
     if (jb.n == 0) return;
-    hipLaunchKernelGGL(k_transcript_pack, dim3((unsigned)((words + 255) / 256), jb.n), dim3(256), 0, s, t, msg,
+    hipLaunchKernelGGL(k_transcript_pack, dim3((unsigned)((threads + 255) / 256), jb.n), dim3(256), 0, s, t, msg,
                        len, jb.stride);
 }
 
