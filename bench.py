@@ -84,8 +84,8 @@ def pct(xs, q):
     return xs[min(len(xs) - 1, int(q * len(xs)))]
 
 
-def run_served(dm, ctx, dev, arrivals):
-    srv = dm.Server(ctx, max_batch=64, max_inflight=4)
+def run_served(dm, ctx, dev, arrivals, files_on_host=False):
+    srv = dm.Server(ctx, max_batch=64, max_inflight=4, files_on_host=files_on_host)
     n = len(dev)
     tickets = [None] * n
     t_sub = [0.0] * n
@@ -136,8 +136,14 @@ def mixed_served(dm, ctx, requests, rate):
     total = sum(len(f) for _, f in trace) / (1 << 20)
     run_served(dm, ctx, dev, arrivals)  # warm: workspaces and captured graphs per (depth, batch)
     served, s_lat, s_span = run_served(dm, ctx, dev, arrivals)
+    # the same trace handed over in host memory (PCIe-inclusive: pinned staging + async upload)
+    host = [f for _, f in trace]
+    run_served(dm, ctx, host, arrivals, files_on_host=True)
+    h_res, h_lat, h_span = run_served(dm, ctx, host, arrivals, files_on_host=True)
     run_plain(dm, dev[: min(len(dev), 32)], arrivals[: min(len(dev), 32)])  # warm
     plain, p_lat, p_span = run_plain(dm, dev, arrivals)
+    if h_res != served:
+        raise SystemExit("host-file and device-file served proofs differ")
     if any(not isinstance(r, tuple) for r in served):
         raise SystemExit("a served request failed")
     if served != plain:
@@ -147,6 +153,8 @@ def mixed_served(dm, ctx, requests, rate):
         "served_p50_ms": pct(s_lat, 0.5) * 1e3, "served_p99_ms": pct(s_lat, 0.99) * 1e3,
         "plain_p50_ms": pct(p_lat, 0.5) * 1e3, "plain_p99_ms": pct(p_lat, 0.99) * 1e3,
         "offered_mib_s": total / max(arrivals[-1] - arrivals[0], 1e-9),
+        "host_files_mib_s": total / h_span, "host_files_p50_ms": pct(h_lat, 0.5) * 1e3,
+        "host_files_p99_ms": pct(h_lat, 0.99) * 1e3,
     }
 
 

@@ -16,7 +16,7 @@ import numpy as np
 __all__ = [
     "Q", "K", "PRIMES", "DamascusError", "SystemParams", "RuntimeConfig", "Context", "backend_ready",
     "device_summary", "ModuleSisCommitter", "DamascusProver", "DamascusVerifier", "ntt_batch", "fold_batch",
-    "ring_mul_batch", "ring_mul_batch_device", "prove", "verify", "prove_verify_device", "prove_verify_many", "Server", "derive_subseed", "lib", "LIB_PATH",
+    "ring_mul_batch", "ring_mul_batch_device", "prove", "verify", "prove_verify_device", "prove_verify_many", "Server", "map_file", "derive_subseed", "lib", "LIB_PATH",
     "DeviceBuffer",
 ]
 
@@ -470,10 +470,12 @@ def prove_verify_device(d_file_ptr, length, params=None, ctx=None, want_proof=Tr
     return None
 
 
-def prove_verify_many(files, params=None, ctx=None, max_batch=0, want_proof=True, raise_on_error=True):
+def prove_verify_many(files, params=None, ctx=None, max_batch=0, want_proof=True, raise_on_error=True,
+                      files_on_host=None):
     """prove + replay of many files in one ragged batch (dmx_prove_verify_many).
 
-    files: bytes objects (copied from host memory) or (device_pointer, length) pairs.
+    files: bytes objects (host memory) or (pointer, length) pairs: device pointers, or host
+    pointers (e.g. MappedFile.job()) with files_on_host=True.
     params: one SystemParams for all, or a list with one per file.
     Returns a list of (statement, records_blob, opening) per file, or of status codes when
     raise_on_error is False and a job failed (then the entry is the negative code)."""
@@ -481,12 +483,12 @@ def prove_verify_many(files, params=None, ctx=None, max_batch=0, want_proof=True
     n = len(files)
     if n == 0:
         return []
-    on_host = not isinstance(files[0], tuple)
+    on_host = (not isinstance(files[0], tuple)) if files_on_host is None else bool(files_on_host)
     plist = params if isinstance(params, (list, tuple)) else [params or SystemParams()] * n
     jobs = (Job * n)()
     keep = []
     for i, f in enumerate(files):
-        if on_host:
+        if not isinstance(f, tuple):
             buf = ctypes.create_string_buffer(bytes(f), max(len(f), 1))
             keep.append(buf)
             jobs[i].file = ctypes.cast(buf, _VP)
@@ -638,3 +640,25 @@ def ring_mul_batch_device(d_a, d_b, d_out, log_n, mode=0, ctx=None):
     ctx = ctx or Context.default()
     logs = np.ascontiguousarray(np.asarray(log_n, dtype=np.uint32))
     _check(lib().dmx_ring_mul_batch_device(ctx.h, _VP(d_a), _VP(d_b), _VP(d_out), _ptr(logs), len(logs), mode))
+
+
+class MappedFile:
+    """a read-only memory map of a file, the way the reference reads its input
+    (src/utils/io.rs:21-70 maps the file): .ptr / .len describe the bytes in host memory, for
+    a files_on_host Server or prove_verify_many with (ptr, len) pairs."""
+
+    def __init__(self, path):
+        self.len = os.path.getsize(path)
+        if self.len:
+            self._map = np.memmap(path, dtype=np.uint8, mode="r")
+            self.ptr = self._map.ctypes.data
+        else:
+            self._map = np.zeros(1, dtype=np.uint8)
+            self.ptr = self._map.ctypes.data
+
+    def job(self):
+        return (self.ptr, self.len)
+
+
+def map_file(path):
+    return MappedFile(path)

@@ -1493,6 +1493,10 @@ struct BatchJob {
     GenEntry* gen = nullptr;
     bool busy = false;
     uint64_t last_use = 0;
+    // pinned staging for files handed over in host memory: the host copies each file in, and
+    // the upload is an asynchronous DMA on the batch's stream, overlapping other batches' work
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
     void drop_resident() {
         if (exec_r) (void)hipGraphExecDestroy(exec_r);
         if (graph_r) (void)hipGraphDestroy(graph_r);
@@ -1502,6 +1506,7 @@ struct BatchJob {
         gen = nullptr;
     }
     ~BatchJob() {
+        if (pinned) (void)hipHostFree(pinned);  // idle batch: its uploads have completed
         drop_resident();
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1628,11 +1633,30 @@ static int batch_launch(BatchJob* J, const JobIn* jobs, bool host_files, bool re
                              hipMemcpyHostToDevice, p->s));
     HIP_TRY(copy_rows(v->meta.p, st, v->stage.data(), sizeof(JobMeta), sizeof(JobMeta), J->n,
                              hipMemcpyHostToDevice, p->s));
-    if (host_files)
-        for (uint32_t b = 0; b < J->n; ++b)
-            if (jobs[b].len)
-                HIP_TRY(hipMemcpyAsync(job_ptr(p->file.p, p->jb, b), jobs[b].file, jobs[b].len, hipMemcpyHostToDevice,
-                                       p->s));
+    if (host_files) {
+        size_t need = 0;
+        for (uint32_t b = 0; b < J->n; ++b) need += (jobs[b].len + 255) & ~(size_t)255;
+        if (need > J->pinned_bytes) {
+            HIP_TRY(hipStreamSynchronize(p->s));  // the previous uploads from the old staging are done
+            if (J->pinned) (void)hipHostFree(J->pinned);
+            J->pinned = nullptr;
+            J->pinned_bytes = 0;
+            if (hipHostMalloc(&J->pinned, need, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                J->pinned = nullptr;
+                return fail(DMX_ERR_OOM, "pinned staging of %zu bytes failed", need);
+            }
+            J->pinned_bytes = need;
+        }
+        size_t off = 0;
+        for (uint32_t b = 0; b < J->n; ++b) {
+            if (!jobs[b].len) continue;
+            uint8_t* st = static_cast<uint8_t*>(J->pinned) + off;
+            memcpy(st, jobs[b].file, jobs[b].len);
+            HIP_TRY(hipMemcpyAsync(job_ptr(p->file.p, p->jb, b), st, jobs[b].len, hipMemcpyHostToDevice, p->s));
+            off += (jobs[b].len + 255) & ~(size_t)255;
+        }
+    }
     bool timing;
     {
         std::lock_guard<std::mutex> g(J->ctx->mu);

@@ -495,3 +495,25 @@ def test_ntt_off_schoolbook_products_match_ntt_products_and_oracle():
             assert np.array_equal(p, O.negacyclic_multiply(x, y))
     with pytest.raises(dm.DamascusError):
         dm.ring_mul_batch(a[:1], b[:1], config=dm.RuntimeConfig(gpu_enabled=False))
+
+
+def test_server_ingests_mapped_host_files_through_pinned_staging(tmp_path):
+    """files handed over as read-only memory maps of files on disk (the reference maps its
+    input): the server copies them through pinned staging with asynchronous uploads"""
+    rng = np.random.default_rng(517)
+    files = [_rand_file(rng, n) for n in (70000, 1000, 300000, 53248)]
+    maps = []
+    for i, f in enumerate(files):
+        path = tmp_path / f"in{i}.bin"
+        path.write_bytes(f)
+        maps.append(dm.map_file(str(path)))
+    srv = dm.Server(files_on_host=True, max_batch=2, max_inflight=2)
+    try:
+        tickets = [srv.submit(m.job()) for m in maps]
+        res = [srv.wait(t) for t in tickets]
+    finally:
+        srv.close()
+    for f, (r, lat) in zip(files, res):
+        assert r == dm.prove(f), len(f)
+    many = dm.prove_verify_many([m.job() for m in maps], files_on_host=True)
+    assert many == [r for r, _ in res]
