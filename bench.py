@@ -84,8 +84,11 @@ def pct(xs, q):
     return xs[min(len(xs) - 1, int(q * len(xs)))]
 
 
+INFLIGHT = 4
+
+
 def run_served(dm, ctx, dev, arrivals, files_on_host=False):
-    srv = dm.Server(ctx, max_batch=64, max_inflight=4, files_on_host=files_on_host)
+    srv = dm.Server(ctx, max_batch=64, max_inflight=INFLIGHT, files_on_host=files_on_host)
     n = len(dev)
     tickets = [None] * n
     t_sub = [0.0] * n
@@ -109,9 +112,14 @@ def run_served(dm, ctx, dev, arrivals, files_on_host=False):
         results[i] = r
         lat[i] = (t_sub[i] - arrivals[i]) + server_lat
     th.join()
+    global LAST_STATS
+    LAST_STATS = srv.stats()
     srv.close()
     makespan = max(a + l for a, l in zip(arrivals, lat)) - arrivals[0]
     return results, lat, makespan
+
+
+LAST_STATS = None
 
 
 def run_plain(dm, dev, arrivals):
@@ -262,6 +270,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--requests", type=int, default=DEFAULT_REQUESTS, help="served trace length")
     ap.add_argument("--rate", type=float, default=DEFAULT_RATE, help="served trace arrival rate (requests/s)")
+    ap.add_argument("--inflight", type=int, default=4, help="server batch slots")
     ap.add_argument("--no-mixed", action="store_true", help="skip the served trace and the 256-file batch")
     ap.add_argument("--only", choices=["roofline", "served"], default=None,
                     help="profiling runs: only the prover runs of the roofline, or only the served trace")
@@ -271,6 +280,8 @@ def main():
     ap.add_argument("--calibrate-after", type=int, default=0,
                     help="after the job, run a device copy of N MiB (HBM-counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    global INFLIGHT
+    INFLIGHT = args.inflight
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (jobs shard across GPUs "
               "with no exchange; see the design notes). Run with --gpus 1.", file=sys.stderr)
@@ -299,7 +310,8 @@ def main():
         run_served(dm, ctx, dev, arrivals)
         _, lat, span = run_served(dm, ctx, dev, arrivals)
         print(json.dumps({"served_mib_s": round(total / span, 1), "p50_ms": round(pct(lat, 0.5) * 1e3, 2),
-                          "p99_ms": round(pct(lat, 0.99) * 1e3, 2), "requests": args.requests, "rate": args.rate}))
+                          "p99_ms": round(pct(lat, 0.99) * 1e3, 2), "requests": args.requests, "rate": args.rate,
+                          "server": LAST_STATS}))
         return
     data = single_file()
     if args.only == "roofline":

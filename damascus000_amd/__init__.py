@@ -87,6 +87,11 @@ class Job(ctypes.Structure):
                 ("status", ctypes.c_int32)]
 
 
+class ServerStats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_uint64), ("jobs", ctypes.c_uint64), ("launch_seconds", ctypes.c_double),
+                ("workspace_creations", ctypes.c_uint64), ("workspace_creation_seconds", ctypes.c_double)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_batch", ctypes.c_uint32),
                 ("max_inflight", ctypes.c_uint32), ("files_on_host", ctypes.c_int32), ("reserved", ctypes.c_int32)]
@@ -133,6 +138,7 @@ _SIGS = {
     "dmx_server_submit": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(ctypes.c_uint64)]),
     "dmx_server_wait": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]),
     "dmx_server_destroy": (None, [_VP]),
+    "dmx_server_stats": (ctypes.c_int, [_VP, _VP]),
     "dmx_device_alloc": (ctypes.c_int, [_VP, _SZ, ctypes.POINTER(_VP)]),
     "dmx_device_free": (ctypes.c_int, [_VP, _VP]),
     "dmx_memcpy_htod": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
@@ -577,6 +583,11 @@ class Server:
         _check(rc)
         j, keep, want = self._live.pop(ticket)
         return _job_result(j, want), lat.value
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().dmx_server_stats(self.h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
 
     def close(self):
         if self.h:

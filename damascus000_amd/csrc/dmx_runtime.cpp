@@ -1754,10 +1754,28 @@ static int batch_collect(BatchJob* J, const JobOut* outs) {
     return first;
 }
 
-// Cache of batch workspaces and their graphs, per (context, depth, n). Idle workspaces beyond
-// the cache budget (DAMASCUS_CACHE_GB, default half the device memory) are freed least
-// recently used first.
+// device memory of one job's block in a batch of depth d
+static size_t job_workspace_bytes(int d) {
+    dmx_prover tp;
+    tp.d = d;
+    tp.N = (size_t)1 << d;
+    tp.hash_cap = max_file_len(d);
+    dmx_verifier tv;
+    tv.rows = tv.glen = tv.clen = tv.c_cap = tp.N;
+    tv.g_cap = tp.N * K * tp.N;
+    Layout L;
+    prover_plan(&tp, L, tp.hash_cap);
+    verifier_plan(&tv, L);
+    return L.stride();
+}
+
+// Cache of batch workspaces and their graphs, per (context, depth, n). The cache holds at most
+// DAMASCUS_CACHE_GB (default 60 % of device memory); idle workspaces are freed, least recently
+// used first, only when a new one would not fit (freeing stalls the device, so never on
+// release).
 static std::mutex g_batches_mu;
+static uint64_t g_batch_creations = 0;
+static double g_batch_create_s = 0;
 static std::vector<std::unique_ptr<BatchJob>> g_batches;
 static uint64_t g_batch_clock = 0;
 
@@ -1770,28 +1788,32 @@ static size_t cache_budget() {
             (void)hipGetLastError();
             return (size_t)32 << 30;
         }
-        return total_b / 2;
+        return total_b / 10 * 6;
     }();
     return half;
 }
 
-static void batch_trim_locked() {
-    size_t idle = 0;
-    for (auto& b : g_batches)
-        if (!b->busy) idle += b->bytes();
-    const size_t budget = cache_budget();
-    while (idle > budget) {
-        size_t victim = SIZE_MAX;
-        for (size_t i = 0; i < g_batches.size(); ++i)
-            if (!g_batches[i]->busy && (victim == SIZE_MAX || g_batches[i]->last_use < g_batches[victim]->last_use))
-                victim = i;
-        if (victim == SIZE_MAX) break;
-        idle -= g_batches[victim]->bytes();
-        g_batches.erase(g_batches.begin() + victim);
-    }
+// device memory held by cached batch workspaces, busy or idle
+static size_t batch_cached_bytes_locked() {
+    size_t t = 0;
+    for (auto& b : g_batches) t += b->bytes();
+    return t;
 }
 
-static int batch_acquire(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
+static bool batch_evict_lru_locked() {
+    size_t victim = SIZE_MAX;
+    for (size_t i = 0; i < g_batches.size(); ++i)
+        if (!g_batches[i]->busy && (victim == SIZE_MAX || g_batches[i]->last_use < g_batches[victim]->last_use))
+            victim = i;
+    if (victim == SIZE_MAX) return false;
+    g_batches.erase(g_batches.begin() + victim);
+    return true;
+}
+
+// An idle cached workspace for (ctx, d, n), else a new one when `may_create` and the cache can
+// hold it within its budget after evicting idle workspaces (least recently used first). Returns
+// DMX_ERR_PENDING when nothing is free: the caller waits for a batch to finish.
+static int batch_try_acquire(dmx_ctx* ctx, int d, uint32_t n, bool may_create, BatchJob** out) {
     {
         std::lock_guard<std::mutex> g(g_batches_mu);
         for (auto& b : g_batches)
@@ -1801,20 +1823,37 @@ static int batch_acquire(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
                 *out = b.get();
                 return DMX_OK;
             }
+        if (!may_create) return DMX_ERR_PENDING;
+        const size_t need = job_workspace_bytes(d) * n, budget = cache_budget();
+        while (batch_cached_bytes_locked() + need > budget)
+            if (!batch_evict_lru_locked()) return DMX_ERR_PENDING;
     }
     BatchJob* J = nullptr;
-    int rc = batch_create(ctx, d, n, &J);
-    if (rc == DMX_ERR_OOM) {  // free idle workspaces and try once more
-        {
-            std::lock_guard<std::mutex> g(g_batches_mu);
-            for (size_t i = 0; i < g_batches.size();)
-                if (!g_batches[i]->busy)
-                    g_batches.erase(g_batches.begin() + i);
-                else
-                    ++i;
-        }
+    int rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
         rc = batch_create(ctx, d, n, &J);
+        if (rc != DMX_ERR_OOM) break;
+        std::lock_guard<std::mutex> g(g_batches_mu);
+        if (!batch_evict_lru_locked()) return DMX_ERR_PENDING;
     }
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(g_batches_mu);
+    g_batch_creations++;
+    g_batch_create_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    J->busy = true;
+    J->last_use = ++g_batch_clock;
+    g_batches.emplace_back(J);
+    *out = J;
+    return DMX_OK;
+}
+
+// the unmanaged callers (one-shot batches): a workspace even beyond the cache budget
+static int batch_acquire(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
+    int rc = batch_try_acquire(ctx, d, n, true, out);
+    if (rc != DMX_ERR_PENDING) return rc;
+    BatchJob* J = nullptr;
+    rc = batch_create(ctx, d, n, &J);
     if (rc) return rc;
     std::lock_guard<std::mutex> g(g_batches_mu);
     J->busy = true;
@@ -1827,7 +1866,6 @@ static int batch_acquire(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
 static void batch_release(BatchJob* J) {
     std::lock_guard<std::mutex> g(g_batches_mu);
     J->busy = false;
-    batch_trim_locked();
 }
 
 static void batch_forget_context(dmx_ctx* ctx) {
@@ -1867,20 +1905,6 @@ extern "C" int dmx_prove_verify_device(dmx_ctx* ctx, const void* d_file, size_t 
     return rc;
 }
 
-// device memory of one job's block in a batch of depth d
-static size_t job_workspace_bytes(int d) {
-    dmx_prover tp;
-    tp.d = d;
-    tp.N = (size_t)1 << d;
-    tp.hash_cap = max_file_len(d);
-    dmx_verifier tv;
-    tv.rows = tv.glen = tv.clen = tv.c_cap = tp.N;
-    tv.g_cap = tp.N * K * tp.N;
-    Layout L;
-    prover_plan(&tp, L, tp.hash_cap);
-    verifier_plan(&tv, L);
-    return L.stride();
-}
 
 // Ragged batch: jobs of any depths. Jobs are grouped by depth into batches of at most
 // `max_batch` (0: sized by workspace, about 16 GiB per batch); every batch is launched before
@@ -2293,6 +2317,8 @@ struct dmx_server {
     std::vector<Flight> flights;
     size_t hbm_used = 0;
     uint64_t next_ticket = 1;
+    uint64_t batches = 0, batched_jobs = 0;
+    double launch_s = 0;  // scheduler time spent acquiring workspaces and launching
     bool stop = false;
     std::thread th;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -2357,11 +2383,26 @@ static void server_loop(dmx_server* S) {
             while (nb * 2 <= pos.size()) nb *= 2;
             const size_t per = job_workspace_bytes(d);
             // one batch's workspace stays within 1/8 of the budget (at least one job), so a few
-            // large batches cannot pin all memory and force workspace churn
+            // large batches cannot pin all memory
             while (nb > 1 && (S->hbm_used + nb * per > S->cfg.hbm_budget_bytes || nb * per > S->cfg.hbm_budget_bytes / 8))
                 nb /= 2;
             if (S->hbm_used + nb * per > S->cfg.hbm_budget_bytes && !S->flights.empty()) break;  // wait for memory
-            dmx_server::Flight f{nullptr, {}, nb * per};
+            // a cached workspace of this size, else a new one if the cache has room, else a
+            // smaller cached one; otherwise wait for a batch to finish
+            lk.unlock();
+            BatchJob* J = nullptr;
+            int arc = batch_try_acquire(S->ctx, d, (uint32_t)nb, false, &J);
+            if (arc == DMX_ERR_PENDING) arc = batch_try_acquire(S->ctx, d, (uint32_t)nb, true, &J);
+            for (size_t m = nb / 2; arc == DMX_ERR_PENDING && m >= 1; m /= 2) {
+                arc = batch_try_acquire(S->ctx, d, (uint32_t)m, false, &J);
+                if (!arc) nb = m;
+            }
+            if (arc == DMX_ERR_PENDING && S->flights.empty()) {  // nothing will free up: create anyway
+                arc = batch_acquire(S->ctx, d, (uint32_t)nb, &J);
+            }
+            lk.lock();
+            if (arc == DMX_ERR_PENDING) break;
+            dmx_server::Flight f{J, {}, nb * per};
             for (size_t k = 0; k < nb; ++k) f.tickets.push_back(S->queue[pos[k]]);
             for (size_t k = nb; k-- > 0;) S->queue.erase(S->queue.begin() + pos[k]);
             std::vector<JobIn> in(nb);
@@ -2371,8 +2412,12 @@ static void server_loop(dmx_server* S) {
             }
             S->hbm_used += f.bytes;
             lk.unlock();
-            int rc = batch_acquire(S->ctx, d, (uint32_t)nb, &f.J);
+            int rc = arc;
+            const auto tl = std::chrono::steady_clock::now();
             if (!rc) rc = batch_launch(f.J, in.data(), S->cfg.files_on_host != 0, true);
+            S->launch_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - tl).count();
+            S->batches++;
+            S->batched_jobs += nb;
             if (rc) {
                 if (f.J) {
                     server_finish(S, f, rc);
@@ -2453,6 +2498,20 @@ extern "C" int dmx_server_wait(dmx_server* S, uint64_t ticket, int64_t timeout_m
     const ServedJob& sj = S->jobs[ticket];
     if (latency_s) *latency_s = sj.t_done - sj.t_submit;
     S->jobs.erase(ticket);
+    return DMX_OK;
+}
+
+extern "C" int dmx_server_stats(dmx_server* S, dmx_server_stats_t* out) {
+    if (!S || !out) return fail(DMX_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(S->mu);
+    out->batches = S->batches;
+    out->jobs = S->batched_jobs;
+    out->launch_seconds = S->launch_s;
+    {
+        std::lock_guard<std::mutex> g2(g_batches_mu);
+        out->workspace_creations = g_batch_creations;
+        out->workspace_creation_seconds = g_batch_create_s;
+    }
     return DMX_OK;
 }
 

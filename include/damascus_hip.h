@@ -222,6 +222,16 @@ int dmx_server_submit(dmx_server *srv, dmx_job *job, uint64_t *ticket);
  * outcome is in job->status; latency_s = submit to verified result) and the ticket is
  * released; DMX_ERR_PENDING when it is still running. */
 int dmx_server_wait(dmx_server *srv, uint64_t ticket, int64_t timeout_ms, double *latency_s);
+/* counters of the scheduler: batches launched and the jobs in them, scheduler seconds spent
+ * launching; workspaces (and graphs) created by the process and the seconds that took */
+typedef struct dmx_server_stats_t {
+    uint64_t batches;
+    uint64_t jobs;
+    double launch_seconds;
+    uint64_t workspace_creations;
+    double workspace_creation_seconds;
+} dmx_server_stats_t;
+int dmx_server_stats(dmx_server *srv, dmx_server_stats_t *out);
 /* finishes every submitted job, then stops the scheduler */
 void dmx_server_destroy(dmx_server *srv);
 
