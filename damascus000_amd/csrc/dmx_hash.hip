@@ -659,6 +659,87 @@ static void launch_b3_root(const uint32_t* cvs, size_t n, const uint8_t* msg, si
 }
 
 // Hash a device buffer: chunk CVs, group merges while more than 1024 values remain, root.
+// A whole challenge in one wave for messages of at most 16 chunks: 16 quads hash the chunks
+// into LDS and merge them to the root, and the wave reduces the digest mod q and inverts it. A
+// zero challenge (transcript.rs:66-75) bumps the counter and hashes the message again. One
+// launch instead of two, no global round trip for the chaining values. (Larger messages need
+// more than one CU to hash their chunks quickly and keep the two-kernel path.)
+constexpr size_t CHALLENGE_MAX_CHUNKS = 16;
+
+__global__ void __launch_bounds__(64) k_b3_challenge(uint8_t* msg, size_t len, KeyW key, uint32_t flags,
+                                                       zq* chal_out, uint32_t* err, size_t js) {
+    __shared__ uint32_t sa[16 * 8], sb[8 * 8 + 8];
+    __shared__ uint32_t dig[8];
+    __shared__ int again;
+    const size_t jo = job_off(js);
+    msg = jp(msg, jo);
+    chal_out = jp(chal_out, jo);
+    err = jp(err, jo);
+    const uint32_t q = threadIdx.x & 3, quad = threadIdx.x >> 2;
+    const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
+    const size_t n = nchunks_of(len);
+    uint32_t counter = 0;  // used by thread 0
+    for (;;) {
+        if (n == 1) {
+            if (threadIdx.x < 4) {
+                uint32_t ca, cb;
+                chunk_cv_quad(msg, len, 0, key.w, flags, true, sel, ca, cb);
+                dig[q] = ca;
+                dig[4 + q] = cb;
+            }
+        } else {
+            for (size_t ci = quad; ci < n; ci += 16) {  // quad-uniform bound
+                uint32_t ca, cb;
+                chunk_cv_quad(msg, len, ci, key.w, flags, false, sel, ca, cb);
+                sa[ci * 8 + q] = ca;
+                sa[ci * 8 + 4 + q] = cb;
+            }
+            __syncthreads();
+            const uint32_t* r = merge_lds(sa, sb, (uint32_t)n, key.w, flags, true);
+            if (threadIdx.x < 8) dig[threadIdx.x] = r[threadIdx.x];
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {  // vector registers, as in k_b3_root
+            uint32_t h[8];
+            for (int j = 0; j < 8; ++j) h[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)dig[j], 0xE4, 0xF, 0xF, false);
+            const zq x = zq_from_digest(h);
+            if (!zq_is_zero(x)) {
+                const zq xi = zq_inv_fermat(x);
+                if (threadIdx.x == 0) {
+                    chal_out[0] = x;
+                    chal_out[1] = xi;
+                }
+            }
+        }
+        if (threadIdx.x == 0) {
+            again = 0;
+            uint32_t h[8];
+            for (int j = 0; j < 8; ++j) h[j] = dig[j];
+            if (zq_is_zero(zq_from_digest(h))) {
+                if (counter >= 64) {
+                    atomicOr(err, 0x80000000u);  // unreachable in practice (2^-112 per try)
+                } else {
+                    counter += 1;
+                    for (uint32_t b = 0; b < 4; ++b) msg[len - 4 + b] = (uint8_t)(counter >> (8 * b));
+                    again = 1;
+                }
+            }
+            __threadfence();
+        }
+        __syncthreads();
+        if (!again) break;  // uniform: every thread reads the same shared flag
+    }
+}
+
+void launch_b3_challenge(uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags, zq* chal_out, uint32_t* err,
+                         hipStream_t s, const Jobs& jb) {
+    KeyW k;
+    for (int i = 0; i < 8; ++i) k.w[i] = key[i];
+    hipLaunchKernelGGL(k_b3_challenge, dim3(1, jb.n), dim3(64), 0, s, msg, len, k, flags, chal_out, err, jb.stride);
+}
+
+bool challenge_fits_one_workgroup(size_t len) { return nchunks_of(len) <= CHALLENGE_MAX_CHUNKS; }
+
 void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags,
                  uint32_t* cv_scratch, size_t cv_scratch_words, uint32_t* digest_out, zq* chal_out,
                  uint32_t* err_flags, hipStream_t s, const Jobs& jb, const uint64_t* dlen) {

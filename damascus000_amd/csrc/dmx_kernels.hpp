@@ -173,6 +173,11 @@ void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t
                  uint32_t* cv_scratch, size_t cv_scratch_words, uint32_t* digest_out, zq* chal_out,
                  uint32_t* err_flags, hipStream_t s, const Jobs& jb = Jobs{}, const uint64_t* dlen = nullptr);
 size_t device_hash_scratch_words(size_t len);
+// a challenge of a transcript message of at most 1024 chunks in one workgroup per job:
+// chal_out[0] = x, chal_out[1] = x^-1 (re-hashing with counter+1 while x = 0)
+void launch_b3_challenge(uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags, zq* chal_out, uint32_t* err,
+                         hipStream_t s, const Jobs& jb = Jobs{});
+bool challenge_fits_one_workgroup(size_t len);
 
 // generator expansion into Z_q rows: g[(i*K + k)*m + c] (src/commitment/sis.rs:212-245)
 // gseed: 8 key words in device memory
