@@ -363,186 +363,9 @@ static void launch_b3_group_merge(const uint32_t* cvs, size_t n, const uint32_t 
                        flags, out, jb.stride, dlen, cap_len);
 }
 
-// modular inverse in Z_q by the binary extended Euclid algorithm (same value as x^(q-2))
-__device__ zq zq_inv_binary(zq a) {
-    const zq qv = zq{{Q0, Q1, Q2, Q3}};
-    zq u = a, v = qv, x1 = zq_from_u64(1), x2 = zq_zero();
-    auto is_one = [](const zq& z) { return z.w[0] == 1 && (z.w[1] | z.w[2] | z.w[3]) == 0; };
-    auto shr1 = [](zq& z) {
-        z.w[0] = (z.w[0] >> 1) | (z.w[1] << 31);
-        z.w[1] = (z.w[1] >> 1) | (z.w[2] << 31);
-        z.w[2] = (z.w[2] >> 1) | (z.w[3] << 31);
-        z.w[3] >>= 1;
-    };
-    auto half = [&](zq& x) {  // x / 2 mod q
-        if (x.w[0] & 1) {
-            // (x + q) / 2 with the 113-bit intermediate
-            uint64_t t = (uint64_t)x.w[0] + Q0;
-            uint32_t r0 = (uint32_t)t;
-            t = (uint64_t)x.w[1] + Q1 + (t >> 32);
-            uint32_t r1 = (uint32_t)t;
-            t = (uint64_t)x.w[2] + Q2 + (t >> 32);
-            uint32_t r2 = (uint32_t)t;
-            uint32_t r3 = x.w[3] + Q3 + (uint32_t)(t >> 32);
-            x = zq{{r0, r1, r2, r3}};
-        }
-        shr1(x);
-    };
-    auto ge = [](const zq& a, const zq& b) {
-        for (int i = 3; i >= 0; --i)
-            if (a.w[i] != b.w[i]) return a.w[i] > b.w[i];
-        return true;
-    };
-    auto sub_raw = [](zq& a, const zq& b) {
-        uint64_t t = (uint64_t)a.w[0] - b.w[0];
-        a.w[0] = (uint32_t)t;
-        uint64_t br = t >> 63;
-        t = (uint64_t)a.w[1] - b.w[1] - br;
-        a.w[1] = (uint32_t)t;
-        br = t >> 63;
-        t = (uint64_t)a.w[2] - b.w[2] - br;
-        a.w[2] = (uint32_t)t;
-        br = t >> 63;
-        a.w[3] = a.w[3] - b.w[3] - (uint32_t)br;
-    };
-    while (!is_one(u) && !is_one(v)) {
-        while (!(u.w[0] & 1)) {
-            shr1(u);
-            half(x1);
-        }
-        while (!(v.w[0] & 1)) {
-            shr1(v);
-            half(x2);
-        }
-        if (ge(u, v)) {
-            sub_raw(u, v);
-            x1 = zq_sub(x1, x2);
-        } else {
-            sub_raw(v, u);
-            x2 = zq_sub(x2, x1);
-        }
-    }
-    return is_one(u) ? x1 : x2;
-}
-
-// Fermat inverse x^(q-2) in 28-bit limbs. q - 2 = 2^112 - 77 has bits 8..111 set and low
-// byte 0xB3, so x^(2^k - 1) for k = 1, 2, 4, ..., 64, 96, 104 by doubling, then the low byte:
-// 111 squarings and 16 multiplications. With 28-bit limbs every product column is a sum of at
-// most four 56-bit terms, so it accumulates in 64 bits with no carry handling.
-struct z28 {
-    uint32_t l[4];  // limbs < 2^28 (value < 2^112, not necessarily reduced)
-};
-constexpr uint32_t M28 = (1u << 28) - 1;
-
-__device__ __forceinline__ z28 z28_from(const zq& x) {
-    z28 r;
-    r.l[0] = x.w[0] & M28;
-    r.l[1] = ((x.w[0] >> 28) | (x.w[1] << 4)) & M28;
-    r.l[2] = ((x.w[1] >> 24) | (x.w[2] << 8)) & M28;
-    r.l[3] = ((x.w[2] >> 20) | (x.w[3] << 12)) & M28;
-    return r;
-}
-
-__device__ __forceinline__ zq z28_to(const z28& a) {  // canonical
-    zq r;
-    r.w[0] = a.l[0] | (a.l[1] << 28);
-    r.w[1] = (a.l[1] >> 4) | (a.l[2] << 24);
-    r.w[2] = (a.l[2] >> 8) | (a.l[3] << 20);
-    r.w[3] = a.l[3] >> 12;
-    return zq_ge_q(r) ? zq_sub_q(r) : r;
-}
-
-__device__ __forceinline__ uint64_t mul32(uint32_t a, uint32_t b) { return (uint64_t)a * b; }
-
-// column sums c0..c6 (each < 2^58) -> 28-bit limbs of an equivalent value < 2^112
-__device__ __forceinline__ z28 z28_reduce(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3, uint64_t c4,
-                                          uint64_t c5, uint64_t c6) {
-    uint32_t r[8];
-    uint64_t t = c0;
-    r[0] = (uint32_t)t & M28;
-    t = c1 + (t >> 28);
-    r[1] = (uint32_t)t & M28;
-    t = c2 + (t >> 28);
-    r[2] = (uint32_t)t & M28;
-    t = c3 + (t >> 28);
-    r[3] = (uint32_t)t & M28;
-    t = c4 + (t >> 28);
-    r[4] = (uint32_t)t & M28;
-    t = c5 + (t >> 28);
-    r[5] = (uint32_t)t & M28;
-    t = c6 + (t >> 28);
-    r[6] = (uint32_t)t & M28;
-    r[7] = (uint32_t)(t >> 28);
-    // 2^112 = 75 (mod q): fold limbs 4..7 onto 0..3, then the overflow above 2^112 twice more
-    t = r[0] + mul32(75, r[4]);
-    uint32_t v0 = (uint32_t)t & M28;
-    t = r[1] + mul32(75, r[5]) + (t >> 28);
-    uint32_t v1 = (uint32_t)t & M28;
-    t = r[2] + mul32(75, r[6]) + (t >> 28);
-    uint32_t v2 = (uint32_t)t & M28;
-    t = r[3] + mul32(75, r[7]) + (t >> 28);
-    uint32_t v3 = (uint32_t)t & M28;
-    uint32_t o = (uint32_t)(t >> 28);
-    for (int k = 0; k < 2; ++k) {
-        uint32_t u = v0 + 75 * o;
-        v0 = u & M28;
-        u = v1 + (u >> 28);
-        v1 = u & M28;
-        u = v2 + (u >> 28);
-        v2 = u & M28;
-        u = v3 + (u >> 28);
-        v3 = u & M28;
-        o = u >> 28;
-    }
-    return z28{{v0, v1, v2, v3}};
-}
-
-__device__ __forceinline__ z28 z28_mul(const z28& a, const z28& b) {
-    const uint32_t *x = a.l, *y = b.l;
-    return z28_reduce(mul32(x[0], y[0]), mul32(x[0], y[1]) + mul32(x[1], y[0]),
-                      mul32(x[0], y[2]) + mul32(x[1], y[1]) + mul32(x[2], y[0]),
-                      mul32(x[0], y[3]) + mul32(x[1], y[2]) + mul32(x[2], y[1]) + mul32(x[3], y[0]),
-                      mul32(x[1], y[3]) + mul32(x[2], y[2]) + mul32(x[3], y[1]),
-                      mul32(x[2], y[3]) + mul32(x[3], y[2]), mul32(x[3], y[3]));
-}
-
-__device__ __forceinline__ z28 z28_sqr(const z28& a) {
-    const uint32_t* x = a.l;
-    const uint32_t d0 = 2 * x[0], d1 = 2 * x[1], d2 = 2 * x[2];
-    return z28_reduce(mul32(x[0], x[0]), mul32(d0, x[1]), mul32(d0, x[2]) + mul32(x[1], x[1]),
-                      mul32(d0, x[3]) + mul32(d1, x[2]), mul32(d1, x[3]) + mul32(x[2], x[2]), mul32(d2, x[3]),
-                      mul32(x[3], x[3]));
-}
-
-__device__ __forceinline__ z28 z28_sqn(z28 a, int n) {
-    for (int i = 0; i < n; ++i) a = z28_sqr(a);
-    return a;
-}
-
-__device__ zq zq_inv_fermat(const zq& xin) {
-    const z28 x = z28_from(xin);
-    const z28 t2 = z28_mul(z28_sqr(x), x);
-    const z28 t4 = z28_mul(z28_sqn(t2, 2), t2);
-    const z28 t8 = z28_mul(z28_sqn(t4, 4), t4);
-    const z28 t16 = z28_mul(z28_sqn(t8, 8), t8);
-    const z28 t32 = z28_mul(z28_sqn(t16, 16), t16);
-    const z28 t64 = z28_mul(z28_sqn(t32, 32), t32);
-    const z28 t96 = z28_mul(z28_sqn(t64, 32), t32);
-    const z28 t104 = z28_mul(z28_sqn(t96, 8), t8);
-    // low byte 0xB3 = 1011 0011, most significant bit first
-    z28 r = z28_mul(z28_sqr(t104), x);  // 1
-    r = z28_sqr(r);                     // 0
-    r = z28_mul(z28_sqr(r), x);         // 1
-    r = z28_mul(z28_sqr(r), x);         // 1
-    r = z28_sqn(r, 2);                  // 0 0
-    r = z28_mul(z28_sqr(r), x);         // 1
-    r = z28_mul(z28_sqr(r), x);         // 1
-    return z28_to(r);
-}
-
 __device__ void write_challenge_pair(zq* chal_out, const zq& x) {
     chal_out[0] = x;
-    chal_out[1] = zq_inv_fermat(x);
+    chal_out[1] = zq_inv_divsteps(x);
 }
 
 // Root merge; one workgroup of 1024 threads per job over n <= 1024 CVs. `chunk_level` says the
@@ -593,16 +416,17 @@ __global__ void __launch_bounds__(1024) k_b3_root(uint32_t* __restrict__ cvs, si
             if (threadIdx.x < 8) dig[threadIdx.x] = r[threadIdx.x];
             __syncthreads();
         }
-        // The challenge and its inverse are computed by all of wave 0 in vector registers
-        // (identical values in every lane); lane 0 stores them. A single-lane block is
-        // compiled to scalar code, with about twice the instructions for the 112-bit products.
+        // The challenge and its inverse are computed by all of wave 0 (identical values in every
+        // lane); lane 0 stores them. The multi-word products of the inverse stay in vector
+        // registers, where a 64-bit multiply-add is one instruction; its divstep loop reads the
+        // low words back into scalar registers.
         if (threadIdx.x < 64 && chal_out) {
             uint32_t h[8];
             // identity quad permutation: same value, but a per-lane register to the compiler
             for (int j = 0; j < 8; ++j) h[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)dig[j], 0xE4, 0xF, 0xF, false);
             const zq x = zq_from_digest(h);
             if (!zq_is_zero(x)) {
-                const zq xi = zq_inv_fermat(x);
+                const zq xi = zq_inv_divsteps(x);
                 if (threadIdx.x == 0) {
                     chal_out[0] = x;
                     chal_out[1] = xi;
@@ -704,7 +528,7 @@ __global__ void __launch_bounds__(64) k_b3_challenge(uint8_t* msg, size_t len, K
             for (int j = 0; j < 8; ++j) h[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)dig[j], 0xE4, 0xF, 0xF, false);
             const zq x = zq_from_digest(h);
             if (!zq_is_zero(x)) {
-                const zq xi = zq_inv_fermat(x);
+                const zq xi = zq_inv_divsteps(x);
                 if (threadIdx.x == 0) {
                     chal_out[0] = x;
                     chal_out[1] = xi;
