@@ -8,6 +8,7 @@
 // final compression.
 #include "dmx_blake3.hpp"
 #include "dmx_kernels.hpp"
+#include "dmx_transcript.hpp"
 
 namespace dmx {
 
@@ -81,139 +82,7 @@ __device__ __forceinline__ void compress_quad(uint32_t& ca, uint32_t& cb, const 
     cb = b ^ d;
 }
 
-// ---------------------------------------------------------------------------------------
-// transcript message layout (transcript.rs:50-61, 77-90):
-//   epoch_seed[32] | u64 len(label) | label | u64 32 | file_id[32] | u64 0 | u64 round |
-//   3 x ( u64 8 | 8 x ( u64 len | len x 16-byte LE coefficients ) ) | u32 counter
-// ---------------------------------------------------------------------------------------
-static inline size_t module_len(uint32_t n) { return 8 + (size_t)K * (8 + 16 * (size_t)n); }
-
-size_t transcript_len(uint32_t label_len, const uint32_t lens[3]) {
-    return 96 + label_len + module_len(lens[0]) + module_len(lens[1]) + module_len(lens[2]) + 4;
-}
-
-__device__ __forceinline__ uint8_t le_byte(uint64_t v, uint32_t i) { return (uint8_t)(v >> (8 * i)); }
-
-__device__ uint8_t module_byte(const zq* m, uint32_t n, size_t o) {
-    if (o < 8) return le_byte(K, (uint32_t)o);
-    o -= 8;
-    const size_t blk = 8 + 16 * (size_t)n;
-    const size_t k = o / blk, r = o - k * blk;
-    if (r < 8) return le_byte(n, (uint32_t)r);
-    const size_t e = r - 8, c = e >> 4;
-    const uint32_t* w = m[k * n + c].w;
-    return (uint8_t)(w[(e >> 2) & 3] >> (8 * (e & 3)));
-}
-
-__device__ uint8_t transcript_byte(const TranscriptMsg& t, size_t o, uint32_t counter) {
-    if (o < 32) return t.epoch_seed[o];
-    o -= 32;
-    if (o < 8) return le_byte(t.label_len, (uint32_t)o);
-    o -= 8;
-    if (o < t.label_len) return t.label[o];
-    o -= t.label_len;
-    if (o < 8) return le_byte(32, (uint32_t)o);
-    o -= 8;
-    if (o < 32) return t.file_id[o];
-    o -= 32;
-    if (o < 8) return 0;  // epoch index
-    o -= 8;
-    if (o < 8) return le_byte(t.round, (uint32_t)o);
-    o -= 8;
-    for (int i = 0; i < 3; ++i) {
-        const size_t ml = 8 + (size_t)K * (8 + 16 * (size_t)t.lens[i]);
-        if (o < ml) return module_byte(t.mods[i], t.lens[i], o);
-        o -= ml;
-    }
-    return le_byte(counter, (uint32_t)o);
-}
-
-// Where message byte o of the header (o < 96 + label_len) comes from: a device byte, or an
-// immediate value.
-__device__ __forceinline__ const uint8_t* header_src(const TranscriptMsg& t, uint32_t o, uint32_t& imm) {
-    imm = 0;
-    if (o < 32) return t.epoch_seed + o;
-    o -= 32;
-    if (o < 8) {
-        imm = le_byte(t.label_len, o);
-        return nullptr;
-    }
-    o -= 8;
-    if (o < t.label_len) {
-        const uint32_t lw = (uint32_t)t.label[0] | ((uint32_t)t.label[1] << 8) | ((uint32_t)t.label[2] << 16) |
-                            ((uint32_t)t.label[3] << 24);
-        imm = (lw >> (8 * o)) & 0xFFu;
-        return nullptr;
-    }
-    o -= t.label_len;
-    if (o < 8) {
-        imm = le_byte(32, o);
-        return nullptr;
-    }
-    o -= 8;
-    if (o < 32) return t.file_id + o;
-    o -= 32;
-    if (o < 8) return nullptr;  // epoch index 0
-    o -= 8;
-    imm = le_byte(t.round, o);
-    return nullptr;
-}
-
-// One thread per coefficient of the three modules: one 16-byte load, 16 byte stores at its
-// message offset (the modules start at an odd offset for a 3-byte label). The thread of
-// coefficient 0 of a coordinate also writes that coordinate's u64 length (and, for coordinate
-// 0, the module's u64 K). Eight more threads write the header (16 bytes each) and the zero
-// counter.
-__global__ void k_transcript_pack(TranscriptMsg t, uint8_t* __restrict__ msg, size_t len, size_t js) {
-    const uint32_t gid = (uint32_t)blockIdx.x * 256 + threadIdx.x;
-    const uint32_t c0 = K * t.lens[0], c1 = c0 + K * t.lens[1], total = c1 + K * t.lens[2];
-    if (gid >= total + 8) return;
-    const size_t jo = job_off(js);
-    msg = jp(msg, jo);
-    const uint32_t H = 96 + t.label_len;
-    if (gid >= total) {  // header slice or counter
-        t.epoch_seed = jp(t.epoch_seed, jo);
-        t.file_id = jp(t.file_id, jo);
-        const uint32_t h0 = (gid - total) * 16;
-        if (h0 < H) {
-            const uint32_t h1 = min(H, h0 + 16);
-            for (uint32_t o = h0; o < h1; ++o) {
-                uint32_t imm;
-                const uint8_t* src = header_src(t, o, imm);
-                msg[o] = src ? *src : (uint8_t)imm;
-            }
-        }
-        if (gid == total + 7)
-            for (uint32_t b = 0; b < 4; ++b) msg[len - 4 + b] = 0;  // counter, zero until a resample
-        return;
-    }
-    const int mi = gid < c0 ? 0 : (gid < c1 ? 1 : 2);
-    const uint32_t rem = gid - (mi == 0 ? 0 : (mi == 1 ? c0 : c1));
-    const uint32_t n = t.lens[mi];
-    const uint32_t k = rem >> __builtin_ctz(n), c = rem & (n - 1);
-    uint32_t mstart = H;
-    for (int i = 0; i < mi; ++i) mstart += 8 + K * (8 + 16 * t.lens[i]);
-    const uint32_t coord = mstart + 8 + k * (8 + 16 * n);  // the coordinate's u64 length
-    const uint4 v = *reinterpret_cast<const uint4*>(jp(t.mods[mi], jo) + rem);
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-    uint8_t* d = msg + coord + 8 + 16 * c;
-#pragma unroll
-    for (int b = 0; b < 16; ++b) d[b] = (uint8_t)(w[b >> 2] >> (8 * (b & 3)));
-    if (c == 0) {
-#pragma unroll
-        for (int b = 0; b < 8; ++b) msg[coord + b] = le_byte(n, b);
-        if (k == 0)
-#pragma unroll
-            for (int b = 0; b < 8; ++b) msg[mstart + b] = le_byte(K, b);
-    }
-}
-
-void launch_transcript_pack(const TranscriptMsg& t, uint8_t* msg, size_t len, hipStream_t s, const Jobs& jb) {
-    const size_t threads = (size_t)K * (t.lens[0] + t.lens[1] + t.lens[2]) + 8;
-    if (jb.n == 0) return;
-    hipLaunchKernelGGL(k_transcript_pack, dim3((unsigned)((threads + 255) / 256), jb.n), dim3(256), 0, s, t, msg,
-                       len, jb.stride);
-}
+size_t transcript_len(uint32_t label_len, const uint32_t lens[3]) { return tx_message_len(label_len, lens); }
 
 // ---------------------------------------------------------------------------------------
 // chunk chaining values
@@ -264,6 +133,87 @@ __device__ void chunk_cv_quad(const uint8_t* msg, size_t len, size_t ci, const u
 }
 
 __host__ __device__ inline size_t nchunks_of(size_t len) { return len == 0 ? 1 : (len + 1023) / 1024; }
+
+// ---------------------------------------------------------------------------------------
+// Transcript messages hashed straight from their sources: a workgroup writes its window of the
+// message (16 chunks, 16 KiB) into LDS as aligned 8-byte words (tx_words4) and its quads
+// compress the chunks from there, so no packed copy goes through memory first.
+// ---------------------------------------------------------------------------------------
+constexpr uint32_t TX_WINDOW = 16 * 1024;  // bytes: 16 chunks, one per quad of the first wave
+
+__device__ __forceinline__ void tx_to_job(TranscriptMsg& t, size_t jo) {
+    t.epoch_seed = jp(t.epoch_seed, jo);
+    t.file_id = jp(t.file_id, jo);
+    for (int i = 0; i < 3; ++i) t.mods[i] = jp(t.mods[i], jo);
+}
+
+// message words [w0, w0 + nw) into win[0 .. nw), nw a multiple of 4, by the whole workgroup
+__device__ void tx_pack_lds(const TranscriptMsg& t, uint32_t w0, uint32_t nw, uint64_t* win) {
+#pragma unroll 2
+    for (uint32_t g = 4 * threadIdx.x; g < nw; g += 4 * blockDim.x) {
+        uint64_t o[4];
+        tx_words4(t, w0 + g, o);
+        *reinterpret_cast<ulonglong2*>(win + g) = make_ulonglong2(o[0], o[1]);
+        *reinterpret_cast<ulonglong2*>(win + g + 2) = make_ulonglong2(o[2], o[3]);
+    }
+}
+
+// chunk_cv_quad over a message window in LDS that starts at message byte `base` and holds every
+// block of the chunk (bytes past `len` read as zero)
+__device__ void chunk_cv_quad_lds(const uint64_t* win, size_t base, size_t len, size_t ci, const uint32_t key[8],
+                                  uint32_t flags, bool root, const QuadSel& sel, uint32_t& ca, uint32_t& cb) {
+    ca = sel(key[0], key[1], key[2], key[3]);
+    cb = sel(key[4], key[5], key[6], key[7]);
+    const size_t start = ci * b3::CHUNK_LEN;
+    const size_t end = (start + b3::CHUNK_LEN < len) ? start + b3::CHUNK_LEN : len;
+    const size_t nb = end > start ? (end - start + 63) / 64 : 1;
+    const uint4* p = reinterpret_cast<const uint4*>(win) + (start - base) / 16;
+    for (size_t b = 0; b < nb; ++b) {
+        uint32_t m[16];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint4 v = p[4 * b + i];
+            m[4 * i] = v.x;
+            m[4 * i + 1] = v.y;
+            m[4 * i + 2] = v.z;
+            m[4 * i + 3] = v.w;
+        }
+        const size_t off = start + 64 * b;
+        const uint32_t blen = (uint32_t)(end - off < 64 ? end - off : 64);
+        uint32_t fl = flags | (b == 0 ? b3::CHUNK_START : 0) | (b + 1 == nb ? b3::CHUNK_END : 0);
+        if (root && b + 1 == nb) fl |= b3::ROOT;
+        compress_quad(ca, cb, m, ci, end > start ? blen : 0, fl, sel);
+    }
+}
+
+// Chunk CVs of a transcript message: 256 threads write the window, its first 16 quads hash the
+// chunks. The chunks holding the counter are also written to `msg`, where a root that meets a
+// zero challenge patches the counter and hashes them again.
+__global__ void __launch_bounds__(256) k_b3_tx_chunks(TranscriptMsg t, size_t len, KeyW key, uint32_t flags,
+                                                     uint32_t* __restrict__ cvs, uint8_t* __restrict__ msg, size_t js) {
+    __shared__ __align__(16) uint64_t win[TX_WINDOW / 8];
+    const size_t jo = job_off(js);
+    tx_to_job(t, jo);
+    cvs = jp(cvs, jo);
+    msg = jp(msg, jo);
+    const size_t n = nchunks_of(len);
+    const size_t base = (size_t)blockIdx.x * TX_WINDOW;
+    const size_t wend = base + TX_WINDOW < len ? base + TX_WINDOW : len;
+    const size_t stop = (wend + 63) & ~(size_t)63;  // through the end of the last block
+    tx_pack_lds(t, (uint32_t)(base / 8), (uint32_t)((stop - base) / 8), win);
+    __syncthreads();
+    const size_t tail = ((len - 4) / b3::CHUNK_LEN) * b3::CHUNK_LEN;
+    const uint8_t* wb = reinterpret_cast<const uint8_t*>(win);
+    for (size_t o = (tail > base ? tail : base) + threadIdx.x; o < wend; o += blockDim.x) msg[o] = wb[o - base];
+    const uint32_t q = threadIdx.x & 3;
+    const size_t ci = (size_t)blockIdx.x * 16 + (threadIdx.x >> 2);
+    if (threadIdx.x >= 64 || ci >= n) return;  // uniform within a quad; no barrier follows
+    const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
+    uint32_t ca, cb;
+    chunk_cv_quad_lds(win, base, len, ci, key.w, flags, false, sel, ca, cb);
+    cvs[ci * 8 + q] = ca;
+    cvs[ci * 8 + 4 + q] = cb;
+}
 
 // a job's own length when lengths live on the device (bounded by the launch capacity)
 __device__ __forceinline__ size_t job_len(size_t cap, const uint64_t* dlen, size_t jo) {
@@ -361,11 +311,6 @@ static void launch_b3_group_merge(const uint32_t* cvs, size_t n, const uint32_t 
     for (int i = 0; i < 8; ++i) k.w[i] = key[i];
     hipLaunchKernelGGL(k_b3_group_merge, dim3((unsigned)((n + 1023) / 1024), jb.n), dim3(1024), 0, s, cvs, n, k,
                        flags, out, jb.stride, dlen, cap_len);
-}
-
-__device__ void write_challenge_pair(zq* chal_out, const zq& x) {
-    chal_out[0] = x;
-    chal_out[1] = zq_inv_divsteps(x);
 }
 
 // Root merge; one workgroup of 1024 threads per job over n <= 1024 CVs. `chunk_level` says the
@@ -483,22 +428,25 @@ static void launch_b3_root(const uint32_t* cvs, size_t n, const uint8_t* msg, si
 }
 
 // Hash a device buffer: chunk CVs, group merges while more than 1024 values remain, root.
-// A whole challenge in one wave for messages of at most 16 chunks: 16 quads hash the chunks
-// into LDS and merge them to the root, and the wave reduces the digest mod q and inverts it. A
-// zero challenge (transcript.rs:66-75) bumps the counter and hashes the message again. One
-// launch instead of two, no global round trip for the chaining values. (Larger messages need
-// more than one CU to hash their chunks quickly and keep the two-kernel path.)
+// A whole challenge in one workgroup for messages of at most 16 chunks: 256 threads write the
+// message into LDS, the 16 quads of wave 0 hash the chunks, the workgroup merges them to the
+// root, and wave 0 reduces the digest mod q and inverts it. A zero challenge (transcript.rs:66-75) bumps the counter in LDS
+// and hashes the message again. (Larger messages need more than one CU to hash their chunks
+// quickly and take the chunk and root kernels.)
 constexpr size_t CHALLENGE_MAX_CHUNKS = 16;
 
-__global__ void __launch_bounds__(64) k_b3_challenge(uint8_t* msg, size_t len, KeyW key, uint32_t flags,
-                                                       zq* chal_out, uint32_t* err, size_t js) {
+__global__ void __launch_bounds__(256) k_b3_tx_challenge(TranscriptMsg t, size_t len, KeyW key, uint32_t flags,
+                                                        zq* chal_out, uint32_t* err, size_t js) {
+    __shared__ __align__(16) uint64_t win[TX_WINDOW / 8];
     __shared__ uint32_t sa[16 * 8], sb[8 * 8 + 8];
     __shared__ uint32_t dig[8];
     __shared__ int again;
     const size_t jo = job_off(js);
-    msg = jp(msg, jo);
+    tx_to_job(t, jo);
     chal_out = jp(chal_out, jo);
     err = jp(err, jo);
+    tx_pack_lds(t, 0, (uint32_t)(((len + 63) & ~(size_t)63) / 8), win);
+    __syncthreads();
     const uint32_t q = threadIdx.x & 3, quad = threadIdx.x >> 2;
     const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
     const size_t n = nchunks_of(len);
@@ -507,23 +455,23 @@ __global__ void __launch_bounds__(64) k_b3_challenge(uint8_t* msg, size_t len, K
         if (n == 1) {
             if (threadIdx.x < 4) {
                 uint32_t ca, cb;
-                chunk_cv_quad(msg, len, 0, key.w, flags, true, sel, ca, cb);
+                chunk_cv_quad_lds(win, 0, len, 0, key.w, flags, true, sel, ca, cb);
                 dig[q] = ca;
                 dig[4 + q] = cb;
             }
         } else {
-            for (size_t ci = quad; ci < n; ci += 16) {  // quad-uniform bound
+            if (threadIdx.x < 64 && quad < n) {
                 uint32_t ca, cb;
-                chunk_cv_quad(msg, len, ci, key.w, flags, false, sel, ca, cb);
-                sa[ci * 8 + q] = ca;
-                sa[ci * 8 + 4 + q] = cb;
+                chunk_cv_quad_lds(win, 0, len, quad, key.w, flags, false, sel, ca, cb);
+                sa[quad * 8 + q] = ca;
+                sa[quad * 8 + 4 + q] = cb;
             }
             __syncthreads();
             const uint32_t* r = merge_lds(sa, sb, (uint32_t)n, key.w, flags, true);
             if (threadIdx.x < 8) dig[threadIdx.x] = r[threadIdx.x];
         }
         __syncthreads();
-        if (threadIdx.x < 64) {  // vector registers, as in k_b3_root
+        if (threadIdx.x < 64) {  // every lane of wave 0 holds the digest; see k_b3_root
             uint32_t h[8];
             for (int j = 0; j < 8; ++j) h[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)dig[j], 0xE4, 0xF, 0xF, false);
             const zq x = zq_from_digest(h);
@@ -544,25 +492,43 @@ __global__ void __launch_bounds__(64) k_b3_challenge(uint8_t* msg, size_t len, K
                     atomicOr(err, 0x80000000u);  // unreachable in practice (2^-112 per try)
                 } else {
                     counter += 1;
-                    for (uint32_t b = 0; b < 4; ++b) msg[len - 4 + b] = (uint8_t)(counter >> (8 * b));
+                    uint8_t* wb = reinterpret_cast<uint8_t*>(win);
+                    for (uint32_t b = 0; b < 4; ++b) wb[len - 4 + b] = (uint8_t)(counter >> (8 * b));
                     again = 1;
                 }
             }
-            __threadfence();
         }
         __syncthreads();
         if (!again) break;  // uniform: every thread reads the same shared flag
     }
 }
 
-void launch_b3_challenge(uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags, zq* chal_out, uint32_t* err,
-                         hipStream_t s, const Jobs& jb) {
+static void launch_b3_tx_chunks(const TranscriptMsg& t, size_t len, const uint32_t key[8], uint32_t flags,
+                                uint32_t* cvs, uint8_t* msg, hipStream_t s, const Jobs& jb) {
     KeyW k;
     for (int i = 0; i < 8; ++i) k.w[i] = key[i];
-    hipLaunchKernelGGL(k_b3_challenge, dim3(1, jb.n), dim3(64), 0, s, msg, len, k, flags, chal_out, err, jb.stride);
+    hipLaunchKernelGGL(k_b3_tx_chunks, dim3((unsigned)((len + TX_WINDOW - 1) / TX_WINDOW), jb.n), dim3(256), 0, s, t,
+                       len, k, flags, cvs, msg, jb.stride);
 }
 
-bool challenge_fits_one_workgroup(size_t len) { return nchunks_of(len) <= CHALLENGE_MAX_CHUNKS; }
+static void hash_tree(uint32_t* cv_scratch, size_t n, const uint8_t* msg, size_t len, const uint32_t key[8],
+                      uint32_t flags, uint32_t* digest_out, zq* chal_out, uint32_t* err_flags, hipStream_t s,
+                      const Jobs& jb);
+
+void transcript_challenge(const TranscriptMsg& t, const uint32_t key[8], uint32_t flags, uint8_t* msg,
+                          uint32_t* cv_scratch, zq* chal_out, uint32_t* err_flags, hipStream_t s, const Jobs& jb) {
+    if (jb.n == 0) return;
+    const size_t len = tx_message_len(t.label_len, t.lens);
+    if (nchunks_of(len) <= CHALLENGE_MAX_CHUNKS) {
+        KeyW k;
+        for (int i = 0; i < 8; ++i) k.w[i] = key[i];
+        hipLaunchKernelGGL(k_b3_tx_challenge, dim3(1, jb.n), dim3(256), 0, s, t, len, k, flags, chal_out, err_flags,
+                           jb.stride);
+        return;
+    }
+    launch_b3_tx_chunks(t, len, key, flags, cv_scratch, msg, s, jb);
+    hash_tree(cv_scratch, nchunks_of(len), msg, len, key, flags, nullptr, chal_out, err_flags, s, jb);
+}
 
 void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags,
                  uint32_t* cv_scratch, size_t cv_scratch_words, uint32_t* digest_out, zq* chal_out,
@@ -576,6 +542,13 @@ void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t
         launch_b3_root(cv_scratch, n, msg, len, key, flags, digest_out, chal_out, err_flags, 1, s, jb, dlen, n * 8);
         return;
     }
+    hash_tree(cv_scratch, n, msg, len, key, flags, digest_out, chal_out, err_flags, s, jb);
+}
+
+// group merges while more than 1024 values remain, then the root
+static void hash_tree(uint32_t* cv_scratch, size_t n, const uint8_t* msg, size_t len, const uint32_t key[8],
+                      uint32_t flags, uint32_t* digest_out, zq* chal_out, uint32_t* err_flags, hipStream_t s,
+                      const Jobs& jb) {
     uint32_t* cur = cv_scratch;
     size_t cnt = n;
     uint32_t* next = cv_scratch + n * 8;

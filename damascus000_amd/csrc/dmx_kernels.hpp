@@ -162,8 +162,13 @@ struct TranscriptMsg {
     uint32_t lens[3];
 };
 size_t transcript_len(uint32_t label_len, const uint32_t lens[3]);
-void launch_transcript_pack(const TranscriptMsg& t, uint8_t* msg, size_t len, hipStream_t s,
-                            const Jobs& jb = Jobs{});
+// Challenge of a transcript message (transcript.rs:42-75), hashed from its sources:
+// chal_out[0] = x = LE256(BLAKE3(message)) mod q, re-hashed with counter+1 while zero, and
+// chal_out[1] = x^-1, for every job. `msg` (transcript_len bytes) receives the chunks that hold
+// the counter; cv_scratch holds device_hash_scratch_words(transcript_len) words.
+void transcript_challenge(const TranscriptMsg& t, const uint32_t key[8], uint32_t flags, uint8_t* msg,
+                          uint32_t* cv_scratch, zq* chal_out, uint32_t* err_flags, hipStream_t s,
+                          const Jobs& jb = Jobs{});
 // Whole-buffer BLAKE3 (chunks, group merges, root). cv_scratch holds device_hash_scratch_words.
 // digest_out (8 words) when non-null. For a challenge (chal_out != nullptr): LE256(digest) mod q,
 // re-hashed with counter+1 while zero, then chal_out[0] = x, chal_out[1] = x^-1.
@@ -173,11 +178,6 @@ void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t
                  uint32_t* cv_scratch, size_t cv_scratch_words, uint32_t* digest_out, zq* chal_out,
                  uint32_t* err_flags, hipStream_t s, const Jobs& jb = Jobs{}, const uint64_t* dlen = nullptr);
 size_t device_hash_scratch_words(size_t len);
-// a challenge of a transcript message of at most 1024 chunks in one workgroup per job:
-// chal_out[0] = x, chal_out[1] = x^-1 (re-hashing with counter+1 while x = 0)
-void launch_b3_challenge(uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags, zq* chal_out, uint32_t* err,
-                         hipStream_t s, const Jobs& jb = Jobs{});
-bool challenge_fits_one_workgroup(size_t len);
 
 // generator expansion into Z_q rows: g[(i*K + k)*m + c] (src/commitment/sis.rs:212-245)
 // gseed: 8 key words in device memory

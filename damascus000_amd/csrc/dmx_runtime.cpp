@@ -646,12 +646,7 @@ struct Transcriber {
         t.lens[0] = (uint32_t)lc;
         t.lens[1] = (uint32_t)ll;
         t.lens[2] = (uint32_t)lr;
-        const size_t len = transcript_len(t.label_len, t.lens);
-        launch_transcript_pack(t, msg.as<uint8_t>(), len, s, jb);
-        if (challenge_fits_one_workgroup(len))
-            launch_b3_challenge(msg.as<uint8_t>(), len, IVW, 0, out, flags, s, jb);
-        else
-            device_hash(msg.as<uint8_t>(), len, IVW, 0, cvs.as<uint32_t>(), cv_words, nullptr, out, flags, s, jb);
+        transcript_challenge(t, IVW, 0, msg.as<uint8_t>(), cvs.as<uint32_t>(), out, flags, s, jb);
     }
 };
 

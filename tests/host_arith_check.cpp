@@ -1,14 +1,21 @@
-// Host build of the device inverse in damascus000_amd/csrc/dmx_arith.hpp, for the CPU tests.
+// Host builds of device routines shared through the headers in damascus000_amd/csrc, for the
+// CPU tests.
 //   host_arith_check inv            reads canonical Z_q values (hex, one per line) and prints
 //                                   zq_inv_divsteps of each
 //   host_arith_check random N SEED  compares zq_inv_divsteps with x^(q-2) (zq_inv) and checks
 //                                   x * x^-1 = 1 on N seeded inputs plus edge cases
+//   host_arith_check tx             reads "label round epoch_hex file_id_hex n0 n1 n2", then the
+//                                   K (n0 + n1 + n2) coefficients (hex, one per line), and prints
+//                                   the challenge message assembled from tx_words4 (hex)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <random>
+#include <string>
+#include <vector>
 
 #include "dmx_arith.hpp"
+#include "dmx_transcript.hpp"
 
 using namespace dmx;
 
@@ -36,7 +43,60 @@ static bool parse(const char* s, zq& out) {
     return !zq_ge_q(out);
 }
 
+static bool hex_bytes(const char* s, uint8_t* out, size_t n) {
+    if (strlen(s) != 2 * n) return false;
+    for (size_t i = 0; i < n; ++i) {
+        unsigned v;
+        if (sscanf(s + 2 * i, "%2x", &v) != 1) return false;
+        out[i] = (uint8_t)v;
+    }
+    return true;
+}
+
+static int run_tx() {
+    char label[8], ep_hex[80], fid_hex[80];
+    unsigned long long rnd;
+    unsigned n[3];
+    if (scanf("%7s %llu %79s %79s %u %u %u", label, &rnd, ep_hex, fid_hex, &n[0], &n[1], &n[2]) != 7) return 2;
+    alignas(8) uint8_t ep[32];
+    alignas(8) uint8_t fid[32];
+    if (!hex_bytes(ep_hex, ep, 32) || !hex_bytes(fid_hex, fid, 32)) return 2;
+    std::vector<zq> mods[3];
+    char line[128];
+    for (int i = 0; i < 3; ++i) {
+        mods[i].resize((size_t)K * n[i]);
+        for (auto& c : mods[i]) {
+            if (scanf("%127s", line) != 1 || !parse(line, c)) return 2;
+        }
+    }
+    TranscriptMsg t;
+    t.epoch_seed = ep;
+    t.file_id = fid;
+    t.label_len = (uint32_t)strlen(label);
+    memset(t.label, 0, 4);
+    memcpy(t.label, label, t.label_len);
+    t.round = rnd;
+    for (int i = 0; i < 3; ++i) {
+        t.mods[i] = mods[i].data();
+        t.lens[i] = n[i];
+    }
+    const size_t len = tx_message_len(t.label_len, t.lens);
+    std::vector<uint64_t> words((len + 31) / 32 * 4);
+    for (size_t w = 0; w < words.size(); w += 4) tx_words4(t, (uint32_t)w, &words[w]);
+    std::string hex;
+    char b[3];
+    for (size_t o = 0; o < len; ++o) {
+        snprintf(b, sizeof b, "%02x", (unsigned)((words[o >> 3] >> (8 * (o & 7))) & 0xFF));
+        hex += b;
+    }
+    for (size_t o = len; o < 8 * words.size(); ++o)  // past the message every byte reads zero
+        if ((words[o >> 3] >> (8 * (o & 7))) & 0xFF) return 3;
+    printf("%s\n", hex.c_str());
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 2 && !strcmp(argv[1], "tx")) return run_tx();
     if (argc >= 2 && !strcmp(argv[1], "inv")) {
         char line[128];
         while (fgets(line, sizeof line, stdin)) {

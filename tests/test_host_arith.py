@@ -52,3 +52,35 @@ def test_inverse_random_against_fermat(checker):
     r = subprocess.run([checker, "random", "200000", "20261203"], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "mismatches 0" in r.stdout
+
+
+def _tx_message(exe, label, rnd, epoch, file_id, mods):
+    ns = [len(m) // 8 for m in mods]
+    inp = f"{label} {rnd} {epoch.hex()} {file_id.hex()} {ns[0]} {ns[1]} {ns[2]}\n"
+    inp += "".join(f"{c:x}\n" for m in mods for c in m)
+    r = subprocess.run([exe, "tx"], input=inp, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return bytes.fromhex(r.stdout.strip())
+
+
+@pytest.mark.parametrize("label,rnd,ns", [("vec", 0, (1, 1, 1)), ("poly", 0, (2, 1, 1)), ("vec", 3, (4, 4, 4)),
+                                          ("poly", 5, (32, 16, 16)), ("vec", 2**40 + 7, (64, 64, 64)),
+                                          ("poly", 1, (8, 64, 2))])
+def test_transcript_words_hash_to_the_oracle_challenge(checker, label, rnd, ns):
+    """The hash kernels build the challenge message (transcript.rs:50-61, 77-90) from aligned
+    8-byte words (dmx_transcript.hpp, tx_words4). BLAKE3 of the assembled bytes, reduced mod q,
+    must equal the oracle's challenge, which absorbs the same fields field by field."""
+    import numpy as np
+
+    import _oracle as O
+
+    rng = np.random.default_rng(sum(ns) + rnd % 1000)
+    epoch = bytes(rng.integers(0, 256, 32, dtype=np.uint8))
+    file_id = bytes(rng.integers(0, 256, 32, dtype=np.uint8))
+    mods = [[int.from_bytes(rng.bytes(14), "little") % Q for _ in range(8 * n)] for n in ns]
+    msg = _tx_message(checker, label, rnd, epoch, file_id, mods)
+    assert len(msg) == 96 + len(label) + sum(8 + 8 * (8 + 16 * n) for n in ns) + 4
+    assert msg[:32] == epoch and msg[-4:] == b"\0\0\0\0"
+    x = O.from_le_bytes_mod_order(O.blake3(msg))
+    arrs = [O.to_words(m).reshape(8, n, 2) for m, n in zip(mods, ns)]
+    assert x == O.challenge(epoch, label, file_id, rnd, *arrs)
