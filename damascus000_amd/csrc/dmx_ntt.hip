@@ -723,9 +723,11 @@ void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, i
 // ---------------------------------------------------------------------------------------
 // MAC passes with their reductions and inverse transforms fused: the segment partials of one
 // (k, p) slice are summed into LDS, transformed back there and scaled by 1/m. One workgroup
-// per (k, p); degrees 1 and 2 keep the scalar kernels and separate inverse launches.
+// per (k, p); degrees 1 and 2 keep the scalar kernels and separate inverse launches. With only
+// 64 workgroups per job, each one has 16 waves with their segment loads in flight together, so
+// the reads of the partials are not bound by one wave's memory latency.
 // ---------------------------------------------------------------------------------------
-constexpr int FIN_THREADS = 256;
+constexpr int FIN_THREADS = 1024;
 
 // Gentleman-Sande inverse layers on `cnt` adjacent slices of 2^log_n words in LDS (no scaling)
 __device__ __forceinline__ void lds_intt_slices(uint32_t* s, int cnt, int log_n, int pi, const NttTables& tw) {
@@ -765,6 +767,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_mac_pairs_finish(const uint32_t
     for (int e = threadIdx.x; e < nslots * m; e += FIN_THREADS) {
         const int sl = e >> log_m, c = e & (m - 1);
         uint64_t acc = 0;
+#pragma unroll 8
         for (uint32_t sg = 0; sg < nseg; ++sg) acc += part[(size_t)sg * 3 * slot + sl * slot + (size_t)kp * m + c];
         sh[e] = reduce_u96(acc, 0, pc);
     }
@@ -800,6 +803,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_mac_split_finish(const uint32_t
     const uint32_t i4m = tw.invn_m[p * (MAX_LOG_N + 1) + 2], i4mp = tw.invn_mp[p * (MAX_LOG_N + 1) + 2];
     for (int t = threadIdx.x; t < h; t += FIN_THREADS) {
         uint64_t a[4] = {0, 0, 0, 0};
+#pragma unroll 4
         for (uint32_t sg = 0; sg < nseg; ++sg)
             for (int j = 0; j < 4; ++j) a[j] += part[(size_t)sg * 4 * slot + j * slot + (size_t)kp * h + t];
         const uint32_t a00 = reduce_u96(a[0], 0, pc), a11 = reduce_u96(a[1], 0, pc),
