@@ -108,6 +108,13 @@ void launch_mac_split_inv(const uint32_t* What, const uint32_t* Ghat, size_t row
 size_t mac_part_words(int max_log_m);
 // Z_q rows only (no transform): dst[row][c] = row_value(src, row, c)
 void launch_fold_rows_zq(const RowSrc& src, size_t rows, int log_m, hipStream_t s, const Jobs& jb = Jobs{});
+// The verifier's terminal generator g_d[0] (K values) from the initial matrix G (rows x K x n,
+// shared by every job when g_shared) and the per-round (x, x^-1, y, y^-1) in chal, as one
+// weighted sum (vec_mask / poly_mask: rounds whose vector / poly stage ran). w (rows), u (n) and
+// S (rows * K) are scratch.
+void launch_terminal_generator(const zq* G, bool g_shared, const zq* chal, uint32_t rounds, uint32_t vec_mask,
+                               uint32_t poly_mask, int log_rows, int log_n, zq* w, zq* u, zq* S, zq* out,
+                               hipStream_t s, const Jobs& jb = Jobs{});
 void launch_final_check(const zq* g, const zq* c, const zq* opening, size_t n, uint32_t* flags,
                         uint32_t bit, hipStream_t s, const Jobs& jb = Jobs{});
 // residues [slice][8][m] -> Z_q [slice][m]

@@ -800,14 +800,18 @@ __global__ void __launch_bounds__(FIN_THREADS) k_mac_split_finish(const uint32_t
     uint32_t* si = sh;
     uint32_t* sl = sh + m;  // L then R, adjacent
     uint32_t* sr = sl + h;
+    uint32_t* sums = sh + 2 * m;  // S00, S11, S01, S10, h words each
     const uint32_t i4m = tw.invn_m[p * (MAX_LOG_N + 1) + 2], i4mp = tw.invn_mp[p * (MAX_LOG_N + 1) + 2];
+    for (int e = threadIdx.x; e < 4 * h; e += FIN_THREADS) {  // one (sum, pair) per thread
+        const int j = e >> (log_m - 1), t = e & (h - 1);
+        uint64_t a = 0;
+#pragma unroll 8
+        for (uint32_t sg = 0; sg < nseg; ++sg) a += part[(size_t)sg * 4 * slot + j * slot + (size_t)kp * h + t];
+        sums[e] = reduce_u96(a, 0, pc);
+    }
+    __syncthreads();
     for (int t = threadIdx.x; t < h; t += FIN_THREADS) {
-        uint64_t a[4] = {0, 0, 0, 0};
-#pragma unroll 4
-        for (uint32_t sg = 0; sg < nseg; ++sg)
-            for (int j = 0; j < 4; ++j) a[j] += part[(size_t)sg * 4 * slot + j * slot + (size_t)kp * h + t];
-        const uint32_t a00 = reduce_u96(a[0], 0, pc), a11 = reduce_u96(a[1], 0, pc),
-                       a01 = reduce_u96(a[2], 0, pc), a10 = reduce_u96(a[3], 0, pc);
+        const uint32_t a00 = sums[t], a11 = sums[h + t], a01 = sums[2 * h + t], a10 = sums[3 * h + t];
         si[2 * t] = a00;
         si[2 * t + 1] = a11;
         const uint32_t d = subp(a00, a11, P);
@@ -874,7 +878,7 @@ void launch_mac_split_inv(const uint32_t* What, const uint32_t* Ghat, size_t row
     hipLaunchKernelGGL(k_mac_split4, dim3((unsigned)((threads + MAC_THREADS - 1) / MAC_THREADS), jb.n),
                        dim3(MAC_THREADS), 0, s, What, Ghat, (uint32_t)rows, log_m, nseg, seg_rows, part, jb.stride,
                        jb.stride);
-    hipLaunchKernelGGL(k_mac_split_finish, dim3(K * NPRIMES, jb.n), dim3(FIN_THREADS), (size_t)2 * m * 4, s, part,
+    hipLaunchKernelGGL(k_mac_split_finish, dim3(K * NPRIMES, jb.n), dim3(FIN_THREADS), (size_t)4 * m * 4, s, part,
                        nseg, log_m, out_inv, out_lr, tw, jb.stride);
 }
 
@@ -1025,6 +1029,105 @@ void launch_fold_rows_zq(const RowSrc& src, size_t rows, int log_m, hipStream_t 
     if (!total || jb.n == 0) return;
     hipLaunchKernelGGL(k_fold_rows_zq, dim3((unsigned)((total + 255) / 256), jb.n), dim3(256), 0, s, src, total,
                        log_m, jb.stride);
+}
+
+// ---------------------------------------------------------------------------------------
+// The verifier's terminal generator (verifier.rs:57-134) without folding the matrix round by
+// round. Both folds are linear. The vector fold (verifier.rs:141-151) sends row i to row
+// i mod (rows/2), scaled by x^-1 when i is in the right half. The poly fold g_e + y^-1 X g_o in
+// Z_q[X]/(X^L + 1) (verifier.rs:153-179) sends coefficient c of a degree-2L element to c/2 when
+// c is even, to (c+1)/2 scaled by y^-1 when c is odd and below 2L-1, and to 0 scaled by -y^-1
+// when c = 2L-1. So after the last round
+//   g_d[0]_k = sum_i w_i sum_c u_c g[i][k][c],
+// w_i the product of x_t^-1 over the vector rounds t that found row i in the right half (round
+// t reads bit log2(rows) - 1 - t of i), u_c the product of the poly rounds' factors along c's
+// path. The matrix is read once, with no folded copies written.
+// chal holds (x, x^-1, y, y^-1) per round.
+// ---------------------------------------------------------------------------------------
+__global__ void k_fold_weights(const zq* chal, uint32_t rounds, uint32_t vec_mask, uint32_t poly_mask, int log_rows,
+                               int log_n, zq* w, zq* u, size_t js) {
+    const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t rows = 1u << log_rows, n = 1u << log_n;
+    if (gid >= rows && gid >= n) return;
+    const size_t jo = job_off(js);
+    chal = jp(chal, jo);
+    if (gid < rows) {
+        zq acc = zq_from_u64(1);
+        int bit = log_rows - 1;
+        for (uint32_t r = 0; r < rounds; ++r) {
+            if (!((vec_mask >> r) & 1)) continue;
+            if ((gid >> bit) & 1) acc = zq_mul(acc, load_zq(chal + 4 * r + 1));
+            --bit;
+        }
+        store_zq(jp(w, jo) + gid, acc);
+    }
+    if (gid < n) {
+        zq acc = zq_from_u64(1);
+        uint32_t c = gid, len = n;
+        for (uint32_t r = 0; r < rounds; ++r) {
+            if (!((poly_mask >> r) & 1)) continue;
+            if (c & 1) {
+                const zq yi = load_zq(chal + 4 * r + 3);
+                const bool wrap = c == len - 1;
+                acc = zq_mul(acc, wrap ? zq_neg(yi) : yi);
+                c = wrap ? 0 : (c + 1) >> 1;
+            } else {
+                c >>= 1;
+            }
+            len >>= 1;
+        }
+        store_zq(jp(u, jo) + gid, acc);
+    }
+}
+
+// S[r] = w[r / K] * sum_c u[c] G[r][c] for generator row r = i*K + k; one workgroup per row
+__global__ void __launch_bounds__(256) k_fold_rowsums(const zq* __restrict__ G, const zq* __restrict__ w,
+                                                      const zq* __restrict__ u, int log_n, zq* __restrict__ S,
+                                                      size_t js, size_t gjs) {
+    __shared__ zq red[256];
+    const size_t jo = job_off(js);
+    const size_t r = blockIdx.x;
+    const zq* g = jp(G, (size_t)blockIdx.y * gjs) + (r << log_n);
+    u = jp(u, jo);
+    zq acc = zq_zero();
+    for (uint32_t c = threadIdx.x; c < (1u << log_n); c += 256) acc = zq_add(acc, zq_mul(load_zq(u + c), load_zq(g + c)));
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] = zq_add(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) store_zq(jp(S, jo) + r, zq_mul(red[0], load_zq(jp(w, jo) + r / K)));
+}
+
+// out[k] = sum_i S[i*K + k]; one workgroup per k
+__global__ void __launch_bounds__(256) k_fold_combine(const zq* __restrict__ S, uint32_t rows, zq* __restrict__ out,
+                                                      size_t js) {
+    __shared__ zq red[256];
+    const size_t jo = job_off(js);
+    S = jp(S, jo);
+    const uint32_t k = blockIdx.x;
+    zq acc = zq_zero();
+    for (uint32_t i = threadIdx.x; i < rows; i += 256) acc = zq_add(acc, load_zq(S + (size_t)i * K + k));
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] = zq_add(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) store_zq(jp(out, jo) + k, red[0]);
+}
+
+void launch_terminal_generator(const zq* G, bool g_shared, const zq* chal, uint32_t rounds, uint32_t vec_mask,
+                               uint32_t poly_mask, int log_rows, int log_n, zq* w, zq* u, zq* S, zq* out,
+                               hipStream_t s, const Jobs& jb) {
+    if (jb.n == 0) return;
+    const uint32_t span = 1u << (log_rows > log_n ? log_rows : log_n);
+    hipLaunchKernelGGL(k_fold_weights, dim3((span + 255) / 256, jb.n), dim3(256), 0, s, chal, rounds, vec_mask,
+                       poly_mask, log_rows, log_n, w, u, jb.stride);
+    hipLaunchKernelGGL(k_fold_rowsums, dim3((unsigned)(((size_t)K) << log_rows), jb.n), dim3(256), 0, s, G, w, u,
+                       log_n, S, jb.stride, g_shared ? (size_t)0 : jb.stride);
+    hipLaunchKernelGGL(k_fold_combine, dim3(K, jb.n), dim3(256), 0, s, S, 1u << log_rows, out, jb.stride);
 }
 
 // final opening (verifier.rs:122-134): flags |= bit when g[i] * opening != c[i]

@@ -1050,30 +1050,36 @@ struct dmx_verifier {
     hipStream_t s = nullptr;
     Jobs jb;
     size_t rounds = 0, round = 0;
-    size_t rows = 0, glen = 0, clen = 0;
+    size_t rows = 0, glen = 0, clen = 0;  // current dimensions
+    size_t rows0 = 0, n0 = 0;             // the initial generator matrix: rows0 x K x n0
     size_t g_cap = 0, c_cap = 0;
+    uint32_t vec_mask = 0, poly_mask = 0;  // rounds whose vector / poly stage ran
     DevBuf store;  // the workspace, unless a batch owns it
-    Slice meta, G0, G1, C, Ct, rec, scal, flags, opening;
-    int gcur = 0;  // which of G0/G1 holds the state
+    // G0: the generator matrix, never folded (the terminal generator is one weighted sum over
+    // it); chal: (x, x^-1, y, y^-1) per round; wts, uts, sums, gfin: the weighted sum's scratch
+    Slice meta, G0, C, Ct, rec, chal, wts, uts, sums, gfin, flags, opening;
     Transcriber tr;
     std::vector<JobMeta> stage;
     const zq* G0_src = nullptr;  // resident generator matrix shared by every job (see dmx_prover)
     ~dmx_verifier() {
         if (s) (void)hipStreamDestroy(s);
     }
-    zq* G() const { return (gcur ? G1 : G0).as<zq>(); }
-    zq* Gother() const { return (gcur ? G0 : G1).as<zq>(); }
+    const zq* G() const { return G0_src ? G0_src : G0.as<zq>(); }
+    zq* round_chal(size_t r) const { return chal.as<zq>() + 4 * r; }
     const uint32_t* gseed() const { return reinterpret_cast<const uint32_t*>(meta.as<uint8_t>() + META_GSEED); }
 };
 
 static void verifier_plan(dmx_verifier* v, Layout& L) {
     L.add(v->meta, sizeof(JobMeta));
     L.add(v->G0, v->g_cap * 16);
-    L.add(v->G1, std::max<size_t>(v->g_cap / 2, 1) * 16);
     L.add(v->C, (size_t)K * v->c_cap * 16);
     L.add(v->Ct, (size_t)K * v->c_cap * 16);
     L.add(v->rec, (size_t)4 * K * v->c_cap * 16);
-    L.add(v->scal, 4 * 16);
+    L.add(v->chal, std::max<size_t>(v->rounds, 1) * 4 * 16);
+    L.add(v->wts, v->rows0 * 16);
+    L.add(v->uts, v->n0 * 16);
+    L.add(v->sums, v->rows0 * K * 16);
+    L.add(v->gfin, K * 16);
     L.add(v->flags, 256);
     L.add(v->opening, 16);
     v->tr.plan(L, v->c_cap, 0);
@@ -1086,7 +1092,7 @@ static void verifier_bind(dmx_verifier* v) { v->tr.seeds = v->meta.as<uint8_t>()
 static int verifier_enqueue_gen(dmx_verifier* v) {
     launch_fill32(v->flags.as<uint32_t>(), 64, 0, v->s, v->jb);
     if (!v->G0_src)
-        launch_gen_generators(v->gseed(), v->rows * K, log2_exact(v->glen), v->G0.as<zq>(), v->s, v->jb);
+        launch_gen_generators(v->gseed(), v->rows0 * K, log2_exact(v->n0), v->G0.as<zq>(), v->s, v->jb);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
@@ -1105,8 +1111,8 @@ static int verifier_new_object(dmx_ctx* ctx, size_t d, size_t rows, size_t ring_
     v->ctx = ctx;
     v->jb = Jobs{n, 0};
     v->rounds = d;
-    v->rows = rows;
-    v->glen = v->clen = ring_len;
+    v->rows = v->rows0 = rows;
+    v->glen = v->clen = v->n0 = ring_len;
     v->g_cap = v->rows * K * v->glen;
     v->c_cap = v->clen;
     v->stage.resize(n);
@@ -1167,14 +1173,22 @@ extern "C" dmx_verifier* dmx_verifier_clone(const dmx_verifier* v) {
     if (verifier_new_object(v->ctx, v->rounds, v->rows, v->clen, 1, &raw)) return nullptr;
     std::unique_ptr<dmx_verifier> c(raw);
     c->round = v->round;
+    c->rows = v->rows;
     c->glen = v->glen;
-    c->g_cap = v->rows * K * v->glen;
+    c->rows0 = v->rows0;
+    c->n0 = v->n0;
+    c->g_cap = v->g_cap;
+    c->vec_mask = v->vec_mask;
+    c->poly_mask = v->poly_mask;
     Layout L;
     verifier_plan(c.get(), L);
     if (L.commit(c->store, 1)) return nullptr;
     verifier_bind(c.get());
     (void)hipStreamSynchronize(v->s);
     if (hipMemcpyAsync(c->G0.p, v->G(), c->g_cap * 16, hipMemcpyDeviceToDevice, c->s) != hipSuccess) return nullptr;
+    if (hipMemcpyAsync(c->chal.p, v->chal.p, std::max<size_t>(v->rounds, 1) * 4 * 16, hipMemcpyDeviceToDevice,
+                       c->s) != hipSuccess)
+        return nullptr;
     if (hipMemcpyAsync(c->C.p, v->C.p, (size_t)K * v->clen * 16, hipMemcpyDeviceToDevice, c->s) != hipSuccess)
         return nullptr;
     if (hipMemcpyAsync(c->meta.p, v->meta.p, sizeof(JobMeta), hipMemcpyDeviceToDevice, c->s) != hipSuccess)
@@ -1206,49 +1220,35 @@ static bool all_zero(const std::vector<zq>& m) {
 }
 
 // One replay round on the device (verifier.rs:57-120) with the round's four cross-term
-// modules already in device memory and their shapes already checked. No host sync.
+// modules already in device memory and their shapes already checked. No host sync. The
+// generator folds of the round (verifier.rs:75, 102) are not materialised: the round's challenge
+// inverses are kept and the terminal generator is formed once, at the final opening.
 static int verifier_round_launch(dmx_verifier* v, const zq* d_l, const zq* d_r, const zq* d_lp, const zq* d_rp) {
     hipStream_t s = v->s;
     const Jobs& jb = v->jb;
     const size_t n = v->clen;
     const bool vec = v->rows > 1, poly = v->glen > 1;
     uint32_t* flags = v->flags.as<uint32_t>();
-    const int lg = log2_exact(v->glen);
-    size_t rows = v->rows;
+    zq* ch = v->round_chal(v->round);
     const zq* Ct = v->C.as<zq>();
     if (vec) {
-        v->tr.challenge("vec", v->round, v->C.as<zq>(), n, d_l, n, d_r, n, v->scal.as<zq>(), flags, s, jb);
-        launch_com_update(v->C.as<zq>(), d_l, d_r, v->scal.as<zq>(), 0, log2_exact(n), v->Ct.as<zq>(), s, jb);
+        v->tr.challenge("vec", v->round, v->C.as<zq>(), n, d_l, n, d_r, n, ch, flags, s, jb);
+        launch_com_update(v->C.as<zq>(), d_l, d_r, ch, 0, log2_exact(n), v->Ct.as<zq>(), s, jb);
         Ct = v->Ct.as<zq>();
-        RowSrc f{};
-        f.mode = ROW_VFOLD;
-        f.a = (v->round == 0 && v->G0_src) ? v->G0_src : v->G();
-        f.a_shared = v->round == 0 && v->G0_src;
-        f.b_off = (rows / 2) * K;
-        f.scal = v->scal.as<zq>() + 1;  // g_L + x^-1 g_R (verifier.rs:141-151)
-        f.dst = v->Gother();
-        launch_fold_rows_zq(f, (rows / 2) * K, lg, s, jb);
-        v->gcur ^= 1;
-        rows /= 2;
+        v->vec_mask |= 1u << v->round;
     }
     size_t nclen = n, nglen = v->glen;
     if (poly) {
-        v->tr.challenge("poly", v->round, Ct, n, d_lp, n / 2, d_rp, n / 2, v->scal.as<zq>() + 2, flags, s, jb);
-        launch_com_update(Ct, d_lp, d_rp, v->scal.as<zq>() + 2, 1, log2_exact(n / 2), v->C.as<zq>(), s, jb);
-        RowSrc f{};
-        f.mode = ROW_PFOLD_G;
-        f.a = v->G();
-        f.scal = v->scal.as<zq>() + 3;  // g_e + y^-1 X g_o (verifier.rs:153-179)
-        f.dst = v->Gother();
-        launch_fold_rows_zq(f, rows * K, lg - 1, s, jb);
-        v->gcur ^= 1;
+        v->tr.challenge("poly", v->round, Ct, n, d_lp, n / 2, d_rp, n / 2, ch + 2, flags, s, jb);
+        launch_com_update(Ct, d_lp, d_rp, ch + 2, 1, log2_exact(n / 2), v->C.as<zq>(), s, jb);
+        v->poly_mask |= 1u << v->round;
         nclen = n / 2;
         nglen = v->glen / 2;
     } else if (vec) {
         launch_copy16(v->C.p, v->Ct.p, (size_t)K * n, s, jb);
     }
     HIP_TRY(hipGetLastError());
-    v->rows = rows;
+    if (vec) v->rows /= 2;
     v->clen = nclen;
     v->glen = nglen;
     v->round += 1;
@@ -1293,7 +1293,7 @@ extern "C" int dmx_verifier_update_commitment(dmx_verifier* v, const uint8_t* re
     }
     // state (rows/lens/round) only advances after the device work is known good
     const size_t rows0 = v->rows, clen0 = v->clen, glen0 = v->glen, round0 = v->round;
-    const int gcur0 = v->gcur;
+    const uint32_t vmask0 = v->vec_mask, pmask0 = v->poly_mask;
     rc = verifier_round_launch(v, d_l, d_r, d_lp, d_rp);
     uint32_t f = 0;
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(DMX_ERR_HIP, "stream sync failed");
@@ -1304,7 +1304,8 @@ extern "C" int dmx_verifier_update_commitment(dmx_verifier* v, const uint8_t* re
         v->clen = clen0;
         v->glen = glen0;
         v->round = round0;
-        v->gcur = gcur0;
+        v->vec_mask = vmask0;
+        v->poly_mask = pmask0;
     }
     return rc;
 }
@@ -1313,8 +1314,14 @@ static int verifier_final_launch(dmx_verifier* v, const zq* d_opening) {
     if (v->round != v->rounds) return fail(DMX_ERR_NOT_FINAL, "cannot verify final opening before all rounds replay");
     if (v->rows != 1) return fail(DMX_ERR_NOT_FINAL, "terminal generator family must have length one");
     if (v->glen != v->clen) return fail(DMX_ERR_OPENING, "final opening mismatch");
+    if (v->glen != 1 || (size_t)__builtin_popcount(v->vec_mask) != (size_t)log2_exact(v->rows0) ||
+        (size_t)__builtin_popcount(v->poly_mask) != (size_t)log2_exact(v->n0))
+        return fail(DMX_ERR_NOT_FINAL, "the generator family has not been folded to scalars");
+    launch_terminal_generator(v->G(), v->G0_src != nullptr, v->chal.as<zq>(), (uint32_t)v->rounds, v->vec_mask,
+                              v->poly_mask, log2_exact(v->rows0), log2_exact(v->n0), v->wts.as<zq>(), v->uts.as<zq>(),
+                              v->sums.as<zq>(), v->gfin.as<zq>(), v->s, v->jb);
     launch_fill32(v->flags.as<uint32_t>(), 1, 0, v->s, v->jb);
-    launch_final_check(v->G(), v->C.as<zq>(), d_opening, (size_t)K * v->glen, v->flags.as<uint32_t>(), 1, v->s, v->jb);
+    launch_final_check(v->gfin.as<zq>(), v->C.as<zq>(), d_opening, (size_t)K, v->flags.as<uint32_t>(), 1, v->s, v->jb);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
@@ -1553,7 +1560,7 @@ static int batch_enqueue(BatchJob* J) {
     v->round = 0;
     v->rows = p->N;
     v->glen = v->clen = p->N;
-    v->gcur = 0;
+    v->vec_mask = v->poly_mask = 0;
     launch_copy_in(p->meta.as<uint64_t>(), p->file.as<uint8_t>(), p->hash_cap, p->s, p->jb);
     int rc = prover_init_file(p);
     if (rc) return rc;
@@ -1759,7 +1766,8 @@ static size_t job_workspace_bytes(int d) {
     tp.N = (size_t)1 << d;
     tp.hash_cap = max_file_len(d);
     dmx_verifier tv;
-    tv.rows = tv.glen = tv.clen = tv.c_cap = tp.N;
+    tv.rows = tv.glen = tv.clen = tv.c_cap = tv.rows0 = tv.n0 = tp.N;
+    tv.rounds = (size_t)d;
     tv.g_cap = tp.N * K * tp.N;
     Layout L;
     prover_plan(&tp, L, tp.hash_cap);
