@@ -1502,6 +1502,10 @@ struct BatchJob {
     // the upload is an asynchronous DMA on the batch's stream, overlapping other batches' work
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    // pinned staging for the results (flags, file ids, com_0, records, openings): DMA copies,
+    // not the runtime's pageable path through a bounce buffer
+    void* pinned_out = nullptr;
+    size_t pinned_out_bytes = 0;
     void drop_resident() {
         if (exec_r) (void)hipGraphExecDestroy(exec_r);
         if (graph_r) (void)hipGraphDestroy(graph_r);
@@ -1512,6 +1516,7 @@ struct BatchJob {
     }
     ~BatchJob() {
         if (pinned) (void)hipHostFree(pinned);  // idle batch: its uploads have completed
+        if (pinned_out) (void)hipHostFree(pinned_out);
         drop_resident();
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1713,18 +1718,33 @@ static int batch_collect(BatchJob* J, const JobOut* outs) {
     const uint32_t n = J->n;
     const size_t st = p->jb.stride;
     HIP_TRY(hipEventSynchronize(J->done));
-    std::vector<uint32_t> pf(n), vf(n);
-    std::vector<zq> com((size_t)n * K * p->N), recs((size_t)n * p->rec_off[p->d]), mstar(n);
-    std::vector<uint8_t> fid((size_t)n * 32);
+    const size_t com_b = (size_t)K * p->N * 16, rb = p->rec_off[p->d] * 16;
+    const size_t need = (size_t)n * (4 + 4 + 32 + 16 + com_b + rb) + 64;
+    if (need > J->pinned_out_bytes) {
+        if (J->pinned_out) (void)hipHostFree(J->pinned_out);
+        J->pinned_out = nullptr;
+        J->pinned_out_bytes = 0;
+        if (hipHostMalloc(&J->pinned_out, need, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            J->pinned_out = nullptr;
+            return fail(DMX_ERR_OOM, "pinned result staging of %zu bytes failed", need);
+        }
+        J->pinned_out_bytes = need;
+    }
+    uint8_t* base = static_cast<uint8_t*>(J->pinned_out);
+    zq* com = reinterpret_cast<zq*>(base);  // 16-byte aligned pieces first
+    zq* recs = reinterpret_cast<zq*>(base + n * com_b);
+    zq* mstar = reinterpret_cast<zq*>(base + n * (com_b + rb));
+    uint32_t* pf = reinterpret_cast<uint32_t*>(base + n * (com_b + rb + 16));
+    uint32_t* vf = pf + n;
+    uint8_t* fid = reinterpret_cast<uint8_t*>(vf + n);
     hipStream_t s = p->s;
-    HIP_TRY(copy_rows(pf.data(), 4, p->flags.p, st, 4, n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(copy_rows(vf.data(), 4, v->flags.p, st, 4, n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(copy_rows(fid.data(), 32, p->tr.file_id(), st, 32, n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(copy_rows(com.data(), (size_t)K * p->N * 16, p->com0.p, st, (size_t)K * p->N * 16, n,
-                             hipMemcpyDeviceToHost, s));
-    const size_t rb = p->rec_off[p->d] * 16;
-    HIP_TRY(copy_rows(recs.data(), rb, p->recs.p, st, rb, n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(copy_rows(mstar.data(), 16, p->W_a.p, st, 16, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(copy_rows(pf, 4, p->flags.p, st, 4, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(copy_rows(vf, 4, v->flags.p, st, 4, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(copy_rows(fid, 32, p->tr.file_id(), st, 32, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(copy_rows(com, com_b, p->com0.p, st, com_b, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(copy_rows(recs, rb, p->recs.p, st, rb, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(copy_rows(mstar, 16, p->W_a.p, st, 16, n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     int first = DMX_OK;
     for (uint32_t b = 0; b < n; ++b) {
@@ -1739,7 +1759,7 @@ static int batch_collect(BatchJob* J, const JobOut* outs) {
             if (o.statement_cap < st_need)
                 rc = fail(DMX_ERR_BUFFER, "statement buffer too small");
             else
-                encode_statement_from(p, b, fid.data() + 32 * b, com.data() + (size_t)b * K * p->N, o.statement);
+                encode_statement_from(p, b, fid + 32 * b, com + (size_t)b * K * p->N, o.statement);
         }
         const size_t rec_need = dmx_proof_records_size(p->orig_len[b]);
         if (o.records_len) *o.records_len = rec_need;
@@ -1747,7 +1767,7 @@ static int batch_collect(BatchJob* J, const JobOut* outs) {
             if (o.records_cap < rec_need)
                 rc = fail(DMX_ERR_BUFFER, "records buffer too small");
             else
-                encode_records_from(p, recs.data() + (size_t)b * p->rec_off[p->d], o.records);
+                encode_records_from(p, recs + (size_t)b * p->rec_off[p->d], o.records);
         }
         if (!rc && o.opening) {
             o.opening[0] = (uint64_t)mstar[b].w[0] | ((uint64_t)mstar[b].w[1] << 32);
@@ -2320,13 +2340,15 @@ struct dmx_server {
         std::vector<uint64_t> tickets;
         size_t bytes;
     };
-    std::vector<Flight> flights;
+    std::vector<Flight> flights;  // running on the device
+    std::deque<Flight> done_q;     // finished on the device, results not yet collected
+    std::condition_variable cv_fin;
     size_t hbm_used = 0;
     uint64_t next_ticket = 1;
     uint64_t batches = 0, batched_jobs = 0;
     double launch_s = 0;  // scheduler time spent acquiring workspaces and launching
-    bool stop = false;
-    std::thread th;
+    bool stop = false, fin_stop = false;
+    std::thread th, fth;  // scheduler; finisher (result copies and encoding, off the scheduler)
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -2361,15 +2383,34 @@ static void server_finish(dmx_server* S, dmx_server::Flight& f, int launch_rc) {
     S->cv_done.notify_all();
 }
 
+// Collects finished batches (device-to-host copies, proof encoding, workspace release) so the
+// scheduler never stalls on them and keeps the device fed.
+static void server_finisher(dmx_server* S) {
+    (void)hipSetDevice(S->ctx->device);
+    for (;;) {
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->cv_fin.wait(lk, [S] { return S->fin_stop || !S->done_q.empty(); });
+        if (S->done_q.empty()) break;  // stopping and drained
+        dmx_server::Flight f = std::move(S->done_q.front());
+        S->done_q.pop_front();
+        lk.unlock();
+        server_finish(S, f, DMX_OK);
+    }
+}
+
 static void server_loop(dmx_server* S) {
     (void)hipSetDevice(S->ctx->device);
     const uint32_t max_batch = S->cfg.max_batch ? S->cfg.max_batch : 64;
     const uint32_t max_inflight = S->cfg.max_inflight ? S->cfg.max_inflight : 4;
     for (;;) {
-        // completions, in any order
+        // completions, in any order, handed to the finisher
         for (size_t i = 0; i < S->flights.size();) {
             if (hipEventQuery(S->flights[i].J->done) == hipSuccess) {
-                server_finish(S, S->flights[i], DMX_OK);
+                {
+                    std::lock_guard<std::mutex> g(S->mu);
+                    S->done_q.push_back(std::move(S->flights[i]));
+                }
+                S->cv_fin.notify_one();
                 S->flights.erase(S->flights.begin() + i);
             } else {
                 (void)hipGetLastError();  // "not ready" must not surface at the next launch check
@@ -2392,7 +2433,9 @@ static void server_loop(dmx_server* S) {
             // large batches cannot pin all memory
             while (nb > 1 && (S->hbm_used + nb * per > S->cfg.hbm_budget_bytes || nb * per > S->cfg.hbm_budget_bytes / 8))
                 nb /= 2;
-            if (S->hbm_used + nb * per > S->cfg.hbm_budget_bytes && !S->flights.empty()) break;  // wait for memory
+            // batches still running or still being collected will return memory
+            const bool pending = !S->flights.empty() || !S->done_q.empty();
+            if (S->hbm_used + nb * per > S->cfg.hbm_budget_bytes && pending) break;  // wait for memory
             // a cached workspace of this size, else a new one if the cache has room, else a
             // smaller cached one; otherwise wait for a batch to finish
             lk.unlock();
@@ -2403,7 +2446,7 @@ static void server_loop(dmx_server* S) {
                 arc = batch_try_acquire(S->ctx, d, (uint32_t)m, false, &J);
                 if (!arc) nb = m;
             }
-            if (arc == DMX_ERR_PENDING && S->flights.empty()) {  // nothing will free up: create anyway
+            if (arc == DMX_ERR_PENDING && !pending) {  // nothing will free up: create anyway
                 arc = batch_acquire(S->ctx, d, (uint32_t)nb, &J);
             }
             lk.lock();
@@ -2464,6 +2507,7 @@ extern "C" int dmx_server_create(dmx_ctx* ctx, const dmx_server_config* cfg, dmx
         HIP_TRY(hipMemGetInfo(&free_b, &total_b));
         S->cfg.hbm_budget_bytes = free_b / 4 * 3;
     }
+    S->fth = std::thread(server_finisher, S.get());
     S->th = std::thread(server_loop, S.get());
     *out = S.release();
     return DMX_OK;
@@ -2529,5 +2573,11 @@ extern "C" void dmx_server_destroy(dmx_server* S) {
         S->cv_work.notify_all();
     }
     if (S->th.joinable()) S->th.join();
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->fin_stop = true;
+    }
+    S->cv_fin.notify_all();
+    if (S->fth.joinable()) S->fth.join();
     delete S;
 }
