@@ -249,6 +249,7 @@ struct TimedLaunch {
 
 struct dmx_ctx {
     int device = 0;
+    unsigned cus = 256;  // compute units of the device
     DevBuf tables;
     NttTables tw{};
     PrimeConsts pc[NPRIMES];
@@ -515,6 +516,10 @@ extern "C" int dmx_ctx_create(int device_index, dmx_ctx** out) {
     HIP_TRY(hipSetDevice(device_index));
     std::unique_ptr<dmx_ctx> c(new dmx_ctx);
     c->device = device_index;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_index) == hipSuccess && cus > 0)
+        c->cus = (unsigned)cus;
+    (void)hipGetLastError();
     std::vector<uint32_t> host;
     build_constants(c.get(), host);
     int rc = c->tables.alloc(host.size() * 4);
@@ -1089,10 +1094,10 @@ static void verifier_bind(dmx_verifier* v) { v->tr.seeds = v->meta.as<uint8_t>()
 
 // Generator derivation from g_0_seed (verifier.rs:31-33); depends on the seed only, so a fused
 // job starts it before the prover has produced the statement. Kernels only.
-static int verifier_enqueue_gen(dmx_verifier* v) {
+static int verifier_enqueue_gen(dmx_verifier* v, hipStream_t gs = nullptr) {
     launch_fill32(v->flags.as<uint32_t>(), 64, 0, v->s, v->jb);
     if (!v->G0_src)
-        launch_gen_generators(v->gseed(), v->rows0 * K, log2_exact(v->n0), v->G0.as<zq>(), v->s, v->jb);
+        launch_gen_generators(v->gseed(), v->rows0 * K, log2_exact(v->n0), v->G0.as<zq>(), gs ? gs : v->s, v->jb);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
@@ -1491,6 +1496,11 @@ struct BatchJob {
     std::unique_ptr<dmx_verifier> v;
     std::vector<hipEvent_t> evs;
     hipEvent_t done = nullptr;
+    // the verifier's generator expansion runs on its own stream, started once the prover's
+    // transform-heavy first rounds are done, so it fills the latency-bound rounds instead of
+    // competing with those transforms; the replay needs the matrix only at the final opening
+    hipStream_t gs = nullptr;
+    hipEvent_t gen_done = nullptr;
     hipGraph_t graph = nullptr;  // jobs expand their own generators
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph_r = nullptr;  // jobs read the resident matrix `gen`
@@ -1522,6 +1532,8 @@ struct BatchJob {
         if (graph) (void)hipGraphDestroy(graph);
         for (auto e : evs) (void)hipEventDestroy(e);
         if (done) (void)hipEventDestroy(done);
+        if (gen_done) (void)hipEventDestroy(gen_done);
+        if (gs) (void)hipStreamDestroy(gs);
     }
     size_t bytes() const { return store.bytes; }
 };
@@ -1553,6 +1565,8 @@ static int batch_create(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
     J->evs.resize((size_t)d + 3);
     for (auto& e : J->evs) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&J->done, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&J->gen_done, hipEventDisableTiming));
+    HIP_TRY(hipStreamCreate(&J->gs));
     *out = J.release();
     return DMX_OK;
 }
@@ -1569,12 +1583,18 @@ static int batch_enqueue(BatchJob* J) {
     launch_copy_in(p->meta.as<uint64_t>(), p->file.as<uint8_t>(), p->hash_cap, p->s, p->jb);
     int rc = prover_init_file(p);
     if (rc) return rc;
-    // fork the verifier stream: its generator expansion (it needs only the seed) overlaps the
-    // prover's generator transforms rather than the latency-bound file hash before them
+    // fork the verifier stream (its generator expansion goes to J->gs after the prover's first
+    // rounds, below)
     HIP_TRY(hipEventRecord(J->evs[0], p->s));
     HIP_TRY(hipStreamWaitEvent(v->s, J->evs[0], 0));
-    rc = verifier_enqueue_gen(v);
-    if (rc) return rc;
+    const bool own_gen = v->G0_src == nullptr;
+    const size_t gen_after = p->d >= 3 ? 1 : 0;  // prover round after which the expansion starts
+    if (!own_gen) {
+        rc = verifier_enqueue_gen(v);
+        if (rc) return rc;
+    } else {
+        launch_fill32(v->flags.as<uint32_t>(), 64, 0, v->s, v->jb);
+    }
     rc = prover_init_commit(p);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(J->evs[1], p->s));
@@ -1586,10 +1606,19 @@ static int batch_enqueue(BatchJob* J) {
         if (rc) return rc;
         HIP_TRY(hipEventRecord(J->evs[2 + j], p->s));
         HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + j], 0));
+        if (own_gen && j == gen_after) {
+            HIP_TRY(hipStreamWaitEvent(J->gs, J->evs[2 + j], 0));
+            // one workgroup per CU: the expansion has the whole tail of the proof to finish in
+            launch_gen_generators(v->gseed(), v->rows0 * K, log2_exact(v->n0), v->G0.as<zq>(), J->gs, v->jb,
+                                  p->ctx->cus);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipEventRecord(J->gen_done, J->gs));
+        }
         rc = verifier_round_launch(v, p->rec_lvec(j), p->rec_rvec(j), p->rec_lpoly(j), p->rec_rpoly(j));
         if (rc) return rc;
     }
     p->round = (size_t)p->d;
+    if (own_gen) HIP_TRY(hipStreamWaitEvent(v->s, J->gen_done, 0));
     rc = verifier_final_launch(v, p->W_a.as<zq>());  // m* = message[0].coeffs[0] (prover.rs:133-138)
     if (rc) return rc;
     HIP_TRY(hipEventRecord(J->evs[J->evs.size() - 1], v->s));
