@@ -320,7 +320,7 @@ static void launch_b3_group_merge(const uint32_t* cvs, size_t n, const uint32_t 
 __global__ void __launch_bounds__(1024) k_b3_root(uint32_t* __restrict__ cvs, size_t n, uint8_t* msg, size_t len,
                                                   KeyW key, uint32_t flags, uint32_t* digest_out, zq* chal_out,
                                                   uint32_t* err, int chunk_level, size_t js, const uint64_t* dlen,
-                                                  size_t l1_off) {
+                                                  size_t l1_off, int want_inv) {
     __shared__ uint32_t sa[1024 * 8], sb[512 * 8 + 8];
     __shared__ uint32_t dig[8];
     __shared__ int again;
@@ -364,17 +364,18 @@ __global__ void __launch_bounds__(1024) k_b3_root(uint32_t* __restrict__ cvs, si
         // The challenge and its inverse are computed by all of wave 0 (identical values in every
         // lane); lane 0 stores them. The multi-word products of the inverse stay in vector
         // registers, where a 64-bit multiply-add is one instruction; its divstep loop reads the
-        // low words back into scalar registers.
+        // low words back into scalar registers. Without want_inv only x is written (the caller
+        // inverts it off its critical path).
         if (threadIdx.x < 64 && chal_out) {
             uint32_t h[8];
             // identity quad permutation: same value, but a per-lane register to the compiler
             for (int j = 0; j < 8; ++j) h[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)dig[j], 0xE4, 0xF, 0xF, false);
             const zq x = zq_from_digest(h);
             if (!zq_is_zero(x)) {
-                const zq xi = zq_inv_divsteps(x);
+                const zq xi = want_inv ? zq_inv_divsteps(x) : x;
                 if (threadIdx.x == 0) {
                     chal_out[0] = x;
-                    chal_out[1] = xi;
+                    if (want_inv) chal_out[1] = xi;
                 }
             }
         }
@@ -419,12 +420,12 @@ __global__ void __launch_bounds__(1024) k_b3_root(uint32_t* __restrict__ cvs, si
 
 static void launch_b3_root(const uint32_t* cvs, size_t n, const uint8_t* msg, size_t len, const uint32_t key[8],
                            uint32_t flags, uint32_t* digest_out, zq* chal_out, uint32_t* err_flags, int chunk_level,
-                           hipStream_t s, const Jobs& jb, const uint64_t* dlen, size_t l1_off) {
+                           hipStream_t s, const Jobs& jb, const uint64_t* dlen, size_t l1_off, int want_inv = 1) {
     KeyW k;
     for (int i = 0; i < 8; ++i) k.w[i] = key[i];
     hipLaunchKernelGGL(k_b3_root, dim3(1, jb.n), dim3(1024), 0, s, const_cast<uint32_t*>(cvs), n,
                        const_cast<uint8_t*>(msg), len, k, flags, digest_out, chal_out, err_flags, chunk_level,
-                       jb.stride, dlen, l1_off);
+                       jb.stride, dlen, l1_off, want_inv);
 }
 
 // Hash a device buffer: chunk CVs, group merges while more than 1024 values remain, root.
@@ -436,7 +437,7 @@ static void launch_b3_root(const uint32_t* cvs, size_t n, const uint8_t* msg, si
 constexpr size_t CHALLENGE_MAX_CHUNKS = 16;
 
 __global__ void __launch_bounds__(256) k_b3_tx_challenge(TranscriptMsg t, size_t len, KeyW key, uint32_t flags,
-                                                        zq* chal_out, uint32_t* err, size_t js) {
+                                                        zq* chal_out, uint32_t* err, size_t js, int want_inv) {
     __shared__ __align__(16) uint64_t win[TX_WINDOW / 8];
     __shared__ uint32_t sa[16 * 8], sb[8 * 8 + 8];
     __shared__ uint32_t dig[8];
@@ -476,10 +477,10 @@ __global__ void __launch_bounds__(256) k_b3_tx_challenge(TranscriptMsg t, size_t
             for (int j = 0; j < 8; ++j) h[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)dig[j], 0xE4, 0xF, 0xF, false);
             const zq x = zq_from_digest(h);
             if (!zq_is_zero(x)) {
-                const zq xi = zq_inv_divsteps(x);
+                const zq xi = want_inv ? zq_inv_divsteps(x) : x;
                 if (threadIdx.x == 0) {
                     chal_out[0] = x;
-                    chal_out[1] = xi;
+                    if (want_inv) chal_out[1] = xi;
                 }
             }
         }
@@ -513,21 +514,22 @@ static void launch_b3_tx_chunks(const TranscriptMsg& t, size_t len, const uint32
 
 static void hash_tree(uint32_t* cv_scratch, size_t n, const uint8_t* msg, size_t len, const uint32_t key[8],
                       uint32_t flags, uint32_t* digest_out, zq* chal_out, uint32_t* err_flags, hipStream_t s,
-                      const Jobs& jb);
+                      const Jobs& jb, int want_inv = 1);
 
 void transcript_challenge(const TranscriptMsg& t, const uint32_t key[8], uint32_t flags, uint8_t* msg,
-                          uint32_t* cv_scratch, zq* chal_out, uint32_t* err_flags, hipStream_t s, const Jobs& jb) {
+                          uint32_t* cv_scratch, zq* chal_out, uint32_t* err_flags, hipStream_t s, const Jobs& jb,
+                          bool inverse) {
     if (jb.n == 0) return;
     const size_t len = tx_message_len(t.label_len, t.lens);
     if (nchunks_of(len) <= CHALLENGE_MAX_CHUNKS) {
         KeyW k;
         for (int i = 0; i < 8; ++i) k.w[i] = key[i];
         hipLaunchKernelGGL(k_b3_tx_challenge, dim3(1, jb.n), dim3(256), 0, s, t, len, k, flags, chal_out, err_flags,
-                           jb.stride);
+                           jb.stride, (int)inverse);
         return;
     }
     launch_b3_tx_chunks(t, len, key, flags, cv_scratch, msg, s, jb);
-    hash_tree(cv_scratch, nchunks_of(len), msg, len, key, flags, nullptr, chal_out, err_flags, s, jb);
+    hash_tree(cv_scratch, nchunks_of(len), msg, len, key, flags, nullptr, chal_out, err_flags, s, jb, (int)inverse);
 }
 
 void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags,
@@ -548,7 +550,7 @@ void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t
 // group merges while more than 1024 values remain, then the root
 static void hash_tree(uint32_t* cv_scratch, size_t n, const uint8_t* msg, size_t len, const uint32_t key[8],
                       uint32_t flags, uint32_t* digest_out, zq* chal_out, uint32_t* err_flags, hipStream_t s,
-                      const Jobs& jb) {
+                      const Jobs& jb, int want_inv) {
     uint32_t* cur = cv_scratch;
     size_t cnt = n;
     uint32_t* next = cv_scratch + n * 8;
@@ -559,7 +561,7 @@ static void hash_tree(uint32_t* cv_scratch, size_t n, const uint8_t* msg, size_t
         next = next + cnt * 8;
     }
     launch_b3_root(cur, cnt, msg, len, key, flags, digest_out, chal_out, err_flags, cur == cv_scratch, s, jb, nullptr,
-                   0);
+                   0, want_inv);
 }
 
 size_t device_hash_scratch_words(size_t len) {

@@ -40,6 +40,8 @@ enum RowMode : int {
     ROW_PFOLD_W = 2, // x = a[row][2c] + s * a[row][2c+1]                (poly fold, witness)
     ROW_PFOLD_G = 3, // x = a[row][2c] + s * (X * odd)[c]                 (poly fold, generators)
     ROW_GEN = 4,     // x = keyed_blake3(g_seed, "g"||i||k||c) mod q, row = i*K + k
+    ROW_VFOLD_L = 5, // x = s * a[row][c] + a[row + b_off][c]            (vector fold, left scaled)
+    ROW_PFOLD_GL = 6,// x = s * a[row][2c] + (X * odd)[c]                 (poly fold, even scaled)
 };
 
 struct RowSrc {
@@ -79,8 +81,17 @@ void set_device_constants(const PrimeConsts* pc, const CrtConsts* crt, hipStream
 void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out,
                          const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
 // two row sets of the same degree in one launch (segment 2 may be empty)
+// A challenge inverse carried by a forward-transform launch: an extra workgroup 0 sets
+// chal[1] = chal[0]^-1 and *u_out = *u_prev * chal[0]^-1 (u_prev null: 1) while the other
+// workgroups transform rows. chal null: no task.
+struct InvTask {
+    zq* chal = nullptr;
+    const zq* u_prev = nullptr;
+    zq* u_out = nullptr;
+};
 void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, const RowSrc& src2, size_t rows2,
-                          uint32_t* out2, int log_m, const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
+                          uint32_t* out2, int log_m, const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{},
+                          const InvTask& it = InvTask{});
 // inverse NTT in place on `slices` slices of [8][m]: data[(slice*8 + p)*m + c], times m^-1
 void launch_ntt_inv(uint32_t* data, size_t slices, int log_m, const NttTables& tw, hipStream_t s,
                     const Jobs& jb = Jobs{});
@@ -128,6 +139,7 @@ struct ComUpd {
     const zq* r;
     const zq* scal;
     int poly;
+    const zq* unscale = nullptr;  // when set, every lifted value is multiplied by *unscale first
 };
 // lift K commitment slices (checking them against the update of `cu`, written to com_out) and
 // lr_slices cross-term slices (to out_lr) in one launch; see k_lift_check
@@ -173,9 +185,10 @@ size_t transcript_len(uint32_t label_len, const uint32_t lens[3]);
 // chal_out[0] = x = LE256(BLAKE3(message)) mod q, re-hashed with counter+1 while zero, and
 // chal_out[1] = x^-1, for every job. `msg` (transcript_len bytes) receives the chunks that hold
 // the counter; cv_scratch holds device_hash_scratch_words(transcript_len) words.
+// With inverse = false only chal_out[0] is written (the caller inverts it later, see InvTask).
 void transcript_challenge(const TranscriptMsg& t, const uint32_t key[8], uint32_t flags, uint8_t* msg,
                           uint32_t* cv_scratch, zq* chal_out, uint32_t* err_flags, hipStream_t s,
-                          const Jobs& jb = Jobs{});
+                          const Jobs& jb = Jobs{}, bool inverse = true);
 // Whole-buffer BLAKE3 (chunks, group merges, root). cv_scratch holds device_hash_scratch_words.
 // digest_out (8 words) when non-null. For a challenge (chal_out != nullptr): LE256(digest) mod q,
 // re-hashed with counter+1 while zero, then chal_out[0] = x, chal_out[1] = x^-1.

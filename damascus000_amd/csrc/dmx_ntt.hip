@@ -79,6 +79,17 @@ __device__ __forceinline__ zq row_value(const RowSrc& src, const uint32_t* key, 
             zq xo = c ? load_zq(base + 2 * c - 1) : zq_neg(load_zq(base + 2 * m - 1));
             return zq_add(e, zq_mul(xo, *src.scal));
         }
+        case ROW_VFOLD_L: {
+            zq a = load_zq(src.a + row * m + c);
+            zq b = load_zq(src.a + (row + src.b_off) * m + c);
+            return zq_add(zq_mul(a, *src.scal), b);
+        }
+        case ROW_PFOLD_GL: {
+            const zq* base = src.a + row * (2 * m);
+            zq e = load_zq(base + 2 * c);
+            zq xo = c ? load_zq(base + 2 * c - 1) : zq_neg(load_zq(base + 2 * m - 1));
+            return zq_add(zq_mul(e, *src.scal), xo);
+        }
         default: {  // ROW_GEN
             uint64_t i = row / K;
             uint32_t k = (uint32_t)(row % K);
@@ -91,10 +102,28 @@ __device__ __forceinline__ zq row_value(const RowSrc& src, const uint32_t* key, 
 // forward negacyclic NTT (Cooley-Tukey, natural order in, bit-reversed order out)
 // butterfly: (a, b) -> (a + z*b, a - z*b), z = zeta[m/(2*len) + block]
 // ---------------------------------------------------------------------------------------
+// InvTask: wave 0 of the extra workgroup 0 (every lane computes the same value; lane 0 stores)
+__device__ void inverse_task(const InvTask& it, size_t jo) {
+    if (threadIdx.x >= 64) return;
+    zq* chal = jp(it.chal, jo);
+    const zq x = chal[0];
+    const zq xi = zq_inv_divsteps(x);
+    const zq u = it.u_prev ? zq_mul(*jp(it.u_prev, jo), xi) : xi;
+    if (threadIdx.x == 0) {
+        chal[1] = xi;
+        *jp(it.u_out, jo) = u;
+    }
+}
+
 __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(FwdSeg s1, FwdSeg s2, uint32_t blocks1, int log_m,
-                                                               NttTables tw, size_t js) {
+                                                               NttTables tw, size_t js, InvTask it) {
     extern __shared__ uint32_t sh[];
-    const bool second = blockIdx.x >= blocks1;  // uniform over the workgroup
+    uint32_t bx = blockIdx.x;
+    if (it.chal) {  // uniform over the workgroup; the task workgroup meets no barrier
+        if (bx == 0) return inverse_task(it, job_off(js));
+        --bx;
+    }
+    const bool second = bx >= blocks1;
     RowSrc src = second ? s2.src : s1.src;
     uint32_t* out = second ? s2.out : s1.out;
     const size_t jo = job_off(js);
@@ -109,7 +138,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(FwdSeg s1, FwdSeg 
         for (int i = 0; i < 8; ++i) key[i] = src.gseed[i];
     }
     const int m = 1 << log_m;
-    const size_t row = second ? blockIdx.x - blocks1 : blockIdx.x;
+    const size_t row = second ? bx - blocks1 : bx;
     const int pg = min(NPRIMES, LDS_WORDS >> log_m);  // primes per LDS pass
     for (int p0 = 0; p0 < NPRIMES; p0 += pg) {
         // 1) produce the row and reduce it into the primes of this group
@@ -184,10 +213,15 @@ __device__ __forceinline__ void ct_butterfly(uint32_t& x, uint32_t& y, uint32_t 
 
 template <int E, int TP>
 __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(FwdSeg s1, FwdSeg s2, uint32_t blocks1, NttTables tw,
-                                                             size_t js) {
+                                                             size_t js, InvTask it) {
     using G = RegNtt<E, TP>;
     extern __shared__ uint32_t sh[];
-    const bool second = blockIdx.x >= blocks1;  // uniform over the workgroup
+    uint32_t bx = blockIdx.x;
+    if (it.chal) {  // uniform over the workgroup; the task workgroup meets no barrier
+        if (bx == 0) return inverse_task(it, job_off(js));
+        --bx;
+    }
+    const bool second = bx >= blocks1;
     RowSrc src = second ? s2.src : s1.src;
     uint32_t* out = second ? s2.out : s1.out;
     const uint32_t rows = second ? s2.rows : s1.rows;
@@ -204,7 +238,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(FwdSeg s1, FwdSeg s
     }
     const int tr = threadIdx.x % G::TPR;
     const int rsub = threadIdx.x / G::TPR;
-    const size_t row = (size_t)(second ? blockIdx.x - blocks1 : blockIdx.x) * G::RPB + rsub;
+    const size_t row = (size_t)(second ? bx - blocks1 : bx) * G::RPB + rsub;
     const bool valid = row < rows;
     uint32_t* S = sh + rsub * NPRIMES * G::M;
     // produce the row once, reduce into the 8 primes
@@ -260,11 +294,12 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(FwdSeg s1, FwdSeg s
 }
 
 template <int E, int TP>
-static void launch_reg(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, hipStream_t s, const Jobs& jb) {
+static void launch_reg(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, hipStream_t s, const Jobs& jb,
+                       const InvTask& it) {
     using G = RegNtt<E, TP>;
     const size_t b1 = (s1.rows + G::RPB - 1) / G::RPB, b2 = (s2.rows + G::RPB - 1) / G::RPB;
-    hipLaunchKernelGGL((k_ntt_fwd_reg<E, TP>), dim3((unsigned)(b1 + b2), jb.n), dim3(NTT_THREADS), G::LDS_BYTES, s,
-                       s1, s2, (uint32_t)b1, tw, jb.stride);
+    hipLaunchKernelGGL((k_ntt_fwd_reg<E, TP>), dim3((unsigned)(b1 + b2 + (it.chal ? 1 : 0)), jb.n), dim3(NTT_THREADS),
+                       G::LDS_BYTES, s, s1, s2, (uint32_t)b1, tw, jb.stride, it);
 }
 
 // DAMASCUS_NTT_KERNEL=layer keeps every degree on the per-layer kernel (for comparisons)
@@ -277,25 +312,26 @@ static bool reg_ntt_enabled() {
 }
 
 void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, const RowSrc& src2, size_t rows2,
-                          uint32_t* out2, int log_m, const NttTables& tw, hipStream_t s, const Jobs& jb) {
+                          uint32_t* out2, int log_m, const NttTables& tw, hipStream_t s, const Jobs& jb,
+                          const InvTask& it) {
     if (rows1 + rows2 == 0 || jb.n == 0) return;
     const FwdSeg s1{src1, out1, (uint32_t)rows1}, s2{src2, out2, (uint32_t)rows2};
     if (reg_ntt_enabled()) {
         switch (log_m) {
-            case 6: return launch_reg<8, 8>(s1, s2, tw, s, jb);
-            case 7: return launch_reg<16, 8>(s1, s2, tw, s, jb);
-            case 8: return launch_reg<16, 16>(s1, s2, tw, s, jb);
-            case 9: return launch_reg<32, 16>(s1, s2, tw, s, jb);
-            case 10: return launch_reg<32, 32>(s1, s2, tw, s, jb);
-            case 11: return launch_reg<64, 32>(s1, s2, tw, s, jb);
+            case 6: return launch_reg<8, 8>(s1, s2, tw, s, jb, it);
+            case 7: return launch_reg<16, 8>(s1, s2, tw, s, jb, it);
+            case 8: return launch_reg<16, 16>(s1, s2, tw, s, jb, it);
+            case 9: return launch_reg<32, 16>(s1, s2, tw, s, jb, it);
+            case 10: return launch_reg<32, 32>(s1, s2, tw, s, jb, it);
+            case 11: return launch_reg<64, 32>(s1, s2, tw, s, jb, it);
             default: break;
         }
     }
     const int m = 1 << log_m;
     const int pg = std::min(NPRIMES, LDS_WORDS >> log_m);
     const size_t lds = (size_t)pg * m * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_ntt_fwd_rows, dim3((unsigned)(rows1 + rows2), jb.n), dim3(NTT_THREADS), lds, s, s1, s2,
-                       (uint32_t)rows1, log_m, tw, jb.stride);
+    hipLaunchKernelGGL(k_ntt_fwd_rows, dim3((unsigned)(rows1 + rows2 + (it.chal ? 1 : 0)), jb.n), dim3(NTT_THREADS),
+                       lds, s, s1, s2, (uint32_t)rows1, log_m, tw, jb.stride, it);
 }
 
 void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out, const NttTables& tw,
@@ -976,7 +1012,8 @@ __global__ void __launch_bounds__(256) k_lift_check(const uint32_t* __restrict__
         res_a = jp(res_a, jo);
 #pragma unroll
         for (int p = 0; p < NPRIMES; ++p) r[p] = res_a[(k * NPRIMES + p) * m + i];
-        const zq v = crt_lift_residues(r);
+        zq v = crt_lift_residues(r);
+        if (cu.unscale) v = zq_mul(v, *jp(cu.unscale, jo));
         if (!cu.c) {
             store_zq(jp(com_out, jo) + gid, v);
             if (com_out2) store_zq(jp(com_out2, jo) + gid, v);
@@ -997,7 +1034,9 @@ __global__ void __launch_bounds__(256) k_lift_check(const uint32_t* __restrict__
         res_b = jp(res_b, jo);
 #pragma unroll
         for (int p = 0; p < NPRIMES; ++p) r[p] = res_b[(sl * NPRIMES + p) * m + c];
-        store_zq(jp(out_lr, jo) + g, crt_lift_residues(r));
+        zq v = crt_lift_residues(r);
+        if (cu.unscale) v = zq_mul(v, *jp(cu.unscale, jo));
+        store_zq(jp(out_lr, jo) + g, v);
     }
 }
 

@@ -287,16 +287,19 @@ static double fwd_row_work(int log_m, int mode) {
     const double m = (double)((size_t)1 << log_m);
     double w = NPRIMES * m * 42.0 + NPRIMES * (m / 2) * log_m * 19.0;
     if (mode == ROW_GEN) w += m * 860.0;
-    if (mode == ROW_VFOLD || mode == ROW_PFOLD_W || mode == ROW_PFOLD_G) w += m * 204.0;
+    if (mode == ROW_VFOLD || mode == ROW_PFOLD_W || mode == ROW_PFOLD_G || mode == ROW_VFOLD_L ||
+        mode == ROW_PFOLD_GL)
+        w += m * 204.0;
     return w;
 }
 
 // forward transforms go through here so that kernel timing can bracket them; two row sets of
 // one degree share a launch
 static void fwd_rows2(dmx_ctx* ctx, const RowSrc& src1, size_t rows1, uint32_t* out1, const RowSrc& src2,
-                      size_t rows2, uint32_t* out2, int log_m, hipStream_t s, const Jobs& jb) {
+                      size_t rows2, uint32_t* out2, int log_m, hipStream_t s, const Jobs& jb,
+                      const InvTask& it = InvTask{}) {
     if (!ctx->timing || rows1 + rows2 == 0 || (ctx->timing_log_m >= 0 && ctx->timing_log_m != log_m)) {
-        launch_ntt_fwd_rows2(src1, rows1, out1, src2, rows2, out2, log_m, ctx->tw, s, jb);
+        launch_ntt_fwd_rows2(src1, rows1, out1, src2, rows2, out2, log_m, ctx->tw, s, jb, it);
         return;
     }
     std::lock_guard<std::mutex> g(ctx->mu);
@@ -304,7 +307,7 @@ static void fwd_rows2(dmx_ctx* ctx, const RowSrc& src1, size_t rows1, uint32_t* 
                                         (double)rows2 * fwd_row_work(log_m, src2.mode));
     TimedLaunch t{ctx->take_event(), ctx->take_event(), work};
     (void)hipEventRecord(t.a, s);
-    launch_ntt_fwd_rows2(src1, rows1, out1, src2, rows2, out2, log_m, ctx->tw, s, jb);
+    launch_ntt_fwd_rows2(src1, rows1, out1, src2, rows2, out2, log_m, ctx->tw, s, jb, it);
     (void)hipEventRecord(t.b, s);
     ctx->launches.push_back(t);
 }
@@ -637,7 +640,8 @@ struct Transcriber {
     uint8_t* file_id() const { return seeds + 32; }
     // challenge (transcript.rs:42-63) written as (x, x^-1) to out, for every job
     void challenge(const char* label, uint64_t round, const zq* c, size_t lc, const zq* l, size_t ll,
-                   const zq* r, size_t lr, zq* out, uint32_t* flags, hipStream_t s, const Jobs& jb) {
+                   const zq* r, size_t lr, zq* out, uint32_t* flags, hipStream_t s, const Jobs& jb,
+                   bool inverse = true) {
         TranscriptMsg t;
         t.epoch_seed = epoch();
         t.file_id = file_id();
@@ -651,7 +655,7 @@ struct Transcriber {
         t.lens[0] = (uint32_t)lc;
         t.lens[1] = (uint32_t)ll;
         t.lens[2] = (uint32_t)lr;
-        transcript_challenge(t, IVW, 0, msg.as<uint8_t>(), cvs.as<uint32_t>(), out, flags, s, jb);
+        transcript_challenge(t, IVW, 0, msg.as<uint8_t>(), cvs.as<uint32_t>(), out, flags, s, jb, inverse);
     }
 };
 
@@ -677,7 +681,12 @@ struct dmx_prover {
     size_t hash_cap = 0;     // bound on every job's file length (sizes the file hash launch)
     DevBuf store;            // the workspace, unless a batch owns it
     Slice meta, file, W_a, W_b, G_a, G_b, What_a, What_b, Ghat_a, Ghat_b, res, part, lifted, C, Ct, com0, recs,
-        scal, flags;
+        scal, unsc, flags;
+    // Challenge inverses are off the critical path: the folds take the challenge itself and
+    // the generator state is kept scaled (x g_L + g_R = x (g_L + x^-1 g_R), y g_e + X g_o =
+    // y (g_e + y^-1 X g_o)); stage t's lifts multiply by unsc[t+1], the product of every
+    // inverse so far, which a workgroup of the stage's forward-transform launch computes while
+    // the others transform (InvTask).
     std::vector<size_t> rec_off;  // offset (coefficients) of round j's record block
     Transcriber tr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -700,6 +709,7 @@ struct dmx_prover {
     zq* rec_rpoly(size_t j) const { return rec_lpoly(j) + (size_t)K * (m_at(j) / 2); }
     uint32_t* res_at(size_t words) const { return res.as<uint32_t>() + words; }
     zq* sc(size_t i) const { return scal.as<zq>() + i; }
+    zq* un(size_t t) const { return unsc.as<zq>() + t; }
     const uint64_t* dlen() const { return meta.as<uint64_t>(); }
     const uint32_t* gseed() const { return reinterpret_cast<const uint32_t*>(meta.as<uint8_t>() + META_GSEED); }
 };
@@ -733,6 +743,7 @@ static void prover_plan(dmx_prover* p, Layout& L, size_t file_cap) {
     }
     L.add(p->recs, total * 16);
     L.add(p->scal, (size_t)4 * (p->d + 1) * 16);
+    L.add(p->unsc, (size_t)2 * (p->d + 1) * 16);
     L.add(p->flags, 256);
     p->tr.plan(L, N, p->hash_cap);
 }
@@ -798,7 +809,9 @@ static int prover_round(dmx_prover* p, size_t j) {
     uint32_t* flags = p->flags.as<uint32_t>();
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[0], s));
     // ---- vector stage ----
-    p->tr.challenge("vec", j, p->C.as<zq>(), m, p->rec_lvec(j), m, p->rec_rvec(j), m, p->sc(4 * j), flags, s, jb);
+    p->tr.challenge("vec", j, p->C.as<zq>(), m, p->rec_lvec(j), m, p->rec_rvec(j), m, p->sc(4 * j), flags, s, jb,
+                    false);
+
     RowSrc vw{};
     vw.mode = ROW_VFOLD;
     vw.a = p->W_a.as<zq>();
@@ -806,35 +819,39 @@ static int prover_round(dmx_prover* p, size_t j) {
     vw.scal = p->sc(4 * j);  // m_L + x * m_R
     vw.dst = p->W_b.as<zq>();
     RowSrc vg{};
-    vg.mode = ROW_VFOLD;
+    vg.mode = ROW_VFOLD_L;
     vg.a = (j == 0 && p->G0_src) ? p->G0_src : p->G_a.as<zq>();
     vg.a_shared = j == 0 && p->G0_src;
     vg.b_off = h * K;
-    vg.scal = p->sc(4 * j + 1);  // g_L + x^-1 * g_R
+    vg.scal = p->sc(4 * j);  // x g_L + g_R = x (g_L + x^-1 g_R)
     vg.dst = p->G_b.as<zq>();
-    fwd_rows2(p->ctx, vw, h, p->What_b.as<uint32_t>(), vg, h * K, p->Ghat_b.as<uint32_t>(), lm, s, jb);
+    fwd_rows2(p->ctx, vw, h, p->What_b.as<uint32_t>(), vg, h * K, p->Ghat_b.as<uint32_t>(), lm, s, jb,
+              InvTask{p->sc(4 * j), j ? p->un(2 * j) : nullptr, p->un(2 * j + 1)});
     // invariant at degree m and the poly cross terms at degree m/2 from one pass; the lift
     // checks the invariant against C_t = C + x^-1 L + x R (written to Ct) and emits L/R_poly
     const size_t s0 = (size_t)K * NPRIMES * m;
     launch_mac_split_inv(p->What_b.as<uint32_t>(), p->Ghat_b.as<uint32_t>(), h, lm, p->res_at(0), p->res_at(s0),
                          p->part.as<uint32_t>(), tw, s, jb);
-    launch_lift_check(p->res_at(0), lm, ComUpd{p->C.as<zq>(), p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j), 0},
+    launch_lift_check(p->res_at(0), lm,
+                      ComUpd{p->C.as<zq>(), p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j), 0, p->un(2 * j + 1)},
                       p->Ct.as<zq>(), nullptr, flags, vec_bit(j), p->res_at(s0), p->rec_lpoly(j), 2 * K, lm - 1, s, jb);
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[1], s));
     // ---- poly stage ----
     p->tr.challenge("poly", j, p->Ct.as<zq>(), m, p->rec_lpoly(j), h, p->rec_rpoly(j), h, p->sc(4 * j + 2), flags, s,
-                    jb);
+                    jb, false);
+
     RowSrc pw{};
     pw.mode = ROW_PFOLD_W;
     pw.a = p->W_b.as<zq>();
     pw.scal = p->sc(4 * j + 2);  // e + y * o
     pw.dst = p->W_a.as<zq>();
     RowSrc pg{};
-    pg.mode = ROW_PFOLD_G;
+    pg.mode = ROW_PFOLD_GL;
     pg.a = p->G_b.as<zq>();
-    pg.scal = p->sc(4 * j + 3);  // g_e + y^-1 * X g_o
+    pg.scal = p->sc(4 * j + 2);  // y g_e + X g_o = y (g_e + y^-1 X g_o)
     pg.dst = p->G_a.as<zq>();
-    fwd_rows2(p->ctx, pw, h, p->What_a.as<uint32_t>(), pg, h * K, p->Ghat_a.as<uint32_t>(), lm - 1, s, jb);
+    fwd_rows2(p->ctx, pw, h, p->What_a.as<uint32_t>(), pg, h * K, p->Ghat_a.as<uint32_t>(), lm - 1, s, jb,
+              InvTask{p->sc(4 * j + 2), p->un(2 * j + 1), p->un(2 * j + 2)});
     // post-poly invariant, plus the next round's vector cross terms from the same pass
     const size_t s1 = (size_t)K * NPRIMES * h;
     const bool next = h > 1;
@@ -843,7 +860,8 @@ static int prover_round(dmx_prover* p, size_t j) {
                          s, jb);
     // the lift checks the invariant against C' = even(C_t) + y^-1 L + y R (written to C)
     launch_lift_check(p->res_at(0), lm - 1,
-                      ComUpd{p->Ct.as<zq>(), p->rec_lpoly(j), p->rec_rpoly(j), p->sc(4 * j + 2), 1}, p->C.as<zq>(),
+                      ComUpd{p->Ct.as<zq>(), p->rec_lpoly(j), p->rec_rpoly(j), p->sc(4 * j + 2), 1, p->un(2 * j + 2)},
+                      p->C.as<zq>(),
                       nullptr, flags, poly_bit(j), next ? p->res_at(s1) : nullptr, p->rec_lvec(j + 1),
                       next ? 2 * K : 0, lm - 1, s, jb);
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[2], s));
