@@ -80,7 +80,6 @@ void set_device_constants(const PrimeConsts* pc, const CrtConsts* crt, hipStream
 // forward negacyclic NTT of `rows` rows of degree 2^log_m, all 8 primes; out[(row*8+p)*m + c]
 void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out,
                          const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
-// two row sets of the same degree in one launch (segment 2 may be empty)
 // A challenge inverse carried by a forward-transform launch: an extra workgroup 0 sets
 // chal[1] = chal[0]^-1 and *u_out = *u_prev * chal[0]^-1 (u_prev null: 1) while the other
 // workgroups transform rows. chal null: no task.
@@ -89,6 +88,7 @@ struct InvTask {
     const zq* u_prev = nullptr;
     zq* u_out = nullptr;
 };
+// two row sets of the same degree in one launch (segment 2 may be empty)
 void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, const RowSrc& src2, size_t rows2,
                           uint32_t* out2, int log_m, const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{},
                           const InvTask& it = InvTask{});
@@ -141,6 +141,28 @@ struct ComUpd {
     int poly;
     const zq* unscale = nullptr;  // when set, every lifted value is multiplied by *unscale first
 };
+// One prover stage at degree <= 32 in one launch (see k_stage_small): fold, transforms, MAC
+// pass, inverse transforms, lifts, the invariant check and the challenge inverse of `it`
+// (it.chal[1], and it.u_out = *it.u_prev * chal^-1, the lifts' unscale). ctr: the job's stage
+// counter, zero at the start of the proof; this launch raises it to `target`.
+constexpr uint32_t STAGE_WAIT_EXPIRED = 0x20000000u;  // flags bit: a stage waited too long
+struct SmallStage {
+    int poly;        // 0: vector stage (split pass), 1: poly stage (pairs pass)
+    int log_m;       // degree of the stage's input rows
+    uint32_t rows;   // rows of the MAC pass: vector stage m/2, poly stage h
+    RowSrc w, g;     // witness and generator folds (dst: the next Z_q rows)
+    ComUpd cu;       // the invariant's expected update (cu.unscale unused: u is formed here)
+    zq* com_out;
+    zq* out_lr;      // cross terms: L slices then R slices, [2K][2^log_mb]
+    int cross;       // poly stage: cross terms are emitted
+    uint32_t* flags;
+    uint32_t bit;
+    InvTask it;
+    uint32_t* ctr;
+    uint32_t target;
+};
+void launch_stage_small(const SmallStage& st, const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
+
 // lift K commitment slices (checking them against the update of `cu`, written to com_out) and
 // lr_slices cross-term slices (to out_lr) in one launch; see k_lift_check
 void launch_lift_check(const uint32_t* res_a, int log_ma, const ComUpd& cu, zq* com_out, zq* com_out2,

@@ -997,14 +997,32 @@ void launch_crt_lift(const uint32_t* res, zq* out, size_t slices, int log_m, hip
 // c' = c + s^-1 l + s r (poly: even(c) + ...), which is written to com_out, and a mismatch sets
 // `bit` in flags (the folding invariant); with cu.c == nullptr the lift itself is written to
 // com_out (and com_out2). Part B: lr_slices slices of degree 2^log_mb lifted to out_lr.
-__global__ void __launch_bounds__(256) k_lift_check(const uint32_t* __restrict__ res_a, int log_ma, ComUpd cu,
-                                                    zq* com_out, zq* com_out2, uint32_t* flags, uint32_t bit,
-                                                    const uint32_t* __restrict__ res_b, zq* __restrict__ out_lr,
-                                                    uint32_t lr_slices, int log_mb, size_t js) {
-    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t ta = (size_t)K << log_ma, tb = (size_t)lr_slices << log_mb;
-    if (gid >= ta + tb) return;
-    const size_t jo = job_off(js);
+// a lifted commitment coefficient v (unscaled) at gid = k * 2^log_ma + i: stored when cu.c is
+// null, else checked against the update of `cu` with the scalars (s, sinv), which is stored
+__device__ __forceinline__ void commit_check(size_t gid, const zq& v, int log_ma, const ComUpd& cu, const zq& s,
+                                             const zq& sinv, zq* com_out, zq* com_out2, uint32_t* flags,
+                                             uint32_t bit, size_t jo) {
+    if (!cu.c) {
+        store_zq(jp(com_out, jo) + gid, v);
+        if (com_out2) store_zq(jp(com_out2, jo) + gid, v);
+        return;
+    }
+    const size_t m = (size_t)1 << log_ma;
+    const size_t k = gid >> log_ma, i = gid & (m - 1);
+    const zq* c = jp(cu.c, jo);
+    const zq base = cu.poly ? load_zq(c + k * (2 * m) + 2 * i) : load_zq(c + gid);
+    zq e = zq_add(base, zq_mul(load_zq(jp(cu.l, jo) + gid), sinv));
+    e = zq_add(e, zq_mul(load_zq(jp(cu.r, jo) + gid), s));
+    store_zq(jp(com_out, jo) + gid, e);
+    if (!zq_eq(v, e)) atomicOr(jp(flags, jo), bit);
+}
+
+// one output of a lift launch: gid < K * 2^log_ma lifts a commitment coefficient (and checks it
+// against the update of `cu`, or stores it when cu.c is null), the rest lift cross-term slices
+__device__ __forceinline__ void lift_check_elem(size_t gid, const uint32_t* res_a, int log_ma, const ComUpd& cu,
+                                                zq* com_out, zq* com_out2, uint32_t* flags, uint32_t bit,
+                                                const uint32_t* res_b, zq* out_lr, int log_mb, size_t jo) {
+    const size_t ta = (size_t)K << log_ma;
     uint32_t r[NPRIMES];
     if (gid < ta) {
         const size_t m = (size_t)1 << log_ma;
@@ -1014,19 +1032,13 @@ __global__ void __launch_bounds__(256) k_lift_check(const uint32_t* __restrict__
         for (int p = 0; p < NPRIMES; ++p) r[p] = res_a[(k * NPRIMES + p) * m + i];
         zq v = crt_lift_residues(r);
         if (cu.unscale) v = zq_mul(v, *jp(cu.unscale, jo));
-        if (!cu.c) {
-            store_zq(jp(com_out, jo) + gid, v);
-            if (com_out2) store_zq(jp(com_out2, jo) + gid, v);
-            return;
+        zq s = zq_zero(), sinv = zq_zero();
+        if (cu.c) {
+            const zq* scal = jp(cu.scal, jo);
+            s = scal[0];
+            sinv = scal[1];
         }
-        const zq* c = jp(cu.c, jo);
-        const zq* scal = jp(cu.scal, jo);
-        const zq s = scal[0], sinv = scal[1];
-        const zq base = cu.poly ? load_zq(c + k * (2 * m) + 2 * i) : load_zq(c + gid);
-        zq e = zq_add(base, zq_mul(load_zq(jp(cu.l, jo) + gid), sinv));
-        e = zq_add(e, zq_mul(load_zq(jp(cu.r, jo) + gid), s));
-        store_zq(jp(com_out, jo) + gid, e);
-        if (!zq_eq(v, e)) atomicOr(jp(flags, jo), bit);
+        commit_check(gid, v, log_ma, cu, s, sinv, com_out, com_out2, flags, bit, jo);
     } else {
         const size_t g = gid - ta;
         const size_t m = (size_t)1 << log_mb;
@@ -1040,6 +1052,16 @@ __global__ void __launch_bounds__(256) k_lift_check(const uint32_t* __restrict__
     }
 }
 
+__global__ void __launch_bounds__(256) k_lift_check(const uint32_t* __restrict__ res_a, int log_ma, ComUpd cu,
+                                                    zq* com_out, zq* com_out2, uint32_t* flags, uint32_t bit,
+                                                    const uint32_t* __restrict__ res_b, zq* __restrict__ out_lr,
+                                                    uint32_t lr_slices, int log_mb, size_t js) {
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t ta = (size_t)K << log_ma, tb = (size_t)lr_slices << log_mb;
+    if (gid >= ta + tb) return;
+    lift_check_elem(gid, res_a, log_ma, cu, com_out, com_out2, flags, bit, res_b, out_lr, log_mb, job_off(js));
+}
+
 void launch_lift_check(const uint32_t* res_a, int log_ma, const ComUpd& cu, zq* com_out, zq* com_out2,
                        uint32_t* flags, uint32_t bit, const uint32_t* res_b, zq* out_lr, size_t lr_slices, int log_mb,
                        hipStream_t s, const Jobs& jb) {
@@ -1047,6 +1069,242 @@ void launch_lift_check(const uint32_t* res_a, int log_ma, const ComUpd& cu, zq* 
     if (!total || jb.n == 0) return;
     hipLaunchKernelGGL(k_lift_check, dim3((unsigned)((total + 255) / 256), jb.n), dim3(256), 0, s, res_a, log_ma, cu,
                        com_out, com_out2, flags, bit, res_b, out_lr, (uint32_t)lr_slices, log_mb, jb.stride);
+}
+
+// ---------------------------------------------------------------------------------------
+// One prover stage at small degree in one launch. At m <= 32 the stage's work is a few hundred
+// thousand instructions, so the five-launch chain (fold + transform, MAC pass, finish, lift)
+// costs its launch latencies rather than its work. Workgroup k (of K) folds the stage's witness
+// rows and the generator rows of coordinate k, reduces and transforms them in LDS, runs the
+// MAC pass of coordinate k (split or pairs), transforms the sums back and lifts coordinate k of
+// the invariant commitment (checked as k_lift_check does) and of the cross terms: every
+// residue of coordinate k is in its own LDS. Workgroup 0 computes the challenge's inverse
+// meanwhile and publishes it with a device-scope release increment of the job's stage counter;
+// the coordinate workgroups need it only for the lifts, so by then it is normally there (their
+// wait is bounded and flags the job on expiry). Same values as the multi-launch chain.
+// ---------------------------------------------------------------------------------------
+constexpr int SMALL_THREADS = 1024;
+
+// forward layers (as k_ntt_fwd_rows) on cnt slices of 2^log_n words; slice s uses prime s % 8
+__device__ void lds_ntt_fwd_slices_pp(uint32_t* sh, int cnt, int log_n, const NttTables& tw) {
+    const int n = 1 << log_n, half = n >> 1;
+    for (int len = half; len >= 1; len >>= 1) {
+        const int lg = __builtin_ctz(len);
+        for (int t = threadIdx.x; t < cnt * half; t += SMALL_THREADS) {
+            const int sl = t >> (log_n - 1), b = t & (half - 1);
+            const int pi = sl & (NPRIMES - 1);
+            const int blk = b >> lg;
+            const int j = (blk << (lg + 1)) | (b & (len - 1));
+            const int kz = (half >> lg) + blk;
+            const uint32_t p = c_pc[pi].p;
+            uint32_t* q = sh + sl * n;
+            const uint32_t u = q[j];
+            const uint32_t v = mulc(q[j + len], tw.zeta_m[pi * TW_SIZE + kz], tw.zeta_mp[pi * TW_SIZE + kz], p);
+            q[j] = addp(u, v, p);
+            q[j + len] = subp(u, v, p);
+        }
+        __syncthreads();
+    }
+}
+
+// inverse layers (as lds_intt_slices, no scaling) on cnt slices; slice s uses prime s % 8
+__device__ void lds_intt_slices_pp(uint32_t* sh, int cnt, int log_n, const NttTables& tw) {
+    const int n = 1 << log_n, half = n >> 1;
+    for (int len = 1; len < n; len <<= 1) {
+        const int lg = __builtin_ctz(len);
+        for (int t = threadIdx.x; t < cnt * half; t += SMALL_THREADS) {
+            const int sl = t >> (log_n - 1), b = t & (half - 1);
+            const int pi = sl & (NPRIMES - 1);
+            const int blk = b >> lg;
+            const int j = (blk << (lg + 1)) | (b & (len - 1));
+            const int kz = (half >> lg) + blk;
+            const uint32_t p = c_pc[pi].p;
+            uint32_t* q = sh + sl * n;
+            const uint32_t u = q[j], v = q[j + len];
+            q[j] = addp(u, v, p);
+            q[j + len] = mulc(subp(u, v, p), tw.izeta_m[pi * TW_SIZE + kz], tw.izeta_mp[pi * TW_SIZE + kz], p);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(SMALL_THREADS) k_stage_small(SmallStage st, NttTables tw, size_t js) {
+    extern __shared__ uint32_t sh[];
+    __shared__ zq s_x, s_xi, s_u;
+    const size_t jo = job_off(js);
+    if (blockIdx.x == 0) {  // the challenge inverse, published by a release increment
+        if (threadIdx.x < 64) {
+            zq* chal = jp(st.it.chal, jo);
+            const zq x = chal[0];  // every lane loads it: a per-lane register, as the inverse wants
+            const zq xi = zq_inv_divsteps(x);
+            const zq u = st.it.u_prev ? zq_mul(*jp(st.it.u_prev, jo), xi) : xi;
+            if (threadIdx.x == 0) {
+                chal[1] = xi;
+                *jp(st.it.u_out, jo) = u;
+                __hip_atomic_fetch_add(jp(st.ctr, jo), 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        return;
+    }
+    const int poly = st.poly;
+    const int lnd = poly ? st.log_m - 1 : st.log_m;  // degree of the transforms
+    const int nd = 1 << lnd;
+    const uint32_t H = st.rows;
+    const int log_ma = lnd, log_mb = poly ? lnd : lnd - 1;  // commitment / cross-term degrees
+    const int nb = 1 << log_mb;
+    const bool lr = !poly || st.cross;
+    const uint32_t k = blockIdx.x - 1;
+    RowSrc w = st.w, g = st.g;
+    if (!w.a_shared) w.a = jp(w.a, jo);
+    if (!g.a_shared) g.a = jp(g.a, jo);
+    w.dst = jp(w.dst, jo);
+    g.dst = jp(g.dst, jo);
+    w.scal = jp(w.scal, jo);
+    g.scal = jp(g.scal, jo);
+    const uint32_t key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // 1) rows 0..H-1 witness, H..2H-1 generators of coordinate k, as residues T[row][p][c]
+    uint32_t* T = sh;
+    for (uint32_t e = threadIdx.x; e < (2 * H) << lnd; e += SMALL_THREADS) {
+        const uint32_t rl = e >> lnd, c = e & (nd - 1);
+        zq x;
+        if (rl < H) {
+            x = row_value(w, key, rl, c, lnd);
+            if (k == 0 && w.dst) store_zq(w.dst + ((size_t)rl << lnd) + c, x);
+        } else {
+            const size_t grow = (size_t)(rl - H) * K + k;
+            x = row_value(g, key, grow, c, lnd);
+            if (g.dst) store_zq(g.dst + (grow << lnd) + c, x);
+        }
+#pragma unroll
+        for (int q = 0; q < NPRIMES; ++q) T[((rl * NPRIMES + q) << lnd) + c] = zq_mod_p_centred(x, c_pc[q]);
+    }
+    __syncthreads();
+    // 2) forward transforms
+    lds_ntt_fwd_slices_pp(T, (int)(2 * H) * NPRIMES, lnd, tw);
+    // 3) the MAC pass of coordinate k: O = commitment slots [p][nd], then L [p][nb], R [p][nb]
+    uint32_t* O = T + ((size_t)(2 * H) * NPRIMES << lnd);
+    uint32_t* OL = O + (NPRIMES << lnd);
+    uint32_t* OR = OL + (NPRIMES << log_mb);
+    const uint32_t* Wt = T;
+    const uint32_t* Gt = T + ((size_t)H * NPRIMES << lnd);
+    if (!poly) {  // split: (S00, S11, S01, S10) per pair, as k_mac_split4 and its finish
+        const int hm = nd >> 1;
+        for (int e = threadIdx.x; e < NPRIMES * hm; e += SMALL_THREADS) {
+            const int p = e / hm, t = e - p * hm;
+            Acc96 s00, s11, s01, s10;
+            s00.zero();
+            s11.zero();
+            s01.zero();
+            s10.zero();
+            for (uint32_t i = 0; i < H; ++i) {
+                const uint32_t* wr = Wt + ((i * NPRIMES + p) << lnd) + 2 * t;
+                const uint32_t* gr = Gt + ((i * NPRIMES + p) << lnd) + 2 * t;
+                s00.mac(wr[0], gr[0]);
+                s11.mac(wr[1], gr[1]);
+                s01.mac(wr[0], gr[1]);
+                s10.mac(wr[1], gr[0]);
+            }
+            const PrimeConsts pc = c_pc[p];
+            const uint32_t P = pc.p;
+            const uint32_t a00 = reduce_u96(s00.lo, s00.hi, pc), a11 = reduce_u96(s11.lo, s11.hi, pc);
+            const uint32_t a01 = reduce_u96(s01.lo, s01.hi, pc), a10 = reduce_u96(s10.lo, s10.hi, pc);
+            O[(p << lnd) + 2 * t] = a00;
+            O[(p << lnd) + 2 * t + 1] = a11;
+            const uint32_t d = subp(a00, a11, P);
+            const uint32_t Lp = addp(subp(d, a01, P), a10, P);
+            const uint32_t Rp = subp(addp(d, a01, P), a10, P);
+            const uint32_t kz = (uint32_t)hm + t;
+            const uint32_t i4m = tw.invn_m[p * (MAX_LOG_N + 1) + 2], i4mp = tw.invn_mp[p * (MAX_LOG_N + 1) + 2];
+            OL[(p << log_mb) + t] =
+                mulc(mulc(Lp, tw.zeta_m[p * TW_SIZE + kz], tw.zeta_mp[p * TW_SIZE + kz], P), i4m, i4mp, P);
+            OR[(p << log_mb) + t] =
+                mulc(mulc(Rp, tw.izeta_m[p * TW_SIZE + kz], tw.izeta_mp[p * TW_SIZE + kz], P), i4m, i4mp, P);
+        }
+    } else {  // pairs: the invariant over every row, or with cross terms over the two halves
+        const uint32_t hh = st.cross ? H / 2 : H;
+        for (int e = threadIdx.x; e < (NPRIMES << lnd); e += SMALL_THREADS) {
+            const int p = e >> lnd, c = e & (nd - 1);
+            Acc96 inv, L, R;
+            inv.zero();
+            L.zero();
+            R.zero();
+            for (uint32_t i = 0; i < hh; ++i) {
+                const uint32_t w0 = Wt[((i * NPRIMES + p) << lnd) + c], g0 = Gt[((i * NPRIMES + p) << lnd) + c];
+                inv.mac(w0, g0);
+                if (st.cross) {
+                    const uint32_t w1 = Wt[(((i + hh) * NPRIMES + p) << lnd) + c];
+                    const uint32_t g1 = Gt[(((i + hh) * NPRIMES + p) << lnd) + c];
+                    inv.mac(w1, g1);
+                    L.mac(g1, w0);
+                    R.mac(g0, w1);
+                }
+            }
+            const PrimeConsts pc = c_pc[p];
+            O[e] = reduce_u96(inv.lo, inv.hi, pc);
+            if (st.cross) {
+                OL[e] = reduce_u96(L.lo, L.hi, pc);
+                OR[e] = reduce_u96(R.lo, R.hi, pc);
+            }
+        }
+    }
+    __syncthreads();
+    // 4) inverse transforms, scaled by the inverse degrees
+    lds_intt_slices_pp(O, NPRIMES, log_ma, tw);
+    if (lr) lds_intt_slices_pp(OL, 2 * NPRIMES, log_mb, tw);
+    for (int e = threadIdx.x; e < (NPRIMES << log_ma); e += SMALL_THREADS) {
+        const int p = e >> log_ma;
+        O[e] = mulc(O[e], tw.invn_m[p * (MAX_LOG_N + 1) + log_ma], tw.invn_mp[p * (MAX_LOG_N + 1) + log_ma], c_pc[p].p);
+    }
+    if (lr) {
+        for (int e = threadIdx.x; e < (2 * NPRIMES << log_mb); e += SMALL_THREADS) {
+            const int p = (e >> log_mb) & (NPRIMES - 1);
+            OL[e] = mulc(OL[e], tw.invn_m[p * (MAX_LOG_N + 1) + log_mb], tw.invn_mp[p * (MAX_LOG_N + 1) + log_mb],
+                         c_pc[p].p);
+        }
+    }
+    // 5) wait (bounded) for the inverse, then lift coordinate k, unscaled by u
+    if (threadIdx.x == 0) {
+        uint32_t spins = 0;
+        while (__hip_atomic_load(jp(st.ctr, jo), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < st.target) {
+            if (++spins > (1u << 22)) {
+                atomicOr(jp(st.flags, jo), STAGE_WAIT_EXPIRED);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        s_x = jp(st.it.chal, jo)[0];
+        s_xi = jp(st.it.chal, jo)[1];
+        s_u = *jp(st.it.u_out, jo);
+    }
+    __syncthreads();
+    const zq u = s_u, cx = s_x, cxi = s_xi;
+    const int na = 1 << log_ma;
+    const int cnt = na + (lr ? 2 * nb : 0);
+    for (int e = threadIdx.x; e < cnt; e += SMALL_THREADS) {
+        uint32_t r[NPRIMES];
+        if (e < na) {
+#pragma unroll
+            for (int p = 0; p < NPRIMES; ++p) r[p] = O[(p << log_ma) + e];
+            const zq v = zq_mul(crt_lift_residues(r), u);
+            commit_check(((size_t)k << log_ma) + e, v, log_ma, st.cu, cx, cxi, st.com_out, nullptr, st.flags, st.bit,
+                         jo);
+        } else {
+            const int f = e - na, side = f >> log_mb, c = f & (nb - 1);  // side 0: L, 1: R
+#pragma unroll
+            for (int p = 0; p < NPRIMES; ++p) r[p] = OL[(((side * NPRIMES) + p) << log_mb) + c];
+            const size_t slot = side ? K + k : k;
+            store_zq(jp(st.out_lr, jo) + (slot << log_mb) + c, zq_mul(crt_lift_residues(r), u));
+        }
+    }
+}
+
+void launch_stage_small(const SmallStage& st, const NttTables& tw, hipStream_t s, const Jobs& jb) {
+    if (jb.n == 0) return;
+    const int lnd = st.poly ? st.log_m - 1 : st.log_m;
+    const int log_mb = st.poly ? lnd : lnd - 1;
+    const size_t lds = 4 * ((2 * (size_t)st.rows * NPRIMES << lnd) + ((size_t)NPRIMES << lnd) +
+                            ((size_t)2 * NPRIMES << (log_mb > 0 ? log_mb : 0)));
+    hipLaunchKernelGGL(k_stage_small, dim3(K + 1, jb.n), dim3(SMALL_THREADS), lds, s, st, tw, jb.stride);
 }
 
 // Z_q-only row production (the verifier's generator folds, verifier.rs:141-179)

@@ -799,6 +799,19 @@ static int prover_initialize(dmx_prover* p) {
     return rc ? rc : prover_init_commit(p);
 }
 
+// Stages of degree <= 32 take one launch each (launch_stage_small); DAMASCUS_SMALL_STAGE=0 keeps
+// them on the multi-launch chain (for comparisons). Their stage counter is a word of the job's
+// flags block.
+constexpr size_t SMALL_STAGE_MAX_M = 32;
+constexpr size_t STAGE_CTR_WORD = 32;
+static bool small_stages_enabled() {
+    static const bool on = [] {
+        const char* e = getenv("DAMASCUS_SMALL_STAGE");
+        return !(e && strcmp(e, "0") == 0);
+    }();
+    return on;
+}
+
 // fold_round (prover.rs:140-313) for round j, m = N >> j: both stages and both invariants.
 static int prover_round(dmx_prover* p, size_t j) {
     hipStream_t s = p->s;
@@ -808,6 +821,11 @@ static int prover_round(dmx_prover* p, size_t j) {
     const int lm = p->d - (int)j;
     uint32_t* flags = p->flags.as<uint32_t>();
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[0], s));
+    const bool small = small_stages_enabled() && m <= SMALL_STAGE_MAX_M;
+    // the job's small stages count up on flags[STAGE_CTR_WORD] (zeroed with the flags)
+    size_t first_small = 0;
+    while ((p->N >> first_small) > SMALL_STAGE_MAX_M) ++first_small;
+    const uint32_t small_ord = small ? (uint32_t)(2 * (j - first_small)) : 0;
     // ---- vector stage ----
     p->tr.challenge("vec", j, p->C.as<zq>(), m, p->rec_lvec(j), m, p->rec_rvec(j), m, p->sc(4 * j), flags, s, jb,
                     false);
@@ -825,16 +843,35 @@ static int prover_round(dmx_prover* p, size_t j) {
     vg.b_off = h * K;
     vg.scal = p->sc(4 * j);  // x g_L + g_R = x (g_L + x^-1 g_R)
     vg.dst = p->G_b.as<zq>();
-    fwd_rows2(p->ctx, vw, h, p->What_b.as<uint32_t>(), vg, h * K, p->Ghat_b.as<uint32_t>(), lm, s, jb,
-              InvTask{p->sc(4 * j), j ? p->un(2 * j) : nullptr, p->un(2 * j + 1)});
+    const InvTask vit{p->sc(4 * j), j ? p->un(2 * j) : nullptr, p->un(2 * j + 1)};
     // invariant at degree m and the poly cross terms at degree m/2 from one pass; the lift
     // checks the invariant against C_t = C + x^-1 L + x R (written to Ct) and emits L/R_poly
     const size_t s0 = (size_t)K * NPRIMES * m;
-    launch_mac_split_inv(p->What_b.as<uint32_t>(), p->Ghat_b.as<uint32_t>(), h, lm, p->res_at(0), p->res_at(s0),
-                         p->part.as<uint32_t>(), tw, s, jb);
-    launch_lift_check(p->res_at(0), lm,
-                      ComUpd{p->C.as<zq>(), p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j), 0, p->un(2 * j + 1)},
-                      p->Ct.as<zq>(), nullptr, flags, vec_bit(j), p->res_at(s0), p->rec_lpoly(j), 2 * K, lm - 1, s, jb);
+    const ComUpd vcu{p->C.as<zq>(), p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j), 0, p->un(2 * j + 1)};
+    if (small) {
+        SmallStage st{};
+        st.poly = 0;
+        st.log_m = lm;
+        st.rows = (uint32_t)h;
+        st.w = vw;
+        st.g = vg;
+        st.cu = vcu;
+        st.com_out = p->Ct.as<zq>();
+        st.out_lr = p->rec_lpoly(j);
+        st.cross = 1;
+        st.flags = flags;
+        st.bit = vec_bit(j);
+        st.it = vit;
+        st.ctr = flags + STAGE_CTR_WORD;
+        st.target = small_ord + 1;
+        launch_stage_small(st, tw, s, jb);
+    } else {
+        fwd_rows2(p->ctx, vw, h, p->What_b.as<uint32_t>(), vg, h * K, p->Ghat_b.as<uint32_t>(), lm, s, jb, vit);
+        launch_mac_split_inv(p->What_b.as<uint32_t>(), p->Ghat_b.as<uint32_t>(), h, lm, p->res_at(0), p->res_at(s0),
+                             p->part.as<uint32_t>(), tw, s, jb);
+        launch_lift_check(p->res_at(0), lm, vcu, p->Ct.as<zq>(), nullptr, flags, vec_bit(j), p->res_at(s0),
+                          p->rec_lpoly(j), 2 * K, lm - 1, s, jb);
+    }
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[1], s));
     // ---- poly stage ----
     p->tr.challenge("poly", j, p->Ct.as<zq>(), m, p->rec_lpoly(j), h, p->rec_rpoly(j), h, p->sc(4 * j + 2), flags, s,
@@ -850,20 +887,37 @@ static int prover_round(dmx_prover* p, size_t j) {
     pg.a = p->G_b.as<zq>();
     pg.scal = p->sc(4 * j + 2);  // y g_e + X g_o = y (g_e + y^-1 X g_o)
     pg.dst = p->G_a.as<zq>();
-    fwd_rows2(p->ctx, pw, h, p->What_a.as<uint32_t>(), pg, h * K, p->Ghat_a.as<uint32_t>(), lm - 1, s, jb,
-              InvTask{p->sc(4 * j + 2), p->un(2 * j + 1), p->un(2 * j + 2)});
-    // post-poly invariant, plus the next round's vector cross terms from the same pass
+    const InvTask pit{p->sc(4 * j + 2), p->un(2 * j + 1), p->un(2 * j + 2)};
+    // post-poly invariant, plus the next round's vector cross terms from the same pass; the lift
+    // checks the invariant against C' = even(C_t) + y^-1 L + y R (written to C)
     const size_t s1 = (size_t)K * NPRIMES * h;
     const bool next = h > 1;
-    launch_mac_pairs_inv(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), h, lm - 1, p->res_at(0),
-                         next ? p->res_at(s1) : nullptr, next ? p->res_at(2 * s1) : nullptr, p->part.as<uint32_t>(), tw,
-                         s, jb);
-    // the lift checks the invariant against C' = even(C_t) + y^-1 L + y R (written to C)
-    launch_lift_check(p->res_at(0), lm - 1,
-                      ComUpd{p->Ct.as<zq>(), p->rec_lpoly(j), p->rec_rpoly(j), p->sc(4 * j + 2), 1, p->un(2 * j + 2)},
-                      p->C.as<zq>(),
-                      nullptr, flags, poly_bit(j), next ? p->res_at(s1) : nullptr, p->rec_lvec(j + 1),
-                      next ? 2 * K : 0, lm - 1, s, jb);
+    const ComUpd pcu{p->Ct.as<zq>(), p->rec_lpoly(j), p->rec_rpoly(j), p->sc(4 * j + 2), 1, p->un(2 * j + 2)};
+    if (small) {
+        SmallStage st{};
+        st.poly = 1;
+        st.log_m = lm;
+        st.rows = (uint32_t)h;
+        st.w = pw;
+        st.g = pg;
+        st.cu = pcu;
+        st.com_out = p->C.as<zq>();
+        st.out_lr = p->rec_lvec(j + 1);
+        st.cross = next ? 1 : 0;
+        st.flags = flags;
+        st.bit = poly_bit(j);
+        st.it = pit;
+        st.ctr = flags + STAGE_CTR_WORD;
+        st.target = small_ord + 2;
+        launch_stage_small(st, tw, s, jb);
+    } else {
+        fwd_rows2(p->ctx, pw, h, p->What_a.as<uint32_t>(), pg, h * K, p->Ghat_a.as<uint32_t>(), lm - 1, s, jb, pit);
+        launch_mac_pairs_inv(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), h, lm - 1, p->res_at(0),
+                             next ? p->res_at(s1) : nullptr, next ? p->res_at(2 * s1) : nullptr,
+                             p->part.as<uint32_t>(), tw, s, jb);
+        launch_lift_check(p->res_at(0), lm - 1, pcu, p->C.as<zq>(), nullptr, flags, poly_bit(j),
+                          next ? p->res_at(s1) : nullptr, p->rec_lvec(j + 1), next ? 2 * K : 0, lm - 1, s, jb);
+    }
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[2], s));
     HIP_TRY(hipGetLastError());
     return DMX_OK;
@@ -871,6 +925,7 @@ static int prover_round(dmx_prover* p, size_t j) {
 
 static int check_round_flags(size_t j, uint32_t f) {
     if (f & FLAG_ZERO_CHALLENGE) return fail(DMX_ERR_HIP, "challenge resampling exhausted");
+    if (f & STAGE_WAIT_EXPIRED) return fail(DMX_ERR_HIP, "a stage waited too long for its challenge inverse");
     if (f & vec_bit(j)) return fail(DMX_ERR_INVARIANT, "vector folding invariant failed (round %zu)", j);
     if (f & poly_bit(j)) return fail(DMX_ERR_INVARIANT, "round folding invariant failed (round %zu)", j);
     return DMX_OK;

@@ -1,5 +1,6 @@
 """GPU parity: every HIP result through the C ABI against the oracle (the CPU restatement of the
 reference), bit for bit. Integer work, so the bar is exact equality."""
+import os
 import struct
 import sys
 
@@ -358,7 +359,11 @@ def test_kernel_timing_counts_forward_transforms():
     dm.prove_verify_device(buf.ptr, len(data), want_proof=False)
     ms, n, work = ctx.kernel_timing()
     ctx.kernel_timing(False)
-    assert n == 1 + 2 * 6 and ms > 0 and work > 0  # init (witness + generators) + 2 per round
+    # init (witness + generators) + 2 per round; stages of degree <= 32 run as single launches
+    # (k_stage_small) whose transforms are not timed separately
+    small = os.environ.get("DAMASCUS_SMALL_STAGE", "1") != "0"
+    rounds_timed = sum(1 for j in range(6) if not small or (64 >> j) > 32)
+    assert n == 1 + 2 * rounds_timed and ms > 0 and work > 0
 
 
 def test_every_byte_flip_of_first_two_records_is_rejected_on_gpu():
