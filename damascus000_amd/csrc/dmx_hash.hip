@@ -149,7 +149,6 @@ __device__ __forceinline__ void tx_to_job(TranscriptMsg& t, size_t jo) {
 
 // message words [w0, w0 + nw) into win[0 .. nw), nw a multiple of 4, by the whole workgroup
 __device__ void tx_pack_lds(const TranscriptMsg& t, uint32_t w0, uint32_t nw, uint64_t* win) {
-#pragma unroll 2
     for (uint32_t g = 4 * threadIdx.x; g < nw; g += 4 * blockDim.x) {
         uint64_t o[4];
         tx_words4(t, w0 + g, o);
@@ -191,6 +190,7 @@ __device__ void chunk_cv_quad_lds(const uint64_t* win, size_t base, size_t len, 
 // zero challenge patches the counter and hashes them again.
 __global__ void __launch_bounds__(256) k_b3_tx_chunks(TranscriptMsg t, size_t len, KeyW key, uint32_t flags,
                                                      uint32_t* __restrict__ cvs, uint8_t* __restrict__ msg, size_t js) {
+    critical_path_priority();
     __shared__ __align__(16) uint64_t win[TX_WINDOW / 8];
     const size_t jo = job_off(js);
     tx_to_job(t, jo);
@@ -313,14 +313,17 @@ static void launch_b3_group_merge(const uint32_t* cvs, size_t n, const uint32_t 
                        flags, out, jb.stride, dlen, cap_len);
 }
 
-// Root merge; one workgroup of 1024 threads per job over n <= 1024 CVs. `chunk_level` says the
+// Root merge; one workgroup of 512 threads per job over n <= 1024 CVs. (Its waves need about 120
+// VGPRs each: at 1024 threads the workgroup would take a whole CU's register file and wait for
+// any concurrent kernel to leave one; at 512 it shares a CU.) `chunk_level` says the
 // CVs are chunk CVs (needed to redo the counter chunks of a zero challenge). With `dlen` the job's
 // own length picks the level: chunk values at `cvs` (<= 1024 chunks) or group values at
 // cvs + l1_off.
-__global__ void __launch_bounds__(1024) k_b3_root(uint32_t* __restrict__ cvs, size_t n, uint8_t* msg, size_t len,
+__global__ void __launch_bounds__(512) k_b3_root(uint32_t* __restrict__ cvs, size_t n, uint8_t* msg, size_t len,
                                                   KeyW key, uint32_t flags, uint32_t* digest_out, zq* chal_out,
                                                   uint32_t* err, int chunk_level, size_t js, const uint64_t* dlen,
                                                   size_t l1_off, int want_inv) {
+    critical_path_priority();
     __shared__ uint32_t sa[1024 * 8], sb[512 * 8 + 8];
     __shared__ uint32_t dig[8];
     __shared__ int again;
@@ -423,7 +426,7 @@ static void launch_b3_root(const uint32_t* cvs, size_t n, const uint8_t* msg, si
                            hipStream_t s, const Jobs& jb, const uint64_t* dlen, size_t l1_off, int want_inv = 1) {
     KeyW k;
     for (int i = 0; i < 8; ++i) k.w[i] = key[i];
-    hipLaunchKernelGGL(k_b3_root, dim3(1, jb.n), dim3(1024), 0, s, const_cast<uint32_t*>(cvs), n,
+    hipLaunchKernelGGL(k_b3_root, dim3(1, jb.n), dim3(512), 0, s, const_cast<uint32_t*>(cvs), n,
                        const_cast<uint8_t*>(msg), len, k, flags, digest_out, chal_out, err_flags, chunk_level,
                        jb.stride, dlen, l1_off, want_inv);
 }
@@ -438,6 +441,7 @@ constexpr size_t CHALLENGE_MAX_CHUNKS = 16;
 
 __global__ void __launch_bounds__(256) k_b3_tx_challenge(TranscriptMsg t, size_t len, KeyW key, uint32_t flags,
                                                         zq* chal_out, uint32_t* err, size_t js, int want_inv) {
+    critical_path_priority();
     __shared__ __align__(16) uint64_t win[TX_WINDOW / 8];
     __shared__ uint32_t sa[16 * 8], sb[8 * 8 + 8];
     __shared__ uint32_t dig[8];

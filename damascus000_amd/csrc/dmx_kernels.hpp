@@ -75,6 +75,11 @@ __device__ __forceinline__ T* jp(T* p, size_t jo) {
 }
 __device__ __forceinline__ size_t job_off(size_t js) { return (size_t)blockIdx.y * js; }
 
+// Latency-bound kernels on a proof's critical path raise their waves' issue priority above the
+// default, so a concurrent throughput kernel (the verifier's generator expansion) takes only the
+// issue slots they leave free.
+__device__ __forceinline__ void critical_path_priority() { __builtin_amdgcn_s_setprio(2); }
+
 void set_device_constants(const PrimeConsts* pc, const CrtConsts* crt, hipStream_t s);
 
 // forward negacyclic NTT of `rows` rows of degree 2^log_m, all 8 primes; out[(row*8+p)*m + c]

@@ -1129,6 +1129,7 @@ __device__ void lds_intt_slices_pp(uint32_t* sh, int cnt, int log_n, const NttTa
 }
 
 __global__ void __launch_bounds__(SMALL_THREADS) k_stage_small(SmallStage st, NttTables tw, size_t js) {
+    critical_path_priority();
     extern __shared__ uint32_t sh[];
     __shared__ zq s_x, s_xi, s_u;
     const size_t jo = job_off(js);

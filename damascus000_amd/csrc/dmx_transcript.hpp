@@ -95,19 +95,32 @@ DMX_HD void tx_head_words(const TranscriptMsg& t, uint64_t out[13]) {
     for (int j = 1; j < 8; ++j) out[5 + j] = (S[j - 1] >> (64 - sl)) | (S[j] << sl);
 }
 
+struct TxWords4 {
+    uint64_t w[4];
+};
+
+// message words w0 .. w0+3 for w0 < 13 (the few words that touch the header; kept out of line,
+// returned in registers)
+__host__ __device__ __attribute__((noinline)) inline TxWords4 tx_words4_head(const TranscriptMsg& t, uint32_t w0) {
+    uint64_t head[13];
+    tx_head_words(t, head);
+    TxWords4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        uint64_t v = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 13; ++j) v = (w0 + i == j) ? head[j] : v;  // no indexed private array
+        r.w[i] = w0 + i < 13 ? v : tx_body_msg_word(t, w0 + i);
+    }
+    return r;
+}
+
 // message words w0 .. w0+3; past the header the five body words are loaded together
 DMX_HD void tx_words4(const TranscriptMsg& t, uint32_t w0, uint64_t out[4]) {
     const uint32_t H = tx_header_len(t);
     if (8 * w0 < H) {  // w0 < 13
-        uint64_t head[13];
-        tx_head_words(t, head);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            uint64_t v = 0;
-#pragma unroll
-            for (uint32_t j = 0; j < 13; ++j) v = (w0 + i == j) ? head[j] : v;
-            out[i] = w0 + i < 13 ? v : tx_body_msg_word(t, w0 + i);
-        }
+        const TxWords4 r = tx_words4_head(t, w0);
+        for (int i = 0; i < 4; ++i) out[i] = r.w[i];
         return;
     }
     const uint32_t b = 8 * w0 - H, v = b >> 3, sh = 8 * (b & 7);
