@@ -813,7 +813,9 @@ static bool small_stages_enabled() {
 }
 
 // fold_round (prover.rs:140-313) for round j, m = N >> j: both stages and both invariants.
-static int prover_round(dmx_prover* p, size_t j) {
+// `vec_done`, when given, is recorded after the vector stage: the round's poly cross terms are
+// then in place for a verifier that follows the proof half a round behind.
+static int prover_round(dmx_prover* p, size_t j, hipEvent_t vec_done = nullptr) {
     hipStream_t s = p->s;
     const Jobs& jb = p->jb;
     const NttTables& tw = p->ctx->tw;
@@ -873,6 +875,7 @@ static int prover_round(dmx_prover* p, size_t j) {
                           p->rec_lpoly(j), 2 * K, lm - 1, s, jb);
     }
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[1], s));
+    if (vec_done) HIP_TRY(hipEventRecord(vec_done, s));
     // ---- poly stage ----
     p->tr.challenge("poly", j, p->Ct.as<zq>(), m, p->rec_lpoly(j), h, p->rec_rpoly(j), h, p->sc(4 * j + 2), flags, s,
                     jb, false);
@@ -1297,27 +1300,31 @@ static bool all_zero(const std::vector<zq>& m) {
     return true;
 }
 
-// One replay round on the device (verifier.rs:57-120) with the round's four cross-term
-// modules already in device memory and their shapes already checked. No host sync. The
-// generator folds of the round (verifier.rs:75, 102) are not materialised: the round's challenge
-// inverses are kept and the terminal generator is formed once, at the final opening.
-static int verifier_round_launch(dmx_verifier* v, const zq* d_l, const zq* d_r, const zq* d_lp, const zq* d_rp) {
+// The vector half of a replay round: the vector challenge from C and the vector cross terms,
+// and C_t = C + x^-1 L + x R into Ct. Needs only the round's vector cross terms, so a pipelined
+// caller can start it before the prover has produced the poly cross terms.
+static int verifier_round_vec(dmx_verifier* v, const zq* d_l, const zq* d_r) {
+    if (v->rows <= 1) return DMX_OK;
+    const size_t n = v->clen;
+    zq* ch = v->round_chal(v->round);
+    v->tr.challenge("vec", v->round, v->C.as<zq>(), n, d_l, n, d_r, n, ch, v->flags.as<uint32_t>(), v->s, v->jb);
+    launch_com_update(v->C.as<zq>(), d_l, d_r, ch, 0, log2_exact(n), v->Ct.as<zq>(), v->s, v->jb);
+    v->vec_mask |= 1u << v->round;
+    HIP_TRY(hipGetLastError());
+    return DMX_OK;
+}
+
+// The poly half: the poly challenge and the folded commitment back into C; advances the round.
+static int verifier_round_poly(dmx_verifier* v, const zq* d_lp, const zq* d_rp) {
     hipStream_t s = v->s;
     const Jobs& jb = v->jb;
     const size_t n = v->clen;
     const bool vec = v->rows > 1, poly = v->glen > 1;
-    uint32_t* flags = v->flags.as<uint32_t>();
-    zq* ch = v->round_chal(v->round);
-    const zq* Ct = v->C.as<zq>();
-    if (vec) {
-        v->tr.challenge("vec", v->round, v->C.as<zq>(), n, d_l, n, d_r, n, ch, flags, s, jb);
-        launch_com_update(v->C.as<zq>(), d_l, d_r, ch, 0, log2_exact(n), v->Ct.as<zq>(), s, jb);
-        Ct = v->Ct.as<zq>();
-        v->vec_mask |= 1u << v->round;
-    }
+    const zq* Ct = vec ? v->Ct.as<zq>() : v->C.as<zq>();
     size_t nclen = n, nglen = v->glen;
     if (poly) {
-        v->tr.challenge("poly", v->round, Ct, n, d_lp, n / 2, d_rp, n / 2, ch + 2, flags, s, jb);
+        zq* ch = v->round_chal(v->round);
+        v->tr.challenge("poly", v->round, Ct, n, d_lp, n / 2, d_rp, n / 2, ch + 2, v->flags.as<uint32_t>(), s, jb);
         launch_com_update(Ct, d_lp, d_rp, ch + 2, 1, log2_exact(n / 2), v->C.as<zq>(), s, jb);
         v->poly_mask |= 1u << v->round;
         nclen = n / 2;
@@ -1331,6 +1338,15 @@ static int verifier_round_launch(dmx_verifier* v, const zq* d_l, const zq* d_r, 
     v->glen = nglen;
     v->round += 1;
     return DMX_OK;
+}
+
+// One replay round on the device (verifier.rs:57-120) with the round's four cross-term
+// modules already in device memory and their shapes already checked. No host sync. The
+// generator folds of the round (verifier.rs:75, 102) are not materialised: the round's challenge
+// inverses are kept and the terminal generator is formed once, at the final opening.
+static int verifier_round_launch(dmx_verifier* v, const zq* d_l, const zq* d_r, const zq* d_lp, const zq* d_rp) {
+    int rc = verifier_round_vec(v, d_l, d_r);
+    return rc ? rc : verifier_round_poly(v, d_lp, d_rp);
 }
 
 // update_commitment (verifier.rs:57-120); every check runs before any device work, so a
@@ -1635,7 +1651,7 @@ static int batch_create(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
     p->jb = v->jb = Jobs{n, L.stride()};
     prover_bind(p);
     verifier_bind(v);
-    J->evs.resize((size_t)d + 3);
+    J->evs.resize(2 * (size_t)d + 3);
     for (auto& e : J->evs) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&J->done, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&J->gen_done, hipEventDisableTiming));
@@ -1674,11 +1690,14 @@ static int batch_enqueue(BatchJob* J) {
     HIP_TRY(hipStreamWaitEvent(v->s, J->evs[1], 0));
     rc = verifier_enqueue_statement(v, p->tr.file_id(), p->com0.as<zq>());
     if (rc) return rc;
-    for (size_t j = 0; j < (size_t)p->d; ++j) {
-        rc = prover_round(p, j);
+    // The verifier follows half a round behind: its vector half of round j needs the prover's
+    // round j-1 (the vector cross terms of round j), its poly half the prover's vector stage of
+    // round j (the poly cross terms).
+    const size_t d = (size_t)p->d;
+    for (size_t j = 0; j < d; ++j) {
+        rc = prover_round(p, j, J->evs[2 + d + j]);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(J->evs[2 + j], p->s));
-        HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + j], 0));
         if (own_gen && j == gen_after) {
             HIP_TRY(hipStreamWaitEvent(J->gs, J->evs[2 + j], 0));
             // one workgroup per CU: the expansion has the whole tail of the proof to finish in
@@ -1687,8 +1706,12 @@ static int batch_enqueue(BatchJob* J) {
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipEventRecord(J->gen_done, J->gs));
         }
-        rc = verifier_round_launch(v, p->rec_lvec(j), p->rec_rvec(j), p->rec_lpoly(j), p->rec_rpoly(j));
+        rc = verifier_round_vec(v, p->rec_lvec(j), p->rec_rvec(j));
         if (rc) return rc;
+        HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + d + j], 0));
+        rc = verifier_round_poly(v, p->rec_lpoly(j), p->rec_rpoly(j));
+        if (rc) return rc;
+        HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + j], 0));
     }
     p->round = (size_t)p->d;
     if (own_gen) HIP_TRY(hipStreamWaitEvent(v->s, J->gen_done, 0));
