@@ -124,15 +124,22 @@ void launch_mac_split_inv(const uint32_t* What, const uint32_t* Ghat, size_t row
 size_t mac_part_words(int max_log_m);
 // Z_q rows only (no transform): dst[row][c] = row_value(src, row, c)
 void launch_fold_rows_zq(const RowSrc& src, size_t rows, int log_m, hipStream_t s, const Jobs& jb = Jobs{});
-// The verifier's terminal generator g_d[0] (K values) from the initial matrix G (rows x K x n,
-// shared by every job when g_shared) and the per-round (x, x^-1, y, y^-1) in chal, as one
-// weighted sum (vec_mask / poly_mask: rounds whose vector / poly stage ran). w (rows), u (n) and
-// S (rows * K) are scratch.
-void launch_terminal_generator(const zq* G, bool g_shared, const zq* chal, uint32_t rounds, uint32_t vec_mask,
-                               uint32_t poly_mask, int log_rows, int log_n, zq* w, zq* u, zq* S, zq* out,
-                               hipStream_t s, const Jobs& jb = Jobs{});
-void launch_final_check(const zq* g, const zq* c, const zq* opening, size_t n, uint32_t* flags,
-                        uint32_t bit, hipStream_t s, const Jobs& jb = Jobs{});
+// The verifier's terminal generator g_d[0] (K values) as one weighted sum over the initial
+// matrix G (rows x K x n; shared by every job when g_shared), in two parts. chal: (x, x^-1, y,
+// y^-1) per round; vmask / pmask: the rounds (bit r for round r, counted from chal) that folded
+// rows / coefficients.
+// Early: the first masked rounds fold G (2^log_hi * 2^log_lo rows of 2^log_n) into
+// H[2^log_lo][K][2^log_nt]; 2^log_hi <= EARLY_MAX_HI and 2^log_nt <= EARLY_MAX_NT.
+constexpr uint32_t EARLY_MAX_HI = 128;
+constexpr uint32_t EARLY_MAX_NT = 256;
+void launch_gen_contract_early(const zq* G, bool g_shared, const zq* chal, uint32_t vmask, uint32_t pmask,
+                               int log_hi, int log_lo, int log_n, int log_nt, zq* H, hipStream_t s,
+                               const Jobs& jb = Jobs{});
+// Late: the remaining rounds fold H (2^log_rows x K x 2^log_n, both <= 32) to g_d[0] (gfin) and
+// check the final opening: flags |= bit when g_d[0]_k * opening != com_k for some k.
+void launch_gen_contract_late(const zq* H, bool h_shared, const zq* chal, uint32_t vmask, uint32_t pmask,
+                              int log_rows, int log_n, const zq* com, const zq* opening, uint32_t* flags,
+                              uint32_t bit, zq* gfin, hipStream_t s, const Jobs& jb = Jobs{});
 // residues [slice][8][m] -> Z_q [slice][m]
 void launch_crt_lift(const uint32_t* res, zq* out, size_t slices, int log_m, hipStream_t s,
                      const Jobs& jb = Jobs{});

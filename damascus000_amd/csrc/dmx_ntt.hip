@@ -1339,113 +1339,176 @@ void launch_fold_rows_zq(const RowSrc& src, size_t rows, int log_m, hipStream_t 
 //   g_d[0]_k = sum_i w_i sum_c u_c g[i][k][c],
 // w_i the product of x_t^-1 over the vector rounds t that found row i in the right half (round
 // t reads bit log2(rows) - 1 - t of i), u_c the product of the poly rounds' factors along c's
-// path. The matrix is read once, with no folded copies written.
-// chal holds (x, x^-1, y, y^-1) per round.
+// path. Both weights factor at any round T: the rounds before T fold the matrix to
+//   H[i_lo][k][c_T] = sum_{i_hi} w_early(i_hi) sum_{c -> c_T} u_early(c) g[i_hi * R + i_lo][k][c]
+// (R = rows >> early vector folds; c_T = c's position after the early poly folds), and the rounds
+// from T on finish on H with their own weights. The early part reads the matrix once and runs as
+// soon as the early challenges are known, while the proof's last rounds are still in flight; the
+// late part is one small kernel at the end. chal holds (x, x^-1, y, y^-1) per round; the masks
+// say which rounds folded rows (vec) and coefficients (poly), bit r for round r.
 // ---------------------------------------------------------------------------------------
-__global__ void k_fold_weights(const zq* chal, uint32_t rounds, uint32_t vec_mask, uint32_t poly_mask, int log_rows,
-                               int log_n, zq* w, zq* u, size_t js) {
-    const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t rows = 1u << log_rows, n = 1u << log_n;
-    if (gid >= rows && gid >= n) return;
+
+// product of x_r^-1 over the vector rounds that see row i in the right half; the first of the
+// masked rounds reads bit nbits - 1 of i
+__device__ zq row_weight(uint32_t i, int nbits, const zq* chal, uint32_t vmask) {
+    zq acc = zq_from_u64(1);
+    int bit = nbits - 1;
+    for (uint32_t r = 0; vmask >> r; ++r) {
+        if (!((vmask >> r) & 1)) continue;
+        if ((i >> bit) & 1) acc = zq_mul(acc, load_zq(chal + 4 * r + 1));
+        --bit;
+    }
+    return acc;
+}
+
+// position of coefficient c (of a length-len element) after the masked poly rounds; `wrapped`
+// when its path took the X^L = -1 wrap (every later position is then 0)
+__device__ __forceinline__ uint32_t coeff_path(uint32_t c, uint32_t len, uint32_t pmask, bool& wrapped) {
+    wrapped = false;
+    for (uint32_t r = 0; pmask >> r; ++r) {
+        if (!((pmask >> r) & 1)) continue;
+        if (c & 1) {
+            const bool w = c == len - 1;
+            wrapped = wrapped || w;
+            c = w ? 0 : (c + 1) >> 1;
+        } else {
+            c >>= 1;
+        }
+        len >>= 1;
+    }
+    return c;
+}
+
+// the poly rounds' factor along c's path
+__device__ zq coeff_weight(uint32_t c, uint32_t len, const zq* chal, uint32_t pmask) {
+    zq acc = zq_from_u64(1);
+    for (uint32_t r = 0; pmask >> r; ++r) {
+        if (!((pmask >> r) & 1)) continue;
+        if (c & 1) {
+            const zq yi = load_zq(chal + 4 * r + 3);
+            const bool wrap = c == len - 1;
+            acc = zq_mul(acc, wrap ? zq_neg(yi) : yi);
+            c = wrap ? 0 : (c + 1) >> 1;
+        } else {
+            c >>= 1;
+        }
+        len >>= 1;
+    }
+    return acc;
+}
+
+constexpr uint32_t EARLY_TILE = 1024;  // coefficients staged in LDS per step
+
+// One workgroup per (i_lo, k): H[(i_lo*K + k)*n_t + j]. The positions c -> c_T are nondecreasing
+// in c up to the first coefficient whose path wraps and 0 from there on, so position j collects
+// the run [lo_j, lo_{j+1}) (and position 0 also the wrapped tail), found by bisection.
+__global__ void __launch_bounds__(256) k_gen_contract_early(const zq* __restrict__ G, size_t gjs, const zq* chal,
+                                                            uint32_t vmask, uint32_t pmask, int log_hi, int log_lo,
+                                                            int log_n, int log_nt, zq* __restrict__ H, size_t js) {
+    __shared__ zq w[EARLY_MAX_HI];
+    __shared__ zq sp[EARLY_TILE];
     const size_t jo = job_off(js);
     chal = jp(chal, jo);
-    if (gid < rows) {
-        zq acc = zq_from_u64(1);
-        int bit = log_rows - 1;
-        for (uint32_t r = 0; r < rounds; ++r) {
-            if (!((vec_mask >> r) & 1)) continue;
-            if ((gid >> bit) & 1) acc = zq_mul(acc, load_zq(chal + 4 * r + 1));
-            --bit;
+    H = jp(H, jo);
+    G = jp(G, (size_t)blockIdx.y * gjs);
+    const uint32_t i_lo = blockIdx.x / K, k = blockIdx.x % K;
+    const uint32_t n_hi = 1u << log_hi, n = 1u << log_n, nt = 1u << log_nt, t = threadIdx.x;
+    for (uint32_t h = t; h < n_hi; h += blockDim.x) w[h] = row_weight(h, log_hi, chal, vmask);
+    // run bounds of position t (t < nt): first coefficient of the run and one past its end
+    uint32_t lo = 0, hi = 0, wrap_start = n;
+    if (t < nt) {
+        uint32_t a = 0, b = n;  // first c whose path wraps
+        while (a < b) {
+            const uint32_t mid = (a + b) >> 1;
+            bool wr;
+            (void)coeff_path(mid, n, pmask, wr);
+            if (wr) b = mid; else a = mid + 1;
         }
-        store_zq(jp(w, jo) + gid, acc);
-    }
-    if (gid < n) {
-        zq acc = zq_from_u64(1);
-        uint32_t c = gid, len = n;
-        for (uint32_t r = 0; r < rounds; ++r) {
-            if (!((poly_mask >> r) & 1)) continue;
-            if (c & 1) {
-                const zq yi = load_zq(chal + 4 * r + 3);
-                const bool wrap = c == len - 1;
-                acc = zq_mul(acc, wrap ? zq_neg(yi) : yi);
-                c = wrap ? 0 : (c + 1) >> 1;
-            } else {
-                c >>= 1;
+        wrap_start = a;
+        auto first_at_least = [&](uint32_t j) {  // first c < wrap_start with c_T >= j
+            uint32_t x = 0, y = wrap_start;
+            while (x < y) {
+                const uint32_t mid = (x + y) >> 1;
+                bool wr;
+                if (coeff_path(mid, n, pmask, wr) >= j) y = mid; else x = mid + 1;
             }
-            len >>= 1;
+            return x;
+        };
+        lo = first_at_least(t);
+        hi = t + 1 < nt ? first_at_least(t + 1) : wrap_start;
+    }
+    __syncthreads();
+    zq acc = zq_zero();
+    const size_t row_stride = ((size_t)K << log_n) << log_lo;  // from row block h to h + 1
+    const zq* g0 = G + (((size_t)i_lo * K + k) << log_n);
+    for (uint32_t c0 = 0; c0 < n; c0 += EARLY_TILE) {
+        const uint32_t tl = n - c0 < EARLY_TILE ? n - c0 : EARLY_TILE;
+        for (uint32_t cc = t; cc < tl; cc += blockDim.x) {
+            const uint32_t c = c0 + cc;
+            zq s = zq_zero();
+#pragma unroll 4
+            for (uint32_t h = 0; h < n_hi; ++h) s = zq_add(s, zq_mul(w[h], load_zq(g0 + h * row_stride + c)));
+            sp[cc] = zq_mul(s, coeff_weight(c, n, chal, pmask));
         }
-        store_zq(jp(u, jo) + gid, acc);
-    }
-}
-
-// S[r] = w[r / K] * sum_c u[c] G[r][c] for generator row r = i*K + k; one workgroup per row
-__global__ void __launch_bounds__(256) k_fold_rowsums(const zq* __restrict__ G, const zq* __restrict__ w,
-                                                      const zq* __restrict__ u, int log_n, zq* __restrict__ S,
-                                                      size_t js, size_t gjs) {
-    __shared__ zq red[256];
-    const size_t jo = job_off(js);
-    const size_t r = blockIdx.x;
-    const zq* g = jp(G, (size_t)blockIdx.y * gjs) + (r << log_n);
-    u = jp(u, jo);
-    zq acc = zq_zero();
-    for (uint32_t c = threadIdx.x; c < (1u << log_n); c += 256) acc = zq_add(acc, zq_mul(load_zq(u + c), load_zq(g + c)));
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (uint32_t h = 128; h > 0; h >>= 1) {
-        if (threadIdx.x < h) red[threadIdx.x] = zq_add(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+        if (t < nt) {
+            const uint32_t a = lo > c0 ? lo : c0, b = hi < c0 + tl ? hi : c0 + tl;
+            for (uint32_t c = a; c < b; ++c) acc = zq_add(acc, sp[c - c0]);
+            if (t == 0)
+                for (uint32_t c = (wrap_start > c0 ? wrap_start : c0); c < c0 + tl; ++c) acc = zq_add(acc, sp[c - c0]);
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) store_zq(jp(S, jo) + r, zq_mul(red[0], load_zq(jp(w, jo) + r / K)));
+    if (t < nt) store_zq(H + (((size_t)i_lo * K + k) << log_nt) + t, acc);
 }
 
-// out[k] = sum_i S[i*K + k]; one workgroup per k
-__global__ void __launch_bounds__(256) k_fold_combine(const zq* __restrict__ S, uint32_t rows, zq* __restrict__ out,
-                                                      size_t js) {
-    __shared__ zq red[256];
-    const size_t jo = job_off(js);
-    S = jp(S, jo);
-    const uint32_t k = blockIdx.x;
-    zq acc = zq_zero();
-    for (uint32_t i = threadIdx.x; i < rows; i += 256) acc = zq_add(acc, load_zq(S + (size_t)i * K + k));
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (uint32_t h = 128; h > 0; h >>= 1) {
-        if (threadIdx.x < h) red[threadIdx.x] = zq_add(red[threadIdx.x], red[threadIdx.x + h]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) store_zq(jp(out, jo) + k, red[0]);
-}
-
-void launch_terminal_generator(const zq* G, bool g_shared, const zq* chal, uint32_t rounds, uint32_t vec_mask,
-                               uint32_t poly_mask, int log_rows, int log_n, zq* w, zq* u, zq* S, zq* out,
-                               hipStream_t s, const Jobs& jb) {
+void launch_gen_contract_early(const zq* G, bool g_shared, const zq* chal, uint32_t vmask, uint32_t pmask,
+                               int log_hi, int log_lo, int log_n, int log_nt, zq* H, hipStream_t s,
+                               const Jobs& jb) {
     if (jb.n == 0) return;
-    const uint32_t span = 1u << (log_rows > log_n ? log_rows : log_n);
-    hipLaunchKernelGGL(k_fold_weights, dim3((span + 255) / 256, jb.n), dim3(256), 0, s, chal, rounds, vec_mask,
-                       poly_mask, log_rows, log_n, w, u, jb.stride);
-    hipLaunchKernelGGL(k_fold_rowsums, dim3((unsigned)(((size_t)K) << log_rows), jb.n), dim3(256), 0, s, G, w, u,
-                       log_n, S, jb.stride, g_shared ? (size_t)0 : jb.stride);
-    hipLaunchKernelGGL(k_fold_combine, dim3(K, jb.n), dim3(256), 0, s, S, 1u << log_rows, out, jb.stride);
+    hipLaunchKernelGGL(k_gen_contract_early, dim3((unsigned)(K << log_lo), jb.n), dim3(256), 0, s, G,
+                       g_shared ? (size_t)0 : jb.stride, chal, vmask, pmask, log_hi, log_lo, log_n, log_nt, H,
+                       jb.stride);
 }
 
-// final opening (verifier.rs:122-134): flags |= bit when g[i] * opening != c[i]
-__global__ void k_final_check(const zq* g, const zq* c, const zq* opening, size_t n, uint32_t* flags,
-                              uint32_t bit, size_t js) {
-    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= n) return;
+// g_d[0]_k from H (rows x K x n, n and rows <= 32: what the late rounds fold) with the late
+// rounds' weights, and the final opening check (verifier.rs:122-134): flags |= bit when
+// g_d[0]_k * opening != C_k. One workgroup per k.
+__global__ void __launch_bounds__(256) k_gen_contract_late(const zq* __restrict__ H, size_t hjs, const zq* chal,
+                                                           uint32_t vmask, uint32_t pmask, int log_rows, int log_n,
+                                                           const zq* com, const zq* opening, uint32_t* flags,
+                                                           uint32_t bit, zq* gfin, size_t js) {
+    __shared__ zq wr[32], wc[32], red[256];
     const size_t jo = job_off(js);
-    g = jp(g, jo);
-    c = jp(c, jo);
-    opening = jp(opening, jo);
-    flags = jp(flags, jo);
-    if (!zq_eq(zq_mul(load_zq(g + gid), *opening), load_zq(c + gid))) atomicOr(flags, bit);
+    chal = jp(chal, jo);
+    H = jp(H, (size_t)blockIdx.y * hjs);
+    const uint32_t k = blockIdx.x, t = threadIdx.x, rows = 1u << log_rows, n = 1u << log_n;
+    if (t < rows) wr[t] = row_weight(t, log_rows, chal, vmask);
+    if (t >= 64 && t - 64 < n) wc[t - 64] = coeff_weight(t - 64, n, chal, pmask);
+    __syncthreads();
+    zq acc = zq_zero();
+    for (uint32_t e = t; e < rows * n; e += blockDim.x) {
+        const uint32_t i = e >> log_n, c = e & (n - 1);
+        acc = zq_add(acc, zq_mul(zq_mul(wr[i], wc[c]), load_zq(H + (((size_t)i * K + k) << log_n) + c)));
+    }
+    red[t] = acc;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] = zq_add(red[t], red[t + h]);
+        __syncthreads();
+    }
+    if (t == 0) {
+        store_zq(jp(gfin, jo) + k, red[0]);
+        if (!zq_eq(zq_mul(red[0], load_zq(jp(opening, jo))), load_zq(jp(com, jo) + k))) atomicOr(jp(flags, jo), bit);
+    }
 }
 
-void launch_final_check(const zq* g, const zq* c, const zq* opening, size_t n, uint32_t* flags,
-                        uint32_t bit, hipStream_t s, const Jobs& jb) {
-    if (!n || jb.n == 0) return;
-    hipLaunchKernelGGL(k_final_check, dim3((unsigned)((n + 255) / 256), jb.n), dim3(256), 0, s, g, c, opening,
-                       n, flags, bit, jb.stride);
+void launch_gen_contract_late(const zq* H, bool h_shared, const zq* chal, uint32_t vmask, uint32_t pmask,
+                              int log_rows, int log_n, const zq* com, const zq* opening, uint32_t* flags,
+                              uint32_t bit, zq* gfin, hipStream_t s, const Jobs& jb) {
+    if (jb.n == 0) return;
+    hipLaunchKernelGGL(k_gen_contract_late, dim3(K, jb.n), dim3(256), 0, s, H, h_shared ? (size_t)0 : jb.stride,
+                       chal, vmask, pmask, log_rows, log_n, com, opening, flags, bit, gfin, jb.stride);
 }
 
 // ---------------------------------------------------------------------------------------

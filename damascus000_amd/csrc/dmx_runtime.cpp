@@ -1136,9 +1136,9 @@ struct dmx_verifier {
     size_t g_cap = 0, c_cap = 0;
     uint32_t vec_mask = 0, poly_mask = 0;  // rounds whose vector / poly stage ran
     DevBuf store;  // the workspace, unless a batch owns it
-    // G0: the generator matrix, never folded (the terminal generator is one weighted sum over
-    // it); chal: (x, x^-1, y, y^-1) per round; wts, uts, sums, gfin: the weighted sum's scratch
-    Slice meta, G0, C, Ct, rec, chal, wts, uts, sums, gfin, flags, opening;
+    // G0: the generator matrix, never folded round by round; chal: (x, x^-1, y, y^-1) per
+    // round; H: the matrix folded by the early rounds' weights (gen_split); gfin: g_d[0]
+    Slice meta, G0, C, Ct, rec, chal, H, gfin, flags, opening;
     Transcriber tr;
     std::vector<JobMeta> stage;
     const zq* G0_src = nullptr;  // resident generator matrix shared by every job (see dmx_prover)
@@ -1150,6 +1150,30 @@ struct dmx_verifier {
     const uint32_t* gseed() const { return reinterpret_cast<const uint32_t*>(meta.as<uint8_t>() + META_GSEED); }
 };
 
+// The terminal generator is one weighted sum over the matrix, split at round T: the rounds
+// before T fold the matrix into H as soon as their challenges are known, the last (at most
+// LATE_ROUNDS) rounds fold H at the final opening. A round folds rows while more than one is left
+// and coefficients while the ring length is above one, so rounds 0 .. log2(rows0) - 1 are the
+// vector folds and 0 .. log2(n0) - 1 the poly folds.
+constexpr size_t LATE_ROUNDS = 5;
+struct GenSplit {
+    size_t T = 0;
+    int log_hi = 0, log_lo = 0, log_n = 0, log_nt = 0;  // early rows folded into one, rows left, ...
+    uint32_t vm = 0, pm = 0;                           // the early rounds' vector / poly folds
+};
+static GenSplit gen_split(const dmx_verifier* v) {
+    GenSplit g;
+    g.T = v->rounds > LATE_ROUNDS ? v->rounds - LATE_ROUNDS : 0;
+    const int lr = log2_exact(v->rows0), ln = log2_exact(v->n0);
+    g.log_hi = std::min<int>((int)g.T, lr);
+    g.log_lo = lr - g.log_hi;
+    g.log_n = ln;
+    g.log_nt = ln - std::min<int>((int)g.T, ln);
+    g.vm = (1u << g.log_hi) - 1;
+    g.pm = (1u << (ln - g.log_nt)) - 1;
+    return g;
+}
+
 static void verifier_plan(dmx_verifier* v, Layout& L) {
     L.add(v->meta, sizeof(JobMeta));
     L.add(v->G0, v->g_cap * 16);
@@ -1157,9 +1181,8 @@ static void verifier_plan(dmx_verifier* v, Layout& L) {
     L.add(v->Ct, (size_t)K * v->c_cap * 16);
     L.add(v->rec, (size_t)4 * K * v->c_cap * 16);
     L.add(v->chal, std::max<size_t>(v->rounds, 1) * 4 * 16);
-    L.add(v->wts, v->rows0 * 16);
-    L.add(v->uts, v->n0 * 16);
-    L.add(v->sums, v->rows0 * K * 16);
+    const GenSplit g = gen_split(v);
+    L.add(v->H, g.T ? ((size_t)K << (g.log_lo + g.log_nt)) * 16 : 16);
     L.add(v->gfin, K * 16);
     L.add(v->flags, 256);
     L.add(v->opening, 16);
@@ -1404,18 +1427,41 @@ extern "C" int dmx_verifier_update_commitment(dmx_verifier* v, const uint8_t* re
     return rc;
 }
 
-static int verifier_final_launch(dmx_verifier* v, const zq* d_opening) {
+// The early fold of the terminal generator (k_gen_contract_early) on stream `s`; needs the
+// challenges of the rounds before gen_split's T and the generator matrix.
+static int verifier_enqueue_early(dmx_verifier* v, hipStream_t s) {
+    const GenSplit g = gen_split(v);
+    if (!g.T) return DMX_OK;
+    if ((v->vec_mask & ((1u << g.T) - 1)) != g.vm || (v->poly_mask & ((1u << g.T) - 1)) != g.pm)
+        return fail(DMX_ERR_NOT_FINAL, "the early rounds have not been replayed");
+    if ((1u << g.log_hi) > EARLY_MAX_HI || (1u << g.log_nt) > EARLY_MAX_NT)
+        return fail(DMX_ERR_SHAPE, "generator matrix shape outside the early fold's limits");
+    launch_gen_contract_early(v->G(), v->G0_src != nullptr, v->chal.as<zq>(), g.vm, g.pm, g.log_hi, g.log_lo,
+                              g.log_n, g.log_nt, v->H.as<zq>(), s, v->jb);
+    HIP_TRY(hipGetLastError());
+    return DMX_OK;
+}
+
+// `early_done`: the early fold was already enqueued (a fused job runs it on its side stream)
+static int verifier_final_launch(dmx_verifier* v, const zq* d_opening, bool early_done = false) {
     if (v->round != v->rounds) return fail(DMX_ERR_NOT_FINAL, "cannot verify final opening before all rounds replay");
     if (v->rows != 1) return fail(DMX_ERR_NOT_FINAL, "terminal generator family must have length one");
     if (v->glen != v->clen) return fail(DMX_ERR_OPENING, "final opening mismatch");
     if (v->glen != 1 || (size_t)__builtin_popcount(v->vec_mask) != (size_t)log2_exact(v->rows0) ||
         (size_t)__builtin_popcount(v->poly_mask) != (size_t)log2_exact(v->n0))
         return fail(DMX_ERR_NOT_FINAL, "the generator family has not been folded to scalars");
-    launch_terminal_generator(v->G(), v->G0_src != nullptr, v->chal.as<zq>(), (uint32_t)v->rounds, v->vec_mask,
-                              v->poly_mask, log2_exact(v->rows0), log2_exact(v->n0), v->wts.as<zq>(), v->uts.as<zq>(),
-                              v->sums.as<zq>(), v->gfin.as<zq>(), v->s, v->jb);
+    const GenSplit g = gen_split(v);
+    if (g.log_lo > 5 || g.log_nt > 5) return fail(DMX_ERR_SHAPE, "generator matrix shape outside the late fold's limits");
+    if (!early_done) {
+        int rc = verifier_enqueue_early(v, v->s);
+        if (rc) return rc;
+    }
+    const zq* H = g.T ? v->H.as<zq>() : v->G();
+    const bool h_shared = !g.T && v->G0_src != nullptr;
     launch_fill32(v->flags.as<uint32_t>(), 1, 0, v->s, v->jb);
-    launch_final_check(v->gfin.as<zq>(), v->C.as<zq>(), d_opening, (size_t)K, v->flags.as<uint32_t>(), 1, v->s, v->jb);
+    launch_gen_contract_late(H, h_shared, v->chal.as<zq>() + 4 * g.T, v->vec_mask >> g.T, v->poly_mask >> g.T, g.log_lo,
+                             g.log_nt, v->C.as<zq>(), d_opening, v->flags.as<uint32_t>(), 1, v->gfin.as<zq>(), v->s,
+                             v->jb);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
@@ -1651,7 +1697,7 @@ static int batch_create(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
     p->jb = v->jb = Jobs{n, L.stride()};
     prover_bind(p);
     verifier_bind(v);
-    J->evs.resize(2 * (size_t)d + 3);
+    J->evs.resize(2 * (size_t)d + 4);
     for (auto& e : J->evs) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&J->done, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&J->gen_done, hipEventDisableTiming));
@@ -1692,8 +1738,13 @@ static int batch_enqueue(BatchJob* J) {
     if (rc) return rc;
     // The verifier follows half a round behind: its vector half of round j needs the prover's
     // round j-1 (the vector cross terms of round j), its poly half the prover's vector stage of
-    // round j (the poly cross terms).
+    // round j (the poly cross terms). The side stream gs expands the generator matrix after the
+    // prover's round gen_after and folds it by the early rounds' challenges once the verifier
+    // has them (after its round early_after), so the final opening only folds what is left.
     const size_t d = (size_t)p->d;
+    const size_t T = gen_split(v).T;
+    const size_t early_after = T ? std::max(T - 1, own_gen ? gen_after : 0) : d;
+    const bool side = own_gen || T;
     for (size_t j = 0; j < d; ++j) {
         rc = prover_round(p, j, J->evs[2 + d + j]);
         if (rc) return rc;
@@ -1704,18 +1755,27 @@ static int batch_enqueue(BatchJob* J) {
             launch_gen_generators(v->gseed(), v->rows0 * K, log2_exact(v->n0), v->G0.as<zq>(), J->gs, v->jb,
                                   p->ctx->cus);
             HIP_TRY(hipGetLastError());
-            HIP_TRY(hipEventRecord(J->gen_done, J->gs));
         }
         rc = verifier_round_vec(v, p->rec_lvec(j), p->rec_rvec(j));
         if (rc) return rc;
         HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + d + j], 0));
         rc = verifier_round_poly(v, p->rec_lpoly(j), p->rec_rpoly(j));
         if (rc) return rc;
+        if (j == early_after) {
+            HIP_TRY(hipEventRecord(J->evs[2 + 2 * d], v->s));
+            HIP_TRY(hipStreamWaitEvent(J->gs, J->evs[2 + 2 * d], 0));
+            rc = verifier_enqueue_early(v, J->gs);
+            if (rc) return rc;
+        }
         HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + j], 0));
     }
     p->round = (size_t)p->d;
-    if (own_gen) HIP_TRY(hipStreamWaitEvent(v->s, J->gen_done, 0));
-    rc = verifier_final_launch(v, p->W_a.as<zq>());  // m* = message[0].coeffs[0] (prover.rs:133-138)
+    if (side) {
+        HIP_TRY(hipEventRecord(J->gen_done, J->gs));
+        HIP_TRY(hipStreamWaitEvent(v->s, J->gen_done, 0));
+    }
+    // m* = message[0].coeffs[0] (prover.rs:133-138)
+    rc = verifier_final_launch(v, p->W_a.as<zq>(), T != 0);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(J->evs[J->evs.size() - 1], v->s));
     HIP_TRY(hipStreamWaitEvent(p->s, J->evs[J->evs.size() - 1], 0));  // join

@@ -243,6 +243,28 @@ def test_gpu_verifier_zero_statement_depth_sweep():
         v.verify_final_opening(0)
 
 
+def test_gpu_verifier_split_generator_fold_accepts_and_rejects():
+    """The terminal generator is folded in two parts, the rounds before the last five and then
+    those five. A real proof replayed record by record accepts, and the same replay with the
+    opening changed by one is rejected, at d = 7 and 8 (2 and 3 early rounds)."""
+    for d in [7, 8]:
+        data = W.pattern_bytes(W.byte_len_for_depth(d), d)
+        say("split generator fold, d =", d)
+        p, st, recs, op = _gpu_prove(data)
+        rc, ost, orecs, oop = O.full_flow(data)
+        assert rc == 0 and st == ost and b"".join(recs) == orecs and op == oop
+        for opening, ok in [(op, True), ((op + 1) % O.Q, False)]:
+            v = dm.DamascusVerifier(None, st)
+            for r in recs:
+                v.update_commitment(r)
+            if ok:
+                v.verify_final_opening(opening)
+            else:
+                with pytest.raises(dm.DamascusError) as e:
+                    v.verify_final_opening(opening)
+                assert e.value.name == "ERR_OPENING"
+
+
 def test_gpu_verifier_error_paths():
     data = W.pattern_bytes(8192, 2)
     p, st, recs, op = _gpu_prove(data)
