@@ -47,6 +47,7 @@ enum RowMode : int {
 struct RowSrc {
     const zq* a;            // source rows (ROW_GEN: unused)
     zq* dst;                // when non-null the Z_q row is stored here (row-major, m per row)
+    zq* dst2;               // when non-null a second copy goes here (forward transforms only)
     size_t b_off;           // ROW_VFOLD partner offset in rows
     const zq* scal;         // ROW_*FOLD scalar (device)
     int mode;
@@ -184,8 +185,10 @@ void launch_lift_check(const uint32_t* res_a, int log_ma, const ComUpd& cu, zq* 
 // commitment updates (tiny, K x m)
 // out = c + sinv*l + s*r          (VEC; scalars at scal[0]=s, scal[1]=s^-1)
 // out = even(c) + sinv*l + s*r    (POLY; c has 2m coefficients per coordinate)
-void launch_com_update(const zq* c, const zq* l, const zq* r, const zq* scal, int poly, int log_m,
-                       zq* out, hipStream_t s, const Jobs& jb = Jobs{});
+// With `hint`, s^-1 is taken from *hint once s * *hint == 1 is checked (computed from s when the
+// check fails) and written to scal[1]; without it scal[1] must already hold s^-1.
+void launch_com_update(const zq* c, const zq* l, const zq* r, zq* scal, int poly, int log_m,
+                       zq* out, hipStream_t s, const Jobs& jb = Jobs{}, const zq* hint = nullptr);
 // fill n 32-bit words (graph-safe replacement for memset)
 void launch_fill32(uint32_t* p, size_t n, uint32_t v, hipStream_t s, const Jobs& jb = Jobs{});
 // plain 16-byte copy (HBM calibration, graph-safe device copies)

@@ -129,6 +129,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(FwdSeg s1, FwdSeg 
     const size_t jo = job_off(js);
     if (!src.a_shared) src.a = jp(src.a, jo);
     src.dst = jp(src.dst, jo);
+    src.dst2 = jp(src.dst2, jo);
     src.scal = jp(src.scal, jo);
     src.gseed = jp(src.gseed, jo);
     out = jp(out, jo);
@@ -145,6 +146,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(FwdSeg s1, FwdSeg 
         for (int c = threadIdx.x; c < m; c += NTT_THREADS) {
             zq x = row_value(src, key, row, c, log_m);
             if (src.dst && p0 == 0) store_zq(src.dst + row * m + c, x);
+            if (src.dst2 && p0 == 0) store_zq(src.dst2 + row * m + c, x);
 #pragma unroll 1
             for (int q = 0; q < pg; ++q) sh[q * m + c] = zq_mod_p_centred(x, c_pc[p0 + q]);
         }
@@ -228,6 +230,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(FwdSeg s1, FwdSeg s
     const size_t jo = job_off(js);
     if (!src.a_shared) src.a = jp(src.a, jo);
     src.dst = jp(src.dst, jo);
+    src.dst2 = jp(src.dst2, jo);
     src.scal = jp(src.scal, jo);
     src.gseed = jp(src.gseed, jo);
     out = jp(out, jo);
@@ -246,6 +249,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(FwdSeg s1, FwdSeg s
         for (int c = tr; c < G::M; c += G::TPR) {
             const zq x = row_value(src, key, row, c, G::LOG_M);
             if (src.dst) store_zq(src.dst + row * G::M + c, x);
+            if (src.dst2) store_zq(src.dst2 + row * G::M + c, x);
 #pragma unroll
             for (int q = 0; q < NPRIMES; ++q) S[G::slot(q, c)] = zq_mod_p_centred(x, c_pc[q]);
         }
@@ -1514,8 +1518,8 @@ void launch_gen_contract_late(const zq* H, bool h_shared, const zq* chal, uint32
 // ---------------------------------------------------------------------------------------
 // commitment bookkeeping
 // ---------------------------------------------------------------------------------------
-__global__ void k_com_update(const zq* c, const zq* l, const zq* r, const zq* scal, int poly,
-                             int log_m, zq* out, size_t js) {
+__global__ void k_com_update(const zq* c, const zq* l, const zq* r, zq* scal, int poly, int log_m, zq* out,
+                             const zq* hint, size_t js) {
     const size_t m = (size_t)1 << log_m;
     const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= K * m) return;
@@ -1525,20 +1529,28 @@ __global__ void k_com_update(const zq* c, const zq* l, const zq* r, const zq* sc
     r = jp(r, jo);
     scal = jp(scal, jo);
     out = jp(out, jo);
+    const zq s = scal[0];
+    zq sinv;
+    if (hint) {  // uniform: every thread checks the same candidate
+        sinv = load_zq(jp(hint, jo));
+        if (!zq_eq(zq_mul(s, sinv), zq_from_u64(1))) sinv = zq_inv_divsteps(s);
+        if (gid == 0) scal[1] = sinv;
+    } else {
+        sinv = scal[1];
+    }
     const size_t k = gid >> log_m, i = gid & (m - 1);
-    const zq s = scal[0], sinv = scal[1];
     const zq base = poly ? load_zq(c + k * (2 * m) + 2 * i) : load_zq(c + gid);
     zq v = zq_add(base, zq_mul(load_zq(l + gid), sinv));
     v = zq_add(v, zq_mul(load_zq(r + gid), s));
     store_zq(out + gid, v);
 }
 
-void launch_com_update(const zq* c, const zq* l, const zq* r, const zq* scal, int poly, int log_m,
-                       zq* out, hipStream_t s, const Jobs& jb) {
+void launch_com_update(const zq* c, const zq* l, const zq* r, zq* scal, int poly, int log_m, zq* out,
+                       hipStream_t s, const Jobs& jb, const zq* hint) {
     const size_t n = (size_t)K << log_m;
     if (jb.n == 0) return;
-    hipLaunchKernelGGL(k_com_update, dim3((unsigned)((n + 255) / 256), jb.n), dim3(256), 0, s, c, l, r,
-                       scal, poly, log_m, out, jb.stride);
+    hipLaunchKernelGGL(k_com_update, dim3((unsigned)((n + 255) / 256), jb.n), dim3(256), 0, s, c, l, r, scal, poly,
+                       log_m, out, hint, jb.stride);
 }
 
 __global__ void k_fill32(uint32_t* __restrict__ p, size_t n, uint32_t v, size_t js) {

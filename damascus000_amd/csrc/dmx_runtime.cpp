@@ -767,7 +767,8 @@ static int prover_init_file(dmx_prover* p) {
 
 // second part: generators, com_0 and the round-0 vector cross terms, all from one pass over
 // the generator transforms.
-static int prover_init_commit(dmx_prover* p) {
+// `g_copy`: where the generator rows also go (the fused job's verifier matrix), or null
+static int prover_init_commit(dmx_prover* p, zq* g_copy = nullptr) {
     hipStream_t s = p->s;
     const Jobs& jb = p->jb;
     const size_t N = p->N;
@@ -779,6 +780,7 @@ static int prover_init_commit(dmx_prover* p) {
     RowSrc gen{};
     gen.mode = ROW_GEN;
     gen.dst = p->G_a.as<zq>();
+    gen.dst2 = g_copy;
     gen.gseed = p->gseed();
     // witness rows, and the generator rows unless a resident matrix serves every job
     fwd_rows2(p->ctx, pl, N, p->What_a.as<uint32_t>(), gen, p->Ghat0_src ? 0 : (size_t)K * N,
@@ -1323,37 +1325,57 @@ static bool all_zero(const std::vector<zq>& m) {
     return true;
 }
 
-// The vector half of a replay round: the vector challenge from C and the vector cross terms,
-// and C_t = C + x^-1 L + x R into Ct. Needs only the round's vector cross terms, so a pipelined
-// caller can start it before the prover has produced the poly cross terms.
-static int verifier_round_vec(dmx_verifier* v, const zq* d_l, const zq* d_r) {
+// A replay round in four steps: the vector challenge x from C and the vector cross terms; the
+// update C_t = C + x^-1 L + x R into Ct; the poly challenge y from C_t and the poly cross terms;
+// the update C = even(C_t) + y^-1 L + y R, which also advances the round. A round folds rows
+// while more than one is left and coefficients while the ring length is above one.
+//
+// `inverse`: the challenge kernel also computes the inverse. Otherwise the update takes it from
+// `hint`, a candidate the verifier accepts only after checking x * hint == 1 (the update computes
+// the inverse itself when the check fails), so the result never depends on where the candidate
+// came from. The fused prove-and-verify job passes the prover's own inverse of the same
+// challenge: the verifier's chain then carries no inversion.
+static int verifier_vec_challenge(dmx_verifier* v, const zq* d_l, const zq* d_r, bool inverse) {
     if (v->rows <= 1) return DMX_OK;
     const size_t n = v->clen;
-    zq* ch = v->round_chal(v->round);
-    v->tr.challenge("vec", v->round, v->C.as<zq>(), n, d_l, n, d_r, n, ch, v->flags.as<uint32_t>(), v->s, v->jb);
-    launch_com_update(v->C.as<zq>(), d_l, d_r, ch, 0, log2_exact(n), v->Ct.as<zq>(), v->s, v->jb);
+    v->tr.challenge("vec", v->round, v->C.as<zq>(), n, d_l, n, d_r, n, v->round_chal(v->round),
+                    v->flags.as<uint32_t>(), v->s, v->jb, inverse);
     v->vec_mask |= 1u << v->round;
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
 
-// The poly half: the poly challenge and the folded commitment back into C; advances the round.
-static int verifier_round_poly(dmx_verifier* v, const zq* d_lp, const zq* d_rp) {
-    hipStream_t s = v->s;
-    const Jobs& jb = v->jb;
+static int verifier_vec_update(dmx_verifier* v, const zq* d_l, const zq* d_r, const zq* hint) {
+    if (v->rows <= 1) return DMX_OK;
+    launch_com_update(v->C.as<zq>(), d_l, d_r, v->round_chal(v->round), 0, log2_exact(v->clen), v->Ct.as<zq>(), v->s,
+                      v->jb, hint);
+    HIP_TRY(hipGetLastError());
+    return DMX_OK;
+}
+
+static int verifier_poly_challenge(dmx_verifier* v, const zq* d_lp, const zq* d_rp, bool inverse) {
+    if (v->glen <= 1) return DMX_OK;
+    const size_t n = v->clen;
+    const zq* Ct = v->rows > 1 ? v->Ct.as<zq>() : v->C.as<zq>();
+    v->tr.challenge("poly", v->round, Ct, n, d_lp, n / 2, d_rp, n / 2, v->round_chal(v->round) + 2,
+                    v->flags.as<uint32_t>(), v->s, v->jb, inverse);
+    v->poly_mask |= 1u << v->round;
+    HIP_TRY(hipGetLastError());
+    return DMX_OK;
+}
+
+static int verifier_poly_update(dmx_verifier* v, const zq* d_lp, const zq* d_rp, const zq* hint) {
     const size_t n = v->clen;
     const bool vec = v->rows > 1, poly = v->glen > 1;
     const zq* Ct = vec ? v->Ct.as<zq>() : v->C.as<zq>();
     size_t nclen = n, nglen = v->glen;
     if (poly) {
-        zq* ch = v->round_chal(v->round);
-        v->tr.challenge("poly", v->round, Ct, n, d_lp, n / 2, d_rp, n / 2, ch + 2, v->flags.as<uint32_t>(), s, jb);
-        launch_com_update(Ct, d_lp, d_rp, ch + 2, 1, log2_exact(n / 2), v->C.as<zq>(), s, jb);
-        v->poly_mask |= 1u << v->round;
+        launch_com_update(Ct, d_lp, d_rp, v->round_chal(v->round) + 2, 1, log2_exact(n / 2), v->C.as<zq>(), v->s,
+                          v->jb, hint);
         nclen = n / 2;
         nglen = v->glen / 2;
     } else if (vec) {
-        launch_copy16(v->C.p, v->Ct.p, (size_t)K * n, s, jb);
+        launch_copy16(v->C.p, v->Ct.p, (size_t)K * n, v->s, v->jb);
     }
     HIP_TRY(hipGetLastError());
     if (vec) v->rows /= 2;
@@ -1368,8 +1390,10 @@ static int verifier_round_poly(dmx_verifier* v, const zq* d_lp, const zq* d_rp) 
 // generator folds of the round (verifier.rs:75, 102) are not materialised: the round's challenge
 // inverses are kept and the terminal generator is formed once, at the final opening.
 static int verifier_round_launch(dmx_verifier* v, const zq* d_l, const zq* d_r, const zq* d_lp, const zq* d_rp) {
-    int rc = verifier_round_vec(v, d_l, d_r);
-    return rc ? rc : verifier_round_poly(v, d_lp, d_rp);
+    int rc = verifier_vec_challenge(v, d_l, d_r, true);
+    if (!rc) rc = verifier_vec_update(v, d_l, d_r, nullptr);
+    if (!rc) rc = verifier_poly_challenge(v, d_lp, d_rp, true);
+    return rc ? rc : verifier_poly_update(v, d_lp, d_rp, nullptr);
 }
 
 // update_commitment (verifier.rs:57-120); every check runs before any device work, so a
@@ -1631,9 +1655,8 @@ struct BatchJob {
     std::unique_ptr<dmx_verifier> v;
     std::vector<hipEvent_t> evs;
     hipEvent_t done = nullptr;
-    // the verifier's generator expansion runs on its own stream, started once the prover's
-    // transform-heavy first rounds are done, so it fills the latency-bound rounds instead of
-    // competing with those transforms; the replay needs the matrix only at the final opening
+    // the early fold of the verifier's generator matrix runs on its own stream while the proof's
+    // last rounds are in flight; the replay needs the folded matrix only at the final opening
     hipStream_t gs = nullptr;
     hipEvent_t gen_done = nullptr;
     hipGraph_t graph = nullptr;  // jobs expand their own generators
@@ -1718,59 +1741,55 @@ static int batch_enqueue(BatchJob* J) {
     launch_copy_in(p->meta.as<uint64_t>(), p->file.as<uint8_t>(), p->hash_cap, p->s, p->jb);
     int rc = prover_init_file(p);
     if (rc) return rc;
-    // fork the verifier stream (its generator expansion goes to J->gs after the prover's first
-    // rounds, below)
+    // fork the verifier stream
     HIP_TRY(hipEventRecord(J->evs[0], p->s));
     HIP_TRY(hipStreamWaitEvent(v->s, J->evs[0], 0));
+    launch_fill32(v->flags.as<uint32_t>(), 64, 0, v->s, v->jb);
+    // The generator matrix is a function of the statement's seed alone (verifier.rs:31-33).
+    // Without a resident copy, the prover's first transform, which expands every generator row
+    // for com_0 anyway, also stores the rows into the verifier's matrix: one expansion per job.
     const bool own_gen = v->G0_src == nullptr;
-    const size_t gen_after = p->d >= 3 ? 1 : 0;  // prover round after which the expansion starts
-    if (!own_gen) {
-        rc = verifier_enqueue_gen(v);
-        if (rc) return rc;
-    } else {
-        launch_fill32(v->flags.as<uint32_t>(), 64, 0, v->s, v->jb);
-    }
-    rc = prover_init_commit(p);
+    rc = prover_init_commit(p, own_gen ? v->G0.as<zq>() : nullptr);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(J->evs[1], p->s));
     HIP_TRY(hipStreamWaitEvent(v->s, J->evs[1], 0));
     rc = verifier_enqueue_statement(v, p->tr.file_id(), p->com0.as<zq>());
     if (rc) return rc;
-    // The verifier follows half a round behind: its vector half of round j needs the prover's
-    // round j-1 (the vector cross terms of round j), its poly half the prover's vector stage of
-    // round j (the poly cross terms). The side stream gs expands the generator matrix after the
-    // prover's round gen_after and folds it by the early rounds' challenges once the verifier
-    // has them (after its round early_after), so the final opening only folds what is left.
+    // The verifier follows the prover stage by stage. Its vector challenge of round j needs the
+    // prover's round j-1 (the vector cross terms of round j); the update with x^-1 and the poly
+    // challenge need the prover's vector stage of round j (the poly cross terms, and the inverse
+    // it offers as a hint); the poly update needs the prover's poly stage, so it runs at the start
+    // of the next round. The side stream gs folds the generator matrix by the early rounds'
+    // challenges once the verifier has them all (at the start of round T), so the final opening
+    // only folds what is left.
     const size_t d = (size_t)p->d;
     const size_t T = gen_split(v).T;
-    const size_t early_after = T ? std::max(T - 1, own_gen ? gen_after : 0) : d;
-    const bool side = own_gen || T;
     for (size_t j = 0; j < d; ++j) {
         rc = prover_round(p, j, J->evs[2 + d + j]);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(J->evs[2 + j], p->s));
-        if (own_gen && j == gen_after) {
-            HIP_TRY(hipStreamWaitEvent(J->gs, J->evs[2 + j], 0));
-            // one workgroup per CU: the expansion has the whole tail of the proof to finish in
-            launch_gen_generators(v->gseed(), v->rows0 * K, log2_exact(v->n0), v->G0.as<zq>(), J->gs, v->jb,
-                                  p->ctx->cus);
-            HIP_TRY(hipGetLastError());
+        if (j > 0) {
+            rc = verifier_poly_update(v, p->rec_lpoly(j - 1), p->rec_rpoly(j - 1), p->sc(4 * (j - 1) + 3));
+            if (rc) return rc;
         }
-        rc = verifier_round_vec(v, p->rec_lvec(j), p->rec_rvec(j));
-        if (rc) return rc;
-        HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + d + j], 0));
-        rc = verifier_round_poly(v, p->rec_lpoly(j), p->rec_rpoly(j));
-        if (rc) return rc;
-        if (j == early_after) {
+        if (T && j == T) {
             HIP_TRY(hipEventRecord(J->evs[2 + 2 * d], v->s));
             HIP_TRY(hipStreamWaitEvent(J->gs, J->evs[2 + 2 * d], 0));
             rc = verifier_enqueue_early(v, J->gs);
             if (rc) return rc;
         }
+        rc = verifier_vec_challenge(v, p->rec_lvec(j), p->rec_rvec(j), false);
+        if (rc) return rc;
+        HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + d + j], 0));
+        rc = verifier_vec_update(v, p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j + 1));
+        if (!rc) rc = verifier_poly_challenge(v, p->rec_lpoly(j), p->rec_rpoly(j), false);
+        if (rc) return rc;
         HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + j], 0));
     }
+    rc = verifier_poly_update(v, p->rec_lpoly(d - 1), p->rec_rpoly(d - 1), p->sc(4 * (d - 1) + 3));
+    if (rc) return rc;
     p->round = (size_t)p->d;
-    if (side) {
+    if (T) {
         HIP_TRY(hipEventRecord(J->gen_done, J->gs));
         HIP_TRY(hipStreamWaitEvent(v->s, J->gen_done, 0));
     }
