@@ -9,8 +9,9 @@ invariant commitments and every Fiat-Shamir challenge, the verifier's replay (it
 own challenges) and the final opening check, then the proof bytes copied to the host. The
 served path is the continuous-batching server (dmx_server_*), which keeps the generator matrix
 of each (seed, depth) resident in HBM (derived once, in the untimed warm-up pass); the plain
-path runs the same trace one request at a time through dmx_prove_verify_device, which derives
-the generators in both prover and verifier on every call. `value` is MiB of request files per
+path runs the same trace one request at a time through dmx_prove_verify_device, which expands
+the generators on every call (once: the prover's first transform also fills the verifier's
+matrix). `value` is MiB of request files per
 second from the first arrival to the last verified result; p50/p99 are arrival-to-verified-
 result latencies. Every served proof is compared with the plain path's.
 
