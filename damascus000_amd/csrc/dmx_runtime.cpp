@@ -2605,7 +2605,7 @@ static void server_finisher(dmx_server* S) {
 static void server_loop(dmx_server* S) {
     (void)hipSetDevice(S->ctx->device);
     const uint32_t max_batch = S->cfg.max_batch ? S->cfg.max_batch : 64;
-    const uint32_t max_inflight = S->cfg.max_inflight ? S->cfg.max_inflight : 3;
+    const uint32_t max_inflight = S->cfg.max_inflight ? S->cfg.max_inflight : 2;
     for (;;) {
         // completions, in any order, handed to the finisher
         for (size_t i = 0; i < S->flights.size();) {
