@@ -238,9 +238,8 @@ size_t device_hash_scratch_words(size_t len);
 
 // generator expansion into Z_q rows: g[(i*K + k)*m + c] (src/commitment/sis.rs:212-245)
 // gseed: 8 key words in device memory
-// max_wgs > 0 caps the grid (per job): the expansion then loops and leaves CUs to other streams
 void launch_gen_generators(const uint32_t* gseed, size_t rows, int log_m, zq* out, hipStream_t s,
-                           const Jobs& jb = Jobs{}, unsigned max_wgs = 0);
+                           const Jobs& jb = Jobs{});
 
 // the reference-compatible batch entry points
 void launch_fold_pairs(const uint64_t* l_lo, const uint64_t* l_hi, const uint64_t* r_lo,

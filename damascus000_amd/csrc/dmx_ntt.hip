@@ -1656,7 +1656,6 @@ void launch_expand(const uint8_t* file, size_t len, zq* out, size_t capacity, hi
                        out, capacity, jb.stride, dlen);
 }
 
-// grid-stride over the coefficients, so a capped grid leaves CUs free for concurrent work
 __global__ void k_gen_generators(const uint32_t* gseed, size_t total, int log_m, zq* __restrict__ out,
                                  size_t js) {
     const size_t jo = job_off(js);
@@ -1666,20 +1665,17 @@ __global__ void k_gen_generators(const uint32_t* gseed, size_t total, int log_m,
     for (int i = 0; i < 8; ++i) key[i] = g[i];
     out = jp(out, jo);
     const size_t m = (size_t)1 << log_m;
-    for (size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x; gid < total; gid += (size_t)gridDim.x * 256) {
-        const size_t row = gid >> log_m, c = gid & (m - 1);
-        store_zq(out + gid, gen_coeff(key, row / K, (uint32_t)(row % K), c));
-    }
+    const size_t gid = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= total) return;
+    const size_t row = gid >> log_m, c = gid & (m - 1);
+    store_zq(out + gid, gen_coeff(key, row / K, (uint32_t)(row % K), c));
 }
 
-void launch_gen_generators(const uint32_t* gseed, size_t rows, int log_m, zq* out, hipStream_t s, const Jobs& jb,
-                           unsigned max_wgs) {
+void launch_gen_generators(const uint32_t* gseed, size_t rows, int log_m, zq* out, hipStream_t s, const Jobs& jb) {
     const size_t total = rows << log_m;
     if (!total || jb.n == 0) return;
-    size_t wgs = (total + 255) / 256;
-    if (max_wgs && wgs > max_wgs) wgs = max_wgs;
-    hipLaunchKernelGGL(k_gen_generators, dim3((unsigned)wgs, jb.n), dim3(256), 0, s, gseed, total, log_m, out,
-                       jb.stride);
+    hipLaunchKernelGGL(k_gen_generators, dim3((unsigned)((total + 255) / 256), jb.n), dim3(256), 0, s, gseed, total,
+                       log_m, out, jb.stride);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -249,7 +249,6 @@ struct TimedLaunch {
 
 struct dmx_ctx {
     int device = 0;
-    unsigned cus = 256;  // compute units of the device
     DevBuf tables;
     NttTables tw{};
     PrimeConsts pc[NPRIMES];
@@ -519,10 +518,6 @@ extern "C" int dmx_ctx_create(int device_index, dmx_ctx** out) {
     HIP_TRY(hipSetDevice(device_index));
     std::unique_ptr<dmx_ctx> c(new dmx_ctx);
     c->device = device_index;
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_index) == hipSuccess && cus > 0)
-        c->cus = (unsigned)cus;
-    (void)hipGetLastError();
     std::vector<uint32_t> host;
     build_constants(c.get(), host);
     int rc = c->tables.alloc(host.size() * 4);
@@ -1193,12 +1188,11 @@ static void verifier_plan(dmx_verifier* v, Layout& L) {
 
 static void verifier_bind(dmx_verifier* v) { v->tr.seeds = v->meta.as<uint8_t>() + META_SEEDS; }
 
-// Generator derivation from g_0_seed (verifier.rs:31-33); depends on the seed only, so a fused
-// job starts it before the prover has produced the statement. Kernels only.
-static int verifier_enqueue_gen(dmx_verifier* v, hipStream_t gs = nullptr) {
+// Generator derivation from g_0_seed (verifier.rs:31-33) for a standalone verifier (a fused job
+// takes the rows from the prover's first transform instead). Kernels only.
+static int verifier_enqueue_gen(dmx_verifier* v) {
     launch_fill32(v->flags.as<uint32_t>(), 64, 0, v->s, v->jb);
-    if (!v->G0_src)
-        launch_gen_generators(v->gseed(), v->rows0 * K, log2_exact(v->n0), v->G0.as<zq>(), gs ? gs : v->s, v->jb);
+    if (!v->G0_src) launch_gen_generators(v->gseed(), v->rows0 * K, log2_exact(v->n0), v->G0.as<zq>(), v->s, v->jb);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
@@ -1658,7 +1652,7 @@ struct BatchJob {
     // the early fold of the verifier's generator matrix runs on its own stream while the proof's
     // last rounds are in flight; the replay needs the folded matrix only at the final opening
     hipStream_t gs = nullptr;
-    hipEvent_t gen_done = nullptr;
+    hipEvent_t side_done = nullptr;
     hipGraph_t graph = nullptr;  // jobs expand their own generators
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph_r = nullptr;  // jobs read the resident matrix `gen`
@@ -1690,7 +1684,7 @@ struct BatchJob {
         if (graph) (void)hipGraphDestroy(graph);
         for (auto e : evs) (void)hipEventDestroy(e);
         if (done) (void)hipEventDestroy(done);
-        if (gen_done) (void)hipEventDestroy(gen_done);
+        if (side_done) (void)hipEventDestroy(side_done);
         if (gs) (void)hipStreamDestroy(gs);
     }
     size_t bytes() const { return store.bytes; }
@@ -1723,7 +1717,7 @@ static int batch_create(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
     J->evs.resize(2 * (size_t)d + 4);
     for (auto& e : J->evs) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&J->done, hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&J->gen_done, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&J->side_done, hipEventDisableTiming));
     HIP_TRY(hipStreamCreate(&J->gs));
     *out = J.release();
     return DMX_OK;
@@ -1790,8 +1784,8 @@ static int batch_enqueue(BatchJob* J) {
     if (rc) return rc;
     p->round = (size_t)p->d;
     if (T) {
-        HIP_TRY(hipEventRecord(J->gen_done, J->gs));
-        HIP_TRY(hipStreamWaitEvent(v->s, J->gen_done, 0));
+        HIP_TRY(hipEventRecord(J->side_done, J->gs));
+        HIP_TRY(hipStreamWaitEvent(v->s, J->side_done, 0));
     }
     // m* = message[0].coeffs[0] (prover.rs:133-138)
     rc = verifier_final_launch(v, p->W_a.as<zq>(), T != 0);
