@@ -390,6 +390,8 @@ def main():
                                "K=8, files resident in HBM", "requests": args.requests, "rate_per_s": args.rate,
                    "module_rank": 8, "parallelism": "one GPU; jobs of one depth share launches"},
         "single_file": single,
+        "transform_tuning": {f"2^{lg}": dict(zip(("variant", "variants", "tuned_us"), ctx.transform_variant(lg)))
+                             for lg in range(6, 12)},
         "roofline": {"bound": "valu", "kernel": "k_ntt_fwd_reg<32,32> (forward transform, degree 1024, 8 primes, "
                                                 "fused row producer)",
                      "achieved": round(achieved, 3), "peak": round(PEAK_LANE_CYCLES / 1e12, 3),

@@ -146,6 +146,9 @@ _SIGS = {
     "dmx_device_copy": (ctypes.c_int, [_VP, _VP, _VP, _SZ]),
     "dmx_kernel_timing_enable": (ctypes.c_int, [_VP, ctypes.c_int]),
     "dmx_kernel_timing_degree": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "dmx_ctx_transform_variant": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]),
+    "dmx_ctx_set_transform_variant": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     "dmx_kernel_timing": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
                                          ctypes.POINTER(ctypes.c_double)]),
     "dmx_statement_size": (_SZ, [_SZ]),
@@ -244,6 +247,17 @@ class Context:
         ms, n, w = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double()
         _check(lib().dmx_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(w)))
         return ms.value, n.value, w.value
+
+    def transform_variant(self, log_m):
+        """(chosen variant, variant count, tuning times in us) of the degree-2^log_m forward
+        transform"""
+        v, n = ctypes.c_int(), ctypes.c_int()
+        us = (ctypes.c_float * 4)()
+        _check(lib().dmx_ctx_transform_variant(self.h, log_m, ctypes.byref(v), ctypes.byref(n), us))
+        return v.value, n.value, [round(us[i], 2) for i in range(n.value)]
+
+    def set_transform_variant(self, log_m, variant):
+        _check(lib().dmx_ctx_set_transform_variant(self.h, log_m, variant))
 
 
 class DeviceBuffer:

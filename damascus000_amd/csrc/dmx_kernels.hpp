@@ -85,7 +85,7 @@ void set_device_constants(const PrimeConsts* pc, const CrtConsts* crt, hipStream
 
 // forward negacyclic NTT of `rows` rows of degree 2^log_m, all 8 primes; out[(row*8+p)*m + c]
 void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out,
-                         const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
+                         const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{}, int variant = -1);
 // A challenge inverse carried by a forward-transform launch: an extra workgroup 0 sets
 // chal[1] = chal[0]^-1 and *u_out = *u_prev * chal[0]^-1 (u_prev null: 1) while the other
 // workgroups transform rows. chal null: no task.
@@ -95,9 +95,12 @@ struct InvTask {
     zq* u_out = nullptr;
 };
 // two row sets of the same degree in one launch (segment 2 may be empty)
+// `variant` picks the transform kernel (fwd_variant_count(log_m) of them, all computing the same
+// residues; -1: variant 0)
 void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, const RowSrc& src2, size_t rows2,
                           uint32_t* out2, int log_m, const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{},
-                          const InvTask& it = InvTask{});
+                          const InvTask& it = InvTask{}, int variant = -1);
+int fwd_variant_count(int log_m);
 // inverse NTT in place on `slices` slices of [8][m]: data[(slice*8 + p)*m + c], times m^-1
 void launch_ntt_inv(uint32_t* data, size_t slices, int log_m, const NttTables& tw, hipStream_t s,
                     const Jobs& jb = Jobs{});

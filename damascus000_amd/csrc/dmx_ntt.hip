@@ -315,19 +315,34 @@ static bool reg_ntt_enabled() {
     return on;
 }
 
+// Variants per degree: variant 0 and 1 are register blockings (E elements x TP threads per prime
+// slice; TP <= E, TP <= 32, E * TP = m), the last is the per-layer kernel. Every variant computes
+// the same residues; the context picks the fastest per degree by timing.
+int fwd_variant_count(int log_m) {
+    if (!reg_ntt_enabled()) return 1;
+    return log_m >= 6 && log_m <= 10 ? 3 : log_m == 11 ? 2 : 1;
+}
+
 void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, const RowSrc& src2, size_t rows2,
                           uint32_t* out2, int log_m, const NttTables& tw, hipStream_t s, const Jobs& jb,
-                          const InvTask& it) {
+                          const InvTask& it, int variant) {
     if (rows1 + rows2 == 0 || jb.n == 0) return;
     const FwdSeg s1{src1, out1, (uint32_t)rows1}, s2{src2, out2, (uint32_t)rows2};
-    if (reg_ntt_enabled()) {
-        switch (log_m) {
-            case 6: return launch_reg<8, 8>(s1, s2, tw, s, jb, it);
-            case 7: return launch_reg<16, 8>(s1, s2, tw, s, jb, it);
-            case 8: return launch_reg<16, 16>(s1, s2, tw, s, jb, it);
-            case 9: return launch_reg<32, 16>(s1, s2, tw, s, jb, it);
-            case 10: return launch_reg<32, 32>(s1, s2, tw, s, jb, it);
-            case 11: return launch_reg<64, 32>(s1, s2, tw, s, jb, it);
+    const int nv = fwd_variant_count(log_m);
+    const int v = variant >= 0 && variant < nv ? variant : 0;
+    if (v < nv - 1) {  // a register blocking (nv > 1 only for degrees 2^6..2^11 with them enabled)
+        switch (log_m * 2 + v) {
+            case 12: return launch_reg<8, 8>(s1, s2, tw, s, jb, it);
+            case 13: return launch_reg<16, 4>(s1, s2, tw, s, jb, it);
+            case 14: return launch_reg<16, 8>(s1, s2, tw, s, jb, it);
+            case 15: return launch_reg<32, 4>(s1, s2, tw, s, jb, it);
+            case 16: return launch_reg<16, 16>(s1, s2, tw, s, jb, it);
+            case 17: return launch_reg<32, 8>(s1, s2, tw, s, jb, it);
+            case 18: return launch_reg<32, 16>(s1, s2, tw, s, jb, it);
+            case 19: return launch_reg<64, 8>(s1, s2, tw, s, jb, it);
+            case 20: return launch_reg<32, 32>(s1, s2, tw, s, jb, it);
+            case 21: return launch_reg<64, 16>(s1, s2, tw, s, jb, it);
+            case 22: return launch_reg<64, 32>(s1, s2, tw, s, jb, it);
             default: break;
         }
     }
@@ -339,8 +354,8 @@ void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, cons
 }
 
 void launch_ntt_fwd_rows(const RowSrc& src, size_t rows, int log_m, uint32_t* out, const NttTables& tw,
-                         hipStream_t s, const Jobs& jb) {
-    launch_ntt_fwd_rows2(src, rows, out, RowSrc{}, 0, nullptr, log_m, tw, s, jb);
+                         hipStream_t s, const Jobs& jb, int variant) {
+    launch_ntt_fwd_rows2(src, rows, out, RowSrc{}, 0, nullptr, log_m, tw, s, jb, InvTask{}, variant);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -249,6 +249,11 @@ struct TimedLaunch {
 
 struct dmx_ctx {
     int device = 0;
+    // forward-transform kernel per degree (launch_ntt_fwd_rows2's variant), timed at creation;
+    // fwd_version counts changes so captured graphs can be retaken
+    int fwd_variant[MAX_LOG_N + 1];
+    float fwd_us[MAX_LOG_N + 1][4];  // tuning times per variant (0: not timed)
+    uint64_t fwd_version = 0;
     DevBuf tables;
     NttTables tw{};
     PrimeConsts pc[NPRIMES];
@@ -297,8 +302,9 @@ static double fwd_row_work(int log_m, int mode) {
 static void fwd_rows2(dmx_ctx* ctx, const RowSrc& src1, size_t rows1, uint32_t* out1, const RowSrc& src2,
                       size_t rows2, uint32_t* out2, int log_m, hipStream_t s, const Jobs& jb,
                       const InvTask& it = InvTask{}) {
+    const int v = ctx->fwd_variant[log_m];
     if (!ctx->timing || rows1 + rows2 == 0 || (ctx->timing_log_m >= 0 && ctx->timing_log_m != log_m)) {
-        launch_ntt_fwd_rows2(src1, rows1, out1, src2, rows2, out2, log_m, ctx->tw, s, jb, it);
+        launch_ntt_fwd_rows2(src1, rows1, out1, src2, rows2, out2, log_m, ctx->tw, s, jb, it, v);
         return;
     }
     std::lock_guard<std::mutex> g(ctx->mu);
@@ -306,7 +312,7 @@ static void fwd_rows2(dmx_ctx* ctx, const RowSrc& src1, size_t rows1, uint32_t* 
                                         (double)rows2 * fwd_row_work(log_m, src2.mode));
     TimedLaunch t{ctx->take_event(), ctx->take_event(), work};
     (void)hipEventRecord(t.a, s);
-    launch_ntt_fwd_rows2(src1, rows1, out1, src2, rows2, out2, log_m, ctx->tw, s, jb, it);
+    launch_ntt_fwd_rows2(src1, rows1, out1, src2, rows2, out2, log_m, ctx->tw, s, jb, it, v);
     (void)hipEventRecord(t.b, s);
     ctx->launches.push_back(t);
 }
@@ -512,6 +518,88 @@ static void build_constants(dmx_ctx* c, std::vector<uint32_t>& host_tables) {
     for (int i = 0; i < NPRIMES; ++i) cr.half_digits[i] = big_divmod(h, PRIMES[i]);
 }
 
+// Forward-transform autotuning: per degree 2^6..2^11, every kernel variant transforms the same
+// synthetic fold rows (a + s*b over 2^21 / m rows, the shape of the prover's folds) and the
+// fastest of three timed launches each is kept. All variants give the same residues, so the
+// choice changes speed only. DAMASCUS_NTT_TUNE=0 keeps variant 0 everywhere.
+static int tune_transforms(dmx_ctx* c) {
+    for (int l = 0; l <= MAX_LOG_N; ++l) {
+        c->fwd_variant[l] = 0;
+        for (int v = 0; v < 4; ++v) c->fwd_us[l][v] = 0.f;
+    }
+    const char* e = getenv("DAMASCUS_NTT_TUNE");
+    if (e && strcmp(e, "0") == 0) return DMX_OK;
+    const size_t max_rows = (size_t)1 << 15, coeffs = (size_t)1 << 21;
+    DevBuf in, outb, scal;
+    int rc = in.alloc(2 * coeffs * 16);
+    if (!rc) rc = outb.alloc(coeffs * NPRIMES * 4);
+    if (!rc) rc = scal.alloc(16);
+    if (rc) return rc;
+    launch_fill32(in.as<uint32_t>(), in.bytes / 4, 0x0000ABCDu, 0);  // every 16-byte word below q
+    const uint32_t s_words[4] = {0x12345678u, 0x9ABCDEF0u, 0x0FEDCBA9u, 0x00001234u};  // < q
+    HIP_TRY(hipMemcpy(scal.p, s_words, 16, hipMemcpyHostToDevice));
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    int result = DMX_OK;
+    for (int l = 6; l <= 11 && result == DMX_OK; ++l) {
+        const int nv = fwd_variant_count(l);
+        if (nv < 2) continue;
+        const size_t rows = std::min(max_rows, coeffs >> l);
+        RowSrc src{};
+        src.mode = ROW_VFOLD;
+        src.a = in.as<zq>();
+        src.b_off = rows;
+        src.scal = scal.as<zq>();
+        float best = 0.f;
+        for (int v = 0; v < nv && result == DMX_OK; ++v) {
+            float t_min = 0.f;
+            for (int rep = 0; rep < 4; ++rep) {  // the first launch loads the code object
+                (void)hipEventRecord(a, 0);
+                launch_ntt_fwd_rows(src, rows, l, outb.as<uint32_t>(), c->tw, 0, Jobs{}, v);
+                (void)hipEventRecord(b, 0);
+                if (hipEventSynchronize(b) != hipSuccess || hipGetLastError() != hipSuccess) {
+                    result = fail(DMX_ERR_HIP, "transform tuning launch failed (degree 2^%d, variant %d)", l, v);
+                    break;
+                }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, a, b);
+                if (rep && (t_min == 0.f || ms < t_min)) t_min = ms;
+            }
+            c->fwd_us[l][v] = t_min * 1e3f;
+            if (result == DMX_OK && (v == 0 || t_min < best)) {
+                best = t_min;
+                c->fwd_variant[l] = v;
+            }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return result;
+}
+
+extern "C" int dmx_ctx_transform_variant(dmx_ctx* ctx, int log_m, int* variant, int* count, float* tuned_us) {
+    if (!ctx || log_m < 0 || log_m > MAX_LOG_N) return fail(DMX_ERR_ARG, "bad context or degree");
+    const int nv = fwd_variant_count(log_m);
+    if (variant) *variant = ctx->fwd_variant[log_m];
+    if (count) *count = nv;
+    if (tuned_us)
+        for (int v = 0; v < nv; ++v) tuned_us[v] = ctx->fwd_us[log_m][v];
+    return DMX_OK;
+}
+
+extern "C" int dmx_ctx_set_transform_variant(dmx_ctx* ctx, int log_m, int variant) {
+    if (!ctx || log_m < 0 || log_m > MAX_LOG_N) return fail(DMX_ERR_ARG, "bad context or degree");
+    if (variant < 0 || variant >= fwd_variant_count(log_m))
+        return fail(DMX_ERR_ARG, "degree 2^%d has %d transform variants", log_m, fwd_variant_count(log_m));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->fwd_variant[log_m] != variant) {
+        ctx->fwd_variant[log_m] = variant;
+        ++ctx->fwd_version;
+    }
+    return DMX_OK;
+}
+
 extern "C" int dmx_ctx_create(int device_index, dmx_ctx** out) {
     if (!out) return fail(DMX_ERR_ARG, "null output");
     if (!dmx_backend_ready()) return fail(DMX_ERR_NO_DEVICE, "no usable HIP device (or DAMASCUS_GPU=0)");
@@ -532,6 +620,8 @@ extern "C" int dmx_ctx_create(int device_index, dmx_ctx** out) {
     c->tw.invn_mp = base + 4 * NPRIMES * TW_SIZE + NPRIMES * (MAX_LOG_N + 1);
     set_device_constants(c->pc, &c->crt, 0);
     HIP_TRY(hipDeviceSynchronize());
+    rc = tune_transforms(c.get());
+    if (rc) return rc;
     *out = c.release();
     return DMX_OK;
 }
@@ -1625,7 +1715,7 @@ static int gen_acquire(dmx_ctx* ctx, const uint8_t seed[32], int d, hipStream_t 
     gen.mode = ROW_GEN;
     gen.dst = e->G.as<zq>();
     gen.gseed = e->key.as<uint32_t>();
-    launch_ntt_fwd_rows(gen, K * N, d, e->Ghat.as<uint32_t>(), ctx->tw, s);
+    launch_ntt_fwd_rows(gen, K * N, d, e->Ghat.as<uint32_t>(), ctx->tw, s, Jobs{}, ctx->fwd_variant[d]);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e->ready, s));
     e->refs = 1;
@@ -1657,6 +1747,7 @@ struct BatchJob {
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph_r = nullptr;  // jobs read the resident matrix `gen`
     hipGraphExec_t exec_r = nullptr;
+    uint64_t graph_version = 0;  // the context's fwd_version the graphs were captured with
     GenEntry* gen = nullptr;
     bool busy = false;
     uint64_t last_use = 0;
@@ -1668,6 +1759,14 @@ struct BatchJob {
     // not the runtime's pageable path through a bounce buffer
     void* pinned_out = nullptr;
     size_t pinned_out_bytes = 0;
+    void drop_graphs() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (exec_r) (void)hipGraphExecDestroy(exec_r);
+        if (graph_r) (void)hipGraphDestroy(graph_r);
+        exec = exec_r = nullptr;
+        graph = graph_r = nullptr;
+    }
     void drop_resident() {
         if (exec_r) (void)hipGraphExecDestroy(exec_r);
         if (graph_r) (void)hipGraphDestroy(graph_r);
@@ -1866,9 +1965,15 @@ static int batch_launch(BatchJob* J, const JobIn* jobs, bool host_files, bool re
         }
     }
     bool timing;
+    uint64_t fwd_version;
     {
         std::lock_guard<std::mutex> g(J->ctx->mu);
         timing = J->ctx->timing;
+        fwd_version = J->ctx->fwd_version;
+    }
+    if (J->graph_version != fwd_version) {  // the transform kernels changed: capture again
+        J->drop_graphs();
+        J->graph_version = fwd_version;
     }
     if (!graphs_enabled() || timing) {  // eager: kernel timing needs per-launch events
         int rc = batch_enqueue(J);
@@ -2318,13 +2423,13 @@ static int commit_impl(dmx_ctx* ctx, const uint64_t* witness, const uint64_t* ge
         RowSrc pw{};
         pw.mode = ROW_PLAIN;
         pw.a = W.as<zq>();
-        launch_ntt_fwd_rows(pw, nr, lg, What.as<uint32_t>(), ctx->tw, s);
+        launch_ntt_fwd_rows(pw, nr, lg, What.as<uint32_t>(), ctx->tw, s, Jobs{}, ctx->fwd_variant[lg]);
         RowSrc pg{};
         if (gens) {
             HIP_TRY(hipMemcpyAsync(G.p, gens + 2 * r0 * K * m, nr * K * m * 16, hipMemcpyHostToDevice, s));
             pg.mode = ROW_PLAIN;
             pg.a = G.as<zq>();
-            launch_ntt_fwd_rows(pg, nr * K, lg, Ghat.as<uint32_t>(), ctx->tw, s);
+            launch_ntt_fwd_rows(pg, nr * K, lg, Ghat.as<uint32_t>(), ctx->tw, s, Jobs{}, ctx->fwd_variant[lg]);
         } else {
             // generator rows r0.. of the family: generate the full family prefix, keep the slice
             DevBuf full;
@@ -2333,7 +2438,7 @@ static int commit_impl(dmx_ctx* ctx, const uint64_t* witness, const uint64_t* ge
             launch_gen_generators(key.as<uint32_t>(), (r0 + nr) * K, lg, full.as<zq>(), s);
             pg.mode = ROW_PLAIN;
             pg.a = full.as<zq>() + r0 * K * m;
-            launch_ntt_fwd_rows(pg, nr * K, lg, Ghat.as<uint32_t>(), ctx->tw, s);
+            launch_ntt_fwd_rows(pg, nr * K, lg, Ghat.as<uint32_t>(), ctx->tw, s, Jobs{}, ctx->fwd_variant[lg]);
             HIP_TRY(hipStreamSynchronize(s));
         }
         launch_mac_pairs_inv(What.as<uint32_t>(), Ghat.as<uint32_t>(), nr, lg, res.as<uint32_t>(), nullptr, nullptr,
@@ -2434,8 +2539,8 @@ static int ring_mul_device(dmx_ctx* ctx, const zq* dA, const zq* dB, zq* dO, con
         sa.mode = sb.mode = ROW_PLAIN;
         sa.a = ctx->ring_ga.as<zq>();
         sb.a = ctx->ring_gb.as<zq>();
-        launch_ntt_fwd_rows(sa, cnt, lg, ctx->ring_ra.as<uint32_t>(), ctx->tw, s);
-        launch_ntt_fwd_rows(sb, cnt, lg, ctx->ring_rb.as<uint32_t>(), ctx->tw, s);
+        launch_ntt_fwd_rows(sa, cnt, lg, ctx->ring_ra.as<uint32_t>(), ctx->tw, s, Jobs{}, ctx->fwd_variant[lg]);
+        launch_ntt_fwd_rows(sb, cnt, lg, ctx->ring_rb.as<uint32_t>(), ctx->tw, s, Jobs{}, ctx->fwd_variant[lg]);
         launch_pointwise(ctx->ring_ra.as<uint32_t>(), ctx->ring_rb.as<uint32_t>(), cnt, lg, s);
         launch_ntt_inv(ctx->ring_ra.as<uint32_t>(), cnt, lg, ctx->tw, s);
         launch_crt_lift(ctx->ring_ra.as<uint32_t>(), ctx->ring_ga.as<zq>(), cnt, lg, s);

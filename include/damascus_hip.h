@@ -255,6 +255,15 @@ int dmx_kernel_timing(dmx_ctx *ctx, double *total_ms, uint64_t *launches, double
 /* record only launches of degree 2^log_m (negative: every degree) */
 int dmx_kernel_timing_degree(dmx_ctx *ctx, int log_m);
 
+/* ---- forward-transform autotuning: per degree 2^log_m the context holds one of *count kernel
+ * variants (register blockings of E elements x TP threads per prime slice, or the per-layer
+ * kernel), all computing the same residues. dmx_ctx_create times each on synthetic fold rows and
+ * keeps the fastest (DAMASCUS_NTT_TUNE=0: variant 0). tuned_us (count floats, may be null)
+ * receives the tuning times, 0 where none ran. Setting a variant makes the context's cached
+ * graphs be captured again. No reference counterpart (its NTT has one kernel per call). */
+int dmx_ctx_transform_variant(dmx_ctx *ctx, int log_m, int *variant, int *count, float *tuned_us);
+int dmx_ctx_set_transform_variant(dmx_ctx *ctx, int log_m, int variant);
+
 /* sizes for buffer planning */
 size_t dmx_statement_size(size_t ring_len);
 size_t dmx_record_size(size_t vec_ring_len, size_t poly_ring_len);

@@ -340,6 +340,34 @@ def _rand_file(rng, n):
     return rng.integers(0, 256, size=n, dtype=np.uint8).tobytes()
 
 
+def test_transform_variants_give_identical_proofs():
+    """Every forward-transform variant (register blockings of the prime slices and the per-layer
+    kernel) gives the same proof: at d = 7 each against the oracle, at d = 11 (degrees 2^11 down to
+    2^6) across variants through the fused job, whose captured graph is retaken after a change."""
+    ctx = dm.Context.default()
+    saved = {lg: ctx.transform_variant(lg)[0] for lg in range(6, 12)}
+    say("tuned transform variants (variant, count, us):", {lg: ctx.transform_variant(lg) for lg in range(6, 12)})
+    small = W.pattern_bytes(W.byte_len_for_depth(7), 5)
+    big = np.random.default_rng(11).integers(0, 256, size=16 << 20, dtype=np.uint8).tobytes()  # d = 11
+    rc, ost, orecs, oop = O.full_flow(small)
+    assert rc == 0
+    buf = ctx.upload(big)
+    outs = []
+    try:
+        for v in range(3):
+            for lg in range(6, 12):
+                ctx.set_transform_variant(lg, min(v, ctx.transform_variant(lg)[1] - 1))
+            say("all degrees at variant", v)
+            assert dm.prove(small) == (ost, orecs, oop)
+            outs.append(dm.prove_verify_device(buf.ptr, len(big)))
+        assert outs[0] == outs[1] == outs[2]
+        with pytest.raises(dm.DamascusError):
+            ctx.set_transform_variant(11, 2)  # degree 2^11 has two variants
+    finally:
+        for lg, v in saved.items():
+            ctx.set_transform_variant(lg, v)
+
+
 def test_batch_of_one_depth_equals_solo_runs():
     """five d=6 files of different lengths and seeds proved side by side in one launch sequence:
     each proof equals proving that file alone and the oracle's, so no job reads another's data"""
