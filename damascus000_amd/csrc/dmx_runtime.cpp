@@ -1857,12 +1857,17 @@ static int batch_enqueue(BatchJob* J) {
     // only folds what is left.
     const size_t d = (size_t)p->d;
     const size_t T = gen_split(v).T;
+    // The inverse candidates offered to the verifier: the prover's x^-1 and y^-1 slots.
+    // DAMASCUS_VERIFIER_HINT=wrong offers x and y themselves (a test of the verifier's own
+    // inversion when the check x * candidate == 1 fails).
+    const char* he = getenv("DAMASCUS_VERIFIER_HINT");
+    const size_t hint_off = he && strcmp(he, "wrong") == 0 ? 0 : 1;
     for (size_t j = 0; j < d; ++j) {
         rc = prover_round(p, j, J->evs[2 + d + j]);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(J->evs[2 + j], p->s));
         if (j > 0) {
-            rc = verifier_poly_update(v, p->rec_lpoly(j - 1), p->rec_rpoly(j - 1), p->sc(4 * (j - 1) + 3));
+            rc = verifier_poly_update(v, p->rec_lpoly(j - 1), p->rec_rpoly(j - 1), p->sc(4 * (j - 1) + 2 + hint_off));
             if (rc) return rc;
         }
         if (T && j == T) {
@@ -1874,12 +1879,12 @@ static int batch_enqueue(BatchJob* J) {
         rc = verifier_vec_challenge(v, p->rec_lvec(j), p->rec_rvec(j), false);
         if (rc) return rc;
         HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + d + j], 0));
-        rc = verifier_vec_update(v, p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j + 1));
+        rc = verifier_vec_update(v, p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j + hint_off));
         if (!rc) rc = verifier_poly_challenge(v, p->rec_lpoly(j), p->rec_rpoly(j), false);
         if (rc) return rc;
         HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + j], 0));
     }
-    rc = verifier_poly_update(v, p->rec_lpoly(d - 1), p->rec_rpoly(d - 1), p->sc(4 * (d - 1) + 3));
+    rc = verifier_poly_update(v, p->rec_lpoly(d - 1), p->rec_rpoly(d - 1), p->sc(4 * (d - 1) + 2 + hint_off));
     if (rc) return rc;
     p->round = (size_t)p->d;
     if (T) {

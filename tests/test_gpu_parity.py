@@ -340,6 +340,26 @@ def _rand_file(rng, n):
     return rng.integers(0, 256, size=n, dtype=np.uint8).tobytes()
 
 
+def test_verifier_computes_inverses_when_the_candidate_fails_its_check():
+    """The fused job's verifier takes the prover's challenge inverses as candidates it checks
+    (x * candidate == 1); offered x itself instead, it inverts on its own and still accepts the
+    proof, whose bytes do not change. Eager launches, so no graph captured earlier is replayed."""
+    data = np.random.default_rng(31).integers(0, 256, size=13 * 4096 + 7, dtype=np.uint8).tobytes()  # d = 7
+    buf = dm.Context.default().upload(data)
+    want = dm.prove_verify_device(buf.ptr, len(data))
+    os.environ["DAMASCUS_GRAPH"] = "0"
+    os.environ["DAMASCUS_VERIFIER_HINT"] = "wrong"
+    try:
+        say("verifier with wrong inverse candidates")
+        got = dm.prove_verify_device(buf.ptr, len(data))
+    finally:
+        del os.environ["DAMASCUS_VERIFIER_HINT"]
+        del os.environ["DAMASCUS_GRAPH"]
+    assert got == want
+    rc, ost, orecs, oop = O.full_flow(data)
+    assert rc == 0 and got == (ost, orecs, oop)
+
+
 def test_transform_variants_give_identical_proofs():
     """Every forward-transform variant (register blockings of the prime slices and the per-layer
     kernel) gives the same proof: at d = 7 each against the oracle, at d = 11 (degrees 2^11 down to
