@@ -819,12 +819,34 @@ __global__ void __launch_bounds__(FIN_THREADS) k_mac_pairs_finish(const uint32_t
     const uint32_t kp = blockIdx.x, p = kp % NPRIMES;
     const PrimeConsts pc = c_pc[p];
     const size_t slot = (size_t)K * NPRIMES * m;
-    for (int e = threadIdx.x; e < nslots * m; e += FIN_THREADS) {
-        const int sl = e >> log_m, c = e & (m - 1);
-        uint64_t acc = 0;
+    if ((m & 3) == 0) {  // four positions per thread, one 16-byte load per segment
+        const int qm = m >> 2;
+        for (int e = threadIdx.x; e < nslots * qm; e += FIN_THREADS) {
+            const int sl = e >> (log_m - 2), c = (e & (qm - 1)) << 2;
+            const uint32_t* b = part + sl * slot + (size_t)kp * m + c;
+            uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
 #pragma unroll 8
-        for (uint32_t sg = 0; sg < nseg; ++sg) acc += part[(size_t)sg * 3 * slot + sl * slot + (size_t)kp * m + c];
-        sh[e] = reduce_u96(acc, 0, pc);
+            for (uint32_t sg = 0; sg < nseg; ++sg) {
+                const uint4 v = ld4(b + (size_t)sg * 3 * slot);
+                a0 += v.x;
+                a1 += v.y;
+                a2 += v.z;
+                a3 += v.w;
+            }
+            uint32_t* o = sh + sl * m + c;
+            o[0] = reduce_u96(a0, 0, pc);
+            o[1] = reduce_u96(a1, 0, pc);
+            o[2] = reduce_u96(a2, 0, pc);
+            o[3] = reduce_u96(a3, 0, pc);
+        }
+    } else {
+        for (int e = threadIdx.x; e < nslots * m; e += FIN_THREADS) {
+            const int sl = e >> log_m, c = e & (m - 1);
+            uint64_t acc = 0;
+#pragma unroll 8
+            for (uint32_t sg = 0; sg < nseg; ++sg) acc += part[(size_t)sg * 3 * slot + sl * slot + (size_t)kp * m + c];
+            sh[e] = reduce_u96(acc, 0, pc);
+        }
     }
     __syncthreads();
     lds_intt_slices(sh, nslots, log_m, (int)p, tw);
@@ -857,12 +879,34 @@ __global__ void __launch_bounds__(FIN_THREADS) k_mac_split_finish(const uint32_t
     uint32_t* sr = sl + h;
     uint32_t* sums = sh + 2 * m;  // S00, S11, S01, S10, h words each
     const uint32_t i4m = tw.invn_m[p * (MAX_LOG_N + 1) + 2], i4mp = tw.invn_mp[p * (MAX_LOG_N + 1) + 2];
-    for (int e = threadIdx.x; e < 4 * h; e += FIN_THREADS) {  // one (sum, pair) per thread
-        const int j = e >> (log_m - 1), t = e & (h - 1);
-        uint64_t a = 0;
+    if ((h & 3) == 0) {  // four pairs of one sum per thread, one 16-byte load per segment
+        const int qh = h >> 2;
+        for (int e = threadIdx.x; e < 4 * qh; e += FIN_THREADS) {
+            const int j = e >> (log_m - 3), t = (e & (qh - 1)) << 2;
+            const uint32_t* b = part + j * slot + (size_t)kp * h + t;
+            uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
 #pragma unroll 8
-        for (uint32_t sg = 0; sg < nseg; ++sg) a += part[(size_t)sg * 4 * slot + j * slot + (size_t)kp * h + t];
-        sums[e] = reduce_u96(a, 0, pc);
+            for (uint32_t sg = 0; sg < nseg; ++sg) {
+                const uint4 v = ld4(b + (size_t)sg * 4 * slot);
+                a0 += v.x;
+                a1 += v.y;
+                a2 += v.z;
+                a3 += v.w;
+            }
+            uint32_t* o = sums + j * h + t;
+            o[0] = reduce_u96(a0, 0, pc);
+            o[1] = reduce_u96(a1, 0, pc);
+            o[2] = reduce_u96(a2, 0, pc);
+            o[3] = reduce_u96(a3, 0, pc);
+        }
+    } else {
+        for (int e = threadIdx.x; e < 4 * h; e += FIN_THREADS) {  // one (sum, pair) per thread
+            const int j = e >> (log_m - 1), t = e & (h - 1);
+            uint64_t a = 0;
+#pragma unroll 8
+            for (uint32_t sg = 0; sg < nseg; ++sg) a += part[(size_t)sg * 4 * slot + j * slot + (size_t)kp * h + t];
+            sums[e] = reduce_u96(a, 0, pc);
+        }
     }
     __syncthreads();
     for (int t = threadIdx.x; t < h; t += FIN_THREADS) {
