@@ -284,12 +284,22 @@ __device__ uint32_t* merge_lds(uint32_t* a, uint32_t* b, uint32_t n, const uint3
     return a;
 }
 
+// n chaining values (8 words each, 32-byte aligned) from global memory into LDS with 16-byte
+// loads, a thread's loads issued together
+__device__ __forceinline__ void load_cvs_lds(uint32_t* dst, const uint32_t* src, uint32_t n) {
+    const uint4* s4 = reinterpret_cast<const uint4*>(src);
+    uint4* d4 = reinterpret_cast<uint4*>(dst);
+#pragma unroll 4
+    for (uint32_t i = threadIdx.x; i < 2 * n; i += blockDim.x) d4[i] = s4[i];
+}
+
 // With `dlen`, `n` is the capacity's chunk count and each job uses its own; a job whose chunk
 // values fit one group has no group level and skips.
 __global__ void __launch_bounds__(1024) k_b3_group_merge(const uint32_t* __restrict__ cvs, size_t n, KeyW key,
                                                          uint32_t flags, uint32_t* __restrict__ out, size_t js,
                                                          const uint64_t* dlen, size_t cap_len) {
-    __shared__ uint32_t sa[1024 * 8], sb[512 * 8 + 8];
+    __shared__ __align__(16) uint32_t sa[1024 * 8];
+    __shared__ uint32_t sb[512 * 8 + 8];
     const size_t base = (size_t)blockIdx.x * 1024;
     const size_t jo = job_off(js);
     if (dlen) {
@@ -299,7 +309,7 @@ __global__ void __launch_bounds__(1024) k_b3_group_merge(const uint32_t* __restr
     cvs = jp(cvs, jo);
     out = jp(out, jo);
     const uint32_t cnt = (uint32_t)((n - base) < 1024 ? (n - base) : 1024);
-    for (uint32_t i = threadIdx.x; i < cnt * 8; i += blockDim.x) sa[i] = cvs[base * 8 + i];
+    load_cvs_lds(sa, cvs + base * 8, cnt);
     __syncthreads();
     const uint32_t* r = merge_lds(sa, sb, cnt, key.w, flags, false);
     if (threadIdx.x < 8) out[blockIdx.x * 8 + threadIdx.x] = r[threadIdx.x];
@@ -324,7 +334,8 @@ __global__ void __launch_bounds__(512) k_b3_root(uint32_t* __restrict__ cvs, siz
                                                   uint32_t* err, int chunk_level, size_t js, const uint64_t* dlen,
                                                   size_t l1_off, int want_inv) {
     critical_path_priority();
-    __shared__ uint32_t sa[1024 * 8], sb[512 * 8 + 8];
+    __shared__ __align__(16) uint32_t sa[1024 * 8];
+    __shared__ uint32_t sb[512 * 8 + 8];
     __shared__ uint32_t dig[8];
     __shared__ int again;
     const size_t jo = job_off(js);
@@ -358,7 +369,7 @@ __global__ void __launch_bounds__(512) k_b3_root(uint32_t* __restrict__ cvs, siz
             }
             __syncthreads();
         } else {
-            for (uint32_t i = threadIdx.x; i < n * 8; i += blockDim.x) sa[i] = cvs[i];
+            load_cvs_lds(sa, cvs, (uint32_t)n);
             __syncthreads();
             const uint32_t* r = merge_lds(sa, sb, (uint32_t)n, key.w, flags, true);
             if (threadIdx.x < 8) dig[threadIdx.x] = r[threadIdx.x];
