@@ -11,6 +11,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 #include <cstdarg>
@@ -534,7 +535,7 @@ static int tune_transforms(dmx_ctx* c) {
     int rc = in.alloc(2 * coeffs * 16);
     if (!rc) rc = outb.alloc(coeffs * NPRIMES * 4);
     if (!rc) rc = scal.alloc(16);
-    if (rc) return rc;
+    if (rc) return DMX_OK;  // no room for the tuning rows: keep the defaults
     launch_fill32(in.as<uint32_t>(), in.bytes / 4, 0x0000ABCDu, 0);  // every 16-byte word below q
     const uint32_t s_words[4] = {0x12345678u, 0x9ABCDEF0u, 0x0FEDCBA9u, 0x00001234u};  // < q
     HIP_TRY(hipMemcpy(scal.p, s_words, 16, hipMemcpyHostToDevice));
@@ -1925,9 +1926,17 @@ static void staged_copy(uint8_t* dst, const uint8_t* src, size_t len) {
     const size_t per = (len + nt - 1) / nt;
     std::vector<std::thread> th;
     th.reserve(nt - 1);
-    for (size_t t = 1; t < nt; ++t) {
-        const size_t off = t * per, n = off < len ? std::min(per, len - off) : 0;
-        th.emplace_back([=] { memcpy(dst + off, src + off, n); });
+    size_t t = 1;
+    try {
+        for (; t < nt; ++t) {
+            const size_t off = t * per, n = off < len ? std::min(per, len - off) : 0;
+            th.emplace_back([=] { memcpy(dst + off, src + off, n); });
+        }
+    } catch (const std::system_error&) {  // no thread to spare: copy the remaining pieces here
+        for (; t < nt; ++t) {
+            const size_t off = t * per;
+            if (off < len) memcpy(dst + off, src + off, std::min(per, len - off));
+        }
     }
     memcpy(dst, src, std::min(per, len));
     for (auto& x : th) x.join();
