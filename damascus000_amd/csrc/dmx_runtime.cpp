@@ -1551,8 +1551,13 @@ static int verifier_enqueue_early(dmx_verifier* v, hipStream_t s) {
     return DMX_OK;
 }
 
+// The final check's result bit lives in its own flag word (word 0 collects the challenge kernels'
+// errors), cleared here unless `fresh_flags` (a fused job's flags were zeroed at its start).
+constexpr size_t FINAL_WORD = 1;
+
 // `early_done`: the early fold was already enqueued (a fused job runs it on its side stream)
-static int verifier_final_launch(dmx_verifier* v, const zq* d_opening, bool early_done = false) {
+static int verifier_final_launch(dmx_verifier* v, const zq* d_opening, bool early_done = false,
+                                 bool fresh_flags = false) {
     if (v->round != v->rounds) return fail(DMX_ERR_NOT_FINAL, "cannot verify final opening before all rounds replay");
     if (v->rows != 1) return fail(DMX_ERR_NOT_FINAL, "terminal generator family must have length one");
     if (v->glen != v->clen) return fail(DMX_ERR_OPENING, "final opening mismatch");
@@ -1567,17 +1572,17 @@ static int verifier_final_launch(dmx_verifier* v, const zq* d_opening, bool earl
     }
     const zq* H = g.T ? v->H.as<zq>() : v->G();
     const bool h_shared = !g.T && v->G0_src != nullptr;
-    launch_fill32(v->flags.as<uint32_t>(), 1, 0, v->s, v->jb);
+    if (!fresh_flags) launch_fill32(v->flags.as<uint32_t>() + FINAL_WORD, 1, 0, v->s, v->jb);
     launch_gen_contract_late(H, h_shared, v->chal.as<zq>() + 4 * g.T, v->vec_mask >> g.T, v->poly_mask >> g.T, g.log_lo,
-                             g.log_nt, v->C.as<zq>(), d_opening, v->flags.as<uint32_t>(), 1, v->gfin.as<zq>(), v->s,
-                             v->jb);
+                             g.log_nt, v->C.as<zq>(), d_opening, v->flags.as<uint32_t>() + FINAL_WORD, 1,
+                             v->gfin.as<zq>(), v->s, v->jb);
     HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
 
 static int verifier_final_result(dmx_verifier* v) {
     uint32_t f = 0;
-    HIP_TRY(hipMemcpyAsync(&f, v->flags.p, 4, hipMemcpyDeviceToHost, v->s));
+    HIP_TRY(hipMemcpyAsync(&f, v->flags.as<uint32_t>() + FINAL_WORD, 4, hipMemcpyDeviceToHost, v->s));
     HIP_TRY(hipStreamSynchronize(v->s));
     if (f) return fail(DMX_ERR_OPENING, "final opening mismatch");
     return DMX_OK;
@@ -1893,7 +1898,11 @@ static int batch_enqueue(BatchJob* J) {
         HIP_TRY(hipStreamWaitEvent(v->s, J->side_done, 0));
     }
     // m* = message[0].coeffs[0] (prover.rs:133-138)
-    rc = verifier_final_launch(v, p->W_a.as<zq>(), T != 0);
+    // DAMASCUS_TEST_OPENING_OFFSET=k checks the final opening against witness coefficient k
+    // instead of 0 (a test that the fused job's verifier rejects a wrong opening)
+    const char* oe = getenv("DAMASCUS_TEST_OPENING_OFFSET");
+    const size_t open_off = oe ? std::min<size_t>(strtoull(oe, nullptr, 10), p->N - 1) : 0;
+    rc = verifier_final_launch(v, p->W_a.as<zq>() + open_off, T != 0, true);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(J->evs[J->evs.size() - 1], v->s));
     HIP_TRY(hipStreamWaitEvent(p->s, J->evs[J->evs.size() - 1], 0));  // join
@@ -2057,7 +2066,7 @@ static int batch_collect(BatchJob* J, const JobOut* outs) {
     const size_t st = p->jb.stride;
     HIP_TRY(hipEventSynchronize(J->done));
     const size_t com_b = (size_t)K * p->N * 16, rb = p->rec_off[p->d] * 16;
-    const size_t need = (size_t)n * (4 + 4 + 32 + 16 + com_b + rb) + 64;
+    const size_t need = (size_t)n * (4 + 8 + 32 + 16 + com_b + rb) + 64;
     if (need > J->pinned_out_bytes) {
         if (J->pinned_out) (void)hipHostFree(J->pinned_out);
         J->pinned_out = nullptr;
@@ -2074,11 +2083,11 @@ static int batch_collect(BatchJob* J, const JobOut* outs) {
     zq* recs = reinterpret_cast<zq*>(base + n * com_b);
     zq* mstar = reinterpret_cast<zq*>(base + n * (com_b + rb));
     uint32_t* pf = reinterpret_cast<uint32_t*>(base + n * (com_b + rb + 16));
-    uint32_t* vf = pf + n;
-    uint8_t* fid = reinterpret_cast<uint8_t*>(vf + n);
+    uint32_t* vf = pf + n;  // per job: the challenge-error word and the final-check word
+    uint8_t* fid = reinterpret_cast<uint8_t*>(vf + 2 * n);
     hipStream_t s = p->s;
     HIP_TRY(copy_rows(pf, 4, p->flags.p, st, 4, n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(copy_rows(vf, 4, v->flags.p, st, 4, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(copy_rows(vf, 8, v->flags.p, st, 8, n, hipMemcpyDeviceToHost, s));
     HIP_TRY(copy_rows(fid, 32, p->tr.file_id(), st, 32, n, hipMemcpyDeviceToHost, s));
     HIP_TRY(copy_rows(com, com_b, p->com0.p, st, com_b, n, hipMemcpyDeviceToHost, s));
     HIP_TRY(copy_rows(recs, rb, p->recs.p, st, rb, n, hipMemcpyDeviceToHost, s));
@@ -2089,8 +2098,8 @@ static int batch_collect(BatchJob* J, const JobOut* outs) {
         const JobOut& o = outs[b];
         int rc = DMX_OK;
         for (size_t j = 0; !rc && j < (size_t)p->d; ++j) rc = check_round_flags(j, pf[b]);
-        if (!rc && (vf[b] & 1u)) rc = fail(DMX_ERR_OPENING, "final opening mismatch");
-        if (!rc && vf[b]) rc = fail(DMX_ERR_HIP, "challenge resampling exhausted");
+        if (!rc && vf[2 * b]) rc = fail(DMX_ERR_HIP, "challenge resampling exhausted");
+        if (!rc && vf[2 * b + 1]) rc = fail(DMX_ERR_OPENING, "final opening mismatch");
         const size_t st_need = dmx_statement_size(p->N);
         if (o.statement_len) *o.statement_len = st_need;
         if (!rc && o.statement) {

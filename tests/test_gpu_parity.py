@@ -360,6 +360,26 @@ def test_verifier_computes_inverses_when_the_candidate_fails_its_check():
     assert rc == 0 and got == (ost, orecs, oop)
 
 
+def test_fused_job_rejects_a_wrong_final_opening():
+    """The fused job's verifier checks g_d[0] * m* == C_d: checked against another witness
+    coefficient instead of m*, the job fails with ERR_OPENING (eager launches, so no graph
+    captured earlier is replayed); with the right opening the same file proves and verifies."""
+    data = np.random.default_rng(41).integers(0, 256, size=13 * 4096 + 9, dtype=np.uint8).tobytes()  # d = 7
+    buf = dm.Context.default().upload(data)
+    os.environ["DAMASCUS_GRAPH"] = "0"
+    os.environ["DAMASCUS_TEST_OPENING_OFFSET"] = "1"
+    try:
+        say("fused job with a wrong final opening")
+        with pytest.raises(dm.DamascusError) as e:
+            dm.prove_verify_device(buf.ptr, len(data))
+        assert e.value.name == "ERR_OPENING"
+    finally:
+        del os.environ["DAMASCUS_TEST_OPENING_OFFSET"]
+        del os.environ["DAMASCUS_GRAPH"]
+    rc, ost, orecs, oop = O.full_flow(data)
+    assert rc == 0 and dm.prove_verify_device(buf.ptr, len(data)) == (ost, orecs, oop)
+
+
 def test_transform_variants_give_identical_proofs():
     """Every forward-transform variant (register blockings of the prime slices and the per-layer
     kernel) gives the same proof: at d = 7 each against the oracle, at d = 11 (degrees 2^11 down to
