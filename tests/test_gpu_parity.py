@@ -189,6 +189,23 @@ def test_c1_random_file_full_flow_matches_oracle():
     dm.verify(st, recs, op)
 
 
+def test_edge_file_sizes_match_oracle():
+    """io.rs:72-120 edges: an empty file (one zero coefficient, d = 6), one byte, exactly 13 * 4^6
+    bytes (the largest d = 6 file) and one byte more (the smallest d = 7 file), through the
+    host-bytes path and the fused device path, each equal to the oracle's proof."""
+    ctx = dm.Context.default()
+    for n in [0, 1, 13 * 4 ** 6, 13 * 4 ** 6 + 1]:
+        data = np.random.default_rng(n + 3).integers(0, 256, size=n, dtype=np.uint8).tobytes()
+        say("edge size", n)
+        rc, ost, orecs, oop = O.full_flow(data)
+        assert rc == 0
+        assert dm.prove(data) == (ost, orecs, oop), n
+        if n:
+            buf = ctx.upload(data)
+            assert dm.prove_verify_device(buf.ptr, n) == (ost, orecs, oop), n
+        dm.verify(ost, orecs, oop)
+
+
 def test_d7_random_file_matches_oracle_with_other_seeds():
     params_seed, epoch = bytes(range(32)), bytes(range(100, 132))
     data = np.random.default_rng(7).integers(0, 256, size=13 * 4096 + 1, dtype=np.uint8).tobytes()  # d = 7
