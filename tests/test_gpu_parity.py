@@ -206,6 +206,23 @@ def test_edge_file_sizes_match_oracle():
         dm.verify(ost, orecs, oop)
 
 
+def test_deepest_supported_file_proves_and_verifies():
+    """d = 12, the largest depth (4096 x 4096 witness, 2^24 coefficients, a 2 GiB generator
+    matrix): the smallest such file proves and its fused verifier replay accepts; the host-bytes
+    prover gives the same proof bytes. One byte past the largest d = 12 file is refused."""
+    n = 13 * 4 ** 11 + 1
+    data = np.random.default_rng(12).integers(0, 256, size=n, dtype=np.uint8).tobytes()
+    buf = dm.Context.default().upload(data)
+    say("d = 12 fused job")
+    fused = dm.prove_verify_device(buf.ptr, n)
+    del buf
+    say("d = 12 host-bytes prover")
+    assert dm.prove(data) == fused
+    with pytest.raises(dm.DamascusError) as e:
+        dm.DamascusProver.initialize(b"\0" * (13 * 4 ** 12 + 1))
+    assert e.value.name in ("ERR_ARG", "ERR_MEMORY_LIMIT")
+
+
 def test_d7_random_file_matches_oracle_with_other_seeds():
     params_seed, epoch = bytes(range(32)), bytes(range(100, 132))
     data = np.random.default_rng(7).integers(0, 256, size=13 * 4096 + 1, dtype=np.uint8).tobytes()  # d = 7
