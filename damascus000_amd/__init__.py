@@ -446,20 +446,34 @@ class DamascusVerifier:
         _check(lib().dmx_verifier_verify_final_opening(self.h, _ptr(to_words([opening]))))
 
 
+def _ring_len(nbytes):
+    """N = 2^d of a file of nbytes: c = max(1, ceil(len / 13)) coefficients and the smallest
+    d >= 6 with 4^d >= c (src/utils/io.rs:72-120); sizes the statement buffer."""
+    c = max(1, -(-int(nbytes) // 13))
+    d = 6
+    while 4**d < c:
+        d += 1
+    return 1 << d
+
+
+def _out_buffer(n):
+    """an uninitialised host buffer of at least one byte (the library writes what it reports)"""
+    return np.empty(max(int(n), 1), dtype=np.uint8)
+
+
 def prove(data, params=None, config=None, ctx=None):
     """initialize + every fold_round; returns (statement, records_blob, opening)"""
     ctx = ctx or Context.default()
     params = params or SystemParams()
     config = config or RuntimeConfig(max_preprocess_bytes=(1 << 64) - 1)
-    st_cap = lib().dmx_statement_size(2048 * 4)
-    rec_cap = lib().dmx_proof_records_size(len(data))
-    st = ctypes.create_string_buffer(st_cap)
-    recs = ctypes.create_string_buffer(max(rec_cap, 1))
+    st = _out_buffer(lib().dmx_statement_size(_ring_len(len(data))))
+    recs = _out_buffer(lib().dmx_proof_records_size(len(data)))
     sl, rl = _SZ(), _SZ()
     op = np.zeros(2, dtype=np.uint64)
-    _check(lib().dmx_prove(ctx.h, _bytes_ptr(data), len(data), ctypes.byref(params), ctypes.byref(config), st, st_cap,
-                           ctypes.byref(sl), recs, rec_cap, ctypes.byref(rl), _ptr(op)))
-    return st.raw[: sl.value], recs.raw[: rl.value], from_words(op)[0]
+    _check(lib().dmx_prove(ctx.h, _bytes_ptr(data), len(data), ctypes.byref(params), ctypes.byref(config),
+                           st.ctypes.data, st.size, ctypes.byref(sl), recs.ctypes.data, recs.size, ctypes.byref(rl),
+                           _ptr(op)))
+    return st[: sl.value].tobytes(), recs[: rl.value].tobytes(), from_words(op)[0]
 
 
 def verify(statement, records, opening, params=None, ctx=None):
@@ -476,15 +490,14 @@ def prove_verify_device(d_file_ptr, length, params=None, ctx=None, want_proof=Tr
     ctx = ctx or Context.default()
     params = params or SystemParams()
     if want_proof:
-        st_cap = lib().dmx_statement_size(4096)
-        rec_cap = lib().dmx_proof_records_size(length)
-        st = ctypes.create_string_buffer(st_cap)
-        recs = ctypes.create_string_buffer(max(rec_cap, 1))
+        st = _out_buffer(lib().dmx_statement_size(_ring_len(length)))
+        recs = _out_buffer(lib().dmx_proof_records_size(length))
         sl, rl = _SZ(), _SZ()
         op = np.zeros(2, dtype=np.uint64)
-        _check(lib().dmx_prove_verify_device(ctx.h, _VP(d_file_ptr), length, ctypes.byref(params), st, st_cap,
-                                             ctypes.byref(sl), recs, rec_cap, ctypes.byref(rl), _ptr(op)))
-        return st.raw[: sl.value], recs.raw[: rl.value], from_words(op)[0]
+        _check(lib().dmx_prove_verify_device(ctx.h, _VP(d_file_ptr), length, ctypes.byref(params), st.ctypes.data,
+                                             st.size, ctypes.byref(sl), recs.ctypes.data, recs.size, ctypes.byref(rl),
+                                             _ptr(op)))
+        return st[: sl.value].tobytes(), recs[: rl.value].tobytes(), from_words(op)[0]
     _check(lib().dmx_prove_verify_device(ctx.h, _VP(d_file_ptr), length, ctypes.byref(params), None, 0, None, None, 0,
                                          None, None))
     return None
@@ -518,11 +531,11 @@ def prove_verify_many(files, params=None, ctx=None, max_batch=0, want_proof=True
             jobs[i].len = f[1]
         jobs[i].params = plist[i]
         if want_proof:
-            st = ctypes.create_string_buffer(lib().dmx_statement_size(4096))
-            rc = ctypes.create_string_buffer(max(lib().dmx_proof_records_size(jobs[i].len), 1))
+            st = _out_buffer(lib().dmx_statement_size(_ring_len(jobs[i].len)))
+            rc = _out_buffer(lib().dmx_proof_records_size(jobs[i].len))
             keep += [st, rc]
-            jobs[i].statement, jobs[i].statement_cap = ctypes.cast(st, _VP), len(st)
-            jobs[i].records, jobs[i].records_cap = ctypes.cast(rc, _VP), len(rc)
+            jobs[i].statement, jobs[i].statement_cap = st.ctypes.data, st.size
+            jobs[i].records, jobs[i].records_cap = rc.ctypes.data, rc.size
     rc = lib().dmx_prove_verify_many(ctx.h, jobs, n, int(on_host), max_batch)
     if rc != DMX_OK and raise_on_error:
         _check(rc)
@@ -550,11 +563,11 @@ def _job_for(f, params, want_proof, keep):
         j.file, j.len = ctypes.cast(buf, _VP), len(f)
     j.params = params
     if want_proof:
-        st = ctypes.create_string_buffer(lib().dmx_statement_size(4096))
-        rc = ctypes.create_string_buffer(max(lib().dmx_proof_records_size(j.len), 1))
+        st = _out_buffer(lib().dmx_statement_size(_ring_len(j.len)))
+        rc = _out_buffer(lib().dmx_proof_records_size(j.len))
         keep += [st, rc]
-        j.statement, j.statement_cap = ctypes.cast(st, _VP), len(st)
-        j.records, j.records_cap = ctypes.cast(rc, _VP), len(rc)
+        j.statement, j.statement_cap = st.ctypes.data, st.size
+        j.records, j.records_cap = rc.ctypes.data, rc.size
     return j
 
 

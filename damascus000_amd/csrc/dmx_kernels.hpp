@@ -202,6 +202,13 @@ void launch_zq_add(const zq* a, const zq* b, zq* out, size_t n, hipStream_t s);
 void launch_com_check(const zq* a, const zq* b, size_t n, uint32_t* flags, uint32_t bit,
                       hipStream_t s, const Jobs& jb = Jobs{});
 
+// A job's result head (16 words): the prover's flag word, the verifier's two flag words (challenge
+// errors, final check), file_id and m* (the first witness coefficient after the last fold).
+constexpr size_t RESULT_HEAD_BYTES = 64;
+enum : uint32_t { RESULT_HEAD_PFLAGS = 0, RESULT_HEAD_VFLAGS = 1, RESULT_HEAD_FID = 4, RESULT_HEAD_MSTAR = 12 };
+void launch_result_head(uint32_t* head, const uint32_t* pflags, const uint32_t* vflags, const uint8_t* file_id,
+                        const zq* mstar, hipStream_t s, const Jobs& jb);
+
 // copy each job's file from the device address in its header (u64 len, u64 src) into dst
 void launch_copy_in(const uint64_t* meta, uint8_t* dst, size_t cap, hipStream_t s, const Jobs& jb);
 

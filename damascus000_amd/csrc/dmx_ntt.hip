@@ -1612,6 +1612,30 @@ void launch_com_update(const zq* c, const zq* l, const zq* r, zq* scal, int poly
                        log_m, out, hint, jb.stride);
 }
 
+__global__ void k_result_head(uint32_t* head, const uint32_t* pflags, const uint32_t* vflags,
+                              const uint32_t* file_id, const uint32_t* mstar, size_t js) {
+    const size_t jo = job_off(js);
+    const uint32_t t = threadIdx.x;
+    uint32_t w = 0;
+    if (t == RESULT_HEAD_PFLAGS)
+        w = jp(pflags, jo)[0];
+    else if (t == RESULT_HEAD_VFLAGS || t == RESULT_HEAD_VFLAGS + 1)
+        w = jp(vflags, jo)[t - RESULT_HEAD_VFLAGS];
+    else if (t >= RESULT_HEAD_FID && t < RESULT_HEAD_FID + 8)
+        w = jp(file_id, jo)[t - RESULT_HEAD_FID];
+    else if (t >= RESULT_HEAD_MSTAR && t < RESULT_HEAD_MSTAR + 4)
+        w = jp(mstar, jo)[t - RESULT_HEAD_MSTAR];
+    if (t < RESULT_HEAD_BYTES / 4) jp(head, jo)[t] = w;
+}
+
+void launch_result_head(uint32_t* head, const uint32_t* pflags, const uint32_t* vflags, const uint8_t* file_id,
+                        const zq* mstar, hipStream_t s, const Jobs& jb) {
+    if (jb.n == 0) return;
+    hipLaunchKernelGGL(k_result_head, dim3(1, jb.n), dim3(64), 0, s, head, pflags, vflags,
+                       reinterpret_cast<const uint32_t*>(file_id), reinterpret_cast<const uint32_t*>(mstar),
+                       jb.stride);
+}
+
 __global__ void k_fill32(uint32_t* __restrict__ p, size_t n, uint32_t v, size_t js) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) jp(p, job_off(js))[i] = v;

@@ -766,8 +766,9 @@ struct dmx_prover {
     bool capturing = false;  // inside hipGraph capture: no timing events
     size_t hash_cap = 0;     // bound on every job's file length (sizes the file hash launch)
     DevBuf store;            // the workspace, unless a batch owns it
-    Slice meta, file, W_a, W_b, G_a, G_b, What_a, What_b, Ghat_a, Ghat_b, res, part, lifted, C, Ct, com0, recs,
-        scal, unsc, flags;
+    // out_head, com0 and recs are consecutive: a job's result is one contiguous range
+    Slice meta, file, W_a, W_b, G_a, G_b, What_a, What_b, Ghat_a, Ghat_b, res, part, lifted, C, Ct, out_head, com0,
+        recs, scal, unsc, flags;
     // Challenge inverses are off the critical path: the folds take the challenge itself and
     // the generator state is kept scaled (x g_L + g_R = x (g_L + x^-1 g_R), y g_e + X g_o =
     // y (g_e + y^-1 X g_o)); stage t's lifts multiply by unsc[t+1], the product of every
@@ -819,6 +820,7 @@ static void prover_plan(dmx_prover* p, Layout& L, size_t file_cap) {
     L.add(p->lifted, (size_t)K * N * 16);
     L.add(p->C, (size_t)K * N * 16);
     L.add(p->Ct, (size_t)K * N * 16);
+    L.add(p->out_head, RESULT_HEAD_BYTES);
     L.add(p->com0, (size_t)K * N * 16);
     size_t total = 0;
     p->rec_off.resize(p->d + 1);
@@ -1906,6 +1908,10 @@ static int batch_enqueue(BatchJob* J) {
     if (rc) return rc;
     HIP_TRY(hipEventRecord(J->evs[J->evs.size() - 1], v->s));
     HIP_TRY(hipStreamWaitEvent(p->s, J->evs[J->evs.size() - 1], 0));  // join
+    // the job's flags, file_id and m* next to com_0 and the records: one copy brings a result home
+    launch_result_head(p->out_head.as<uint32_t>(), p->flags.as<uint32_t>(), v->flags.as<uint32_t>(), p->tr.file_id(),
+                       p->W_a.as<zq>(), p->s, p->jb);
+    HIP_TRY(hipGetLastError());
     return DMX_OK;
 }
 
@@ -2066,7 +2072,13 @@ static int batch_collect(BatchJob* J, const JobOut* outs) {
     const size_t st = p->jb.stride;
     HIP_TRY(hipEventSynchronize(J->done));
     const size_t com_b = (size_t)K * p->N * 16, rb = p->rec_off[p->d] * 16;
-    const size_t need = (size_t)n * (4 + 8 + 32 + 16 + com_b + rb) + 64;
+    // one job's result: the head (launch_result_head), com_0 and the records, contiguous
+    const size_t com_at = p->com0.as<uint8_t>() - p->out_head.as<uint8_t>();
+    const size_t rec_at = p->recs.as<uint8_t>() - p->out_head.as<uint8_t>();
+    if (com_at < RESULT_HEAD_BYTES || rec_at != com_at + com_b || com_at % 16)
+        return fail(DMX_ERR_HIP, "result layout is not contiguous");
+    const size_t blk = rec_at + rb;
+    const size_t need = (size_t)n * blk;
     if (need > J->pinned_out_bytes) {
         if (J->pinned_out) (void)hipHostFree(J->pinned_out);
         J->pinned_out = nullptr;
@@ -2079,34 +2091,30 @@ static int batch_collect(BatchJob* J, const JobOut* outs) {
         J->pinned_out_bytes = need;
     }
     uint8_t* base = static_cast<uint8_t*>(J->pinned_out);
-    zq* com = reinterpret_cast<zq*>(base);  // 16-byte aligned pieces first
-    zq* recs = reinterpret_cast<zq*>(base + n * com_b);
-    zq* mstar = reinterpret_cast<zq*>(base + n * (com_b + rb));
-    uint32_t* pf = reinterpret_cast<uint32_t*>(base + n * (com_b + rb + 16));
-    uint32_t* vf = pf + n;  // per job: the challenge-error word and the final-check word
-    uint8_t* fid = reinterpret_cast<uint8_t*>(vf + 2 * n);
     hipStream_t s = p->s;
-    HIP_TRY(copy_rows(pf, 4, p->flags.p, st, 4, n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(copy_rows(vf, 8, v->flags.p, st, 8, n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(copy_rows(fid, 32, p->tr.file_id(), st, 32, n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(copy_rows(com, com_b, p->com0.p, st, com_b, n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(copy_rows(recs, rb, p->recs.p, st, rb, n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(copy_rows(mstar, 16, p->W_a.p, st, 16, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(copy_rows(base, blk, p->out_head.p, st, blk, n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    (void)v;
     int first = DMX_OK;
     for (uint32_t b = 0; b < n; ++b) {
         const JobOut& o = outs[b];
+        uint8_t* blk_b = base + (size_t)b * blk;
+        const uint32_t* head = reinterpret_cast<const uint32_t*>(blk_b);
+        const uint8_t* fid = blk_b + 4 * RESULT_HEAD_FID;
+        const zq* mstar = reinterpret_cast<const zq*>(blk_b + 4 * RESULT_HEAD_MSTAR);
+        const zq* com = reinterpret_cast<const zq*>(blk_b + com_at);
+        const zq* recs = reinterpret_cast<const zq*>(blk_b + rec_at);
         int rc = DMX_OK;
-        for (size_t j = 0; !rc && j < (size_t)p->d; ++j) rc = check_round_flags(j, pf[b]);
-        if (!rc && vf[2 * b]) rc = fail(DMX_ERR_HIP, "challenge resampling exhausted");
-        if (!rc && vf[2 * b + 1]) rc = fail(DMX_ERR_OPENING, "final opening mismatch");
+        for (size_t j = 0; !rc && j < (size_t)p->d; ++j) rc = check_round_flags(j, head[RESULT_HEAD_PFLAGS]);
+        if (!rc && head[RESULT_HEAD_VFLAGS]) rc = fail(DMX_ERR_HIP, "challenge resampling exhausted");
+        if (!rc && head[RESULT_HEAD_VFLAGS + 1]) rc = fail(DMX_ERR_OPENING, "final opening mismatch");
         const size_t st_need = dmx_statement_size(p->N);
         if (o.statement_len) *o.statement_len = st_need;
         if (!rc && o.statement) {
             if (o.statement_cap < st_need)
                 rc = fail(DMX_ERR_BUFFER, "statement buffer too small");
             else
-                encode_statement_from(p, b, fid + 32 * b, com + (size_t)b * K * p->N, o.statement);
+                encode_statement_from(p, b, fid, com, o.statement);
         }
         const size_t rec_need = dmx_proof_records_size(p->orig_len[b]);
         if (o.records_len) *o.records_len = rec_need;
@@ -2114,11 +2122,11 @@ static int batch_collect(BatchJob* J, const JobOut* outs) {
             if (o.records_cap < rec_need)
                 rc = fail(DMX_ERR_BUFFER, "records buffer too small");
             else
-                encode_records_from(p, recs + (size_t)b * p->rec_off[p->d], o.records);
+                encode_records_from(p, recs, o.records);
         }
         if (!rc && o.opening) {
-            o.opening[0] = (uint64_t)mstar[b].w[0] | ((uint64_t)mstar[b].w[1] << 32);
-            o.opening[1] = (uint64_t)mstar[b].w[2] | ((uint64_t)mstar[b].w[3] << 32);
+            o.opening[0] = (uint64_t)mstar->w[0] | ((uint64_t)mstar->w[1] << 32);
+            o.opening[1] = (uint64_t)mstar->w[2] | ((uint64_t)mstar->w[3] << 32);
         }
         if (o.status) *o.status = rc;
         if (rc && !first) first = rc;

@@ -64,6 +64,21 @@ def test_sizes_match_the_wire_format():
     assert L.dmx_proof_records_size(53248) == sum(L.dmx_record_size(64 >> j, 32 >> j) for j in range(6))
 
 
+def test_statement_buffers_are_sized_by_the_layout_rule():
+    """The Python mirror sizes each statement buffer by the file's ring length; it must follow
+    the layout rule (io.rs:72-120) that the golden layout cases pin, boundaries included."""
+    import json
+
+    import damascus000_amd as dm
+
+    with open(os.path.join(ROOT, "tests", "golden", "layout.json")) as f:
+        for c in json.load(f):
+            assert dm._ring_len(c["bytes"]) == c["dimension"], c
+    for d in range(6, 13):
+        assert dm._ring_len(13 * 4**d) == 1 << d
+        assert dm._ring_len(13 * 4**d + 1) == 1 << (d + 1)
+
+
 def test_gpu_switch_off_reports_no_device():
     """DAMASCUS_GPU=0 (src/utils/gpu.rs:28-33) disables the backend: compute entry points
     refuse with ERR_NO_DEVICE instead of computing anything on the CPU"""
