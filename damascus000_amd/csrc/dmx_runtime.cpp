@@ -2067,7 +2067,6 @@ struct JobOut {
 // Waits for the batch, checks every job's flags and encodes the proofs.
 static int batch_collect(BatchJob* J, const JobOut* outs) {
     dmx_prover* p = J->p.get();
-    dmx_verifier* v = J->v.get();
     const uint32_t n = J->n;
     const size_t st = p->jb.stride;
     HIP_TRY(hipEventSynchronize(J->done));
@@ -2094,7 +2093,6 @@ static int batch_collect(BatchJob* J, const JobOut* outs) {
     hipStream_t s = p->s;
     HIP_TRY(copy_rows(base, blk, p->out_head.p, st, blk, n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    (void)v;
     int first = DMX_OK;
     for (uint32_t b = 0; b < n; ++b) {
         const JobOut& o = outs[b];
