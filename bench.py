@@ -87,7 +87,7 @@ def pct(xs, q):
     return xs[min(len(xs) - 1, int(q * len(xs)))]
 
 
-INFLIGHT = 2
+INFLIGHT = 4
 
 
 def run_served(dm, ctx, dev, arrivals, files_on_host=False):
@@ -316,7 +316,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--requests", type=int, default=DEFAULT_REQUESTS, help="served trace length")
     ap.add_argument("--rate", type=float, default=DEFAULT_RATE, help="served trace arrival rate (requests/s)")
-    ap.add_argument("--inflight", type=int, default=2, help="server batch slots")
+    ap.add_argument("--inflight", type=int, default=4, help="server batch slots")
     ap.add_argument("--no-mixed", action="store_true", help="skip the served trace and the 256-file batch")
     ap.add_argument("--only", choices=["roofline", "served"], default=None,
                     help="profiling runs: only the prover runs of the roofline, or only the served trace")

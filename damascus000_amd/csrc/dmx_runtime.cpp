@@ -1756,6 +1756,13 @@ struct BatchJob {
     hipGraph_t graph_r = nullptr;  // jobs read the resident matrix `gen`
     hipGraphExec_t exec_r = nullptr;
     uint64_t graph_version = 0;  // the context's fwd_version the graphs were captured with
+    // Stream shape of the job graph. Three streams (prover, verifier, side) let one job's
+    // verifier overlap its prover: the plain and ragged-batch paths. One stream (every step on
+    // the prover's stream) makes the graph a single chain, which the runtime launches in tens of
+    // microseconds and which needs no waits across hardware queues: the server, whose
+    // concurrency comes from several batches in flight.
+    bool one_stream = false;        // the shape of the next capture
+    bool graph_one_stream = false;  // the shape the current graphs were captured with
     GenEntry* gen = nullptr;
     bool busy = false;
     uint64_t last_use = 0;
@@ -1831,7 +1838,23 @@ static int batch_create(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
 }
 
 // all device work of one batch, enqueued on p->s (forking to v->s); no host syncs
+static int batch_enqueue_streams(BatchJob* J);
+
+// The job on its streams, or with J->one_stream every step on the prover's stream (the
+// verifier's and the side stream's work then follows the prover's in order; every event wait
+// becomes a no-op).
 static int batch_enqueue(BatchJob* J) {
+    if (!J->one_stream) return batch_enqueue_streams(J);
+    const hipStream_t vs = J->v->s, gs = J->gs;
+    J->v->s = J->p->s;
+    J->gs = J->p->s;
+    const int rc = batch_enqueue_streams(J);
+    J->v->s = vs;
+    J->gs = gs;
+    return rc;
+}
+
+static int batch_enqueue_streams(BatchJob* J) {
     dmx_prover* p = J->p.get();
     dmx_verifier* v = J->v.get();
     p->round = 0;
@@ -1959,7 +1982,7 @@ static void staged_copy(uint8_t* dst, const uint8_t* src, size_t len) {
 
 // With `resident`, a batch whose jobs share one generator seed reads the resident matrix for
 // that seed instead of expanding generators per job.
-static int batch_launch(BatchJob* J, const JobIn* jobs, bool host_files, bool resident) {
+static int batch_launch(BatchJob* J, const JobIn* jobs, bool host_files, bool resident, bool one_stream = false) {
     dmx_prover* p = J->p.get();
     dmx_verifier* v = J->v.get();
     bool same_seed = true;
@@ -2021,10 +2044,12 @@ static int batch_launch(BatchJob* J, const JobIn* jobs, bool host_files, bool re
         timing = J->ctx->timing;
         fwd_version = J->ctx->fwd_version;
     }
-    if (J->graph_version != fwd_version) {  // the transform kernels changed: capture again
-        J->drop_graphs();
+    if (J->graph_version != fwd_version || J->graph_one_stream != one_stream) {
+        J->drop_graphs();  // the transform kernels or the stream shape changed: capture again
         J->graph_version = fwd_version;
+        J->graph_one_stream = one_stream;
     }
+    J->one_stream = one_stream;
     if (!graphs_enabled() || timing) {  // eager: kernel timing needs per-launch events
         int rc = batch_enqueue(J);
         if (rc) return rc;
@@ -2754,7 +2779,7 @@ static void server_finisher(dmx_server* S) {
 static void server_loop(dmx_server* S) {
     (void)hipSetDevice(S->ctx->device);
     const uint32_t max_batch = S->cfg.max_batch ? S->cfg.max_batch : 64;
-    const uint32_t max_inflight = S->cfg.max_inflight ? S->cfg.max_inflight : 2;
+    const uint32_t max_inflight = S->cfg.max_inflight ? S->cfg.max_inflight : 4;
     for (;;) {
         // completions, in any order, handed to the finisher
         for (size_t i = 0; i < S->flights.size();) {
@@ -2816,7 +2841,7 @@ static void server_loop(dmx_server* S) {
             lk.unlock();
             int rc = arc;
             const auto tl = std::chrono::steady_clock::now();
-            if (!rc) rc = batch_launch(f.J, in.data(), S->cfg.files_on_host != 0, true);
+            if (!rc) rc = batch_launch(f.J, in.data(), S->cfg.files_on_host != 0, true, true);
             S->launch_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - tl).count();
             S->batches++;
             S->batched_jobs += nb;

@@ -211,7 +211,7 @@ int dmx_prove_verify_many(dmx_ctx *ctx, dmx_job *jobs, size_t count, int files_o
 typedef struct dmx_server_config {
     uint64_t hbm_budget_bytes; /* device memory for in-flight workspaces (0: 3/4 of free memory) */
     uint32_t max_batch;        /* jobs per launch (0: 64); batches are powers of two */
-    uint32_t max_inflight;     /* batches on the device at once (0: 2) */
+    uint32_t max_inflight;     /* batches on the device at once (0: 4) */
     int32_t files_on_host;     /* job files are host addresses (copied in by the server) */
     int32_t reserved;
 } dmx_server_config;
