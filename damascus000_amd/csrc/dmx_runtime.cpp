@@ -1751,18 +1751,16 @@ struct BatchJob {
     // last rounds are in flight; the replay needs the folded matrix only at the final opening
     hipStream_t gs = nullptr;
     hipEvent_t side_done = nullptr;
-    hipGraph_t graph = nullptr;  // jobs expand their own generators
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph_r = nullptr;  // jobs read the resident matrix `gen`
-    hipGraphExec_t exec_r = nullptr;
+    // Captured job graphs by [stream shape][generators]. Shape 0: three streams (prover,
+    // verifier, side), so one job's verifier overlaps its prover (the plain and ragged-batch
+    // paths). Shape 1: every step on the prover's stream, a single chain that the runtime
+    // launches in tens of microseconds and that needs no waits across hardware queues (the
+    // server, whose concurrency comes from several batches in flight). Generators 0: the jobs
+    // expand their own; 1: they read the resident matrix `gen`.
+    hipGraph_t graphs[2][2] = {};
+    hipGraphExec_t execs[2][2] = {};
     uint64_t graph_version = 0;  // the context's fwd_version the graphs were captured with
-    // Stream shape of the job graph. Three streams (prover, verifier, side) let one job's
-    // verifier overlap its prover: the plain and ragged-batch paths. One stream (every step on
-    // the prover's stream) makes the graph a single chain, which the runtime launches in tens of
-    // microseconds and which needs no waits across hardware queues: the server, whose
-    // concurrency comes from several batches in flight.
-    bool one_stream = false;        // the shape of the next capture
-    bool graph_one_stream = false;  // the shape the current graphs were captured with
+    bool one_stream = false;     // the shape of the next enqueue
     GenEntry* gen = nullptr;
     bool busy = false;
     uint64_t last_use = 0;
@@ -1774,19 +1772,19 @@ struct BatchJob {
     // not the runtime's pageable path through a bounce buffer
     void* pinned_out = nullptr;
     size_t pinned_out_bytes = 0;
+    void drop_graph(int shape, int r) {
+        if (execs[shape][r]) (void)hipGraphExecDestroy(execs[shape][r]);
+        if (graphs[shape][r]) (void)hipGraphDestroy(graphs[shape][r]);
+        execs[shape][r] = nullptr;
+        graphs[shape][r] = nullptr;
+    }
     void drop_graphs() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (exec_r) (void)hipGraphExecDestroy(exec_r);
-        if (graph_r) (void)hipGraphDestroy(graph_r);
-        exec = exec_r = nullptr;
-        graph = graph_r = nullptr;
+        for (int shape = 0; shape < 2; ++shape)
+            for (int r = 0; r < 2; ++r) drop_graph(shape, r);
     }
     void drop_resident() {
-        if (exec_r) (void)hipGraphExecDestroy(exec_r);
-        if (graph_r) (void)hipGraphDestroy(graph_r);
-        exec_r = nullptr;
-        graph_r = nullptr;
+        drop_graph(0, 1);
+        drop_graph(1, 1);
         if (gen) gen_release(gen);
         gen = nullptr;
     }
@@ -1794,8 +1792,7 @@ struct BatchJob {
         if (pinned) (void)hipHostFree(pinned);  // idle batch: its uploads have completed
         if (pinned_out) (void)hipHostFree(pinned_out);
         drop_resident();
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        drop_graphs();
         for (auto e : evs) (void)hipEventDestroy(e);
         if (done) (void)hipEventDestroy(done);
         if (side_done) (void)hipEventDestroy(side_done);
@@ -2044,18 +2041,17 @@ static int batch_launch(BatchJob* J, const JobIn* jobs, bool host_files, bool re
         timing = J->ctx->timing;
         fwd_version = J->ctx->fwd_version;
     }
-    if (J->graph_version != fwd_version || J->graph_one_stream != one_stream) {
-        J->drop_graphs();  // the transform kernels or the stream shape changed: capture again
+    if (J->graph_version != fwd_version) {  // the transform kernels changed: capture again
+        J->drop_graphs();
         J->graph_version = fwd_version;
-        J->graph_one_stream = one_stream;
     }
     J->one_stream = one_stream;
     if (!graphs_enabled() || timing) {  // eager: kernel timing needs per-launch events
         int rc = batch_enqueue(J);
         if (rc) return rc;
     } else {
-        hipGraph_t& graph_slot = E ? J->graph_r : J->graph;
-        hipGraphExec_t& exec_slot = E ? J->exec_r : J->exec;
+        hipGraph_t& graph_slot = J->graphs[one_stream][E ? 1 : 0];
+        hipGraphExec_t& exec_slot = J->execs[one_stream][E ? 1 : 0];
         if (!exec_slot) {
             p->capturing = true;
             HIP_TRY(hipStreamBeginCapture(p->s, hipStreamCaptureModeThreadLocal));
