@@ -154,6 +154,7 @@ _SIGS = {
     "dmx_statement_size": (_SZ, [_SZ]),
     "dmx_record_size": (_SZ, [_SZ, _SZ]),
     "dmx_proof_records_size": (_SZ, [ctypes.c_uint64]),
+    "dmx_job_workspace_size": (_SZ, [ctypes.c_uint64]),
 }
 
 
@@ -446,6 +447,11 @@ class DamascusVerifier:
         _check(lib().dmx_verifier_verify_final_opening(self.h, _ptr(to_words([opening]))))
 
 
+def job_workspace_size(nbytes):
+    """device bytes of one job's workspace for a file of nbytes (a server budget's unit)"""
+    return int(lib().dmx_job_workspace_size(int(nbytes)))
+
+
 def _ring_len(nbytes):
     """N = 2^d of a file of nbytes: c = max(1, ceil(len / 13)) coefficients and the smallest
     d >= 6 with 4^d >= c (src/utils/io.rs:72-120); sizes the statement buffer."""
@@ -516,7 +522,12 @@ def prove_verify_many(files, params=None, ctx=None, max_batch=0, want_proof=True
     n = len(files)
     if n == 0:
         return []
+    kinds = {isinstance(f, tuple) for f in files}
+    if len(kinds) > 1:
+        raise TypeError("prove_verify_many: every file must be of one kind, bytes or (pointer, length) pairs")
     on_host = (not isinstance(files[0], tuple)) if files_on_host is None else bool(files_on_host)
+    if not on_host and not isinstance(files[0], tuple):
+        raise TypeError("prove_verify_many: bytes are host memory; pass files_on_host=True or device (ptr, len) pairs")
     plist = params if isinstance(params, (list, tuple)) else [params or SystemParams()] * n
     jobs = (Job * n)()
     keep = []
@@ -595,6 +606,9 @@ class Server:
         self._live = {}
 
     def submit(self, f, params=None, want_proof=True):
+        if not isinstance(f, tuple) and not self.files_on_host:
+            raise TypeError("Server.submit: bytes are host memory, but this server takes device (ptr, len) pairs; "
+                            "create it with files_on_host=True or upload the file first")
         keep = []
         j = _job_for(f, params or SystemParams(), want_proof, keep)
         t = ctypes.c_uint64()

@@ -145,6 +145,25 @@ __device__ __forceinline__ void tx_to_job(TranscriptMsg& t, size_t jo) {
     t.epoch_seed = jp(t.epoch_seed, jo);
     t.file_id = jp(t.file_id, jo);
     for (int i = 0; i < 3; ++i) t.mods[i] = jp(t.mods[i], jo);
+    t.reject = jp(t.reject, jo);
+}
+
+// the job's rejection count (TranscriptMsg::reject), 0 without one
+__device__ __forceinline__ uint32_t reject_below(const uint32_t* r) { return r ? *r : 0u; }
+
+// Candidate `counter` of sample_nonzero_challenge (transcript.rs:66-75) is taken when non-zero
+// and not below the job's rejection count. Resampling is bounded on the device: past
+// RESAMPLE_LIMIT the job reports ERR_RESAMPLE (about 2^-7000 for real digests).
+constexpr uint32_t RESAMPLE_LIMIT = 64;
+constexpr uint32_t ERR_RESAMPLE = 0x80000000u;
+__device__ __forceinline__ bool candidate_taken(const zq& x, uint32_t counter, uint32_t rj) {
+    return !zq_is_zero(x) && counter >= rj;
+}
+// A failed job still runs to its end: its challenge slot gets 1 (and inverse 1), so the kernels
+// after it compute on defined values; the error bit fails the job.
+__device__ __forceinline__ void exhausted_challenge(zq* chal_out, int want_inv) {
+    chal_out[0] = zq_from_u64(1);
+    if (want_inv) chal_out[1] = zq_from_u64(1);
 }
 
 // message words [w0, w0 + nw) into win[0 .. nw), nw a multiple of 4, by the whole workgroup
@@ -325,13 +344,18 @@ static void launch_b3_group_merge(const uint32_t* cvs, size_t n, const uint32_t 
 
 // Root merge; one workgroup of 512 threads per job over n <= 1024 CVs. (Its waves need about 120
 // VGPRs each: at 1024 threads the workgroup would take a whole CU's register file and wait for
-// any concurrent kernel to leave one; at 512 it shares a CU.) `chunk_level` says the
-// CVs are chunk CVs (needed to redo the counter chunks of a zero challenge). With `dlen` the job's
-// own length picks the level: chunk values at `cvs` (<= 1024 chunks) or group values at
-// cvs + l1_off.
-__global__ void __launch_bounds__(512) k_b3_root(uint32_t* __restrict__ cvs, size_t n, uint8_t* msg, size_t len,
-                                                  KeyW key, uint32_t flags, uint32_t* digest_out, zq* chal_out,
-                                                  uint32_t* err, int chunk_level, size_t js, const uint64_t* dlen,
+// any concurrent kernel to leave one; at 512 it shares a CU.) `level` says what the CVs at `cvs`
+// are: 0 the chunk values (the same as `chunk_cvs`), 1 the group values of the nchunks chunk
+// values at `chunk_cvs`, 2 anything deeper. With `dlen` the job's own length picks the level:
+// chunk values at `cvs` (<= 1024 chunks) or group values at cvs + l1_off.
+//
+// A rejected challenge candidate (sample_nonzero_challenge, transcript.rs:66-75) bumps the
+// counter in the message tail kept at `msg`, recomputes the chunk(s) holding it and, at level 1,
+// the group value(s) over those chunks, then merges again.
+__global__ void __launch_bounds__(512) k_b3_root(uint32_t* __restrict__ cvs, size_t n, uint32_t* chunk_cvs,
+                                                  size_t nchunks, uint8_t* msg, size_t len, KeyW key, uint32_t flags,
+                                                  uint32_t* digest_out, zq* chal_out, uint32_t* err,
+                                                  const uint32_t* reject, int level, size_t js, const uint64_t* dlen,
                                                   size_t l1_off, int want_inv) {
     critical_path_priority();
     __shared__ __align__(16) uint32_t sa[1024 * 8];
@@ -340,26 +364,28 @@ __global__ void __launch_bounds__(512) k_b3_root(uint32_t* __restrict__ cvs, siz
     __shared__ int again;
     const size_t jo = job_off(js);
     cvs = jp(cvs, jo);
+    chunk_cvs = jp(chunk_cvs, jo);
     msg = jp(msg, jo);
     digest_out = jp(digest_out, jo);
     chal_out = jp(chal_out, jo);
     err = jp(err, jo);
+    const uint32_t rj = reject_below(jp(reject, jo));
     if (dlen) {
         len = job_len(len, dlen, jo);
-        const size_t nj = nchunks_of(len);
-        if (nj > 1024) {
+        nchunks = nchunks_of(len);
+        chunk_cvs = cvs;
+        if (nchunks > 1024) {
             cvs += l1_off;
-            n = (nj + 1023) / 1024;
-            chunk_level = 0;
+            n = (nchunks + 1023) / 1024;
+            level = 1;
         } else {
-            n = nj;
-            chunk_level = 1;
+            n = nchunks;
+            level = 0;
         }
     }
     const uint32_t q = threadIdx.x & 3;
     const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
-    uint32_t counter = 0;  // used by thread 0
-    for (;;) {
+    for (uint32_t counter = 0;; ++counter) {  // every thread counts the same passes
         if (n == 1) {
             if (threadIdx.x < 4) {
                 uint32_t ca, cb;
@@ -385,7 +411,7 @@ __global__ void __launch_bounds__(512) k_b3_root(uint32_t* __restrict__ cvs, siz
             // identity quad permutation: same value, but a per-lane register to the compiler
             for (int j = 0; j < 8; ++j) h[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)dig[j], 0xE4, 0xF, 0xF, false);
             const zq x = zq_from_digest(h);
-            if (!zq_is_zero(x)) {
+            if (candidate_taken(x, counter, rj)) {
                 const zq xi = want_inv ? zq_inv_divsteps(x) : x;
                 if (threadIdx.x == 0) {
                     chal_out[0] = x;
@@ -400,17 +426,17 @@ __global__ void __launch_bounds__(512) k_b3_root(uint32_t* __restrict__ cvs, siz
             if (chal_out) {
                 uint32_t h[8];
                 for (int j = 0; j < 8; ++j) h[j] = dig[j];
-                const zq x = zq_from_digest(h);
-                if (!zq_is_zero(x)) {
+                if (candidate_taken(zq_from_digest(h), counter, rj)) {
                     // written above
-                } else if (!chunk_level) {
-                    atomicOr(err, 0x40000000u);  // group CVs cannot be patched here
-                } else if (counter >= 64) {
-                    atomicOr(err, 0x80000000u);  // unreachable in practice (2^-112 per try)
+                } else if (level > 1) {
+                    atomicOr(err, 0x40000000u);  // deeper trees are not patched (no message has them)
+                } else if (counter + 1 >= RESAMPLE_LIMIT) {
+                    atomicOr(err, ERR_RESAMPLE);
+                    exhausted_challenge(chal_out, want_inv);
                 } else {
                     // sample_nonzero_challenge (transcript.rs:66-75): counter + 1, hash again
-                    counter += 1;
-                    for (uint32_t b = 0; b < 4; ++b) msg[len - 4 + b] = (uint8_t)(counter >> (8 * b));
+                    const uint32_t c1 = counter + 1;
+                    for (uint32_t b = 0; b < 4; ++b) msg[len - 4 + b] = (uint8_t)(c1 >> (8 * b));
                     again = 1;
                 }
             }
@@ -418,28 +444,40 @@ __global__ void __launch_bounds__(512) k_b3_root(uint32_t* __restrict__ cvs, siz
         }
         __syncthreads();
         if (!again) break;  // uniform: every thread reads the same shared flag
+        const size_t c0 = (len - 4) / 1024, c1 = (len - 1) / 1024;
         if (threadIdx.x < 4 && n > 1) {  // quad 0 recomputes the chunk(s) holding the counter
-            const size_t c0 = (len - 4) / 1024, c1 = (len - 1) / 1024;
             for (size_t c = c0; c <= c1; ++c) {
                 uint32_t ca, cb;
                 chunk_cv_quad(msg, len, c, key.w, flags, false, sel, ca, cb);
-                cvs[c * 8 + q] = ca;
-                cvs[c * 8 + 4 + q] = cb;
+                chunk_cvs[c * 8 + q] = ca;
+                chunk_cvs[c * 8 + 4 + q] = cb;
             }
             __threadfence();
         }
         __syncthreads();
+        if (level == 1) {  // the group value(s) over those chunks, from their chunk values
+            for (size_t g = c0 / 1024; g <= c1 / 1024; ++g) {  // uniform bounds
+                const uint32_t cnt = (uint32_t)min((size_t)1024, nchunks - g * 1024);
+                load_cvs_lds(sa, chunk_cvs + g * 1024 * 8, cnt);
+                __syncthreads();
+                const uint32_t* r = merge_lds(sa, sb, cnt, key.w, flags, false);
+                if (threadIdx.x < 8) cvs[g * 8 + threadIdx.x] = r[threadIdx.x];
+                __threadfence();
+                __syncthreads();
+            }
+        }
     }
 }
 
-static void launch_b3_root(const uint32_t* cvs, size_t n, const uint8_t* msg, size_t len, const uint32_t key[8],
-                           uint32_t flags, uint32_t* digest_out, zq* chal_out, uint32_t* err_flags, int chunk_level,
-                           hipStream_t s, const Jobs& jb, const uint64_t* dlen, size_t l1_off, int want_inv = 1) {
+static void launch_b3_root(const uint32_t* cvs, size_t n, const uint32_t* chunk_cvs, size_t nchunks, int level,
+                           const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags, uint32_t* digest_out,
+                           zq* chal_out, uint32_t* err_flags, const uint32_t* reject, hipStream_t s, const Jobs& jb,
+                           const uint64_t* dlen, size_t l1_off, int want_inv = 1) {
     KeyW k;
     for (int i = 0; i < 8; ++i) k.w[i] = key[i];
     hipLaunchKernelGGL(k_b3_root, dim3(1, jb.n), dim3(512), 0, s, const_cast<uint32_t*>(cvs), n,
-                       const_cast<uint8_t*>(msg), len, k, flags, digest_out, chal_out, err_flags, chunk_level,
-                       jb.stride, dlen, l1_off, want_inv);
+                       const_cast<uint32_t*>(chunk_cvs), nchunks, const_cast<uint8_t*>(msg), len, k, flags,
+                       digest_out, chal_out, err_flags, reject, level, jb.stride, dlen, l1_off, want_inv);
 }
 
 // Hash a device buffer: chunk CVs, group merges while more than 1024 values remain, root.
@@ -461,13 +499,13 @@ __global__ void __launch_bounds__(256) k_b3_tx_challenge(TranscriptMsg t, size_t
     tx_to_job(t, jo);
     chal_out = jp(chal_out, jo);
     err = jp(err, jo);
+    const uint32_t rj = reject_below(t.reject);
     tx_pack_lds(t, 0, (uint32_t)(((len + 63) & ~(size_t)63) / 8), win);
     __syncthreads();
     const uint32_t q = threadIdx.x & 3, quad = threadIdx.x >> 2;
     const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
     const size_t n = nchunks_of(len);
-    uint32_t counter = 0;  // used by thread 0
-    for (;;) {
+    for (uint32_t counter = 0;; ++counter) {  // every thread counts the same passes
         if (n == 1) {
             if (threadIdx.x < 4) {
                 uint32_t ca, cb;
@@ -491,7 +529,7 @@ __global__ void __launch_bounds__(256) k_b3_tx_challenge(TranscriptMsg t, size_t
             uint32_t h[8];
             for (int j = 0; j < 8; ++j) h[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)dig[j], 0xE4, 0xF, 0xF, false);
             const zq x = zq_from_digest(h);
-            if (!zq_is_zero(x)) {
+            if (candidate_taken(x, counter, rj)) {
                 const zq xi = want_inv ? zq_inv_divsteps(x) : x;
                 if (threadIdx.x == 0) {
                     chal_out[0] = x;
@@ -503,13 +541,14 @@ __global__ void __launch_bounds__(256) k_b3_tx_challenge(TranscriptMsg t, size_t
             again = 0;
             uint32_t h[8];
             for (int j = 0; j < 8; ++j) h[j] = dig[j];
-            if (zq_is_zero(zq_from_digest(h))) {
-                if (counter >= 64) {
-                    atomicOr(err, 0x80000000u);  // unreachable in practice (2^-112 per try)
+            if (!candidate_taken(zq_from_digest(h), counter, rj)) {
+                if (counter + 1 >= RESAMPLE_LIMIT) {
+                    atomicOr(err, ERR_RESAMPLE);
+                    exhausted_challenge(chal_out, want_inv);
                 } else {
-                    counter += 1;
+                    const uint32_t c1 = counter + 1;
                     uint8_t* wb = reinterpret_cast<uint8_t*>(win);
-                    for (uint32_t b = 0; b < 4; ++b) wb[len - 4 + b] = (uint8_t)(counter >> (8 * b));
+                    for (uint32_t b = 0; b < 4; ++b) wb[len - 4 + b] = (uint8_t)(c1 >> (8 * b));
                     again = 1;
                 }
             }
@@ -528,8 +567,8 @@ static void launch_b3_tx_chunks(const TranscriptMsg& t, size_t len, const uint32
 }
 
 static void hash_tree(uint32_t* cv_scratch, size_t n, const uint8_t* msg, size_t len, const uint32_t key[8],
-                      uint32_t flags, uint32_t* digest_out, zq* chal_out, uint32_t* err_flags, hipStream_t s,
-                      const Jobs& jb, int want_inv = 1);
+                      uint32_t flags, uint32_t* digest_out, zq* chal_out, uint32_t* err_flags,
+                      const uint32_t* reject, hipStream_t s, const Jobs& jb, int want_inv = 1);
 
 void transcript_challenge(const TranscriptMsg& t, const uint32_t key[8], uint32_t flags, uint8_t* msg,
                           uint32_t* cv_scratch, zq* chal_out, uint32_t* err_flags, hipStream_t s, const Jobs& jb,
@@ -544,7 +583,8 @@ void transcript_challenge(const TranscriptMsg& t, const uint32_t key[8], uint32_
         return;
     }
     launch_b3_tx_chunks(t, len, key, flags, cv_scratch, msg, s, jb);
-    hash_tree(cv_scratch, nchunks_of(len), msg, len, key, flags, nullptr, chal_out, err_flags, s, jb, (int)inverse);
+    hash_tree(cv_scratch, nchunks_of(len), msg, len, key, flags, nullptr, chal_out, err_flags, t.reject, s, jb,
+              (int)inverse);
 }
 
 void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t flags,
@@ -556,27 +596,30 @@ void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t
     launch_b3_chunks(msg, len, key, flags, cv_scratch, s, jb, dlen);
     if (dlen) {  // per-job lengths: at most one group level
         if (n > 1024) launch_b3_group_merge(cv_scratch, n, key, flags, cv_scratch + n * 8, s, jb, dlen, len);
-        launch_b3_root(cv_scratch, n, msg, len, key, flags, digest_out, chal_out, err_flags, 1, s, jb, dlen, n * 8);
+        launch_b3_root(cv_scratch, n, cv_scratch, n, 0, msg, len, key, flags, digest_out, chal_out, err_flags, nullptr,
+                       s, jb, dlen, n * 8);
         return;
     }
-    hash_tree(cv_scratch, n, msg, len, key, flags, digest_out, chal_out, err_flags, s, jb);
+    hash_tree(cv_scratch, n, msg, len, key, flags, digest_out, chal_out, err_flags, nullptr, s, jb);
 }
 
 // group merges while more than 1024 values remain, then the root
 static void hash_tree(uint32_t* cv_scratch, size_t n, const uint8_t* msg, size_t len, const uint32_t key[8],
-                      uint32_t flags, uint32_t* digest_out, zq* chal_out, uint32_t* err_flags, hipStream_t s,
-                      const Jobs& jb, int want_inv) {
+                      uint32_t flags, uint32_t* digest_out, zq* chal_out, uint32_t* err_flags,
+                      const uint32_t* reject, hipStream_t s, const Jobs& jb, int want_inv) {
     uint32_t* cur = cv_scratch;
     size_t cnt = n;
     uint32_t* next = cv_scratch + n * 8;
+    int level = 0;
     while (cnt > 1024) {
         launch_b3_group_merge(cur, cnt, key, flags, next, s, jb, nullptr, 0);
         cnt = (cnt + 1023) / 1024;
         cur = next;
         next = next + cnt * 8;
+        ++level;
     }
-    launch_b3_root(cur, cnt, msg, len, key, flags, digest_out, chal_out, err_flags, cur == cv_scratch, s, jb, nullptr,
-                   0, want_inv);
+    launch_b3_root(cur, cnt, cv_scratch, n, level, msg, len, key, flags, digest_out, chal_out, err_flags, reject, s, jb,
+                   nullptr, 0, want_inv);
 }
 
 size_t device_hash_scratch_words(size_t len) {

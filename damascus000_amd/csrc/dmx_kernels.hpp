@@ -226,6 +226,9 @@ struct TranscriptMsg {
     uint64_t round;
     const zq* mods[3];
     uint32_t lens[3];
+    // device word per job (null: 0): candidates of counters below it count as zero (a test of
+    // the resampling path, transcript.rs:66-75, honoured by the oracle as well)
+    const uint32_t* reject = nullptr;
 };
 size_t transcript_len(uint32_t label_len, const uint32_t lens[3]);
 // Challenge of a transcript message (transcript.rs:42-75), hashed from its sources:

@@ -669,13 +669,15 @@ struct Layout {
 struct JobMeta {
     uint64_t len;        // file length in bytes
     uint64_t src;        // device address the file is copied in from (0: already in place)
-    uint64_t pad[2];
+    uint32_t reject;     // challenge candidates rejected per challenge (test knob, 0 normally)
+    uint32_t pad0;
+    uint64_t pad1;
     uint32_t gseed[8];   // generator key words: g_0_seed (sis.rs:70)
     uint8_t epoch[32];   // epoch_seed
     uint8_t file_id[32]; // written by the device
 };
 static_assert(sizeof(JobMeta) == 128, "job header layout");
-constexpr size_t META_GSEED = 32, META_SEEDS = 64;
+constexpr size_t META_REJECT = 16, META_GSEED = 32, META_SEEDS = 64;
 
 template <class T>
 static T* job_ptr(T* p, const Jobs& jb, uint32_t b) {
@@ -701,11 +703,26 @@ static const uint32_t IVW[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53
 
 static void gseed_words(const uint8_t g_seed[32], uint32_t w[8]) { b3::words_le(g_seed, w, 8); }
 
+// Test knob DAMASCUS_TEST_REJECT: "n" makes every job's challenges reject the candidates of
+// counters 0..n-1 (transcript.rs:66-75's resampling branch, which real digests reach with
+// probability about 2^-112); "n@len" does so only for jobs whose file is `len` bytes long. Read
+// when a job's header is written, so captured graphs replay with it. n >= 64 exhausts the
+// device's bounded resampling and fails the job.
+static uint32_t test_reject_for(uint64_t len) {
+    const char* e = getenv("DAMASCUS_TEST_REJECT");
+    if (!e || !*e) return 0;
+    char* end = nullptr;
+    const unsigned long n = strtoul(e, &end, 10);
+    if (end && *end == '@') return strtoull(end + 1, nullptr, 10) == len ? (uint32_t)n : 0;
+    return (uint32_t)n;
+}
+
 static JobMeta make_meta(uint64_t len, const void* src, const uint8_t g_seed[32], const uint8_t epoch[32]) {
     JobMeta m;
     memset(&m, 0, sizeof m);
     m.len = len;
     m.src = reinterpret_cast<uint64_t>(src);
+    m.reject = test_reject_for(len);
     gseed_words(g_seed, m.gseed);
     memcpy(m.epoch, epoch, 32);
     return m;
@@ -714,6 +731,7 @@ static JobMeta make_meta(uint64_t len, const void* src, const uint8_t g_seed[32]
 struct Transcriber {
     Slice msg, cvs;
     uint8_t* seeds = nullptr;  // epoch_seed[32] | file_id[32], inside the job header
+    const uint32_t* reject = nullptr;  // the job header's rejection count
     size_t msg_cap = 0, cv_words = 0;
     void plan(Layout& L, size_t max_ring, size_t extra_hash_len) {
         const uint32_t lens[3] = {(uint32_t)max_ring, (uint32_t)max_ring, (uint32_t)max_ring};
@@ -741,6 +759,7 @@ struct Transcriber {
         t.lens[0] = (uint32_t)lc;
         t.lens[1] = (uint32_t)ll;
         t.lens[2] = (uint32_t)lr;
+        t.reject = reject;
         transcript_challenge(t, IVW, 0, msg.as<uint8_t>(), cvs.as<uint32_t>(), out, flags, s, jb, inverse);
     }
 };
@@ -838,6 +857,7 @@ static void prover_plan(dmx_prover* p, Layout& L, size_t file_cap) {
 
 static void prover_bind(dmx_prover* p) {
     p->tr.seeds = p->meta.as<uint8_t>() + META_SEEDS;
+    p->tr.reject = reinterpret_cast<const uint32_t*>(p->meta.as<uint8_t>() + META_REJECT);
     if (p->file.p) p->d_file = p->file.as<uint8_t>();
 }
 
@@ -1279,7 +1299,10 @@ static void verifier_plan(dmx_verifier* v, Layout& L) {
     v->tr.plan(L, v->c_cap, 0);
 }
 
-static void verifier_bind(dmx_verifier* v) { v->tr.seeds = v->meta.as<uint8_t>() + META_SEEDS; }
+static void verifier_bind(dmx_verifier* v) {
+    v->tr.seeds = v->meta.as<uint8_t>() + META_SEEDS;
+    v->tr.reject = reinterpret_cast<const uint32_t*>(v->meta.as<uint8_t>() + META_REJECT);
+}
 
 // Generator derivation from g_0_seed (verifier.rs:31-33) for a standalone verifier (a fused job
 // takes the rows from the prover's first transform instead). Kernels only.
@@ -1940,6 +1963,46 @@ static bool graphs_enabled() {
     return !(e && strcmp(e, "0") == 0);
 }
 
+// A file handed over as a device address is read by the copy-in kernel: it must lie inside one
+// allocation the device can read. One handed over as a host address is copied by the host into
+// pinned staging: it must not be device memory. Checked before any work is enqueued, so a
+// mismatched pointer is an argument error instead of a GPU memory fault or a host crash.
+static int check_device_file(const void* p, uint64_t len, size_t i) {
+    if (!len) return DMX_OK;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(DMX_ERR_ARG, "job %zu: the file is not device memory (files are device addresses)", i);
+    }
+    if (a.type == hipMemoryTypeUnregistered)
+        return fail(DMX_ERR_ARG, "job %zu: the file is not device memory (files are device addresses)", i);
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(DMX_ERR_ARG, "job %zu: the file is not inside a device allocation", i);
+    }
+    const uintptr_t b = reinterpret_cast<uintptr_t>(base), q = reinterpret_cast<uintptr_t>(p);
+    if (q < b || q - b > size || len > size - (q - b))
+        return fail(DMX_ERR_ARG, "job %zu: %llu file bytes run past the end of their device allocation", i,
+                    (unsigned long long)len);
+    return DMX_OK;
+}
+static int check_host_file(const void* p, uint64_t len, size_t i) {
+    if (!len) return DMX_OK;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {  // unknown to HIP: pageable host memory
+        (void)hipGetLastError();
+        return DMX_OK;
+    }
+    if (a.type == hipMemoryTypeDevice)
+        return fail(DMX_ERR_ARG, "job %zu: the file is device memory, but the files are host addresses", i);
+    return DMX_OK;
+}
+static int check_file(const void* p, uint64_t len, bool on_host, size_t i) {
+    return on_host ? check_host_file(p, len, i) : check_device_file(p, len, i);
+}
+
 // Writes the job headers (lengths, copy-in sources, seeds) and launches the batch. Files are
 // device addresses (copied in by the first kernel) or host addresses (copied here).
 struct JobIn {
@@ -2169,6 +2232,11 @@ static size_t job_workspace_bytes(int d) {
     return L.stride();
 }
 
+extern "C" size_t dmx_job_workspace_size(uint64_t file_len) {
+    const int d = depth_for(file_len);
+    return d > 12 ? 0 : job_workspace_bytes(d);
+}
+
 // Cache of batch workspaces and their graphs, per (context, depth, n). The cache holds at most
 // DAMASCUS_CACHE_GB (default 60 % of device memory); idle workspaces are freed, least recently
 // used first, only when a new one would not fit (freeing stalls the device, so never on
@@ -2293,8 +2361,10 @@ extern "C" int dmx_prove_verify_device(dmx_ctx* ctx, const void* d_file, size_t 
     const int d = depth_for(len);
     if (d > 12) return fail(DMX_ERR_ARG, "depth %d exceeds the supported 12", d);
     HIP_TRY(hipSetDevice(ctx->device));
+    int rc = check_device_file(d_file, len, 0);
+    if (rc) return rc;
     BatchJob* J = nullptr;
-    int rc = batch_acquire(ctx, d, 1, &J);
+    rc = batch_acquire(ctx, d, 1, &J);
     if (rc) return rc;
     const JobIn in{d_file, len, params};
     int status = DMX_OK;
@@ -2319,6 +2389,8 @@ extern "C" int dmx_prove_verify_many(dmx_ctx* ctx, dmx_job* jobs, size_t count, 
         if (!jobs[i].file && jobs[i].len) return fail(DMX_ERR_ARG, "job %zu: null file", i);
         const int d = depth_for(jobs[i].len);
         if (d > 12) return fail(DMX_ERR_ARG, "job %zu: depth %d exceeds the supported 12", i, d);
+        const int crc = check_file(jobs[i].file, jobs[i].len, files_on_host != 0, i);
+        if (crc) return crc;
         jobs[i].depth = d;
         jobs[i].status = DMX_ERR_HIP;
         by_d[d].push_back(i);
@@ -2894,6 +2966,8 @@ extern "C" int dmx_server_submit(dmx_server* S, dmx_job* job, uint64_t* ticket) 
     if (d > 12) return fail(DMX_ERR_ARG, "depth %d exceeds the supported 12", d);
     if (job_workspace_bytes(d) > S->cfg.hbm_budget_bytes)
         return fail(DMX_ERR_MEMORY_LIMIT, "job workspace exceeds the server's device memory budget");
+    const int crc = check_file(job->file, job->len, S->cfg.files_on_host != 0, 0);
+    if (crc) return crc;
     job->depth = d;
     job->status = DMX_ERR_PENDING;
     std::lock_guard<std::mutex> g(S->mu);
@@ -2912,16 +2986,22 @@ extern "C" int dmx_server_submit(dmx_server* S, dmx_job* job, uint64_t* ticket) 
 extern "C" int dmx_server_wait(dmx_server* S, uint64_t ticket, int64_t timeout_ms, double* latency_s) {
     if (!S) return fail(DMX_ERR_ARG, "null server");
     std::unique_lock<std::mutex> lk(S->mu);
-    auto it = S->jobs.find(ticket);
-    if (it == S->jobs.end()) return fail(DMX_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
-    auto ready = [&] { return S->jobs[ticket].done; };
+    if (S->jobs.find(ticket) == S->jobs.end())
+        return fail(DMX_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+    // find(), not operator[]: a ticket released by another waiter must not be re-created
+    auto ready = [&] {
+        auto f = S->jobs.find(ticket);
+        return f == S->jobs.end() || f->second.done;
+    };
     if (timeout_ms < 0)
         S->cv_done.wait(lk, ready);
     else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
         return DMX_ERR_PENDING;
-    const ServedJob& sj = S->jobs[ticket];
-    if (latency_s) *latency_s = sj.t_done - sj.t_submit;
-    S->jobs.erase(ticket);
+    auto it = S->jobs.find(ticket);
+    if (it == S->jobs.end())
+        return fail(DMX_ERR_ARG, "ticket %llu was released by another waiter", (unsigned long long)ticket);
+    if (latency_s) *latency_s = it->second.t_done - it->second.t_submit;
+    S->jobs.erase(it);
     return DMX_OK;
 }
 

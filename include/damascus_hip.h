@@ -268,6 +268,9 @@ int dmx_ctx_set_transform_variant(dmx_ctx *ctx, int log_m, int variant);
 size_t dmx_statement_size(size_t ring_len);
 size_t dmx_record_size(size_t vec_ring_len, size_t poly_ring_len);
 size_t dmx_proof_records_size(uint64_t file_len);
+/* device memory of one job's workspace (prover + verifier) for a file of file_len bytes: the unit
+ * a server's hbm_budget_bytes admits jobs by. No reference counterpart. */
+size_t dmx_job_workspace_size(uint64_t file_len);
 
 #ifdef __cplusplus
 }

@@ -149,6 +149,13 @@ int orc_full_flow(const uint8_t *file, size_t len, const uint8_t seed_generators
                   size_t *statement_len, uint8_t *records_out, size_t records_cap,
                   size_t *records_len, uint64_t opening_out[2]);
 
+/* transcript.rs:66-75 driven by the mock candidate of the reference's test (transcript.rs:130-143):
+ * zero below `zero_below`, then `value`; records the counters it was called with */
+uint64_t orc_sample_nonzero_mock(uint32_t zero_below, uint64_t value, uint32_t *seen, size_t cap,
+                                 size_t *n_seen);
+/* test knob: every challenge rejects the candidates of counters below n (0: the reference) */
+void orc_set_challenge_reject(uint32_t n);
+
 /* number of worker threads used by the commit loop (1 = the reference's sequential loop) */
 void orc_set_threads(int threads);
 int orc_get_threads(void);

@@ -171,32 +171,85 @@ static void absorb_commitment(orc_b3 *h, modref m) {
     }
 }
 
-/* transcript.rs:42-75: stateless challenge with a counter rejecting zero */
+/* transcript.rs:66-75: sample_nonzero_challenge. Candidates are derived for counter = 0, 1, ...
+ * until one is non-zero. */
+typedef u128 (*candidate_fn)(uint32_t counter, void *ctx);
+static u128 sample_nonzero_challenge(candidate_fn derive, void *ctx) {
+    uint32_t counter = 0;
+    for (;;) {
+        u128 c = derive(counter, ctx);
+        if (c != 0) return c;
+        counter += 1; /* wrapping_add */
+    }
+}
+
+/* Test knob: challenges treat the candidates of counters below orc_challenge_reject as zero, the
+ * branch a real digest reaches with probability about 2^-112 per candidate. 0 (the default) is
+ * the reference. The device honours the same knob, so both sides can be checked on it. */
+static uint32_t orc_challenge_reject = 0;
+void orc_set_challenge_reject(uint32_t n) { orc_challenge_reject = n; }
+
+typedef struct {
+    const uint8_t *epoch_seed, *file_id;
+    const char *label;
+    uint64_t round;
+    modref cur, l, r;
+} tx_ctx;
+
+/* transcript.rs:50-61: the candidate for one counter */
+static u128 tx_candidate(uint32_t counter, void *vctx) {
+    const tx_ctx *t = (const tx_ctx *)vctx;
+    size_t llen = strlen(t->label);
+    orc_b3 h;
+    orc_b3_init(&h);
+    orc_b3_update(&h, t->epoch_seed, 32);
+    absorb_u64(&h, llen);
+    orc_b3_update(&h, t->label, llen);
+    absorb_u64(&h, 32);
+    orc_b3_update(&h, t->file_id, 32);
+    absorb_u64(&h, 0); /* epoch_index */
+    absorb_u64(&h, t->round);
+    absorb_commitment(&h, t->cur);
+    absorb_commitment(&h, t->l);
+    absorb_commitment(&h, t->r);
+    uint8_t c4[4] = {(uint8_t)counter, (uint8_t)(counter >> 8), (uint8_t)(counter >> 16),
+                     (uint8_t)(counter >> 24)};
+    orc_b3_update(&h, c4, 4);
+    uint8_t dig[32];
+    orc_b3_finalize(&h, dig, 32);
+    if (counter < orc_challenge_reject) return 0;
+    return zq_from_le_bytes_mod_order(dig, 32);
+}
+
+/* transcript.rs:42-63: stateless challenge */
 static u128 challenge_u128(const uint8_t epoch_seed[32], const char *label,
                            const uint8_t file_id[32], uint64_t round, modref cur, modref l,
                            modref r) {
-    size_t llen = strlen(label);
-    for (uint32_t counter = 0;; ++counter) {
-        orc_b3 h;
-        orc_b3_init(&h);
-        orc_b3_update(&h, epoch_seed, 32);
-        absorb_u64(&h, llen);
-        orc_b3_update(&h, label, llen);
-        absorb_u64(&h, 32);
-        orc_b3_update(&h, file_id, 32);
-        absorb_u64(&h, 0); /* epoch_index */
-        absorb_u64(&h, round);
-        absorb_commitment(&h, cur);
-        absorb_commitment(&h, l);
-        absorb_commitment(&h, r);
-        uint8_t c4[4] = {(uint8_t)counter, (uint8_t)(counter >> 8), (uint8_t)(counter >> 16),
-                         (uint8_t)(counter >> 24)};
-        orc_b3_update(&h, c4, 4);
-        uint8_t dig[32];
-        orc_b3_finalize(&h, dig, 32);
-        u128 x = zq_from_le_bytes_mod_order(dig, 32);
-        if (x != 0) return x;
-    }
+    tx_ctx t = {epoch_seed, file_id, label, round, cur, l, r};
+    return sample_nonzero_challenge(tx_candidate, &t);
+}
+
+/* The reference's sampler test (transcript.rs:130-143) drives sample_nonzero_challenge with a
+ * mock candidate: zero for counters below `zero_below`, `value` after. Records the counters
+ * seen (at most `cap`) and returns the sampled value. */
+typedef struct {
+    uint32_t zero_below;
+    uint64_t value;
+    uint32_t *seen;
+    size_t cap, n;
+} mock_ctx;
+static u128 mock_candidate(uint32_t counter, void *vctx) {
+    mock_ctx *m = (mock_ctx *)vctx;
+    if (m->n < m->cap) m->seen[m->n] = counter;
+    m->n += 1;
+    return counter < m->zero_below ? 0 : (u128)m->value;
+}
+uint64_t orc_sample_nonzero_mock(uint32_t zero_below, uint64_t value, uint32_t *seen, size_t cap,
+                                 size_t *n_seen) {
+    mock_ctx m = {zero_below, value, seen, cap, 0};
+    u128 v = sample_nonzero_challenge(mock_candidate, &m);
+    if (n_seen) *n_seen = m.n;
+    return (uint64_t)v;
 }
 
 static u128 *load_mod(const uint64_t *w, size_t len) {

@@ -87,6 +87,22 @@ def set_threads(n):
     lib().orc_set_threads(ctypes.c_int(n))
 
 
+def set_challenge_reject(n):
+    """test knob: every challenge rejects the candidates of counters below n (0: the reference)"""
+    lib().orc_set_challenge_reject(ctypes.c_uint32(n))
+
+
+def sample_nonzero_mock(zero_below, value, cap=64):
+    """sample_nonzero_challenge (transcript.rs:66-75) on the mock candidate of transcript.rs:130-143:
+    returns (sampled value, counters seen)"""
+    seen = (ctypes.c_uint32 * cap)()
+    n = ctypes.c_size_t(0)
+    lib().orc_sample_nonzero_mock.restype = ctypes.c_uint64
+    v = lib().orc_sample_nonzero_mock(ctypes.c_uint32(zero_below), ctypes.c_uint64(value), seen,
+                                      ctypes.c_size_t(cap), ctypes.byref(n))
+    return int(v), [int(seen[i]) for i in range(min(n.value, cap))]
+
+
 # ---- Z_q ----
 def zq_binop(name, a, b):
     a = np.ascontiguousarray(a, dtype=np.uint64)

@@ -358,3 +358,55 @@ def test_full_flow_accepts_and_is_deterministic():
     rc2, st2, recs2, op2 = O.full_flow(data)
     assert (st, recs, op) == (st2, recs2, op2)
     assert len(W.split_records(recs)) == 6
+
+
+def test_challenge_sampler_rejects_zero_candidates():
+    """transcript.rs:130-143: the sampler asks for counters 0, 1, 2 and returns the first
+    non-zero candidate"""
+    v, seen = O.sample_nonzero_mock(2, 9)
+    assert v == 9 and seen == [0, 1, 2]
+    v, seen = O.sample_nonzero_mock(0, 5)
+    assert v == 5 and seen == [0]
+
+
+def test_rejected_candidate_resamples_with_the_next_counter():
+    """With the first candidate rejected (the test knob), the challenge is the counter-1
+    candidate: the same message with u32le(1) as its last field, rebuilt byte by byte."""
+    fid = bytes(range(32))
+    cur, lhs, rhs = _tagged(5), _tagged(6)[:, :32], _tagged(7)[:, :32]
+
+    def enc(m):
+        out = struct.pack("<Q", 8)
+        for k in range(8):
+            out += struct.pack("<Q", m.shape[1]) + m[k].astype("<u8").tobytes()
+        return out
+
+    head = (EPOCH + struct.pack("<Q", 3) + b"vec" + struct.pack("<Q", 32) + fid + struct.pack("<QQ", 0, 4)
+            + enc(cur) + enc(lhs) + enc(rhs))
+    c0 = int.from_bytes(O.blake3(head + struct.pack("<I", 0)), "little") % O.Q
+    c1 = int.from_bytes(O.blake3(head + struct.pack("<I", 1)), "little") % O.Q
+    c3 = int.from_bytes(O.blake3(head + struct.pack("<I", 3)), "little") % O.Q
+    assert c0 != c1
+    try:
+        assert O.challenge(EPOCH, "vec", fid, 4, cur, lhs, rhs) == c0
+        O.set_challenge_reject(1)
+        assert O.challenge(EPOCH, "vec", fid, 4, cur, lhs, rhs) == c1
+        O.set_challenge_reject(3)
+        assert O.challenge(EPOCH, "vec", fid, 4, cur, lhs, rhs) == c3
+    finally:
+        O.set_challenge_reject(0)
+    assert O.challenge(EPOCH, "vec", fid, 4, cur, lhs, rhs) == c0
+
+
+def test_full_flow_with_rejected_candidates_replays():
+    """every challenge of a d = 6 proof resampled once: a different proof that the oracle's own
+    verifier replays and accepts under the same knob"""
+    data = W.pattern_bytes(8192, 5)
+    rc0, st0, recs0, op0 = O.full_flow(data)
+    try:
+        O.set_challenge_reject(1)
+        rc, st, recs, op = O.full_flow(data)
+    finally:
+        O.set_challenge_reject(0)
+    assert rc0 == 0 and rc == 0
+    assert st == st0 and recs != recs0
