@@ -318,8 +318,9 @@ def main():
     ap.add_argument("--rate", type=float, default=DEFAULT_RATE, help="served trace arrival rate (requests/s)")
     ap.add_argument("--inflight", type=int, default=4, help="server batch slots")
     ap.add_argument("--no-mixed", action="store_true", help="skip the served trace and the 256-file batch")
-    ap.add_argument("--only", choices=["roofline", "served"], default=None,
-                    help="profiling runs: only the prover runs of the roofline, or only the served trace")
+    ap.add_argument("--only", choices=["roofline", "served", "ragged"], default=None,
+                    help="profiling runs: only the prover runs of the roofline, the served trace, or the "
+                         "ragged products")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run a device copy of N MiB and exit")
@@ -358,6 +359,9 @@ def main():
         print(json.dumps({"served_mib_s": round(total / span, 1), "p50_ms": round(pct(lat, 0.5) * 1e3, 2),
                           "p99_ms": round(pct(lat, 0.99) * 1e3, 2), "requests": args.requests, "rate": args.rate,
                           "server": LAST_STATS}))
+        return
+    if args.only == "ragged":
+        print(json.dumps(ragged_products(dm, ctx)))
         return
     data = single_file()
     if args.only == "roofline":

@@ -269,6 +269,32 @@ void launch_pointwise(uint32_t* a, const uint32_t* b, size_t slices, int log_m, 
 void launch_ring_mul_persistent(const zq* A, const zq* B, zq* O, const uint64_t* offs, const uint32_t* logn,
                                 const uint32_t* order, uint32_t count, uint32_t* counter, uint32_t* res,
                                 int max_log_n, const NttTables& tw, hipStream_t s);
+// LDS-resident ragged products of degrees 2^0..2^12 in one persistent launch. The products'
+// coefficient blocks lie back to back in A, B and O in input order; logn (device, `count` words)
+// gives each one's degree. A one-workgroup kernel first builds the launch's queue in `sc`
+// (offsets, the order by degree largest first, items of up to ring_lds_per_item products of one
+// degree, and a header with the item count and a zeroed queue counter), so the launch pair needs
+// no host-side preparation and no memset. Scratch: ring_scratch_bytes(count) bytes.
+constexpr int RING_LDS_MAX_LOG_N = 12;
+struct RingScratch {
+    uint64_t* offs;    // count
+    uint64_t* items;   // count (at most one item per product)
+    uint32_t* order;   // count
+    uint32_t* hdr;     // 4 words
+};
+inline size_t ring_scratch_bytes(size_t count) { return count * (8 + 8 + 4) + 16; }
+inline RingScratch ring_scratch_at(void* base, size_t count) {  // base 16-byte aligned
+    uint8_t* p = static_cast<uint8_t*>(base);
+    RingScratch r;
+    r.offs = reinterpret_cast<uint64_t*>(p);
+    r.items = reinterpret_cast<uint64_t*>(p + count * 8);
+    r.hdr = reinterpret_cast<uint32_t*>(p + count * 16);
+    r.order = reinterpret_cast<uint32_t*>(p + count * 16 + 16);
+    return r;
+}
+uint32_t ring_lds_per_item(int log_n);
+void launch_ring_mul_lds(const zq* A, const zq* B, zq* O, const uint32_t* logn, uint32_t count, RingScratch sc,
+                         uint32_t workgroups, const NttTables& tw, hipStream_t s);
 // schoolbook products (the reference's ntt_enabled = false path), one workgroup per product
 void launch_ring_mul_naive(const zq* A, const zq* B, zq* O, const uint64_t* offs, const uint32_t* logn,
                            uint32_t count, hipStream_t s);

@@ -2015,6 +2015,376 @@ void launch_ring_mul_persistent(const zq* A, const zq* B, zq* O, const uint64_t*
                        offs, logn, order, count, counter, res, tw, lds_words);
 }
 
+// ---------------------------------------------------------------------------------------
+// LDS-resident ragged ring products (degrees 2^0 .. 2^12) in one persistent launch.
+//
+// A queue item is up to RPB products of one degree 2^LOG_M = E * TP. Each product's 8 prime slices
+// are held by TP lanes each (TP <= 64, so a slice never leaves its wave): lane l of slice q loads
+// the coefficients l + TP*i (i < E) of an operand straight from memory and reduces them mod p_q,
+// runs the transform layers len >= TP in registers, swaps to the contiguous layout (E*l .. E*l+E-1)
+// through its wave's part of LDS, and runs the layers len < TP. With both operands transformed,
+// the pointwise product is scaled by 1/m; the inverse runs the same two passes backwards. The
+// residues of all 8 primes then meet in LDS, where each output coefficient gets its centred CRT
+// lift. The transposes are wave-local, so an item has two workgroup barriers (before and after
+// the lift); operands are read once and nothing goes through global scratch.
+// ---------------------------------------------------------------------------------------
+constexpr int RING_LDS_THREADS = 512;
+constexpr int RING_LDS_MAX_LOG = 12;
+// 8 primes at degree 4096, one pad word per 64 coefficients: 130 KiB
+constexpr uint32_t RING_LDS_WORDS = NPRIMES * ((1u << RING_LDS_MAX_LOG) + 64);
+
+template <int E, int TP>
+struct RingCfg {
+    static constexpr int M = E * TP;
+    static constexpr int LOG_E = __builtin_ctz(E);
+    static constexpr int LOG_TP = __builtin_ctz(TP);
+    static constexpr int LOG_M = LOG_E + LOG_TP;
+    static constexpr int TPR = NPRIMES * TP;                // lanes per product
+    static constexpr int RPB = RING_LDS_THREADS / TPR;     // products per item
+    static constexpr int SLICE = M + TP;                    // words per prime slice, padded
+    static_assert(TP <= E && TP <= 64 && TPR >= 64 && RPB >= 1, "ring blocking");
+    static_assert(RPB * NPRIMES * SLICE <= (int)RING_LDS_WORDS, "ring LDS");
+    // LDS word of coefficient x of prime slice q: one pad word after every E coefficients, so the
+    // contiguous pattern (lane l at E*l + e) has lane stride E + 1 (odd: distinct banks) and the
+    // strided pattern (lane l at l + TP*i) is the lane's base plus a constant per i
+    __device__ static __forceinline__ int slot(int q, int x) { return q * SLICE + x + (x >> LOG_E); }
+    __device__ static __forceinline__ int strided(int q, int l, int i) {  // slot(q, l + TP * i)
+        return q * SLICE + l + TP * i + (i >> (LOG_E - LOG_TP));
+    }
+    __device__ static __forceinline__ int contiguous(int q, int l, int e) {  // slot(q, E * l + e)
+        return q * SLICE + (E + 1) * l + e;
+    }
+};
+
+// orders this lane's LDS accesses against the rest of its wave's (the transposes stay in a wave)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// keeps the compiler's scheduler from moving instructions across a phase boundary (loads hoisted
+// from a later phase would hold registers through this one)
+__device__ __forceinline__ void phase_boundary() { __builtin_amdgcn_sched_barrier(0); }
+
+__device__ __forceinline__ void gs_butterfly(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp, uint32_t p) {
+    const uint32_t u = x, v = y;
+    x = addp(u, v, p);
+    y = mulc(subp(u, v, p), w, wp, p);
+}
+
+// forward negacyclic transform of one prime slice: x[i] holds coefficient l + TP*i on entry and
+// bit-reversed position E*l + i on exit
+template <int E, int TP>
+__device__ __forceinline__ void ring_fwd(uint32_t* S, int q, int l, uint32_t (&x)[E], const uint32_t* zt,
+                                         uint32_t p, uint32_t pinv) {
+    using C = RingCfg<E, TP>;
+#pragma unroll
+    for (int s = E / 2; s >= 1; s >>= 1) {
+        phase_boundary();  // one layer's twiddles in registers at a time
+#pragma unroll
+        for (int b = 0; b < E / (2 * s); ++b) {
+            const uint32_t w = zt[E / (2 * s) + b], wp = w * pinv;
+#pragma unroll
+            for (int k = 0; k < s; ++k) ct_butterfly(x[2 * s * b + k], x[2 * s * b + k + s], w, wp, p);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < E; ++i) S[C::strided(q, l, i)] = x[i];
+    wave_lds_sync();
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = S[C::contiguous(q, l, e)];
+    wave_lds_sync();  // every lane has read its elements before the slice's LDS is written again
+#pragma unroll
+    for (int len = TP / 2; len >= 1; len >>= 1) {
+        phase_boundary();
+        const int base = C::M / (2 * len) + (E * l) / (2 * len);
+#pragma unroll
+        for (int b = 0; b < E / (2 * len); ++b) {
+            const uint32_t w = zt[base + b], wp = w * pinv;
+#pragma unroll
+            for (int k = 0; k < len; ++k) ct_butterfly(x[2 * len * b + k], x[2 * len * b + k + len], w, wp, p);
+        }
+    }
+}
+
+// the inverse (Gentleman-Sande, no 1/m): bit-reversed positions E*l + i in, coefficients
+// l + TP*i out
+template <int E, int TP>
+__device__ __forceinline__ void ring_inv(uint32_t* S, int q, int l, uint32_t (&x)[E], const uint32_t* izt,
+                                         uint32_t p, uint32_t pinv) {
+    using C = RingCfg<E, TP>;
+#pragma unroll
+    for (int len = 1; len < TP; len <<= 1) {
+        phase_boundary();
+        const int base = C::M / (2 * len) + (E * l) / (2 * len);
+#pragma unroll
+        for (int b = 0; b < E / (2 * len); ++b) {
+            const uint32_t w = izt[base + b], wp = w * pinv;
+#pragma unroll
+            for (int k = 0; k < len; ++k) gs_butterfly(x[2 * len * b + k], x[2 * len * b + k + len], w, wp, p);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) S[C::contiguous(q, l, e)] = x[e];
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < E; ++i) x[i] = S[C::strided(q, l, i)];
+    wave_lds_sync();
+#pragma unroll
+    for (int s = 1; s < E; s <<= 1) {
+        phase_boundary();
+#pragma unroll
+        for (int b = 0; b < E / (2 * s); ++b) {
+            const uint32_t w = izt[E / (2 * s) + b], wp = w * pinv;
+#pragma unroll
+            for (int k = 0; k < s; ++k) gs_butterfly(x[2 * s * b + k], x[2 * s * b + k + s], w, wp, p);
+        }
+    }
+}
+
+// An operand of every product of the item into LDS: each coefficient is loaded once (16-byte
+// loads, consecutive threads on consecutive coefficients) and reduced into all 8 primes, stored at
+// its slot in the product's region. The whole workgroup takes part. (Issuing a thread's E/8 loads
+// together costs registers the E = 64 blockings do not have; the kernel is bound by VALU issue.)
+template <int E, int TP>
+__device__ __forceinline__ void ring_stage(uint32_t* sh, const zq* __restrict__ src, const uint64_t* offs,
+                                           const uint32_t* order, uint32_t first, uint32_t cnt, int tid) {
+    using C = RingCfg<E, TP>;
+#pragma unroll 1
+    for (int idx = tid; idx < (int)cnt * C::M; idx += RING_LDS_THREADS) {
+        const int prod = idx >> C::LOG_M, c = idx & (C::M - 1);
+        const zq x = load_zq(src + offs[order[first + prod]] + c);
+        uint32_t* S = sh + prod * NPRIMES * C::SLICE;
+#pragma unroll
+        for (int q = 0; q < NPRIMES; ++q) S[C::slot(q, c)] = zq_mod_p_centred(x, c_pc[q]);
+    }
+}
+
+// one queue item of `cnt` products of degree E*TP; ends with a workgroup barrier
+template <int E, int TP>
+__device__ void ring_item(uint32_t* sh, const zq* __restrict__ A, const zq* __restrict__ B, zq* __restrict__ O,
+                          const uint64_t* offs, const uint32_t* order, uint32_t first, uint32_t cnt,
+                          const NttTables& tw, int tid) {
+    using C = RingCfg<E, TP>;
+    const int rsub = tid / C::TPR, tr = tid % C::TPR;
+    const bool valid = rsub < (int)cnt;  // uniform per wave (TPR >= 64)
+    const int q = tr >> C::LOG_TP, l = tr & (TP - 1);
+    uint32_t* S = sh + rsub * NPRIMES * C::SLICE;
+    const PrimeConsts pc = c_pc[q];
+    const uint32_t* zt = tw.zeta_m + q * TW_SIZE;
+    uint32_t a[E], b[E];
+    ring_stage<E, TP>(sh, A, offs, order, first, cnt, tid);
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) a[i] = S[C::strided(q, l, i)];
+        ring_fwd<E, TP>(S, q, l, a, zt, pc.p, pc.pinv);
+    }
+    phase_boundary();
+    __syncthreads();  // every slice has read its elements of a: the regions take b
+    ring_stage<E, TP>(sh, B, offs, order, first, cnt, tid);
+    __syncthreads();
+    phase_boundary();
+    if (valid) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) b[i] = S[C::strided(q, l, i)];
+        ring_fwd<E, TP>(S, q, l, b, zt, pc.p, pc.pinv);
+        // a * b / m: redc gives a*b/R, then * (R/m) as a mulc constant (R^2/m mod p)
+        const uint32_t im = tw.invn_m[q * (MAX_LOG_N + 1) + C::LOG_M];  // R/m
+        const uint32_t sm = mulc(im, pc.two64, pc.two64 * pc.pinv, pc.p), smp = sm * pc.pinv;
+#pragma unroll
+        for (int e = 0; e < E; ++e) a[e] = mulc(redc((uint64_t)a[e] * b[e], pc.p, pc.pinv), sm, smp, pc.p);
+        phase_boundary();
+        ring_inv<E, TP>(S, q, l, a, tw.izeta_m + q * TW_SIZE, pc.p, pc.pinv);
+#pragma unroll
+        for (int i = 0; i < E; ++i) S[C::strided(q, l, i)] = a[i];
+    }
+    __syncthreads();  // every prime's residues of the item in LDS
+    phase_boundary();
+    if (valid) {
+        const size_t off = offs[order[first + rsub]];
+#pragma unroll 1
+        for (int c = tr; c < C::M; c += C::TPR) {
+            uint32_t r[NPRIMES];
+#pragma unroll
+            for (int i = 0; i < NPRIMES; ++i) r[i] = S[C::slot(i, c)];
+            store_zq(O + off + c, crt_lift_residues(r));
+        }
+    }
+    __syncthreads();  // LDS and the item word are free for the next item
+}
+
+// degrees 2^0 .. 2^5: one product, 8 * 2^log_n lanes, transforms layer by layer in LDS
+__device__ void ring_item_small(uint32_t* sh, const zq* __restrict__ A, const zq* __restrict__ B,
+                                zq* __restrict__ O, uint64_t off, int log_n, const NttTables& tw, int tid) {
+    const int n = 1 << log_n, half = n >> 1;
+    const int t = tid;
+    uint32_t* sa = sh;
+    uint32_t* sb = sh + NPRIMES * n;
+    if (t < NPRIMES * n) {
+        const int q = t >> log_n, c = t & (n - 1);
+        sa[t] = zq_mod_p_centred(load_zq(A + off + c), c_pc[q]);
+        sb[t] = zq_mod_p_centred(load_zq(B + off + c), c_pc[q]);
+    }
+    __syncthreads();
+    for (int len = half; len >= 1; len >>= 1) {
+        const int lg = __builtin_ctz(len);
+        if (t < NPRIMES * half) {
+            const int q = t >> (log_n - 1), b = t & (half - 1);
+            const int blk = b >> lg, j = (blk << (lg + 1)) | (b & (len - 1)), kz = (half >> lg) + blk;
+            const uint32_t p = c_pc[q].p, w = tw.zeta_m[q * TW_SIZE + kz], wp = tw.zeta_mp[q * TW_SIZE + kz];
+            ct_butterfly(sa[q * n + j], sa[q * n + j + len], w, wp, p);
+            ct_butterfly(sb[q * n + j], sb[q * n + j + len], w, wp, p);
+        }
+        __syncthreads();
+    }
+    if (t < NPRIMES * n) {
+        const int q = t >> log_n;
+        const PrimeConsts pc = c_pc[q];
+        const uint32_t im = tw.invn_m[q * (MAX_LOG_N + 1) + log_n];
+        const uint32_t sm = mulc(im, pc.two64, pc.two64 * pc.pinv, pc.p);
+        sa[t] = mulc(redc((uint64_t)sa[t] * sb[t], pc.p, pc.pinv), sm, sm * pc.pinv, pc.p);
+    }
+    __syncthreads();
+    for (int len = 1; len < n; len <<= 1) {
+        const int lg = __builtin_ctz(len);
+        if (t < NPRIMES * half) {
+            const int q = t >> (log_n - 1), b = t & (half - 1);
+            const int blk = b >> lg, j = (blk << (lg + 1)) | (b & (len - 1)), kz = (half >> lg) + blk;
+            const uint32_t p = c_pc[q].p, w = tw.izeta_m[q * TW_SIZE + kz], wp = tw.izeta_mp[q * TW_SIZE + kz];
+            gs_butterfly(sa[q * n + j], sa[q * n + j + len], w, wp, p);
+        }
+        __syncthreads();
+    }
+    if (t < n) {
+        uint32_t r[NPRIMES];
+#pragma unroll
+        for (int i = 0; i < NPRIMES; ++i) r[i] = sa[i * n + t];
+        store_zq(O + off + t, crt_lift_residues(r));
+    }
+    __syncthreads();
+}
+
+// items[k]: first index into `order` (bits 0..31), log2 degree (bits 32..39), product count
+// (bits 40..47). hdr[0] = number of items, hdr[1] = the queue counter (both written by
+// k_ring_prep); every workgroup leaves once the items run out.
+__global__ void __launch_bounds__(RING_LDS_THREADS) k_ring_mul_lds(const zq* __restrict__ A, const zq* __restrict__ B,
+                                                                   zq* __restrict__ O, const uint64_t* __restrict__ offs,
+                                                                   const uint32_t* __restrict__ order,
+                                                                   const uint64_t* __restrict__ items, uint32_t* hdr,
+                                                                   NttTables tw) {
+    extern __shared__ uint32_t sh[];
+    __shared__ uint32_t item_sh;
+    const uint32_t n_items = hdr[0];
+    for (;;) {
+        if (threadIdx.x == 0) item_sh = atomicAdd(hdr + 1, 1u);
+        __syncthreads();
+        const uint32_t it = item_sh;
+        if (it >= n_items) break;  // uniform: every thread read the same word
+        const uint64_t d = items[it];
+        const uint32_t first = (uint32_t)d, cnt = (uint32_t)(d >> 40) & 0xFFu;
+        const int log_n = (int)((d >> 32) & 0xFFu);
+        // the thread index as an opaque per-item value: what the item code derives from it is
+        // computed per item instead of being hoisted out of this loop for every degree at once
+        int tid = (int)threadIdx.x;
+        __asm__ volatile("" : "+v"(tid));
+        switch (log_n) {
+            case 6: ring_item<8, 8>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
+            case 7: ring_item<16, 8>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
+            case 8: ring_item<16, 16>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
+            case 9: ring_item<32, 16>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
+            case 10: ring_item<32, 32>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
+            case 11: ring_item<64, 32>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
+            case 12: ring_item<64, 64>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
+            default: ring_item_small(sh, A, B, O, offs[order[first]], log_n, tw, tid); break;
+        }
+    }
+}
+
+__host__ __device__ constexpr uint32_t ring_per_item(int log_n) {
+    return log_n == 6 ? RingCfg<8, 8>::RPB
+         : log_n == 7 ? RingCfg<16, 8>::RPB
+         : log_n == 8 ? RingCfg<16, 16>::RPB
+         : log_n == 9 ? RingCfg<32, 16>::RPB
+         : log_n == 10 ? RingCfg<32, 32>::RPB
+         : log_n == 11 ? RingCfg<64, 32>::RPB
+         : log_n == 12 ? RingCfg<64, 64>::RPB : 1;
+}
+uint32_t ring_lds_per_item(int log_n) { return ring_per_item(log_n); }
+
+// The queue of a ragged batch, built on the device by one workgroup from the products' degrees
+// (logn[i] <= 12, product i's coefficients back to back in input order): offs = exclusive prefix
+// sums of the degrees, order = the products grouped by degree, largest degree first (a counting
+// sort), items = each degree group cut into runs of ring_per_item products, hdr = (item count, 0).
+constexpr int RING_PREP_THREADS = 1024;
+__global__ void __launch_bounds__(RING_PREP_THREADS) k_ring_prep(const uint32_t* __restrict__ logn, uint32_t count,
+                                                                 uint64_t* __restrict__ offs, uint32_t* __restrict__ order,
+                                                                 uint64_t* __restrict__ items, uint32_t* hdr) {
+    __shared__ uint64_t part[RING_PREP_THREADS];
+    __shared__ uint32_t hist[RING_LDS_MAX_LOG + 1], start[RING_LDS_MAX_LOG + 1], fill[RING_LDS_MAX_LOG + 1];
+    __shared__ uint32_t ibase[RING_LDS_MAX_LOG + 2];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (count + RING_PREP_THREADS - 1) / RING_PREP_THREADS;
+    const uint32_t lo = min(count, t * per), hi = min(count, lo + per);
+    if (t <= RING_LDS_MAX_LOG) {
+        hist[t] = 0;
+        fill[t] = 0;
+    }
+    uint64_t sum = 0;
+    for (uint32_t i = lo; i < hi; ++i) sum += (uint64_t)1 << logn[i];
+    part[t] = sum;
+    __syncthreads();
+    // degrees above 2^12 are refused by the host before the launch; skipped here all the same
+    for (uint32_t i = lo; i < hi; ++i)
+        if (logn[i] <= RING_LDS_MAX_LOG) atomicAdd(&hist[logn[i]], 1u);
+    // inclusive scan of the per-thread sums (Hillis-Steele in LDS)
+    for (uint32_t d = 1; d < RING_PREP_THREADS; d <<= 1) {
+        const uint64_t v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint64_t run = part[t] - sum;  // exclusive prefix of this thread's range
+    for (uint32_t i = lo; i < hi; ++i) {
+        offs[i] = run;
+        run += (uint64_t)1 << logn[i];
+    }
+    if (t == 0) {  // group starts, largest degree first, and each group's first item
+        uint32_t at = 0, it = 0;
+        for (int d = RING_LDS_MAX_LOG; d >= 0; --d) {
+            start[d] = at;
+            ibase[d] = it;
+            at += hist[d];
+            it += (hist[d] + ring_per_item(d) - 1) / ring_per_item(d);
+        }
+        ibase[RING_LDS_MAX_LOG + 1] = it;
+        hdr[0] = it;
+        hdr[1] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t d = logn[i];
+        if (d <= RING_LDS_MAX_LOG) order[start[d] + atomicAdd(&fill[d], 1u)] = i;
+    }
+    for (int d = 0; d <= RING_LDS_MAX_LOG; ++d) {
+        const uint32_t pi = ring_per_item(d), n_it = (hist[d] + pi - 1) / pi;
+        for (uint32_t k = t; k < n_it; k += RING_PREP_THREADS) {
+            const uint32_t first = start[d] + k * pi, cnt = min(pi, hist[d] - k * pi);
+            items[ibase[d] + k] = (uint64_t)first | ((uint64_t)d << 32) | ((uint64_t)cnt << 40);
+        }
+    }
+}
+
+void launch_ring_mul_lds(const zq* A, const zq* B, zq* O, const uint32_t* logn, uint32_t count, RingScratch sc,
+                         uint32_t workgroups, const NttTables& tw, hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_ring_prep, dim3(1), dim3(RING_PREP_THREADS), 0, s, logn, count, sc.offs, sc.order, sc.items,
+                       sc.hdr);
+    hipLaunchKernelGGL(k_ring_mul_lds, dim3(std::min(workgroups, count)), dim3(RING_LDS_THREADS),
+                       (size_t)RING_LDS_WORDS * 4, s, A, B, O, sc.offs, sc.order, sc.items, sc.hdr, tw);
+}
+
 // Schoolbook negacyclic products, the reference's product with ntt_enabled = false
 // (naive_negacyclic, src/algebra/ntt.rs:226-240, chosen in RingElement::mul, src/algebra/poly.rs:
 // 134-140). One workgroup per product; thread k sums a_i * b_(k-i) with the wrapped terms

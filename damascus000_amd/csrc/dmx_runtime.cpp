@@ -266,13 +266,21 @@ struct dmx_ctx {
     std::vector<hipEvent_t> event_pool;
     // ring-product scratch (grow-only), one batch at a time
     std::mutex ring_mu;
-    DevBuf ring_offs, ring_logn, ring_order, ring_res, ring_counter, ring_ga, ring_gb, ring_ra, ring_rb;
+    DevBuf ring_offs, ring_logn, ring_order, ring_res, ring_counter, ring_ga, ring_gb, ring_ra, ring_rb, ring_scratch;
+    void* ring_pinned = nullptr;  // host staging of the degrees
+    size_t ring_pinned_bytes = 0;
+    hipStream_t ring_stream = nullptr;  // products run on their own non-blocking stream
+    hipEvent_t ring_done = nullptr;
+    int cus = 256;  // compute units (one LDS-resident product workgroup each)
     ~dmx_ctx() {
         for (auto& t : launches) {
             (void)hipEventDestroy(t.a);
             (void)hipEventDestroy(t.b);
         }
         for (auto e : event_pool) (void)hipEventDestroy(e);
+        if (ring_pinned) (void)hipHostFree(ring_pinned);
+        if (ring_done) (void)hipEventDestroy(ring_done);
+        if (ring_stream) (void)hipStreamDestroy(ring_stream);
     }
     hipEvent_t take_event() {
         if (!event_pool.empty()) {
@@ -607,6 +615,11 @@ extern "C" int dmx_ctx_create(int device_index, dmx_ctx** out) {
     HIP_TRY(hipSetDevice(device_index));
     std::unique_ptr<dmx_ctx> c(new dmx_ctx);
     c->device = device_index;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device_index) == hipSuccess && prop.multiProcessorCount > 0)
+            c->cus = prop.multiProcessorCount;
+    }
     std::vector<uint32_t> host;
     build_constants(c.get(), host);
     int rc = c->tables.alloc(host.size() * 4);
@@ -620,6 +633,8 @@ extern "C" int dmx_ctx_create(int device_index, dmx_ctx** out) {
     c->tw.invn_m = base + 4 * NPRIMES * TW_SIZE;
     c->tw.invn_mp = base + 4 * NPRIMES * TW_SIZE + NPRIMES * (MAX_LOG_N + 1);
     set_device_constants(c->pc, &c->crt, 0);
+    HIP_TRY(hipStreamCreateWithFlags(&c->ring_stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&c->ring_done, hipEventDisableTiming));
     HIP_TRY(hipDeviceSynchronize());
     rc = tune_transforms(c.get());
     if (rc) return rc;
@@ -2610,6 +2625,18 @@ extern "C" int dmx_commit_with_generators(dmx_ctx* ctx, const uint64_t* witness,
     return commit_impl(ctx, witness, gens, nullptr, rows, ring_len, com_out, com_len);
 }
 
+// Waits for the product stream by polling an event (a blocking synchronize adds tens of
+// microseconds of wake-up latency to a call that is itself well under a millisecond).
+static int ring_wait(dmx_ctx* ctx, hipStream_t s) {
+    HIP_TRY(hipEventRecord(ctx->ring_done, s));
+    for (;;) {
+        const hipError_t e = hipEventQuery(ctx->ring_done);
+        if (e == hipSuccess) return DMX_OK;
+        if (e != hipErrorNotReady) return fail(DMX_ERR_HIP, "product stream: %s", hipGetErrorString(e));
+        std::this_thread::yield();
+    }
+}
+
 // ragged negacyclic products: group by degree, transform both operands, multiply, invert, lift
 // Ragged negacyclic products on device-resident operands. mode 0: one persistent launch over
 // the whole batch; mode 1: one launch sequence per degree group (gather, transforms, pointwise,
@@ -2629,7 +2656,32 @@ static int ring_mul_device(dmx_ctx* ctx, const zq* dA, const zq* dB, zq* dO, con
     for (size_t i = 0; i < count; ++i) order[i] = (uint32_t)i;
     std::lock_guard<std::mutex> g(ctx->ring_mu);
     auto grow = [](DevBuf& b, size_t bytes) { return b.bytes >= bytes ? DMX_OK : b.alloc(bytes); };
-    int rc = grow(ctx->ring_offs, count * 8);
+    int rc = DMX_OK;
+    if (mode == 0 && max_lg <= RING_LDS_MAX_LOG_N) {
+        // LDS-resident products: the degrees go up (one copy from pinned staging) and the device
+        // builds the launch's queue itself
+        if (count * 4 > ctx->ring_pinned_bytes) {
+            if (ctx->ring_pinned) (void)hipHostFree(ctx->ring_pinned);
+            ctx->ring_pinned = nullptr;
+            ctx->ring_pinned_bytes = 0;
+            if (hipHostMalloc(&ctx->ring_pinned, count * 4, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                ctx->ring_pinned = nullptr;
+                return fail(DMX_ERR_OOM, "pinned staging of %zu bytes failed", count * 4);
+            }
+            ctx->ring_pinned_bytes = count * 4;
+        }
+        memcpy(ctx->ring_pinned, log_n, count * 4);
+        rc = grow(ctx->ring_logn, count * 4);
+        rc = rc ? rc : grow(ctx->ring_scratch, ring_scratch_bytes(count));
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->ring_logn.p, ctx->ring_pinned, count * 4, hipMemcpyHostToDevice, s));
+        launch_ring_mul_lds(dA, dB, dO, ctx->ring_logn.as<uint32_t>(), (uint32_t)count,
+                            ring_scratch_at(ctx->ring_scratch.p, count), (uint32_t)ctx->cus, ctx->tw, s);
+        HIP_TRY(hipGetLastError());
+        return ring_wait(ctx, s);  // the pinned staging is reused by the next call
+    }
+    rc = grow(ctx->ring_offs, count * 8);
     rc = rc ? rc : grow(ctx->ring_logn, count * 4);
     rc = rc ? rc : grow(ctx->ring_order, count * 4);
     if (rc) return rc;
@@ -2642,7 +2694,7 @@ static int ring_mul_device(dmx_ctx* ctx, const zq* dA, const zq* dB, zq* dO, con
         HIP_TRY(hipStreamSynchronize(s));
         return DMX_OK;
     }
-    if (mode == 0) {
+    if (mode == 0) {  // degree 2^13 present: the layered persistent kernel over the whole batch
         std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return log_n[x] > log_n[y]; });
         HIP_TRY(hipMemcpyAsync(ctx->ring_order.p, order.data(), count * 4, hipMemcpyHostToDevice, s));
         rc = grow(ctx->ring_res, total * NPRIMES * 4);
@@ -2652,8 +2704,7 @@ static int ring_mul_device(dmx_ctx* ctx, const zq* dA, const zq* dB, zq* dO, con
                                    ctx->ring_order.as<uint32_t>(), (uint32_t)count, ctx->ring_counter.as<uint32_t>(),
                                    ctx->ring_res.as<uint32_t>(), max_lg, ctx->tw, s);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipStreamSynchronize(s));  // host vectors above outlive the copies
-        return DMX_OK;
+        return ring_wait(ctx, s);  // host vectors above outlive the copies
     }
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return log_n[x] < log_n[y]; });
     HIP_TRY(hipMemcpyAsync(ctx->ring_order.p, order.data(), count * 4, hipMemcpyHostToDevice, s));
@@ -2706,7 +2757,7 @@ extern "C" int dmx_ring_mul_batch(dmx_ctx* ctx, const uint64_t* a, const uint64_
         total += (size_t)1 << log_n[i];
     }
     HIP_TRY(hipSetDevice(ctx->device));
-    hipStream_t s = 0;
+    hipStream_t s = ctx->ring_stream;
     DevBuf dA, dB, dO;
     int rc = 0;
     rc = rc ? rc : dA.alloc(total * 16);
@@ -2717,7 +2768,8 @@ extern "C" int dmx_ring_mul_batch(dmx_ctx* ctx, const uint64_t* a, const uint64_
     HIP_TRY(hipMemcpyAsync(dB.p, b, total * 16, hipMemcpyHostToDevice, s));
     rc = ring_mul_device(ctx, dA.as<zq>(), dB.as<zq>(), dO.as<zq>(), log_n, count, 0, s);
     if (rc) return rc;
-    HIP_TRY(hipMemcpy(out, dO.p, total * 16, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(out, dO.p, total * 16, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return DMX_OK;
 }
 
@@ -2744,7 +2796,7 @@ extern "C" int dmx_ring_mul_batch_cfg(dmx_ctx* ctx, const uint64_t* a, const uin
     if (rc) return rc;
     HIP_TRY(hipMemcpy(dA.p, a, total * 16, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dB.p, b, total * 16, hipMemcpyHostToDevice));
-    rc = ring_mul_device(ctx, dA.as<zq>(), dB.as<zq>(), dO.as<zq>(), log_n, count, 2, 0);
+    rc = ring_mul_device(ctx, dA.as<zq>(), dB.as<zq>(), dO.as<zq>(), log_n, count, 2, ctx->ring_stream);
     if (rc) return rc;
     HIP_TRY(hipMemcpy(out, dO.p, total * 16, hipMemcpyDeviceToHost));
     return DMX_OK;
@@ -2758,7 +2810,7 @@ extern "C" int dmx_ring_mul_batch_device(dmx_ctx* ctx, const void* d_a, const vo
     if (!count) return DMX_OK;
     HIP_TRY(hipSetDevice(ctx->device));
     return ring_mul_device(ctx, static_cast<const zq*>(d_a), static_cast<const zq*>(d_b), static_cast<zq*>(d_out),
-                           log_n, count, mode, 0);
+                           log_n, count, mode, ctx->ring_stream);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -108,3 +108,145 @@ def test_lds_swizzle_is_a_bijection_without_bank_repeats(E, TP):
             for banks in ([_slot(E, TP, q, l + TP * i) % 32 for q, l in lanes],
                           [_slot(E, TP, q, E * l + i) % 32 for q, l in lanes]):
                 assert len(banks) == len(set(banks)), (E, TP, base, i)
+
+
+# ---------------- the LDS-resident ragged product kernel (k_ring_mul_lds) ----------------
+def _ring_shapes():
+    src = open(os.path.join(ROOT, "damascus000_amd", "csrc", "dmx_ntt.hip")).read()
+    shapes = sorted({(int(c), int(e), int(t)) for c, e, t in
+                     re.findall(r"log_n == (\d+) \? RingCfg<(\d+), (\d+)>::RPB", src)})
+    launched = {(int(e), int(t)) for e, t in re.findall(r"ring_item<(\d+), (\d+)>\(sh,", src)}
+    assert launched == {(e, t) for _, e, t in shapes}, "launched blockings differ from the item table"
+    assert shapes, "no ring blockings found"
+    threads = int(re.search(r"constexpr int RING_LDS_THREADS = (\d+);", src).group(1))
+    words = 8 * ((1 << int(re.search(r"RING_LDS_MAX_LOG = (\d+);", src).group(1))) + 64)
+    return shapes, threads, words
+
+
+RING_SHAPES, RING_THREADS, RING_WORDS = _ring_shapes()
+IZETA = [pow(z, P - 2, P) for z in ZETA]
+
+
+def _ring_slot(E, TP, q, x):
+    return q * (E * TP + TP) + x + (x >> (E.bit_length() - 1))
+
+
+def _ring_strided(E, TP, q, l, i):
+    return q * (E * TP + TP) + l + TP * i + (i >> ((E.bit_length() - 1) - (TP.bit_length() - 1)))
+
+
+def _ring_contiguous(E, TP, q, l, e):
+    return q * (E * TP + TP) + (E + 1) * l + e
+
+
+def _gs(a, i0, i1, w):
+    u, v = a[i0], a[i1]
+    a[i0], a[i1] = (u + v) % P, (u - v) * w % P
+
+
+def ring_forward(x, E, TP, lds):
+    """ring_fwd through a simulated LDS (prime slice 0): registers hold l + TP*i, then E*l + e"""
+    M = E * TP
+    regs = [[x[l + TP * i] for i in range(E)] for l in range(TP)]
+    for l in range(TP):
+        a = regs[l]
+        s = E // 2
+        while s >= 1:
+            for b in range(E // (2 * s)):
+                for k in range(s):
+                    _butterfly(a, 2 * s * b + k, 2 * s * b + k + s, ZETA[E // (2 * s) + b])
+            s //= 2
+        for i in range(E):
+            lds[_ring_strided(E, TP, 0, l, i)] = a[i]
+    for l in range(TP):
+        regs[l] = [lds[_ring_contiguous(E, TP, 0, l, e)] for e in range(E)]
+    for l in range(TP):
+        a, ln = regs[l], TP // 2
+        while ln >= 1:
+            base = M // (2 * ln) + (E * l) // (2 * ln)
+            for b in range(E // (2 * ln)):
+                for k in range(ln):
+                    _butterfly(a, 2 * ln * b + k, 2 * ln * b + k + ln, ZETA[base + b])
+            ln //= 2
+    return regs
+
+
+def ring_inverse(regs, E, TP, lds):
+    """ring_inv: Gentleman-Sande back through the same two layouts (no 1/m)"""
+    M = E * TP
+    for l in range(TP):
+        a, ln = regs[l], 1
+        while ln < TP:
+            base = M // (2 * ln) + (E * l) // (2 * ln)
+            for b in range(E // (2 * ln)):
+                for k in range(ln):
+                    _gs(a, 2 * ln * b + k, 2 * ln * b + k + ln, IZETA[base + b])
+            ln *= 2
+        for e in range(E):
+            lds[_ring_contiguous(E, TP, 0, l, e)] = a[e]
+    out = [None] * M
+    for l in range(TP):
+        a = [lds[_ring_strided(E, TP, 0, l, i)] for i in range(E)]
+        s = 1
+        while s < E:
+            for b in range(E // (2 * s)):
+                for k in range(s):
+                    _gs(a, 2 * s * b + k, 2 * s * b + k + s, IZETA[E // (2 * s) + b])
+            s *= 2
+        for i in range(E):
+            out[l + TP * i] = a[i]
+    return out
+
+
+@pytest.mark.parametrize("log_n,E,TP", RING_SHAPES)
+def test_ring_blocking_shape_rules(log_n, E, TP):
+    assert E * TP == 1 << log_n and TP <= E and TP <= 64
+    tpr = 8 * TP
+    assert tpr >= 64 and RING_THREADS % tpr == 0  # a slice never leaves its wave; items fill the workgroup
+    assert (RING_THREADS // tpr) * 8 * (E * TP + TP) <= RING_WORDS
+
+
+@pytest.mark.parametrize("log_n,E,TP", RING_SHAPES)
+def test_ring_product_algebra_equals_schoolbook(log_n, E, TP):
+    """forward both operands, pointwise product times 1/m, inverse: the negacyclic product mod p"""
+    M = E * TP
+    rng = random.Random(log_n)
+    x = [rng.randrange(P) for _ in range(M)]
+    y = [rng.randrange(P) for _ in range(M)]
+    lds = [None] * (E * TP + TP)
+    fx = ring_forward(x, E, TP, lds)
+    fy = ring_forward(y, E, TP, lds)
+    assert [v for r in fx for v in r] == per_layer(x)  # contiguous layout = bit-reversed order
+    inv_m = pow(M, P - 2, P)
+    prod = [[a * b * inv_m % P for a, b in zip(ra, rb)] for ra, rb in zip(fx, fy)]
+    got = ring_inverse(prod, E, TP, lds)
+    if M <= 256:
+        want = [0] * M
+        for i in range(M):
+            for j in range(M):
+                k, v = i + j, x[i] * y[j]
+                if k >= M:
+                    want[k - M] -= v
+                else:
+                    want[k] += v
+        assert got == [w % P for w in want]
+    else:  # the inverse undoes the forward transform exactly
+        assert ring_inverse(ring_forward(x, E, TP, lds), E, TP, lds) == [v * M % P for v in x]
+
+
+@pytest.mark.parametrize("log_n,E,TP", RING_SHAPES)
+def test_ring_lds_layout_is_a_bijection_without_bank_repeats(log_n, E, TP):
+    M, tpr = E * TP, 8 * TP
+    for q in range(8):
+        slots = [_ring_slot(E, TP, q, x) for x in range(M)]
+        assert len(set(slots)) == M and min(slots) >= q * (M + TP) and max(slots) < (q + 1) * (M + TP)
+        for l in range(TP):
+            for i in range(E):
+                assert _ring_strided(E, TP, q, l, i) == _ring_slot(E, TP, q, l + TP * i)
+                assert _ring_contiguous(E, TP, q, l, i) == _ring_slot(E, TP, q, E * l + i)
+    for base in range(0, tpr, 32):  # 32 lanes of one LDS pass: prime t // TP, lane t % TP
+        lanes = [(t // TP, t % TP) for t in range(base, base + 32)]
+        for i in range(E):
+            for banks in ([_ring_strided(E, TP, q, l, i) % 32 for q, l in lanes],
+                          [_ring_contiguous(E, TP, q, l, i) % 32 for q, l in lanes]):
+                assert len(banks) == len(set(banks)), (E, TP, base, i)
