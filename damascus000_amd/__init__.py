@@ -108,6 +108,7 @@ _SIGS = {
     "dmx_fold_batch": (ctypes.c_int, [_VP] * 6 + [_SZ, ctypes.c_uint64, ctypes.c_uint64]),
     "dmx_ring_mul_batch": (ctypes.c_int, [_VP] * 5 + [_SZ]),
     "dmx_ring_mul_batch_device": (ctypes.c_int, [_VP] * 5 + [_SZ, ctypes.c_int]),
+    "dmx_ring_mul_batch_device_noalloc": (ctypes.c_int, [_VP] * 5 + [_SZ]),
     "dmx_ring_mul_batch_cfg": (ctypes.c_int, [_VP] * 5 + [_SZ, _VP]),
     "dmx_derive_subseed": (ctypes.c_int, [_VP, _VP, _SZ, _VP]),
     "dmx_derive_generators": (ctypes.c_int, [_VP, _VP, _SZ, _SZ, _VP]),

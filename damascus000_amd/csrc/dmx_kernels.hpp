@@ -293,6 +293,14 @@ inline RingScratch ring_scratch_at(void* base, size_t count) {  // base 16-byte 
     return r;
 }
 uint32_t ring_lds_per_item(int log_n);
+// The allocation-free entry point's scratch (dmx_noalloc.hip): a file-scope device array holding
+// the degrees (NOALLOC_MAX_PRODUCTS words, then 256 bytes of alignment) followed by the queue
+// (ring_scratch_bytes). noalloc_scratch_init looks up its device address on the current device,
+// once per device.
+constexpr size_t NOALLOC_MAX_PRODUCTS = 65536;
+constexpr size_t NOALLOC_QUEUE_OFFSET = NOALLOC_MAX_PRODUCTS * 4 + 256;
+size_t noalloc_scratch_bytes();
+hipError_t noalloc_scratch_init(int device, void** out);
 void launch_ring_mul_lds(const zq* A, const zq* B, zq* O, const uint32_t* logn, uint32_t count, RingScratch sc,
                          uint32_t workgroups, const NttTables& tw, hipStream_t s);
 // schoolbook products (the reference's ntt_enabled = false path), one workgroup per product

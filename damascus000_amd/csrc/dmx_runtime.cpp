@@ -271,6 +271,7 @@ struct dmx_ctx {
     size_t ring_pinned_bytes = 0;
     hipStream_t ring_stream = nullptr;  // products run on their own non-blocking stream
     hipEvent_t ring_done = nullptr;
+    void* noalloc = nullptr;  // device address of the allocation-free entry point's scratch
     int cus = 256;  // compute units (one LDS-resident product workgroup each)
     ~dmx_ctx() {
         for (auto& t : launches) {
@@ -633,6 +634,10 @@ extern "C" int dmx_ctx_create(int device_index, dmx_ctx** out) {
     c->tw.invn_m = base + 4 * NPRIMES * TW_SIZE;
     c->tw.invn_mp = base + 4 * NPRIMES * TW_SIZE + NPRIMES * (MAX_LOG_N + 1);
     set_device_constants(c->pc, &c->crt, 0);
+    {
+        const hipError_t e = noalloc_scratch_init(device_index, &c->noalloc);
+        if (e != hipSuccess) return fail(DMX_ERR_HIP, "allocation-free scratch lookup: %s", hipGetErrorString(e));
+    }
     HIP_TRY(hipStreamCreateWithFlags(&c->ring_stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&c->ring_done, hipEventDisableTiming));
     HIP_TRY(hipDeviceSynchronize());
@@ -2811,6 +2816,37 @@ extern "C" int dmx_ring_mul_batch_device(dmx_ctx* ctx, const void* d_a, const vo
     HIP_TRY(hipSetDevice(ctx->device));
     return ring_mul_device(ctx, static_cast<const zq*>(d_a), static_cast<const zq*>(d_b), static_cast<zq*>(d_out),
                            log_n, count, mode, ctx->ring_stream);
+}
+
+// The ragged products of dmx_ring_mul_batch_device (mode 0) without any allocation: the degrees and
+// the launch's queue go to the device array looked up at context creation (ctx->noalloc). Up to
+// NOALLOC_MAX_PRODUCTS products of degree at most 2^12. Calls on one device share that array, so
+// calls that overlap on one device (other contexts, other threads) race; calls through one
+// context are serialised by its product lock.
+extern "C" int dmx_ring_mul_batch_device_noalloc(dmx_ctx* ctx, const void* d_a, const void* d_b, void* d_out,
+                                                 const uint32_t* log_n, size_t count) {
+    if (!ctx || (count && (!d_a || !d_b || !d_out || !log_n))) return fail(DMX_ERR_ARG, "null argument");
+    if (gpu_disabled_by_env()) return fail(DMX_ERR_NO_DEVICE, "disabled by DAMASCUS_GPU=0");
+    if (count > NOALLOC_MAX_PRODUCTS)
+        return fail(DMX_ERR_ARG, "%zu products exceed the allocation-free limit of %zu", count,
+                    (size_t)NOALLOC_MAX_PRODUCTS);
+    for (size_t i = 0; i < count; ++i)
+        if (log_n[i] > (uint32_t)RING_LDS_MAX_LOG_N)
+            return fail(DMX_ERR_SHAPE, "degree 2^%u exceeds the allocation-free limit of 2^%d", log_n[i],
+                        RING_LDS_MAX_LOG_N);
+    if (!count) return DMX_OK;
+    if (!ctx->noalloc) return fail(DMX_ERR_HIP, "the allocation-free scratch was not looked up");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> g(ctx->ring_mu);
+    hipStream_t s = ctx->ring_stream;
+    uint8_t* base = static_cast<uint8_t*>(ctx->noalloc);
+    uint32_t* d_logn = reinterpret_cast<uint32_t*>(base);
+    HIP_TRY(hipMemcpyAsync(d_logn, log_n, count * 4, hipMemcpyHostToDevice, s));
+    launch_ring_mul_lds(static_cast<const zq*>(d_a), static_cast<const zq*>(d_b), static_cast<zq*>(d_out), d_logn,
+                        (uint32_t)count, ring_scratch_at(base + NOALLOC_QUEUE_OFFSET, count), (uint32_t)ctx->cus,
+                        ctx->tw, s);
+    HIP_TRY(hipGetLastError());
+    return ring_wait(ctx, s);  // the caller's degrees array is read by the copy before this returns
 }
 
 // ------------------------------------------------------------------------------------------

@@ -118,6 +118,15 @@ int dmx_ring_mul_batch_cfg(dmx_ctx *ctx, const uint64_t *a, const uint64_t *b, u
 int dmx_ring_mul_batch_device(dmx_ctx *ctx, const void *d_a, const void *d_b, void *d_out,
                               const uint32_t *log_n, size_t count, int mode);
 
+/* The same products (mode 0) with no allocation of any kind: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The launch's queue lives in a fixed device array of about
+ * 1.5 MiB per device, looked up when a context is created; at most 65,536 products of degree at
+ * most 2^12 per call (DMX_ERR_ARG / DMX_ERR_SHAPE beyond). Calls on one device share that array:
+ * calls that overlap on one device race (calls through one context are serialised). */
+int dmx_ring_mul_batch_device_noalloc(dmx_ctx *ctx, const void *d_a, const void *d_b, void *d_out,
+                                      const uint32_t *log_n, size_t count);
+
 /* ---- commitment (src/commitment/sis.rs) ---- */
 /* derive_subseed(seed, "g0") (sis.rs:208-210) */
 int dmx_derive_subseed(const uint8_t seed[32], const uint8_t *label, size_t label_len, uint8_t out[32]);
