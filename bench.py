@@ -227,6 +227,48 @@ def ragged_products(dm, ctx, count=4096, reps=5, seed=3):
             "persistent_products_per_s": round(count / times[0], 1), "bit_exact_between_modes": True}
 
 
+def drop_in_latency(dm, reps=50):
+    """Per-call latency of the reference-compatible entry points, host buffers in and out as the
+    reference's FFI has them: dmx_ntt_batch at ring length 1024 (cyclic size 2048, 8 primes) and
+    dmx_fold_batch over 4096 values"""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+
+    S = 2048
+    rng = np.random.default_rng(8)
+    a = np.stack([rng.integers(0, p, size=S, dtype=np.uint64) for p in O.PRIMES])
+    b = np.stack([rng.integers(0, p, size=S, dtype=np.uint64) for p in O.PRIMES])
+    roots, iroots, inv = [], [], []
+    for p in O.PRIMES:
+        g = O.lib().orc_multiplicative_generator(p)
+        w = pow(g, (p - 1) // S, p)
+        wi, ln = pow(w, p - 2, p), 2
+        while ln <= S:
+            roots.append(pow(w, S // ln, p))
+            iroots.append(pow(wi, S // ln, p))
+            ln *= 2
+        inv.append(pow(S, p - 2, p))
+    dm.ntt_batch(a, b, S, O.PRIMES, roots, iroots, inv)  # plan tables built on the first call
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rc, _ = dm.ntt_batch(a, b, S, O.PRIMES, roots, iroots, inv)
+    ntt_us = (time.perf_counter() - t0) / reps * 1e6
+    if rc != 0:
+        raise SystemExit("ntt_batch failed")
+    n = 4096
+    left = [int(x) for x in rng.integers(0, 1 << 62, size=n)]
+    right = [int(x) for x in rng.integers(0, 1 << 62, size=n)]
+    dm.fold_batch(left, right, 12345)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rc, _ = dm.fold_batch(left, right, 12345)
+    fold_us = (time.perf_counter() - t0) / reps * 1e6
+    return {"ntt_batch_ring1024_us": round(ntt_us, 1), "fold_batch_4096_us": round(fold_us, 1),
+            "note": "per call through the Python mirror (ctypes, list conversion for fold_batch included)"}
+
+
 def depth_of(n):
     """the witness depth of an n-byte file (io.rs:72-120: the smallest d >= 6 with 4^d >= ceil(n/13))"""
     c, d = max(1, -(-n // 13)), 6
@@ -463,6 +505,7 @@ def main():
         line["served"]["bit_exact_vs_plain"] = True
         line["batched_files"] = batched_files(dm, ctx)
         line["ragged_products"] = ragged_products(dm, ctx)
+        line["drop_in"] = drop_in_latency(dm)
     else:
         line["value"] = single["mib_s"]
         line["ms_per_step"] = single["ms_per_job"]

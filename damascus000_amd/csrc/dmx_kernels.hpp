@@ -255,12 +255,19 @@ void launch_gen_generators(const uint32_t* gseed, size_t rows, int log_m, zq* ou
                            const Jobs& jb = Jobs{});
 
 // the reference-compatible batch entry points
-void launch_fold_pairs(const uint64_t* l_lo, const uint64_t* l_hi, const uint64_t* r_lo,
-                       const uint64_t* r_hi, uint64_t* o_lo, uint64_t* o_hi, size_t len,
-                       uint64_t c_lo, uint64_t c_hi, hipStream_t s);
-void launch_cyclic_ntt_batch(uint64_t* a, uint64_t* b, size_t n, const uint64_t* primes,
-                             const uint64_t* stage_roots, const uint64_t* inv_roots,
-                             const uint64_t* inv_sizes, hipStream_t s);
+// out = left + c * right mod q; in = [l_lo | l_hi | r_lo | r_hi] (len words each), out = [lo | hi];
+// a non-canonical input sets *flag
+void launch_fold_soa(const uint64_t* in, uint64_t* out, size_t len, uint64_t c_lo, uint64_t c_hi, uint32_t* flag,
+                     hipStream_t s);
+// per prime of the cyclic product: p, p^-1 mod 2^32, R^2 mod p and inv_size as mulc pairs
+struct CycPrime {
+    uint32_t p, pinv, r2m, r2mp, invs_m, invs_mp, pad[2];
+};
+constexpr int CYC_MAX_LOG_N = 14;  // both operands of one prime in LDS: 2 x 2^14 words
+// cyclic products of a and b (prime-major, 8 x 2^log_n values each) with the caller's twiddle
+// tables tw (per prime: 2 (n - 1) forward then 2 (n - 1) inverse words); an input >= p sets *flag
+void launch_cyc_lds(const uint64_t* a, const uint64_t* b, uint64_t* out, int log_n, const CycPrime* cp,
+                    const uint32_t* tw, uint32_t* flag, hipStream_t s);
 // elementwise residue products: a[i] = a[i] * b[i] (mod p of slice)
 void launch_pointwise(uint32_t* a, const uint32_t* b, size_t slices, int log_m, hipStream_t s);
 // one persistent launch over a ragged batch of negacyclic products a[i] * b[i] of degree

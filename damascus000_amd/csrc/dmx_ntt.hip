@@ -1765,107 +1765,97 @@ void launch_gen_generators(const uint32_t* gseed, size_t rows, int log_m, zq* ou
 // reference-compatible batch entry points (src/utils/gpu.rs:69-93)
 // ---------------------------------------------------------------------------------------
 // out = l + c*r mod q over SoA lo/hi words (cuda/fold_kernel.cu:189-206 behaviour)
-__global__ void k_fold_pairs(const uint64_t* l_lo, const uint64_t* l_hi, const uint64_t* r_lo,
-                             const uint64_t* r_hi, uint64_t* o_lo, uint64_t* o_hi, size_t len,
-                             uint64_t c_lo, uint64_t c_hi) {
+// ---------------------------------------------------------------------------------------
+// The reference-compatible drop-ins (damascus_cuda_fold_batch / damascus_cuda_ntt_batch), on
+// buffers the host side keeps per device.
+// ---------------------------------------------------------------------------------------
+// out = left + c * right mod q over SoA 64-bit halves (fold_kernel.cu:189-206). Inputs must be
+// canonical (< q, as every Fp is); any that is not sets *flag and the call is declined.
+__global__ void __launch_bounds__(256) k_fold_soa(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                  size_t len, uint64_t c_lo, uint64_t c_hi, uint32_t* flag) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= len) return;
-    auto canon = [](uint64_t lo, uint64_t hi) {
-        uint32_t x8[8] = {(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32), 0, 0, 0, 0};
-        return zq_reduce256(x8);
-    };
-    const zq l = canon(l_lo[i], l_hi[i]), r = canon(r_lo[i], r_hi[i]), c = canon(c_lo, c_hi);
+    const uint64_t ll = in[i], lh = in[len + i], rl = in[2 * len + i], rh = in[3 * len + i];
+    const zq l{{(uint32_t)ll, (uint32_t)(ll >> 32), (uint32_t)lh, (uint32_t)(lh >> 32)}};
+    const zq r{{(uint32_t)rl, (uint32_t)(rl >> 32), (uint32_t)rh, (uint32_t)(rh >> 32)}};
+    const zq c{{(uint32_t)c_lo, (uint32_t)(c_lo >> 32), (uint32_t)c_hi, (uint32_t)(c_hi >> 32)}};
+    if (zq_ge_q(l) || zq_ge_q(r)) atomicOr(flag, 1u);
     const zq o = zq_add(l, zq_mul(r, c));
-    o_lo[i] = (uint64_t)o.w[0] | ((uint64_t)o.w[1] << 32);
-    o_hi[i] = (uint64_t)o.w[2] | ((uint64_t)o.w[3] << 32);
+    out[i] = (uint64_t)o.w[0] | ((uint64_t)o.w[1] << 32);
+    out[len + i] = (uint64_t)o.w[2] | ((uint64_t)o.w[3] << 32);
 }
 
-void launch_fold_pairs(const uint64_t* l_lo, const uint64_t* l_hi, const uint64_t* r_lo,
-                       const uint64_t* r_hi, uint64_t* o_lo, uint64_t* o_hi, size_t len,
-                       uint64_t c_lo, uint64_t c_hi, hipStream_t s) {
-    hipLaunchKernelGGL(k_fold_pairs, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, l_lo, l_hi,
-                       r_lo, r_hi, o_lo, o_hi, len, c_lo, c_hi);
+void launch_fold_soa(const uint64_t* in, uint64_t* out, size_t len, uint64_t c_lo, uint64_t c_hi, uint32_t* flag,
+                     hipStream_t s) {
+    if (!len) return;
+    hipLaunchKernelGGL(k_fold_soa, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, in, out, len, c_lo, c_hi,
+                       flag);
 }
 
-// Cyclic product with caller-supplied stage roots, the algorithm of ntt.rs:58-96 on each
-// prime: bit-reverse, radix-2 DIT stages (twiddle wlen^j), pointwise, inverse, * inv_size.
-__device__ __forceinline__ uint64_t mulmod64(uint64_t a, uint64_t b, uint64_t p) {
-    return (a * b) % p;  // a, b < p < 2^32
-}
-__device__ uint64_t powmod64(uint64_t b, uint64_t e, uint64_t p) {
-    uint64_t r = 1 % p;
-    while (e) {
-        if (e & 1) r = mulmod64(r, b, p);
-        b = mulmod64(b, b, p);
-        e >>= 1;
+// The cyclic product of damascus_cuda_ntt_batch (ntt_kernel.cu:106-293) for each of 8 primes, one
+// workgroup per prime with both operands in LDS: bit-reversed load, radix-2 DIT stages whose
+// twiddles are the powers of the caller's stage roots (stage s: root_s^j for j < 2^s, in the
+// table as Montgomery pairs at [2^s - 1 + j]), pointwise product, bit reversal, the inverse stages
+// with the caller's inverse roots, times the caller's inv_size. The same modular operations in
+// the same order as the reference, on residues in [0, p). An input >= p sets *flag (the call is
+// declined). Table per prime: forward pairs then inverse pairs, 4 (n - 1) words.
+constexpr int CYC_THREADS = 1024;
+__device__ __forceinline__ uint32_t brv_bits(uint32_t x, int bits) { return bits ? __brev(x) >> (32 - bits) : 0u; }
+
+__global__ void __launch_bounds__(CYC_THREADS) k_cyc_lds(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b,
+                                                         uint64_t* __restrict__ out, int log_n,
+                                                         const CycPrime* __restrict__ cp, const uint32_t* __restrict__ tw,
+                                                         uint32_t* flag) {
+    extern __shared__ uint32_t sh[];
+    const int q = blockIdx.x, n = 1 << log_n, half_n = n >> 1;
+    const CycPrime c = cp[q];
+    uint32_t* sa = sh;
+    uint32_t* sb = sh + n;
+    const uint64_t* aq = a + (size_t)q * n;
+    const uint64_t* bq = b + (size_t)q * n;
+    uint32_t bad = 0;
+    for (int i = threadIdx.x; i < n; i += CYC_THREADS) {
+        const uint64_t x = aq[i], y = bq[i];
+        bad |= (x >= c.p || y >= c.p) ? 1u : 0u;
+        const uint32_t r = brv_bits((uint32_t)i, log_n);
+        sa[r] = (uint32_t)x;
+        sb[r] = (uint32_t)y;
     }
-    return r;
-}
-
-__global__ void k_cyc_prep(uint64_t* a, size_t n, int log_n, const uint64_t* primes) {
-    // reduce inputs and bit-reverse every prime slice: grid (n/256, NPRIMES)
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const int pi = blockIdx.y;
-    if (i >= n) return;
-    uint64_t* s = a + (size_t)pi * n;
-    const uint64_t p = primes[pi];
-    const size_t r = log_n ? (size_t)(__brevll((unsigned long long)i) >> (64 - log_n)) : 0;
-    if (i < r) {
-        const uint64_t x = s[i] % p, y = s[r] % p;
-        s[i] = y;
-        s[r] = x;
-    } else if (i == r) {
-        s[i] = s[i] % p;
-    }
-}
-
-__global__ void k_cyc_stage(uint64_t* a, size_t n, int stage, const uint64_t* primes,
-                            const uint64_t* roots, int log_n) {
-    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const int pi = blockIdx.y;
-    if (t >= n / 2) return;
-    const uint64_t p = primes[pi];
-    const size_t len = (size_t)2 << stage, half = len >> 1;
-    const size_t grp = t / half, off = t % half;
-    const size_t lft = grp * len + off, rgt = lft + half;
-    uint64_t* s = a + (size_t)pi * n;
-    const uint64_t w = powmod64(roots[(size_t)pi * log_n + stage] % p, off, p);
-    const uint64_t tv = mulmod64(s[rgt], w, p), u = s[lft];
-    s[lft] = (u + tv) % p;
-    s[rgt] = (u + p - tv) % p;
-}
-
-__global__ void k_cyc_pointwise(uint64_t* a, const uint64_t* b, size_t n, const uint64_t* primes) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const int pi = blockIdx.y;
-    if (i >= n) return;
-    a[(size_t)pi * n + i] = mulmod64(a[(size_t)pi * n + i], b[(size_t)pi * n + i], primes[pi]);
-}
-
-__global__ void k_cyc_scale(uint64_t* a, size_t n, const uint64_t* primes, const uint64_t* inv_sizes) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const int pi = blockIdx.y;
-    if (i >= n) return;
-    a[(size_t)pi * n + i] = mulmod64(a[(size_t)pi * n + i], inv_sizes[pi] % primes[pi], primes[pi]);
-}
-
-void launch_cyclic_ntt_batch(uint64_t* a, uint64_t* b, size_t n, const uint64_t* primes,
-                             const uint64_t* stage_roots, const uint64_t* inv_roots,
-                             const uint64_t* inv_sizes, hipStream_t s) {
-    int log_n = 0;
-    while (((size_t)1 << log_n) < n) ++log_n;
-    const dim3 gv((unsigned)((n + 255) / 256), NPRIMES), gb((unsigned)((n / 2 + 255) / 256), NPRIMES);
-    hipLaunchKernelGGL(k_cyc_prep, gv, dim3(256), 0, s, a, n, log_n, primes);
-    hipLaunchKernelGGL(k_cyc_prep, gv, dim3(256), 0, s, b, n, log_n, primes);
+    if (bad) atomicOr(flag, 1u);
+    __syncthreads();
+    const uint32_t* fw = tw + (size_t)q * 4 * (n - 1);
+    const uint32_t* iw = fw + 2 * (n - 1);
     for (int st = 0; st < log_n; ++st) {
-        hipLaunchKernelGGL(k_cyc_stage, gb, dim3(256), 0, s, a, n, st, primes, stage_roots, log_n);
-        hipLaunchKernelGGL(k_cyc_stage, gb, dim3(256), 0, s, b, n, st, primes, stage_roots, log_n);
+        const int half = 1 << st;
+        for (int t = threadIdx.x; t < half_n; t += CYC_THREADS) {
+            const int off = t & (half - 1), left = ((t >> st) << (st + 1)) + off;
+            const uint32_t w = fw[2 * (half - 1 + off)], wp = fw[2 * (half - 1 + off) + 1];
+            ct_butterfly(sa[left], sa[left + half], w, wp, c.p);
+            ct_butterfly(sb[left], sb[left + half], w, wp, c.p);
+        }
+        __syncthreads();
     }
-    hipLaunchKernelGGL(k_cyc_pointwise, gv, dim3(256), 0, s, a, b, n, primes);
-    hipLaunchKernelGGL(k_cyc_prep, gv, dim3(256), 0, s, a, n, log_n, primes);
-    for (int st = 0; st < log_n; ++st)
-        hipLaunchKernelGGL(k_cyc_stage, gb, dim3(256), 0, s, a, n, st, primes, inv_roots, log_n);
-    hipLaunchKernelGGL(k_cyc_scale, gv, dim3(256), 0, s, a, n, primes, inv_sizes);
+    for (int i = threadIdx.x; i < n; i += CYC_THREADS)  // a * b: (a * b / R) * R, in place in sa
+        sa[i] = mulc(redc((uint64_t)sa[i] * sb[i], c.p, c.pinv), c.r2m, c.r2mp, c.p);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += CYC_THREADS) sb[brv_bits((uint32_t)i, log_n)] = sa[i];  // bit reversal
+    __syncthreads();
+    for (int st = 0; st < log_n; ++st) {
+        const int half = 1 << st;
+        for (int t = threadIdx.x; t < half_n; t += CYC_THREADS) {
+            const int off = t & (half - 1), left = ((t >> st) << (st + 1)) + off;
+            ct_butterfly(sb[left], sb[left + half], iw[2 * (half - 1 + off)], iw[2 * (half - 1 + off) + 1], c.p);
+        }
+        __syncthreads();
+    }
+    uint64_t* oq = out + (size_t)q * n;
+    for (int i = threadIdx.x; i < n; i += CYC_THREADS) oq[i] = mulc(sb[i], c.invs_m, c.invs_mp, c.p);
+}
+
+void launch_cyc_lds(const uint64_t* a, const uint64_t* b, uint64_t* out, int log_n, const CycPrime* cp,
+                    const uint32_t* tw, uint32_t* flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_cyc_lds, dim3(NPRIMES), dim3(CYC_THREADS), (size_t)2 * ((size_t)1 << log_n) * 4, s, a, b, out,
+                       log_n, cp, tw, flag);
 }
 
 // ---------------------------------------------------------------------------------------

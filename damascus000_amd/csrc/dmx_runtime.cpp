@@ -2465,51 +2465,236 @@ extern "C" int dmx_prove_verify_many(dmx_ctx* ctx, dmx_job* jobs, size_t count, 
 // ------------------------------------------------------------------------------------------
 // reference-compatible batch entry points (return codes follow cuda/*.cu: 0 ok, 1 args, 2 device)
 // ------------------------------------------------------------------------------------------
+// The reference's entry points take no context: their state lives per device, created on the
+// first call with that device current. Device buffers and pinned staging grow and are kept, the
+// twiddle tables of the last few plans (n, primes, roots) are kept, and every call is one upload,
+// one launch and one readback on the state's own stream. Calls are serialised per device.
+struct DropIn {
+    std::mutex mu;
+    hipStream_t s = nullptr;
+    hipEvent_t done = nullptr;
+    DevBuf dev;
+    void* pin = nullptr;
+    size_t pin_bytes = 0;
+    struct Plan {
+        std::vector<uint64_t> key;  // n, primes, stage roots, inverse roots, inverse sizes
+        DevBuf tables;              // CycPrime[8] then the twiddle pairs
+        uint64_t last_use = 0;
+    };
+    std::vector<std::unique_ptr<Plan>> plans;
+    uint64_t clock = 0;
+    ~DropIn() {
+        if (pin) (void)hipHostFree(pin);
+        if (done) (void)hipEventDestroy(done);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    int stage(size_t bytes) {  // pinned staging of at least `bytes`
+        if (bytes <= pin_bytes) return DMX_OK;
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
+        pin_bytes = 0;
+        if (hipHostMalloc(&pin, bytes, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            pin = nullptr;
+            return DMX_ERR_OOM;
+        }
+        pin_bytes = bytes;
+        return DMX_OK;
+    }
+    int wait() {
+        if (hipEventRecord(done, s) != hipSuccess) return DMX_ERR_HIP;
+        for (;;) {
+            const hipError_t e = hipEventQuery(done);
+            if (e == hipSuccess) return DMX_OK;
+            if (e != hipErrorNotReady) return DMX_ERR_HIP;
+            std::this_thread::yield();
+        }
+    }
+};
+static std::mutex g_dropin_mu;
+static std::unordered_map<int, std::unique_ptr<DropIn>> g_dropin;
+
+static DropIn* dropin_state() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> g(g_dropin_mu);
+    auto& slot = g_dropin[dev];
+    if (!slot) {
+        std::unique_ptr<DropIn> d(new DropIn);
+        if (hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&d->done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        slot = std::move(d);
+    }
+    return slot.get();
+}
+
+// damascus_cuda_fold_batch (cuda/fold_kernel.cu:209-334, src/utils/gpu.rs:95-138): out = left +
+// challenge * right mod q over SoA lo/hi words. Returns 0; 1 for bad arguments, including values
+// that are not canonical elements of Z_q (every Fp is; the call is then declined and the
+// caller's CPU path runs); 2 for a device failure.
 extern "C" int dmx_fold_batch(const uint64_t* l_lo, const uint64_t* l_hi, const uint64_t* r_lo, const uint64_t* r_hi,
                               uint64_t* o_lo, uint64_t* o_hi, size_t len, uint64_t c_lo, uint64_t c_hi) {
     if (!l_lo || !l_hi || !r_lo || !r_hi || !o_lo || !o_hi || len == 0) return 1;
+    if (c_hi > 0xFFFFFFFFFFFFull || (c_hi == 0xFFFFFFFFFFFFull && c_lo >= 0xFFFFFFFFFFFFFFB5ull)) return 1;
     if (!dmx_backend_ready()) return 2;
-    DevBuf b;
-    if (b.alloc(len * 8 * 6)) return 2;
-    uint64_t* d = b.as<uint64_t>();
+    DropIn* D = dropin_state();
+    if (!D) return 2;
+    std::lock_guard<std::mutex> g(D->mu);
     const size_t by = len * 8;
-    if (hipMemcpy(d, l_lo, by, hipMemcpyHostToDevice) || hipMemcpy(d + len, l_hi, by, hipMemcpyHostToDevice) ||
-        hipMemcpy(d + 2 * len, r_lo, by, hipMemcpyHostToDevice) ||
-        hipMemcpy(d + 3 * len, r_hi, by, hipMemcpyHostToDevice))
-        return 2;
-    launch_fold_pairs(d, d + len, d + 2 * len, d + 3 * len, d + 4 * len, d + 5 * len, len, c_lo, c_hi, 0);
+    if (D->stage(6 * by + 256)) return 2;
+    if (D->dev.bytes < 6 * by + 256 && D->dev.alloc(6 * by + 256)) return 2;
+    uint8_t* h = static_cast<uint8_t*>(D->pin);
+    memcpy(h, l_lo, by);
+    memcpy(h + by, l_hi, by);
+    memcpy(h + 2 * by, r_lo, by);
+    memcpy(h + 3 * by, r_hi, by);
+    uint64_t* d = D->dev.as<uint64_t>();
+    uint32_t* dflag = reinterpret_cast<uint32_t*>(d + 6 * len);
+    if (hipMemcpyAsync(d, h, 4 * by, hipMemcpyHostToDevice, D->s) != hipSuccess) return 2;
+    launch_fill32(dflag, 1, 0, D->s);
+    launch_fold_soa(d, d + 4 * len, len, c_lo, c_hi, dflag, D->s);
     if (hipGetLastError() != hipSuccess) return 2;
-    if (hipMemcpy(o_lo, d + 4 * len, by, hipMemcpyDeviceToHost) || hipMemcpy(o_hi, d + 5 * len, by, hipMemcpyDeviceToHost))
-        return 2;
+    if (hipMemcpyAsync(h + 4 * by, d + 4 * len, 2 * by + 4, hipMemcpyDeviceToHost, D->s) != hipSuccess) return 2;
+    if (D->wait()) return 2;
+    uint32_t flag = 0;
+    memcpy(&flag, h + 6 * by, 4);
+    if (flag) return 1;
+    memcpy(o_lo, h + 4 * by, by);
+    memcpy(o_hi, h + 5 * by, by);
     return 0;
 }
 
+static uint64_t mulmod_u64(uint64_t a, uint64_t b, uint64_t p) { return (uint64_t)((unsigned __int128)a * b % p); }
+static uint64_t powmod_u64(uint64_t b, uint64_t e, uint64_t p) {
+    uint64_t r = 1 % p;
+    for (; e; e >>= 1, b = mulmod_u64(b, b, p))
+        if (e & 1) r = mulmod_u64(r, b, p);
+    return r;
+}
+
+// The device tables of a plan: CycPrime per prime, then per prime the Montgomery pairs of
+// root_s^j (j < 2^s) for every forward then every inverse stage. The roots are checked to give
+// the reference's stage spans: each stage-s root (forward and inverse) has order exactly
+// 2^(s+1), i.e. root^(2^s) = p - 1 (stage_len_from_root, ntt_kernel.cu:33-41); roots and
+// inverse sizes are below p. DMX_ERR_ARG when they are not.
+static int dropin_plan(DropIn* D, size_t n, int log_n, const uint64_t* primes, const uint64_t* roots,
+                       const uint64_t* iroots, const uint64_t* inv_sizes, const DropIn::Plan** out) {
+    std::vector<uint64_t> key;
+    key.reserve(1 + 2 * NPRIMES + 2 * NPRIMES * log_n);
+    key.push_back(n);
+    key.insert(key.end(), primes, primes + NPRIMES);
+    key.insert(key.end(), roots, roots + (size_t)NPRIMES * log_n);
+    key.insert(key.end(), iroots, iroots + (size_t)NPRIMES * log_n);
+    key.insert(key.end(), inv_sizes, inv_sizes + NPRIMES);
+    for (auto& pl : D->plans)
+        if (pl->key == key) {
+            pl->last_use = ++D->clock;
+            *out = pl.get();
+            return DMX_OK;
+        }
+    const size_t per = 4 * (n - 1);
+    std::vector<uint32_t> host(sizeof(CycPrime) / 4 * NPRIMES + per * NPRIMES);
+    CycPrime* cps = reinterpret_cast<CycPrime*>(host.data());
+    uint32_t* tw = host.data() + sizeof(CycPrime) / 4 * NPRIMES;
+    for (int q = 0; q < NPRIMES; ++q) {
+        const uint64_t p = primes[q];
+        if (inv_sizes[q] >= p) return DMX_ERR_ARG;
+        const uint32_t pinv = host_inv32((uint32_t)p);
+        auto pair = [&](uint64_t w, uint32_t* dst) {  // (w R mod p, that * p^-1 mod 2^32)
+            const uint32_t wm = (uint32_t)(((unsigned __int128)w << 32) % p);
+            dst[0] = wm;
+            dst[1] = wm * pinv;
+        };
+        CycPrime& c = cps[q];
+        memset(&c, 0, sizeof c);
+        c.p = (uint32_t)p;
+        c.pinv = pinv;
+        // the pointwise product is redc(a b) = a b / R, then times R: mulc(t, wm, wmp) = t w for
+        // wm = w R mod p, so w = R takes wm = R^2 mod p
+        const uint32_t r2 = (uint32_t)(((unsigned __int128)1 << 64) % p);
+        c.r2m = r2;
+        c.r2mp = r2 * pinv;
+        uint32_t inv_pair[2];
+        pair(inv_sizes[q], inv_pair);
+        c.invs_m = inv_pair[0];
+        c.invs_mp = inv_pair[1];
+        for (int dir = 0; dir < 2; ++dir) {
+            const uint64_t* rs = (dir ? iroots : roots) + (size_t)q * log_n;
+            uint32_t* t = tw + (size_t)q * per + (size_t)dir * 2 * (n - 1);
+            for (int st = 0; st < log_n; ++st) {
+                const uint64_t r = rs[st];
+                if (r >= p || powmod_u64(r, (uint64_t)1 << st, p) != p - 1) return DMX_ERR_ARG;
+                uint64_t w = 1;
+                for (size_t j = 0; j < ((size_t)1 << st); ++j) {
+                    pair(w, t + 2 * (((size_t)1 << st) - 1 + j));
+                    w = mulmod_u64(w, r, p);
+                }
+            }
+        }
+    }
+    if (D->plans.size() >= 4) {  // keep the most recently used plans
+        size_t victim = 0;
+        for (size_t i = 1; i < D->plans.size(); ++i)
+            if (D->plans[i]->last_use < D->plans[victim]->last_use) victim = i;
+        D->plans.erase(D->plans.begin() + victim);
+    }
+    std::unique_ptr<DropIn::Plan> pl(new DropIn::Plan);
+    pl->key = std::move(key);
+    if (pl->tables.alloc(host.size() * 4)) return DMX_ERR_OOM;
+    if (hipMemcpy(pl->tables.p, host.data(), host.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return DMX_ERR_HIP;
+    pl->last_use = ++D->clock;
+    *out = pl.get();
+    D->plans.push_back(std::move(pl));
+    return DMX_OK;
+}
+
+// damascus_cuda_ntt_batch (cuda/ntt_kernel.cu:106-293, src/utils/gpu.rs:140-207): for each of 8
+// primes, the length-n cyclic product of host_a and host_b (prime-major, 8 n words) computed as
+// the reference does (bit reversal, radix-2 DIT with the caller's stage roots, pointwise product,
+// the inverse with the caller's inverse roots, times inv_sizes). Returns 0; 1 for bad arguments,
+// which include a plan the reference would not run as radix-2 stages (a stage root whose order is
+// not 2^(s+1), a root or inverse size >= p) and inputs that are not residues below p: the call is
+// declined and the caller's CPU path runs, as for any non-zero code (gpu.rs:191-193); n above
+// 2^14 is declined the same way; 2 for a device failure.
 extern "C" int dmx_ntt_batch(const uint64_t* a, const uint64_t* b, uint64_t* out, size_t n, const uint64_t* primes,
                              const uint64_t* stage_roots, const uint64_t* inv_roots, const uint64_t* inv_sizes) {
     if (!a || !b || !out || !primes || !stage_roots || !inv_roots || !inv_sizes || n == 0 || (n & (n - 1))) return 1;
     for (int i = 0; i < NPRIMES; ++i)
-        if (primes[i] < 2 || primes[i] >= (1ull << 32)) return 1;
-    if (!dmx_backend_ready()) return 2;
+        if (primes[i] < 3 || primes[i] >= (1ull << 32) || !(primes[i] & 1)) return 1;
     int log_n = 0;
     while (((size_t)1 << log_n) < n) ++log_n;
-    const size_t words = (size_t)NPRIMES * n, roots = (size_t)NPRIMES * std::max(log_n, 1);
-    DevBuf buf;
-    if (buf.alloc((2 * words + 2 * roots + 2 * NPRIMES) * 8)) return 2;
-    uint64_t* da = buf.as<uint64_t>();
-    uint64_t* db = da + words;
-    uint64_t* dr = db + words;
-    uint64_t* dir = dr + roots;
-    uint64_t* dp = dir + roots;
-    uint64_t* dinv = dp + NPRIMES;
-    if (hipMemcpy(da, a, words * 8, hipMemcpyHostToDevice) || hipMemcpy(db, b, words * 8, hipMemcpyHostToDevice) ||
-        (log_n && hipMemcpy(dr, stage_roots, (size_t)NPRIMES * log_n * 8, hipMemcpyHostToDevice)) ||
-        (log_n && hipMemcpy(dir, inv_roots, (size_t)NPRIMES * log_n * 8, hipMemcpyHostToDevice)) ||
-        hipMemcpy(dp, primes, NPRIMES * 8, hipMemcpyHostToDevice) ||
-        hipMemcpy(dinv, inv_sizes, NPRIMES * 8, hipMemcpyHostToDevice))
-        return 2;
-    launch_cyclic_ntt_batch(da, db, n, dp, dr, dir, dinv, 0);
+    if (log_n > CYC_MAX_LOG_N) return 1;
+    if (!dmx_backend_ready()) return 2;
+    DropIn* D = dropin_state();
+    if (!D) return 2;
+    std::lock_guard<std::mutex> g(D->mu);
+    const DropIn::Plan* plan = nullptr;
+    const int prc = dropin_plan(D, n, log_n, primes, stage_roots, inv_roots, inv_sizes, &plan);
+    if (prc == DMX_ERR_ARG) return 1;
+    if (prc) return 2;
+    const size_t words = (size_t)NPRIMES * n, by = words * 8;
+    if (D->stage(3 * by + 256)) return 2;
+    if (D->dev.bytes < 3 * by + 256 && D->dev.alloc(3 * by + 256)) return 2;
+    uint8_t* h = static_cast<uint8_t*>(D->pin);
+    memcpy(h, a, by);
+    memcpy(h + by, b, by);
+    uint64_t* d = D->dev.as<uint64_t>();
+    uint32_t* dflag = reinterpret_cast<uint32_t*>(d + 3 * words);
+    if (hipMemcpyAsync(d, h, 2 * by, hipMemcpyHostToDevice, D->s) != hipSuccess) return 2;
+    launch_fill32(dflag, 1, 0, D->s);
+    const CycPrime* cp = plan->tables.as<CycPrime>();
+    const uint32_t* tw = plan->tables.as<uint32_t>() + sizeof(CycPrime) / 4 * NPRIMES;
+    launch_cyc_lds(d, d + words, d + 2 * words, log_n, cp, tw, dflag, D->s);
     if (hipGetLastError() != hipSuccess) return 2;
-    if (hipMemcpy(out, da, words * 8, hipMemcpyDeviceToHost)) return 2;
+    if (hipMemcpyAsync(h + 2 * by, d + 2 * words, by + 4, hipMemcpyDeviceToHost, D->s) != hipSuccess) return 2;
+    if (D->wait()) return 2;
+    uint32_t flag = 0;
+    memcpy(&flag, h + 3 * by, 4);
+    if (flag) return 1;
+    memcpy(out, h + 2 * by, by);
     return 0;
 }
 

@@ -77,6 +77,54 @@ def test_ntt_batch_matches_reference_cyclic_product():
             assert np.array_equal(out[i], want), (S, p)
 
 
+def test_ntt_batch_large_plans_and_repeat_calls():
+    """n up to 2^14 (ring length 8192), each plan called twice (its twiddle tables are kept), a
+    second plan between the calls: every product equals the oracle's cyclic NTT product"""
+    rng = np.random.default_rng(6)
+    for S in [4096, 16384, 4096]:
+        a = np.stack([rng.integers(0, p, size=S, dtype=np.uint64) for p in O.PRIMES])
+        b = np.stack([rng.integers(0, p, size=S, dtype=np.uint64) for p in O.PRIMES])
+        roots, iroots, inv_sizes = _plans(S)
+        say("ntt_batch S", S)
+        rc, out = dm.ntt_batch(a, b, S, O.PRIMES, roots, iroots, inv_sizes)
+        assert rc == 0
+        for i, p in enumerate(O.PRIMES[:3]):
+            fa = O.ntt_forward(a[i], p)
+            fb = O.ntt_forward(b[i], p)
+            want = O.ntt_inverse((fa.astype(object) * fb.astype(object) % p).astype(np.uint64), p)
+            assert np.array_equal(out[i], want), (S, p)
+        rc2, out2 = dm.ntt_batch(a, b, S, O.PRIMES, roots, iroots, inv_sizes)
+        assert rc2 == 0 and np.array_equal(out2, out)
+
+
+def test_drop_ins_decline_what_the_reference_would_not_compute_the_same_way():
+    """Return code 1 (the caller's CPU path runs, gpu.rs:191-193) for a stage root whose order is
+    not 2^(s+1) (the reference's stage span would differ), a root or input not below p, a size past
+    2^14, and fold values that are not canonical elements of Z_q"""
+    S = 64
+    a = np.zeros((8, S), dtype=np.uint64)
+    b = np.zeros((8, S), dtype=np.uint64)
+    roots, iroots, inv_sizes = _plans(S)
+    bad = list(roots)
+    bad[1] = 1  # stage 1 of prime 0 with a root of order 1
+    assert dm.ntt_batch(a, b, S, O.PRIMES, bad, iroots, inv_sizes)[0] == 1
+    big = list(iroots)
+    big[2] += O.PRIMES[0]  # a root not below p
+    assert dm.ntt_batch(a, b, S, O.PRIMES, roots, big, inv_sizes)[0] == 1
+    a2 = a.copy()
+    a2[3, 5] = O.PRIMES[3]  # an input that is not a residue below p
+    assert dm.ntt_batch(a2, b, S, O.PRIMES, roots, iroots, inv_sizes)[0] == 1
+    assert dm.ntt_batch(a, b, S, O.PRIMES, roots, iroots, inv_sizes)[0] == 0  # the same plan still runs
+    n = 1 << 15
+    r15, i15, s15 = _plans(n)
+    z = np.zeros((8, n), dtype=np.uint64)
+    assert dm.ntt_batch(z, z, n, O.PRIMES, r15, i15, s15)[0] == 1
+    assert dm.fold_batch([O.Q], [1], 3)[0] == 1
+    assert dm.fold_batch([1], [2], O.Q)[0] == 1
+    rc, out = dm.fold_batch([O.Q - 1, 0], [O.Q - 1, 5], O.Q - 1)
+    assert rc == 0 and out == [(O.Q - 1 + (O.Q - 1) ** 2) % O.Q, (5 * (O.Q - 1)) % O.Q]
+
+
 # ---------------- ring products ----------------
 def test_ring_products_match_golden_schoolbook():
     import json
