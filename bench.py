@@ -3,26 +3,32 @@
 
 Headline (`value`): a served trace of open-loop Poisson arrivals of commit+fold requests, file
 sizes log-uniform 64 KiB..16 MiB (d = 7..11, up to 4,194,304 coefficients), seeded uniform
-bytes, K = 8, default SystemParams, files resident in HBM before the clock starts. Each request
-is the whole job: file_id, witness expansion, the initial commitment, every fold round with both
-invariant commitments and every Fiat-Shamir challenge, the verifier's replay (its folds and its
-own challenges) and the final opening check, then the proof bytes copied to the host. The
-served path is the continuous-batching server (dmx_server_*), which keeps the generator matrix
-of each (seed, depth) resident in HBM (derived once, in the untimed warm-up pass); the plain
-path runs the same trace one request at a time through dmx_prove_verify_device, which expands
-the generators on every call (once: the prover's first transform also fills the verifier's
-matrix). `value` is MiB of request files per
-second from the first arrival to the last verified result; p50/p99 are arrival-to-verified-
-result latencies. Every served proof is compared with the plain path's.
+bytes, K = 8, default SystemParams, files resident in HBM before the clock starts. A step is one
+request: the whole job (file_id, witness expansion, the initial commitment, every fold round with
+both invariant commitments and every Fiat-Shamir challenge, the verifier's replay and the final
+opening check, then the proof bytes copied to the host). `--steps` requests are timed in one
+served pass after `--warmup` requests of the same trace in an untimed pass (workspaces, captured
+graphs, the resident generator matrix of the trace's seed). The served path is the
+continuous-batching server (dmx_server_*); the plain path runs the same trace one request at a
+time through dmx_prove_verify_device, which expands the generators on every call. `value` is MiB
+of request files per second from the first arrival to the last verified result; p50/p99 are
+arrival-to-verified-result latencies. Every served proof is compared with the plain path's.
 
 Also reported on the same JSON line:
-  single_file   one 8 MiB file (645,278 coefficients, d = 10), the plain path, graph replays
+  served        the headline pass, the plain path on the same trace, the same trace from host
+                memory, a sweep of arrival rates, and a trace whose requests each carry their own
+                generator seed (so no resident generator matrix applies)
+  single_file   one 8 MiB file (645,278 coefficients, d = 10), the plain path, graph replays, and a
+                16 MiB file (d = 11)
   batched_files 256 files log-uniform 4 KiB..4 MiB (d = 6..10), each with its own generator seed,
                 one ragged batch (dmx_prove_verify_many) against a loop of the plain path
   ragged_products 4,096 negacyclic products, n uniform in 2^6..2^12, device-resident: one
                 persistent launch against per-degree launch sequences
+  drop_in       per-call latency of the reference-compatible dmx_ntt_batch / dmx_fold_batch
   roofline      the degree-1024 forward transform (the dominant kernel), from HIP events around
-                each of its launches in prover runs on the 8 MiB file
+                each of its launches in prover runs on the 8 MiB file; algorithmic work at the
+                fixed weights and at full-rate 32-bit multiplies; counters from the committed
+                rocprofv3 runs
   cpu_baseline  the oracle (C restatement of the reference CPU algorithm) on a stratified sample of
                 the served trace (one file per depth, weighted by the trace's bytes per depth), or on
                 the 8 MiB file with --no-mixed
@@ -40,9 +46,61 @@ sys.path.insert(0, ROOT)
 FILE_BYTES = 8 << 20
 SEED = 2
 PEAK_LANE_CYCLES = 256 * 64 * 2.4e9  # CUs x lanes x clock: 39.32 T lane-cycles/s
-WORK_WEIGHTS = "butterfly 19, mod-p reduction 42, Z_q mul-add 204, BLAKE3 compression 860 lane-cycles"
 DEFAULT_REQUESTS = 400
 DEFAULT_RATE = 2000.0  # requests per second (about 5.6 GiB/s offered)
+SWEEP_RATES = (600.0, 1000.0, 3000.0, 4000.0)  # besides the headline rate; 600/s is near the plain path's capacity
+K = 8
+
+# ---- algorithmic work of a prover (fused negacyclic CRT-NTT formulation, 8 primes, K = 8) ----
+# Op counts summed over the 6d + 1 commitments of a depth-d proof (the initial one with N rows at
+# degree N, then per round j with m = N >> j: three with m/2 rows at degree m and three with m/2
+# rows at degree m/2): B butterflies, P pointwise MACs, R mod-p reductions, C CRT lifts, Z Z_q
+# multiply-adds of the folds, H BLAKE3 compressions (generators and transcript).
+FIXED_WEIGHTS = {"B": 19, "P": 16, "R": 42, "C": 768, "Z": 204, "H": 860}
+# the 32-bit multiplies inside each fixed weight (which prices one at 4 lane-cycles)
+MULS_PER_OP = {"B": 3, "P": 2, "R": 9, "C": 84, "Z": 32, "H": 0}
+# 32-bit multiplies at the full issue rate (as the dominant kernel's counters show on gfx950)
+FULL_RATE_WEIGHTS = {k: FIXED_WEIGHTS[k] - 3 * MULS_PER_OP[k] for k in FIXED_WEIGHTS}
+
+
+def b3_compressions(nbytes):
+    """BLAKE3 compressions of an nbytes message: one per 64-byte block of each 1 KiB chunk (at
+    least one), plus one per parent node"""
+    chunks = max(1, -(-nbytes // 1024))
+    blocks = sum(max(1, -(-min(1024, nbytes - 1024 * c) // 64)) for c in range(chunks)) if nbytes else 1
+    return blocks + chunks - 1
+
+
+def transcript_len(label_len, lens):
+    return 96 + label_len + 4 + sum(8 + K * (8 + 16 * n) for n in lens)
+
+
+def prover_ops(d):
+    ops = {k: 0 for k in FIXED_WEIGHTS}
+    N = 1 << d
+
+    def commit(t, m):
+        lg = m.bit_length() - 1
+        ops["B"] += 8 * (t * (1 + K) + K) * (m // 2) * lg
+        ops["P"] += 8 * t * K * m
+        ops["R"] += 8 * t * (1 + K) * m
+        ops["C"] += K * m
+
+    commit(N, N)
+    ops["H"] = K * 4 ** d
+    for j in range(d):
+        m = N >> j
+        for _ in range(3):
+            commit(m // 2, m)
+        for _ in range(3):
+            commit(m // 2, m // 2)
+        ops["Z"] += (1 + K) * (m * m // 2 + m * m // 4)
+        ops["H"] += b3_compressions(transcript_len(3, [m, m, m])) + b3_compressions(transcript_len(4, [m, m // 2, m // 2]))
+    return ops
+
+
+def work(ops, weights):
+    return sum(ops[k] * weights[k] for k in ops)
 
 
 def rand_bytes(rng, n):
@@ -58,7 +116,8 @@ def single_file():
 
 
 def served_trace(requests, rate, seed=5):
-    """(arrival_seconds, file_bytes) with Poisson arrivals and log-uniform 64 KiB..16 MiB sizes"""
+    """(arrival_seconds, file_bytes) with Poisson arrivals and log-uniform 64 KiB..16 MiB sizes;
+    the sizes and bytes do not depend on the rate (it only scales the gaps)"""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -66,6 +125,11 @@ def served_trace(requests, rate, seed=5):
     arrivals = np.cumsum(gaps) - gaps[0]
     sizes = np.floor(65536 * 256.0 ** rng.random(requests)).astype(np.int64)
     return [(float(a), rand_bytes(rng, n)) for a, n in zip(arrivals, sizes)]
+
+
+def own_seed_params(dm, n, tag=b"request"):
+    """per-request SystemParams: generator seeds derived from a tag and the request index"""
+    return [dm.SystemParams(dm.derive_subseed(bytes([7] * 32), tag + b"-%d" % i), bytes([11] * 32)) for i in range(n)]
 
 
 def batch_files(count=256, seed=4):
@@ -90,7 +154,7 @@ def pct(xs, q):
 INFLIGHT = 4
 
 
-def run_served(dm, ctx, dev, arrivals, files_on_host=False):
+def run_served(dm, ctx, dev, arrivals, files_on_host=False, params=None):
     srv = dm.Server(ctx, max_batch=64, max_inflight=INFLIGHT, files_on_host=files_on_host)
     n = len(dev)
     tickets = [None] * n
@@ -103,7 +167,7 @@ def run_served(dm, ctx, dev, arrivals, files_on_host=False):
             if now < a:
                 time.sleep(a - now)
             t_sub[i] = time.perf_counter() - t0
-            tickets[i] = srv.submit(d)
+            tickets[i] = srv.submit(d, params[i] if params else None)
 
     th = threading.Thread(target=submitter)
     th.start()
@@ -123,10 +187,10 @@ def run_served(dm, ctx, dev, arrivals, files_on_host=False):
 
 
 LAST_STATS = None
-LAST_TRACE = None  # (files, served proofs) of the last mixed_served run, for the CPU baseline
+LAST_TRACE = None  # (files, served proofs) of the headline pass, for the CPU baseline
 
 
-def run_plain(dm, dev, arrivals):
+def run_plain(dm, dev, arrivals, params=None):
     n = len(dev)
     results, lat = [None] * n, [0.0] * n
     t0 = time.perf_counter()
@@ -134,42 +198,77 @@ def run_plain(dm, dev, arrivals):
         now = time.perf_counter() - t0
         if now < a:
             time.sleep(a - now)
-        results[i] = dm.prove_verify_device(p, ln)
+        results[i] = dm.prove_verify_device(p, ln, params[i] if params else None)
         lat[i] = time.perf_counter() - t0 - a
     makespan = max(a + l for a, l in zip(arrivals, lat)) - arrivals[0]
     return results, lat, makespan
 
 
-def mixed_served(dm, ctx, requests, rate):
+def summary(mib, lat, span, arrivals):
+    return {"mib_s": round(mib / span, 1), "p50_ms": round(pct(lat, 0.5) * 1e3, 2),
+            "p99_ms": round(pct(lat, 0.99) * 1e3, 2),
+            "offered_mib_s": round(mib / max(arrivals[-1] - arrivals[0], 1e-9), 1)}
+
+
+def mixed_served(dm, ctx, requests, rate, warmup):
+    """the headline pass (timed) and the comparisons on the same trace"""
     trace = served_trace(requests, rate)
     arrivals = [a for a, _ in trace]
     bufs = [ctx.upload(f) for _, f in trace]
     dev = [(b.ptr, len(f)) for b, (_, f) in zip(bufs, trace)]
-    total = sum(len(f) for _, f in trace) / (1 << 20)
-    run_served(dm, ctx, dev, arrivals)  # warm: workspaces and captured graphs per (depth, batch)
-    served, s_lat, s_span = run_served(dm, ctx, dev, arrivals)
-    # the same trace handed over in host memory (PCIe-inclusive: pinned staging + async upload)
-    host = [f for _, f in trace]
-    run_served(dm, ctx, host, arrivals, files_on_host=True)
-    h_res, h_lat, h_span = run_served(dm, ctx, host, arrivals, files_on_host=True)
-    run_plain(dm, dev[: min(len(dev), 32)], arrivals[: min(len(dev), 32)])  # warm
+    mib = sum(len(f) for _, f in trace) / (1 << 20)
+    w = max(1, min(warmup, requests))
+    run_served(dm, ctx, dev[:w], arrivals[:w])  # untimed: workspaces, graphs, resident generators
+    served, s_lat, s_span = run_served(dm, ctx, dev, arrivals)  # the timed region
+    head = summary(mib, s_lat, s_span, arrivals)
+    head.update({"requests": requests, "rate_per_s": rate, "makespan_s": round(s_span, 4),
+                 "mib": round(mib, 2), "server": LAST_STATS})
+    out = {"headline": head}
+    # the plain path on the same trace
+    run_plain(dm, dev[: min(len(dev), 32)], arrivals[: min(len(dev), 32)])
     plain, p_lat, p_span = run_plain(dm, dev, arrivals)
-    if h_res != served:
-        raise SystemExit("host-file and device-file served proofs differ")
-    if any(not isinstance(r, tuple) for r in served):
-        raise SystemExit("a served request failed")
+    out["plain"] = summary(mib, p_lat, p_span, arrivals)
     if served != plain:
         raise SystemExit("served and plain proofs differ")
+    if any(not isinstance(r, tuple) for r in served):
+        raise SystemExit("a served request failed")
+    out["speedup_vs_plain"] = round(out["plain"]["mib_s"] and head["mib_s"] / out["plain"]["mib_s"], 2)
+    # the same trace handed over in host memory (PCIe-inclusive: pinned staging + async upload)
+    host = [f for _, f in trace]
+    run_served(dm, ctx, host[:w], arrivals[:w], files_on_host=True)
+    h_res, h_lat, h_span = run_served(dm, ctx, host, arrivals, files_on_host=True)
+    if h_res != served:
+        raise SystemExit("host-file and device-file served proofs differ")
+    out["host_files"] = summary(mib, h_lat, h_span, arrivals)
+    # arrival-rate sweep over the same files (the gaps scale with the rate)
+    sweep = {}
+    for r in SWEEP_RATES:
+        arr = [a * rate / r for a in arrivals]
+        res, lat, span = run_served(dm, ctx, dev, arr)
+        if res != served:
+            raise SystemExit(f"served proofs at {r}/s differ")
+        sweep[f"{r:g}"] = summary(mib, lat, span, arr)
+        if r == SWEEP_RATES[0]:  # near the plain path's capacity: the plain path at the same rate
+            _, pl_lat, pl_span = run_plain(dm, dev, arr)
+            sweep[f"{r:g}"]["plain"] = summary(mib, pl_lat, pl_span, arr)
+    sweep[f"{rate:g}"] = {k: head[k] for k in ("mib_s", "p50_ms", "p99_ms", "offered_mib_s")}
+    out["rate_sweep"] = sweep
+    # every request with its own generator seed, none of them seen before the timed pass (the
+    # warm-up uses other seeds): no generator matrix is reused across requests, so this separates
+    # batching and overlap from the generator cache
+    params = own_seed_params(dm, len(dev))
+    run_served(dm, ctx, dev[:w], arrivals[:w], params=own_seed_params(dm, w, b"warm-up"))
+    o_res, o_lat, o_span = run_served(dm, ctx, dev, arrivals, params=params)
+    run_plain(dm, dev[:16], arrivals[:16], params[:16])
+    op_res, op_lat, op_span = run_plain(dm, dev, arrivals, params)
+    if o_res != op_res:
+        raise SystemExit("own-seed served and plain proofs differ")
+    out["own_seeds"] = {"served": summary(mib, o_lat, o_span, arrivals), "plain": summary(mib, op_lat, op_span, arrivals)}
+    out["own_seeds"]["speedup_vs_plain"] = round((mib / o_span) / (mib / op_span), 2)
+    out["bit_exact_vs_plain"] = True
     global LAST_TRACE
     LAST_TRACE = (host, served)
-    return {
-        "mib": total, "served_mib_s": total / s_span, "plain_mib_s": total / p_span,
-        "served_p50_ms": pct(s_lat, 0.5) * 1e3, "served_p99_ms": pct(s_lat, 0.99) * 1e3,
-        "plain_p50_ms": pct(p_lat, 0.5) * 1e3, "plain_p99_ms": pct(p_lat, 0.99) * 1e3,
-        "offered_mib_s": total / max(arrivals[-1] - arrivals[0], 1e-9),
-        "host_files_mib_s": total / h_span, "host_files_p50_ms": pct(h_lat, 0.5) * 1e3,
-        "host_files_p99_ms": pct(h_lat, 0.99) * 1e3,
-    }
+    return out
 
 
 def batched_files(dm, ctx):
@@ -222,9 +321,32 @@ def ragged_products(dm, ctx, count=4096, reps=5, seed=3):
         outs.append(do.read())
     if outs[0] != outs[1]:
         raise SystemExit("persistent and per-degree ring products differ")
+    # per product of degree n: 3 transforms x 8 primes x (n/2) log2 n butterflies, 2 x 8 n mod-p
+    # reductions, 8 n pointwise products, n CRT lifts (full-rate weights)
+    w = FULL_RATE_WEIGHTS
+    lane_cycles = sum((1 << int(l)) * (12 * int(l) * w["B"] + 16 * w["R"] + 8 * w["P"] + w["C"]) for l in logs)
     return {"pairs": count, "coefficients": total, "persistent_ms": round(times[0] * 1e3, 3),
             "per_degree_ms": round(times[1] * 1e3, 3), "speedup": round(times[1] / times[0], 2),
-            "persistent_products_per_s": round(count / times[0], 1), "bit_exact_between_modes": True}
+            "persistent_products_per_s": round(count / times[0], 1), "bit_exact_between_modes": True,
+            "work_G_lane_cycles_full_rate": round(lane_cycles / 1e9, 3),
+            "frac_of_peak_per_call": round(lane_cycles / times[0] / PEAK_LANE_CYCLES, 3)}
+
+
+def primitive_root(p):
+    """the smallest generator of the multiplicative group mod the prime p (trial division of p - 1)"""
+    m, fs, f = p - 1, [], 2
+    while f * f <= m:
+        if m % f == 0:
+            fs.append(f)
+            while m % f == 0:
+                m //= f
+        f += 1
+    if m > 1:
+        fs.append(m)
+    g = 2
+    while any(pow(g, (p - 1) // f, p) == 1 for f in fs):
+        g += 1
+    return g
 
 
 def drop_in_latency(dm, reps=50):
@@ -233,16 +355,13 @@ def drop_in_latency(dm, reps=50):
     dmx_fold_batch over 4096 values"""
     import numpy as np
 
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import _oracle as O
-
     S = 2048
     rng = np.random.default_rng(8)
-    a = np.stack([rng.integers(0, p, size=S, dtype=np.uint64) for p in O.PRIMES])
-    b = np.stack([rng.integers(0, p, size=S, dtype=np.uint64) for p in O.PRIMES])
+    a = np.stack([rng.integers(0, p, size=S, dtype=np.uint64) for p in dm.PRIMES])
+    b = np.stack([rng.integers(0, p, size=S, dtype=np.uint64) for p in dm.PRIMES])
     roots, iroots, inv = [], [], []
-    for p in O.PRIMES:
-        g = O.lib().orc_multiplicative_generator(p)
+    for p in dm.PRIMES:
+        g = primitive_root(p)
         w = pow(g, (p - 1) // S, p)
         wi, ln = pow(w, p - 2, p), 2
         while ln <= S:
@@ -250,23 +369,27 @@ def drop_in_latency(dm, reps=50):
             iroots.append(pow(wi, S // ln, p))
             ln *= 2
         inv.append(pow(S, p - 2, p))
-    dm.ntt_batch(a, b, S, O.PRIMES, roots, iroots, inv)  # plan tables built on the first call
+    dm.ntt_batch(a, b, S, dm.PRIMES, roots, iroots, inv)  # plan tables built on the first call
     t0 = time.perf_counter()
     for _ in range(reps):
-        rc, _ = dm.ntt_batch(a, b, S, O.PRIMES, roots, iroots, inv)
+        rc, _ = dm.ntt_batch(a, b, S, dm.PRIMES, roots, iroots, inv)
     ntt_us = (time.perf_counter() - t0) / reps * 1e6
     if rc != 0:
         raise SystemExit("ntt_batch failed")
     n = 4096
-    left = [int(x) for x in rng.integers(0, 1 << 62, size=n)]
-    right = [int(x) for x in rng.integers(0, 1 << 62, size=n)]
-    dm.fold_batch(left, right, 12345)
+    arrs = [np.ascontiguousarray(rng.integers(0, 1 << 48, size=n, dtype=np.uint64)) for _ in range(6)]
+    ptrs = [a.ctypes.data for a in arrs]
+    fold = dm.lib().dmx_fold_batch
+    fold(*ptrs, n, 12345, 678)
     t0 = time.perf_counter()
     for _ in range(reps):
-        rc, _ = dm.fold_batch(left, right, 12345)
+        rc = fold(*ptrs, n, 12345, 678)
     fold_us = (time.perf_counter() - t0) / reps * 1e6
+    if rc != 0:
+        raise SystemExit("fold_batch failed")
     return {"ntt_batch_ring1024_us": round(ntt_us, 1), "fold_batch_4096_us": round(fold_us, 1),
-            "note": "per call through the Python mirror (ctypes, list conversion for fold_batch included)"}
+            "note": "per call, host buffers in and out: ntt_batch through the Python mirror, fold_batch on "
+                    "numpy arrays through the C ABI"}
 
 
 def depth_of(n):
@@ -277,8 +400,20 @@ def depth_of(n):
     return d
 
 
+CPU_KIND_NOTE = ("oracle: the C restatement of the reference's CPU algorithm (2n-point cyclic NTTs per "
+                 "product, %-based modular arithmetic, exact per-coefficient CRT lifts); its commitment loop "
+                 "runs over rows on all `cores` threads where the reference's is sequential (sis.rs:163-165)")
+
+
+def host_cores():
+    """(threads used, nproc, cpu_count): the baseline uses the box's CPU share, at most 16 threads
+    per GPU, whatever the host reports"""
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return min(16, nproc), nproc, os.cpu_count() or 1
+
+
 def cpu_baseline(data, gpu_proof, threads):
-    """the oracle (C restatement of the reference CPU algorithm) on the same job, all threads"""
+    """the oracle on the same job (the 8 MiB file), all threads"""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
 
@@ -289,7 +424,7 @@ def cpu_baseline(data, gpu_proof, threads):
     same = rc == 0 and (st, recs, op) == gpu_proof
     return {"value": round(FILE_BYTES / (1 << 20) / dt, 4), "unit": "MiB/s", "cores": threads, "kind": "port",
             "sample": f"the full job (prove + verify) on the 8 MiB single file, {dt:.1f} s",
-            "bit_exact_with_gpu": bool(same)}
+            "bit_exact_with_gpu": bool(same), "note": CPU_KIND_NOTE}
 
 
 def cpu_baseline_trace(files, gpu_proofs, threads):
@@ -321,23 +456,35 @@ def cpu_baseline_trace(files, gpu_proofs, threads):
             "sample": f"the served trace, one file per depth (the one nearest the depth's median size), prove + "
                       f"verify each ({'; '.join(parts)}; {cpu_s:.1f} s), rates weighted by the trace's bytes "
                       f"per depth",
-            "bit_exact_with_gpu": bool(same)}
+            "bit_exact_with_gpu": bool(same), "note": CPU_KIND_NOTE}
 
 
-def load_valu_issue():
-    path = os.path.join(ROOT, "profiles", "ntt_fwd_c2_valu.json")
+def source_digest():
+    """SHA-256 over the kernel sources with comments and blank lines removed, to tell whether a
+    committed counter profile was taken on the kernels of this build (a comment edit keeps it)"""
+    import hashlib
+    import re
+
+    h = hashlib.sha256()
+    for f in ("dmx_ntt.hip", "dmx_hash.hip", "dmx_arith.hpp", "dmx_kernels.hpp", "dmx_transcript.hpp", "dmx_blake3.hpp"):
+        with open(os.path.join(ROOT, "damascus000_amd", "csrc", f)) as fh:
+            text = re.sub(r"/\*.*?\*/", "", fh.read(), flags=re.S)
+        for ln in text.splitlines():
+            ln = re.sub(r"//.*", "", ln).rstrip()
+            if ln:
+                h.update(ln.encode() + b"\n")
+    return h.hexdigest()[:16]
+
+
+def load_profile(name):
+    """a committed counter summary (profiles/<name>.json) and whether it matches these kernels"""
+    path = os.path.join(ROOT, "profiles", name + ".json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("valu_issue_frac")
-
-
-def load_traffic():
-    path = os.path.join(ROOT, "profiles", "ntt_fwd_c2_traffic.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        prof = json.load(f)
+    prof["current"] = prof.get("source_digest") == source_digest()
+    return prof
 
 
 def calibrate(dm, ctx, mib):
@@ -354,15 +501,17 @@ def calibrate(dm, ctx, mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10, help="timed graph replays of the 8 MiB single file")
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--requests", type=int, default=DEFAULT_REQUESTS, help="served trace length")
+    ap.add_argument("--steps", type=int, default=DEFAULT_REQUESTS,
+                    help="requests in the timed served pass (a step is one request)")
+    ap.add_argument("--warmup", type=int, default=DEFAULT_REQUESTS,
+                    help="requests of the same trace in the untimed served pass before it")
     ap.add_argument("--rate", type=float, default=DEFAULT_RATE, help="served trace arrival rate (requests/s)")
+    ap.add_argument("--c2-reps", type=int, default=10, help="timed graph replays of the 8 MiB single file")
     ap.add_argument("--inflight", type=int, default=4, help="server batch slots")
     ap.add_argument("--no-mixed", action="store_true", help="skip the served trace and the 256-file batch")
-    ap.add_argument("--only", choices=["roofline", "served", "ragged"], default=None,
-                    help="profiling runs: only the prover runs of the roofline, the served trace, or the "
-                         "ragged products")
+    ap.add_argument("--only", choices=["roofline", "served", "ragged", "prover11"], default=None,
+                    help="profiling runs: the prover runs of the roofline, the served trace, the ragged products, "
+                         "or d = 11 prover runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run a device copy of N MiB and exit")
@@ -391,19 +540,29 @@ def main():
         return
 
     if args.only == "served":
-        trace = served_trace(args.requests, args.rate)
+        trace = served_trace(args.steps, args.rate)
         arrivals = [a for a, _ in trace]
         bufs = [ctx.upload(f) for _, f in trace]
         dev = [(b.ptr, len(f)) for b, (_, f) in zip(bufs, trace)]
-        total = sum(len(f) for _, f in trace) / (1 << 20)
+        mib = sum(len(f) for _, f in trace) / (1 << 20)
         run_served(dm, ctx, dev, arrivals)
         _, lat, span = run_served(dm, ctx, dev, arrivals)
-        print(json.dumps({"served_mib_s": round(total / span, 1), "p50_ms": round(pct(lat, 0.5) * 1e3, 2),
-                          "p99_ms": round(pct(lat, 0.99) * 1e3, 2), "requests": args.requests, "rate": args.rate,
-                          "server": LAST_STATS}))
+        out = summary(mib, lat, span, arrivals)
+        out.update({"requests": args.steps, "rate": args.rate, "server": LAST_STATS})
+        print(json.dumps(out))
         return
     if args.only == "ragged":
         print(json.dumps(ragged_products(dm, ctx)))
+        return
+    if args.only == "prover11":  # two d = 11 prover runs (dmx_prove), for a kernel trace
+        import numpy as np
+
+        big = np.random.default_rng(SEED + 11).integers(0, 256, size=16 << 20, dtype=np.uint8).tobytes()
+        for _ in range(2):
+            dm.prove(big)
+        print(json.dumps({"prover_runs": 2, "depth": 11,
+                          "work_fixed_lane_cycles": work(prover_ops(11), FIXED_WEIGHTS),
+                          "work_full_rate_lane_cycles": work(prover_ops(11), FULL_RATE_WEIGHTS)}))
         return
     data = single_file()
     if args.only == "roofline":
@@ -411,28 +570,29 @@ def main():
         ctx.kernel_timing()
         for _ in range(3):
             dm.prove(data)
-        k_ms, k_launches, k_work = ctx.kernel_timing()
+        k_ms, k_launches, k_work, k_full = ctx.kernel_timing(full_rate=True)
         ctx.kernel_timing(False)
         calibrate(dm, ctx, args.calibrate_after)
         print(json.dumps({"forward_transform_launches": k_launches, "ms": round(k_ms, 3),
                           "avg_us": round(k_ms * 1e3 / max(k_launches, 1), 2),
-                          "achieved_T_lane_cycles_s": round(k_work / (k_ms * 1e-3) / 1e12, 3)}))
+                          "achieved_T_lane_cycles_s": round(k_work / (k_ms * 1e-3) / 1e12, 3),
+                          "achieved_full_rate_T_lane_cycles_s": round(k_full / (k_ms * 1e-3) / 1e12, 3)}))
         return
 
     # ---- single 8 MiB file, plain path (graph replays) ----
     buf = ctx.upload(data)
     proof = None
-    for _ in range(args.warmup):
+    for _ in range(2):
         proof = dm.prove_verify_device(buf.ptr, FILE_BYTES)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(args.c2_reps):
         out = dm.prove_verify_device(buf.ptr, FILE_BYTES)
     t1 = time.perf_counter()
     if proof is not None and out != proof:
         raise SystemExit("proof changed between runs")
-    single_ms = (t1 - t0) * 1e3 / args.steps
+    single_ms = (t1 - t0) * 1e3 / args.c2_reps
     single = {"mib_s": round(FILE_BYTES / (1 << 20) / (single_ms * 1e-3), 1), "ms_per_job": round(single_ms, 3),
-              "file_bytes": FILE_BYTES, "depth": 10, "steps": args.steps}
+              "file_bytes": FILE_BYTES, "depth": 10, "reps": args.c2_reps}
     # the largest served depth: a 16 MiB file is d = 11 (2048 x 2048 witness)
     import numpy as np
 
@@ -443,26 +603,56 @@ def main():
     for _ in range(3):
         dm.prove_verify_device(bbuf.ptr, len(big), want_proof=False)
     d11_ms = (time.perf_counter() - t0) * 1e3 / 3
-    W_PROVER_D11 = 225.25e9  # lane-cycles of the d = 11 prover at the fixed weights
+    ops11 = prover_ops(11)
+    w11_fixed, w11_full = work(ops11, FIXED_WEIGHTS), work(ops11, FULL_RATE_WEIGHTS)
     single["d11"] = {"file_bytes": len(big), "ms_per_job": round(d11_ms, 3),
                      "mib_s": round(len(big) / (1 << 20) / (d11_ms * 1e-3), 1),
-                     "prover_work_frac_of_peak": round(W_PROVER_D11 / (d11_ms * 1e-3) / PEAK_LANE_CYCLES, 3),
-                     "note": "prove + verifier replay timed together; the prover's fixed-weight work over the "
-                             "whole job's time is a lower bound on the prover's own fraction"}
+                     "prover_work_G_lane_cycles": {"fixed_weights": round(w11_fixed / 1e9, 2),
+                                                   "full_rate_multiplies": round(w11_full / 1e9, 2)},
+                     "prover_frac_full_rate_lower_bound": round(w11_full / (d11_ms * 1e-3) / PEAK_LANE_CYCLES, 3),
+                     "note": "prove + verifier replay timed together, so the prover's work over the whole job's "
+                             "time is a lower bound on the prover's own fraction; the prover-only fraction from "
+                             "a kernel trace is in profiles/kernel_trace_prover_16MiB_d11.md"}
     del bbuf
 
     # ---- roofline of the dominant kernel, the degree-1024 forward transform (k_ntt_fwd_reg<32,32>):
-    # a prover run alone, HIP events around each of its launches ----
+    # prover runs on the 8 MiB file, HIP events around each of its launches on its own stream ----
     ctx.kernel_timing(True, log_m=10)
     ctx.kernel_timing()  # clear
     timing_runs = 3
     for _ in range(timing_runs):
         if dm.prove(data) != out:
             raise SystemExit("prover API and fused job disagree")
-    k_ms, k_launches, k_work = ctx.kernel_timing()
+    k_ms, k_launches, k_work, k_full = ctx.kernel_timing(full_rate=True)
     ctx.kernel_timing(False)
-    achieved = (k_work / max(k_launches, 1)) / ((k_ms / max(k_launches, 1)) * 1e-3) / 1e12 if k_ms > 0 else 0.0
+    per = max(k_launches, 1)
+    achieved = (k_work / per) / ((k_ms / per) * 1e-3) / 1e12 if k_ms > 0 else 0.0
+    achieved_full = (k_full / per) / ((k_ms / per) * 1e-3) / 1e12 if k_ms > 0 else 0.0
+    peak_t = PEAK_LANE_CYCLES / 1e12
+    counters = load_profile("ntt_fwd_c2_counters")
+    traffic = load_profile("ntt_fwd_c2_traffic")
+    roof = {"bound": "valu", "kernel": "k_ntt_fwd_reg<32,32> (forward transform, degree 1024, 8 primes, fused row "
+                                       "producer)",
+            "achieved": round(achieved_full, 3), "peak": round(peak_t, 3), "unit": "T lane-cycles/s",
+            "frac": round(achieved_full / peak_t, 4), "frac_reweighted": round(achieved_full / peak_t, 4),
+            "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None,
+            "weights": "full-rate 32-bit multiplies: butterfly 10, mod-p reduction 15, Z_q mul-add 108, BLAKE3 "
+                       "compression 860 lane-cycles",
+            "achieved_fixed_weights": round(achieved, 3), "frac_fixed_weights": round(achieved / peak_t, 4),
+            "fixed_weights": "butterfly 19, mod-p reduction 42, Z_q mul-add 204, BLAKE3 compression 860 (a 32-bit "
+                             "multiply priced at 4 lane-cycles; gfx950 issues it at 1, so this fraction can pass 1)",
+            "launches_per_job": k_launches // timing_runs, "kernel_ms_per_job": round(k_ms / timing_runs, 3),
+            "timing": f"HIP events around each degree-1024 forward-transform launch of {timing_runs} prover runs "
+                      "on the 8 MiB file, after the timed region"}
+    if counters:
+        roof["counters"] = {k: counters.get(k) for k in ("valu_issue_frac", "quad_cycles_per_valu_instr",
+                                                          "int32_share_of_valu", "lds_bank_conflicts_per_launch",
+                                                          "launches", "avg_us")}
+        roof["counters"]["current_build"] = counters["current"]
+    if traffic:
+        roof["traffic_current_build"] = traffic["current"]
 
+    threads, nproc, ncpu = host_cores()
     line = {
         "metric": "bit-exact commit+fold MiB/s on mixed d=6–11 load; p50/p99 latency vs plain GPU",
         "value": None,
@@ -476,46 +666,37 @@ def main():
         "vs_baseline": None,
         "dtype": "u32 (mod-p residues), u128 (Z_q)",
         "data": "synthetic: seeded uniform random bytes",
-        "config": {"workload": f"served trace: {args.requests} commit+fold requests (prove + verifier replay), "
+        "config": {"workload": f"served trace: {args.steps} commit+fold requests (prove + verifier replay), "
                                f"Poisson arrivals at {args.rate:g}/s, sizes log-uniform 64 KiB-16 MiB (d=7..11), "
-                               "K=8, files resident in HBM", "requests": args.requests, "rate_per_s": args.rate,
-                   "module_rank": 8, "parallelism": "one GPU; jobs of one depth share launches"},
+                               "K=8, files resident in HBM; a step is one request", "requests": args.steps,
+                   "rate_per_s": args.rate, "module_rank": 8, "parallelism": "one GPU; jobs of one depth share launches"},
         "single_file": single,
         "transform_tuning": {f"2^{lg}": dict(zip(("variant", "variants", "tuned_us"), ctx.transform_variant(lg)))
                              for lg in range(6, 12)},
-        "roofline": {"bound": "valu", "kernel": "k_ntt_fwd_reg<32,32> (forward transform, degree 1024, 8 primes, "
-                                                "fused row producer)",
-                     "achieved": round(achieved, 3), "peak": round(PEAK_LANE_CYCLES / 1e12, 3),
-                     "unit": "T lane-cycles/s", "frac": round(achieved / (PEAK_LANE_CYCLES / 1e12), 4),
-                     "traffic": load_traffic(), "launches_per_job": k_launches // timing_runs,
-                     "valu_issue_frac_counters": load_valu_issue(),
-                     "note": "the fixed weights price 32-bit multiplies at quarter rate; gfx950 issues them at "
-                             "full rate, so frac overstates VALU use. valu_issue_frac_counters is SQ_INSTS_VALU x 64 "
-                             "/ duration / 39.32T for this kernel (profiles/valu_rates_gfx950.md)",
-                     "kernel_ms_per_job": round(k_ms / timing_runs, 3), "work_weights": WORK_WEIGHTS,
-                     "timing": f"HIP events around each degree-1024 forward-transform launch of {timing_runs} "
-                               "prover runs on the 8 MiB file, after the timed region"},
+        "roofline": roof,
+        "host": {"nproc": nproc, "cpu_count": ncpu},
     }
     if not args.no_mixed:
-        mx = mixed_served(dm, ctx, args.requests, args.rate)
-        line["value"] = round(mx["served_mib_s"], 3)
-        line["ms_per_step"] = round(mx["mib"] / mx["served_mib_s"] * 1e3 / args.requests, 4)
-        line["served"] = {k: round(v, 2) for k, v in mx.items()}
-        line["served"]["speedup_vs_plain"] = round(mx["served_mib_s"] / mx["plain_mib_s"], 2)
-        line["served"]["bit_exact_vs_plain"] = True
+        mx = mixed_served(dm, ctx, args.steps, args.rate, args.warmup)
+        head = mx["headline"]
+        line["value"] = head["mib_s"]
+        line["ms_per_step"] = round(head["makespan_s"] * 1e3 / args.steps, 4)
+        line["timed_region_s"] = head["makespan_s"]
+        line["served"] = mx
         line["batched_files"] = batched_files(dm, ctx)
         line["ragged_products"] = ragged_products(dm, ctx)
         line["drop_in"] = drop_in_latency(dm)
     else:
         line["value"] = single["mib_s"]
         line["ms_per_step"] = single["ms_per_job"]
-        line["config"]["workload"] = "single 8 MiB file, plain path (mixed load skipped)"
+        line["steps"], line["warmup"] = args.c2_reps, 2
+        line["config"]["workload"] = "single 8 MiB file, plain path (mixed load skipped); a step is one job"
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        t = args.cpu_threads or threads
         if LAST_TRACE is not None:  # the workload `value` is quoted on
-            line["cpu_baseline"] = cpu_baseline_trace(LAST_TRACE[0], LAST_TRACE[1], threads)
+            line["cpu_baseline"] = cpu_baseline_trace(LAST_TRACE[0], LAST_TRACE[1], t)
         else:
-            line["cpu_baseline"] = cpu_baseline(data, out, threads)
+            line["cpu_baseline"] = cpu_baseline(data, out, t)
     if calibrate(dm, ctx, args.calibrate_after):
         line["calibration_copy_bytes"] = args.calibrate_after << 20
     print(json.dumps(line), flush=True)

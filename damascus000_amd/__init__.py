@@ -151,7 +151,7 @@ _SIGS = {
                                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]),
     "dmx_ctx_set_transform_variant": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
     "dmx_kernel_timing": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
-                                         ctypes.POINTER(ctypes.c_double)]),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "dmx_statement_size": (_SZ, [_SZ]),
     "dmx_record_size": (_SZ, [_SZ, _SZ]),
     "dmx_proof_records_size": (_SZ, [ctypes.c_uint64]),
@@ -239,15 +239,18 @@ class Context:
         _check(lib().dmx_memcpy_htod(self.h, _VP(buf.ptr), _bytes_ptr(data), len(data)))
         return buf
 
-    def kernel_timing(self, on=None, log_m=-1):
+    def kernel_timing(self, on=None, log_m=-1, full_rate=False):
         """enable (on=True/False; log_m >= 0 records only degree 2^log_m) or read the
-        forward-transform timing record: returns (total_ms, launches, lane_cycles) and clears it"""
+        forward-transform timing record: returns (total_ms, launches, lane_cycles) and clears it;
+        with full_rate, (total_ms, launches, lane_cycles, lane_cycles at full-rate multiplies)"""
         if on is not None:
             _check(lib().dmx_kernel_timing_degree(self.h, log_m if on else -1))
             _check(lib().dmx_kernel_timing_enable(self.h, int(bool(on))))
             return None
-        ms, n, w = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double()
-        _check(lib().dmx_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(w)))
+        ms, n, w, wf = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().dmx_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(w), ctypes.byref(wf)))
+        if full_rate:
+            return ms.value, n.value, w.value, wf.value
         return ms.value, n.value, w.value
 
     def transform_variant(self, log_m):

@@ -1762,12 +1762,8 @@ void launch_gen_generators(const uint32_t* gseed, size_t rows, int log_m, zq* ou
 }
 
 // ---------------------------------------------------------------------------------------
-// reference-compatible batch entry points (src/utils/gpu.rs:69-93)
-// ---------------------------------------------------------------------------------------
-// out = l + c*r mod q over SoA lo/hi words (cuda/fold_kernel.cu:189-206 behaviour)
-// ---------------------------------------------------------------------------------------
-// The reference-compatible drop-ins (damascus_cuda_fold_batch / damascus_cuda_ntt_batch), on
-// buffers the host side keeps per device.
+// The reference-compatible drop-ins (damascus_cuda_fold_batch / damascus_cuda_ntt_batch,
+// src/utils/gpu.rs:69-93), on buffers the host side keeps per device.
 // ---------------------------------------------------------------------------------------
 // out = left + c * right mod q over SoA 64-bit halves (fold_kernel.cu:189-206). Inputs must be
 // canonical (< q, as every Fp is); any that is not sets *flag and the call is declined.
@@ -2009,14 +2005,14 @@ void launch_ring_mul_persistent(const zq* A, const zq* B, zq* O, const uint64_t*
 // LDS-resident ragged ring products (degrees 2^0 .. 2^12) in one persistent launch.
 //
 // A queue item is up to RPB products of one degree 2^LOG_M = E * TP. Each product's 8 prime slices
-// are held by TP lanes each (TP <= 64, so a slice never leaves its wave): lane l of slice q loads
-// the coefficients l + TP*i (i < E) of an operand straight from memory and reduces them mod p_q,
-// runs the transform layers len >= TP in registers, swaps to the contiguous layout (E*l .. E*l+E-1)
-// through its wave's part of LDS, and runs the layers len < TP. With both operands transformed,
-// the pointwise product is scaled by 1/m; the inverse runs the same two passes backwards. The
-// residues of all 8 primes then meet in LDS, where each output coefficient gets its centred CRT
-// lift. The transposes are wave-local, so an item has two workgroup barriers (before and after
-// the lift); operands are read once and nothing goes through global scratch.
+// are held by TP lanes each (TP <= 64, so a slice never leaves its wave). The workgroup stages an
+// operand into LDS (each coefficient loaded once and reduced into all 8 primes); lane l of slice
+// q then takes the coefficients l + TP*i (i < E) of its prime, runs the transform layers
+// len >= TP in registers, swaps to the contiguous layout (E*l .. E*l+E-1) through its wave's part
+// of LDS, and runs the layers len < TP. With both operands transformed, the pointwise product is
+// scaled by 1/m; the inverse runs the same two passes backwards. The residues of all 8 primes
+// then meet in LDS, where each output coefficient gets its centred CRT lift. The transposes are
+// wave-local; operands are read once and nothing goes through global scratch.
 // ---------------------------------------------------------------------------------------
 constexpr int RING_LDS_THREADS = 512;
 constexpr int RING_LDS_MAX_LOG = 12;

@@ -245,7 +245,8 @@ struct DevBuf {
 
 struct TimedLaunch {
     hipEvent_t a, b;
-    double work;  // algorithmic lane-cycles of the launch
+    double work;       // algorithmic lane-cycles of the launch, fixed weights
+    double work_full;  // the same with 32-bit multiplies at the full issue rate
 };
 
 struct dmx_ctx {
@@ -295,16 +296,24 @@ struct dmx_ctx {
     }
 };
 
-// algorithmic VALU work of one forward-transform row of degree 2^log_m, in lane-cycles with the
-// fixed weights: butterfly 19, mod-p reduction 42, Z_q multiply-add 204, BLAKE3 compression 860
-static double fwd_row_work(int log_m, int mode) {
+// Algorithmic VALU work of one forward-transform row of degree 2^log_m in lane-cycles. Fixed
+// weights: butterfly 19, mod-p reduction 42, Z_q multiply-add 204, BLAKE3 compression 860; they
+// price a 32-bit multiply at 4 lane-cycles. Full-rate weights count it as 1 (gfx950 issues
+// v_mul_lo_u32 / v_mul_hi_u32 at the full rate): butterfly 10 (3 multiplies), reduction 15 (9),
+// multiply-add 108 (32), BLAKE3 860 (none).
+struct OpWeights {
+    double butterfly, reduction, muladd, blake3;
+};
+constexpr OpWeights FIXED_WEIGHTS{19.0, 42.0, 204.0, 860.0};
+constexpr OpWeights FULL_RATE_WEIGHTS{10.0, 15.0, 108.0, 860.0};
+static double fwd_row_work(int log_m, int mode, const OpWeights& w = FIXED_WEIGHTS) {
     const double m = (double)((size_t)1 << log_m);
-    double w = NPRIMES * m * 42.0 + NPRIMES * (m / 2) * log_m * 19.0;
-    if (mode == ROW_GEN) w += m * 860.0;
+    double v = NPRIMES * m * w.reduction + NPRIMES * (m / 2) * log_m * w.butterfly;
+    if (mode == ROW_GEN) v += m * w.blake3;
     if (mode == ROW_VFOLD || mode == ROW_PFOLD_W || mode == ROW_PFOLD_G || mode == ROW_VFOLD_L ||
         mode == ROW_PFOLD_GL)
-        w += m * 204.0;
-    return w;
+        v += m * w.muladd;
+    return v;
 }
 
 // forward transforms go through here so that kernel timing can bracket them; two row sets of
@@ -320,7 +329,9 @@ static void fwd_rows2(dmx_ctx* ctx, const RowSrc& src1, size_t rows1, uint32_t* 
     std::lock_guard<std::mutex> g(ctx->mu);
     const double work = (double)jb.n * ((double)rows1 * fwd_row_work(log_m, src1.mode) +
                                         (double)rows2 * fwd_row_work(log_m, src2.mode));
-    TimedLaunch t{ctx->take_event(), ctx->take_event(), work};
+    const double work_full = (double)jb.n * ((double)rows1 * fwd_row_work(log_m, src1.mode, FULL_RATE_WEIGHTS) +
+                                             (double)rows2 * fwd_row_work(log_m, src2.mode, FULL_RATE_WEIGHTS));
+    TimedLaunch t{ctx->take_event(), ctx->take_event(), work, work_full};
     (void)hipEventRecord(t.a, s);
     launch_ntt_fwd_rows2(src1, rows1, out1, src2, rows2, out2, log_m, ctx->tw, s, jb, it, v);
     (void)hipEventRecord(t.b, s);
@@ -341,22 +352,25 @@ extern "C" int dmx_kernel_timing_degree(dmx_ctx* ctx, int log_m) {
     return DMX_OK;
 }
 
-extern "C" int dmx_kernel_timing(dmx_ctx* ctx, double* total_ms, uint64_t* launches, double* lane_cycles) {
+extern "C" int dmx_kernel_timing(dmx_ctx* ctx, double* total_ms, uint64_t* launches, double* lane_cycles,
+                                 double* lane_cycles_full_rate) {
     if (!ctx) return fail(DMX_ERR_ARG, "null context");
     std::lock_guard<std::mutex> g(ctx->mu);
-    double ms = 0, work = 0;
+    double ms = 0, work = 0, work_full = 0;
     for (auto& t : ctx->launches) {
         HIP_TRY(hipEventSynchronize(t.b));
         float e = 0;
         HIP_TRY(hipEventElapsedTime(&e, t.a, t.b));
         ms += e;
         work += t.work;
+        work_full += t.work_full;
         ctx->event_pool.push_back(t.a);
         ctx->event_pool.push_back(t.b);
     }
     if (total_ms) *total_ms = ms;
     if (launches) *launches = ctx->launches.size();
     if (lane_cycles) *lane_cycles = work;
+    if (lane_cycles_full_rate) *lane_cycles_full_rate = work_full;
     ctx->launches.clear();
     return DMX_OK;
 }

@@ -257,10 +257,13 @@ int dmx_device_copy(dmx_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* ---- kernel timing: with timing on, each forward-transform launch of every prover made
  * from this context is bracketed by HIP events on its own stream. dmx_kernel_timing returns
  * the summed device time (ms), the launch count and the summed algorithmic work of those
- * launches in VALU lane-cycles (fixed weights per butterfly, mod-p reduction, Z_q fold
- * multiply-add and BLAKE3 compression), then clears the record. No reference counterpart. */
+ * launches in VALU lane-cycles: with the fixed weights per butterfly (19), mod-p reduction (42),
+ * Z_q fold multiply-add (204) and BLAKE3 compression (860), which price a 32-bit multiply at 4,
+ * and with 32-bit multiplies at the full rate (10, 15, 108, 860); then clears the record. Any
+ * output pointer may be null. No reference counterpart. */
 int dmx_kernel_timing_enable(dmx_ctx *ctx, int on);
-int dmx_kernel_timing(dmx_ctx *ctx, double *total_ms, uint64_t *launches, double *lane_cycles);
+int dmx_kernel_timing(dmx_ctx *ctx, double *total_ms, uint64_t *launches, double *lane_cycles,
+                      double *lane_cycles_full_rate);
 /* record only launches of degree 2^log_m (negative: every degree) */
 int dmx_kernel_timing_degree(dmx_ctx *ctx, int log_m);
 
