@@ -1,6 +1,8 @@
 /* Oracle: commitment, transcript, witness expansion, prover and verifier (test
  * infrastructure only). Restates src/commitment/sis.rs, src/protocol/{transcript,prover,
  * verifier}.rs and src/utils/io.rs of the reference, step for step. */
+#include <pthread.h>
+#include <stdatomic.h>
 #include <stdlib.h>
 
 #include "internal.h"
@@ -9,10 +11,51 @@
 #define K ORC_K
 
 int orc_threads = 1;
-void orc_set_threads(int t) { orc_threads = t < 1 ? 1 : t; }
+void orc_set_threads(int t) { orc_threads = t < 1 ? 1 : (t > 256 ? 256 : t); }
 int orc_get_threads(void) { return orc_threads; }
 
 static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
+
+/* A parallel loop over [0, n) on orc_threads POSIX threads: each worker takes the next index from
+ * a shared counter and calls body(ctx, worker, index). Workers are joined before it returns, so
+ * everything they wrote is visible to the caller afterwards. */
+typedef void (*par_body)(void *ctx, int worker, size_t index);
+typedef struct {
+    par_body body;
+    void *ctx;
+    size_t n;
+    atomic_size_t next;
+    int worker;
+} par_job;
+typedef struct {
+    par_job *job;
+    int worker;
+} par_arg;
+
+static void *par_worker(void *p) {
+    const par_arg *a = (const par_arg *)p;
+    for (;;) {
+        const size_t i = atomic_fetch_add(&a->job->next, 1);
+        if (i >= a->job->n) break;
+        a->job->body(a->job->ctx, a->worker, i);
+    }
+    return NULL;
+}
+
+static void par_for(size_t n, int workers, par_body body, void *ctx) {
+    par_job job = {body, ctx, n, 0, 0};
+    pthread_t tid[256];
+    par_arg args[256];
+    int started = 0;
+    for (int w = 1; w < workers; ++w) { /* worker 0 is the calling thread */
+        args[w] = (par_arg){&job, w};
+        if (pthread_create(&tid[w], NULL, par_worker, &args[w]) != 0) break;
+        started = w;
+    }
+    args[0] = (par_arg){&job, 0};
+    par_worker(&args[0]);
+    for (int w = 1; w <= started; ++w) pthread_join(tid[w], NULL);
+}
 
 /* ---------------- generators (sis.rs:198-245) ---------------- */
 
@@ -37,11 +80,21 @@ static u128 derive_field(const uint8_t seed[32], size_t i, size_t k, size_t c) {
     return zq_from_le_bytes_mod_order(h, 32);
 }
 
+typedef struct {
+    const uint8_t *seed;
+    size_t ring_len;
+    u128 *g;
+} gen_ctx;
+
+static void gen_row(void *p, int worker, size_t i) {
+    const gen_ctx *c = (const gen_ctx *)p;
+    for (size_t k = 0; k < K; ++k)
+        for (size_t j = 0; j < c->ring_len; ++j) c->g[(i * K + k) * c->ring_len + j] = derive_field(c->seed, i, k, j);
+}
+
 static void derive_generators_u128(const uint8_t seed[32], size_t rows, size_t ring_len, u128 *g) {
-#pragma omp parallel for num_threads(orc_threads) schedule(static)
-    for (size_t i = 0; i < rows; ++i)
-        for (size_t k = 0; k < K; ++k)
-            for (size_t c = 0; c < ring_len; ++c) g[(i * K + k) * ring_len + c] = derive_field(seed, i, k, c);
+    gen_ctx c = {seed, ring_len, g};
+    par_for(rows, orc_threads, gen_row, &c);
 }
 
 void orc_derive_generators(const uint8_t g_seed[32], size_t rows, size_t ring_len, uint64_t *out) {
@@ -53,6 +106,24 @@ void orc_derive_generators(const uint8_t g_seed[32], size_t rows, size_t ring_le
 }
 
 /* ---------------- commitment (sis.rs:146-167) ---------------- */
+
+/* one term g[i]_k * w[i] (t = i * K + k) added into the worker's own accumulator */
+typedef struct {
+    const u128 *w, *g;
+    size_t m;
+    int ntt;
+    u128 *prod, *acc; /* per worker: m and K * m values */
+    int *rc;          /* per worker */
+} commit_ctx;
+
+static void commit_term(void *p, int worker, size_t t) {
+    const commit_ctx *c = (const commit_ctx *)p;
+    if (c->rc[worker] != ORC_OK) return;
+    const size_t i = t / K, k = t % K, m = c->m;
+    u128 *prod = c->prod + (size_t)worker * m, *acc = c->acc + (size_t)worker * K * m;
+    c->rc[worker] = ring_mul(c->g + (i * K + k) * m, c->w + i * m, m, c->ntt, prod);
+    for (size_t j = 0; j < m; ++j) acc[k * m + j] = zq_add(acc[k * m + j], prod[j]);
+}
 
 /* acc_k = sum_i g[i]_k * w[i]; rows == 0 -> zero(1). out holds 8*m (or 8 when rows == 0). */
 static int commit_u128(const u128 *w, const u128 *g, size_t rows, size_t m, int ntt, u128 *out,
@@ -79,27 +150,22 @@ static int commit_u128(const u128 *w, const u128 *g, size_t rows, size_t m, int 
     }
     /* same products, summed mod q in another order: Z_q addition is exact, so the
      * result is identical to the sequential loop */
-#pragma omp parallel num_threads(orc_threads)
-    {
-        u128 *prod = (u128 *)xcalloc(m, sizeof(u128));
-        u128 *acc = (u128 *)xcalloc(K * m, sizeof(u128));
-        int lrc = (prod && acc) ? ORC_OK : ORC_ERR_OOM;
-#pragma omp for schedule(dynamic)
-        for (size_t t = 0; t < rows * K; ++t) {
-            if (lrc != ORC_OK) continue;
-            size_t i = t / K, k = t % K;
-            lrc = ring_mul(g + (i * K + k) * m, w + i * m, m, ntt, prod);
-            for (size_t c = 0; c < m; ++c) acc[k * m + c] = zq_add(acc[k * m + c], prod[c]);
+    const int workers = orc_threads;
+    commit_ctx c = {w, g, m, ntt, (u128 *)xcalloc((size_t)workers * m, sizeof(u128)),
+                    (u128 *)xcalloc((size_t)workers * K * m, sizeof(u128)),
+                    (int *)xcalloc((size_t)workers, sizeof(int))};
+    if (c.prod && c.acc && c.rc) {
+        par_for(rows * K, workers, commit_term, &c);
+        for (int t = 0; t < workers; ++t) {
+            if (c.rc[t] != ORC_OK) rc = c.rc[t];
+            for (size_t j = 0; j < K * m; ++j) out[j] = zq_add(out[j], c.acc[(size_t)t * K * m + j]);
         }
-#pragma omp critical
-        {
-            if (lrc != ORC_OK) rc = lrc;
-            if (acc)
-                for (size_t c = 0; c < K * m; ++c) out[c] = zq_add(out[c], acc[c]);
-        }
-        free(prod);
-        free(acc);
+    } else {
+        rc = ORC_ERR_OOM;
     }
+    free(c.prod);
+    free(c.acc);
+    free(c.rc);
     return rc;
 }
 
