@@ -6,81 +6,9 @@
 // of BLAKE3's left-balanced tree); a last workgroup merges the group values pairwise bottom-up,
 // carrying an odd node upward, which reproduces the left-balanced tree, and sets ROOT on the
 // final compression.
-#include "dmx_blake3.hpp"
-#include "dmx_kernels.hpp"
-#include "dmx_transcript.hpp"
+#include "dmx_b3quad.hpp"
 
 namespace dmx {
-
-struct KeyW {
-    uint32_t w[8];
-};
-
-// ---------------------------------------------------------------------------------------
-// Quad-cooperative compression: four lanes (q = lane & 3) hold the four columns of the 4x4
-// BLAKE3 state, lane q keeping (s[q], s[4+q], s[8+q], s[12+q]). The column step is one G per
-// lane; the diagonal step rotates b, c, d within the quad by 1, 2, 3 lanes (DPP quad_perm),
-// applies G and rotates back. Every lane holds the 16 message words and picks its pair per
-// round with two lane-mask selects. A compression costs ~300 VALU instructions per lane
-// instead of ~680 on one lane, which is what bounds a 16-block chunk's latency.
-// All four lanes of a quad must be active together.
-// ---------------------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t qrot1(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x39, 0xF, 0xF, false); }
-__device__ __forceinline__ uint32_t qrot2(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false); }
-__device__ __forceinline__ uint32_t qrot3(uint32_t v) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x93, 0xF, 0xF, false); }
-
-struct QuadSel {
-    bool b0, b1;  // bits of the lane's column index
-    __device__ __forceinline__ uint32_t operator()(uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3) const {
-        const uint32_t t0 = b0 ? v1 : v0, t1 = b0 ? v3 : v2;
-        return b1 ? t1 : t0;
-    }
-};
-
-#define DMX_QG(mx, my)                 \
-    a = a + b + (mx);                  \
-    d = b3::rotr(d ^ a, 16);           \
-    c = c + d;                         \
-    b = b3::rotr(b ^ c, 12);           \
-    a = a + b + (my);                  \
-    d = b3::rotr(d ^ a, 8);            \
-    c = c + d;                         \
-    b = b3::rotr(b ^ c, 7);
-
-#define DMX_QROUND(s0, s1, s2, s3, s4, s5, s6, s7, s8, s9, s10, s11, s12, s13, s14, s15) \
-    {                                                                                     \
-        const uint32_t mx = sel(m[s0], m[s2], m[s4], m[s6]);                              \
-        const uint32_t my = sel(m[s1], m[s3], m[s5], m[s7]);                              \
-        DMX_QG(mx, my)                                                                    \
-    }                                                                                     \
-    b = qrot1(b);                                                                         \
-    c = qrot2(c);                                                                         \
-    d = qrot3(d);                                                                         \
-    {                                                                                     \
-        const uint32_t mx = sel(m[s8], m[s10], m[s12], m[s14]);                           \
-        const uint32_t my = sel(m[s9], m[s11], m[s13], m[s15]);                           \
-        DMX_QG(mx, my)                                                                    \
-    }                                                                                     \
-    b = qrot3(b);                                                                         \
-    c = qrot2(c);                                                                         \
-    d = qrot1(d);
-
-// (ca, cb) = lane q's chaining-value words (cv[q], cv[4+q]); returns them updated
-__device__ __forceinline__ void compress_quad(uint32_t& ca, uint32_t& cb, const uint32_t m[16], uint64_t counter,
-                                              uint32_t block_len, uint32_t flags, const QuadSel& sel) {
-    uint32_t a = ca, b = cb;
-    uint32_t c = sel(b3::iv(0), b3::iv(1), b3::iv(2), b3::iv(3));
-    uint32_t d = sel((uint32_t)counter, (uint32_t)(counter >> 32), block_len, flags);
-    DMX_QROUND(0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15)
-    DMX_QROUND(2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8)
-    DMX_QROUND(3, 4, 10, 12, 13, 2, 7, 14, 6, 5, 9, 0, 11, 15, 8, 1)
-    DMX_QROUND(10, 7, 12, 9, 14, 3, 13, 15, 4, 0, 11, 2, 5, 8, 1, 6)
-    DMX_QROUND(12, 13, 9, 11, 15, 10, 14, 8, 7, 2, 5, 3, 0, 1, 6, 4)
-    DMX_QROUND(9, 14, 11, 5, 8, 12, 15, 1, 13, 3, 0, 10, 2, 6, 4, 7)
-    DMX_QROUND(11, 15, 5, 0, 1, 9, 8, 6, 14, 10, 2, 12, 3, 4, 7, 13)
-    ca = a ^ c;
-    cb = b ^ d;
-}
 
 size_t transcript_len(uint32_t label_len, const uint32_t lens[3]) { return tx_message_len(label_len, lens); }
 
@@ -132,77 +60,7 @@ __device__ void chunk_cv_quad(const uint8_t* msg, size_t len, size_t ci, const u
     }
 }
 
-__host__ __device__ inline size_t nchunks_of(size_t len) { return len == 0 ? 1 : (len + 1023) / 1024; }
 
-// ---------------------------------------------------------------------------------------
-// Transcript messages hashed straight from their sources: a workgroup writes its window of the
-// message (16 chunks, 16 KiB) into LDS as aligned 8-byte words (tx_words4) and its quads
-// compress the chunks from there, so no packed copy goes through memory first.
-// ---------------------------------------------------------------------------------------
-constexpr uint32_t TX_WINDOW = 16 * 1024;  // bytes: 16 chunks, one per quad of the first wave
-
-__device__ __forceinline__ void tx_to_job(TranscriptMsg& t, size_t jo) {
-    t.epoch_seed = jp(t.epoch_seed, jo);
-    t.file_id = jp(t.file_id, jo);
-    for (int i = 0; i < 3; ++i) t.mods[i] = jp(t.mods[i], jo);
-    t.reject = jp(t.reject, jo);
-}
-
-// the job's rejection count (TranscriptMsg::reject), 0 without one
-__device__ __forceinline__ uint32_t reject_below(const uint32_t* r) { return r ? *r : 0u; }
-
-// Candidate `counter` of sample_nonzero_challenge (transcript.rs:66-75) is taken when non-zero
-// and not below the job's rejection count. Resampling is bounded on the device: past
-// RESAMPLE_LIMIT the job reports ERR_RESAMPLE (about 2^-7000 for real digests).
-constexpr uint32_t RESAMPLE_LIMIT = 64;
-constexpr uint32_t ERR_RESAMPLE = 0x80000000u;
-__device__ __forceinline__ bool candidate_taken(const zq& x, uint32_t counter, uint32_t rj) {
-    return !zq_is_zero(x) && counter >= rj;
-}
-// A failed job still runs to its end: its challenge slot gets 1 (and inverse 1), so the kernels
-// after it compute on defined values; the error bit fails the job.
-__device__ __forceinline__ void exhausted_challenge(zq* chal_out, int want_inv) {
-    chal_out[0] = zq_from_u64(1);
-    if (want_inv) chal_out[1] = zq_from_u64(1);
-}
-
-// message words [w0, w0 + nw) into win[0 .. nw), nw a multiple of 4, by the whole workgroup
-__device__ void tx_pack_lds(const TranscriptMsg& t, uint32_t w0, uint32_t nw, uint64_t* win) {
-    for (uint32_t g = 4 * threadIdx.x; g < nw; g += 4 * blockDim.x) {
-        uint64_t o[4];
-        tx_words4(t, w0 + g, o);
-        *reinterpret_cast<ulonglong2*>(win + g) = make_ulonglong2(o[0], o[1]);
-        *reinterpret_cast<ulonglong2*>(win + g + 2) = make_ulonglong2(o[2], o[3]);
-    }
-}
-
-// chunk_cv_quad over a message window in LDS that starts at message byte `base` and holds every
-// block of the chunk (bytes past `len` read as zero)
-__device__ void chunk_cv_quad_lds(const uint64_t* win, size_t base, size_t len, size_t ci, const uint32_t key[8],
-                                  uint32_t flags, bool root, const QuadSel& sel, uint32_t& ca, uint32_t& cb) {
-    ca = sel(key[0], key[1], key[2], key[3]);
-    cb = sel(key[4], key[5], key[6], key[7]);
-    const size_t start = ci * b3::CHUNK_LEN;
-    const size_t end = (start + b3::CHUNK_LEN < len) ? start + b3::CHUNK_LEN : len;
-    const size_t nb = end > start ? (end - start + 63) / 64 : 1;
-    const uint4* p = reinterpret_cast<const uint4*>(win) + (start - base) / 16;
-    for (size_t b = 0; b < nb; ++b) {
-        uint32_t m[16];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const uint4 v = p[4 * b + i];
-            m[4 * i] = v.x;
-            m[4 * i + 1] = v.y;
-            m[4 * i + 2] = v.z;
-            m[4 * i + 3] = v.w;
-        }
-        const size_t off = start + 64 * b;
-        const uint32_t blen = (uint32_t)(end - off < 64 ? end - off : 64);
-        uint32_t fl = flags | (b == 0 ? b3::CHUNK_START : 0) | (b + 1 == nb ? b3::CHUNK_END : 0);
-        if (root && b + 1 == nb) fl |= b3::ROOT;
-        compress_quad(ca, cb, m, ci, end > start ? blen : 0, fl, sel);
-    }
-}
 
 // Chunk CVs of a transcript message: 256 threads write the window, its first 16 quads hash the
 // chunks. The chunks holding the counter are also written to `msg`, where a root that meets a
@@ -270,38 +128,6 @@ static void launch_b3_chunks(const uint8_t* msg, size_t len, const uint32_t key[
                        cvs, n, jb.stride, dlen);
 }
 
-// Pairwise bottom-up merge of the n CVs in `a` (LDS), one quad per parent node, ping-ponging
-// between `a` and `b`; returns the buffer holding the single result. With `final_root` the last
-// compression carries ROOT (n >= 2). Loop bounds are uniform, so every thread reaches every
-// barrier; the caller has synchronised after filling `a`.
-__device__ uint32_t* merge_lds(uint32_t* a, uint32_t* b, uint32_t n, const uint32_t key[8], uint32_t flags,
-                               bool final_root) {
-    const uint32_t q = threadIdx.x & 3, quad = threadIdx.x >> 2, nquads = blockDim.x >> 2;
-    const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
-    while (n > 1) {
-        const uint32_t pairs = n >> 1;
-        const uint32_t fl = flags | b3::PARENT | ((n == 2 && final_root) ? b3::ROOT : 0);
-        for (uint32_t base = 0; base < pairs; base += nquads) {
-            const uint32_t pr = base + quad;
-            if (pr < pairs) {
-                uint32_t m[16];
-#pragma unroll
-                for (int i = 0; i < 16; ++i) m[i] = a[2 * pr * 8 + i];
-                uint32_t ca = sel(key[0], key[1], key[2], key[3]), cb = sel(key[4], key[5], key[6], key[7]);
-                compress_quad(ca, cb, m, 0, b3::BLOCK_LEN, fl, sel);
-                b[pr * 8 + q] = ca;
-                b[pr * 8 + 4 + q] = cb;
-            }
-        }
-        if ((n & 1) && threadIdx.x < 8) b[pairs * 8 + threadIdx.x] = a[(n - 1) * 8 + threadIdx.x];
-        __syncthreads();
-        uint32_t* t = a;
-        a = b;
-        b = t;
-        n = pairs + (n & 1);
-    }
-    return a;
-}
 
 // n chaining values (8 words each, 32-byte aligned) from global memory into LDS with 16-byte
 // loads, a thread's loads issued together
@@ -481,81 +307,16 @@ static void launch_b3_root(const uint32_t* cvs, size_t n, const uint32_t* chunk_
 }
 
 // Hash a device buffer: chunk CVs, group merges while more than 1024 values remain, root.
-// A whole challenge in one workgroup for messages of at most 16 chunks: 256 threads write the
-// message into LDS, the 16 quads of wave 0 hash the chunks, the workgroup merges them to the
-// root, and wave 0 reduces the digest mod q and inverts it. A zero challenge (transcript.rs:66-75) bumps the counter in LDS
-// and hashes the message again. (Larger messages need more than one CU to hash their chunks
-// quickly and take the chunk and root kernels.)
-constexpr size_t CHALLENGE_MAX_CHUNKS = 16;
-
+// A whole challenge in one workgroup for messages of at most 16 chunks (wg_tx_challenge).
+// (Larger messages need more than one CU to hash their chunks quickly and take the chunk and
+// root kernels.)
 __global__ void __launch_bounds__(256) k_b3_tx_challenge(TranscriptMsg t, size_t len, KeyW key, uint32_t flags,
                                                         zq* chal_out, uint32_t* err, size_t js, int want_inv) {
     critical_path_priority();
-    __shared__ __align__(16) uint64_t win[TX_WINDOW / 8];
-    __shared__ uint32_t sa[16 * 8], sb[8 * 8 + 8];
-    __shared__ uint32_t dig[8];
-    __shared__ int again;
+    __shared__ WgChal sm;
     const size_t jo = job_off(js);
     tx_to_job(t, jo);
-    chal_out = jp(chal_out, jo);
-    err = jp(err, jo);
-    const uint32_t rj = reject_below(t.reject);
-    tx_pack_lds(t, 0, (uint32_t)(((len + 63) & ~(size_t)63) / 8), win);
-    __syncthreads();
-    const uint32_t q = threadIdx.x & 3, quad = threadIdx.x >> 2;
-    const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
-    const size_t n = nchunks_of(len);
-    for (uint32_t counter = 0;; ++counter) {  // every thread counts the same passes
-        if (n == 1) {
-            if (threadIdx.x < 4) {
-                uint32_t ca, cb;
-                chunk_cv_quad_lds(win, 0, len, 0, key.w, flags, true, sel, ca, cb);
-                dig[q] = ca;
-                dig[4 + q] = cb;
-            }
-        } else {
-            if (threadIdx.x < 64 && quad < n) {
-                uint32_t ca, cb;
-                chunk_cv_quad_lds(win, 0, len, quad, key.w, flags, false, sel, ca, cb);
-                sa[quad * 8 + q] = ca;
-                sa[quad * 8 + 4 + q] = cb;
-            }
-            __syncthreads();
-            const uint32_t* r = merge_lds(sa, sb, (uint32_t)n, key.w, flags, true);
-            if (threadIdx.x < 8) dig[threadIdx.x] = r[threadIdx.x];
-        }
-        __syncthreads();
-        if (threadIdx.x < 64) {  // every lane of wave 0 holds the digest; see k_b3_root
-            uint32_t h[8];
-            for (int j = 0; j < 8; ++j) h[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)dig[j], 0xE4, 0xF, 0xF, false);
-            const zq x = zq_from_digest(h);
-            if (candidate_taken(x, counter, rj)) {
-                const zq xi = want_inv ? zq_inv_divsteps(x) : x;
-                if (threadIdx.x == 0) {
-                    chal_out[0] = x;
-                    if (want_inv) chal_out[1] = xi;
-                }
-            }
-        }
-        if (threadIdx.x == 0) {
-            again = 0;
-            uint32_t h[8];
-            for (int j = 0; j < 8; ++j) h[j] = dig[j];
-            if (!candidate_taken(zq_from_digest(h), counter, rj)) {
-                if (counter + 1 >= RESAMPLE_LIMIT) {
-                    atomicOr(err, ERR_RESAMPLE);
-                    exhausted_challenge(chal_out, want_inv);
-                } else {
-                    const uint32_t c1 = counter + 1;
-                    uint8_t* wb = reinterpret_cast<uint8_t*>(win);
-                    for (uint32_t b = 0; b < 4; ++b) wb[len - 4 + b] = (uint8_t)(c1 >> (8 * b));
-                    again = 1;
-                }
-            }
-        }
-        __syncthreads();
-        if (!again) break;  // uniform: every thread reads the same shared flag
-    }
+    (void)wg_tx_challenge(t, len, key.w, flags, sm, jp(chal_out, jo), jp(err, jo), want_inv);
 }
 
 static void launch_b3_tx_chunks(const TranscriptMsg& t, size_t len, const uint32_t key[8], uint32_t flags,

@@ -133,9 +133,12 @@ void launch_fold_rows_zq(const RowSrc& src, size_t rows, int log_m, hipStream_t 
 // y^-1) per round; vmask / pmask: the rounds (bit r for round r, counted from chal) that folded
 // rows / coefficients.
 // Early: the first masked rounds fold G (2^log_hi * 2^log_lo rows of 2^log_n) into
-// H[2^log_lo][K][2^log_nt]; 2^log_hi <= EARLY_MAX_HI and 2^log_nt <= EARLY_MAX_NT.
+// H[2^log_lo][K][2^log_nt] (formed as early_chunks(log_n) partial blocks over coefficient chunks,
+// then added into the first: H holds that many blocks); 2^log_hi <= EARLY_MAX_HI and
+// 2^log_nt <= EARLY_MAX_NT.
 constexpr uint32_t EARLY_MAX_HI = 128;
 constexpr uint32_t EARLY_MAX_NT = 256;
+uint32_t early_chunks(int log_n);
 void launch_gen_contract_early(const zq* G, bool g_shared, const zq* chal, uint32_t vmask, uint32_t pmask,
                                int log_hi, int log_lo, int log_n, int log_nt, zq* H, hipStream_t s,
                                const Jobs& jb = Jobs{});
@@ -178,6 +181,7 @@ struct SmallStage {
     uint32_t target;
 };
 void launch_stage_small(const SmallStage& st, const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
+
 
 // lift K commitment slices (checking them against the update of `cu`, written to com_out) and
 // lr_slices cross-term slices (to out_lr) in one launch; see k_lift_check
@@ -248,6 +252,42 @@ void device_hash(const uint8_t* msg, size_t len, const uint32_t key[8], uint32_t
                  uint32_t* cv_scratch, size_t cv_scratch_words, uint32_t* digest_out, zq* chal_out,
                  uint32_t* err_flags, hipStream_t s, const Jobs& jb = Jobs{}, const uint64_t* dlen = nullptr);
 size_t device_hash_scratch_words(size_t len);
+
+// The persistent tail of a proof (k_tail): every stage of degree <= 32 (the last five rounds, ten
+// stages) and, optionally, the verifier's replay of them, in one launch per batch. The
+// descriptor lives in device memory and holds job 0's addresses; jobs are offset as elsewhere.
+constexpr int TAIL_MAX_STAGES = 10;
+constexpr size_t TAIL_MAX_M = 8;  // the tail's rounds have ring degree <= 8
+// One replay step of the verifier: the challenge of `tx`, then the commitment update
+// out = c + s^-1 l + s r (poly: even(c) + ...) at degree 2^log_m, with the candidate inverse at
+// hint + hint_off checked before use (the update computes the inverse itself when it fails).
+struct TailVerStep {
+    TranscriptMsg tx;
+    const zq* c;
+    const zq* l;
+    const zq* r;
+    zq* scal;         // the verifier's challenge slot: [0] = s, [1] = s^-1
+    zq* out;
+    const zq* hint;   // the prover's challenge slot of the same stage
+    int poly;
+    int log_m;
+};
+struct TailDesc {
+    uint32_t nstages;           // 2 x rounds, at most TAIL_MAX_STAGES
+    uint32_t* ctr;              // stages the prover has published (zero at the start of the job)
+    uint32_t* vflags;           // the verifier's challenge error word
+    SmallStage st[TAIL_MAX_STAGES];      // the prover's stages (ctr and target unused)
+    TranscriptMsg ptx[TAIL_MAX_STAGES];  // the challenge message of each stage
+    TailVerStep vs[TAIL_MAX_STAGES];
+};
+// dynamic LDS of a tail launch over the stages of `d`; *msg_zq: the Z_q values of the largest
+// message the launch keeps in LDS
+size_t tail_lds_bytes(const TailDesc& d, uint32_t* msg_zq);
+// allows the tail kernel that much dynamic LDS (above the default limit); not during capture
+hipError_t tail_set_lds_limit(size_t lds_bytes);
+// grid (1 + verifier, jobs); hint_off selects the candidate inverse (1: the prover's inverse)
+void launch_tail(const TailDesc* d_desc, size_t lds_bytes, uint32_t msg_zq, int verifier, int hint_off,
+                 const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
 
 // generator expansion into Z_q rows: g[(i*K + k)*m + c] (src/commitment/sis.rs:212-245)
 // gseed: 8 key words in device memory

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "dmx_b3quad.hpp"
 #include "dmx_blake3.hpp"
 #include "dmx_kernels.hpp"
 
@@ -1149,11 +1150,12 @@ void launch_lift_check(const uint32_t* res_a, int log_ma, const ComUpd& cu, zq* 
 constexpr int SMALL_THREADS = 1024;
 
 // forward layers (as k_ntt_fwd_rows) on cnt slices of 2^log_n words; slice s uses prime s % 8
+template <int NT>
 __device__ void lds_ntt_fwd_slices_pp(uint32_t* sh, int cnt, int log_n, const NttTables& tw) {
     const int n = 1 << log_n, half = n >> 1;
     for (int len = half; len >= 1; len >>= 1) {
         const int lg = __builtin_ctz(len);
-        for (int t = threadIdx.x; t < cnt * half; t += SMALL_THREADS) {
+        for (int t = threadIdx.x; t < cnt * half; t += NT) {
             const int sl = t >> (log_n - 1), b = t & (half - 1);
             const int pi = sl & (NPRIMES - 1);
             const int blk = b >> lg;
@@ -1171,11 +1173,12 @@ __device__ void lds_ntt_fwd_slices_pp(uint32_t* sh, int cnt, int log_n, const Nt
 }
 
 // inverse layers (as lds_intt_slices, no scaling) on cnt slices; slice s uses prime s % 8
+template <int NT>
 __device__ void lds_intt_slices_pp(uint32_t* sh, int cnt, int log_n, const NttTables& tw) {
     const int n = 1 << log_n, half = n >> 1;
     for (int len = 1; len < n; len <<= 1) {
         const int lg = __builtin_ctz(len);
-        for (int t = threadIdx.x; t < cnt * half; t += SMALL_THREADS) {
+        for (int t = threadIdx.x; t < cnt * half; t += NT) {
             const int sl = t >> (log_n - 1), b = t & (half - 1);
             const int pi = sl & (NPRIMES - 1);
             const int blk = b >> lg;
@@ -1191,52 +1194,62 @@ __device__ void lds_intt_slices_pp(uint32_t* sh, int cnt, int log_n, const NttTa
     }
 }
 
-__global__ void __launch_bounds__(SMALL_THREADS) k_stage_small(SmallStage st, NttTables tw, size_t js) {
-    critical_path_priority();
-    extern __shared__ uint32_t sh[];
-    __shared__ zq s_x, s_xi, s_u;
-    const size_t jo = job_off(js);
-    if (blockIdx.x == 0) {  // the challenge inverse, published by a release increment
-        if (threadIdx.x < 64) {
-            zq* chal = jp(st.it.chal, jo);
-            const zq x = chal[0];  // every lane loads it: a per-lane register, as the inverse wants
-            const zq xi = zq_inv_divsteps(x);
-            const zq u = st.it.u_prev ? zq_mul(*jp(st.it.u_prev, jo), xi) : xi;
-            if (threadIdx.x == 0) {
-                chal[1] = xi;
-                *jp(st.it.u_out, jo) = u;
-                __hip_atomic_fetch_add(jp(st.ctr, jo), 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
+// a fold row's coefficient c with the challenge s given by value (the fold modes of row_value)
+__device__ __forceinline__ zq fold_value(const RowSrc& src, size_t row, uint32_t c, int log_m, const zq& s) {
+    const size_t m = (size_t)1 << log_m;
+    switch (src.mode) {
+        case ROW_VFOLD:
+            return zq_add(load_zq(src.a + row * m + c), zq_mul(load_zq(src.a + (row + src.b_off) * m + c), s));
+        case ROW_VFOLD_L:
+            return zq_add(zq_mul(load_zq(src.a + row * m + c), s), load_zq(src.a + (row + src.b_off) * m + c));
+        case ROW_PFOLD_W: {
+            const zq* base = src.a + row * (2 * m);
+            return zq_add(load_zq(base + 2 * c), zq_mul(load_zq(base + 2 * c + 1), s));
         }
-        return;
+        case ROW_PFOLD_G: {
+            const zq* base = src.a + row * (2 * m);
+            const zq xo = c ? load_zq(base + 2 * c - 1) : zq_neg(load_zq(base + 2 * m - 1));
+            return zq_add(load_zq(base + 2 * c), zq_mul(xo, s));
+        }
+        case ROW_PFOLD_GL: {
+            const zq* base = src.a + row * (2 * m);
+            const zq xo = c ? load_zq(base + 2 * c - 1) : zq_neg(load_zq(base + 2 * m - 1));
+            return zq_add(zq_mul(load_zq(base + 2 * c), s), xo);
+        }
+        default:
+            return load_zq(src.a + row * m + c);
     }
+}
+
+// Steps 1-4 of a small stage for coordinate k, with the stage's challenge s: fold the witness
+// rows and coordinate k's generator rows, reduce and transform them in LDS (sh), run the MAC
+// pass, transform the sums back and scale them. Leaves the commitment slots and cross terms in
+// sh for small_stage_lift; the caller synchronises between the two.
+template <int NT>
+__device__ void small_stage_compute(const SmallStage& st, const NttTables& tw, size_t jo, uint32_t k, uint32_t* sh,
+                                    const zq& s) {
     const int poly = st.poly;
     const int lnd = poly ? st.log_m - 1 : st.log_m;  // degree of the transforms
     const int nd = 1 << lnd;
     const uint32_t H = st.rows;
     const int log_ma = lnd, log_mb = poly ? lnd : lnd - 1;  // commitment / cross-term degrees
-    const int nb = 1 << log_mb;
     const bool lr = !poly || st.cross;
-    const uint32_t k = blockIdx.x - 1;
     RowSrc w = st.w, g = st.g;
     if (!w.a_shared) w.a = jp(w.a, jo);
     if (!g.a_shared) g.a = jp(g.a, jo);
     w.dst = jp(w.dst, jo);
     g.dst = jp(g.dst, jo);
-    w.scal = jp(w.scal, jo);
-    g.scal = jp(g.scal, jo);
-    const uint32_t key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // 1) rows 0..H-1 witness, H..2H-1 generators of coordinate k, as residues T[row][p][c]
     uint32_t* T = sh;
-    for (uint32_t e = threadIdx.x; e < (2 * H) << lnd; e += SMALL_THREADS) {
+    for (uint32_t e = threadIdx.x; e < (2 * H) << lnd; e += NT) {
         const uint32_t rl = e >> lnd, c = e & (nd - 1);
         zq x;
         if (rl < H) {
-            x = row_value(w, key, rl, c, lnd);
+            x = fold_value(w, rl, c, lnd, s);
             if (k == 0 && w.dst) store_zq(w.dst + ((size_t)rl << lnd) + c, x);
         } else {
             const size_t grow = (size_t)(rl - H) * K + k;
-            x = row_value(g, key, grow, c, lnd);
+            x = fold_value(g, grow, c, lnd, s);
             if (g.dst) store_zq(g.dst + (grow << lnd) + c, x);
         }
 #pragma unroll
@@ -1244,7 +1257,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_stage_small(SmallStage st, Nt
     }
     __syncthreads();
     // 2) forward transforms
-    lds_ntt_fwd_slices_pp(T, (int)(2 * H) * NPRIMES, lnd, tw);
+    lds_ntt_fwd_slices_pp<NT>(T, (int)(2 * H) * NPRIMES, lnd, tw);
     // 3) the MAC pass of coordinate k: O = commitment slots [p][nd], then L [p][nb], R [p][nb]
     uint32_t* O = T + ((size_t)(2 * H) * NPRIMES << lnd);
     uint32_t* OL = O + (NPRIMES << lnd);
@@ -1253,7 +1266,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_stage_small(SmallStage st, Nt
     const uint32_t* Gt = T + ((size_t)H * NPRIMES << lnd);
     if (!poly) {  // split: (S00, S11, S01, S10) per pair, as k_mac_split4 and its finish
         const int hm = nd >> 1;
-        for (int e = threadIdx.x; e < NPRIMES * hm; e += SMALL_THREADS) {
+        for (int e = threadIdx.x; e < NPRIMES * hm; e += NT) {
             const int p = e / hm, t = e - p * hm;
             Acc96 s00, s11, s01, s10;
             s00.zero();
@@ -1286,7 +1299,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_stage_small(SmallStage st, Nt
         }
     } else {  // pairs: the invariant over every row, or with cross terms over the two halves
         const uint32_t hh = st.cross ? H / 2 : H;
-        for (int e = threadIdx.x; e < (NPRIMES << lnd); e += SMALL_THREADS) {
+        for (int e = threadIdx.x; e < (NPRIMES << lnd); e += NT) {
             const int p = e >> lnd, c = e & (nd - 1);
             Acc96 inv, L, R;
             inv.zero();
@@ -1313,38 +1326,37 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_stage_small(SmallStage st, Nt
     }
     __syncthreads();
     // 4) inverse transforms, scaled by the inverse degrees
-    lds_intt_slices_pp(O, NPRIMES, log_ma, tw);
-    if (lr) lds_intt_slices_pp(OL, 2 * NPRIMES, log_mb, tw);
-    for (int e = threadIdx.x; e < (NPRIMES << log_ma); e += SMALL_THREADS) {
+    lds_intt_slices_pp<NT>(O, NPRIMES, log_ma, tw);
+    if (lr) lds_intt_slices_pp<NT>(OL, 2 * NPRIMES, log_mb, tw);
+    for (int e = threadIdx.x; e < (NPRIMES << log_ma); e += NT) {
         const int p = e >> log_ma;
         O[e] = mulc(O[e], tw.invn_m[p * (MAX_LOG_N + 1) + log_ma], tw.invn_mp[p * (MAX_LOG_N + 1) + log_ma], c_pc[p].p);
     }
     if (lr) {
-        for (int e = threadIdx.x; e < (2 * NPRIMES << log_mb); e += SMALL_THREADS) {
+        for (int e = threadIdx.x; e < (2 * NPRIMES << log_mb); e += NT) {
             const int p = (e >> log_mb) & (NPRIMES - 1);
             OL[e] = mulc(OL[e], tw.invn_m[p * (MAX_LOG_N + 1) + log_mb], tw.invn_mp[p * (MAX_LOG_N + 1) + log_mb],
                          c_pc[p].p);
         }
     }
-    // 5) wait (bounded) for the inverse, then lift coordinate k, unscaled by u
-    if (threadIdx.x == 0) {
-        uint32_t spins = 0;
-        while (__hip_atomic_load(jp(st.ctr, jo), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < st.target) {
-            if (++spins > (1u << 22)) {
-                atomicOr(jp(st.flags, jo), STAGE_WAIT_EXPIRED);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        s_x = jp(st.it.chal, jo)[0];
-        s_xi = jp(st.it.chal, jo)[1];
-        s_u = *jp(st.it.u_out, jo);
-    }
-    __syncthreads();
-    const zq u = s_u, cx = s_x, cxi = s_xi;
+}
+
+// Step 5: lift coordinate k of the commitment (checked against the update of st.cu with the
+// scalars (cx, cxi), as k_lift_check does) and of the cross terms, each unscaled by u.
+template <int NT>
+__device__ void small_stage_lift(const SmallStage& st, size_t jo, uint32_t k, uint32_t* sh, const zq& u, const zq& cx,
+                                 const zq& cxi) {
+    const int poly = st.poly;
+    const int lnd = poly ? st.log_m - 1 : st.log_m;
+    const uint32_t H = st.rows;
+    const int log_ma = lnd, log_mb = poly ? lnd : lnd - 1;
+    const int nb = 1 << log_mb;
+    const bool lr = !poly || st.cross;
+    uint32_t* O = sh + ((size_t)(2 * H) * NPRIMES << lnd);
+    uint32_t* OL = O + (NPRIMES << lnd);
     const int na = 1 << log_ma;
     const int cnt = na + (lr ? 2 * nb : 0);
-    for (int e = threadIdx.x; e < cnt; e += SMALL_THREADS) {
+    for (int e = threadIdx.x; e < cnt; e += NT) {
         uint32_t r[NPRIMES];
         if (e < na) {
 #pragma unroll
@@ -1362,6 +1374,45 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_stage_small(SmallStage st, Nt
     }
 }
 
+__global__ void __launch_bounds__(SMALL_THREADS) k_stage_small(SmallStage st, NttTables tw, size_t js) {
+    critical_path_priority();
+    extern __shared__ uint32_t sh[];
+    __shared__ zq s_x, s_xi, s_u;
+    const size_t jo = job_off(js);
+    if (blockIdx.x == 0) {  // the challenge inverse, published by a release increment
+        if (threadIdx.x < 64) {
+            zq* chal = jp(st.it.chal, jo);
+            const zq x = chal[0];  // every lane loads it: a per-lane register, as the inverse wants
+            const zq xi = zq_inv_divsteps(x);
+            const zq u = st.it.u_prev ? zq_mul(*jp(st.it.u_prev, jo), xi) : xi;
+            if (threadIdx.x == 0) {
+                chal[1] = xi;
+                *jp(st.it.u_out, jo) = u;
+                __hip_atomic_fetch_add(jp(st.ctr, jo), 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        return;
+    }
+    const uint32_t k = blockIdx.x - 1;
+    small_stage_compute<SMALL_THREADS>(st, tw, jo, k, sh, *jp(st.w.scal, jo));
+    // wait (bounded) for the inverse, then lift coordinate k, unscaled by u
+    if (threadIdx.x == 0) {
+        uint32_t spins = 0;
+        while (__hip_atomic_load(jp(st.ctr, jo), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < st.target) {
+            if (++spins > (1u << 22)) {
+                atomicOr(jp(st.flags, jo), STAGE_WAIT_EXPIRED);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        s_x = jp(st.it.chal, jo)[0];
+        s_xi = jp(st.it.chal, jo)[1];
+        s_u = *jp(st.it.u_out, jo);
+    }
+    __syncthreads();
+    small_stage_lift<SMALL_THREADS>(st, jo, k, sh, s_u, s_x, s_xi);
+}
+
 void launch_stage_small(const SmallStage& st, const NttTables& tw, hipStream_t s, const Jobs& jb) {
     if (jb.n == 0) return;
     const int lnd = st.poly ? st.log_m - 1 : st.log_m;
@@ -1369,6 +1420,598 @@ void launch_stage_small(const SmallStage& st, const NttTables& tw, hipStream_t s
     const size_t lds = 4 * ((2 * (size_t)st.rows * NPRIMES << lnd) + ((size_t)NPRIMES << lnd) +
                             ((size_t)2 * NPRIMES << (log_mb > 0 ? log_mb : 0)));
     hipLaunchKernelGGL(k_stage_small, dim3(K + 1, jb.n), dim3(SMALL_THREADS), lds, s, st, tw, jb.stride);
+}
+
+// ---------------------------------------------------------------------------------------
+// The persistent tail. In a proof's last rounds a stage's work is small, and as a chain of
+// launches (a challenge kernel, then the stage kernel, each a few microseconds of launch and
+// drain, with waits across hardware queues for the verifier) it costs more than its work; spread
+// over several workgroups, each stage would also pay for hand-offs that write back and invalidate
+// caches. So one workgroup per job runs the last stages, every coordinate: per stage it hashes
+// the transcript message (the previous stage's lift outputs, kept in LDS), then wave 0 inverts the
+// challenge x while the other waves fold the witness rows and, in groups of coordinates that fit
+// LDS, the generator rows with x, transform them, run the MAC pass and transform the sums back;
+// then the whole workgroup lifts every coordinate, unscaled by the running inverse product u. Its
+// re-reads of the rows it stored hit a warm cache. After each stage it publishes with a release
+// increment of the job's stage counter. With `verifier`, a second workgroup replays the same
+// stages (verifier.rs:57-120) behind it: it hashes each message from its own commitment and the
+// prover's cross terms and, once the prover's stage is published, updates its commitment with the
+// checked inverse candidate. The prover never waits for it. Values are those of the multi-launch
+// chain.
+// ---------------------------------------------------------------------------------------
+// Waits until the job's stage counter reaches `target`, bounded by the device's constant-rate
+// clock (s_memrealtime, 100 MHz): after TAIL_WAIT_TICKS the job is flagged and the workgroup goes
+// on, so a launch always ends.
+constexpr uint64_t TAIL_WAIT_TICKS = 50000000;  // 0.5 s
+__device__ __forceinline__ void tail_wait(const uint32_t* ctr, uint32_t target, uint32_t* flags) {
+    if (threadIdx.x == 0) {
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__builtin_amdgcn_s_memrealtime() - t0 > TAIL_WAIT_TICKS) {
+                atomicOr(flags, STAGE_WAIT_EXPIRED);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
+
+// the verifier's commitment update of one replay step (as k_com_update), by the whole workgroup
+__device__ __forceinline__ void tail_com_update(const TailVerStep& vs, const zq& s, const zq* hint, size_t jo) {
+    zq sinv = load_zq(hint);  // uniform: every thread checks the same candidate
+    if (!zq_eq(zq_mul(s, sinv), zq_from_u64(1))) sinv = zq_inv_divsteps(s);
+    zq* scal = jp(vs.scal, jo);
+    if (threadIdx.x == 0) scal[1] = sinv;
+    const zq* c = jp(vs.c, jo);
+    const zq* l = jp(vs.l, jo);
+    const zq* r = jp(vs.r, jo);
+    zq* out = jp(vs.out, jo);
+    const uint32_t m = 1u << vs.log_m;
+    for (uint32_t gid = threadIdx.x; gid < K * m; gid += blockDim.x) {
+        const uint32_t k = gid >> vs.log_m, i = gid & (m - 1);
+        const zq base = vs.poly ? load_zq(c + k * (2 * m) + 2 * i) : load_zq(c + gid);
+        zq v = zq_add(base, zq_mul(load_zq(l + gid), sinv));
+        v = zq_add(v, zq_mul(load_zq(r + gid), s));
+        store_zq(out + gid, v);
+    }
+}
+
+// 512 threads: the challenge hash needs more than the 128 VGPRs a 1024-thread workgroup allows
+constexpr int TAIL_THREADS = 512;
+constexpr int TAIL_COMPUTE_THREADS = TAIL_THREADS - 64;  // waves 1.. while wave 0 inverts
+constexpr size_t TAIL_DESC_LDS = (sizeof(TailDesc) + 15) & ~(size_t)15;
+constexpr size_t TAIL_SCRATCH_BUDGET = 128 * 1024;  // bytes of LDS for one stage's residues
+constexpr int TAIL_MAX_N = 32;                      // the tail's largest transform degree
+
+// The twiddles of degrees <= 32 (the first 32 entries of each prime's nested tables) and the
+// inverse degrees, copied to LDS once per launch.
+struct TailTw {
+    uint32_t z[NPRIMES][4][TAIL_MAX_N];  // zeta_m, zeta_mp, izeta_m, izeta_mp
+    uint32_t inv[NPRIMES][2][8];         // invn_m, invn_mp for log degrees 0..5 (and 2 = 1/4)
+};
+
+// A stage's residue scratch in words, with the generator rows transformed `grp` coordinates at a
+// time: witness rows Wt [H][8][nd], commitment slots O [K][8][nd], cross terms OL [K][2][8][nb],
+// generator rows Gt [grp][H][8][nd].
+__host__ __device__ inline size_t tail_stage_words(const SmallStage& st, uint32_t grp) {
+    const int lnd = st.poly ? st.log_m - 1 : st.log_m;
+    const int log_mb = st.poly ? lnd : lnd - 1;
+    const size_t H = st.rows;
+    return (((H + K + grp * H) * NPRIMES) << lnd) + (((size_t)K * 2 * NPRIMES) << (log_mb > 0 ? log_mb : 0));
+}
+__host__ __device__ inline uint32_t tail_group(const SmallStage& st) {
+    uint32_t g = K;
+    while (g > 1 && 4 * tail_stage_words(st, g) > TAIL_SCRATCH_BUDGET) g >>= 1;
+    return g;
+}
+// the lift outputs kept for the next message: K commitment coordinates and 2K cross-term slices
+__host__ __device__ inline size_t tail_msg_zq(const SmallStage& st) {
+    const int lnd = st.poly ? st.log_m - 1 : st.log_m;
+    const int log_mb = st.poly ? lnd : lnd - 1;
+    return ((size_t)K << lnd) + ((size_t)2 * K << (log_mb > 0 ? log_mb : 0));
+}
+
+// A barrier among the workgroup's waves 1.. (wave 0 computes the challenge inverse meanwhile):
+// every wave adds its arrival to an LDS counter and waits for the count of this phase. All
+// participating waves pass the same barriers in the same order.
+struct SubBarrier {
+    uint32_t* cnt;
+    uint32_t target;
+    __device__ __forceinline__ void sync() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        target += TAIL_COMPUTE_THREADS / 64;
+        if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < target)
+            __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+};
+
+// forward / inverse layers (as lds_ntt_fwd_slices_pp / lds_intt_slices_pp) on cnt slices of the
+// compute waves, twiddles from LDS
+__device__ __forceinline__ void tail_fwd_layers(uint32_t* sh, int cnt, int log_n, const TailTw& tt, SubBarrier& bar) {
+    const int n = 1 << log_n, half = n >> 1;
+    for (int len = half; len >= 1; len >>= 1) {
+        const int lg = __builtin_ctz(len);
+        for (int t = (int)threadIdx.x - 64; t < cnt * half; t += TAIL_COMPUTE_THREADS) {
+            const int sl = t >> (log_n - 1), b = t & (half - 1);
+            const int pi = sl & (NPRIMES - 1);
+            const int blk = b >> lg;
+            const int j = (blk << (lg + 1)) | (b & (len - 1));
+            const int kz = (half >> lg) + blk;
+            const uint32_t p = c_pc[pi].p;
+            uint32_t* q = sh + sl * n;
+            const uint32_t u = q[j];
+            const uint32_t v = mulc(q[j + len], tt.z[pi][0][kz], tt.z[pi][1][kz], p);
+            q[j] = addp(u, v, p);
+            q[j + len] = subp(u, v, p);
+        }
+        bar.sync();
+    }
+}
+__device__ __forceinline__ void tail_inv_layers(uint32_t* sh, int cnt, int log_n, const TailTw& tt, SubBarrier& bar) {
+    const int n = 1 << log_n, half = n >> 1;
+    for (int len = 1; len < n; len <<= 1) {
+        const int lg = __builtin_ctz(len);
+        for (int t = (int)threadIdx.x - 64; t < cnt * half; t += TAIL_COMPUTE_THREADS) {
+            const int sl = t >> (log_n - 1), b = t & (half - 1);
+            const int pi = sl & (NPRIMES - 1);
+            const int blk = b >> lg;
+            const int j = (blk << (lg + 1)) | (b & (len - 1));
+            const int kz = (half >> lg) + blk;
+            const uint32_t p = c_pc[pi].p;
+            uint32_t* q = sh + sl * n;
+            const uint32_t u = q[j], v = q[j + len];
+            q[j] = addp(u, v, p);
+            q[j + len] = mulc(subp(u, v, p), tt.z[pi][2][kz], tt.z[pi][3][kz], p);
+        }
+        bar.sync();
+    }
+}
+
+// A fold's operands, loaded apart from the combination so that a thread's loads for several
+// coefficients are in flight together.
+struct FoldIn {
+    zq a, b;
+};
+__device__ __forceinline__ FoldIn fold_load(const RowSrc& src, size_t row, uint32_t c, int log_m) {
+    const size_t m = (size_t)1 << log_m;
+    FoldIn o;
+    switch (src.mode) {
+        case ROW_VFOLD:
+        case ROW_VFOLD_L:
+            o.a = load_zq(src.a + row * m + c);
+            o.b = load_zq(src.a + (row + src.b_off) * m + c);
+            break;
+        case ROW_PFOLD_W: {
+            const zq* base = src.a + row * (2 * m);
+            o.a = load_zq(base + 2 * c);
+            o.b = load_zq(base + 2 * c + 1);
+            break;
+        }
+        case ROW_PFOLD_G:
+        case ROW_PFOLD_GL: {  // b: odd[c - 1], or odd[m - 1] (negated in fold_combine) at c = 0
+            const zq* base = src.a + row * (2 * m);
+            o.a = load_zq(base + 2 * c);
+            o.b = load_zq(base + (c ? 2 * c - 1 : 2 * m - 1));
+            break;
+        }
+        default:
+            o.a = load_zq(src.a + row * m + c);
+            o.b = zq_zero();
+    }
+    return o;
+}
+// the same values as fold_value
+__device__ __forceinline__ zq fold_combine(int mode, const FoldIn& o, uint32_t c, const zq& s) {
+    switch (mode) {
+        case ROW_VFOLD:
+        case ROW_PFOLD_W:
+            return zq_add(o.a, zq_mul(o.b, s));
+        case ROW_VFOLD_L:
+            return zq_add(zq_mul(o.a, s), o.b);
+        case ROW_PFOLD_G:
+            return zq_add(o.a, zq_mul(c ? o.b : zq_neg(o.b), s));
+        case ROW_PFOLD_GL:
+            return zq_add(zq_mul(o.a, s), c ? o.b : zq_neg(o.b));
+        default:
+            return o.a;
+    }
+}
+
+// `total` fold outputs by the compute waves, FOLD_BATCH coefficients per thread at a time: output e
+// is row row_of(e) coefficient e mod nd, stored at dst (when set) and reduced into the residue
+// slots T[(slot_of(e) * 8 + q) * nd + c].
+constexpr int FOLD_BATCH = 4;
+template <class RowOf, class SlotOf>
+__device__ __forceinline__ void tail_fold(const RowSrc& src, uint32_t total, int lnd, const zq& x, RowOf row_of, SlotOf slot_of,
+                          uint32_t* T) {
+    const uint32_t nd = 1u << lnd;
+    for (uint32_t e0 = threadIdx.x - 64; e0 < total; e0 += TAIL_COMPUTE_THREADS * FOLD_BATCH) {
+        FoldIn in[FOLD_BATCH];
+#pragma unroll
+        for (int i = 0; i < FOLD_BATCH; ++i) {
+            const uint32_t e = e0 + i * TAIL_COMPUTE_THREADS;
+            if (e < total) in[i] = fold_load(src, row_of(e), e & (nd - 1), lnd);
+        }
+#pragma unroll
+        for (int i = 0; i < FOLD_BATCH; ++i) {
+            const uint32_t e = e0 + i * TAIL_COMPUTE_THREADS;
+            if (e < total) {
+                const uint32_t c = e & (nd - 1);
+                const size_t row = row_of(e);
+                const zq v = fold_combine(src.mode, in[i], c, x);
+                if (src.dst) store_zq(src.dst + (row << lnd) + c, v);
+                const uint32_t sl = slot_of(e);
+#pragma unroll
+                for (int q = 0; q < NPRIMES; ++q) T[((sl * NPRIMES + q) << lnd) + c] = zq_mod_p_centred(v, c_pc[q]);
+            }
+        }
+    }
+}
+
+// Fold, transforms, MAC pass and inverse transforms of one stage for every coordinate, by the
+// compute waves (the multi-coordinate form of small_stage_compute); leaves O and OL in T for
+// tail_stage_lift.
+// `w` and `g` are the stage's row sources with every address resolved (LDS or the job's memory).
+__device__ __forceinline__ void tail_stage_compute(const SmallStage& st, const RowSrc& w, const RowSrc& g, uint32_t* T, const zq& x,
+                                   const TailTw& tt, SubBarrier& bar) {
+    const int poly = st.poly;
+    const int lnd = poly ? st.log_m - 1 : st.log_m;
+    const int nd = 1 << lnd;
+    const uint32_t H = st.rows;
+    const int log_ma = lnd, log_mb = poly ? lnd : lnd - 1;
+    const bool lr = !poly || st.cross;
+    const uint32_t grp = tail_group(st);
+    uint32_t* Wt = T;
+    uint32_t* O = Wt + ((H * NPRIMES) << lnd);
+    uint32_t* OL = O + ((K * NPRIMES) << lnd);
+    uint32_t* Gt = OL + (((uint32_t)K * 2 * NPRIMES) << (log_mb > 0 ? log_mb : 0));
+    // 1) the witness rows (stored: the next stage folds them)
+    tail_fold(w, H << lnd, lnd, x, [=](uint32_t e) { return (size_t)(e >> lnd); }, [=](uint32_t e) { return e >> lnd; },
+              Wt);
+    bar.sync();
+    tail_fwd_layers(Wt, (int)H * NPRIMES, lnd, tt, bar);
+    // 2) generator rows, grp coordinates at a time, and their MAC passes
+    for (uint32_t k0 = 0; k0 < K; k0 += grp) {
+        tail_fold(
+            g, (grp * H) << lnd, lnd, x,
+            [=](uint32_t e) {  // rk = kk * H + row
+                const uint32_t rk = e >> lnd, kk = rk / H;
+                return (size_t)(rk - kk * H) * K + k0 + kk;
+            },
+            [=](uint32_t e) { return e >> lnd; }, Gt);
+        bar.sync();
+        tail_fwd_layers(Gt, (int)(grp * H) * NPRIMES, lnd, tt, bar);
+        if (!poly) {  // split: (S00, S11, S01, S10) per pair, as k_mac_split4 and its finish
+            const int hm = nd >> 1;
+            for (int e = (int)threadIdx.x - 64; e < (int)grp * NPRIMES * hm; e += TAIL_COMPUTE_THREADS) {
+                const int kk = e / (NPRIMES * hm), r0 = e - kk * NPRIMES * hm;
+                const int p = r0 / hm, t = r0 - p * hm;
+                const uint32_t k = k0 + kk;
+                const uint32_t* Gk = Gt + ((kk * H * NPRIMES) << lnd);
+                Acc96 s00, s11, s01, s10;
+                s00.zero();
+                s11.zero();
+                s01.zero();
+                s10.zero();
+                for (uint32_t i = 0; i < H; ++i) {
+                    const uint32_t* wr = Wt + ((i * NPRIMES + p) << lnd) + 2 * t;
+                    const uint32_t* gr = Gk + ((i * NPRIMES + p) << lnd) + 2 * t;
+                    s00.mac(wr[0], gr[0]);
+                    s11.mac(wr[1], gr[1]);
+                    s01.mac(wr[0], gr[1]);
+                    s10.mac(wr[1], gr[0]);
+                }
+                const PrimeConsts pc = c_pc[p];
+                const uint32_t P = pc.p;
+                const uint32_t a00 = reduce_u96(s00.lo, s00.hi, pc), a11 = reduce_u96(s11.lo, s11.hi, pc);
+                const uint32_t a01 = reduce_u96(s01.lo, s01.hi, pc), a10 = reduce_u96(s10.lo, s10.hi, pc);
+                uint32_t* Ok = O + ((k * NPRIMES) << lnd);
+                Ok[(p << lnd) + 2 * t] = a00;
+                Ok[(p << lnd) + 2 * t + 1] = a11;
+                const uint32_t d = subp(a00, a11, P);
+                const uint32_t Lp = addp(subp(d, a01, P), a10, P);
+                const uint32_t Rp = subp(addp(d, a01, P), a10, P);
+                const uint32_t kz = (uint32_t)hm + t;
+                const uint32_t i4m = tt.inv[p][0][2], i4mp = tt.inv[p][1][2];
+                uint32_t* OLk = OL + ((k * 2 * NPRIMES) << log_mb);
+                OLk[(p << log_mb) + t] = mulc(mulc(Lp, tt.z[p][0][kz], tt.z[p][1][kz], P), i4m, i4mp, P);
+                OLk[((NPRIMES + p) << log_mb) + t] = mulc(mulc(Rp, tt.z[p][2][kz], tt.z[p][3][kz], P), i4m, i4mp, P);
+            }
+        } else {  // pairs: the invariant over every row, or with cross terms over the two halves
+            const uint32_t hh = st.cross ? H / 2 : H;
+            for (int e = (int)threadIdx.x - 64; e < (int)((grp * NPRIMES) << lnd); e += TAIL_COMPUTE_THREADS) {
+                const int kk = e >> (lnd + 3), r0 = e & ((NPRIMES << lnd) - 1);
+                const int p = r0 >> lnd, c = r0 & (nd - 1);
+                const uint32_t k = k0 + kk;
+                const uint32_t* Gk = Gt + ((kk * H * NPRIMES) << lnd);
+                Acc96 inv, L, R;
+                inv.zero();
+                L.zero();
+                R.zero();
+                for (uint32_t i = 0; i < hh; ++i) {
+                    const uint32_t w0 = Wt[((i * NPRIMES + p) << lnd) + c], g0 = Gk[((i * NPRIMES + p) << lnd) + c];
+                    inv.mac(w0, g0);
+                    if (st.cross) {
+                        const uint32_t w1 = Wt[(((i + hh) * NPRIMES + p) << lnd) + c];
+                        const uint32_t g1 = Gk[(((i + hh) * NPRIMES + p) << lnd) + c];
+                        inv.mac(w1, g1);
+                        L.mac(g1, w0);
+                        R.mac(g0, w1);
+                    }
+                }
+                const PrimeConsts pc = c_pc[p];
+                O[((k * NPRIMES) << lnd) + r0] = reduce_u96(inv.lo, inv.hi, pc);
+                if (st.cross) {
+                    uint32_t* OLk = OL + ((k * 2 * NPRIMES) << log_mb);  // poly: log_mb == lnd
+                    OLk[r0] = reduce_u96(L.lo, L.hi, pc);
+                    OLk[(NPRIMES << lnd) + r0] = reduce_u96(R.lo, R.hi, pc);
+                }
+            }
+        }
+        bar.sync();
+    }
+    // 3) inverse transforms, scaled by the inverse degrees
+    tail_inv_layers(O, K * NPRIMES, log_ma, tt, bar);
+    if (lr) tail_inv_layers(OL, K * 2 * NPRIMES, log_mb, tt, bar);
+    for (int e = (int)threadIdx.x - 64; e < ((K * NPRIMES) << log_ma); e += TAIL_COMPUTE_THREADS) {
+        const int p = (e >> log_ma) & (NPRIMES - 1);
+        O[e] = mulc(O[e], tt.inv[p][0][log_ma], tt.inv[p][1][log_ma], c_pc[p].p);
+    }
+    if (lr) {
+        for (int e = (int)threadIdx.x - 64; e < ((K * 2 * NPRIMES) << log_mb); e += TAIL_COMPUTE_THREADS) {
+            const int p = (e >> log_mb) & (NPRIMES - 1);
+            OL[e] = mulc(OL[e], tt.inv[p][0][log_mb], tt.inv[p][1][log_mb], c_pc[p].p);
+        }
+    }
+}
+
+// Lifts of every coordinate (as small_stage_lift with commit_check), each unscaled by u, by the
+// whole workgroup; a commitment coefficient's update operands (cc, cl, cr: the previous stage's
+// commitment and cross terms) are loaded before its lift. The commitment goes to msg only (the
+// next stage's transcript message and update operands; nothing reads it after the tail), the
+// cross terms to the round record olr and after the commitment in msg.
+__device__ __forceinline__ void tail_stage_lift(const SmallStage& st, const zq* cc, const zq* cl, const zq* cr, zq* olr,
+                                uint32_t* flags, const uint32_t* T, zq* msg, const zq& u, const zq& cx, const zq& cxi) {
+    const int poly = st.poly;
+    const int lnd = poly ? st.log_m - 1 : st.log_m;
+    const uint32_t H = st.rows;
+    const int log_ma = lnd, log_mb = poly ? lnd : lnd - 1;
+    const bool lr = !poly || st.cross;
+    const uint32_t* O = T + ((H * NPRIMES) << lnd);
+    const uint32_t* OL = O + ((K * NPRIMES) << lnd);
+    const int na = 1 << log_ma, nb = lr ? 1 << log_mb : 0;
+    const int ta = K * na;
+    for (int e = threadIdx.x; e < ta + 2 * K * nb; e += TAIL_THREADS) {
+        uint32_t r[NPRIMES];
+        if (e < ta) {  // the invariant: C' = c + s^-1 l + s r (poly: even(c) + ...), as commit_check
+            const int k = e >> log_ma, i = e & (na - 1);
+            const zq base = poly ? load_zq(cc + (size_t)k * (2 * na) + 2 * i) : load_zq(cc + e);
+            const zq lv = load_zq(cl + e), rv = load_zq(cr + e);
+#pragma unroll
+            for (int p = 0; p < NPRIMES; ++p) r[p] = O[((k * NPRIMES + p) << log_ma) + i];
+            const zq v = zq_mul(crt_lift_residues(r), u);
+            const zq ex = zq_add(zq_add(base, zq_mul(lv, cxi)), zq_mul(rv, cx));
+            msg[e] = ex;
+            if (!zq_eq(v, ex)) atomicOr(flags, st.bit);
+        } else {
+            const int f = e - ta, k = f / (2 * nb), side = (f / nb) & 1, c = f & (nb - 1);
+#pragma unroll
+            for (int p = 0; p < NPRIMES; ++p) r[p] = OL[(((k * 2 + side) * NPRIMES + p) << log_mb) + c];
+            const size_t slot = side ? K + k : k;
+            const zq v = zq_mul(crt_lift_residues(r), u);
+            store_zq(olr + (slot << log_mb) + c, v);
+            msg[ta + (slot << log_mb) + c] = v;
+        }
+    }
+}
+
+// LDS of a tail launch: the descriptor, the twiddles, the message kept between stages, then the
+// stage scratch (shared with the challenge's window)
+constexpr size_t TAIL_TW_LDS = (sizeof(TailTw) + 15) & ~(size_t)15;
+
+// The folded rows between stages, in LDS: X holds a vector stage's input (m rows of degree m for
+// the witness, then K m rows for the generators; m <= TAIL_MAX_M), Y a poly stage's input (half
+// the rows); a vector stage folds X into Y, a poly stage Y into the next round's X.
+constexpr size_t TAIL_STATE_X = TAIL_MAX_M * TAIL_MAX_M * (1 + K);
+constexpr size_t TAIL_STATE_Y = TAIL_STATE_X / 2;
+
+// The LDS regions of the prover's workgroup (see k_tail).
+struct TailProverLds {
+    zq* msgs;       // two lift outputs: stage s writes msgs[s & 1], stage s + 1 reads it
+    uint32_t msg_zq;
+    zq* stX;        // folded rows: a vector stage's input, a poly stage's output
+    zq* stY;        // a vector stage's output, a poly stage's input
+    uint32_t* work; // stage scratch, shared with the challenge window
+    const uint8_t* seeds;
+    const uint32_t* reject;
+    zq* s_xi;
+    zq* s_u;
+};
+
+// One prover stage of the tail. The first stage reads the job's rows, commitment and cross terms
+// in memory; later ones only LDS (the rows folded by the stage before, the previous lift
+// outputs), instantiated separately so that every access there is an LDS instruction, which
+// waits for LDS alone and not for the global stores still in flight (the round records).
+template <bool FIRST>
+__device__ __forceinline__ void tail_prover_stage(uint32_t s, const TailDesc& D, size_t jo, const TailProverLds& L,
+                                                  WgChal& wc, const TailTw& tt, SubBarrier& bar,
+                                                  const uint32_t key[8], zq& u) {
+    const SmallStage& st = D.st[s];
+    uint32_t* flags = jp(st.flags, jo);
+    zq* const prev = L.msgs + (size_t)((s + 1) & 1) * L.msg_zq;
+    zq* const next = L.msgs + (size_t)(s & 1) * L.msg_zq;
+    TranscriptMsg t = D.ptx[s];
+    tx_to_job(t, jo);
+    t.epoch_seed = L.seeds;
+    t.file_id = L.seeds + 32;
+    t.reject = L.reject;
+    // the update operands: the previous stage's commitment and cross terms
+    const zq *cc, *cl, *cr;
+    if (FIRST) {
+        cc = jp(st.cu.c, jo);
+        cl = jp(st.cu.l, jo);
+        cr = jp(st.cu.r, jo);
+    } else {  // the message is the previous stage's lift outputs (tail_stage_lift)
+        t.mods[0] = prev;
+        t.mods[1] = prev + (size_t)K * t.lens[0];
+        t.mods[2] = t.mods[1] + (size_t)K * t.lens[1];
+        cc = t.mods[0];
+        cl = t.mods[1];
+        cr = t.mods[2];
+    }
+    // rows: folded from the job's rows (first stage) or the LDS state, into the LDS state
+    const int lnd = st.poly ? st.log_m - 1 : st.log_m;
+    const size_t w_in = (size_t)(st.poly ? st.rows : 2 * st.rows) << st.log_m, w_out = (size_t)st.rows << lnd;
+    zq* const in = st.poly ? L.stY : L.stX;
+    zq* const out = st.poly ? L.stX : L.stY;
+    RowSrc w = st.w, g = st.g;
+    if (FIRST) {
+        if (!w.a_shared) w.a = jp(w.a, jo);
+        if (!g.a_shared) g.a = jp(g.a, jo);
+    } else {
+        w.a = in;
+        g.a = in + w_in;
+    }
+    w.dst = out;
+    g.dst = out + w_out;
+    const size_t len = tx_message_len(t.label_len, t.lens);
+    zq* chal = jp(st.it.chal, jo);
+    const zq x = wg_tx_challenge(t, len, key, 0, wc, chal, flags, 0);
+    __syncthreads();  // every thread holds x; the window's LDS is reused below
+    if (threadIdx.x < 64) {  // wave 0 inverts x (every lane: a per-lane register, as the inverse wants)
+        const zq xi = zq_inv_divsteps(x);
+        const zq un = zq_mul(u, xi);
+        if (threadIdx.x == 0) {
+            *L.s_xi = xi;
+            *L.s_u = un;
+            chal[1] = xi;
+            *jp(st.it.u_out, jo) = un;
+        }
+    } else {  // the other waves fold, transform and run the MAC pass meanwhile
+        tail_stage_compute(st, w, g, L.work, x, tt, bar);
+    }
+    __syncthreads();
+    u = *L.s_u;
+    tail_stage_lift(st, cc, cl, cr, jp(st.out_lr, jo), flags, L.work, next, u, x, *L.s_xi);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_fetch_add(jp(D.ctr, jo), 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void __launch_bounds__(TAIL_THREADS) k_tail(const TailDesc* __restrict__ gdesc, NttTables tw, size_t js,
+                                                        int hint_off, uint32_t msg_zq) {
+    critical_path_priority();
+    extern __shared__ __align__(16) uint32_t dsh[];
+    __shared__ zq s_xi, s_u;
+    __shared__ uint32_t s_bar;
+    __shared__ __align__(16) uint8_t s_seeds[64];  // this workgroup's epoch_seed | file_id
+    __shared__ uint32_t s_reject;                 // and its resampling test count
+    const size_t jo = job_off(js);
+    TailTw& tt = *reinterpret_cast<TailTw*>(dsh + TAIL_DESC_LDS / 4);
+    {  // the descriptor, the twiddles and the transcript seeds into LDS
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(gdesc);
+        for (uint32_t i = threadIdx.x; i < sizeof(TailDesc) / 4; i += blockDim.x) dsh[i] = src[i];
+        for (uint32_t i = threadIdx.x; i < NPRIMES * 4 * TAIL_MAX_N; i += blockDim.x) {
+            const uint32_t p = i / (4 * TAIL_MAX_N), a = (i / TAIL_MAX_N) & 3, k = i & (TAIL_MAX_N - 1);
+            const uint32_t* t = a == 0 ? tw.zeta_m : a == 1 ? tw.zeta_mp : a == 2 ? tw.izeta_m : tw.izeta_mp;
+            tt.z[p][a][k] = t[p * TW_SIZE + k];
+        }
+        if (threadIdx.x < NPRIMES * 2 * 8) {
+            const uint32_t p = threadIdx.x >> 4, a = (threadIdx.x >> 3) & 1, l = threadIdx.x & 7;
+            const uint32_t* t = a ? tw.invn_mp : tw.invn_m;
+            tt.inv[p][a][l] = t[p * (MAX_LOG_N + 1) + l];
+        }
+        if (threadIdx.x == 0) s_bar = 0;
+    }
+    __syncthreads();
+    const TailDesc& D = *reinterpret_cast<const TailDesc*>(dsh);
+    {
+        const TranscriptMsg& t0 = blockIdx.x == 0 ? D.ptx[0] : D.vs[0].tx;
+        if (threadIdx.x < 8) {
+            const uint32_t* e = reinterpret_cast<const uint32_t*>(jp(t0.epoch_seed, jo));
+            const uint32_t* f = reinterpret_cast<const uint32_t*>(jp(t0.file_id, jo));
+            reinterpret_cast<uint32_t*>(s_seeds)[threadIdx.x] = e[threadIdx.x];
+            reinterpret_cast<uint32_t*>(s_seeds)[8 + threadIdx.x] = f[threadIdx.x];
+        }
+        if (threadIdx.x == 0) s_reject = t0.reject ? *jp(t0.reject, jo) : 0u;
+    }
+    __syncthreads();
+    zq* const msgs = reinterpret_cast<zq*>(dsh + (TAIL_DESC_LDS + TAIL_TW_LDS) / 4);  // two messages
+    zq* const stX = msgs + 2 * (size_t)msg_zq;
+    zq* const stY = stX + TAIL_STATE_X;
+    uint32_t* work = reinterpret_cast<uint32_t*>(stY + TAIL_STATE_Y);
+    WgChal& wc = *reinterpret_cast<WgChal*>(work);
+    const uint32_t S = D.nstages;
+    uint32_t key[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) key[i] = b3::iv(i);  // transcript hashes are unkeyed
+    if (blockIdx.x == 0) {  // the prover
+        SubBarrier bar;
+        bar.cnt = &s_bar;
+        bar.target = 0;
+        zq u = *jp(D.st[0].it.u_prev, jo);  // the inverses of every earlier stage
+        TailProverLds L;
+        L.msgs = msgs;
+        L.msg_zq = msg_zq;
+        L.stX = stX;
+        L.stY = stY;
+        L.work = work;
+        L.seeds = s_seeds;
+        L.reject = &s_reject;
+        L.s_xi = &s_xi;
+        L.s_u = &s_u;
+        tail_prover_stage<true>(0, D, jo, L, wc, tt, bar, key, u);
+        for (uint32_t s = 1; s < S; ++s) tail_prover_stage<false>(s, D, jo, L, wc, tt, bar, key, u);
+        if (threadIdx.x == 0) store_zq(jp(D.st[S - 1].w.dst, jo), load_zq(S & 1 ? stY : stX));  // m*
+        return;
+    }
+    // the verifier: stage s's challenge needs the prover's stage s - 1 (its cross terms), the
+    // update needs stage s (the inverse candidate)
+    const uint32_t* ctr = jp(D.ctr, jo);
+    uint32_t* vflags = jp(D.vflags, jo);
+    for (uint32_t s = 0; s < S; ++s) {
+        const TailVerStep& vs = D.vs[s];
+        if (s) tail_wait(ctr, s, vflags);
+        TranscriptMsg t = vs.tx;
+        tx_to_job(t, jo);
+        t.epoch_seed = s_seeds;
+        t.file_id = s_seeds + 32;
+        t.reject = &s_reject;
+        const size_t len = tx_message_len(t.label_len, t.lens);
+        const zq x = wg_tx_challenge(t, len, key, 0, wc, jp(vs.scal, jo), vflags, 0);
+        tail_wait(ctr, s + 1, vflags);
+        tail_com_update(vs, x, jp(vs.hint, jo) + hint_off, jo);
+        __syncthreads();  // the next message reads the updated commitment
+    }
+}
+
+size_t tail_lds_bytes(const TailDesc& d, uint32_t* msg_zq) {
+    size_t work = sizeof(WgChal), msg = 0;
+    for (uint32_t s = 0; s < d.nstages; ++s) {
+        const SmallStage& st = d.st[s];
+        const size_t w = 4 * tail_stage_words(st, tail_group(st));
+        if (w > work) work = w;
+        if (tail_msg_zq(st) > msg) msg = tail_msg_zq(st);
+    }
+    *msg_zq = (uint32_t)msg;
+    return TAIL_DESC_LDS + TAIL_TW_LDS + 2 * 16 * msg + 16 * (TAIL_STATE_X + TAIL_STATE_Y) + work;
+}
+
+hipError_t tail_set_lds_limit(size_t lds_bytes) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tail), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_bytes);
+}
+
+void launch_tail(const TailDesc* d_desc, size_t lds_bytes, uint32_t msg_zq, int verifier, int hint_off,
+                 const NttTables& tw, hipStream_t s, const Jobs& jb) {
+    if (jb.n == 0) return;
+    hipLaunchKernelGGL(k_tail, dim3(1 + (verifier ? 1 : 0), jb.n), dim3(TAIL_THREADS), lds_bytes, s, d_desc, tw,
+                       jb.stride, hint_off, msg_zq);
 }
 
 // Z_q-only row production (the verifier's generator folds, verifier.rs:141-179)
@@ -1462,9 +2105,14 @@ __device__ zq coeff_weight(uint32_t c, uint32_t len, const zq* chal, uint32_t pm
 
 constexpr uint32_t EARLY_TILE = 1024;  // coefficients staged in LDS per step
 
-// One workgroup per (i_lo, k): H[(i_lo*K + k)*n_t + j]. The positions c -> c_T are nondecreasing
-// in c up to the first coefficient whose path wraps and 0 from there on, so position j collects
-// the run [lo_j, lo_{j+1}) (and position 0 also the wrapped tail), found by bisection.
+uint32_t early_chunks(int log_n) { return log_n > 8 ? 1u << (log_n - 8) : 1u; }
+
+// One workgroup per (i_lo, k) and chunk z of 256 coefficients (blockIdx.z; one chunk below 512):
+// the chunk's partial sums H[z][(i_lo*K + k)*n_t + j], which the late fold adds up. The positions
+// c -> c_T are nondecreasing in c up to the first coefficient whose path wraps and 0 from there
+// on, so position j collects the run [lo_j, lo_{j+1}) (and position 0 also the wrapped tail), found
+// by bisection. Splitting the coefficients puts several workgroups on each CU, enough loads in
+// flight to stream the matrix.
 __global__ void __launch_bounds__(256) k_gen_contract_early(const zq* __restrict__ G, size_t gjs, const zq* chal,
                                                             uint32_t vmask, uint32_t pmask, int log_hi, int log_lo,
                                                             int log_n, int log_nt, zq* __restrict__ H, size_t js) {
@@ -1472,7 +2120,7 @@ __global__ void __launch_bounds__(256) k_gen_contract_early(const zq* __restrict
     __shared__ zq sp[EARLY_TILE];
     const size_t jo = job_off(js);
     chal = jp(chal, jo);
-    H = jp(H, jo);
+    H = jp(H, jo) + ((size_t)blockIdx.z * ((size_t)K << log_lo) << log_nt);
     G = jp(G, (size_t)blockIdx.y * gjs);
     const uint32_t i_lo = blockIdx.x / K, k = blockIdx.x % K;
     const uint32_t n_hi = 1u << log_hi, n = 1u << log_n, nt = 1u << log_nt, t = threadIdx.x;
@@ -1504,8 +2152,9 @@ __global__ void __launch_bounds__(256) k_gen_contract_early(const zq* __restrict
     zq acc = zq_zero();
     const size_t row_stride = ((size_t)K << log_n) << log_lo;  // from row block h to h + 1
     const zq* g0 = G + (((size_t)i_lo * K + k) << log_n);
-    for (uint32_t c0 = 0; c0 < n; c0 += EARLY_TILE) {
-        const uint32_t tl = n - c0 < EARLY_TILE ? n - c0 : EARLY_TILE;
+    const uint32_t cw = n / gridDim.z, cbeg = blockIdx.z * cw, cend = cbeg + cw;  // this chunk
+    for (uint32_t c0 = cbeg; c0 < cend; c0 += EARLY_TILE) {
+        const uint32_t tl = cend - c0 < EARLY_TILE ? cend - c0 : EARLY_TILE;
         for (uint32_t cc = t; cc < tl; cc += blockDim.x) {
             const uint32_t c = c0 + cc;
             zq s = zq_zero();
@@ -1525,13 +2174,29 @@ __global__ void __launch_bounds__(256) k_gen_contract_early(const zq* __restrict
     if (t < nt) store_zq(H + (((size_t)i_lo * K + k) << log_nt) + t, acc);
 }
 
+// H[0][e] += H[1][e] + ... + H[nch - 1][e]: the early fold's chunk partials into its first block
+__global__ void __launch_bounds__(256) k_gen_sum_chunks(zq* __restrict__ H, uint32_t count, uint32_t nch, size_t js) {
+    const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= count) return;
+    H = jp(H, job_off(js));
+    zq acc = load_zq(H + e);
+    for (uint32_t z = 1; z < nch; ++z) acc = zq_add(acc, load_zq(H + (size_t)z * count + e));
+    store_zq(H + e, acc);
+}
+
 void launch_gen_contract_early(const zq* G, bool g_shared, const zq* chal, uint32_t vmask, uint32_t pmask,
                                int log_hi, int log_lo, int log_n, int log_nt, zq* H, hipStream_t s,
                                const Jobs& jb) {
     if (jb.n == 0) return;
-    hipLaunchKernelGGL(k_gen_contract_early, dim3((unsigned)(K << log_lo), jb.n), dim3(256), 0, s, G,
+    const uint32_t nch = early_chunks(log_n);
+    hipLaunchKernelGGL(k_gen_contract_early, dim3((unsigned)(K << log_lo), jb.n, nch), dim3(256), 0, s, G,
                        g_shared ? (size_t)0 : jb.stride, chal, vmask, pmask, log_hi, log_lo, log_n, log_nt, H,
                        jb.stride);
+    if (nch > 1) {
+        const uint32_t count = ((uint32_t)K << log_lo) << log_nt;
+        hipLaunchKernelGGL(k_gen_sum_chunks, dim3((count + 255) / 256, jb.n), dim3(256), 0, s, H, count, nch,
+                           jb.stride);
+    }
 }
 
 // g_d[0]_k from H (rows x K x n, n and rows <= 32: what the late rounds fold) with the late

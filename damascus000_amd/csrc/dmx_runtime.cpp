@@ -776,10 +776,9 @@ struct Transcriber {
     }
     const uint8_t* epoch() const { return seeds; }
     uint8_t* file_id() const { return seeds + 32; }
-    // challenge (transcript.rs:42-63) written as (x, x^-1) to out, for every job
-    void challenge(const char* label, uint64_t round, const zq* c, size_t lc, const zq* l, size_t ll,
-                   const zq* r, size_t lr, zq* out, uint32_t* flags, hipStream_t s, const Jobs& jb,
-                   bool inverse = true) {
+    // the challenge message (transcript.rs:50-61, 77-90) over the modules (c, l, r)
+    TranscriptMsg message(const char* label, uint64_t round, const zq* c, size_t lc, const zq* l, size_t ll,
+                          const zq* r, size_t lr) const {
         TranscriptMsg t;
         t.epoch_seed = epoch();
         t.file_id = file_id();
@@ -794,6 +793,13 @@ struct Transcriber {
         t.lens[1] = (uint32_t)ll;
         t.lens[2] = (uint32_t)lr;
         t.reject = reject;
+        return t;
+    }
+    // challenge (transcript.rs:42-63) written as (x, x^-1) to out, for every job
+    void challenge(const char* label, uint64_t round, const zq* c, size_t lc, const zq* l, size_t ll,
+                   const zq* r, size_t lr, zq* out, uint32_t* flags, hipStream_t s, const Jobs& jb,
+                   bool inverse = true) {
+        const TranscriptMsg t = message(label, round, c, lc, l, ll, r, lr);
         transcript_challenge(t, IVW, 0, msg.as<uint8_t>(), cvs.as<uint32_t>(), out, flags, s, jb, inverse);
     }
 };
@@ -956,6 +962,90 @@ static bool small_stages_enabled() {
     return on;
 }
 
+// The operands of one stage of fold_round (prover.rs:140-313): the witness and generator folds,
+// the challenge-inverse task, the invariant's expected update and where the lifts go.
+struct StageArgs {
+    RowSrc w, g;
+    InvTask it;
+    ComUpd cu;
+    zq* com_out;
+    zq* out_lr;
+    int cross;  // the stage emits cross terms
+    uint32_t bit;
+};
+
+// vector stage of round j: m_L + x m_R and x g_L + g_R = x (g_L + x^-1 g_R); the lift checks the
+// invariant against C_t = C + x^-1 L + x R (written to Ct) and emits L/R_poly
+static StageArgs vec_stage_args(const dmx_prover* p, size_t j) {
+    const size_t h = p->m_at(j) / 2;
+    StageArgs a{};
+    a.w.mode = ROW_VFOLD;
+    a.w.a = p->W_a.as<zq>();
+    a.w.b_off = h;
+    a.w.scal = p->sc(4 * j);
+    a.w.dst = p->W_b.as<zq>();
+    a.g.mode = ROW_VFOLD_L;
+    a.g.a = (j == 0 && p->G0_src) ? p->G0_src : p->G_a.as<zq>();
+    a.g.a_shared = j == 0 && p->G0_src;
+    a.g.b_off = h * K;
+    a.g.scal = p->sc(4 * j);
+    a.g.dst = p->G_b.as<zq>();
+    a.it = InvTask{p->sc(4 * j), j ? p->un(2 * j) : nullptr, p->un(2 * j + 1)};
+    a.cu = ComUpd{p->C.as<zq>(), p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j), 0, p->un(2 * j + 1)};
+    a.com_out = p->Ct.as<zq>();
+    a.out_lr = p->rec_lpoly(j);
+    a.cross = 1;
+    a.bit = vec_bit(j);
+    return a;
+}
+
+// poly stage of round j: e + y o and y g_e + X g_o = y (g_e + y^-1 X g_o); the lift checks the
+// invariant against C' = even(C_t) + y^-1 L + y R (written to C) and emits the next round's
+// vector cross terms
+static StageArgs poly_stage_args(const dmx_prover* p, size_t j) {
+    const size_t h = p->m_at(j) / 2;
+    StageArgs a{};
+    a.w.mode = ROW_PFOLD_W;
+    a.w.a = p->W_b.as<zq>();
+    a.w.scal = p->sc(4 * j + 2);
+    a.w.dst = p->W_a.as<zq>();
+    a.g.mode = ROW_PFOLD_GL;
+    a.g.a = p->G_b.as<zq>();
+    a.g.scal = p->sc(4 * j + 2);
+    a.g.dst = p->G_a.as<zq>();
+    a.it = InvTask{p->sc(4 * j + 2), p->un(2 * j + 1), p->un(2 * j + 2)};
+    a.cu = ComUpd{p->Ct.as<zq>(), p->rec_lpoly(j), p->rec_rpoly(j), p->sc(4 * j + 2), 1, p->un(2 * j + 2)};
+    a.com_out = p->C.as<zq>();
+    a.out_lr = p->rec_lvec(j + 1);
+    a.cross = h > 1 ? 1 : 0;
+    a.bit = poly_bit(j);
+    return a;
+}
+
+// one stage of degree <= 32 as a single launch (k_stage_small) or a step of the tail (k_tail)
+static SmallStage small_stage_of(const dmx_prover* p, size_t j, int poly, const StageArgs& a) {
+    SmallStage st{};
+    st.poly = poly;
+    st.log_m = p->d - (int)j;
+    st.rows = (uint32_t)(p->m_at(j) / 2);
+    st.w = a.w;
+    st.g = a.g;
+    st.cu = a.cu;
+    st.com_out = a.com_out;
+    st.out_lr = a.out_lr;
+    st.cross = a.cross;
+    st.flags = p->flags.as<uint32_t>();
+    st.bit = a.bit;
+    st.it = a.it;
+    return st;
+}
+
+static size_t first_small_round(const dmx_prover* p) {
+    size_t j = 0;
+    while ((p->N >> j) > SMALL_STAGE_MAX_M) ++j;
+    return j;
+}
+
 // fold_round (prover.rs:140-313) for round j, m = N >> j: both stages and both invariants.
 // `vec_done`, when given, is recorded after the vector stage: the round's poly cross terms are
 // then in place for a verifier that follows the proof half a round behind.
@@ -969,101 +1059,47 @@ static int prover_round(dmx_prover* p, size_t j, hipEvent_t vec_done = nullptr) 
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[0], s));
     const bool small = small_stages_enabled() && m <= SMALL_STAGE_MAX_M;
     // the job's small stages count up on flags[STAGE_CTR_WORD] (zeroed with the flags)
-    size_t first_small = 0;
-    while ((p->N >> first_small) > SMALL_STAGE_MAX_M) ++first_small;
-    const uint32_t small_ord = small ? (uint32_t)(2 * (j - first_small)) : 0;
+    const uint32_t small_ord = small ? (uint32_t)(2 * (j - first_small_round(p))) : 0;
     // ---- vector stage ----
     p->tr.challenge("vec", j, p->C.as<zq>(), m, p->rec_lvec(j), m, p->rec_rvec(j), m, p->sc(4 * j), flags, s, jb,
                     false);
-
-    RowSrc vw{};
-    vw.mode = ROW_VFOLD;
-    vw.a = p->W_a.as<zq>();
-    vw.b_off = h;
-    vw.scal = p->sc(4 * j);  // m_L + x * m_R
-    vw.dst = p->W_b.as<zq>();
-    RowSrc vg{};
-    vg.mode = ROW_VFOLD_L;
-    vg.a = (j == 0 && p->G0_src) ? p->G0_src : p->G_a.as<zq>();
-    vg.a_shared = j == 0 && p->G0_src;
-    vg.b_off = h * K;
-    vg.scal = p->sc(4 * j);  // x g_L + g_R = x (g_L + x^-1 g_R)
-    vg.dst = p->G_b.as<zq>();
-    const InvTask vit{p->sc(4 * j), j ? p->un(2 * j) : nullptr, p->un(2 * j + 1)};
-    // invariant at degree m and the poly cross terms at degree m/2 from one pass; the lift
-    // checks the invariant against C_t = C + x^-1 L + x R (written to Ct) and emits L/R_poly
+    const StageArgs va = vec_stage_args(p, j);
+    // invariant at degree m and the poly cross terms at degree m/2 from one pass
     const size_t s0 = (size_t)K * NPRIMES * m;
-    const ComUpd vcu{p->C.as<zq>(), p->rec_lvec(j), p->rec_rvec(j), p->sc(4 * j), 0, p->un(2 * j + 1)};
     if (small) {
-        SmallStage st{};
-        st.poly = 0;
-        st.log_m = lm;
-        st.rows = (uint32_t)h;
-        st.w = vw;
-        st.g = vg;
-        st.cu = vcu;
-        st.com_out = p->Ct.as<zq>();
-        st.out_lr = p->rec_lpoly(j);
-        st.cross = 1;
-        st.flags = flags;
-        st.bit = vec_bit(j);
-        st.it = vit;
+        SmallStage st = small_stage_of(p, j, 0, va);
         st.ctr = flags + STAGE_CTR_WORD;
         st.target = small_ord + 1;
         launch_stage_small(st, tw, s, jb);
     } else {
-        fwd_rows2(p->ctx, vw, h, p->What_b.as<uint32_t>(), vg, h * K, p->Ghat_b.as<uint32_t>(), lm, s, jb, vit);
+        fwd_rows2(p->ctx, va.w, h, p->What_b.as<uint32_t>(), va.g, h * K, p->Ghat_b.as<uint32_t>(), lm, s, jb, va.it);
         launch_mac_split_inv(p->What_b.as<uint32_t>(), p->Ghat_b.as<uint32_t>(), h, lm, p->res_at(0), p->res_at(s0),
                              p->part.as<uint32_t>(), tw, s, jb);
-        launch_lift_check(p->res_at(0), lm, vcu, p->Ct.as<zq>(), nullptr, flags, vec_bit(j), p->res_at(s0),
-                          p->rec_lpoly(j), 2 * K, lm - 1, s, jb);
+        launch_lift_check(p->res_at(0), lm, va.cu, va.com_out, nullptr, flags, va.bit, p->res_at(s0), va.out_lr,
+                          2 * K, lm - 1, s, jb);
     }
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[1], s));
     if (vec_done) HIP_TRY(hipEventRecord(vec_done, s));
     // ---- poly stage ----
     p->tr.challenge("poly", j, p->Ct.as<zq>(), m, p->rec_lpoly(j), h, p->rec_rpoly(j), h, p->sc(4 * j + 2), flags, s,
                     jb, false);
-
-    RowSrc pw{};
-    pw.mode = ROW_PFOLD_W;
-    pw.a = p->W_b.as<zq>();
-    pw.scal = p->sc(4 * j + 2);  // e + y * o
-    pw.dst = p->W_a.as<zq>();
-    RowSrc pg{};
-    pg.mode = ROW_PFOLD_GL;
-    pg.a = p->G_b.as<zq>();
-    pg.scal = p->sc(4 * j + 2);  // y g_e + X g_o = y (g_e + y^-1 X g_o)
-    pg.dst = p->G_a.as<zq>();
-    const InvTask pit{p->sc(4 * j + 2), p->un(2 * j + 1), p->un(2 * j + 2)};
-    // post-poly invariant, plus the next round's vector cross terms from the same pass; the lift
-    // checks the invariant against C' = even(C_t) + y^-1 L + y R (written to C)
+    const StageArgs pa = poly_stage_args(p, j);
+    // post-poly invariant, plus the next round's vector cross terms from the same pass
     const size_t s1 = (size_t)K * NPRIMES * h;
-    const bool next = h > 1;
-    const ComUpd pcu{p->Ct.as<zq>(), p->rec_lpoly(j), p->rec_rpoly(j), p->sc(4 * j + 2), 1, p->un(2 * j + 2)};
+    const bool next = pa.cross != 0;
     if (small) {
-        SmallStage st{};
-        st.poly = 1;
-        st.log_m = lm;
-        st.rows = (uint32_t)h;
-        st.w = pw;
-        st.g = pg;
-        st.cu = pcu;
-        st.com_out = p->C.as<zq>();
-        st.out_lr = p->rec_lvec(j + 1);
-        st.cross = next ? 1 : 0;
-        st.flags = flags;
-        st.bit = poly_bit(j);
-        st.it = pit;
+        SmallStage st = small_stage_of(p, j, 1, pa);
         st.ctr = flags + STAGE_CTR_WORD;
         st.target = small_ord + 2;
         launch_stage_small(st, tw, s, jb);
     } else {
-        fwd_rows2(p->ctx, pw, h, p->What_a.as<uint32_t>(), pg, h * K, p->Ghat_a.as<uint32_t>(), lm - 1, s, jb, pit);
+        fwd_rows2(p->ctx, pa.w, h, p->What_a.as<uint32_t>(), pa.g, h * K, p->Ghat_a.as<uint32_t>(), lm - 1, s, jb,
+                  pa.it);
         launch_mac_pairs_inv(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), h, lm - 1, p->res_at(0),
                              next ? p->res_at(s1) : nullptr, next ? p->res_at(2 * s1) : nullptr,
                              p->part.as<uint32_t>(), tw, s, jb);
-        launch_lift_check(p->res_at(0), lm - 1, pcu, p->C.as<zq>(), nullptr, flags, poly_bit(j),
-                          next ? p->res_at(s1) : nullptr, p->rec_lvec(j + 1), next ? 2 * K : 0, lm - 1, s, jb);
+        launch_lift_check(p->res_at(0), lm - 1, pa.cu, pa.com_out, nullptr, flags, pa.bit,
+                          next ? p->res_at(s1) : nullptr, pa.out_lr, next ? 2 * K : 0, lm - 1, s, jb);
     }
     if (!p->capturing) HIP_TRY(hipEventRecord(p->ev[2], s));
     HIP_TRY(hipGetLastError());
@@ -1326,7 +1362,7 @@ static void verifier_plan(dmx_verifier* v, Layout& L) {
     L.add(v->rec, (size_t)4 * K * v->c_cap * 16);
     L.add(v->chal, std::max<size_t>(v->rounds, 1) * 4 * 16);
     const GenSplit g = gen_split(v);
-    L.add(v->H, g.T ? ((size_t)K << (g.log_lo + g.log_nt)) * 16 : 16);
+    L.add(v->H, g.T ? ((size_t)K << (g.log_lo + g.log_nt)) * early_chunks(g.log_n) * 16 : 16);
     L.add(v->gfin, K * 16);
     L.add(v->flags, 256);
     L.add(v->opening, 16);
@@ -1819,6 +1855,12 @@ struct BatchJob {
     uint64_t graph_version = 0;  // the context's fwd_version the graphs were captured with
     bool one_stream = false;     // the shape of the next enqueue
     GenEntry* gen = nullptr;
+    // the persistent tail (k_tail) of every job: stage descriptors for the prover's stages of
+    // degree <= 32 and the verifier's replay of them, built once from the workspace layout
+    DevBuf tail_desc;
+    size_t tail_lds = 0;
+    uint32_t tail_msg = 0;
+    size_t tail_round = 0;  // the first round of the tail (0: no tail)
     bool busy = false;
     uint64_t last_use = 0;
     // pinned staging for files handed over in host memory: the host copies each file in, and
@@ -1860,6 +1902,69 @@ struct BatchJob {
 
 static size_t max_file_len(int d) { return (size_t)BYTES_PER_COEFF << (2 * d); }
 
+// The tail of a batch (k_tail): the prover's stages from the first round of degree <= TAIL_MAX_M
+// on and the verifier's replay of those rounds, one launch. Above that degree a stage's work is
+// large enough to want more than the tail's one workgroup per job, and those stages keep their
+// per-stage launches. DAMASCUS_TAIL=0 keeps every stage on them (for comparisons).
+constexpr size_t TAIL_CTR_WORD = 33;  // flags word: the tail's stage counter (zeroed with the flags)
+static bool tail_enabled() {
+    static const bool on = [] {
+        const char* e = getenv("DAMASCUS_TAIL");
+        return !(e && strcmp(e, "0") == 0);
+    }();
+    return on;
+}
+
+static int tail_build(BatchJob* J) {
+    dmx_prover* p = J->p.get();
+    dmx_verifier* v = J->v.get();
+    const size_t d = (size_t)p->d;
+    size_t T0 = 0;
+    while ((p->N >> T0) > TAIL_MAX_M) ++T0;
+    J->tail_round = 0;
+    if (T0 == 0 || 2 * (d - T0) > (size_t)TAIL_MAX_STAGES || T0 < gen_split(v).T) return DMX_OK;
+    std::unique_ptr<TailDesc> td(new TailDesc());
+    memset(td.get(), 0, sizeof(TailDesc));
+    td->nstages = (uint32_t)(2 * (d - T0));
+    td->ctr = p->flags.as<uint32_t>() + TAIL_CTR_WORD;
+    td->vflags = v->flags.as<uint32_t>();
+    for (size_t j = T0; j < d; ++j) {
+        const size_t s = 2 * (j - T0), m = p->m_at(j), h = m / 2;
+        td->st[s] = small_stage_of(p, j, 0, vec_stage_args(p, j));
+        td->st[s + 1] = small_stage_of(p, j, 1, poly_stage_args(p, j));
+        td->ptx[s] = p->tr.message("vec", j, p->C.as<zq>(), m, p->rec_lvec(j), m, p->rec_rvec(j), m);
+        td->ptx[s + 1] = p->tr.message("poly", j, p->Ct.as<zq>(), m, p->rec_lpoly(j), h, p->rec_rpoly(j), h);
+        TailVerStep& a = td->vs[s];  // vector challenge x, then C_t = C + x^-1 L + x R
+        a.tx = v->tr.message("vec", j, v->C.as<zq>(), m, p->rec_lvec(j), m, p->rec_rvec(j), m);
+        a.c = v->C.as<zq>();
+        a.l = p->rec_lvec(j);
+        a.r = p->rec_rvec(j);
+        a.scal = v->round_chal(j);
+        a.out = v->Ct.as<zq>();
+        a.hint = p->sc(4 * j);
+        a.poly = 0;
+        a.log_m = log2_exact(m);
+        TailVerStep& b = td->vs[s + 1];  // poly challenge y, then C = even(C_t) + y^-1 L + y R
+        b.tx = v->tr.message("poly", j, v->Ct.as<zq>(), m, p->rec_lpoly(j), h, p->rec_rpoly(j), h);
+        b.c = v->Ct.as<zq>();
+        b.l = p->rec_lpoly(j);
+        b.r = p->rec_rpoly(j);
+        b.scal = v->round_chal(j) + 2;
+        b.out = v->C.as<zq>();
+        b.hint = p->sc(4 * j + 2);
+        b.poly = 1;
+        b.log_m = log2_exact(h);
+    }
+    int rc = J->tail_desc.alloc(sizeof(TailDesc));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(J->tail_desc.p, td.get(), sizeof(TailDesc), hipMemcpyHostToDevice, p->s));
+    HIP_TRY(hipStreamSynchronize(p->s));
+    J->tail_lds = tail_lds_bytes(*td, &J->tail_msg);
+    HIP_TRY(tail_set_lds_limit(J->tail_lds));
+    J->tail_round = T0;
+    return DMX_OK;
+}
+
 static int batch_create(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
     std::unique_ptr<BatchJob> J(new BatchJob);
     J->ctx = ctx;
@@ -1882,11 +1987,13 @@ static int batch_create(dmx_ctx* ctx, int d, uint32_t n, BatchJob** out) {
     p->jb = v->jb = Jobs{n, L.stride()};
     prover_bind(p);
     verifier_bind(v);
-    J->evs.resize(2 * (size_t)d + 4);
+    J->evs.resize(2 * (size_t)d + 5);
     for (auto& e : J->evs) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&J->done, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&J->side_done, hipEventDisableTiming));
     HIP_TRY(hipStreamCreate(&J->gs));
+    rc = tail_build(J.get());
+    if (rc) return rc;
     *out = J.release();
     return DMX_OK;
 }
@@ -1947,7 +2054,12 @@ static int batch_enqueue_streams(BatchJob* J) {
     // inversion when the check x * candidate == 1 fails).
     const char* he = getenv("DAMASCUS_VERIFIER_HINT");
     const size_t hint_off = he && strcmp(he, "wrong") == 0 ? 0 : 1;
-    for (size_t j = 0; j < d; ++j) {
+    // The last stages (degree <= TAIL_MAX_M) and their replay run as one persistent launch (k_tail)
+    // from round T0 on: the loop stops there, and the launch follows the verifier's poly update of
+    // the round before it.
+    const size_t T0 = (tail_enabled() && small_stages_enabled()) ? J->tail_round : 0;
+    const size_t jend = T0 ? T0 : d;
+    for (size_t j = 0; j < jend; ++j) {
         rc = prover_round(p, j, J->evs[2 + d + j]);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(J->evs[2 + j], p->s));
@@ -1968,6 +2080,45 @@ static int batch_enqueue_streams(BatchJob* J) {
         if (!rc) rc = verifier_poly_challenge(v, p->rec_lpoly(j), p->rec_rpoly(j), false);
         if (rc) return rc;
         HIP_TRY(hipStreamWaitEvent(v->s, J->evs[2 + j], 0));
+    }
+    if (T0) {
+        rc = verifier_poly_update(v, p->rec_lpoly(T0 - 1), p->rec_rpoly(T0 - 1), p->sc(4 * (T0 - 1) + 2 + hint_off));
+        if (rc) return rc;
+        // the tail's verifier workgroup continues from the verifier's state after round T0 - 1
+        hipEvent_t vready = J->evs[2 + 2 * d + 1];
+        HIP_TRY(hipEventRecord(vready, v->s));
+        HIP_TRY(hipStreamWaitEvent(p->s, vready, 0));
+        launch_tail(J->tail_desc.as<TailDesc>(), J->tail_lds, J->tail_msg, 1, (int)hint_off, p->ctx->tw, p->s,
+                    p->jb);
+        HIP_TRY(hipGetLastError());
+        if (T && T == T0) {  // the early generator fold needs the rounds before T only: it overlaps the tail
+            HIP_TRY(hipStreamWaitEvent(J->gs, vready, 0));
+            rc = verifier_enqueue_early(v, J->gs);
+            if (rc) return rc;
+        }  // (for T < T0 the loop enqueued it at round T)
+        for (size_t j = T0; j < d; ++j) {  // what the replayed rounds leave on the host side
+            v->vec_mask |= 1u << j;
+            v->poly_mask |= 1u << j;
+        }
+        v->rows = 1;
+        v->clen = v->glen = 1;
+        v->round = d;
+        p->round = d;
+        if (T) {
+            HIP_TRY(hipEventRecord(J->side_done, J->gs));
+            HIP_TRY(hipStreamWaitEvent(p->s, J->side_done, 0));
+        }
+        const char* oe = getenv("DAMASCUS_TEST_OPENING_OFFSET");
+        const size_t open_off = oe ? std::min<size_t>(strtoull(oe, nullptr, 10), p->N - 1) : 0;
+        const hipStream_t vs = v->s;  // the final opening follows the tail on the prover's stream
+        v->s = p->s;
+        rc = verifier_final_launch(v, p->W_a.as<zq>() + open_off, T != 0, true);
+        v->s = vs;
+        if (rc) return rc;
+        launch_result_head(p->out_head.as<uint32_t>(), p->flags.as<uint32_t>(), v->flags.as<uint32_t>(),
+                           p->tr.file_id(), p->W_a.as<zq>(), p->s, p->jb);
+        HIP_TRY(hipGetLastError());
+        return DMX_OK;
     }
     rc = verifier_poly_update(v, p->rec_lpoly(d - 1), p->rec_rpoly(d - 1), p->sc(4 * (d - 1) + 2 + hint_off));
     if (rc) return rc;

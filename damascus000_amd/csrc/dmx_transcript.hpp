@@ -78,49 +78,34 @@ DMX_HD uint64_t tx_body_msg_word(const TranscriptMsg& t, uint32_t w) {
     return sh ? (lo >> sh) | (tx_body_word(t, v + 1) << (64 - sh)) : lo;
 }
 
-// Message words 0..12 (bytes [0, 104)): epoch_seed, u64 len(label), then from byte 40 the label
-// followed by u64 32, file_id, u64 epoch 0 and u64 round, each shifted by len(label) bytes, then
-// from byte H the first body word. Needs 1 <= len(label) <= 4 and 8-byte aligned epoch_seed and
-// file_id.
-DMX_HD void tx_head_words(const TranscriptMsg& t, uint64_t out[13]) {
+// Message word j < 13 (bytes [8j, 8j + 8)) of the header, computed alone: epoch_seed, u64
+// len(label), then from byte 40 the label followed by u64 32, file_id, u64 epoch 0 and u64 round,
+// each shifted by len(label) bytes, then from byte H the first body word. Needs 1 <= len(label)
+// <= 4 and 8-byte aligned epoch_seed and file_id.
+DMX_HD uint64_t tx_head_word(const TranscriptMsg& t, uint32_t j) {
     const uint64_t* e = reinterpret_cast<const uint64_t*>(t.epoch_seed);
     const uint64_t* f = reinterpret_cast<const uint64_t*>(t.file_id);
+    if (j < 4) return e[j];
+    if (j == 4) return t.label_len;
     const uint32_t sl = 8 * t.label_len;
-    uint64_t lab = 0;
-    for (uint32_t i = 0; i < t.label_len; ++i) lab |= (uint64_t)t.label[i] << (8 * i);
-    const uint64_t S[8] = {32, f[0], f[1], f[2], f[3], 0, t.round, tx_body_word(t, 0)};
-    for (int i = 0; i < 4; ++i) out[i] = e[i];
-    out[4] = t.label_len;
-    out[5] = lab | (S[0] << sl);
-    for (int j = 1; j < 8; ++j) out[5 + j] = (S[j - 1] >> (64 - sl)) | (S[j] << sl);
-}
-
-struct TxWords4 {
-    uint64_t w[4];
-};
-
-// message words w0 .. w0+3 for w0 < 13 (the few words that touch the header; kept out of line,
-// returned in registers)
-__host__ __device__ __attribute__((noinline)) inline TxWords4 tx_words4_head(const TranscriptMsg& t, uint32_t w0) {
-    uint64_t head[13];
-    tx_head_words(t, head);
-    TxWords4 r;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        uint64_t v = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < 13; ++j) v = (w0 + i == j) ? head[j] : v;  // no indexed private array
-        r.w[i] = w0 + i < 13 ? v : tx_body_msg_word(t, w0 + i);
+    // S = {32, f[0..3], 0, round, first body word}: the fields after the label, as words
+    auto S = [&](uint32_t i) -> uint64_t {
+        return i == 0 ? 32u : i <= 4 ? f[i - 1] : i == 5 ? 0u : i == 6 ? t.round : tx_body_word(t, 0);
+    };
+    if (j == 5) {  // the label's bytes with constant indices, so the message stays in registers
+        const uint64_t lw = (uint64_t)t.label[0] | ((uint64_t)t.label[1] << 8) | ((uint64_t)t.label[2] << 16) |
+                            ((uint64_t)t.label[3] << 24);
+        const uint64_t lab = lw & ((1ull << sl) - 1);
+        return lab | (S(0) << sl);
     }
-    return r;
+    return (S(j - 6) >> (64 - sl)) | (S(j - 5) << sl);
 }
 
 // message words w0 .. w0+3; past the header the five body words are loaded together
 DMX_HD void tx_words4(const TranscriptMsg& t, uint32_t w0, uint64_t out[4]) {
     const uint32_t H = tx_header_len(t);
     if (8 * w0 < H) {  // w0 < 13
-        const TxWords4 r = tx_words4_head(t, w0);
-        for (int i = 0; i < 4; ++i) out[i] = r.w[i];
+        for (uint32_t i = 0; i < 4; ++i) out[i] = w0 + i < 13 ? tx_head_word(t, w0 + i) : tx_body_msg_word(t, w0 + i);
         return;
     }
     const uint32_t b = 8 * w0 - H, v = b >> 3, sh = 8 * (b & 7);
