@@ -673,6 +673,9 @@ def main():
         "single_file": single,
         "transform_tuning": {f"2^{lg}": dict(zip(("variant", "variants", "tuned_us"), ctx.transform_variant(lg)))
                              for lg in range(6, 12)},
+        "mac_tuning": {f"2^{lg}": dict(zip(("shape", "shapes", "tuned_us"), ctx.mac_shape(lg)))
+                       for lg in range(6, 11)},
+        "tuning_source": ctx.tuning_source()[0],
         "roofline": roof,
         "host": {"nproc": nproc, "cpu_count": ncpu},
     }

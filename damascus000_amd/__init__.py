@@ -150,6 +150,10 @@ _SIGS = {
     "dmx_ctx_transform_variant": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]),
     "dmx_ctx_set_transform_variant": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    "dmx_ctx_mac_shape": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]),
+    "dmx_ctx_set_mac_shape": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    "dmx_ctx_tuning_source": (ctypes.c_int, [_VP, ctypes.c_char_p, _SZ]),
     "dmx_kernel_timing": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "dmx_statement_size": (_SZ, [_SZ]),
@@ -263,6 +267,25 @@ class Context:
 
     def set_transform_variant(self, log_m, variant):
         _check(lib().dmx_ctx_set_transform_variant(self.h, log_m, variant))
+
+    MAC_SHAPES = 9
+
+    def mac_shape(self, log_m):
+        """(chosen shape, shape count, tuning times in us) of the degree-2^log_m MAC passes; shape
+        i splits rows into segments of at least 2^(3 + i // 3) rows, at most 2^(17 + i % 3) threads"""
+        v, n = ctypes.c_int(), ctypes.c_int()
+        us = (ctypes.c_float * self.MAC_SHAPES)()
+        _check(lib().dmx_ctx_mac_shape(self.h, log_m, ctypes.byref(v), ctypes.byref(n), us))
+        return v.value, n.value, [round(us[i], 2) for i in range(n.value)]
+
+    def set_mac_shape(self, log_m, shape):
+        _check(lib().dmx_ctx_set_mac_shape(self.h, log_m, shape))
+
+    def tuning_source(self):
+        """("defaults" | "timed" | "cache", cache file or "")"""
+        buf = ctypes.create_string_buffer(1024)
+        src = lib().dmx_ctx_tuning_source(self.h, buf, len(buf))
+        return {0: "defaults", 1: "timed", 2: "cache"}.get(src, "unknown"), buf.value.decode()
 
 
 class DeviceBuffer:

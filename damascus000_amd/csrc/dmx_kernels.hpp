@@ -104,26 +104,41 @@ int fwd_variant_count(int log_m);
 // inverse NTT in place on `slices` slices of [8][m]: data[(slice*8 + p)*m + c], times m^-1
 void launch_ntt_inv(uint32_t* data, size_t slices, int log_m, const NttTables& tw, hipStream_t s,
                     const Jobs& jb = Jobs{});
+// How a MAC pass splits its rows into segments (see mac_segments): segments keep at least
+// 2^log_min_rows rows and the pass at most 2^log_max_threads threads. Every shape gives the same
+// sums; the context picks one per degree by timing them.
+struct MacShape {
+    uint8_t log_min_rows = 4;
+    uint8_t log_max_threads = 18;
+};
+constexpr int MAC_SHAPES = 9;          // min rows 8 / 16 / 32 x thread cap 2^17 / 2^18 / 2^19
+constexpr int MAC_SHAPE_DEFAULT = 4;   // 16 rows, 2^18 threads
+inline MacShape mac_shape_of(int i) {
+    MacShape m;
+    m.log_min_rows = (uint8_t)(3 + i / 3);
+    m.log_max_threads = (uint8_t)(17 + i % 3);
+    return m;
+}
 // inv = sum over all rows W[i]*G[i,k]; L = sum_{i<h} G[i+h,k]*W[i]; R = sum_{i<h} G[i,k]*W[i+h]
 // (h = rows/2; L/R skipped when out_L == nullptr). Outputs [k][8][m] residues.
 // `part` is scratch of mac_part_words(log_m) words: rows are split into segments whose partial
 // sums a second kernel adds.
 void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
                       uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, uint32_t* part, hipStream_t s,
-                      const Jobs& jb = Jobs{}, bool g_shared = false);
+                      const Jobs& jb = Jobs{}, bool g_shared = false, MacShape sh = MacShape{});
 // vector-stage invariant at degree m plus the poly-stage cross terms at degree m/2, from the
 // NTT-domain even/odd split: out_inv [k][8][m], out_L/out_R [k][8][m/2]
 void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m,
                       uint32_t* out_inv, uint32_t* out_L, uint32_t* out_R, uint32_t* part,
-                      const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{});
+                      const NttTables& tw, hipStream_t s, const Jobs& jb = Jobs{}, MacShape sh = MacShape{});
 // the same passes with the partial-sum reduction and the inverse transforms fused (outputs are
 // the inverse-transformed residues, scaled by 1/m); split: out_lr holds L then R
 void launch_mac_pairs_inv(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
                           uint32_t* out_L, uint32_t* out_R, uint32_t* part, const NttTables& tw, hipStream_t s,
-                          const Jobs& jb = Jobs{}, bool g_shared = false);
+                          const Jobs& jb = Jobs{}, bool g_shared = false, MacShape sh = MacShape{});
 void launch_mac_split_inv(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
                           uint32_t* out_lr, uint32_t* part, const NttTables& tw, hipStream_t s,
-                          const Jobs& jb = Jobs{});
+                          const Jobs& jb = Jobs{}, MacShape sh = MacShape{});
 // scratch words the MAC passes need at degrees up to 2^max_log_m
 size_t mac_part_words(int max_log_m);
 // Z_q rows only (no transform): dst[row][c] = row_value(src, row, c)

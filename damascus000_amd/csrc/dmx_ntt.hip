@@ -535,11 +535,15 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_split(const uint32_t* __res
 }
 
 
-// Row segments per MAC pass: enough waves to keep HBM busy, each segment at least 16 rows.
-static inline uint32_t mac_segments(size_t rows, int log_m) {
+// Row segments per MAC pass: doubled while each segment keeps at least 2^log_min_rows rows and
+// the pass stays under 2^log_max_threads threads (at most 32 segments). The default shape (16
+// rows, 2^18 threads) keeps HBM busy at every degree; the context can tune it per degree.
+static inline uint32_t mac_segments(size_t rows, int log_m, MacShape sh) {
     const size_t quads = ((size_t)K * NPRIMES << log_m) / 4;  // threads per segment
     size_t seg = 1;
-    while (seg < 32 && rows / (2 * seg) >= 16 && quads * seg < (size_t)1 << 18) seg *= 2;
+    while (seg < 32 && rows / (2 * seg) >= ((size_t)1 << sh.log_min_rows) &&
+           quads * seg < (size_t)1 << sh.log_max_threads)
+        seg *= 2;
     return (uint32_t)seg;
 }
 
@@ -639,12 +643,13 @@ __global__ void k_mac_pairs_reduce(const uint32_t* __restrict__ part, uint32_t n
 }
 
 void launch_mac_pairs(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
-                      uint32_t* out_L, uint32_t* out_R, uint32_t* part, hipStream_t s, const Jobs& jb, bool g_shared) {
+                      uint32_t* out_L, uint32_t* out_R, uint32_t* part, hipStream_t s, const Jobs& jb, bool g_shared,
+                      MacShape sh) {
     if (jb.n == 0) return;
     const size_t gjs = g_shared ? 0 : jb.stride;
     const bool cross = out_L != nullptr && rows > 1;
     const size_t half = cross ? rows / 2 : rows;
-    const uint32_t nseg = log_m >= 2 ? mac_segments(half, log_m) : 1;
+    const uint32_t nseg = log_m >= 2 ? mac_segments(half, log_m, sh) : 1;
     const uint32_t seg_rows = (uint32_t)((half + nseg - 1) / nseg);
     const size_t m = (size_t)1 << log_m;
     if (log_m < 2) {  // degrees 1 and 2: the scalar kernel
@@ -754,7 +759,7 @@ __global__ void k_mac_split_reduce(const uint32_t* __restrict__ part, uint32_t n
 
 void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
                       uint32_t* out_L, uint32_t* out_R, uint32_t* part, const NttTables& tw, hipStream_t s,
-                      const Jobs& jb) {
+                      const Jobs& jb, MacShape sh) {
     if (jb.n == 0) return;
     const size_t gjs = jb.stride;
     if (log_m < 2) {  // degree 2: the scalar kernel
@@ -764,7 +769,7 @@ void launch_mac_split(const uint32_t* What, const uint32_t* Ghat, size_t rows, i
                            jb.stride, gjs);
         return;
     }
-    const uint32_t nseg = mac_segments(rows * 2, log_m);
+    const uint32_t nseg = mac_segments(rows * 2, log_m, sh);
     const uint32_t seg_rows = (uint32_t)((rows + nseg - 1) / nseg);
     const size_t m = (size_t)1 << log_m;
     const size_t threads = (size_t)nseg * K * NPRIMES * m / 4;
@@ -935,11 +940,11 @@ __global__ void __launch_bounds__(FIN_THREADS) k_mac_split_finish(const uint32_t
 
 void launch_mac_pairs_inv(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
                           uint32_t* out_L, uint32_t* out_R, uint32_t* part, const NttTables& tw, hipStream_t s,
-                          const Jobs& jb, bool g_shared) {
+                          const Jobs& jb, bool g_shared, MacShape sh) {
     if (jb.n == 0) return;
     const bool cross = out_L != nullptr && rows > 1;
     if (log_m < 2) {
-        launch_mac_pairs(What, Ghat, rows, log_m, out_inv, out_L, out_R, part, s, jb, g_shared);
+        launch_mac_pairs(What, Ghat, rows, log_m, out_inv, out_L, out_R, part, s, jb, g_shared, sh);
         launch_ntt_inv(out_inv, K, log_m, tw, s, jb);
         if (cross) {
             launch_ntt_inv(out_L, K, log_m, tw, s, jb);
@@ -949,7 +954,7 @@ void launch_mac_pairs_inv(const uint32_t* What, const uint32_t* Ghat, size_t row
     }
     const size_t gjs = g_shared ? 0 : jb.stride;
     const size_t half = cross ? rows / 2 : rows;
-    const uint32_t nseg = mac_segments(half, log_m);
+    const uint32_t nseg = mac_segments(half, log_m, sh);
     const uint32_t seg_rows = (uint32_t)((half + nseg - 1) / nseg);
     const size_t m = (size_t)1 << log_m;
     const size_t threads = (size_t)nseg * K * NPRIMES * m / 4;
@@ -962,17 +967,18 @@ void launch_mac_pairs_inv(const uint32_t* What, const uint32_t* Ghat, size_t row
 }
 
 void launch_mac_split_inv(const uint32_t* What, const uint32_t* Ghat, size_t rows, int log_m, uint32_t* out_inv,
-                          uint32_t* out_lr, uint32_t* part, const NttTables& tw, hipStream_t s, const Jobs& jb) {
+                          uint32_t* out_lr, uint32_t* part, const NttTables& tw, hipStream_t s, const Jobs& jb,
+                          MacShape sh) {
     if (jb.n == 0) return;
     const size_t m = (size_t)1 << log_m;
     if (log_m < 2) {
         launch_mac_split(What, Ghat, rows, log_m, out_inv, out_lr, out_lr + (size_t)K * NPRIMES * (m / 2), part, tw, s,
-                         jb);
+                         jb, sh);
         launch_ntt_inv(out_inv, K, log_m, tw, s, jb);
         launch_ntt_inv(out_lr, 2 * K, log_m - 1, tw, s, jb);
         return;
     }
-    const uint32_t nseg = mac_segments(rows * 2, log_m);
+    const uint32_t nseg = mac_segments(rows * 2, log_m, sh);
     const uint32_t seg_rows = (uint32_t)((rows + nseg - 1) / nseg);
     const size_t threads = (size_t)nseg * K * NPRIMES * m / 4;
     hipLaunchKernelGGL(k_mac_split4, dim3((unsigned)((threads + MAC_THREADS - 1) / MAC_THREADS), jb.n),
@@ -986,7 +992,7 @@ size_t mac_part_words(int max_log_m) {
     // a launch holds at most max(2^18, quads) threads of 4 positions, in at most 3 slots
     size_t w = 0;
     for (int lg = 2; lg <= max_log_m; ++lg)
-        w = std::max(w, (size_t)mac_segments(SIZE_MAX / 2, lg) * 3 * K * NPRIMES << lg);
+        w = std::max(w, (size_t)mac_segments(SIZE_MAX / 2, lg, mac_shape_of(MAC_SHAPES - 1)) * 3 * K * NPRIMES << lg);
     return std::max(w, (size_t)1);
 }
 

@@ -19,6 +19,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -255,7 +257,12 @@ struct dmx_ctx {
     // fwd_version counts changes so captured graphs can be retaken
     int fwd_variant[MAX_LOG_N + 1];
     float fwd_us[MAX_LOG_N + 1][4];  // tuning times per variant (0: not timed)
-    uint64_t fwd_version = 0;
+    // MAC-pass segmentation per degree (mac_shape_of index), timed at creation like the transforms
+    int mac_idx[MAX_LOG_N + 1];
+    float mac_us[MAX_LOG_N + 1][MAC_SHAPES];
+    int tune_source = 0;  // 0: defaults, 1: timed at creation, 2: read from the tuning cache
+    std::string tune_path;
+    uint64_t fwd_version = 0;  // counts changes of either choice (captured graphs are retaken)
     DevBuf tables;
     NttTables tw{};
     PrimeConsts pc[NPRIMES];
@@ -284,6 +291,7 @@ struct dmx_ctx {
         if (ring_done) (void)hipEventDestroy(ring_done);
         if (ring_stream) (void)hipStreamDestroy(ring_stream);
     }
+    MacShape mac(int log_m) const { return mac_shape_of(mac_idx[log_m]); }
     hipEvent_t take_event() {
         if (!event_pool.empty()) {
             hipEvent_t e = event_pool.back();
@@ -602,6 +610,222 @@ static int tune_transforms(dmx_ctx* c) {
     return result;
 }
 
+// MAC-pass tuning: per degree 2^6..2^10, every segmentation shape runs the pass pair of a round at
+// that degree on synthetic residues (the pairs pass with cross terms over 2^l rows and the split
+// pass over 2^(l-1) rows, each with its finish kernel), three timed launches each after one
+// untimed. Every shape gives the same sums, so the choice changes speed only; a shape replaces the
+// default only when it is at least 2 % faster, so timing noise does not move the choice.
+static int tune_mac(dmx_ctx* c) {
+    for (int l = 0; l <= MAX_LOG_N; ++l) {
+        c->mac_idx[l] = MAC_SHAPE_DEFAULT;
+        for (int v = 0; v < MAC_SHAPES; ++v) c->mac_us[l][v] = 0.f;
+    }
+    const char* e = getenv("DAMASCUS_NTT_TUNE");
+    if (e && strcmp(e, "0") == 0) return DMX_OK;
+    const int lmin = 6, lmax = 10;
+    const size_t mmax = (size_t)1 << lmax;
+    DevBuf w, g, part, out;
+    int rc = w.alloc(mmax * NPRIMES * mmax * 4);
+    if (!rc) rc = g.alloc(mmax * K * NPRIMES * mmax * 4);
+    if (!rc) rc = part.alloc(mac_part_words(lmax) * 4);
+    if (!rc) rc = out.alloc((size_t)3 * K * NPRIMES * mmax * 4);
+    if (rc) {  // no room for the tuning rows: keep the defaults
+        (void)hipGetLastError();
+        return DMX_OK;
+    }
+    launch_fill32(w.as<uint32_t>(), w.bytes / 4, 0x0000ABCDu, 0);  // below every prime
+    launch_fill32(g.as<uint32_t>(), g.bytes / 4, 0x00001234u, 0);
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    int result = DMX_OK;
+    for (int l = lmin; l <= lmax && result == DMX_OK; ++l) {
+        const size_t rows = (size_t)1 << l, m = (size_t)1 << l, slot = (size_t)K * NPRIMES * m;
+        uint32_t* o = out.as<uint32_t>();
+        for (int v = 0; v < MAC_SHAPES && result == DMX_OK; ++v) {
+            const MacShape sh = mac_shape_of(v);
+            float t_min = 0.f;
+            for (int rep = 0; rep < 4; ++rep) {
+                (void)hipEventRecord(a, 0);
+                launch_mac_pairs_inv(w.as<uint32_t>(), g.as<uint32_t>(), rows, l, o, o + slot, o + 2 * slot,
+                                     part.as<uint32_t>(), c->tw, 0, Jobs{}, false, sh);
+                launch_mac_split_inv(w.as<uint32_t>(), g.as<uint32_t>(), rows / 2, l, o, o + slot, part.as<uint32_t>(),
+                                     c->tw, 0, Jobs{}, sh);
+                (void)hipEventRecord(b, 0);
+                if (hipEventSynchronize(b) != hipSuccess || hipGetLastError() != hipSuccess) {
+                    result = fail(DMX_ERR_HIP, "MAC tuning launch failed (degree 2^%d, shape %d)", l, v);
+                    break;
+                }
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, a, b);
+                if (rep && (t_min == 0.f || ms < t_min)) t_min = ms;
+            }
+            c->mac_us[l][v] = t_min * 1e3f;
+        }
+        float best = c->mac_us[l][MAC_SHAPE_DEFAULT];
+        for (int v = 0; v < MAC_SHAPES; ++v)
+            if (c->mac_us[l][v] > 0.f && c->mac_us[l][v] < 0.98f * c->mac_us[l][MAC_SHAPE_DEFAULT] &&
+                c->mac_us[l][v] < best) {
+                best = c->mac_us[l][v];
+                c->mac_idx[l] = v;
+            }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return result;
+}
+
+// The tuning cache: the choices and timings of a context, keyed by this build and the device
+// (architecture name, compute units), so that later contexts on the same kind of device skip the
+// timing runs. DAMASCUS_TUNE_CACHE names the file ("0": none); by default
+// $XDG_CACHE_HOME/damascus000_amd/tuning.txt or ~/.cache/damascus000_amd/tuning.txt. The file is
+// replaced by a rename, so concurrent processes read either the old or the new one.
+static const char* const TUNE_BUILD = __DATE__ "_" __TIME__;
+
+static std::string tune_cache_path() {
+    const char* e = getenv("DAMASCUS_TUNE_CACHE");
+    if (e) return strcmp(e, "0") == 0 ? std::string() : std::string(e);
+    const char* x = getenv("XDG_CACHE_HOME");
+    const char* h = getenv("HOME");
+    if (x && *x) return std::string(x) + "/damascus000_amd/tuning.txt";
+    if (h && *h) return std::string(h) + "/.cache/damascus000_amd/tuning.txt";
+    return std::string();
+}
+
+static std::string tune_key(const dmx_ctx* c) {
+    hipDeviceProp_t prop;
+    std::string arch = "unknown";
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) arch = prop.gcnArchName;
+    for (char& ch : arch)
+        if (ch == ' ') ch = '_';
+    std::string build = TUNE_BUILD;
+    for (char& ch : build)
+        if (ch == ' ') ch = '_';
+    return "dmx-tune 2 " + build + " " + arch + " " + std::to_string(c->cus);
+}
+
+static bool tune_load(dmx_ctx* c, const std::string& path) {
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return false;
+    char line[512];
+    bool ok = fgets(line, sizeof line, f) != nullptr;
+    if (ok) {
+        std::string head(line);
+        while (!head.empty() && (head.back() == '\n' || head.back() == '\r')) head.pop_back();
+        ok = head == tune_key(c);
+    }
+    int fwd_seen = 0, mac_seen = 0;
+    while (ok && fgets(line, sizeof line, f)) {
+        char kind[8] = {0};
+        int l = -1, v = -1, n = 0;
+        if (sscanf(line, "%7s %d %d%n", kind, &l, &v, &n) < 3 || l < 0 || l > MAX_LOG_N) {
+            ok = false;
+            break;
+        }
+        const char* p = line + n;
+        float us[MAC_SHAPES] = {0};
+        const int want = strcmp(kind, "fwd") == 0 ? 4 : MAC_SHAPES;
+        for (int i = 0; i < want; ++i) {
+            int used = 0;
+            if (sscanf(p, "%f%n", &us[i], &used) != 1) {
+                ok = false;
+                break;
+            }
+            p += used;
+        }
+        if (!ok) break;
+        if (strcmp(kind, "fwd") == 0 && v >= 0 && v < std::max(1, fwd_variant_count(l))) {
+            c->fwd_variant[l] = v;
+            for (int i = 0; i < 4; ++i) c->fwd_us[l][i] = us[i];
+            ++fwd_seen;
+        } else if (strcmp(kind, "mac") == 0 && v >= 0 && v < MAC_SHAPES) {
+            c->mac_idx[l] = v;
+            for (int i = 0; i < MAC_SHAPES; ++i) c->mac_us[l][i] = us[i];
+            ++mac_seen;
+        } else {
+            ok = false;
+        }
+    }
+    fclose(f);
+    return ok && fwd_seen == MAX_LOG_N + 1 && mac_seen == MAX_LOG_N + 1;
+}
+
+static void tune_save(const dmx_ctx* c, const std::string& path) {
+    const size_t slash = path.rfind('/');
+    if (slash != std::string::npos && slash > 0) {  // create the directory (and one parent), if missing
+        const std::string dir = path.substr(0, slash);
+        const size_t s2 = dir.rfind('/');
+        if (s2 != std::string::npos && s2 > 0) (void)mkdir(dir.substr(0, s2).c_str(), 0755);
+        (void)mkdir(dir.c_str(), 0755);
+    }
+    const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+    FILE* f = fopen(tmp.c_str(), "w");
+    if (!f) return;
+    fprintf(f, "%s\n", tune_key(c).c_str());
+    for (int l = 0; l <= MAX_LOG_N; ++l) {
+        fprintf(f, "fwd %d %d", l, c->fwd_variant[l]);
+        for (int i = 0; i < 4; ++i) fprintf(f, " %.3f", c->fwd_us[l][i]);
+        fprintf(f, "\n");
+    }
+    for (int l = 0; l <= MAX_LOG_N; ++l) {
+        fprintf(f, "mac %d %d", l, c->mac_idx[l]);
+        for (int i = 0; i < MAC_SHAPES; ++i) fprintf(f, " %.3f", c->mac_us[l][i]);
+        fprintf(f, "\n");
+    }
+    const bool good = fclose(f) == 0;
+    if (!good || rename(tmp.c_str(), path.c_str()) != 0) (void)remove(tmp.c_str());
+}
+
+// Transform variants and MAC shapes: from the cache when it holds this build's choices for this
+// kind of device, else timed now and saved.
+static int tune_context(dmx_ctx* c) {
+    const char* e = getenv("DAMASCUS_NTT_TUNE");
+    const bool off = e && strcmp(e, "0") == 0;
+    c->tune_path = off ? std::string() : tune_cache_path();
+    if (!c->tune_path.empty()) {
+        for (int l = 0; l <= MAX_LOG_N; ++l) {
+            c->fwd_variant[l] = 0;
+            c->mac_idx[l] = MAC_SHAPE_DEFAULT;
+        }
+        if (tune_load(c, c->tune_path)) {
+            c->tune_source = 2;
+            return DMX_OK;
+        }
+    }
+    int rc = tune_transforms(c);
+    if (!rc) rc = tune_mac(c);
+    if (rc) return rc;
+    c->tune_source = off ? 0 : 1;
+    if (!off && !c->tune_path.empty()) tune_save(c, c->tune_path);
+    return DMX_OK;
+}
+
+extern "C" int dmx_ctx_mac_shape(dmx_ctx* ctx, int log_m, int* shape, int* count, float* tuned_us) {
+    if (!ctx || log_m < 0 || log_m > MAX_LOG_N) return fail(DMX_ERR_ARG, "bad context or degree");
+    if (shape) *shape = ctx->mac_idx[log_m];
+    if (count) *count = MAC_SHAPES;
+    if (tuned_us)
+        for (int v = 0; v < MAC_SHAPES; ++v) tuned_us[v] = ctx->mac_us[log_m][v];
+    return DMX_OK;
+}
+
+extern "C" int dmx_ctx_set_mac_shape(dmx_ctx* ctx, int log_m, int shape) {
+    if (!ctx || log_m < 0 || log_m > MAX_LOG_N) return fail(DMX_ERR_ARG, "bad context or degree");
+    if (shape < 0 || shape >= MAC_SHAPES) return fail(DMX_ERR_ARG, "MAC shape %d is not one of %d", shape, MAC_SHAPES);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->mac_idx[log_m] != shape) {
+        ctx->mac_idx[log_m] = shape;
+        ++ctx->fwd_version;
+    }
+    return DMX_OK;
+}
+
+extern "C" int dmx_ctx_tuning_source(const dmx_ctx* ctx, char* path, size_t cap) {
+    if (!ctx) return -1;
+    if (path && cap) snprintf(path, cap, "%s", ctx->tune_path.c_str());
+    return ctx->tune_source;
+}
+
 extern "C" int dmx_ctx_transform_variant(dmx_ctx* ctx, int log_m, int* variant, int* count, float* tuned_us) {
     if (!ctx || log_m < 0 || log_m > MAX_LOG_N) return fail(DMX_ERR_ARG, "bad context or degree");
     const int nv = fwd_variant_count(log_m);
@@ -655,7 +879,7 @@ extern "C" int dmx_ctx_create(int device_index, dmx_ctx** out) {
     HIP_TRY(hipStreamCreateWithFlags(&c->ring_stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&c->ring_done, hipEventDisableTiming));
     HIP_TRY(hipDeviceSynchronize());
-    rc = tune_transforms(c.get());
+    rc = tune_context(c.get());
     if (rc) return rc;
     *out = c.release();
     return DMX_OK;
@@ -936,7 +1160,8 @@ static int prover_init_commit(dmx_prover* p, zq* g_copy = nullptr) {
     const size_t slot = (size_t)K * NPRIMES * N;
     const bool shared = p->Ghat0_src != nullptr;
     launch_mac_pairs_inv(p->What_a.as<uint32_t>(), shared ? p->Ghat0_src : p->Ghat_a.as<uint32_t>(), N, d,
-                         p->res_at(0), p->res_at(slot), p->res_at(2 * slot), p->part.as<uint32_t>(), tw, s, jb, shared);
+                         p->res_at(0), p->res_at(slot), p->res_at(2 * slot), p->part.as<uint32_t>(), tw, s, jb, shared,
+                         p->ctx->mac(d));
     // com_0 (to C and com0) and L_vec, R_vec (adjacent) in one lift
     launch_lift_check(p->res_at(0), d, ComUpd{}, p->C.as<zq>(), p->com0.as<zq>(), nullptr, 0, p->res_at(slot),
                       p->rec_lvec(0), 2 * K, d, s, jb);
@@ -1074,7 +1299,7 @@ static int prover_round(dmx_prover* p, size_t j, hipEvent_t vec_done = nullptr) 
     } else {
         fwd_rows2(p->ctx, va.w, h, p->What_b.as<uint32_t>(), va.g, h * K, p->Ghat_b.as<uint32_t>(), lm, s, jb, va.it);
         launch_mac_split_inv(p->What_b.as<uint32_t>(), p->Ghat_b.as<uint32_t>(), h, lm, p->res_at(0), p->res_at(s0),
-                             p->part.as<uint32_t>(), tw, s, jb);
+                             p->part.as<uint32_t>(), tw, s, jb, p->ctx->mac(lm));
         launch_lift_check(p->res_at(0), lm, va.cu, va.com_out, nullptr, flags, va.bit, p->res_at(s0), va.out_lr,
                           2 * K, lm - 1, s, jb);
     }
@@ -1097,7 +1322,7 @@ static int prover_round(dmx_prover* p, size_t j, hipEvent_t vec_done = nullptr) 
                   pa.it);
         launch_mac_pairs_inv(p->What_a.as<uint32_t>(), p->Ghat_a.as<uint32_t>(), h, lm - 1, p->res_at(0),
                              next ? p->res_at(s1) : nullptr, next ? p->res_at(2 * s1) : nullptr,
-                             p->part.as<uint32_t>(), tw, s, jb);
+                             p->part.as<uint32_t>(), tw, s, jb, false, p->ctx->mac(lm - 1));
         launch_lift_check(p->res_at(0), lm - 1, pa.cu, pa.com_out, nullptr, flags, pa.bit,
                           next ? p->res_at(s1) : nullptr, pa.out_lr, next ? 2 * K : 0, lm - 1, s, jb);
     }
@@ -2955,7 +3180,7 @@ static int commit_impl(dmx_ctx* ctx, const uint64_t* witness, const uint64_t* ge
             HIP_TRY(hipStreamSynchronize(s));
         }
         launch_mac_pairs_inv(What.as<uint32_t>(), Ghat.as<uint32_t>(), nr, lg, res.as<uint32_t>(), nullptr, nullptr,
-                             mpart.as<uint32_t>(), ctx->tw, s);
+                             mpart.as<uint32_t>(), ctx->tw, s, Jobs{}, false, ctx->mac(lg));
         launch_crt_lift(res.as<uint32_t>(), part.as<zq>(), K, lg, s);
         launch_zq_add(acc.as<zq>(), part.as<zq>(), acc.as<zq>(), (size_t)K * m, s);
         HIP_TRY(hipStreamSynchronize(s));

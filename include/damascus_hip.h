@@ -276,6 +276,23 @@ int dmx_kernel_timing_degree(dmx_ctx *ctx, int log_m);
 int dmx_ctx_transform_variant(dmx_ctx *ctx, int log_m, int *variant, int *count, float *tuned_us);
 int dmx_ctx_set_transform_variant(dmx_ctx *ctx, int log_m, int variant);
 
+/* ---- MAC-pass tuning: per degree 2^log_m the context splits each NTT-domain MAC pass into row
+ * segments by one of *count shapes (shape i: segments of at least 2^(3 + i / 3) rows, at most
+ * 2^(17 + i % 3) threads per pass; shape 4 is the default), all giving the same sums.
+ * dmx_ctx_create times each shape on synthetic residues at degrees 2^6..2^10 and switches from the
+ * default only to a shape at least 2 % faster (DAMASCUS_NTT_TUNE=0: shape 4). tuned_us (count
+ * floats, may be null) receives the times, 0 where none ran. Setting a shape makes the context's
+ * cached graphs be captured again. No reference counterpart. */
+int dmx_ctx_mac_shape(dmx_ctx *ctx, int log_m, int *shape, int *count, float *tuned_us);
+int dmx_ctx_set_mac_shape(dmx_ctx *ctx, int log_m, int shape);
+/* Where the context's transform variants and MAC shapes came from: 0 defaults (tuning off),
+ * 1 timed at creation (and written to the tuning cache), 2 read from the tuning cache, which holds
+ * the choices of this build for this kind of device (architecture name, compute units).
+ * DAMASCUS_TUNE_CACHE names the cache file ("0": none); by default
+ * $XDG_CACHE_HOME/damascus000_amd/tuning.txt or ~/.cache/damascus000_amd/tuning.txt. path (cap
+ * bytes, may be null) receives the file name, empty when there is none. No reference counterpart. */
+int dmx_ctx_tuning_source(const dmx_ctx *ctx, char *path, size_t cap);
+
 /* sizes for buffer planning */
 size_t dmx_statement_size(size_t ring_len);
 size_t dmx_record_size(size_t vec_ring_len, size_t poly_ring_len);

@@ -490,6 +490,63 @@ def test_transform_variants_give_identical_proofs():
             ctx.set_transform_variant(lg, v)
 
 
+
+def test_mac_shapes_give_identical_proofs():
+    """Every MAC-pass segmentation shape gives the same proof: at d = 7 each against the oracle
+    (the host-API prover, per-stage launches), at d = 10 across the fewest, default and most
+    segments through the fused job, whose captured graph is retaken after a change."""
+    ctx = dm.Context.default()
+    saved = {lg: ctx.mac_shape(lg)[0] for lg in range(6, 12)}
+    say("tuned MAC shapes (shape, count, us):", {lg: ctx.mac_shape(lg) for lg in range(6, 11)})
+    small = W.pattern_bytes(W.byte_len_for_depth(7), 7)
+    mid = np.random.default_rng(10).integers(0, 256, size=8 << 20, dtype=np.uint8).tobytes()  # d = 10
+    rc, ost, orecs, oop = O.full_flow(small)
+    assert rc == 0
+    buf = ctx.upload(mid)
+    outs = []
+    try:
+        for v in range(ctx.MAC_SHAPES):
+            for lg in range(6, 12):
+                ctx.set_mac_shape(lg, v)
+            assert dm.prove(small) == (ost, orecs, oop), v
+            if v in (0, 4, 8):
+                say("fused d = 10 job with MAC shape", v)
+                outs.append(dm.prove_verify_device(buf.ptr, len(mid)))
+        assert outs[0] == outs[1] == outs[2]
+        with pytest.raises(dm.DamascusError):
+            ctx.set_mac_shape(10, ctx.MAC_SHAPES)
+    finally:
+        for lg, v in saved.items():
+            ctx.set_mac_shape(lg, v)
+
+
+def test_tuning_cache_round_trip(tmp_path):
+    """A context created with a tuning cache file times its choices and writes them there; a
+    second context reads them back instead of timing (same transform variants and MAC shapes)."""
+    path = str(tmp_path / "tuning.txt")
+    old = os.environ.get("DAMASCUS_TUNE_CACHE")
+    os.environ["DAMASCUS_TUNE_CACHE"] = path
+    try:
+        a = dm.Context()
+        assert a.tuning_source() == ("timed", path) and os.path.exists(path)
+        b = dm.Context()
+        assert b.tuning_source() == ("cache", path)
+        for lg in range(0, 14):
+            va, na, ta = a.transform_variant(lg)
+            vb, nb, tb = b.transform_variant(lg)
+            assert (va, na) == (vb, nb) and all(abs(x - y) < 0.01 for x, y in zip(ta, tb))
+            sa, ca, ua = a.mac_shape(lg)
+            sb, cb, ub = b.mac_shape(lg)
+            assert (sa, ca) == (sb, cb) and all(abs(x - y) < 0.01 for x, y in zip(ua, ub))
+        with open(path) as f:
+            assert f.readline().split()[:2] == ["dmx-tune", "2"]
+        say("tuning cache", path, "MAC shapes", {lg: b.mac_shape(lg)[0] for lg in range(6, 11)})
+    finally:
+        if old is None:
+            del os.environ["DAMASCUS_TUNE_CACHE"]
+        else:
+            os.environ["DAMASCUS_TUNE_CACHE"] = old
+
 def test_batch_of_one_depth_equals_solo_runs():
     """five d=6 files of different lengths and seeds proved side by side in one launch sequence:
     each proof equals proving that file alone and the oracle's, so no job reads another's data"""
