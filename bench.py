@@ -466,7 +466,8 @@ def source_digest():
     import re
 
     h = hashlib.sha256()
-    for f in ("dmx_ntt.hip", "dmx_hash.hip", "dmx_arith.hpp", "dmx_kernels.hpp", "dmx_transcript.hpp", "dmx_blake3.hpp"):
+    for f in ("dmx_ntt.hip", "dmx_hash.hip", "dmx_arith.hpp", "dmx_kernels.hpp", "dmx_transcript.hpp", "dmx_blake3.hpp",
+              "dmx_b3quad.hpp"):
         with open(os.path.join(ROOT, "damascus000_amd", "csrc", f)) as fh:
             text = re.sub(r"/\*.*?\*/", "", fh.read(), flags=re.S)
         for ln in text.splitlines():
@@ -509,9 +510,9 @@ def main():
     ap.add_argument("--c2-reps", type=int, default=10, help="timed graph replays of the 8 MiB single file")
     ap.add_argument("--inflight", type=int, default=4, help="server batch slots")
     ap.add_argument("--no-mixed", action="store_true", help="skip the served trace and the 256-file batch")
-    ap.add_argument("--only", choices=["roofline", "served", "ragged", "prover11"], default=None,
+    ap.add_argument("--only", choices=["roofline", "served", "ragged", "prover11", "fused"], default=None,
                     help="profiling runs: the prover runs of the roofline, the served trace, the ragged products, "
-                         "or d = 11 prover runs")
+                         "d = 11 prover runs, or fused prove + verify jobs on the 8 MiB file")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run a device copy of N MiB and exit")
@@ -565,6 +566,12 @@ def main():
                           "work_full_rate_lane_cycles": work(prover_ops(11), FULL_RATE_WEIGHTS)}))
         return
     data = single_file()
+    if args.only == "fused":  # four prove + verify jobs of the 8 MiB file (graph replays), for a kernel trace
+        buf = ctx.upload(data)
+        for _ in range(4):
+            dm.prove_verify_device(buf.ptr, FILE_BYTES)
+        print(json.dumps({"fused_jobs": 4, "depth": 10}))
+        return
     if args.only == "roofline":
         ctx.kernel_timing(True, log_m=10)
         ctx.kernel_timing()
