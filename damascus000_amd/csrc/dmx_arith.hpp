@@ -7,9 +7,11 @@
 //
 // Mod-p arithmetic for the eight CRT primes (config.rs:9-18), all below 2^32, uses Montgomery
 // reduction with R = 2^32 built on 32-bit mul_lo / mul_hi:
-//   redc(T)  = T * R^-1 mod p          for T < p * R, via t = T_hi - mulhi(T_lo * p^-1, p)
-//   mulc(b,w) = b * w mod p            for any 32-bit b, with w stored as (w*R mod p) and
-//                                       (w*R*p^-1 mod 2^32) so one product is never formed.
+//   redc64(h,l) = (h*R + l) * R^-1 mod p  via t = h - mulhi(l * p^-1, p), plus p on a borrow
+//   mulc(b,w)   = b * w mod p             for any 32-bit b, with w stored as (w*R mod p) and
+//                                          (w*R*p^-1 mod 2^32) so one product is never formed.
+// Half of the primes exceed 2^31, so sums of two residues can carry out of 32 bits: the
+// corrections test the carry or borrow of the 32-bit add or subtract itself.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -366,7 +368,7 @@ struct PrimeConsts {
     uint32_t r4_m, r4_mp;  // R^3 as a mulc constant: (R^4 mod p, (R^4 mod p) * pinv)
     uint32_t two64;      // 2^64 mod p
     uint32_t q_mod_p;    // q mod p
-    uint32_t pad[2];
+    uint32_t r3_m, r3_mp;  // R^2 as a mulc constant: (R^3 mod p, (R^3 mod p) * pinv)
 };
 
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -375,40 +377,44 @@ DMX_HD uint32_t mulhi32(uint32_t a, uint32_t b) { return __umulhi(a, b); }
 DMX_HD uint32_t mulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
 #endif
 
-// T * R^-1 mod p for T < p*R (result in [0, p))
-DMX_HD uint32_t redc(uint64_t T, uint32_t p, uint32_t pinv) {
-    uint32_t lo = (uint32_t)T, hi = (uint32_t)(T >> 32);
-    uint32_t m = lo * pinv;
-    uint32_t mh = mulhi32(m, p);
-    uint32_t t = hi - mh;
-    return hi < mh ? t + p : t;
+// a - b, plus p when the subtraction borrows. Canonical for a, b in [0, p); for any 32-bit a
+// and b < p the result is the representative of a - b in [0, 2^32) (below p when a < p).
+DMX_HD uint32_t sub_fix(uint32_t a, uint32_t b, uint32_t p) {
+    uint32_t t;
+    const bool borrow = __builtin_usub_overflow(a, b, &t);
+    return t + (borrow ? p : 0u);
 }
+
+// (hi * R + lo) * R^-1 mod p for any 32-bit hi and lo: the result lies in [0, 2^32), and in
+// [0, p) when hi < p. With m = lo * p^-1 mod R, hi*R + lo - m*p = (hi - mulhi(m, p)) * R exactly.
+DMX_HD uint32_t redc64(uint32_t hi, uint32_t lo, uint32_t p, uint32_t pinv) {
+    const uint32_t m = lo * pinv;
+    return sub_fix(hi, mulhi32(m, p), p);
+}
+
+// T * R^-1 mod p for T < p*R (result in [0, p))
+DMX_HD uint32_t redc(uint64_t T, uint32_t p, uint32_t pinv) { return redc64((uint32_t)(T >> 32), (uint32_t)T, p, pinv); }
 
 // b * w mod p for any 32-bit b; (wm, wmp) = (w*R mod p, wm*pinv mod 2^32)
 DMX_HD uint32_t mulc(uint32_t b, uint32_t wm, uint32_t wmp, uint32_t p) {
-    uint32_t hi = mulhi32(b, wm);
-    uint32_t m = b * wmp;
-    uint32_t mh = mulhi32(m, p);
-    uint32_t t = hi - mh;
-    return hi < mh ? t + p : t;
+    return sub_fix(mulhi32(b, wm), mulhi32(b * wmp, p), p);
 }
 
+// a + b mod p for a, b in [0, p): the 32-bit sum may carry (p > 2^31), then sum - p is right
 DMX_HD uint32_t addp(uint32_t a, uint32_t b, uint32_t p) {
-    uint32_t d = p - b;  // in (0, p]
-    uint32_t s = a - d;
-    return a < d ? s + p : s;
+    uint32_t s, t;
+    const bool carry = __builtin_uadd_overflow(a, b, &s);
+    const bool below = __builtin_usub_overflow(s, p, &t);
+    return (carry || !below) ? t : s;
 }
-DMX_HD uint32_t subp(uint32_t a, uint32_t b, uint32_t p) {
-    uint32_t s = a - b;
-    return a < b ? s + p : s;
-}
+DMX_HD uint32_t subp(uint32_t a, uint32_t b, uint32_t p) { return sub_fix(a, b, p); }
 
-// x mod p for a canonical Z_q coefficient: y = x*R^-3 by three REDC steps, then * R^3.
+// x mod p for x < 2^128 given as four limbs w0..w3 (w3 < p): Horner over limb pairs,
+// y = (w0 + w1 R) R^-1, then (y + w2 R) R^-1, then (y + w3 R) R^-1 = x R^-3, then * R^3.
 DMX_HD uint32_t zq_mod_p(const zq& x, const PrimeConsts& c) {
-    uint32_t y = redc((uint64_t)x.w[0], c.p, c.pinv);
-    y = redc((uint64_t)y + x.w[1], c.p, c.pinv);
-    y = redc((uint64_t)y + x.w[2], c.p, c.pinv);
-    y = y + x.w[3];  // < p + 2^16 < 2^32
+    uint32_t y = redc64(x.w[1], x.w[0], c.p, c.pinv);
+    y = redc64(x.w[2], y, c.p, c.pinv);
+    y = redc64(x.w[3], y, c.p, c.pinv);
     return mulc(y, c.r4_m, c.r4_mp, c.p);
 }
 
@@ -425,13 +431,12 @@ DMX_HD uint32_t zq_mod_p_centred(const zq& x, const PrimeConsts& c) {
     return zq_gt_half(x) ? subp(r, c.q_mod_p, c.p) : r;
 }
 
-// (hi * 2^64 + lo) mod p for the u96 accumulators of the MAC kernels
-// (three REDC steps give V * R^-3, then * R^3; each REDC input stays below p*R)
+// (hi * 2^64 + lo) mod p for the u96 accumulators of the MAC kernels: two REDC steps give
+// V * R^-2 (in [0, 2^32)), then * R^2
 DMX_HD uint32_t reduce_u96(uint64_t lo, uint32_t hi, const PrimeConsts& c) {
-    uint32_t y = redc((uint32_t)lo, c.p, c.pinv);
-    y = redc((uint64_t)y + (uint32_t)(lo >> 32), c.p, c.pinv);
-    y = redc((uint64_t)y + hi, c.p, c.pinv);
-    return mulc(y, c.r4_m, c.r4_mp, c.p);
+    uint32_t y = redc64((uint32_t)(lo >> 32), (uint32_t)lo, c.p, c.pinv);
+    y = redc64(hi, y, c.p, c.pinv);
+    return mulc(y, c.r3_m, c.r3_mp, c.p);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -189,8 +189,10 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_rows(FwdSeg s1, FwdSeg 
 // elements l + TP*i and runs every layer with len >= TP in registers (its twiddles depend only
 // on i, so they are shared by the slice's threads). One LDS transpose later, pass 2: thread l
 // holds the contiguous elements E*l .. E*l+E-1 and runs the layers len < TP. Two LDS round
-// trips per row instead of one per layer. LDS slots are XOR-swizzled so that both access
-// patterns, and the slices sharing a 32-lane LDS pass, hit distinct banks.
+// trips per row instead of one per layer. A slice's LDS words are padded by one word after
+// every E elements, and slices are TP * (E + 1) words apart: pass 1 reads slot l + TP*i + i*TP/E
+// and pass 2 slot (E + 1)*l + e, both a per-thread base plus a constant offset, and in both
+// passes (and for the slices sharing a 32-lane LDS pass) the lanes hit distinct banks.
 // ---------------------------------------------------------------------------------------
 template <int E, int TP>
 struct RegNtt {
@@ -200,11 +202,9 @@ struct RegNtt {
     static constexpr int LOG_M = LOG_E + LOG_TP;
     static constexpr int TPR = NPRIMES * TP;  // threads per row
     static constexpr int RPB = NTT_THREADS / TPR;
-    static constexpr int SWM = (E < 32 ? E : 32) - 1;
-    static constexpr size_t LDS_BYTES = (size_t)RPB * NPRIMES * M * 4;
-    __device__ static __forceinline__ int slot(int q, int x) {
-        return q * M + (x ^ ((x >> LOG_E) & SWM) ^ ((q * TP) & 31));
-    }
+    static constexpr int SL = TP * (E + 1);  // LDS words per prime slice
+    static constexpr size_t LDS_BYTES = (size_t)RPB * NPRIMES * SL * 4;
+    __device__ static __forceinline__ int slot(int q, int x) { return q * SL + x + (x >> LOG_E); }
 };
 
 __device__ __forceinline__ void ct_butterfly(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp, uint32_t p) {
@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(FwdSeg s1, FwdSeg s
     const int rsub = threadIdx.x / G::TPR;
     const size_t row = (size_t)(second ? bx - blocks1 : bx) * G::RPB + rsub;
     const bool valid = row < rows;
-    uint32_t* S = sh + rsub * NPRIMES * G::M;
+    uint32_t* S = sh + rsub * NPRIMES * G::SL;
     // produce the row once, reduce into the 8 primes
     if (valid) {
         for (int c = tr; c < G::M; c += G::TPR) {
@@ -303,6 +303,9 @@ static void launch_reg(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, 
                        const InvTask& it) {
     using G = RegNtt<E, TP>;
     const size_t b1 = (s1.rows + G::RPB - 1) / G::RPB, b2 = (s2.rows + G::RPB - 1) / G::RPB;
+    if (G::LDS_BYTES > 65536)  // past the default dynamic LDS limit
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ntt_fwd_reg<E, TP>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)G::LDS_BYTES);
     hipLaunchKernelGGL((k_ntt_fwd_reg<E, TP>), dim3((unsigned)(b1 + b2 + (it.chal ? 1 : 0)), jb.n), dim3(NTT_THREADS),
                        G::LDS_BYTES, s, s1, s2, (uint32_t)b1, tw, jb.stride, it);
 }

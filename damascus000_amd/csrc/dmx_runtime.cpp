@@ -484,6 +484,9 @@ static void build_constants(dmx_ctx* c, std::vector<uint32_t>& host_tables) {
         const uint64_t r4 = host_powmod(R, 4, p);
         k.r4_m = (uint32_t)r4;
         k.r4_mp = (uint32_t)r4 * k.pinv;
+        const uint64_t r3 = host_powmod(R, 3, p);
+        k.r3_m = (uint32_t)r3;
+        k.r3_mp = (uint32_t)r3 * k.pinv;
         k.two64 = (uint32_t)host_powmod(R, 2, p);
         // q mod p
         uint64_t qm = 0;

@@ -7,6 +7,9 @@
 //   host_arith_check tx             reads "label round epoch_hex file_id_hex n0 n1 n2", then the
 //                                   K (n0 + n1 + n2) coefficients (hex, one per line), and prints
 //                                   the challenge message assembled from tx_words4 (hex)
+//   host_arith_check modp N SEED    prints N seeded cases per CRT prime (plus edge cases) of the
+//                                   mod-p routines, one per line: "op prime_index operands result"
+//                                   in hex, for the caller to check with exact integers
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -95,8 +98,81 @@ static int run_tx() {
     return 0;
 }
 
+// constants of prime i as the runtime builds them
+static PrimeConsts prime_consts(int i) {
+    const uint64_t p = PRIMES[i];
+    PrimeConsts k;
+    memset(&k, 0, sizeof k);
+    k.p = (uint32_t)p;
+    k.pinv = host_inv32((uint32_t)p);
+    const uint64_t R = (1ull << 32) % p;
+    k.r4_m = (uint32_t)host_powmod(R, 4, p);
+    k.r4_mp = k.r4_m * k.pinv;
+    k.r3_m = (uint32_t)host_powmod(R, 3, p);
+    k.r3_mp = k.r3_m * k.pinv;
+    const unsigned __int128 q = ((unsigned __int128)1 << 112) - 75;
+    k.q_mod_p = (uint32_t)(q % p);
+    return k;
+}
+
+static int run_modp(long n, uint64_t seed) {
+    std::mt19937_64 rng(seed);
+    const zq qm1 = zq{{Q0 - 1, Q1, Q2, Q3}};
+    for (int i = 0; i < NPRIMES; ++i) {
+        const PrimeConsts c = prime_consts(i);
+        const uint32_t p = c.p;
+        const uint32_t edge[] = {0u, 1u, 2u, p - 2, p - 1, p / 2, p / 2 + 1};
+        const uint32_t edge32[] = {0u, 1u, p - 1, p, p + 1, 0x7FFFFFFFu, 0x80000000u, 0xFFFFFFFEu, 0xFFFFFFFFu};
+        auto residue = [&] { return (uint32_t)(rng() % p); };
+        auto word = [&] { return (uint32_t)rng(); };
+        auto pair = [&](uint32_t a, uint32_t b) {
+            printf("add %d %x %x %x\n", i, a, b, addp(a, b, p));
+            printf("sub %d %x %x %x\n", i, a, b, subp(a, b, p));
+        };
+        auto mul = [&](uint32_t b, uint32_t w) {
+            const uint32_t wm = (uint32_t)(((uint64_t)w << 32) % p);
+            printf("mulc %d %x %x %x\n", i, b, w, mulc(b, wm, wm * c.pinv, p));
+        };
+        auto red = [&](uint32_t hi, uint32_t lo) { printf("redc %d %x %x %x\n", i, hi, lo, redc64(hi, lo, p, c.pinv)); };
+        auto zqc = [&](const zq& x) {
+            printf("modp %d %x%08x%08x%08x %x %x\n", i, x.w[3], x.w[2], x.w[1], x.w[0], zq_mod_p(x, c),
+                   zq_mod_p_centred(x, c));
+        };
+        auto u96 = [&](uint64_t lo, uint32_t hi) {
+            printf("u96 %d %llx %x %x\n", i, (unsigned long long)lo, hi, reduce_u96(lo, hi, c));
+        };
+        for (uint32_t a : edge)
+            for (uint32_t b : edge) pair(a, b);
+        for (uint32_t b : edge32)
+            for (uint32_t w : edge) mul(b, w);
+        for (uint32_t h : edge32)
+            for (uint32_t l : edge32) red(h, l);
+        const zq xs[] = {zq_zero(), zq_from_u64(1), qm1, zq{{0xFFFFFFDAu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0x7FFFu}},
+                         zq{{0xFFFFFFDBu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0x7FFFu}}, zq{{0u, 0u, 0u, 0x8000u}},
+                         zq{{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0x7FFEu}}};
+        for (const zq& x : xs) zqc(x);
+        for (uint32_t h : edge32) {
+            u96(~0ull, h);
+            u96(0, h);
+            u96(((uint64_t)p << 32) | (p - 1), h);
+        }
+        for (long t = 0; t < n; ++t) {
+            pair(residue(), residue());
+            mul(word(), residue());
+            red(word(), word());
+            uint64_t lo = rng(), hi = rng() & 0xFFFFFFFFFFFFull;
+            zq x = zq{{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32) & 0xFFFFu}};
+            if (zq_ge_q(x)) x = zq_sub_q(x);
+            zqc(x);
+            u96(rng(), word());
+        }
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 2 && !strcmp(argv[1], "tx")) return run_tx();
+    if (argc >= 4 && !strcmp(argv[1], "modp")) return run_modp(atol(argv[2]), strtoull(argv[3], nullptr, 10));
     if (argc >= 2 && !strcmp(argv[1], "inv")) {
         char line[128];
         while (fgets(line, sizeof line, stdin)) {

@@ -84,3 +84,43 @@ def test_transcript_words_hash_to_the_oracle_challenge(checker, label, rnd, ns):
     x = O.from_le_bytes_mod_order(O.blake3(msg))
     arrs = [O.to_words(m).reshape(8, n, 2) for m, n in zip(mods, ns)]
     assert x == O.challenge(epoch, label, file_id, rnd, *arrs)
+
+
+PRIMES = [3892314113, 2281701377, 2013265921, 2885681153, 2483027969, 1811939329, 469762049, 4194304001]
+
+
+def test_mod_p_routines_against_exact_integers(checker):
+    """The Montgomery routines every transform, MAC pass and lift is built on: addp/subp
+    (canonical sums of residues, which carry out of 32 bits for the primes above 2^31),
+    mulc (b*w mod p for any 32-bit b), redc64 ((hi*2^32 + lo)*2^-32 mod p, below 2^32, below p
+    when hi < p), zq_mod_p and its centred form (the residue of x, or of x - q when
+    x > (q-1)/2), and reduce_u96 (the u96 MAC accumulators), checked against Python integers on
+    edge and seeded cases for all eight primes."""
+    out = subprocess.run([checker, "modp", "3000", "7"], capture_output=True, text=True, check=True).stdout
+    R = 1 << 32
+    counts = {}
+    for line in out.splitlines():
+        f = line.split()
+        op, p = f[0], PRIMES[int(f[1])]
+        v = [int(t, 16) for t in f[2:]]
+        counts[op] = counts.get(op, 0) + 1
+        if op == "add":
+            assert v[2] == (v[0] + v[1]) % p, line
+        elif op == "sub":
+            assert v[2] == (v[0] - v[1]) % p, line
+        elif op == "mulc":
+            assert v[2] == v[0] * v[1] % p, line
+        elif op == "redc":
+            hi, lo, r = v
+            assert r < R and (r * R - (hi * R + lo)) % p == 0, line
+            if hi < p:
+                assert r < p, line
+        elif op == "modp":
+            x, r, rc = v
+            assert x < Q
+            assert r == x % p, line
+            assert rc == ((x - Q) % p if x > (Q - 1) // 2 else x % p), line
+        elif op == "u96":
+            lo, hi, r = v
+            assert r == (hi * 2**64 + lo) % p, line
+    assert all(counts.get(k, 0) >= 8 * 3000 for k in ("add", "sub", "mulc", "redc", "modp", "u96")), counts
