@@ -409,13 +409,35 @@ DMX_HD uint32_t addp(uint32_t a, uint32_t b, uint32_t p) {
 }
 DMX_HD uint32_t subp(uint32_t a, uint32_t b, uint32_t p) { return sub_fix(a, b, p); }
 
-// x mod p for x < 2^128 given as four limbs w0..w3 (w3 < p): Horner over limb pairs,
-// y = (w0 + w1 R) R^-1, then (y + w2 R) R^-1, then (y + w3 R) R^-1 = x R^-3, then * R^3.
+// a - b - bin and the borrow out of that subtraction
+DMX_HD uint32_t sub_borrow(uint32_t a, uint32_t b, bool bin, bool& bout) {
+    uint32_t t, r;
+    const bool b1 = __builtin_usub_overflow(a, b, &t);
+    const bool b2 = __builtin_usub_overflow(t, (uint32_t)bin, &r);
+    bout = b1 || b2;
+    return r;
+}
+
+// A chain of REDC steps over the limbs w0, w1, ... of a value: step k forms
+// d_k = w_k - mulhi(t_{k-1} * p^-1, p) - b_{k-1}, where t_{k-1} is the previous d mod 2^32
+// and b_{k-1} its sign. Leaving out each step's +p correction and subtracting the borrow from
+// the next high word instead keeps d_k = (w_k R + d_{k-1}) R^-1 (mod p) with d_k in [-p, 2^32).
+// Starts with t_0 = w0, b_0 = 0.
+struct RedcChain {
+    uint32_t t;
+    bool b;
+    DMX_HD void step(uint32_t hi, uint32_t p, uint32_t pinv) { t = sub_borrow(hi, mulhi32(t * pinv, p), b, b); }
+    DMX_HD uint32_t value(uint32_t p) const { return t + (b ? p : 0u); }  // in [0, 2^32), below p if d < 0
+};
+
+// x mod p for x < 2^128 given as four limbs w0..w3 (w3 < 2^16 here): the chain gives
+// d_3 = x R^-3 (mod p), d_3 in [-p, 2^16); then * R^3.
 DMX_HD uint32_t zq_mod_p(const zq& x, const PrimeConsts& c) {
-    uint32_t y = redc64(x.w[1], x.w[0], c.p, c.pinv);
-    y = redc64(x.w[2], y, c.p, c.pinv);
-    y = redc64(x.w[3], y, c.p, c.pinv);
-    return mulc(y, c.r4_m, c.r4_mp, c.p);
+    RedcChain r{x.w[0], false};
+    r.step(x.w[1], c.p, c.pinv);
+    r.step(x.w[2], c.p, c.pinv);
+    r.step(x.w[3], c.p, c.pinv);
+    return mulc(r.value(c.p), c.r4_m, c.r4_mp, c.p);
 }
 
 // Residue of the centred representative of x (x - q when x > (q-1)/2). Products of centred
@@ -432,11 +454,12 @@ DMX_HD uint32_t zq_mod_p_centred(const zq& x, const PrimeConsts& c) {
 }
 
 // (hi * 2^64 + lo) mod p for the u96 accumulators of the MAC kernels: two REDC steps give
-// V * R^-2 (in [0, 2^32)), then * R^2
+// V * R^-2 (mod p), then * R^2
 DMX_HD uint32_t reduce_u96(uint64_t lo, uint32_t hi, const PrimeConsts& c) {
-    uint32_t y = redc64((uint32_t)(lo >> 32), (uint32_t)lo, c.p, c.pinv);
-    y = redc64(hi, y, c.p, c.pinv);
-    return mulc(y, c.r3_m, c.r3_mp, c.p);
+    RedcChain r{(uint32_t)lo, false};
+    r.step((uint32_t)(lo >> 32), c.p, c.pinv);
+    r.step(hi, c.p, c.pinv);
+    return mulc(r.value(c.p), c.r3_m, c.r3_mp, c.p);
 }
 
 // ------------------------------------------------------------------------------------------

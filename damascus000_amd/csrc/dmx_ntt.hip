@@ -257,8 +257,9 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(FwdSeg s1, FwdSeg s
     }
     __syncthreads();
     const int q = tr >> G::LOG_TP, l = tr & (TP - 1);
-    const uint32_t p = c_pc[q].p, pinv = c_pc[q].pinv;
+    const uint32_t p = c_pc[q].p;
     const uint32_t* zt = tw.zeta_m + q * TW_SIZE;
+    const uint32_t* ztp = tw.zeta_mp + q * TW_SIZE;
     uint32_t a[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) a[i] = S[G::slot(q, l + TP * i)];
@@ -267,8 +268,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(FwdSeg s1, FwdSeg s
     for (int s = E / 2; s >= 1; s >>= 1) {
 #pragma unroll
         for (int b = 0; b < E / (2 * s); ++b) {
-            const uint32_t w = zt[E / (2 * s) + b];
-            const uint32_t wp = w * pinv;
+            const uint32_t w = zt[E / (2 * s) + b], wp = ztp[E / (2 * s) + b];
 #pragma unroll
             for (int k = 0; k < s; ++k) ct_butterfly(a[2 * s * b + k], a[2 * s * b + k + s], w, wp, p);
         }
@@ -285,8 +285,7 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_fwd_reg(FwdSeg s1, FwdSeg s
         const int base = G::M / (2 * len) + (E * l) / (2 * len);
 #pragma unroll
         for (int b = 0; b < E / (2 * len); ++b) {
-            const uint32_t w = zt[base + b];
-            const uint32_t wp = w * pinv;
+            const uint32_t w = zt[base + b], wp = ztp[base + b];
 #pragma unroll
             for (int k = 0; k < len; ++k) ct_butterfly(a[2 * len * b + k], a[2 * len * b + k + len], w, wp, p);
         }
