@@ -309,6 +309,115 @@ static void launch_reg(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, 
                        G::LDS_BYTES, s, s1, s2, (uint32_t)b1, tw, jb.stride, it);
 }
 
+// ---------------------------------------------------------------------------------------
+// Three-pass register transform for degree 2^11: 64 threads per prime slice holding 32 elements
+// each, 512 threads per row. Halving the elements per thread against the two-pass <64, 32>
+// blocking halves the LDS words per resident thread, so a CU holds twice the waves. Pass 1:
+// thread l holds x = l + 64 i (layers len 1024..64); pass 2: x = b + 2 e + 64 h for l = 2 h + b
+// (layers 32..2); pass 3: x = 32 l + e (layer 1). One padded LDS layout, word x + (x >> 5), serves
+// the producer and all three passes without bank conflicts, and each pass writes back the words
+// it read, so three barriers order the row.
+// ---------------------------------------------------------------------------------------
+constexpr int R3_M = 2048, R3_LOG_M = 11, R3_TP = 64, R3_THREADS = NPRIMES * R3_TP;
+constexpr int R3_SL = 2112;  // LDS words per prime slice: slots x + (x >> 5) < 2111
+constexpr size_t R3_LDS_BYTES = (size_t)NPRIMES * R3_SL * 4;
+__device__ __forceinline__ int r3_slot(int x) { return x + (x >> 5); }
+
+__global__ void __launch_bounds__(R3_THREADS, 4) k_ntt_fwd_reg3(FwdSeg s1, FwdSeg s2, uint32_t blocks1, NttTables tw,
+                                                                 size_t js, InvTask it) {
+    extern __shared__ uint32_t sh[];
+    uint32_t bx = blockIdx.x;
+    if (it.chal) {  // uniform over the workgroup; the task workgroup meets no barrier
+        if (bx == 0) return inverse_task(it, job_off(js));
+        --bx;
+    }
+    const bool second = bx >= blocks1;
+    RowSrc src = second ? s2.src : s1.src;
+    uint32_t* out = second ? s2.out : s1.out;
+    const size_t jo = job_off(js);
+    if (!src.a_shared) src.a = jp(src.a, jo);
+    src.dst = jp(src.dst, jo);
+    src.dst2 = jp(src.dst2, jo);
+    src.scal = jp(src.scal, jo);
+    src.gseed = jp(src.gseed, jo);
+    out = jp(out, jo);
+    uint32_t key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (src.mode == ROW_GEN) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) key[i] = src.gseed[i];
+    }
+    const size_t row = second ? bx - blocks1 : bx;  // one row per workgroup, every row valid
+    const int tid = threadIdx.x;
+    for (int c = tid; c < R3_M; c += R3_THREADS) {
+        const zq x = row_value(src, key, row, c, R3_LOG_M);
+        if (src.dst) store_zq(src.dst + row * R3_M + c, x);
+        if (src.dst2) store_zq(src.dst2 + row * R3_M + c, x);
+#pragma unroll
+        for (int q = 0; q < NPRIMES; ++q) sh[q * R3_SL + r3_slot(c)] = zq_mod_p_centred(x, c_pc[q]);
+    }
+    __syncthreads();
+    const int q = tid >> 6, l = tid & 63;
+    uint32_t* S = sh + q * R3_SL;
+    const uint32_t p = c_pc[q].p;
+    const uint32_t* zt = tw.zeta_m + q * TW_SIZE;
+    const uint32_t* ztp = tw.zeta_mp + q * TW_SIZE;
+    uint32_t a[32];
+    // pass 1: registers i = 0..31 hold x = l + 64 i; layer len = 64 s pairs i, i + s and uses
+    // zeta[16 / s + i / (2 s)]
+#pragma unroll
+    for (int i = 0; i < 32; ++i) a[i] = S[r3_slot(l + 64 * i)];
+#pragma unroll
+    for (int s = 16; s >= 1; s >>= 1) {
+#pragma unroll
+        for (int b = 0; b < 16 / s; ++b) {
+            const uint32_t w = zt[16 / s + b], wp = ztp[16 / s + b];
+#pragma unroll
+            for (int k = 0; k < s; ++k) ct_butterfly(a[2 * s * b + k], a[2 * s * b + k + s], w, wp, p);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 32; ++i) S[r3_slot(l + 64 * i)] = a[i];
+    __syncthreads();
+    // pass 2: registers e hold x = b + 2 e + 64 h; layer len = 2 s pairs e, e + s and uses
+    // zeta[512 / s + 16 h / s + e / (2 s)]
+    const int h = l >> 1, b0 = l & 1;
+#pragma unroll
+    for (int e = 0; e < 32; ++e) a[e] = S[r3_slot(b0 + 2 * e + 64 * h)];
+#pragma unroll
+    for (int s = 16; s >= 1; s >>= 1) {
+        const int base = 512 / s + (16 / s) * h;
+#pragma unroll
+        for (int b = 0; b < 16 / s; ++b) {
+            const uint32_t w = zt[base + b], wp = ztp[base + b];
+#pragma unroll
+            for (int k = 0; k < s; ++k) ct_butterfly(a[2 * s * b + k], a[2 * s * b + k + s], w, wp, p);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 32; ++e) S[r3_slot(b0 + 2 * e + 64 * h)] = a[e];
+    __syncthreads();
+    // pass 3: registers e hold x = 32 l + e; layer len = 1 pairs e, e + 1 (e even) and uses
+    // zeta[1024 + 16 l + e / 2]
+#pragma unroll
+    for (int e = 0; e < 32; ++e) a[e] = S[r3_slot(32 * l + e)];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const uint32_t w = zt[1024 + 16 * l + j], wp = ztp[1024 + 16 * l + j];
+        ct_butterfly(a[2 * j], a[2 * j + 1], w, wp, p);
+    }
+    uint4* o = reinterpret_cast<uint4*>(out + (row * NPRIMES + q) * R3_M + 32 * l);
+#pragma unroll
+    for (int e = 0; e < 32; e += 4) o[e / 4] = make_uint4(a[e], a[e + 1], a[e + 2], a[e + 3]);
+}
+
+static void launch_reg3(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, hipStream_t s, const Jobs& jb,
+                        const InvTask& it) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ntt_fwd_reg3),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)R3_LDS_BYTES);
+    hipLaunchKernelGGL(k_ntt_fwd_reg3, dim3((unsigned)(s1.rows + s2.rows + (it.chal ? 1 : 0)), jb.n),
+                       dim3(R3_THREADS), R3_LDS_BYTES, s, s1, s2, (uint32_t)s1.rows, tw, jb.stride, it);
+}
+
 // DAMASCUS_NTT_KERNEL=layer keeps every degree on the per-layer kernel (for comparisons)
 static bool reg_ntt_enabled() {
     static const bool on = [] {
@@ -319,11 +428,12 @@ static bool reg_ntt_enabled() {
 }
 
 // Variants per degree: variant 0 and 1 are register blockings (E elements x TP threads per prime
-// slice; TP <= E, TP <= 32, E * TP = m), the last is the per-layer kernel. Every variant computes
-// the same residues; the context picks the fastest per degree by timing.
+// slice; TP <= E, TP <= 32, E * TP = m; at 2^11 variant 1 is the three-pass kernel), the last is
+// the per-layer kernel. Every variant computes the same residues; the context picks the fastest
+// per degree by timing.
 int fwd_variant_count(int log_m) {
     if (!reg_ntt_enabled()) return 1;
-    return log_m >= 6 && log_m <= 10 ? 3 : log_m == 11 ? 2 : 1;
+    return log_m >= 6 && log_m <= 11 ? 3 : 1;
 }
 
 void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, const RowSrc& src2, size_t rows2,
@@ -346,6 +456,7 @@ void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, cons
             case 20: return launch_reg<32, 32>(s1, s2, tw, s, jb, it);
             case 21: return launch_reg<64, 16>(s1, s2, tw, s, jb, it);
             case 22: return launch_reg<64, 32>(s1, s2, tw, s, jb, it);
+            case 23: return launch_reg3(s1, s2, tw, s, jb, it);
             default: break;
         }
     }

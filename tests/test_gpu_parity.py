@@ -463,9 +463,10 @@ def test_fused_job_rejects_a_wrong_final_opening():
 
 
 def test_transform_variants_give_identical_proofs():
-    """Every forward-transform variant (register blockings of the prime slices and the per-layer
-    kernel) gives the same proof: at d = 7 each against the oracle, at d = 11 (degrees 2^11 down to
-    2^6) across variants through the fused job, whose captured graph is retaken after a change."""
+    """Every forward-transform variant (register blockings of the prime slices, the three-pass
+    kernel at 2^11 and the per-layer kernel) gives the same proof: at d = 7 each against the
+    oracle, at d = 11 (degrees 2^11 down to 2^6) across variants through the fused job, whose
+    captured graph is retaken after a change."""
     ctx = dm.Context.default()
     saved = {lg: ctx.transform_variant(lg)[0] for lg in range(6, 12)}
     say("tuned transform variants (variant, count, us):", {lg: ctx.transform_variant(lg) for lg in range(6, 12)})
@@ -484,7 +485,7 @@ def test_transform_variants_give_identical_proofs():
             outs.append(dm.prove_verify_device(buf.ptr, len(big)))
         assert outs[0] == outs[1] == outs[2]
         with pytest.raises(dm.DamascusError):
-            ctx.set_transform_variant(11, 2)  # degree 2^11 has two variants
+            ctx.set_transform_variant(11, 3)  # degree 2^11 has three variants
     finally:
         for lg, v in saved.items():
             ctx.set_transform_variant(lg, v)
