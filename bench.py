@@ -46,7 +46,9 @@ sys.path.insert(0, ROOT)
 FILE_BYTES = 8 << 20
 SEED = 2
 PEAK_LANE_CYCLES = 256 * 64 * 2.4e9  # CUs x lanes x clock: 39.32 T lane-cycles/s
-DEFAULT_REQUESTS = 400
+SEGMENT = 100  # requests per step: one step is a 100-request segment of the open-loop served trace
+DEFAULT_STEPS = 4  # 400 requests, 0.2 s of arrivals at the default rate
+COMPARE_REQUESTS = 400  # the rate sweep and the own-seed runs use the first 400 requests of the trace
 DEFAULT_RATE = 2000.0  # requests per second (about 5.6 GiB/s offered)
 SWEEP_RATES = (600.0, 1000.0, 3000.0, 4000.0)  # besides the headline rate; 600/s is near the plain path's capacity
 K = 8
@@ -120,11 +122,19 @@ def served_trace(requests, rate, seed=5):
     the sizes and bytes do not depend on the rate (it only scales the gaps)"""
     import numpy as np
 
+    from concurrent.futures import ThreadPoolExecutor
+
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, size=requests)
     arrivals = np.cumsum(gaps) - gaps[0]
     sizes = np.floor(65536 * 256.0 ** rng.random(requests)).astype(np.int64)
-    return [(float(a), rand_bytes(rng, n)) for a, n in zip(arrivals, sizes)]
+
+    def file_bytes(i):  # file i from its own stream, so files can be drawn in parallel
+        return rand_bytes(np.random.default_rng((seed, i)), sizes[i])
+
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        files = list(ex.map(file_bytes, range(requests)))
+    return [(float(a), f) for a, f in zip(arrivals, files)]
 
 
 def own_seed_params(dm, n, tag=b"request"):
@@ -217,8 +227,11 @@ def mixed_served(dm, ctx, requests, rate, warmup):
     bufs = [ctx.upload(f) for _, f in trace]
     dev = [(b.ptr, len(f)) for b, (_, f) in zip(bufs, trace)]
     mib = sum(len(f) for _, f in trace) / (1 << 20)
-    w = max(1, min(warmup, requests))
-    run_served(dm, ctx, dev[:w], arrivals[:w])  # untimed: workspaces, graphs, resident generators
+    # untimed: another trace of the same size and depth mix (workspaces, graphs, resident generators)
+    warm = served_trace(max(1, warmup), rate, seed=6)
+    warm_bufs = [ctx.upload(f) for _, f in warm]
+    run_served(dm, ctx, [(b.ptr, len(f)) for b, (_, f) in zip(warm_bufs, warm)], [a for a, _ in warm])
+    del warm_bufs, warm
     served, s_lat, s_span = run_served(dm, ctx, dev, arrivals)  # the timed region
     head = summary(mib, s_lat, s_span, arrivals)
     head.update({"requests": requests, "rate_per_s": rate, "makespan_s": round(s_span, 4),
@@ -233,41 +246,47 @@ def mixed_served(dm, ctx, requests, rate, warmup):
     if any(not isinstance(r, tuple) for r in served):
         raise SystemExit("a served request failed")
     out["speedup_vs_plain"] = round(out["plain"]["mib_s"] and head["mib_s"] / out["plain"]["mib_s"], 2)
-    # the same trace handed over in host memory (PCIe-inclusive: pinned staging + async upload)
-    host = [f for _, f in trace]
-    run_served(dm, ctx, host[:w], arrivals[:w], files_on_host=True)
-    h_res, h_lat, h_span = run_served(dm, ctx, host, arrivals, files_on_host=True)
-    if h_res != served:
+    # The comparisons below use the first COMPARE_REQUESTS requests of the trace, so their cost
+    # does not grow with the number of timed steps.
+    c = min(len(dev), COMPARE_REQUESTS)
+    cdev, carr, cserved = dev[:c], arrivals[:c], served[:c]
+    cmib = sum(n for _, n in cdev) / (1 << 20)
+    w = min(c, 32)
+    out["comparison_requests"] = c
+    # the same files handed over in host memory (PCIe-inclusive: pinned staging + async upload)
+    host = [f for _, f in trace[:c]]
+    run_served(dm, ctx, host[:w], carr[:w], files_on_host=True)
+    h_res, h_lat, h_span = run_served(dm, ctx, host, carr, files_on_host=True)
+    if h_res != cserved:
         raise SystemExit("host-file and device-file served proofs differ")
-    out["host_files"] = summary(mib, h_lat, h_span, arrivals)
+    out["host_files"] = summary(cmib, h_lat, h_span, carr)
     # arrival-rate sweep over the same files (the gaps scale with the rate)
     sweep = {}
-    for r in SWEEP_RATES:
-        arr = [a * rate / r for a in arrivals]
-        res, lat, span = run_served(dm, ctx, dev, arr)
-        if res != served:
+    for r in SWEEP_RATES + (rate,):
+        arr = [a * rate / r for a in carr]
+        res, lat, span = run_served(dm, ctx, cdev, arr)
+        if res != cserved:
             raise SystemExit(f"served proofs at {r}/s differ")
-        sweep[f"{r:g}"] = summary(mib, lat, span, arr)
+        sweep[f"{r:g}"] = summary(cmib, lat, span, arr)
         if r == SWEEP_RATES[0]:  # near the plain path's capacity: the plain path at the same rate
-            _, pl_lat, pl_span = run_plain(dm, dev, arr)
-            sweep[f"{r:g}"]["plain"] = summary(mib, pl_lat, pl_span, arr)
-    sweep[f"{rate:g}"] = {k: head[k] for k in ("mib_s", "p50_ms", "p99_ms", "offered_mib_s")}
+            _, pl_lat, pl_span = run_plain(dm, cdev, arr)
+            sweep[f"{r:g}"]["plain"] = summary(cmib, pl_lat, pl_span, arr)
     out["rate_sweep"] = sweep
     # every request with its own generator seed, none of them seen before the timed pass (the
     # warm-up uses other seeds): no generator matrix is reused across requests, so this separates
     # batching and overlap from the generator cache
-    params = own_seed_params(dm, len(dev))
-    run_served(dm, ctx, dev[:w], arrivals[:w], params=own_seed_params(dm, w, b"warm-up"))
-    o_res, o_lat, o_span = run_served(dm, ctx, dev, arrivals, params=params)
-    run_plain(dm, dev[:16], arrivals[:16], params[:16])
-    op_res, op_lat, op_span = run_plain(dm, dev, arrivals, params)
+    params = own_seed_params(dm, c)
+    run_served(dm, ctx, cdev[:w], carr[:w], params=own_seed_params(dm, w, b"warm-up"))
+    o_res, o_lat, o_span = run_served(dm, ctx, cdev, carr, params=params)
+    run_plain(dm, cdev[:16], carr[:16], params[:16])
+    op_res, op_lat, op_span = run_plain(dm, cdev, carr, params)
     if o_res != op_res:
         raise SystemExit("own-seed served and plain proofs differ")
-    out["own_seeds"] = {"served": summary(mib, o_lat, o_span, arrivals), "plain": summary(mib, op_lat, op_span, arrivals)}
-    out["own_seeds"]["speedup_vs_plain"] = round((mib / o_span) / (mib / op_span), 2)
+    out["own_seeds"] = {"served": summary(cmib, o_lat, o_span, carr), "plain": summary(cmib, op_lat, op_span, carr)}
+    out["own_seeds"]["speedup_vs_plain"] = round((cmib / o_span) / (cmib / op_span), 2)
     out["bit_exact_vs_plain"] = True
     global LAST_TRACE
-    LAST_TRACE = (host, served)
+    LAST_TRACE = ([f for _, f in trace], served)
     return out
 
 
@@ -502,10 +521,11 @@ def calibrate(dm, ctx, mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=DEFAULT_REQUESTS,
-                    help="requests in the timed served pass (a step is one request)")
-    ap.add_argument("--warmup", type=int, default=DEFAULT_REQUESTS,
-                    help="requests of the same trace in the untimed served pass before it")
+    ap.add_argument("--steps", type=int, default=DEFAULT_STEPS,
+                    help="segments of %d requests in the timed served trace (a step is one segment; the "
+                         "segments form one continuous Poisson stream)" % SEGMENT)
+    ap.add_argument("--warmup", type=int, default=DEFAULT_STEPS,
+                    help="segments of %d requests of another trace, served untimed before it" % SEGMENT)
     ap.add_argument("--rate", type=float, default=DEFAULT_RATE, help="served trace arrival rate (requests/s)")
     ap.add_argument("--c2-reps", type=int, default=10, help="timed graph replays of the 8 MiB single file")
     ap.add_argument("--inflight", type=int, default=4, help="server batch slots")
@@ -541,7 +561,7 @@ def main():
         return
 
     if args.only == "served":
-        trace = served_trace(args.steps, args.rate)
+        trace = served_trace(args.steps * SEGMENT, args.rate)
         arrivals = [a for a, _ in trace]
         bufs = [ctx.upload(f) for _, f in trace]
         dev = [(b.ptr, len(f)) for b, (_, f) in zip(bufs, trace)]
@@ -549,7 +569,7 @@ def main():
         run_served(dm, ctx, dev, arrivals)
         _, lat, span = run_served(dm, ctx, dev, arrivals)
         out = summary(mib, lat, span, arrivals)
-        out.update({"requests": args.steps, "rate": args.rate, "server": LAST_STATS})
+        out.update({"requests": args.steps * SEGMENT, "rate": args.rate, "server": LAST_STATS})
         print(json.dumps(out))
         return
     if args.only == "ragged":
@@ -673,9 +693,10 @@ def main():
         "vs_baseline": None,
         "dtype": "u32 (mod-p residues), u128 (Z_q)",
         "data": "synthetic: seeded uniform random bytes",
-        "config": {"workload": f"served trace: {args.steps} commit+fold requests (prove + verifier replay), "
+        "config": {"workload": f"served trace: {args.steps * SEGMENT} commit+fold requests (prove + verifier replay), "
                                f"Poisson arrivals at {args.rate:g}/s, sizes log-uniform 64 KiB-16 MiB (d=7..11), "
-                               "K=8, files resident in HBM; a step is one request", "requests": args.steps,
+                               f"K=8, files resident in HBM; a step is one segment of {SEGMENT} requests",
+                   "requests": args.steps * SEGMENT,
                    "rate_per_s": args.rate, "module_rank": 8, "parallelism": "one GPU; jobs of one depth share launches"},
         "single_file": single,
         "transform_tuning": {f"2^{lg}": dict(zip(("variant", "variants", "tuned_us"), ctx.transform_variant(lg)))
@@ -687,7 +708,7 @@ def main():
         "host": {"nproc": nproc, "cpu_count": ncpu},
     }
     if not args.no_mixed:
-        mx = mixed_served(dm, ctx, args.steps, args.rate, args.warmup)
+        mx = mixed_served(dm, ctx, args.steps * SEGMENT, args.rate, args.warmup * SEGMENT)
         head = mx["headline"]
         line["value"] = head["mib_s"]
         line["ms_per_step"] = round(head["makespan_s"] * 1e3 / args.steps, 4)
