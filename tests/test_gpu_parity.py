@@ -532,13 +532,15 @@ def test_tuning_cache_round_trip(tmp_path):
         assert a.tuning_source() == ("timed", path) and os.path.exists(path)
         b = dm.Context()
         assert b.tuning_source() == ("cache", path)
+        # the cache keeps times to 0.001 us and the mirror rounds them to 0.01 us, so a time on a
+        # rounding boundary may come back one step of 0.01 away
         for lg in range(0, 14):
             va, na, ta = a.transform_variant(lg)
             vb, nb, tb = b.transform_variant(lg)
-            assert (va, na) == (vb, nb) and all(abs(x - y) < 0.01 for x, y in zip(ta, tb))
+            assert (va, na) == (vb, nb) and all(abs(x - y) < 0.015 for x, y in zip(ta, tb)), (lg, ta, tb)
             sa, ca, ua = a.mac_shape(lg)
             sb, cb, ub = b.mac_shape(lg)
-            assert (sa, ca) == (sb, cb) and all(abs(x - y) < 0.01 for x, y in zip(ua, ub))
+            assert (sa, ca) == (sb, cb) and all(abs(x - y) < 0.015 for x, y in zip(ua, ub)), (lg, ua, ub)
         with open(path) as f:
             assert f.readline().split()[:2] == ["dmx-tune", "2"]
         say("tuning cache", path, "MAC shapes", {lg: b.mac_shape(lg)[0] for lg in range(6, 11)})
