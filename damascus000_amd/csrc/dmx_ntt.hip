@@ -2814,6 +2814,7 @@ struct RingCfg {
     static constexpr int SLICE = M + TP;                    // words per prime slice, padded
     static_assert(TP <= E && TP <= 64 && TPR >= 64 && RPB >= 1, "ring blocking");
     static_assert(RPB * NPRIMES * SLICE <= (int)RING_LDS_WORDS, "ring LDS");
+    static_assert(RPB <= 8, "an item's product offsets are staged in 8 LDS words");
     // LDS word of coefficient x of prime slice q: one pad word after every E coefficients, so the
     // contiguous pattern (lane l at E*l + e) has lane stride E + 1 (odd: distinct banks) and the
     // strided pattern (lane l at l + TP*i) is the lane's base plus a constant per i
@@ -2915,27 +2916,37 @@ __device__ __forceinline__ void ring_inv(uint32_t* S, int q, int l, uint32_t (&x
 
 // An operand of every product of the item into LDS: each coefficient is loaded once (16-byte
 // loads, consecutive threads on consecutive coefficients) and reduced into all 8 primes, stored at
-// its slot in the product's region. The whole workgroup takes part. (Issuing a thread's E/8 loads
-// together costs registers the E = 64 blockings do not have; the kernel is bound by VALU issue.)
+// its slot in the product's region. The whole workgroup takes part. A thread issues all of its
+// loads (at most 8) before reducing any, so the item waits for one load latency, not one per
+// coefficient; `item_off` (LDS) holds each product's first coefficient.
 template <int E, int TP>
-__device__ __forceinline__ void ring_stage(uint32_t* sh, const zq* __restrict__ src, const uint64_t* offs,
-                                           const uint32_t* order, uint32_t first, uint32_t cnt, int tid) {
+__device__ __forceinline__ void ring_stage(uint32_t* sh, const zq* __restrict__ src, const uint64_t* item_off,
+                                           uint32_t cnt, int tid) {
     using C = RingCfg<E, TP>;
-#pragma unroll 1
-    for (int idx = tid; idx < (int)cnt * C::M; idx += RING_LDS_THREADS) {
-        const int prod = idx >> C::LOG_M, c = idx & (C::M - 1);
-        const zq x = load_zq(src + offs[order[first + prod]] + c);
-        uint32_t* S = sh + prod * NPRIMES * C::SLICE;
+    constexpr int ITERS = (C::RPB * C::M + RING_LDS_THREADS - 1) / RING_LDS_THREADS;
+    const int total = (int)cnt * C::M;
+    zq x[ITERS];
 #pragma unroll
-        for (int q = 0; q < NPRIMES; ++q) S[C::slot(q, c)] = zq_mod_p_centred(x, c_pc[q]);
+    for (int u = 0; u < ITERS; ++u) {
+        const int idx = tid + u * RING_LDS_THREADS;
+        if (idx < total) x[u] = load_zq(src + item_off[idx >> C::LOG_M] + (idx & (C::M - 1)));
+    }
+#pragma unroll
+    for (int u = 0; u < ITERS; ++u) {
+        const int idx = tid + u * RING_LDS_THREADS;
+        if (idx < total) {
+            const int prod = idx >> C::LOG_M, c = idx & (C::M - 1);
+            uint32_t* S = sh + prod * NPRIMES * C::SLICE;
+#pragma unroll
+            for (int q = 0; q < NPRIMES; ++q) S[C::slot(q, c)] = zq_mod_p_centred(x[u], c_pc[q]);
+        }
     }
 }
 
 // one queue item of `cnt` products of degree E*TP; ends with a workgroup barrier
 template <int E, int TP>
 __device__ void ring_item(uint32_t* sh, const zq* __restrict__ A, const zq* __restrict__ B, zq* __restrict__ O,
-                          const uint64_t* offs, const uint32_t* order, uint32_t first, uint32_t cnt,
-                          const NttTables& tw, int tid) {
+                          const uint64_t* item_off, uint32_t cnt, const NttTables& tw, int tid) {
     using C = RingCfg<E, TP>;
     const int rsub = tid / C::TPR, tr = tid % C::TPR;
     const bool valid = rsub < (int)cnt;  // uniform per wave (TPR >= 64)
@@ -2944,7 +2955,7 @@ __device__ void ring_item(uint32_t* sh, const zq* __restrict__ A, const zq* __re
     const PrimeConsts pc = c_pc[q];
     const uint32_t* zt = tw.zeta_m + q * TW_SIZE;
     uint32_t a[E], b[E];
-    ring_stage<E, TP>(sh, A, offs, order, first, cnt, tid);
+    ring_stage<E, TP>(sh, A, item_off, cnt, tid);
     __syncthreads();
     if (valid) {
 #pragma unroll
@@ -2953,7 +2964,7 @@ __device__ void ring_item(uint32_t* sh, const zq* __restrict__ A, const zq* __re
     }
     phase_boundary();
     __syncthreads();  // every slice has read its elements of a: the regions take b
-    ring_stage<E, TP>(sh, B, offs, order, first, cnt, tid);
+    ring_stage<E, TP>(sh, B, item_off, cnt, tid);
     __syncthreads();
     phase_boundary();
     if (valid) {
@@ -2973,7 +2984,7 @@ __device__ void ring_item(uint32_t* sh, const zq* __restrict__ A, const zq* __re
     __syncthreads();  // every prime's residues of the item in LDS
     phase_boundary();
     if (valid) {
-        const size_t off = offs[order[first + rsub]];
+        const size_t off = item_off[rsub];
 #pragma unroll 1
         for (int c = tr; c < C::M; c += C::TPR) {
             uint32_t r[NPRIMES];
@@ -3046,6 +3057,7 @@ __global__ void __launch_bounds__(RING_LDS_THREADS) k_ring_mul_lds(const zq* __r
                                                                    NttTables tw) {
     extern __shared__ uint32_t sh[];
     __shared__ uint32_t item_sh;
+    __shared__ uint64_t item_off[8];  // each product's first coefficient (at most 8 products per item)
     const uint32_t n_items = hdr[0];
     for (;;) {
         if (threadIdx.x == 0) item_sh = atomicAdd(hdr + 1, 1u);
@@ -3055,19 +3067,21 @@ __global__ void __launch_bounds__(RING_LDS_THREADS) k_ring_mul_lds(const zq* __r
         const uint64_t d = items[it];
         const uint32_t first = (uint32_t)d, cnt = (uint32_t)(d >> 40) & 0xFFu;
         const int log_n = (int)((d >> 32) & 0xFFu);
+        if (threadIdx.x < cnt) item_off[threadIdx.x] = offs[order[first + threadIdx.x]];
+        __syncthreads();
         // the thread index as an opaque per-item value: what the item code derives from it is
         // computed per item instead of being hoisted out of this loop for every degree at once
         int tid = (int)threadIdx.x;
         __asm__ volatile("" : "+v"(tid));
         switch (log_n) {
-            case 6: ring_item<8, 8>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
-            case 7: ring_item<16, 8>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
-            case 8: ring_item<16, 16>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
-            case 9: ring_item<32, 16>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
-            case 10: ring_item<32, 32>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
-            case 11: ring_item<64, 32>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
-            case 12: ring_item<64, 64>(sh, A, B, O, offs, order, first, cnt, tw, tid); break;
-            default: ring_item_small(sh, A, B, O, offs[order[first]], log_n, tw, tid); break;
+            case 6: ring_item<8, 8>(sh, A, B, O, item_off, cnt, tw, tid); break;
+            case 7: ring_item<16, 8>(sh, A, B, O, item_off, cnt, tw, tid); break;
+            case 8: ring_item<16, 16>(sh, A, B, O, item_off, cnt, tw, tid); break;
+            case 9: ring_item<32, 16>(sh, A, B, O, item_off, cnt, tw, tid); break;
+            case 10: ring_item<32, 32>(sh, A, B, O, item_off, cnt, tw, tid); break;
+            case 11: ring_item<64, 32>(sh, A, B, O, item_off, cnt, tw, tid); break;
+            case 12: ring_item<64, 64>(sh, A, B, O, item_off, cnt, tw, tid); break;
+            default: ring_item_small(sh, A, B, O, item_off[0], log_n, tw, tid); break;
         }
     }
 }
