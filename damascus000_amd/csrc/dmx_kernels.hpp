@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "dmx_arith.hpp"
 
 namespace dmx {
@@ -70,9 +72,13 @@ struct Jobs {
     size_t stride = 0;
 };
 
+// (Byte arithmetic on a char pointer, not through an integer: the compiler then still knows a
+// kernel argument's pointer is global memory and emits global loads and stores instead of flat
+// ones, which would also count against the LDS wait counter.)
 template <class T>
 __device__ __forceinline__ T* jp(T* p, size_t jo) {
-    return p ? reinterpret_cast<T*>(reinterpret_cast<uintptr_t>(p) + jo) : p;
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return p ? reinterpret_cast<T*>(reinterpret_cast<B*>(p) + jo) : p;
 }
 __device__ __forceinline__ size_t job_off(size_t js) { return (size_t)blockIdx.y * js; }
 

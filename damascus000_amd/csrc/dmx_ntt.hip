@@ -694,6 +694,7 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_pairs4(const uint32_t* __re
     }
     if (!cross) {
         const uint32_t i0 = seg * seg_rows, i1 = min(rows, i0 + seg_rows);
+#pragma unroll 4
         for (uint32_t i = i0; i < i1; ++i) {
             const uint4 w = ld4(wp + i * wstride), g = ld4(gp + i * gstride);
             inv[0].mac(w.x, g.x);
@@ -704,9 +705,7 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_pairs4(const uint32_t* __re
     } else {
         const uint32_t h = rows >> 1;
         const uint32_t i0 = seg * seg_rows, i1 = min(h, i0 + seg_rows);
-        for (uint32_t i = i0; i < i1; ++i) {
-            const uint4 w0 = ld4(wp + i * wstride), w1 = ld4(wp + (i + h) * wstride);
-            const uint4 g0 = ld4(gp + i * gstride), g1 = ld4(gp + (i + h) * gstride);
+        auto pair_mac = [&](const uint4& w0, const uint4& w1, const uint4& g0, const uint4& g1) {
             const uint32_t W0[4] = {w0.x, w0.y, w0.z, w0.w}, W1[4] = {w1.x, w1.y, w1.z, w1.w};
             const uint32_t G0[4] = {g0.x, g0.y, g0.z, g0.w}, G1[4] = {g1.x, g1.y, g1.z, g1.w};
 #pragma unroll
@@ -716,7 +715,22 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_pairs4(const uint32_t* __re
                 L[e].mac(G1[e], W0[e]);
                 R[e].mac(G0[e], W1[e]);
             }
+        };
+        // two rows' eight loads are issued before either row's multiply-adds (the scheduler
+        // barrier keeps them there), so a wave has twice the bytes in flight
+        uint32_t i = i0;
+        for (; i + 1 < i1; i += 2) {
+            const uint4 a0 = ld4(wp + i * wstride), a1 = ld4(wp + (i + h) * wstride);
+            const uint4 b0 = ld4(gp + i * gstride), b1 = ld4(gp + (i + h) * gstride);
+            const uint4 c0 = ld4(wp + (i + 1) * wstride), c1 = ld4(wp + (i + 1 + h) * wstride);
+            const uint4 d0 = ld4(gp + (i + 1) * gstride), d1 = ld4(gp + (i + 1 + h) * gstride);
+            __builtin_amdgcn_sched_barrier(0);
+            pair_mac(a0, a1, b0, b1);
+            pair_mac(c0, c1, d0, d1);
         }
+        if (i < i1)
+            pair_mac(ld4(wp + i * wstride), ld4(wp + (i + h) * wstride), ld4(gp + i * gstride),
+                     ld4(gp + (i + h) * gstride));
     }
     const PrimeConsts pc = c_pc[p];
     const size_t slot = (size_t)K * NPRIMES * m;
@@ -814,8 +828,7 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_split4(const uint32_t* __re
         s10[e].zero();
     }
     const uint32_t i0 = seg * seg_rows, i1 = min(rows, i0 + seg_rows);
-    for (uint32_t i = i0; i < i1; ++i) {
-        const uint4 w = ld4(wp + i * wstride), g = ld4(gp + i * gstride);
+    auto row_mac = [&](const uint4& w, const uint4& g) {
         s00[0].mac(w.x, g.x);
         s11[0].mac(w.y, g.y);
         s01[0].mac(w.x, g.y);
@@ -824,7 +837,22 @@ __global__ void __launch_bounds__(MAC_THREADS) k_mac_split4(const uint32_t* __re
         s11[1].mac(w.w, g.w);
         s01[1].mac(w.z, g.w);
         s10[1].mac(w.w, g.z);
+    };
+    // four rows' loads are issued before their multiply-adds (the scheduler barrier keeps them
+    // there), so a wave has four rows in flight
+    uint32_t i = i0;
+    for (; i + 3 < i1; i += 4) {
+        uint4 w[4], g[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            w[u] = ld4(wp + (i + u) * wstride);
+            g[u] = ld4(gp + (i + u) * gstride);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) row_mac(w[u], g[u]);
     }
+    for (; i < i1; ++i) row_mac(ld4(wp + i * wstride), ld4(gp + i * gstride));
     const PrimeConsts pc = c_pc[p];
     const size_t slot = (size_t)K * NPRIMES * h;
     uint32_t* o = part + (size_t)seg * 4 * slot + (size_t)kp * h + c / 2;
