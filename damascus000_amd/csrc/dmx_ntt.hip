@@ -2305,8 +2305,18 @@ __global__ void __launch_bounds__(256) k_gen_contract_early(const zq* __restrict
         for (uint32_t cc = t; cc < tl; cc += blockDim.x) {
             const uint32_t c = c0 + cc;
             zq s = zq_zero();
-#pragma unroll 4
-            for (uint32_t h = 0; h < n_hi; ++h) s = zq_add(s, zq_mul(w[h], load_zq(g0 + h * row_stride + c)));
+            // four row blocks' loads are issued before their multiply-adds (the scheduler barrier
+            // keeps them there), so a thread has four coefficients in flight
+            uint32_t h = 0;
+            for (; h + 3 < n_hi; h += 4) {
+                zq gv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) gv[u] = load_zq(g0 + (h + u) * row_stride + c);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) s = zq_add(s, zq_mul(w[h + u], gv[u]));
+            }
+            for (; h < n_hi; ++h) s = zq_add(s, zq_mul(w[h], load_zq(g0 + h * row_stride + c)));
             sp[cc] = zq_mul(s, coeff_weight(c, n, chal, pmask));
         }
         __syncthreads();
