@@ -59,3 +59,25 @@ def test_primitive_root_is_the_smallest_generator():
 
     for p in O.PRIMES:
         assert bench.primitive_root(p) == O.lib().orc_multiplicative_generator(p)
+
+
+def test_served_trace_is_a_seeded_open_loop_stream():
+    """The served workload: a step is a 100-request segment of one Poisson stream. The trace is a
+    function of (requests, rate, seed) alone, arrivals start at 0 and never decrease, sizes stay
+    in 64 KiB..16 MiB (d = 7..11), and the rate only scales the gaps (the same files at any
+    rate, which the rate sweep relies on)."""
+    n = 3 * bench.SEGMENT
+    a = bench.served_trace(n, 2000.0)
+    b = bench.served_trace(n, 2000.0)
+    c = bench.served_trace(n, 500.0)
+    assert len(a) == n and a == b
+    arr = [t for t, _ in a]
+    assert arr[0] == 0.0 and all(x <= y for x, y in zip(arr, arr[1:]))
+    assert all(65536 <= len(f) < 16 << 20 for _, f in a)
+    assert [f for _, f in c] == [f for _, f in a]
+    assert all(abs(tc - 4 * ta) < 1e-9 for (tc, _), (ta, _) in zip(c, a))
+    # mean gap near 1/rate over 300 requests (exponential gaps: within 25 %)
+    assert abs(arr[-1] / (n - 1) - 1 / 2000.0) < 0.25 / 2000.0
+    # the warm-up trace (another seed) is another set of files of the same mix
+    w = bench.served_trace(bench.SEGMENT, 2000.0, seed=6)
+    assert [f for _, f in w] != [f for _, f in a[: bench.SEGMENT]]
