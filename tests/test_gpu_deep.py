@@ -5,6 +5,8 @@ restatement of the reference) bit for bit:
 * file_id of multi-MiB files, whose BLAKE3 tree has a group level (src/utils/io.rs:33);
 * a batch of d = 6..10 files with their own generator seeds, and a served mix with d = 9..11
   requests: every job equal to its solo run and to the oracle;
+* the files on either side of the depth boundaries 7/8, 8/9 and 9/10 in one batch, each equal to
+  the oracle;
 * the committed C1 golden fixture (tests/golden/c1_*);
 * the resampling branch of sample_nonzero_challenge (src/protocol/transcript.rs:66-75), forced
   by a test knob the oracle honours too, at the chunk level and at the group level (d = 12);
@@ -141,6 +143,25 @@ def test_batch_with_deep_files_and_own_seeds():
         assert r == dm.prove_verify_device(b.ptr, len(f), p), len(f)
     assert dev[1] == oracle(files[1], seeds[1])
     assert dev[4] == oracle(files[4], seeds[4])
+
+
+def test_depth_boundary_files_in_one_batch_match_oracle():
+    """the largest file of depths 7, 8 and 9 (13 * 4^d bytes: a full witness, no padding) and one
+    byte more (the next depth, a last coefficient of one byte; io.rs:72-76, 100-117), each with its own
+    generator seed, proved in one mixed-depth batch: every proof equals the oracle's"""
+    sizes = [n for d in (7, 8, 9) for n in (13 * 4 ** d, 13 * 4 ** d + 1)]
+    files = [rand(n, 300 + i) for i, n in enumerate(sizes)]
+    seeds = [_file_params(10 + i) for i in range(len(files))]
+    ctx = dm.Context.default()
+    bufs = [ctx.upload(f) for f in files]
+    say("depth-boundary batch", sizes)
+    got = dm.prove_verify_many([(b.ptr, len(f)) for b, f in zip(bufs, files)], [params_of(x) for x in seeds])
+    for f, x, r in zip(files, seeds, got):
+        st = W.decode_statement(r[0])
+        assert st["original_len_bytes"] == len(f)
+        assert r == oracle(f, x), len(f)
+    depths = [W.decode_statement(r[0])["d"] for r in got]
+    assert depths == [7, 8, 8, 9, 9, 10]
 
 
 def test_server_mix_with_deep_requests():
