@@ -157,46 +157,13 @@ __device__ void chunk_cv_quad_lds(const uint64_t* win, size_t base, size_t len, 
     }
 }
 
-// Pairwise bottom-up merge of the n CVs in `a` (LDS), one quad per parent node, ping-ponging
-// between `a` and `b`; returns the buffer holding the single result. With `final_root` the last
-// compression carries ROOT (n >= 2). Loop bounds are uniform, so every thread reaches every
-// barrier; the caller has synchronised after filling `a`.
-__device__ uint32_t* merge_lds(uint32_t* a, uint32_t* b, uint32_t n, const uint32_t key[8], uint32_t flags,
-                               bool final_root) {
-    const uint32_t q = threadIdx.x & 3, quad = threadIdx.x >> 2, nquads = blockDim.x >> 2;
-    const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
-    while (n > 1) {
-        const uint32_t pairs = n >> 1;
-        const uint32_t fl = flags | b3::PARENT | ((n == 2 && final_root) ? b3::ROOT : 0);
-        for (uint32_t base = 0; base < pairs; base += nquads) {
-            const uint32_t pr = base + quad;
-            if (pr < pairs) {
-                uint32_t m[16];
-#pragma unroll
-                for (int i = 0; i < 16; ++i) m[i] = a[2 * pr * 8 + i];
-                uint32_t ca = sel(key[0], key[1], key[2], key[3]), cb = sel(key[4], key[5], key[6], key[7]);
-                compress_quad(ca, cb, m, 0, b3::BLOCK_LEN, fl, sel);
-                b[pr * 8 + q] = ca;
-                b[pr * 8 + 4 + q] = cb;
-            }
-        }
-        if ((n & 1) && threadIdx.x < 8) b[pairs * 8 + threadIdx.x] = a[(n - 1) * 8 + threadIdx.x];
-        __syncthreads();
-        uint32_t* t = a;
-        a = b;
-        b = t;
-        n = pairs + (n & 1);
-    }
-    return a;
-}
-
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// merge_lds by wave 0 alone, for n <= 16 chaining values (at most 8 parents per level, one quad
+// merge_lds by wave 0 alone, for n <= 32 chaining values (at most 16 parents per level, one quad
 // each): the levels synchronise within the wave instead of across the workgroup. Every lane of
 // wave 0 calls it.
 __device__ uint32_t* merge_lds_wave(uint32_t* a, uint32_t* b, uint32_t n, const uint32_t key[8], uint32_t flags,
@@ -223,6 +190,48 @@ __device__ uint32_t* merge_lds_wave(uint32_t* a, uint32_t* b, uint32_t n, const 
         n = pairs + (n & 1);
     }
     return a;
+}
+
+// Pairwise bottom-up merge of the n CVs in `a` (LDS), one quad per parent node, ping-ponging
+// between `a` and `b`; returns the buffer holding the single result. With `final_root` the last
+// compression carries ROOT (n >= 2). Levels with more than 16 parents use the whole workgroup
+// and its barriers; the last levels (at most 32 values) run in wave 0 alone with wave-level
+// synchronisation (merge_lds_wave), and a final barrier publishes the result. Loop bounds are
+// uniform, so every thread reaches every barrier; the caller has synchronised after filling `a`.
+__device__ uint32_t* merge_lds(uint32_t* a, uint32_t* b, uint32_t n, const uint32_t key[8], uint32_t flags,
+                               bool final_root) {
+    const uint32_t q = threadIdx.x & 3, quad = threadIdx.x >> 2, nquads = blockDim.x >> 2;
+    const QuadSel sel{(q & 1) != 0, (q & 2) != 0};
+    while (n > 32) {
+        const uint32_t pairs = n >> 1;
+        const uint32_t fl = flags | b3::PARENT;  // n > 2: never the root here
+        for (uint32_t base = 0; base < pairs; base += nquads) {
+            const uint32_t pr = base + quad;
+            if (pr < pairs) {
+                uint32_t m[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) m[i] = a[2 * pr * 8 + i];
+                uint32_t ca = sel(key[0], key[1], key[2], key[3]), cb = sel(key[4], key[5], key[6], key[7]);
+                compress_quad(ca, cb, m, 0, b3::BLOCK_LEN, fl, sel);
+                b[pr * 8 + q] = ca;
+                b[pr * 8 + 4 + q] = cb;
+            }
+        }
+        if ((n & 1) && threadIdx.x < 8) b[pairs * 8 + threadIdx.x] = a[(n - 1) * 8 + threadIdx.x];
+        __syncthreads();
+        uint32_t* t = a;
+        a = b;
+        b = t;
+        n = pairs + (n & 1);
+    }
+    if (n <= 1) return a;
+    // the buffer the wave's levels end in: one swap per level
+    uint32_t levels = 0;
+    for (uint32_t v = n; v > 1; v = (v >> 1) + (v & 1)) ++levels;
+    uint32_t* const res = (levels & 1) ? b : a;
+    if (threadIdx.x < 64) (void)merge_lds_wave(a, b, n, key, flags, final_root);
+    __syncthreads();
+    return res;
 }
 
 // A whole challenge (transcript.rs:42-75) in one workgroup, for messages of at most

@@ -62,10 +62,10 @@ __device__ void chunk_cv_quad(const uint8_t* msg, size_t len, size_t ci, const u
 
 
 
-// Chunk CVs of a transcript message: 256 threads write the window, its first 16 quads hash the
+// Chunk CVs of a transcript message: 512 threads write the window, its first 16 quads hash the
 // chunks. The chunks holding the counter are also written to `msg`, where a root that meets a
 // zero challenge patches the counter and hashes them again.
-__global__ void __launch_bounds__(256) k_b3_tx_chunks(TranscriptMsg t, size_t len, KeyW key, uint32_t flags,
+__global__ void __launch_bounds__(512) k_b3_tx_chunks(TranscriptMsg t, size_t len, KeyW key, uint32_t flags,
                                                      uint32_t* __restrict__ cvs, uint8_t* __restrict__ msg, size_t js) {
     critical_path_priority();
     __shared__ __align__(16) uint64_t win[TX_WINDOW / 8];
@@ -310,7 +310,7 @@ static void launch_b3_root(const uint32_t* cvs, size_t n, const uint32_t* chunk_
 // A whole challenge in one workgroup for messages of at most 16 chunks (wg_tx_challenge).
 // (Larger messages need more than one CU to hash their chunks quickly and take the chunk and
 // root kernels.)
-__global__ void __launch_bounds__(256) k_b3_tx_challenge(TranscriptMsg t, size_t len, KeyW key, uint32_t flags,
+__global__ void __launch_bounds__(512) k_b3_tx_challenge(TranscriptMsg t, size_t len, KeyW key, uint32_t flags,
                                                         zq* chal_out, uint32_t* err, size_t js, int want_inv) {
     critical_path_priority();
     __shared__ WgChal sm;
@@ -323,7 +323,7 @@ static void launch_b3_tx_chunks(const TranscriptMsg& t, size_t len, const uint32
                                 uint32_t* cvs, uint8_t* msg, hipStream_t s, const Jobs& jb) {
     KeyW k;
     for (int i = 0; i < 8; ++i) k.w[i] = key[i];
-    hipLaunchKernelGGL(k_b3_tx_chunks, dim3((unsigned)((len + TX_WINDOW - 1) / TX_WINDOW), jb.n), dim3(256), 0, s, t,
+    hipLaunchKernelGGL(k_b3_tx_chunks, dim3((unsigned)((len + TX_WINDOW - 1) / TX_WINDOW), jb.n), dim3(512), 0, s, t,
                        len, k, flags, cvs, msg, jb.stride);
 }
 
@@ -339,7 +339,7 @@ void transcript_challenge(const TranscriptMsg& t, const uint32_t key[8], uint32_
     if (nchunks_of(len) <= CHALLENGE_MAX_CHUNKS) {
         KeyW k;
         for (int i = 0; i < 8; ++i) k.w[i] = key[i];
-        hipLaunchKernelGGL(k_b3_tx_challenge, dim3(1, jb.n), dim3(256), 0, s, t, len, k, flags, chal_out, err_flags,
+        hipLaunchKernelGGL(k_b3_tx_challenge, dim3(1, jb.n), dim3(512), 0, s, t, len, k, flags, chal_out, err_flags,
                            jb.stride, (int)inverse);
         return;
     }
