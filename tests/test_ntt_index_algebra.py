@@ -1,7 +1,8 @@
 """The register-blocked forward transform's index algebra, checked on the CPU for every register
 blocking the library launches (read from damascus000_amd/csrc/dmx_ntt.hip): the two passes equal
-the per-layer negacyclic transform on random input, the LDS swizzle is a bijection per prime slice,
-and no 32-lane LDS access of either pass repeats a bank."""
+the per-layer negacyclic transform on random input, the padded LDS layout keeps each prime slice's
+elements on distinct words of its own slice, and no 32-lane LDS access of either pass repeats a
+bank."""
 import os
 import random
 import re
@@ -85,8 +86,8 @@ def register_blocked(x, E, TP):
 
 
 def _slot(E, TP, q, x):
-    log_e, swm = E.bit_length() - 1, min(E, 32) - 1
-    return q * E * TP + (x ^ ((x >> log_e) & swm) ^ ((q * TP) & 31))
+    """RegNtt::slot: one pad word after every E elements, prime slices TP * (E + 1) words apart"""
+    return q * TP * (E + 1) + x + (x >> (E.bit_length() - 1))
 
 
 @pytest.mark.parametrize("E,TP", _shapes())
@@ -97,11 +98,18 @@ def test_register_blocking_equals_per_layer_transform(E, TP):
     assert register_blocked(x, E, TP) == per_layer(x)
 
 
+def test_register_slot_formula_is_the_kernels():
+    src = open(os.path.join(ROOT, "damascus000_amd", "csrc", "dmx_ntt.hip")).read()
+    assert "static constexpr int SL = TP * (E + 1);" in src
+    assert "return q * SL + x + (x >> LOG_E);" in src
+
+
 @pytest.mark.parametrize("E,TP", _shapes())
-def test_lds_swizzle_is_a_bijection_without_bank_repeats(E, TP):
-    M = E * TP
-    for q in range(8):
-        assert sorted(_slot(E, TP, q, x) for x in range(M)) == list(range(q * M, (q + 1) * M))
+def test_lds_layout_is_injective_without_bank_repeats(E, TP):
+    M, SL = E * TP, TP * (E + 1)
+    for q in range(8):  # each prime slice's M elements land on distinct words of its own SL
+        slots = [_slot(E, TP, q, x) for x in range(M)]
+        assert len(set(slots)) == M and min(slots) >= q * SL and max(slots) < (q + 1) * SL
     for base in range(0, 8 * TP, 32):  # the 32 lanes of one LDS pass: prime q = t // TP, lane l = t % TP
         lanes = [(t // TP, t % TP) for t in range(base, min(base + 32, 8 * TP))]
         for i in range(E):
