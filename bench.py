@@ -3,11 +3,12 @@
 
 Headline (`value`): a served trace of open-loop Poisson arrivals of commit+fold requests, file
 sizes log-uniform 64 KiB..16 MiB (d = 7..11, up to 4,194,304 coefficients), seeded uniform
-bytes, K = 8, default SystemParams, files resident in HBM before the clock starts. A step is one
-request: the whole job (file_id, witness expansion, the initial commitment, every fold round with
-both invariant commitments and every Fiat-Shamir challenge, the verifier's replay and the final
-opening check, then the proof bytes copied to the host). `--steps` requests are timed in one
-served pass after `--warmup` requests of the same trace in an untimed pass (workspaces, captured
+bytes, K = 8, default SystemParams, files resident in HBM before the clock starts. Each request
+is a whole job (file_id, witness expansion, the initial commitment, every fold round with both
+invariant commitments and every Fiat-Shamir challenge, the verifier's replay and the final
+opening check, then the proof bytes copied to the host). A step is a segment of 100 requests:
+`--steps` segments form one continuous Poisson stream, timed in one served pass, after `--warmup`
+segments of another trace of the same mix (seed 6) in an untimed pass (workspaces, captured
 graphs, the resident generator matrix of the trace's seed). The served path is the
 continuous-batching server (dmx_server_*); the plain path runs the same trace one request at a
 time through dmx_prove_verify_device, which expands the generators on every call. `value` is MiB
@@ -15,9 +16,9 @@ of request files per second from the first arrival to the last verified result; 
 arrival-to-verified-result latencies. Every served proof is compared with the plain path's.
 
 Also reported on the same JSON line:
-  served        the headline pass, the plain path on the same trace, the same trace from host
-                memory, a sweep of arrival rates, and a trace whose requests each carry their own
-                generator seed (so no resident generator matrix applies)
+  served        the headline pass and the plain path on the same trace; on its first 400 requests:
+                the same files from host memory, a sweep of arrival rates, and requests that each
+                carry their own generator seed (so no resident generator matrix applies)
   single_file   one 8 MiB file (645,278 coefficients, d = 10), the plain path, graph replays, and a
                 16 MiB file (d = 11)
   batched_files 256 files log-uniform 4 KiB..4 MiB (d = 6..10), each with its own generator seed,
