@@ -44,6 +44,17 @@ class DamascusError(RuntimeError):
 _lib = None
 _c = ctypes
 _VP = ctypes.c_void_p
+
+# The register-blocked forward-transform kernels per degree 2^log_m, variants 0 and 1, as
+# launch_ntt_fwd_rows2 (csrc/dmx_ntt.hip) launches them; the last variant is k_ntt_fwd_rows.
+FWD_KERNELS = {
+    6: ("k_ntt_fwd_reg<8, 8>", "k_ntt_fwd_reg<16, 4>"),
+    7: ("k_ntt_fwd_reg<16, 8>", "k_ntt_fwd_reg<32, 4>"),
+    8: ("k_ntt_fwd_reg<16, 16>", "k_ntt_fwd_reg<32, 8>"),
+    9: ("k_ntt_fwd_reg<32, 16>", "k_ntt_fwd_reg<64, 8>"),
+    10: ("k_ntt_fwd_reg<32, 32>", "k_ntt_fwd_reg3<10>"),
+    11: ("k_ntt_fwd_reg<64, 32>", "k_ntt_fwd_reg3<11>"),
+}
 _SZ = ctypes.c_size_t
 _SZP = ctypes.POINTER(ctypes.c_size_t)
 
@@ -267,6 +278,11 @@ class Context:
 
     def set_transform_variant(self, log_m, variant):
         _check(lib().dmx_ctx_set_transform_variant(self.h, log_m, variant))
+
+    def transform_kernel(self, log_m):
+        """name of the kernel that runs the degree-2^log_m forward transform (the chosen variant)"""
+        v, n, _ = self.transform_variant(log_m)
+        return FWD_KERNELS[log_m][v] if v < n - 1 and log_m in FWD_KERNELS else "k_ntt_fwd_rows"
 
     MAC_SHAPES = 9
 

@@ -314,21 +314,39 @@ static void launch_reg(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, 
 }
 
 // ---------------------------------------------------------------------------------------
-// Three-pass register transform for degree 2^11: 64 threads per prime slice holding 32 elements
-// each, 512 threads per row. Halving the elements per thread against the two-pass <64, 32>
-// blocking halves the LDS words per resident thread, so a CU holds twice the waves. Pass 1:
-// thread l holds x = l + 64 i (layers len 1024..64); pass 2: x = b + 2 e + 64 h for l = 2 h + b
-// (layers 32..2); pass 3: x = 32 l + e (layer 1). One padded LDS layout, word x + (x >> 5), serves
-// the producer and all three passes without bank conflicts, and each pass writes back the words
-// it read, so three barriers order the row.
+// Three-pass register transform for degrees 2^10 and 2^11: 64 threads per prime slice holding
+// E = m / 64 elements each (16 or 32), 512 threads per row, one row per workgroup. Against the
+// two-pass blockings this halves or quarters the LDS words per resident thread, so a CU holds
+// more waves. With G = 64 / E:
+//   pass 1: thread l holds x = l + 64 i (layers len = m/2 .. 64);
+//   pass 2: x = b + G e + 64 h (layers len = 32 .. G), with l = 2 h + b for G = 2 and
+//           l = h + 16 b for G = 4;
+//   pass 3: x = E l + e (layers len = G/2 .. 1).
+// One padded LDS layout, word x + (x >> 5), serves the producer and all three passes without
+// bank conflicts, and each pass writes back the words it read, so three barriers order the row.
 // ---------------------------------------------------------------------------------------
-constexpr int R3_M = 2048, R3_LOG_M = 11, R3_TP = 64, R3_THREADS = NPRIMES * R3_TP;
-constexpr int R3_SL = 2112;  // LDS words per prime slice: slots x + (x >> 5) < 2111
-constexpr size_t R3_LDS_BYTES = (size_t)NPRIMES * R3_SL * 4;
+template <int LOG_M>
+struct Reg3 {
+    static constexpr int M = 1 << LOG_M;
+    static constexpr int TP = 64;
+    static constexpr int E = M / TP;
+    static constexpr int G = TP / E;
+    static constexpr int THREADS = NPRIMES * TP;
+    static constexpr int SL = M + M / 32;  // LDS words per prime slice: slots x + (x >> 5) < SL
+    static constexpr size_t LDS_BYTES = (size_t)NPRIMES * SL * 4;
+    static constexpr int MIN_WAVES = LOG_M == 11 ? 4 : 8;  // waves per SIMD asked of the register allocator
+    static_assert(G == 2 || G == 4, "degrees 2^10 and 2^11 only");
+    // pass-2 lane split, chosen so the 32 lanes of an LDS access hit 32 banks
+    __device__ static __forceinline__ int p2_h(int l) { return G == 2 ? l >> 1 : l & 15; }
+    __device__ static __forceinline__ int p2_b(int l) { return G == 2 ? l & 1 : l >> 4; }
+};
 __device__ __forceinline__ int r3_slot(int x) { return x + (x >> 5); }
 
-__global__ void __launch_bounds__(R3_THREADS, 4) k_ntt_fwd_reg3(FwdSeg s1, FwdSeg s2, uint32_t blocks1, NttTables tw,
-                                                                 size_t js, InvTask it) {
+template <int LOG_M>
+__global__ void __launch_bounds__(Reg3<LOG_M>::THREADS, Reg3<LOG_M>::MIN_WAVES)
+    k_ntt_fwd_reg3(FwdSeg s1, FwdSeg s2, uint32_t blocks1, NttTables tw, size_t js, InvTask it) {
+    using C = Reg3<LOG_M>;
+    constexpr int M = C::M, E = C::E, G = C::G;
     extern __shared__ uint32_t sh[];
     uint32_t bx = blockIdx.x;
     if (it.chal) {  // uniform over the workgroup; the task workgroup meets no barrier
@@ -352,74 +370,82 @@ __global__ void __launch_bounds__(R3_THREADS, 4) k_ntt_fwd_reg3(FwdSeg s1, FwdSe
     }
     const size_t row = second ? bx - blocks1 : bx;  // one row per workgroup, every row valid
     const int tid = threadIdx.x;
-    for (int c = tid; c < R3_M; c += R3_THREADS) {
-        const zq x = row_value(src, key, row, c, R3_LOG_M);
-        if (src.dst) store_zq(src.dst + row * R3_M + c, x);
-        if (src.dst2) store_zq(src.dst2 + row * R3_M + c, x);
+    for (int c = tid; c < M; c += C::THREADS) {
+        const zq x = row_value(src, key, row, c, LOG_M);
+        if (src.dst) store_zq(src.dst + row * M + c, x);
+        if (src.dst2) store_zq(src.dst2 + row * M + c, x);
 #pragma unroll
-        for (int q = 0; q < NPRIMES; ++q) sh[q * R3_SL + r3_slot(c)] = zq_mod_p_centred(x, c_pc[q]);
+        for (int q = 0; q < NPRIMES; ++q) sh[q * C::SL + r3_slot(c)] = zq_mod_p_centred(x, c_pc[q]);
     }
     __syncthreads();
     const int q = tid >> 6, l = tid & 63;
-    uint32_t* S = sh + q * R3_SL;
+    uint32_t* S = sh + q * C::SL;
     const uint32_t p = c_pc[q].p;
     const uint32_t* zt = tw.zeta_m + q * TW_SIZE;
     const uint32_t* ztp = tw.zeta_mp + q * TW_SIZE;
-    uint32_t a[32];
-    // pass 1: registers i = 0..31 hold x = l + 64 i; layer len = 64 s pairs i, i + s and uses
-    // zeta[16 / s + i / (2 s)]
+    uint32_t a[E];
+    // pass 1: registers i hold x = l + 64 i; layer len = 64 s pairs i, i + s and uses
+    // zeta[E / (2 s) + i / (2 s)]
 #pragma unroll
-    for (int i = 0; i < 32; ++i) a[i] = S[r3_slot(l + 64 * i)];
+    for (int i = 0; i < E; ++i) a[i] = S[r3_slot(l + 64 * i)];
 #pragma unroll
-    for (int s = 16; s >= 1; s >>= 1) {
+    for (int s = E / 2; s >= 1; s >>= 1) {
 #pragma unroll
-        for (int b = 0; b < 16 / s; ++b) {
-            const uint32_t w = zt[16 / s + b], wp = ztp[16 / s + b];
+        for (int b = 0; b < E / (2 * s); ++b) {
+            const uint32_t w = zt[E / (2 * s) + b], wp = ztp[E / (2 * s) + b];
 #pragma unroll
             for (int k = 0; k < s; ++k) ct_butterfly(a[2 * s * b + k], a[2 * s * b + k + s], w, wp, p);
         }
     }
 #pragma unroll
-    for (int i = 0; i < 32; ++i) S[r3_slot(l + 64 * i)] = a[i];
+    for (int i = 0; i < E; ++i) S[r3_slot(l + 64 * i)] = a[i];
     __syncthreads();
-    // pass 2: registers e hold x = b + 2 e + 64 h; layer len = 2 s pairs e, e + s and uses
-    // zeta[512 / s + 16 h / s + e / (2 s)]
-    const int h = l >> 1, b0 = l & 1;
+    // pass 2: registers e hold x = b + G e + 64 h; layer len = G s pairs e, e + s and uses
+    // zeta[m / (2 G s) + 32 h / (G s) + e / (2 s)]
+    const int h = C::p2_h(l), b0 = C::p2_b(l);
 #pragma unroll
-    for (int e = 0; e < 32; ++e) a[e] = S[r3_slot(b0 + 2 * e + 64 * h)];
+    for (int e = 0; e < E; ++e) a[e] = S[r3_slot(b0 + G * e + 64 * h)];
 #pragma unroll
-    for (int s = 16; s >= 1; s >>= 1) {
-        const int base = 512 / s + (16 / s) * h;
+    for (int s = E / 2; s >= 1; s >>= 1) {
+        const int base = M / (2 * G * s) + (32 / (G * s)) * h;
 #pragma unroll
-        for (int b = 0; b < 16 / s; ++b) {
+        for (int b = 0; b < E / (2 * s); ++b) {
             const uint32_t w = zt[base + b], wp = ztp[base + b];
 #pragma unroll
             for (int k = 0; k < s; ++k) ct_butterfly(a[2 * s * b + k], a[2 * s * b + k + s], w, wp, p);
         }
     }
 #pragma unroll
-    for (int e = 0; e < 32; ++e) S[r3_slot(b0 + 2 * e + 64 * h)] = a[e];
+    for (int e = 0; e < E; ++e) S[r3_slot(b0 + G * e + 64 * h)] = a[e];
     __syncthreads();
-    // pass 3: registers e hold x = 32 l + e; layer len = 1 pairs e, e + 1 (e even) and uses
-    // zeta[1024 + 16 l + e / 2]
+    // pass 3: registers e hold x = E l + e; layer len = t pairs e, e + t and uses
+    // zeta[m / (2 t) + E l / (2 t) + e / (2 t)]
 #pragma unroll
-    for (int e = 0; e < 32; ++e) a[e] = S[r3_slot(32 * l + e)];
+    for (int e = 0; e < E; ++e) a[e] = S[r3_slot(E * l + e)];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const uint32_t w = zt[1024 + 16 * l + j], wp = ztp[1024 + 16 * l + j];
-        ct_butterfly(a[2 * j], a[2 * j + 1], w, wp, p);
+    for (int t = G / 2; t >= 1; t >>= 1) {
+        const int base = M / (2 * t) + (E / (2 * t)) * l;
+#pragma unroll
+        for (int b = 0; b < E / (2 * t); ++b) {
+            const uint32_t w = zt[base + b], wp = ztp[base + b];
+#pragma unroll
+            for (int k = 0; k < t; ++k) ct_butterfly(a[2 * t * b + k], a[2 * t * b + k + t], w, wp, p);
+        }
     }
-    uint4* o = reinterpret_cast<uint4*>(out + (row * NPRIMES + q) * R3_M + 32 * l);
+    uint4* o = reinterpret_cast<uint4*>(out + (row * NPRIMES + q) * M + E * l);
 #pragma unroll
-    for (int e = 0; e < 32; e += 4) o[e / 4] = make_uint4(a[e], a[e + 1], a[e + 2], a[e + 3]);
+    for (int e = 0; e < E; e += 4) o[e / 4] = make_uint4(a[e], a[e + 1], a[e + 2], a[e + 3]);
 }
 
+template <int LOG_M>
 static void launch_reg3(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, hipStream_t s, const Jobs& jb,
                         const InvTask& it) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ntt_fwd_reg3),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)R3_LDS_BYTES);
-    hipLaunchKernelGGL(k_ntt_fwd_reg3, dim3((unsigned)(s1.rows + s2.rows + (it.chal ? 1 : 0)), jb.n),
-                       dim3(R3_THREADS), R3_LDS_BYTES, s, s1, s2, (uint32_t)s1.rows, tw, jb.stride, it);
+    using C = Reg3<LOG_M>;
+    if (C::LDS_BYTES > 65536)  // past the default dynamic LDS limit
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ntt_fwd_reg3<LOG_M>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS_BYTES);
+    hipLaunchKernelGGL((k_ntt_fwd_reg3<LOG_M>), dim3((unsigned)(s1.rows + s2.rows + (it.chal ? 1 : 0)), jb.n),
+                       dim3(C::THREADS), C::LDS_BYTES, s, s1, s2, (uint32_t)s1.rows, tw, jb.stride, it);
 }
 
 // DAMASCUS_NTT_KERNEL=layer keeps every degree on the per-layer kernel (for comparisons)
@@ -432,8 +458,8 @@ static bool reg_ntt_enabled() {
 }
 
 // Variants per degree: variant 0 and 1 are register blockings (E elements x TP threads per prime
-// slice; TP <= E, TP <= 32, E * TP = m; at 2^11 variant 1 is the three-pass kernel), the last is
-// the per-layer kernel. Every variant computes the same residues; the context picks the fastest
+// slice; TP <= E, TP <= 32, E * TP = m; at 2^10 and 2^11 variant 1 is the three-pass kernel), the
+// last is the per-layer kernel. Every variant computes the same residues; the context picks the fastest
 // per degree by timing.
 int fwd_variant_count(int log_m) {
     if (!reg_ntt_enabled()) return 1;
@@ -458,9 +484,9 @@ void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, cons
             case 18: return launch_reg<32, 16>(s1, s2, tw, s, jb, it);
             case 19: return launch_reg<64, 8>(s1, s2, tw, s, jb, it);
             case 20: return launch_reg<32, 32>(s1, s2, tw, s, jb, it);
-            case 21: return launch_reg<64, 16>(s1, s2, tw, s, jb, it);
+            case 21: return launch_reg3<10>(s1, s2, tw, s, jb, it);
             case 22: return launch_reg<64, 32>(s1, s2, tw, s, jb, it);
-            case 23: return launch_reg3(s1, s2, tw, s, jb, it);
+            case 23: return launch_reg3<11>(s1, s2, tw, s, jb, it);
             default: break;
         }
     }

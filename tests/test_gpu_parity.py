@@ -464,25 +464,28 @@ def test_fused_job_rejects_a_wrong_final_opening():
 
 def test_transform_variants_give_identical_proofs():
     """Every forward-transform variant (register blockings of the prime slices, the three-pass
-    kernel at 2^11 and the per-layer kernel) gives the same proof: at d = 7 each against the
-    oracle, at d = 11 (degrees 2^11 down to 2^6) across variants through the fused job, whose
-    captured graph is retaken after a change."""
+    kernel at 2^10 and 2^11 and the per-layer kernel) gives the same proof: at d = 7 each against
+    the oracle, at d = 10 and d = 11 (generator rows at 2^10 and 2^11, fold rows down to 2^6)
+    across variants through the fused job, whose captured graph is retaken after a change."""
     ctx = dm.Context.default()
     saved = {lg: ctx.transform_variant(lg)[0] for lg in range(6, 12)}
     say("tuned transform variants (variant, count, us):", {lg: ctx.transform_variant(lg) for lg in range(6, 12)})
     small = W.pattern_bytes(W.byte_len_for_depth(7), 5)
+    mid = np.random.default_rng(10).integers(0, 256, size=8 << 20, dtype=np.uint8).tobytes()  # d = 10
     big = np.random.default_rng(11).integers(0, 256, size=16 << 20, dtype=np.uint8).tobytes()  # d = 11
     rc, ost, orecs, oop = O.full_flow(small)
     assert rc == 0
-    buf = ctx.upload(big)
-    outs = []
+    mbuf, buf = ctx.upload(mid), ctx.upload(big)
+    outs, mids = [], []
     try:
         for v in range(3):
             for lg in range(6, 12):
                 ctx.set_transform_variant(lg, min(v, ctx.transform_variant(lg)[1] - 1))
             say("all degrees at variant", v)
             assert dm.prove(small) == (ost, orecs, oop)
+            mids.append(dm.prove_verify_device(mbuf.ptr, len(mid)))
             outs.append(dm.prove_verify_device(buf.ptr, len(big)))
+        assert mids[0] == mids[1] == mids[2]
         assert outs[0] == outs[1] == outs[2]
         with pytest.raises(dm.DamascusError):
             ctx.set_transform_variant(11, 3)  # degree 2^11 has three variants

@@ -1,8 +1,8 @@
-"""The register-blocked forward transform's index algebra, checked on the CPU for every register
-blocking the library launches (read from damascus000_amd/csrc/dmx_ntt.hip): the two passes equal
-the per-layer negacyclic transform on random input, the padded LDS layout keeps each prime slice's
-elements on distinct words of its own slice, and no 32-lane LDS access of either pass repeats a
-bank."""
+"""The register-blocked forward transforms' index algebra, checked on the CPU for every register
+blocking the library launches (read from damascus000_amd/csrc/dmx_ntt.hip): the two passes (and
+the three passes of the 2^10 / 2^11 kernel) equal the per-layer negacyclic transform on random
+input, the padded LDS layouts keep each prime slice's elements on distinct words of its own slice,
+and no 32-lane LDS access of any pass repeats a bank."""
 import os
 import random
 import re
@@ -116,6 +116,105 @@ def test_lds_layout_is_injective_without_bank_repeats(E, TP):
             for banks in ([_slot(E, TP, q, l + TP * i) % 32 for q, l in lanes],
                           [_slot(E, TP, q, E * l + i) % 32 for q, l in lanes]):
                 assert len(banks) == len(set(banks)), (E, TP, base, i)
+
+
+# ---------------- the three-pass register transform (k_ntt_fwd_reg3, degrees 2^10 and 2^11) ----------------
+def _reg3_degrees():
+    src = open(os.path.join(ROOT, "damascus000_amd", "csrc", "dmx_ntt.hip")).read()
+    degrees = sorted({int(d) for d in re.findall(r"launch_reg3<(\d+)>\(s1, s2", src)})
+    assert degrees, "no three-pass launches found"
+    return degrees
+
+
+def _r3_slot(x):
+    return x + (x >> 5)
+
+
+def _r3_p2_lane(G, l):
+    """Reg3::p2_h / p2_b: the pass-2 (h, b) of lane l"""
+    return (l >> 1, l & 1) if G == 2 else (l & 15, l >> 4)
+
+
+def _r3_passes(log_m):
+    """(x of register i for lane l, twiddle index of layer len for lane l and register i) per pass"""
+    M = 1 << log_m
+    E = M // 64
+    G = 64 // E
+
+    def p1(l, i):
+        return l + 64 * i
+
+    def p2(l, e):
+        h, b = _r3_p2_lane(G, l)
+        return b + G * e + 64 * h
+
+    def p3(l, e):
+        return E * l + e
+
+    return M, E, G, (p1, p2, p3)
+
+
+def three_pass(x, log_m):
+    """k_ntt_fwd_reg3 through a simulated LDS slice: pass 1 layers len = 64 s (s = E/2 .. 1) with
+    zeta[E/(2s) + i/(2s)], pass 2 layers len = G s with zeta[m/(2Gs) + 32h/(Gs) + e/(2s)], pass 3
+    layers len = t (t = G/2 .. 1) with zeta[m/(2t) + E l/(2t) + e/(2t)]"""
+    M, E, G, (p1, p2, p3) = _r3_passes(log_m)
+    lds = [None] * (M + M // 32)
+    for c in range(M):
+        lds[_r3_slot(c)] = x[c]
+
+    def run(xof, layers):
+        for l in range(64):
+            a = [lds[_r3_slot(xof(l, i))] for i in range(E)]
+            for s, zeta_of in layers(l):
+                for b in range(E // (2 * s)):
+                    for k in range(s):
+                        _butterfly(a, 2 * s * b + k, 2 * s * b + k + s, ZETA[zeta_of(b)])
+            for i in range(E):
+                lds[_r3_slot(xof(l, i))] = a[i]
+
+    def halving(n):
+        while n >= 1:
+            yield n
+            n //= 2
+
+    run(p1, lambda l: [(s, lambda b, s=s: E // (2 * s) + b) for s in halving(E // 2)])
+    run(p2, lambda l: [(s, lambda b, s=s: M // (2 * G * s) + (32 // (G * s)) * _r3_p2_lane(G, l)[0] + b)
+                       for s in halving(E // 2)])
+    run(p3, lambda l: [(t, lambda b, t=t: M // (2 * t) + (E // (2 * t)) * l + b) for t in halving(G // 2)])
+    return [lds[_r3_slot(c)] for c in range(M)]
+
+
+def test_three_pass_lane_split_is_the_kernels():
+    src = open(os.path.join(ROOT, "damascus000_amd", "csrc", "dmx_ntt.hip")).read()
+    assert "p2_h(int l) { return G == 2 ? l >> 1 : l & 15; }" in src
+    assert "p2_b(int l) { return G == 2 ? l & 1 : l >> 4; }" in src
+    assert "const int base = M / (2 * G * s) + (32 / (G * s)) * h;" in src
+    assert "const int base = M / (2 * t) + (E / (2 * t)) * l;" in src
+    assert "__device__ __forceinline__ int r3_slot(int x) { return x + (x >> 5); }" in src
+
+
+@pytest.mark.parametrize("log_m", _reg3_degrees())
+def test_three_pass_transform_equals_per_layer_transform(log_m):
+    rng = random.Random(log_m)
+    x = [rng.randrange(P) for _ in range(1 << log_m)]
+    assert three_pass(x, log_m) == per_layer(x)
+
+
+@pytest.mark.parametrize("log_m", _reg3_degrees())
+def test_three_pass_lds_accesses_have_no_bank_repeats(log_m):
+    M, E, G, passes = _r3_passes(log_m)
+    SL = M + M // 32
+    assert max(_r3_slot(x) for x in range(M)) < SL
+    for xof in passes:  # each lane's registers are a permutation of the slice's elements
+        assert sorted(xof(l, i) for l in range(64) for i in range(E)) == list(range(M))
+        for half in (0, 32):  # the 32 lanes of one LDS access of a wave (one prime slice)
+            for i in range(E):
+                banks = [_r3_slot(xof(l, i)) % 32 for l in range(half, half + 32)]
+                assert len(set(banks)) == 32, (log_m, half, i)
+    for half in (0, 32):  # the producer: lanes write consecutive coefficients c
+        banks = [_r3_slot(c) % 32 for c in range(half, half + 32)]
+        assert len(set(banks)) == 32
 
 
 # ---------------- the LDS-resident ragged product kernel (k_ring_mul_lds) ----------------
@@ -258,3 +357,16 @@ def test_ring_lds_layout_is_a_bijection_without_bank_repeats(log_n, E, TP):
             for banks in ([_ring_strided(E, TP, q, l, i) % 32 for q, l in lanes],
                           [_ring_contiguous(E, TP, q, l, i) % 32 for q, l in lanes]):
                 assert len(banks) == len(set(banks)), (E, TP, base, i)
+
+
+def test_kernel_names_per_variant_follow_the_launch_switch():
+    """Context.transform_kernel's table names the kernel each (degree, variant) case launches"""
+    import sys
+    sys.path.insert(0, ROOT)
+    from damascus000_amd import FWD_KERNELS
+    src = open(os.path.join(ROOT, "damascus000_amd", "csrc", "dmx_ntt.hip")).read()
+    launched = {}
+    for case, e, t, d in re.findall(r"case (\d+): return launch_reg(?:<(\d+), (\d+)>|3<(\d+)>)\(s1, s2", src):
+        lg, v = divmod(int(case), 2)
+        launched[(lg, v)] = f"k_ntt_fwd_reg<{e}, {t}>" if e else f"k_ntt_fwd_reg3<{d}>"
+    assert launched == {(lg, v): FWD_KERNELS[lg][v] for lg in FWD_KERNELS for v in (0, 1)}
