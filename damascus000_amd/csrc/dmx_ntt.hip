@@ -314,16 +314,18 @@ static void launch_reg(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, 
 }
 
 // ---------------------------------------------------------------------------------------
-// Three-pass register transform for degrees 2^10 and 2^11: 64 threads per prime slice holding
-// E = m / 64 elements each (16 or 32), 512 threads per row, one row per workgroup. Against the
-// two-pass blockings this halves or quarters the LDS words per resident thread, so a CU holds
-// more waves. With G = 64 / E:
+// Three-pass register transform for degrees 2^9, 2^10 and 2^11: 64 threads per prime slice
+// holding E = m / 64 elements each (8, 16 or 32), 512 threads per row, one row per workgroup.
+// Against the two-pass blockings this takes a fraction of the LDS words per resident thread, so a
+// CU holds more waves. With G = 64 / E:
 //   pass 1: thread l holds x = l + 64 i (layers len = m/2 .. 64);
-//   pass 2: x = b + G e + 64 h (layers len = 32 .. G), with l = 2 h + b for G = 2 and
-//           l = h + 16 b for G = 4;
+//   pass 2: x = b + G e + 64 h (layers len = 32 .. G), with l = 2 h + b for G = 2,
+//           l = h + 16 b for G = 4 and l = 8 h + b for G = 8;
 //   pass 3: x = E l + e (layers len = G/2 .. 1).
-// One padded LDS layout, word x + (x >> 5), serves the producer and all three passes without
-// bank conflicts, and each pass writes back the words it read, so three barriers order the row.
+// One LDS layout per degree serves the producer and all three passes without bank conflicts:
+// the padded word x + (x >> 5) at 2^10 and 2^11, and at 2^9, where no simple pad does, the
+// swizzle x ^ s(x >> 5) with s(r) = (r & 7) | ((r >> 1) & 3) << 3 inside each 32-word row.
+// Each pass writes back the words it read, so three barriers order the row.
 // ---------------------------------------------------------------------------------------
 template <int LOG_M>
 struct Reg3 {
@@ -332,15 +334,22 @@ struct Reg3 {
     static constexpr int E = M / TP;
     static constexpr int G = TP / E;
     static constexpr int THREADS = NPRIMES * TP;
-    static constexpr int SL = M + M / 32;  // LDS words per prime slice: slots x + (x >> 5) < SL
+    static constexpr bool SWIZZLE = LOG_M == 9;
+    static constexpr int SL = SWIZZLE ? M : M + M / 32;  // LDS words per prime slice
     static constexpr size_t LDS_BYTES = (size_t)NPRIMES * SL * 4;
     static constexpr int MIN_WAVES = LOG_M == 11 ? 4 : 8;  // waves per SIMD asked of the register allocator
-    static_assert(G == 2 || G == 4, "degrees 2^10 and 2^11 only");
+    static_assert(G == 2 || G == 4 || G == 8, "degrees 2^9, 2^10 and 2^11 only");
+    __device__ static __forceinline__ int slot(int x) {
+        if (SWIZZLE) {
+            const int r = x >> 5;
+            return x ^ ((r & 7) | (((r >> 1) & 3) << 3));
+        }
+        return x + (x >> 5);
+    }
     // pass-2 lane split, chosen so the 32 lanes of an LDS access hit 32 banks
-    __device__ static __forceinline__ int p2_h(int l) { return G == 2 ? l >> 1 : l & 15; }
-    __device__ static __forceinline__ int p2_b(int l) { return G == 2 ? l & 1 : l >> 4; }
+    __device__ static __forceinline__ int p2_h(int l) { return G == 2 ? l >> 1 : G == 4 ? l & 15 : l >> 3; }
+    __device__ static __forceinline__ int p2_b(int l) { return G == 2 ? l & 1 : G == 4 ? l >> 4 : l & 7; }
 };
-__device__ __forceinline__ int r3_slot(int x) { return x + (x >> 5); }
 
 template <int LOG_M>
 __global__ void __launch_bounds__(Reg3<LOG_M>::THREADS, Reg3<LOG_M>::MIN_WAVES)
@@ -375,7 +384,7 @@ __global__ void __launch_bounds__(Reg3<LOG_M>::THREADS, Reg3<LOG_M>::MIN_WAVES)
         if (src.dst) store_zq(src.dst + row * M + c, x);
         if (src.dst2) store_zq(src.dst2 + row * M + c, x);
 #pragma unroll
-        for (int q = 0; q < NPRIMES; ++q) sh[q * C::SL + r3_slot(c)] = zq_mod_p_centred(x, c_pc[q]);
+        for (int q = 0; q < NPRIMES; ++q) sh[q * C::SL + C::slot(c)] = zq_mod_p_centred(x, c_pc[q]);
     }
     __syncthreads();
     const int q = tid >> 6, l = tid & 63;
@@ -387,7 +396,7 @@ __global__ void __launch_bounds__(Reg3<LOG_M>::THREADS, Reg3<LOG_M>::MIN_WAVES)
     // pass 1: registers i hold x = l + 64 i; layer len = 64 s pairs i, i + s and uses
     // zeta[E / (2 s) + i / (2 s)]
 #pragma unroll
-    for (int i = 0; i < E; ++i) a[i] = S[r3_slot(l + 64 * i)];
+    for (int i = 0; i < E; ++i) a[i] = S[C::slot(l + 64 * i)];
 #pragma unroll
     for (int s = E / 2; s >= 1; s >>= 1) {
 #pragma unroll
@@ -398,13 +407,13 @@ __global__ void __launch_bounds__(Reg3<LOG_M>::THREADS, Reg3<LOG_M>::MIN_WAVES)
         }
     }
 #pragma unroll
-    for (int i = 0; i < E; ++i) S[r3_slot(l + 64 * i)] = a[i];
+    for (int i = 0; i < E; ++i) S[C::slot(l + 64 * i)] = a[i];
     __syncthreads();
     // pass 2: registers e hold x = b + G e + 64 h; layer len = G s pairs e, e + s and uses
     // zeta[m / (2 G s) + 32 h / (G s) + e / (2 s)]
     const int h = C::p2_h(l), b0 = C::p2_b(l);
 #pragma unroll
-    for (int e = 0; e < E; ++e) a[e] = S[r3_slot(b0 + G * e + 64 * h)];
+    for (int e = 0; e < E; ++e) a[e] = S[C::slot(b0 + G * e + 64 * h)];
 #pragma unroll
     for (int s = E / 2; s >= 1; s >>= 1) {
         const int base = M / (2 * G * s) + (32 / (G * s)) * h;
@@ -416,12 +425,12 @@ __global__ void __launch_bounds__(Reg3<LOG_M>::THREADS, Reg3<LOG_M>::MIN_WAVES)
         }
     }
 #pragma unroll
-    for (int e = 0; e < E; ++e) S[r3_slot(b0 + G * e + 64 * h)] = a[e];
+    for (int e = 0; e < E; ++e) S[C::slot(b0 + G * e + 64 * h)] = a[e];
     __syncthreads();
     // pass 3: registers e hold x = E l + e; layer len = t pairs e, e + t and uses
     // zeta[m / (2 t) + E l / (2 t) + e / (2 t)]
 #pragma unroll
-    for (int e = 0; e < E; ++e) a[e] = S[r3_slot(E * l + e)];
+    for (int e = 0; e < E; ++e) a[e] = S[C::slot(E * l + e)];
 #pragma unroll
     for (int t = G / 2; t >= 1; t >>= 1) {
         const int base = M / (2 * t) + (E / (2 * t)) * l;
@@ -458,8 +467,8 @@ static bool reg_ntt_enabled() {
 }
 
 // Variants per degree: variant 0 and 1 are register blockings (E elements x TP threads per prime
-// slice; TP <= E, TP <= 32, E * TP = m; at 2^10 and 2^11 variant 1 is the three-pass kernel), the
-// last is the per-layer kernel. Every variant computes the same residues; the context picks the fastest
+// slice; TP <= E, TP <= 32, E * TP = m; at 2^9, 2^10 and 2^11 variant 1 is the three-pass
+// kernel), the last is the per-layer kernel. Every variant computes the same residues; the context picks the fastest
 // per degree by timing.
 int fwd_variant_count(int log_m) {
     if (!reg_ntt_enabled()) return 1;
@@ -482,7 +491,7 @@ void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, cons
             case 16: return launch_reg<16, 16>(s1, s2, tw, s, jb, it);
             case 17: return launch_reg<32, 8>(s1, s2, tw, s, jb, it);
             case 18: return launch_reg<32, 16>(s1, s2, tw, s, jb, it);
-            case 19: return launch_reg<64, 8>(s1, s2, tw, s, jb, it);
+            case 19: return launch_reg3<9>(s1, s2, tw, s, jb, it);
             case 20: return launch_reg<32, 32>(s1, s2, tw, s, jb, it);
             case 21: return launch_reg3<10>(s1, s2, tw, s, jb, it);
             case 22: return launch_reg<64, 32>(s1, s2, tw, s, jb, it);

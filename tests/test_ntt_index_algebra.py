@@ -1,6 +1,6 @@
 """The register-blocked forward transforms' index algebra, checked on the CPU for every register
 blocking the library launches (read from damascus000_amd/csrc/dmx_ntt.hip): the two passes (and
-the three passes of the 2^10 / 2^11 kernel) equal the per-layer negacyclic transform on random
+the three passes of the 2^9 / 2^10 / 2^11 kernel) equal the per-layer negacyclic transform on random
 input, the padded LDS layouts keep each prime slice's elements on distinct words of its own slice,
 and no 32-lane LDS access of any pass repeats a bank."""
 import os
@@ -118,7 +118,7 @@ def test_lds_layout_is_injective_without_bank_repeats(E, TP):
                 assert len(banks) == len(set(banks)), (E, TP, base, i)
 
 
-# ---------------- the three-pass register transform (k_ntt_fwd_reg3, degrees 2^10 and 2^11) ----------------
+# ---------------- the three-pass register transform (k_ntt_fwd_reg3, degrees 2^9, 2^10 and 2^11) ----------------
 def _reg3_degrees():
     src = open(os.path.join(ROOT, "damascus000_amd", "csrc", "dmx_ntt.hip")).read()
     degrees = sorted({int(d) for d in re.findall(r"launch_reg3<(\d+)>\(s1, s2", src)})
@@ -126,13 +126,23 @@ def _reg3_degrees():
     return degrees
 
 
-def _r3_slot(x):
+def _r3_slot(x, log_m=10):
+    """Reg3::slot: padded at 2^10 and 2^11, swizzled inside 32-word rows at 2^9"""
+    if log_m == 9:
+        r = x >> 5
+        return x ^ ((r & 7) | (((r >> 1) & 3) << 3))
     return x + (x >> 5)
+
+
+def _r3_words(log_m):
+    """Reg3::SL"""
+    m = 1 << log_m
+    return m if log_m == 9 else m + m // 32
 
 
 def _r3_p2_lane(G, l):
     """Reg3::p2_h / p2_b: the pass-2 (h, b) of lane l"""
-    return (l >> 1, l & 1) if G == 2 else (l & 15, l >> 4)
+    return {2: (l >> 1, l & 1), 4: (l & 15, l >> 4), 8: (l >> 3, l & 7)}[G]
 
 
 def _r3_passes(log_m):
@@ -159,19 +169,19 @@ def three_pass(x, log_m):
     zeta[E/(2s) + i/(2s)], pass 2 layers len = G s with zeta[m/(2Gs) + 32h/(Gs) + e/(2s)], pass 3
     layers len = t (t = G/2 .. 1) with zeta[m/(2t) + E l/(2t) + e/(2t)]"""
     M, E, G, (p1, p2, p3) = _r3_passes(log_m)
-    lds = [None] * (M + M // 32)
+    lds = [None] * _r3_words(log_m)
     for c in range(M):
-        lds[_r3_slot(c)] = x[c]
+        lds[_r3_slot(c, log_m)] = x[c]
 
     def run(xof, layers):
         for l in range(64):
-            a = [lds[_r3_slot(xof(l, i))] for i in range(E)]
+            a = [lds[_r3_slot(xof(l, i), log_m)] for i in range(E)]
             for s, zeta_of in layers(l):
                 for b in range(E // (2 * s)):
                     for k in range(s):
                         _butterfly(a, 2 * s * b + k, 2 * s * b + k + s, ZETA[zeta_of(b)])
             for i in range(E):
-                lds[_r3_slot(xof(l, i))] = a[i]
+                lds[_r3_slot(xof(l, i), log_m)] = a[i]
 
     def halving(n):
         while n >= 1:
@@ -182,16 +192,19 @@ def three_pass(x, log_m):
     run(p2, lambda l: [(s, lambda b, s=s: M // (2 * G * s) + (32 // (G * s)) * _r3_p2_lane(G, l)[0] + b)
                        for s in halving(E // 2)])
     run(p3, lambda l: [(t, lambda b, t=t: M // (2 * t) + (E // (2 * t)) * l + b) for t in halving(G // 2)])
-    return [lds[_r3_slot(c)] for c in range(M)]
+    return [lds[_r3_slot(c, log_m)] for c in range(M)]
 
 
 def test_three_pass_lane_split_is_the_kernels():
     src = open(os.path.join(ROOT, "damascus000_amd", "csrc", "dmx_ntt.hip")).read()
-    assert "p2_h(int l) { return G == 2 ? l >> 1 : l & 15; }" in src
-    assert "p2_b(int l) { return G == 2 ? l & 1 : l >> 4; }" in src
+    assert "p2_h(int l) { return G == 2 ? l >> 1 : G == 4 ? l & 15 : l >> 3; }" in src
+    assert "p2_b(int l) { return G == 2 ? l & 1 : G == 4 ? l >> 4 : l & 7; }" in src
     assert "const int base = M / (2 * G * s) + (32 / (G * s)) * h;" in src
     assert "const int base = M / (2 * t) + (E / (2 * t)) * l;" in src
-    assert "__device__ __forceinline__ int r3_slot(int x) { return x + (x >> 5); }" in src
+    assert "static constexpr bool SWIZZLE = LOG_M == 9;" in src
+    assert "static constexpr int SL = SWIZZLE ? M : M + M / 32;" in src
+    assert "return x ^ ((r & 7) | (((r >> 1) & 3) << 3));" in src
+    assert "return x + (x >> 5);" in src
 
 
 @pytest.mark.parametrize("log_m", _reg3_degrees())
@@ -204,16 +217,16 @@ def test_three_pass_transform_equals_per_layer_transform(log_m):
 @pytest.mark.parametrize("log_m", _reg3_degrees())
 def test_three_pass_lds_accesses_have_no_bank_repeats(log_m):
     M, E, G, passes = _r3_passes(log_m)
-    SL = M + M // 32
-    assert max(_r3_slot(x) for x in range(M)) < SL
+    slots = [_r3_slot(x, log_m) for x in range(M)]
+    assert len(set(slots)) == M and max(slots) < _r3_words(log_m)
     for xof in passes:  # each lane's registers are a permutation of the slice's elements
         assert sorted(xof(l, i) for l in range(64) for i in range(E)) == list(range(M))
         for half in (0, 32):  # the 32 lanes of one LDS access of a wave (one prime slice)
             for i in range(E):
-                banks = [_r3_slot(xof(l, i)) % 32 for l in range(half, half + 32)]
+                banks = [_r3_slot(xof(l, i), log_m) % 32 for l in range(half, half + 32)]
                 assert len(set(banks)) == 32, (log_m, half, i)
-    for half in (0, 32):  # the producer: lanes write consecutive coefficients c
-        banks = [_r3_slot(c) % 32 for c in range(half, half + 32)]
+    for base in range(0, M, 32):  # the producer: 32 lanes write consecutive coefficients c
+        banks = [_r3_slot(c, log_m) % 32 for c in range(base, base + 32)]
         assert len(set(banks)) == 32
 
 
