@@ -314,47 +314,48 @@ static void launch_reg(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, 
 }
 
 // ---------------------------------------------------------------------------------------
-// Three-pass register transform for degrees 2^9, 2^10 and 2^11: 64 threads per prime slice
-// holding E = m / 64 elements each (8, 16 or 32), 512 threads per row, one row per workgroup.
-// Against the two-pass blockings this takes a fraction of the LDS words per resident thread, so a
-// CU holds more waves. With G = 64 / E:
-//   pass 1: thread l holds x = l + 64 i (layers len = m/2 .. 64);
-//   pass 2: x = b + G e + 64 h (layers len = 32 .. G), with l = 2 h + b for G = 2,
+// Three-pass register transform for degrees 2^9, 2^10 and 2^11: TP threads per prime slice
+// (64, or 128 at 2^11) holding E = m / TP elements each (8, 16 or 32), 8 TP threads per row, one
+// row per workgroup. Against the two-pass blockings this takes a fraction of the LDS words per
+// resident thread, so a CU holds more waves. With G = TP / E:
+//   pass 1: thread l holds x = l + TP i (layers len = m/2 .. TP);
+//   pass 2: x = b + G e + TP h (layers len = TP/2 .. G), with l = 2 h + b for G = 2,
 //           l = h + 16 b for G = 4 and l = 8 h + b for G = 8;
 //   pass 3: x = E l + e (layers len = G/2 .. 1).
-// One LDS layout per degree serves the producer and all three passes without bank conflicts:
-// the padded word x + (x >> 5) at 2^10 and 2^11, and at 2^9, where no simple pad does, the
-// swizzle x ^ s(x >> 5) with s(r) = (r & 7) | ((r >> 1) & 3) << 3 inside each 32-word row.
-// Each pass writes back the words it read, so three barriers order the row.
+// One LDS layout per shape serves the producer and all three passes without bank conflicts: the
+// padded word x + (x >> 5) for TP = 64 at 2^10 and 2^11, and where no simple pad does, a swizzle
+// x ^ s(x >> 5) inside each 32-word row: s(r) = (r & 7) | ((r >> 1) & 3) << 3 at 2^9 and
+// s(r) = (r & 15) | ((r >> 2) & 1) << 4 for TP = 128 at 2^11. Each pass writes back the words it
+// read, so three barriers order the row.
 // ---------------------------------------------------------------------------------------
-template <int LOG_M>
+template <int LOG_M, int TP_ = 64>
 struct Reg3 {
     static constexpr int M = 1 << LOG_M;
-    static constexpr int TP = 64;
+    static constexpr int TP = TP_;
+    static constexpr int LOG_TP = __builtin_ctz(TP);
     static constexpr int E = M / TP;
     static constexpr int G = TP / E;
     static constexpr int THREADS = NPRIMES * TP;
-    static constexpr bool SWIZZLE = LOG_M == 9;
-    static constexpr int SL = SWIZZLE ? M : M + M / 32;  // LDS words per prime slice
+    static constexpr int SWIZZLE = LOG_M == 9 ? 1 : TP == 128 ? 2 : 0;  // 0: padded rows
+    static constexpr int SL = SWIZZLE ? M : M + M / 32;                 // LDS words per prime slice
     static constexpr size_t LDS_BYTES = (size_t)NPRIMES * SL * 4;
-    static constexpr int MIN_WAVES = LOG_M == 11 ? 4 : 8;  // waves per SIMD asked of the register allocator
-    static_assert(G == 2 || G == 4 || G == 8, "degrees 2^9, 2^10 and 2^11 only");
+    static constexpr int MIN_WAVES = E == 32 ? 4 : 8;  // waves per SIMD asked of the register allocator
+    static_assert((G == 2 || G == 4 || G == 8) && (TP == 64 || (TP == 128 && LOG_M == 11)), "shape");
     __device__ static __forceinline__ int slot(int x) {
-        if (SWIZZLE) {
-            const int r = x >> 5;
-            return x ^ ((r & 7) | (((r >> 1) & 3) << 3));
-        }
-        return x + (x >> 5);
+        const int r = x >> 5;
+        if (SWIZZLE == 1) return x ^ ((r & 7) | (((r >> 1) & 3) << 3));
+        if (SWIZZLE == 2) return x ^ ((r & 15) | (((r >> 2) & 1) << 4));
+        return x + r;
     }
     // pass-2 lane split, chosen so the 32 lanes of an LDS access hit 32 banks
     __device__ static __forceinline__ int p2_h(int l) { return G == 2 ? l >> 1 : G == 4 ? l & 15 : l >> 3; }
     __device__ static __forceinline__ int p2_b(int l) { return G == 2 ? l & 1 : G == 4 ? l >> 4 : l & 7; }
 };
 
-template <int LOG_M>
-__global__ void __launch_bounds__(Reg3<LOG_M>::THREADS, Reg3<LOG_M>::MIN_WAVES)
+template <int LOG_M, int TP>
+__global__ void __launch_bounds__((Reg3<LOG_M, TP>::THREADS), (Reg3<LOG_M, TP>::MIN_WAVES))
     k_ntt_fwd_reg3(FwdSeg s1, FwdSeg s2, uint32_t blocks1, NttTables tw, size_t js, InvTask it) {
-    using C = Reg3<LOG_M>;
+    using C = Reg3<LOG_M, TP>;
     constexpr int M = C::M, E = C::E, G = C::G;
     extern __shared__ uint32_t sh[];
     uint32_t bx = blockIdx.x;
@@ -387,16 +388,16 @@ __global__ void __launch_bounds__(Reg3<LOG_M>::THREADS, Reg3<LOG_M>::MIN_WAVES)
         for (int q = 0; q < NPRIMES; ++q) sh[q * C::SL + C::slot(c)] = zq_mod_p_centred(x, c_pc[q]);
     }
     __syncthreads();
-    const int q = tid >> 6, l = tid & 63;
+    const int q = tid >> C::LOG_TP, l = tid & (TP - 1);
     uint32_t* S = sh + q * C::SL;
     const uint32_t p = c_pc[q].p;
     const uint32_t* zt = tw.zeta_m + q * TW_SIZE;
     const uint32_t* ztp = tw.zeta_mp + q * TW_SIZE;
     uint32_t a[E];
-    // pass 1: registers i hold x = l + 64 i; layer len = 64 s pairs i, i + s and uses
+    // pass 1: registers i hold x = l + TP i; layer len = TP s pairs i, i + s and uses
     // zeta[E / (2 s) + i / (2 s)]
 #pragma unroll
-    for (int i = 0; i < E; ++i) a[i] = S[C::slot(l + 64 * i)];
+    for (int i = 0; i < E; ++i) a[i] = S[C::slot(l + TP * i)];
 #pragma unroll
     for (int s = E / 2; s >= 1; s >>= 1) {
 #pragma unroll
@@ -407,16 +408,16 @@ __global__ void __launch_bounds__(Reg3<LOG_M>::THREADS, Reg3<LOG_M>::MIN_WAVES)
         }
     }
 #pragma unroll
-    for (int i = 0; i < E; ++i) S[C::slot(l + 64 * i)] = a[i];
+    for (int i = 0; i < E; ++i) S[C::slot(l + TP * i)] = a[i];
     __syncthreads();
-    // pass 2: registers e hold x = b + G e + 64 h; layer len = G s pairs e, e + s and uses
-    // zeta[m / (2 G s) + 32 h / (G s) + e / (2 s)]
+    // pass 2: registers e hold x = b + G e + TP h; layer len = G s pairs e, e + s and uses
+    // zeta[m / (2 G s) + (TP / 2) h / (G s) + e / (2 s)]
     const int h = C::p2_h(l), b0 = C::p2_b(l);
 #pragma unroll
-    for (int e = 0; e < E; ++e) a[e] = S[C::slot(b0 + G * e + 64 * h)];
+    for (int e = 0; e < E; ++e) a[e] = S[C::slot(b0 + G * e + TP * h)];
 #pragma unroll
     for (int s = E / 2; s >= 1; s >>= 1) {
-        const int base = M / (2 * G * s) + (32 / (G * s)) * h;
+        const int base = M / (2 * G * s) + (TP / 2 / (G * s)) * h;
 #pragma unroll
         for (int b = 0; b < E / (2 * s); ++b) {
             const uint32_t w = zt[base + b], wp = ztp[base + b];
@@ -425,7 +426,7 @@ __global__ void __launch_bounds__(Reg3<LOG_M>::THREADS, Reg3<LOG_M>::MIN_WAVES)
         }
     }
 #pragma unroll
-    for (int e = 0; e < E; ++e) S[C::slot(b0 + G * e + 64 * h)] = a[e];
+    for (int e = 0; e < E; ++e) S[C::slot(b0 + G * e + TP * h)] = a[e];
     __syncthreads();
     // pass 3: registers e hold x = E l + e; layer len = t pairs e, e + t and uses
     // zeta[m / (2 t) + E l / (2 t) + e / (2 t)]
@@ -446,14 +447,14 @@ __global__ void __launch_bounds__(Reg3<LOG_M>::THREADS, Reg3<LOG_M>::MIN_WAVES)
     for (int e = 0; e < E; e += 4) o[e / 4] = make_uint4(a[e], a[e + 1], a[e + 2], a[e + 3]);
 }
 
-template <int LOG_M>
+template <int LOG_M, int TP = 64>
 static void launch_reg3(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, hipStream_t s, const Jobs& jb,
                         const InvTask& it) {
-    using C = Reg3<LOG_M>;
+    using C = Reg3<LOG_M, TP>;
     if (C::LDS_BYTES > 65536)  // past the default dynamic LDS limit
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ntt_fwd_reg3<LOG_M>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ntt_fwd_reg3<LOG_M, TP>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS_BYTES);
-    hipLaunchKernelGGL((k_ntt_fwd_reg3<LOG_M>), dim3((unsigned)(s1.rows + s2.rows + (it.chal ? 1 : 0)), jb.n),
+    hipLaunchKernelGGL((k_ntt_fwd_reg3<LOG_M, TP>), dim3((unsigned)(s1.rows + s2.rows + (it.chal ? 1 : 0)), jb.n),
                        dim3(C::THREADS), C::LDS_BYTES, s, s1, s2, (uint32_t)s1.rows, tw, jb.stride, it);
 }
 
@@ -494,7 +495,7 @@ void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, cons
             case 19: return launch_reg3<9>(s1, s2, tw, s, jb, it);
             case 20: return launch_reg<32, 32>(s1, s2, tw, s, jb, it);
             case 21: return launch_reg3<10>(s1, s2, tw, s, jb, it);
-            case 22: return launch_reg<64, 32>(s1, s2, tw, s, jb, it);
+            case 22: return launch_reg3<11, 128>(s1, s2, tw, s, jb, it);
             case 23: return launch_reg3<11>(s1, s2, tw, s, jb, it);
             default: break;
         }

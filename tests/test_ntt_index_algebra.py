@@ -119,25 +119,28 @@ def test_lds_layout_is_injective_without_bank_repeats(E, TP):
 
 
 # ---------------- the three-pass register transform (k_ntt_fwd_reg3, degrees 2^9, 2^10 and 2^11) ----------------
-def _reg3_degrees():
+def _reg3_shapes():
     src = open(os.path.join(ROOT, "damascus000_amd", "csrc", "dmx_ntt.hip")).read()
-    degrees = sorted({int(d) for d in re.findall(r"launch_reg3<(\d+)>\(s1, s2", src)})
-    assert degrees, "no three-pass launches found"
-    return degrees
+    shapes = sorted({(int(d), int(t or 64)) for d, t in re.findall(r"launch_reg3<(\d+)(?:, (\d+))?>\(s1, s2", src)})
+    assert shapes, "no three-pass launches found"
+    return shapes
 
 
-def _r3_slot(x, log_m=10):
-    """Reg3::slot: padded at 2^10 and 2^11, swizzled inside 32-word rows at 2^9"""
+def _r3_slot(x, log_m, tp):
+    """Reg3::slot: padded rows for 64 threads per slice at 2^10 and 2^11, swizzled inside 32-word
+    rows at 2^9 and for 128 threads per slice at 2^11"""
+    r = x >> 5
     if log_m == 9:
-        r = x >> 5
         return x ^ ((r & 7) | (((r >> 1) & 3) << 3))
-    return x + (x >> 5)
+    if tp == 128:
+        return x ^ ((r & 15) | (((r >> 2) & 1) << 4))
+    return x + r
 
 
-def _r3_words(log_m):
+def _r3_words(log_m, tp):
     """Reg3::SL"""
     m = 1 << log_m
-    return m if log_m == 9 else m + m // 32
+    return m if log_m == 9 or tp == 128 else m + m // 32
 
 
 def _r3_p2_lane(G, l):
@@ -145,18 +148,18 @@ def _r3_p2_lane(G, l):
     return {2: (l >> 1, l & 1), 4: (l & 15, l >> 4), 8: (l >> 3, l & 7)}[G]
 
 
-def _r3_passes(log_m):
-    """(x of register i for lane l, twiddle index of layer len for lane l and register i) per pass"""
+def _r3_passes(log_m, tp):
+    """the element x that register i of lane l holds, per pass"""
     M = 1 << log_m
-    E = M // 64
-    G = 64 // E
+    E = M // tp
+    G = tp // E
 
     def p1(l, i):
-        return l + 64 * i
+        return l + tp * i
 
     def p2(l, e):
         h, b = _r3_p2_lane(G, l)
-        return b + G * e + 64 * h
+        return b + G * e + tp * h
 
     def p3(l, e):
         return E * l + e
@@ -164,24 +167,24 @@ def _r3_passes(log_m):
     return M, E, G, (p1, p2, p3)
 
 
-def three_pass(x, log_m):
-    """k_ntt_fwd_reg3 through a simulated LDS slice: pass 1 layers len = 64 s (s = E/2 .. 1) with
-    zeta[E/(2s) + i/(2s)], pass 2 layers len = G s with zeta[m/(2Gs) + 32h/(Gs) + e/(2s)], pass 3
-    layers len = t (t = G/2 .. 1) with zeta[m/(2t) + E l/(2t) + e/(2t)]"""
-    M, E, G, (p1, p2, p3) = _r3_passes(log_m)
-    lds = [None] * _r3_words(log_m)
+def three_pass(x, log_m, tp):
+    """k_ntt_fwd_reg3 through a simulated LDS slice: pass 1 layers len = TP s (s = E/2 .. 1) with
+    zeta[E/(2s) + i/(2s)], pass 2 layers len = G s with zeta[m/(2Gs) + (TP/2)h/(Gs) + e/(2s)],
+    pass 3 layers len = t (t = G/2 .. 1) with zeta[m/(2t) + E l/(2t) + e/(2t)]"""
+    M, E, G, (p1, p2, p3) = _r3_passes(log_m, tp)
+    lds = [None] * _r3_words(log_m, tp)
     for c in range(M):
-        lds[_r3_slot(c, log_m)] = x[c]
+        lds[_r3_slot(c, log_m, tp)] = x[c]
 
     def run(xof, layers):
-        for l in range(64):
-            a = [lds[_r3_slot(xof(l, i), log_m)] for i in range(E)]
+        for l in range(tp):
+            a = [lds[_r3_slot(xof(l, i), log_m, tp)] for i in range(E)]
             for s, zeta_of in layers(l):
                 for b in range(E // (2 * s)):
                     for k in range(s):
                         _butterfly(a, 2 * s * b + k, 2 * s * b + k + s, ZETA[zeta_of(b)])
             for i in range(E):
-                lds[_r3_slot(xof(l, i), log_m)] = a[i]
+                lds[_r3_slot(xof(l, i), log_m, tp)] = a[i]
 
     def halving(n):
         while n >= 1:
@@ -189,44 +192,45 @@ def three_pass(x, log_m):
             n //= 2
 
     run(p1, lambda l: [(s, lambda b, s=s: E // (2 * s) + b) for s in halving(E // 2)])
-    run(p2, lambda l: [(s, lambda b, s=s: M // (2 * G * s) + (32 // (G * s)) * _r3_p2_lane(G, l)[0] + b)
+    run(p2, lambda l: [(s, lambda b, s=s: M // (2 * G * s) + (tp // 2 // (G * s)) * _r3_p2_lane(G, l)[0] + b)
                        for s in halving(E // 2)])
     run(p3, lambda l: [(t, lambda b, t=t: M // (2 * t) + (E // (2 * t)) * l + b) for t in halving(G // 2)])
-    return [lds[_r3_slot(c, log_m)] for c in range(M)]
+    return [lds[_r3_slot(c, log_m, tp)] for c in range(M)]
 
 
 def test_three_pass_lane_split_is_the_kernels():
     src = open(os.path.join(ROOT, "damascus000_amd", "csrc", "dmx_ntt.hip")).read()
     assert "p2_h(int l) { return G == 2 ? l >> 1 : G == 4 ? l & 15 : l >> 3; }" in src
     assert "p2_b(int l) { return G == 2 ? l & 1 : G == 4 ? l >> 4 : l & 7; }" in src
-    assert "const int base = M / (2 * G * s) + (32 / (G * s)) * h;" in src
+    assert "const int base = M / (2 * G * s) + (TP / 2 / (G * s)) * h;" in src
     assert "const int base = M / (2 * t) + (E / (2 * t)) * l;" in src
-    assert "static constexpr bool SWIZZLE = LOG_M == 9;" in src
+    assert "static constexpr int SWIZZLE = LOG_M == 9 ? 1 : TP == 128 ? 2 : 0;" in src
     assert "static constexpr int SL = SWIZZLE ? M : M + M / 32;" in src
-    assert "return x ^ ((r & 7) | (((r >> 1) & 3) << 3));" in src
-    assert "return x + (x >> 5);" in src
+    assert "if (SWIZZLE == 1) return x ^ ((r & 7) | (((r >> 1) & 3) << 3));" in src
+    assert "if (SWIZZLE == 2) return x ^ ((r & 15) | (((r >> 2) & 1) << 4));" in src
+    assert "return x + r;" in src
 
 
-@pytest.mark.parametrize("log_m", _reg3_degrees())
-def test_three_pass_transform_equals_per_layer_transform(log_m):
-    rng = random.Random(log_m)
+@pytest.mark.parametrize("log_m,tp", _reg3_shapes())
+def test_three_pass_transform_equals_per_layer_transform(log_m, tp):
+    rng = random.Random(log_m * 1000 + tp)
     x = [rng.randrange(P) for _ in range(1 << log_m)]
-    assert three_pass(x, log_m) == per_layer(x)
+    assert three_pass(x, log_m, tp) == per_layer(x)
 
 
-@pytest.mark.parametrize("log_m", _reg3_degrees())
-def test_three_pass_lds_accesses_have_no_bank_repeats(log_m):
-    M, E, G, passes = _r3_passes(log_m)
-    slots = [_r3_slot(x, log_m) for x in range(M)]
-    assert len(set(slots)) == M and max(slots) < _r3_words(log_m)
+@pytest.mark.parametrize("log_m,tp", _reg3_shapes())
+def test_three_pass_lds_accesses_have_no_bank_repeats(log_m, tp):
+    M, E, G, passes = _r3_passes(log_m, tp)
+    slots = [_r3_slot(x, log_m, tp) for x in range(M)]
+    assert len(set(slots)) == M and max(slots) < _r3_words(log_m, tp)
     for xof in passes:  # each lane's registers are a permutation of the slice's elements
-        assert sorted(xof(l, i) for l in range(64) for i in range(E)) == list(range(M))
-        for half in (0, 32):  # the 32 lanes of one LDS access of a wave (one prime slice)
+        assert sorted(xof(l, i) for l in range(tp) for i in range(E)) == list(range(M))
+        for half in range(0, tp, 32):  # the 32 lanes of one LDS access of a wave
             for i in range(E):
-                banks = [_r3_slot(xof(l, i), log_m) % 32 for l in range(half, half + 32)]
-                assert len(set(banks)) == 32, (log_m, half, i)
+                banks = [_r3_slot(xof(l, i), log_m, tp) % 32 for l in range(half, half + 32)]
+                assert len(set(banks)) == 32, (log_m, tp, half, i)
     for base in range(0, M, 32):  # the producer: 32 lanes write consecutive coefficients c
-        banks = [_r3_slot(c, log_m) % 32 for c in range(base, base + 32)]
+        banks = [_r3_slot(c, log_m, tp) % 32 for c in range(base, base + 32)]
         assert len(set(banks)) == 32
 
 
@@ -379,7 +383,8 @@ def test_kernel_names_per_variant_follow_the_launch_switch():
     from damascus000_amd import FWD_KERNELS
     src = open(os.path.join(ROOT, "damascus000_amd", "csrc", "dmx_ntt.hip")).read()
     launched = {}
-    for case, e, t, d in re.findall(r"case (\d+): return launch_reg(?:<(\d+), (\d+)>|3<(\d+)>)\(s1, s2", src):
+    for case, e, t, d, tp in re.findall(r"case (\d+): return launch_reg(?:<(\d+), (\d+)>|3<(\d+)(?:, (\d+))?>)\(s1, s2",
+                                        src):
         lg, v = divmod(int(case), 2)
-        launched[(lg, v)] = f"k_ntt_fwd_reg<{e}, {t}>" if e else f"k_ntt_fwd_reg3<{d}>"
+        launched[(lg, v)] = f"k_ntt_fwd_reg<{e}, {t}>" if e else f"k_ntt_fwd_reg3<{d}, {tp or 64}>"
     assert launched == {(lg, v): FWD_KERNELS[lg][v] for lg in FWD_KERNELS for v in (0, 1)}
