@@ -50,7 +50,7 @@ _VP = ctypes.c_void_p
 FWD_KERNELS = {
     6: ("k_ntt_fwd_reg<8, 8>", "k_ntt_fwd_reg<16, 4>"),
     7: ("k_ntt_fwd_reg<16, 8>", "k_ntt_fwd_reg<32, 4>"),
-    8: ("k_ntt_fwd_reg<16, 16>", "k_ntt_fwd_reg<32, 8>"),
+    8: ("k_ntt_fwd_reg<16, 16>", "k_ntt_fwd_reg3<8, 32>"),
     9: ("k_ntt_fwd_reg<32, 16>", "k_ntt_fwd_reg3<9, 64>"),
     10: ("k_ntt_fwd_reg<32, 32>", "k_ntt_fwd_reg3<10, 64>"),
     11: ("k_ntt_fwd_reg3<11, 128>", "k_ntt_fwd_reg3<11, 64>"),

@@ -314,19 +314,19 @@ static void launch_reg(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, 
 }
 
 // ---------------------------------------------------------------------------------------
-// Three-pass register transform for degrees 2^9, 2^10 and 2^11: TP threads per prime slice
-// (64, or 128 at 2^11) holding E = m / TP elements each (8, 16 or 32), 8 TP threads per row, one
+// Three-pass register transform for degrees 2^8..2^11: TP threads per prime slice (32 at 2^8,
+// 64, or 128 at 2^11) holding E = m / TP elements each (8, 16 or 32), 8 TP threads per row, one
 // row per workgroup. Against the two-pass blockings this takes a fraction of the LDS words per
 // resident thread, so a CU holds more waves. With G = TP / E:
 //   pass 1: thread l holds x = l + TP i (layers len = m/2 .. TP);
 //   pass 2: x = b + G e + TP h (layers len = TP/2 .. G), with l = 2 h + b for G = 2,
-//           l = h + 16 b for G = 4 and l = 8 h + b for G = 8;
+//           l = h + E b for G = 4 and l = 8 h + b for G = 8;
 //   pass 3: x = E l + e (layers len = G/2 .. 1).
 // One LDS layout per shape serves the producer and all three passes without bank conflicts: the
 // padded word x + (x >> 5) for TP = 64 at 2^10 and 2^11, and where no simple pad does, a swizzle
-// x ^ s(x >> 5) inside each 32-word row: s(r) = (r & 7) | ((r >> 1) & 3) << 3 at 2^9 and
-// s(r) = (r & 15) | ((r >> 2) & 1) << 4 for TP = 128 at 2^11. Each pass writes back the words it
-// read, so three barriers order the row.
+// x ^ s(x >> 5) inside each 32-word row: s(r) = (r & 7) | ((r >> 1) & 3) << 3 at 2^9,
+// s(r) = (r & 15) | ((r >> 2) & 1) << 4 for TP = 128 at 2^11 and s(r) = r ^ (r << 2) at 2^8.
+// Each pass writes back the words it read, so three barriers order the row.
 // ---------------------------------------------------------------------------------------
 template <int LOG_M, int TP_ = 64>
 struct Reg3 {
@@ -336,20 +336,23 @@ struct Reg3 {
     static constexpr int E = M / TP;
     static constexpr int G = TP / E;
     static constexpr int THREADS = NPRIMES * TP;
-    static constexpr int SWIZZLE = LOG_M == 9 ? 1 : TP == 128 ? 2 : 0;  // 0: padded rows
+    static constexpr int LOG_E = LOG_M - LOG_TP;
+    static constexpr int SWIZZLE = LOG_M == 9 ? 1 : TP == 128 ? 2 : TP == 32 ? 3 : 0;  // 0: padded rows
     static constexpr int SL = SWIZZLE ? M : M + M / 32;                 // LDS words per prime slice
     static constexpr size_t LDS_BYTES = (size_t)NPRIMES * SL * 4;
     static constexpr int MIN_WAVES = E == 32 ? 4 : 8;  // waves per SIMD asked of the register allocator
-    static_assert((G == 2 || G == 4 || G == 8) && (TP == 64 || (TP == 128 && LOG_M == 11)), "shape");
+    static_assert((G == 2 || G == 4 || G == 8) &&
+                      (TP == 64 || (TP == 128 && LOG_M == 11) || (TP == 32 && LOG_M == 8)), "shape");
     __device__ static __forceinline__ int slot(int x) {
         const int r = x >> 5;
         if (SWIZZLE == 1) return x ^ ((r & 7) | (((r >> 1) & 3) << 3));
         if (SWIZZLE == 2) return x ^ ((r & 15) | (((r >> 2) & 1) << 4));
+        if (SWIZZLE == 3) return x ^ (r ^ (r << 2));
         return x + r;
     }
     // pass-2 lane split, chosen so the 32 lanes of an LDS access hit 32 banks
-    __device__ static __forceinline__ int p2_h(int l) { return G == 2 ? l >> 1 : G == 4 ? l & 15 : l >> 3; }
-    __device__ static __forceinline__ int p2_b(int l) { return G == 2 ? l & 1 : G == 4 ? l >> 4 : l & 7; }
+    __device__ static __forceinline__ int p2_h(int l) { return G == 2 ? l >> 1 : G == 4 ? l & (E - 1) : l >> 3; }
+    __device__ static __forceinline__ int p2_b(int l) { return G == 2 ? l & 1 : G == 4 ? l >> LOG_E : l & 7; }
 };
 
 template <int LOG_M, int TP>
@@ -468,8 +471,8 @@ static bool reg_ntt_enabled() {
 }
 
 // Variants per degree: variant 0 and 1 are register blockings (E elements x TP threads per prime
-// slice; TP <= E, TP <= 32, E * TP = m; at 2^9, 2^10 and 2^11 variant 1 is the three-pass
-// kernel), the last is the per-layer kernel. Every variant computes the same residues; the context picks the fastest
+// slice; TP <= E, TP <= 32, E * TP = m; from 2^8 on one of them is the three-pass kernel), the
+// last is the per-layer kernel. Every variant computes the same residues; the context picks the fastest
 // per degree by timing.
 int fwd_variant_count(int log_m) {
     if (!reg_ntt_enabled()) return 1;
@@ -490,7 +493,7 @@ void launch_ntt_fwd_rows2(const RowSrc& src1, size_t rows1, uint32_t* out1, cons
             case 14: return launch_reg<16, 8>(s1, s2, tw, s, jb, it);
             case 15: return launch_reg<32, 4>(s1, s2, tw, s, jb, it);
             case 16: return launch_reg<16, 16>(s1, s2, tw, s, jb, it);
-            case 17: return launch_reg<32, 8>(s1, s2, tw, s, jb, it);
+            case 17: return launch_reg3<8, 32>(s1, s2, tw, s, jb, it);
             case 18: return launch_reg<32, 16>(s1, s2, tw, s, jb, it);
             case 19: return launch_reg3<9>(s1, s2, tw, s, jb, it);
             case 20: return launch_reg<32, 32>(s1, s2, tw, s, jb, it);

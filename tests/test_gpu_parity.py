@@ -464,30 +464,32 @@ def test_fused_job_rejects_a_wrong_final_opening():
 
 def test_transform_variants_give_identical_proofs():
     """Every forward-transform variant (register blockings of the prime slices, the three-pass
-    kernel at 2^9, 2^10 and 2^11 and the per-layer kernel) gives the same proof: at d = 7 each
-    against the oracle, at d = 9, 10 and 11 (generator rows at 2^9, 2^10 and 2^11, fold rows down
-    to 2^6) across variants through the fused job, whose captured graph is retaken after a
-    change."""
+    kernels at 2^8..2^11 and the per-layer kernel) gives the same proof: at d = 7 each against the
+    oracle, at d = 8..11 (generator rows at 2^8..2^11, fold rows down to 2^6) across variants
+    through the fused job, whose captured graph is retaken after a change."""
     ctx = dm.Context.default()
     saved = {lg: ctx.transform_variant(lg)[0] for lg in range(6, 12)}
     say("tuned transform variants (variant, count, us):", {lg: ctx.transform_variant(lg) for lg in range(6, 12)})
     small = W.pattern_bytes(W.byte_len_for_depth(7), 5)
+    d8 = np.random.default_rng(8).integers(0, 256, size=512 << 10, dtype=np.uint8).tobytes()  # d = 8
     d9 = np.random.default_rng(9).integers(0, 256, size=2 << 20, dtype=np.uint8).tobytes()  # d = 9
     mid = np.random.default_rng(10).integers(0, 256, size=8 << 20, dtype=np.uint8).tobytes()  # d = 10
     big = np.random.default_rng(11).integers(0, 256, size=16 << 20, dtype=np.uint8).tobytes()  # d = 11
     rc, ost, orecs, oop = O.full_flow(small)
     assert rc == 0
-    b9, mbuf, buf = ctx.upload(d9), ctx.upload(mid), ctx.upload(big)
-    outs, mids, d9s = [], [], []
+    b8, b9, mbuf, buf = ctx.upload(d8), ctx.upload(d9), ctx.upload(mid), ctx.upload(big)
+    outs, mids, d9s, d8s = [], [], [], []
     try:
         for v in range(3):
             for lg in range(6, 12):
                 ctx.set_transform_variant(lg, min(v, ctx.transform_variant(lg)[1] - 1))
             say("all degrees at variant", v)
             assert dm.prove(small) == (ost, orecs, oop)
+            d8s.append(dm.prove_verify_device(b8.ptr, len(d8)))
             d9s.append(dm.prove_verify_device(b9.ptr, len(d9)))
             mids.append(dm.prove_verify_device(mbuf.ptr, len(mid)))
             outs.append(dm.prove_verify_device(buf.ptr, len(big)))
+        assert d8s[0] == d8s[1] == d8s[2]
         assert d9s[0] == d9s[1] == d9s[2]
         assert mids[0] == mids[1] == mids[2]
         assert outs[0] == outs[1] == outs[2]
@@ -507,6 +509,7 @@ def test_mac_shapes_give_identical_proofs():
     saved = {lg: ctx.mac_shape(lg)[0] for lg in range(6, 12)}
     say("tuned MAC shapes (shape, count, us):", {lg: ctx.mac_shape(lg) for lg in range(6, 11)})
     small = W.pattern_bytes(W.byte_len_for_depth(7), 7)
+    d8 = np.random.default_rng(8).integers(0, 256, size=512 << 10, dtype=np.uint8).tobytes()  # d = 8
     d9 = np.random.default_rng(9).integers(0, 256, size=2 << 20, dtype=np.uint8).tobytes()  # d = 9
     mid = np.random.default_rng(10).integers(0, 256, size=8 << 20, dtype=np.uint8).tobytes()  # d = 10
     rc, ost, orecs, oop = O.full_flow(small)

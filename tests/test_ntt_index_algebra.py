@@ -1,6 +1,6 @@
 """The register-blocked forward transforms' index algebra, checked on the CPU for every register
 blocking the library launches (read from damascus000_amd/csrc/dmx_ntt.hip): the two passes (and
-the three passes of the 2^9 / 2^10 / 2^11 kernel) equal the per-layer negacyclic transform on random
+the three passes of the 2^8..2^11 kernel) equal the per-layer negacyclic transform on random
 input, the padded LDS layouts keep each prime slice's elements on distinct words of its own slice,
 and no 32-lane LDS access of any pass repeats a bank."""
 import os
@@ -118,7 +118,7 @@ def test_lds_layout_is_injective_without_bank_repeats(E, TP):
                 assert len(banks) == len(set(banks)), (E, TP, base, i)
 
 
-# ---------------- the three-pass register transform (k_ntt_fwd_reg3, degrees 2^9, 2^10 and 2^11) ----------------
+# ---------------- the three-pass register transform (k_ntt_fwd_reg3, degrees 2^8..2^11) ----------------
 def _reg3_shapes():
     src = open(os.path.join(ROOT, "damascus000_amd", "csrc", "dmx_ntt.hip")).read()
     shapes = sorted({(int(d), int(t or 64)) for d, t in re.findall(r"launch_reg3<(\d+)(?:, (\d+))?>\(s1, s2", src)})
@@ -134,18 +134,20 @@ def _r3_slot(x, log_m, tp):
         return x ^ ((r & 7) | (((r >> 1) & 3) << 3))
     if tp == 128:
         return x ^ ((r & 15) | (((r >> 2) & 1) << 4))
+    if tp == 32:
+        return x ^ (r ^ (r << 2))
     return x + r
 
 
 def _r3_words(log_m, tp):
     """Reg3::SL"""
     m = 1 << log_m
-    return m if log_m == 9 or tp == 128 else m + m // 32
+    return m if log_m == 9 or tp != 64 else m + m // 32
 
 
-def _r3_p2_lane(G, l):
+def _r3_p2_lane(G, E, l):
     """Reg3::p2_h / p2_b: the pass-2 (h, b) of lane l"""
-    return {2: (l >> 1, l & 1), 4: (l & 15, l >> 4), 8: (l >> 3, l & 7)}[G]
+    return {2: (l >> 1, l & 1), 4: (l % E, l // E), 8: (l >> 3, l & 7)}[G]
 
 
 def _r3_passes(log_m, tp):
@@ -158,7 +160,7 @@ def _r3_passes(log_m, tp):
         return l + tp * i
 
     def p2(l, e):
-        h, b = _r3_p2_lane(G, l)
+        h, b = _r3_p2_lane(G, E, l)
         return b + G * e + tp * h
 
     def p3(l, e):
@@ -192,7 +194,7 @@ def three_pass(x, log_m, tp):
             n //= 2
 
     run(p1, lambda l: [(s, lambda b, s=s: E // (2 * s) + b) for s in halving(E // 2)])
-    run(p2, lambda l: [(s, lambda b, s=s: M // (2 * G * s) + (tp // 2 // (G * s)) * _r3_p2_lane(G, l)[0] + b)
+    run(p2, lambda l: [(s, lambda b, s=s: M // (2 * G * s) + (tp // 2 // (G * s)) * _r3_p2_lane(G, E, l)[0] + b)
                        for s in halving(E // 2)])
     run(p3, lambda l: [(t, lambda b, t=t: M // (2 * t) + (E // (2 * t)) * l + b) for t in halving(G // 2)])
     return [lds[_r3_slot(c, log_m, tp)] for c in range(M)]
@@ -200,11 +202,12 @@ def three_pass(x, log_m, tp):
 
 def test_three_pass_lane_split_is_the_kernels():
     src = open(os.path.join(ROOT, "damascus000_amd", "csrc", "dmx_ntt.hip")).read()
-    assert "p2_h(int l) { return G == 2 ? l >> 1 : G == 4 ? l & 15 : l >> 3; }" in src
-    assert "p2_b(int l) { return G == 2 ? l & 1 : G == 4 ? l >> 4 : l & 7; }" in src
+    assert "p2_h(int l) { return G == 2 ? l >> 1 : G == 4 ? l & (E - 1) : l >> 3; }" in src
+    assert "p2_b(int l) { return G == 2 ? l & 1 : G == 4 ? l >> LOG_E : l & 7; }" in src
     assert "const int base = M / (2 * G * s) + (TP / 2 / (G * s)) * h;" in src
     assert "const int base = M / (2 * t) + (E / (2 * t)) * l;" in src
-    assert "static constexpr int SWIZZLE = LOG_M == 9 ? 1 : TP == 128 ? 2 : 0;" in src
+    assert "static constexpr int SWIZZLE = LOG_M == 9 ? 1 : TP == 128 ? 2 : TP == 32 ? 3 : 0;" in src
+    assert "if (SWIZZLE == 3) return x ^ (r ^ (r << 2));" in src
     assert "static constexpr int SL = SWIZZLE ? M : M + M / 32;" in src
     assert "if (SWIZZLE == 1) return x ^ ((r & 7) | (((r >> 1) & 3) << 3));" in src
     assert "if (SWIZZLE == 2) return x ^ ((r & 15) | (((r >> 2) & 1) << 4));" in src
