@@ -2004,8 +2004,36 @@ static size_t gen_cache_slots() {
     return e ? (size_t)atoi(e) : 8;
 }
 
-// The matrix for (seed, d), derived on stream s when new; nullptr when the cache is off or full
-// of matrices still in use. The caller waits on ->ready before reading it.
+// Admission: a (seed, depth) gets a matrix on its second miss. A seed met once, such as a request
+// that carries its own generator seed, expands its generators inside its own job: no allocation,
+// no eviction (whose free would wait for the device) and no extra launch. The last GEN_SEEN_MAX
+// misses are remembered, oldest first.
+struct GenSeen {
+    const dmx_ctx* ctx;
+    int d;
+    uint8_t seed[32];
+};
+static std::vector<GenSeen> g_gen_seen;
+constexpr size_t GEN_SEEN_MAX = 256;
+
+// true when (ctx, seed, d) missed before (and forgets it); records the miss otherwise. Called
+// with g_gen_mu held.
+static bool gen_missed_before(const dmx_ctx* ctx, const uint8_t seed[32], int d) {
+    for (auto it = g_gen_seen.begin(); it != g_gen_seen.end(); ++it)
+        if (it->ctx == ctx && it->d == d && !memcmp(it->seed, seed, 32)) {
+            g_gen_seen.erase(it);
+            return true;
+        }
+    if (g_gen_seen.size() >= GEN_SEEN_MAX) g_gen_seen.erase(g_gen_seen.begin());
+    GenSeen m{ctx, d, {}};
+    memcpy(m.seed, seed, 32);
+    g_gen_seen.push_back(m);
+    return false;
+}
+
+// The matrix for (seed, d), derived on stream s when new; nullptr when the cache is off, the
+// pair is met for the first time, or the cache is full of matrices still in use. The caller
+// waits on ->ready before reading it.
 static int gen_acquire(dmx_ctx* ctx, const uint8_t seed[32], int d, hipStream_t s, GenEntry** out) {
     *out = nullptr;
     const size_t slots = gen_cache_slots();
@@ -2018,6 +2046,7 @@ static int gen_acquire(dmx_ctx* ctx, const uint8_t seed[32], int d, hipStream_t 
             *out = e.get();
             return DMX_OK;
         }
+    if (!gen_missed_before(ctx, seed, d)) return DMX_OK;  // first request: derive per job
     while (g_gen.size() >= slots) {
         size_t victim = SIZE_MAX;
         for (size_t i = 0; i < g_gen.size(); ++i)
