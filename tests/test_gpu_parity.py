@@ -685,13 +685,14 @@ def test_server_host_files_and_pending_wait():
 
 
 def test_resident_generators_across_seeds_and_batches():
-    """batches whose jobs share a generator seed read one resident matrix per (seed, depth); a
-    second seed gets its own, the first is reused later, and a batch mixing seeds expands per
-    job: every proof equals its solo run"""
+    """batches whose jobs share a generator seed read one resident matrix per (seed, depth) from
+    the seed's second batch on (its first expands per job, as a request with a seed of its own
+    does); a second seed gets its own matrix, the first is reused later, and a batch mixing seeds
+    expands per job: every proof equals its solo run"""
     rng = np.random.default_rng(808)
     pa = dm.SystemParams(bytes([21] * 32), bytes([22] * 32))
     pb = dm.SystemParams(bytes([23] * 32), bytes([22] * 32))
-    for rep, plist in enumerate([[pa] * 3, [pb] * 3, [pa] * 3, [pa, pb, pa]]):
+    for rep, plist in enumerate([[pa] * 3, [pb] * 3, [pa] * 3, [pb] * 3, [pa] * 3, [pa, pb, pa]]):
         files = [_rand_file(rng, n) for n in (60000, 150000, 200000)]  # d = 7
         say("resident generators rep", rep)
         out = dm.prove_verify_many(files, plist)
