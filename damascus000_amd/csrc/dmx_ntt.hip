@@ -315,8 +315,8 @@ static void launch_reg(const FwdSeg& s1, const FwdSeg& s2, const NttTables& tw, 
 
 // ---------------------------------------------------------------------------------------
 // Three-pass register transform for degrees 2^8..2^11: TP threads per prime slice (32 at 2^8,
-// 64, or 128 at 2^11) holding E = m / TP elements each (8, 16 or 32), 8 TP threads per row, one
-// row per workgroup. Against the two-pass blockings this takes a fraction of the LDS words per
+// 64 at 2^9..2^11, 128 as the other shape at 2^11) holding E = m / TP elements each (8, 16 or
+// 32), 8 TP threads per row, one row per workgroup. Against the two-pass blockings this takes a fraction of the LDS words per
 // resident thread, so a CU holds more waves. With G = TP / E:
 //   pass 1: thread l holds x = l + TP i (layers len = m/2 .. TP);
 //   pass 2: x = b + G e + TP h (layers len = TP/2 .. G), with l = 2 h + b for G = 2,
